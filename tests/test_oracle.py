@@ -1,0 +1,447 @@
+"""Pins the CPU oracle against the reference's own expectations, closed forms,
+independent libraries and energy derivatives (runs on the CPU, no GPU)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+MU0 = 4e-7 * math.pi
+GAMMA = 1.7595e11
+
+
+# ---- demag: reference test expectations ------------------------------------------
+DEMAG_GOLDEN = [
+    # test/demagSmall.mx3:15-27
+    ((3, 4, 1), (1e-9, 2e-9, 0.5e-9), (-0.15768295526, -0.05676037073, -0.78447151184), 1e-6),
+    # test/demagSmall3D.mx3:15-27
+    ((3, 4, 2), (1e-9, 2e-9, 3e-9), (-0.52710835138, -0.20121671756, -0.27112555503), 1e-6),
+    # test/demagcube.mx3:16-28
+    ((2, 2, 2), (1e-9, 1e-9, 1e-9), (-1 / 3, -1 / 3, -1 / 3), 5e-4),
+    # test/demagodd.mx3:15-27
+    ((5, 5, 5), (1e-9, 1e-9, 1e-9), (-1 / 3, -1 / 3, -1 / 3), 5e-4),
+]
+
+
+@pytest.mark.parametrize("N,cs,expect,tol", DEMAG_GOLDEN)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_demag_golden(oracle, N, cs, expect, tol, dtype):
+    d = oracle.Demag(N, cs, dtype=dtype)
+    for c in range(3):
+        m = np.zeros((3, N[2], N[1], N[0]))
+        m[c] = 1
+        B = d.exec(m, 1 / MU0)
+        avg = [float(B[k].mean()) for k in range(3)]
+        for k in range(3):
+            want = expect[c] if k == c else 0.0
+            assert abs(avg[k] - want) < tol, (c, k, avg[k], want)
+
+
+def test_demag_tensor_symmetry_and_far_field(oracle):
+    """K is even in each axis for diagonal terms, odd for the matching off-diagonal
+    ones; far away it approaches the point-dipole tensor V(3rr-I)/(4 pi r^3)."""
+    N, cs = (16, 8, 4), (2e-9, 3e-9, 2.5e-9)
+    K = oracle.demag_kernel_bf(N, cs)
+    xx, yy, zz, yz, xz, xy = K
+    P = xx.shape[::-1]
+    ix, iy, iz = 3, 2, 1
+    mirror = lambda a, sx, sy, sz: a[(-iz) % P[2] if sz else iz, (-iy) % P[1] if sy else iy, (-ix) % P[0] if sx else ix]
+    val = lambda a: a[iz, iy, ix]
+    assert mirror(xx, 1, 0, 0) == val(xx) and mirror(xy, 1, 0, 0) == -val(xy)
+    assert mirror(xz, 0, 0, 1) == -val(xz) and mirror(yz, 0, 1, 0) == -val(yz)
+    # far-field point dipole at displacement (12, 6, 3) cells
+    ix, iy, iz = 12, 6, 3
+    r = np.array([ix * cs[0], iy * cs[1], iz * cs[2]])
+    V = cs[0] * cs[1] * cs[2]
+    rn = np.linalg.norm(r)
+    T = V * (3 * np.outer(r, r) / rn**2 - np.eye(3)) / (4 * math.pi * rn**3)
+    got = np.array([[xx[iz, iy, ix], xy[iz, iy, ix], xz[iz, iy, ix]],
+                    [xy[iz, iy, ix], yy[iz, iy, ix], yz[iz, iy, ix]],
+                    [xz[iz, iy, ix], yz[iz, iy, ix], zz[iz, iy, ix]]])
+    assert np.abs(got - T).max() < 2e-3 * np.abs(T).max()
+    # self term: trace of the self-demag tensor is -1 (cube-like cell)
+    assert abs(xx[0, 0, 0] + yy[0, 0, 0] + zz[0, 0, 0] + 1) < 2e-3
+
+
+# ---- FFT against numpy -----------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 8, 12, 48, 96, 105, 128, 192, 384, 1024, 11, 13])
+def test_fft_vs_numpy(oracle, n):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex128)
+    got = oracle.fft(x, -1)
+    assert np.allclose(got, np.fft.fft(x), rtol=1e-12, atol=1e-12 * n)
+    got = oracle.fft(x, +1)
+    assert np.allclose(got, np.fft.ifft(x) * n, rtol=1e-12, atol=1e-12 * n)
+    x32 = x.astype(np.complex64)
+    got = oracle.fft(x32, -1)
+    assert np.abs(got - np.fft.fft(x)).max() < 5e-6 * max(1, np.abs(np.fft.fft(x)).max())
+
+
+def test_fft3d_vs_numpy(oracle):
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((5, 6, 8)) + 1j * rng.standard_normal((5, 6, 8))
+    got = oracle.fft3d(x, -1)
+    assert np.allclose(got, np.fft.fftn(x), atol=1e-11)
+
+
+# ---- FFT convolution vs brute-force real-space convolution --------------------------
+@pytest.mark.parametrize("N", [(6, 5, 1), (4, 3, 2), (5, 4, 3), (3, 3, 4), (4, 2, 5), (7, 3, 6)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_conv_vs_brute(oracle, N, dtype):
+    """opencl/conv_selftest.go:14-119: FFT convolution equals the O(N^2) sum."""
+    cs = (2e-9, 3e-9, 1.5e-9)
+    rng = np.random.default_rng(sum(N))
+    m = rng.uniform(-1, 1, (3, N[2], N[1], N[0]))
+    Ms = rng.uniform(5e5, 1e6, (N[2], N[1], N[0]))
+    K = oracle.demag_kernel_bf(N, cs)
+    ref = oracle.demag_brute(N, K, m, Ms, dtype=np.float64)
+    d = oracle.Demag(N, cs, dtype=dtype, K=K)
+    got = d.exec(m, Ms)
+    tol = 1e-11 if dtype == np.float64 else 2e-6
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
+# ---- reductions: opencl/reduce_test.go:46-177 -----------------------------------
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_reduce_known_answers(oracle, dt):
+    f = lambda name: oracle.fn(name, dt)
+    p = oracle.ptr
+    in1 = np.arange(1000).astype(dt)
+    assert f("sum")(p(in1), 1000) == 499500
+    a = np.array([1, 2, 3, 4, 5], dt)
+    b = np.array([5, 4, 3, -1, 2], dt)
+    assert f("dot")(p(a), p(b), 5) == 5 + 8 + 9 - 4 + 10
+    assert f("maxabs")(p(in1), 1000) == 999
+    in2 = (-np.arange(100000) / 100).astype(dt)
+    assert f("maxabs")(p(in2), 100000) == float(dt(999.99))
+    N = 32
+    ax, ay, az = [np.full(N, v, dt) for v in (1, 2, 3)]
+    bx, by, bz = [np.full(N, v, dt) for v in (4, 5, 6)]
+    assert f("maxvecdiff")(p(ax), p(ay), p(az), p(bx), p(by), p(bz), N, 0) == math.sqrt(27)
+    bx[5], by[5], bz[5] = 6, 7, 9
+    assert f("maxvecdiff")(p(ax), p(ay), p(az), p(bx), p(by), p(bz), N, 0) == math.sqrt(25 + 25 + 36)
+    # the reference kernel's widening quirk multiplies by sqrt(3)
+    q = f("maxvecdiff")(p(ax), p(ay), p(az), p(bx), p(by), p(bz), N, 1)
+    assert abs(q - math.sqrt(3) * math.sqrt(86)) < 1e-5 * q
+    assert f("maxvecnorm")(p(ax), p(ay), p(az), N) == math.sqrt(14)
+    ax[5], ay[5], az[5] = 6, 7, 8
+    assert f("maxvecnorm")(p(ax), p(ay), p(az), N) == math.sqrt(36 + 49 + 64)
+
+
+# ---- Philox-4x32-10 known-answer vectors (Random123 kat_vectors) ---------------------
+def test_philox_kat(oracle):
+    assert oracle.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert oracle.philox4x32_10([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert oracle.philox4x32_10([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_thermal_normals_distribution(oracle):
+    n = oracle.thermal_normals(200000, seed=1234, step=7)
+    for c in range(3):
+        assert abs(n[c].mean()) < 0.01
+        assert abs(n[c].var() - 1) < 0.01
+    # independent components, deterministic in (seed, step, cell)
+    assert abs(np.corrcoef(n[0], n[1])[0, 1]) < 0.01
+    n2 = oracle.thermal_normals(200000, seed=1234, step=7)
+    assert np.array_equal(n, n2)
+    n3 = oracle.thermal_normals(200000, seed=1234, step=8)
+    assert not np.array_equal(n, n3)
+
+
+# ---- field terms: energies and closed forms -----------------------------------------
+def _ex_lut(oracle, A, dt=np.float64):
+    return np.full(oracle.SYMLUT, A, dt)
+
+
+def _call_exchange(oracle, m, Ms, A, cs, pbc=0, regions=None, aLUT=None, dt=np.float64):
+    Nz, Ny, Nx = m.shape[1:]
+    B = np.zeros_like(m, dt)
+    m = np.ascontiguousarray(m, dt)
+    aLUT = _ex_lut(oracle, A, dt) if aLUT is None else np.ascontiguousarray(aLUT, dt)
+    wx, wy, wz = (2 / c**2 for c in cs)
+    R = oracle._rt(dt)
+    oracle.fn("addexchange", dt)(*[oracle.ptr(x) for x in (B[0], B[1], B[2], m[0], m[1], m[2])],
+                                 None, R(Ms), oracle.ptr(aLUT),
+                                 None if regions is None else oracle.ptr(regions),
+                                 R(wx), R(wy), R(wz), Nx, Ny, Nz, pbc)
+    return B
+
+
+def _call_dmi(oracle, m, Ms, A, D, cs, pbc=0, openbc=0, dt=np.float64):
+    Nz, Ny, Nx = m.shape[1:]
+    B = np.zeros_like(m, dt)
+    m = np.ascontiguousarray(m, dt)
+    a, d = _ex_lut(oracle, A, dt), _ex_lut(oracle, D, dt)
+    R = oracle._rt(dt)
+    oracle.fn("adddmi", dt)(*[oracle.ptr(x) for x in (B[0], B[1], B[2], m[0], m[1], m[2])],
+                            None, R(Ms), oracle.ptr(a), oracle.ptr(d), None,
+                            R(cs[0]), R(cs[1]), R(cs[2]), Nx, Ny, Nz, pbc, openbc)
+    return B
+
+
+def test_exchange_uniform_zero_and_helix(oracle):
+    cs = (2e-9, 3e-9, 4e-9)
+    Ms, A = 8e5, 1.3e-11
+    m = np.zeros((3, 3, 4, 16))
+    m[0] = 1
+    assert np.abs(_call_exchange(oracle, m, Ms, A, cs)).max() == 0
+    # helix along x with PBC: B = (2A/Ms) (2 cos(kc) - 2)/c^2 m
+    Nx = 16
+    k = 2 * math.pi * 2 / (Nx * cs[0])
+    x = np.arange(Nx) * cs[0]
+    m = np.zeros((3, 3, 4, Nx))
+    m[0] = np.cos(k * x)
+    m[1] = np.sin(k * x)
+    B = _call_exchange(oracle, m, Ms, A, cs, pbc=1)
+    want = (2 * A / Ms) * (2 * math.cos(k * cs[0]) - 2) / cs[0] ** 2 * m
+    assert np.abs(B - want).max() < 1e-12 * np.abs(want).max()
+
+
+def _exchange_energy(m, A, cs, Ms):
+    """Discrete Heisenberg energy sum_<ij> A/c^2 |m_i - m_j|^2 V (open boundaries)."""
+    V = cs[0] * cs[1] * cs[2]
+    E = 0.0
+    for axis, c in zip((3, 2, 1), cs):
+        if m.shape[axis] > 1:
+            d = np.diff(m, axis=axis)
+            E += A / c**2 * np.sum(d * d) * V
+    return E
+
+
+def test_exchange_is_energy_gradient(oracle):
+    rng = np.random.default_rng(1)
+    cs, Ms, A = (2e-9, 3e-9, 2.5e-9), 8e5, 1.3e-11
+    m = rng.standard_normal((3, 3, 4, 5))
+    m /= np.linalg.norm(m, axis=0)
+    B = _call_exchange(oracle, m, Ms, A, cs)
+    V = cs[0] * cs[1] * cs[2]
+    h = 1e-6
+    for (c, z, y, x) in [(0, 0, 0, 0), (1, 1, 2, 3), (2, 2, 3, 4), (0, 1, 0, 2)]:
+        mp, mm = m.copy(), m.copy()
+        mp[c, z, y, x] += h
+        mm[c, z, y, x] -= h
+        dE = (_exchange_energy(mp, A, cs, Ms) - _exchange_energy(mm, A, cs, Ms)) / (2 * h)
+        assert abs(B[c, z, y, x] - (-dE / (Ms * V))) < 1e-6 * abs(dE / (Ms * V)) + 1e-9
+
+
+def test_dmi_reduces_to_exchange(oracle):
+    """test/exchange3d_dmi.mx3: the DMI kernel with D->0 gives the exchange field."""
+    rng = np.random.default_rng(2)
+    cs, Ms, A = (2e-9, 2e-9, 2e-9), 8e5, 1.3e-11
+    m = rng.standard_normal((3, 4, 5, 6))
+    m /= np.linalg.norm(m, axis=0)
+    B1 = _call_exchange(oracle, m, Ms, A, cs)
+    B2 = _call_dmi(oracle, m, Ms, A, 1e-20, cs)
+    assert np.abs(B1 - B2).max() < 1e-9 * np.abs(B1).max()
+
+
+def test_dmi_helix_closed_form(oracle):
+    """Interior interfacial DMI field on a periodic cycloid m = (cos kx, 0, sin kx):
+    B_x = (2D/(Ms c)) cos(kx) sin(kc), B_z = -(2D/(Ms c)) (-sin(kx)) sin(kc)... from the
+    continuum field (2D/Ms)(dmz/dx, 0, -dmx/dx) discretised with central differences."""
+    Nx, cs, Ms, D = 24, (2e-9, 2e-9, 1e-9), 1.1e6, 3e-3
+    k = 2 * math.pi * 3 / (Nx * cs[0])
+    x = np.arange(Nx) * cs[0]
+    m = np.zeros((3, 1, 3, Nx))
+    m[0] = np.cos(k * x)
+    m[2] = np.sin(k * x)
+    B = _call_dmi(oracle, m, Ms, 0.0 + 1e-30, D, cs, pbc=1 | 2)
+    s = math.sin(k * cs[0])
+    want_x = (D / (Ms * cs[0])) * 2 * np.cos(k * x) * s
+    want_z = -(D / (Ms * cs[0])) * (-2 * np.sin(k * x) * s)
+    assert np.abs(B[0] - want_x).max() < 1e-9 * np.abs(want_x).max()
+    assert np.abs(B[2] - want_z).max() < 1e-9 * np.abs(want_z).max()
+    assert np.abs(B[1]).max() < 1e-9 * np.abs(want_x).max()
+
+
+def _uniax_energy(m, K1, K2, u):
+    u = np.asarray(u, float) / np.linalg.norm(u)
+    mu = m[0] * u[0] + m[1] * u[1] + m[2] * u[2]
+    return np.sum(-K1 * mu**2 - K2 * mu**4)
+
+
+def _cubic_energy(m, K1, K2, K3, c1, c2):
+    c1 = np.asarray(c1, float) / np.linalg.norm(c1)
+    c2 = np.asarray(c2, float) / np.linalg.norm(c2)
+    c3 = np.cross(c1, c2)
+    a, b, c = [sum(m[i] * v[i] for i in range(3)) for v in (c1, c2, c3)]
+    return np.sum(K1 * (a * a * b * b + b * b * c * c + c * c * a * a) + K2 * a * a * b * b * c * c
+                  + K3 * (a**4 * b**4 + b**4 * c**4 + c**4 * a**4))
+
+
+def test_anisotropy_is_energy_gradient(oracle):
+    """B = -(1/Ms) dE/dm for the uniaxial (oxs_uniaxial4) and cubic (oxs_cubic8) energies."""
+    rng = np.random.default_rng(4)
+    n = 7
+    m = rng.standard_normal((3, n))
+    m /= np.linalg.norm(m, axis=0)
+    Ms, K1, K2, K3 = 8e5, 5e5, -2e5, 1e5
+    u, c1, c2 = (0.3, -0.2, 0.9), (1, 1, 0), (-1, 1, 0.2)
+    dt = np.float64
+    R = oracle._rt(dt)
+    p = oracle.ptr
+    B = np.zeros((3, n))
+    oracle.fn("adduniaxial2", dt)(p(B[0]), p(B[1]), p(B[2]), p(m[0]), p(m[1]), p(m[2]), None, R(Ms),
+                                  None, R(K1), None, R(K2), None, R(u[0]), None, R(u[1]), None, R(u[2]), n)
+    Bc = np.zeros((3, n))
+    c2n = np.asarray(c2) - np.dot(c2, c1) / np.dot(c1, c1) * np.asarray(c1)  # cubic axes orthogonal
+    oracle.fn("addcubic2", dt)(p(Bc[0]), p(Bc[1]), p(Bc[2]), p(m[0]), p(m[1]), p(m[2]), None, R(Ms),
+                               None, R(K1), None, R(K2), None, R(K3), None, R(c1[0]), None, R(c1[1]),
+                               None, R(c1[2]), None, R(c2n[0]), None, R(c2n[1]), None, R(c2n[2]), n)
+    h = 1e-6
+    for i in range(n):
+        for c in range(3):
+            mp, mm = m.copy(), m.copy()
+            mp[c, i] += h
+            mm[c, i] -= h
+            dE = (_uniax_energy(mp, K1, K2, u) - _uniax_energy(mm, K1, K2, u)) / (2 * h)
+            assert abs(B[c, i] + dE / Ms) < 1e-6 * (abs(dE / Ms) + 1)
+            dE = (_cubic_energy(mp, K1, K2, K3, c1, c2n) - _cubic_energy(mm, K1, K2, K3, c1, c2n)) / (2 * h)
+            assert abs(Bc[c, i] + dE / Ms) < 1e-6 * (abs(dE / Ms) + 1)
+
+
+def test_lltorque_invariants(oracle):
+    rng = np.random.default_rng(5)
+    n = 100
+    m = rng.standard_normal((3, n))
+    m /= np.linalg.norm(m, axis=0)
+    H = rng.standard_normal((3, n))
+    alpha = rng.uniform(0, 1, n)
+    t = np.zeros((3, n))
+    p = oracle.ptr
+    oracle.fn("lltorque2", np.float64)(p(t[0]), p(t[1]), p(t[2]), p(m[0]), p(m[1]), p(m[2]),
+                                       p(H[0]), p(H[1]), p(H[2]), p(alpha), oracle._rt(np.float64)(1.0), n)
+    assert np.abs((t * m).sum(0)).max() < 1e-14
+    mxH = np.cross(m.T, H.T).T
+    # |tau| = |m x H| sqrt(1 + alpha^2) / (1 + alpha^2) for unit m
+    assert np.allclose(np.linalg.norm(t, axis=0), np.linalg.norm(mxH, axis=0) / np.sqrt(1 + alpha**2))
+    # alpha = 0: tau = -m x H
+    oracle.fn("lltorque2", np.float64)(p(t[0]), p(t[1]), p(t[2]), p(m[0]), p(m[1]), p(m[2]),
+                                       p(H[0]), p(H[1]), p(H[2]), None, oracle._rt(np.float64)(0.0), n)
+    assert np.allclose(t, -mxH)
+
+
+# ---- step controller host logic: engine/run.go:150-187 -------------------------------
+def test_adapt_dt(oracle):
+    a = oracle.adapt_dt
+    assert a(1e-13, 1.0, 0, 0.8, 0, 0, 0, 1) == pytest.approx(0.8e-13)
+    assert a(1e-13, float("nan"), 0, 0.8, 0, 0, 0, 1) == pytest.approx(0.8e-13)  # regression017
+    assert a(1e-13, 100.0, 0, 0.8, 0, 0, 0, 1) == pytest.approx(2e-13)
+    assert a(1e-13, 0.01, 0, 0.8, 0, 0, 0, 1) == pytest.approx(0.5e-13)
+    assert a(1e-13, 1.0, 2e-14, 0.8, 0, 0, 0, 1) == 2e-14
+    assert a(1e-13, 1.0, 0, 0.8, 1e-13, 0, 0, 1) == 1e-13
+    assert a(1e-13, 2.0, 0, 0.8, 0, 1.2e-13, 0, 1) == 1.2e-13
+    assert a(1e-13, 2.0, 0, 0.8, 0, 0, 0.95e-12, 1e-12) == pytest.approx(0.05e-12)
+    assert a(1e-13, 0.0, 0, 0.8, 0, 0, 0, 1) < 0
+
+
+def test_exch_average(oracle):
+    assert oracle.exch_average(1e-11, 1e-11) == pytest.approx(1e-11)
+    assert oracle.exch_average(1e-11, 2e-11) == pytest.approx(2 / (1e11 + 0.5e11), rel=1e-6)
+    assert oracle.exch_average(1e-11, 0.0) == 0.0
+    v = oracle.exch_average(2e-3, -1e-3)
+    assert v > 0 and v < 2e-3
+
+
+# ---- steppers: macrospin analytic solution -----------------------------------------
+def _macrospin(oracle, solver, fixdt, t_end, alpha=0.1, Bz=0.1, dtype=np.float64, maxerr=1e-5):
+    s = oracle.Sim((1, 1, 1), (1e-9, 1e-9, 1e-9), dtype=dtype)
+    s.set_flags(enable_demag=0)
+    s.set_scalar(oracle.P_MSAT, 8e5)
+    s.set_scalar(oracle.P_ALPHA, alpha)
+    s.set_vector(oracle.P_BEXT, (0.0, 0.0, Bz))
+    s.set_exchange_luts(np.zeros(oracle.SYMLUT))
+    th0, ph0 = 1.2, 0.3
+    s.set_m(np.array([[[[math.sin(th0) * math.cos(ph0)]]], [[[math.sin(th0) * math.sin(ph0)]]], [[[math.cos(th0)]]]]))
+    s.set_solver(solver)
+    s.set_solver_params(dt=1e-14, fixdt=fixdt, maxerr=maxerr)
+    s.run(t_end)
+    # with FixDt the reference does not clamp the last step to the alarm
+    # (engine/run.go:151-154), so compare at the time actually reached
+    t_end = s.state()["t"]
+    g = GAMMA / (1 + alpha**2)
+    th = 2 * math.atan(math.tan(th0 / 2) * math.exp(-g * alpha * Bz * t_end))
+    ph = ph0 + g * Bz * t_end
+    want = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    return s.get_m().reshape(3), want, s.state()
+
+
+@pytest.mark.parametrize("solver,fixdt,tol", [(5, 0.0, 2e-5), (5, 1e-13, 1e-7), (4, 1e-13, 1e-6),
+                                              (4, 0.0, 1e-4), (2, 1e-14, 1e-4), (2, 0.0, 1e-3)])
+def test_macrospin_analytic(oracle, solver, fixdt, tol):
+    m, want, st = _macrospin(oracle, solver, fixdt, 0.2e-9)
+    assert np.abs(m - want).max() < tol, (m, want, st)
+
+
+def test_rk45dp_convergence_order(oracle):
+    """Fixed-step global error of the 5th-order solution scales ~dt^5."""
+    e = []
+    for dt in (4e-13, 2e-13):  # above the FP64 round-off floor
+        m, want, _ = _macrospin(oracle, 5, dt, 0.4e-9, Bz=0.5)
+        e.append(np.abs(m - want).max())
+    assert 20 < e[0] / e[1] < 50
+
+
+def test_steppercache(oracle):
+    """test/steppercache.mx3: FSAL cache is flushed between runs."""
+    def run(mx):
+        s = oracle.Sim((1, 1, 1), (1.0, 1.0, 1.0))
+        s.set_scalar(oracle.P_MSAT, 1000e3)
+        s.set_scalar(oracle.P_ALPHA, 0.1)
+        s.set_vector(oracle.P_BEXT, (0.0, 0.0, 0.05))
+        s.set_exchange_luts(np.zeros(oracle.SYMLUT))
+        s.set_solver_params(fixdt=1e-12)
+        s.set_m(np.array([[[[mx]]], [[[0.0]]], [[[0.0]]]]))
+        s.steps(1)
+        return s.get_m().reshape(3)
+    r1 = math.acos(np.dot([1, 0, 0], run(1.0)))
+    r2 = math.acos(np.dot([-1, 0, 0], run(-1.0)))
+    assert abs(r1 - r2) < 1e-5
+
+
+def test_gammaLL_heun_trajectory(oracle):
+    """test/gammaLL.mx3:6-28 (Heun, adaptive, 16x16x1, two gammaLL values)."""
+    N = 16
+    c = 5e-9
+    s = oracle.Sim((N, N, 1), (c, c, c))
+    s.set_scalar(oracle.P_MSAT, 1 / MU0)
+    s.set_scalar(oracle.P_ALPHA, 0.001)
+    s.set_exchange_luts(np.full(oracle.SYMLUT, 10e-12))
+    s.set_vector(oracle.P_BEXT, (0.0, 0.0, 1.0))
+    s.set_solver(oracle.SOLVER_HEUN)
+    s.set_solver_params(dt=1e-15, maxerr=1e-4)
+    m = np.ones((3, 1, N, N))
+    s.set_m(m)
+    s.run(0.1e-9)
+    s.set_solver_params(dt=s.state()["dt"], maxerr=1e-4, gammaLL=2 * GAMMA)
+    s.run(0.1e-9)
+    got = s.average_m()
+    assert np.abs(got - np.array([-0.286423, -0.746625, 0.5970549])).max() < 1.2e-4, got
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not os.environ.get("UM_SLOW_TESTS"), reason="~5 min on 8 cores; set UM_SLOW_TESTS=1")
+def test_rk4_trajectory(oracle):
+    """test/rk4.mx3:6-28 (RK4 adaptive relax with alpha=3, then fixed-step and adaptive reversal)."""
+    s = oracle.Sim((128, 32, 1), (500e-9 / 128, 125e-9 / 32, 3e-9))
+    s.set_scalar(oracle.P_MSAT, 800e3)
+    s.set_exchange_luts(np.full(oracle.SYMLUT, 13e-12))
+    m = np.zeros((3, 1, 32, 128))
+    m[0] = 1
+    m[1] = 0.1
+    s.set_m(m)
+    s.set_solver(oracle.SOLVER_RK4)
+    s.set_scalar(oracle.P_ALPHA, 3.0)
+    s.set_solver_params(dt=1e-15)
+    s.run(1e-9)
+    s.set_scalar(oracle.P_ALPHA, 0.02)
+    s.set_vector(oracle.P_BEXT, (-24.6e-3, 4.3e-3, 0.0))
+    s.set_solver_params(dt=s.state()["dt"], fixdt=1e-14)
+    s.run(0.1e-9)
+    got = s.average_m()
+    assert np.abs(got - np.array([0.59293, 0.63278, -0.08152])).max() < 1e-3, got
+    s.set_solver_params(dt=1e-14, fixdt=0.0)
+    s.run(0.4e-9)
+    got = s.average_m()
+    assert np.abs(got - np.array([-0.8740, -0.2713, 0.01795])).max() < 1e-3, got
