@@ -1,0 +1,218 @@
+/*
+ * umhip.h -- C ABI of libumhip.so, the MI355X (gfx950) implementation of the
+ * uMagNUS effective-field and time-stepping path.
+ *
+ * The library replaces the reference's cgo OpenCL binding (the Go files of cl/) and its
+ * VkFFT wrapper (cl/vkfft.go, cl/fft_interface.h) underneath the exported API
+ * of package opencl / opencl64.  Every entry point below names the reference
+ * function it stands in for.  Conventions:
+ *
+ *  - Every function returns 0 on success and a negative UM_E* code on error;
+ *    um_last_error() then describes the failure (thread-local string).
+ *  - "prec" selects the arithmetic type: UM_F32 (package opencl) or UM_F64
+ *    (package opencl64).  Device arrays are plain device pointers of that type.
+ *  - Vector quantities are three separate component arrays (SoA), x fastest:
+ *    idx = (iz*Ny + iy)*Nx + ix, as data.Slice (data/slice.go:26-32).
+ *  - A parameter in the reference's MSlice convention (opencl/mslice.go:11-45)
+ *    is passed as (arr, mul): arr == NULL means "uniform, value mul", else the
+ *    per-cell value is mul*arr[i].
+ *  - All device work is enqueued in order on one HIP stream per library
+ *    (um_set_stream) or per simulation handle, like the reference's single
+ *    in-order command queue (opencl/init.go:145).  Host pointers are never
+ *    retained past a call (cgo pointer rules).
+ */
+#ifndef UMHIP_H
+#define UMHIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { UM_F32 = 0, UM_F64 = 1 };
+enum {
+  UM_OK = 0,
+  UM_EINVAL = -1,   /* bad argument (size mismatch, NULL where not allowed) */
+  UM_EHIP = -2,     /* HIP runtime error */
+  UM_ENOMEM = -3,   /* device allocation failed */
+  UM_EUNSUP = -4,   /* unsupported size or option */
+  UM_ESTATE = -5    /* call out of order (e.g. before um_init) */
+};
+
+/* ---- runtime: opencl/init.go, opencl/slice.go, opencl/buffer.go ----------- */
+/* Init(gpu int), opencl/init.go:48 */
+int um_init(int device);
+/* description of the last error on this thread */
+const char *um_last_error(void);
+/* name and compute-unit count of the current device */
+int um_device_info(char *name, int name_len, int *cu_count, size_t *mem_bytes);
+/* stream used by the unfused ops from now on; NULL selects the HIP null
+ * stream.  Until called, the library uses a stream of its own. */
+int um_set_stream(void *hip_stream);
+/* Finish() on the command queue */
+int um_sync(void);
+/* MemAlloc / NewSlice, opencl/slice.go:14-45 */
+int um_malloc(void **ptr, size_t bytes);
+/* memFree, opencl/slice.go:46 */
+int um_free(void *ptr);
+/* MemCpyHtoD / MemCpyDtoH / MemCpy, opencl/slice.go:53-155 (blocking) */
+int um_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int um_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int um_memcpy_d2d(void *dst, const void *src, size_t bytes);
+/* Zero / Memset, opencl/slice.go:156-194 (kernel fill, safe inside graphs) */
+int um_zero(void *dst, size_t bytes);
+int um_memset(int prec, void *dst, double value, size_t n);
+
+/* ---- field terms: opencl/exchange.go, dmi.go, anisotropy.go, region.go ---- */
+/* AddExchange, opencl/exchange.go:16 (kernels_src/cl/exchange.cl) */
+int um_add_exchange(int prec, void *const B[3], const void *const m[3], const void *Ms,
+                    double Ms_mul, const void *aLUT, const uint8_t *regions, double wx,
+                    double wy, double wz, int Nx, int Ny, int Nz, int pbc);
+/* AddDMI, opencl/dmi.go:15 (kernels_src/cl/dmi.cl) */
+int um_add_dmi(int prec, void *const B[3], const void *const m[3], const void *Ms,
+               double Ms_mul, const void *aLUT, const void *dLUT, const uint8_t *regions,
+               double cx, double cy, double cz, int Nx, int Ny, int Nz, int pbc, int openbc);
+/* AddUniaxialAnisotropy2, opencl/anisotropy.go:153 */
+int um_add_uniaxial2(int prec, void *const B[3], const void *const m[3], const void *Ms,
+                     double Ms_mul, const void *K1, double K1_mul, const void *K2,
+                     double K2_mul, const void *const u[3], const double u_mul[3], int N);
+/* AddCubicAnisotropy2, opencl/anisotropy.go:12 */
+int um_add_cubic2(int prec, void *const B[3], const void *const m[3], const void *Ms,
+                  double Ms_mul, const void *K1, double K1_mul, const void *K2, double K2_mul,
+                  const void *K3, double K3_mul, const void *const c1[3],
+                  const double c1_mul[3], const void *const c2[3], const double c2_mul[3],
+                  int N);
+/* RegionAddV, opencl/region.go:13 */
+int um_region_addv(int prec, void *const dst[3], const void *const lut[3],
+                   const uint8_t *regions, int N);
+/* ZeroMask, opencl/zeromask.go:12 */
+int um_zeromask(int prec, void *dst, const void *maskLUT, const uint8_t *regions, int N);
+/* LLTorque, opencl/lltorque.go:16 */
+int um_lltorque(int prec, void *const t[3], const void *const m[3], const void *const B[3],
+                const void *alpha, double alpha_mul, int N);
+/* LLNoPrecess, opencl/lltorque.go:97 */
+int um_llnoprecess(int prec, void *const t[3], const void *const m[3],
+                   const void *const B[3], int N);
+/* SetTemperature, opencl/temperature.go:14, with the noise drawn in place from
+ * the Philox-4x32-10 stream (seed, step, stream) instead of a staging buffer
+ * filled by Generator.Normal (opencl/rng.go:146).  k = 2 kB / (gamma V dt). */
+int um_thermal_field(int prec, void *const B[3], uint64_t seed, uint64_t step,
+                     uint32_t stream, double k, const void *Ms, double Ms_mul,
+                     const void *temp, double temp_mul, const void *alpha, double alpha_mul,
+                     int N);
+
+/* ---- vector arithmetic: opencl/madd.go, normalize.go, vecnorm.go ---------- */
+/* Madd2..Madd7, opencl/madd.go:111-489: dst = sum fac[k]*src[k], nsrc in 1..7 */
+int um_madd(int prec, void *dst, int nsrc, const void *const *src, const double *fac, int N);
+/* Normalize, opencl/normalize.go:13 */
+int um_normalize(int prec, void *const v[3], const void *vol, int N);
+/* VecNorm, opencl/vecnorm.go:12 */
+int um_vecnorm(int prec, void *dst, const void *const a[3], int N);
+/* Div, opencl/madd.go:60 (0 where b == 0) */
+int um_div(int prec, void *dst, const void *a, const void *b, int N);
+/* Mul, opencl/madd.go:13 */
+int um_mul(int prec, void *dst, const void *a, const void *b, int N);
+
+/* ---- reductions: opencl/reduce.go (results in FP64, one readback) --------- */
+int um_reduce_sum(int prec, const void *x, int N, double *out);
+int um_reduce_dot(int prec, const void *a, const void *b, int N, double *out);
+int um_reduce_maxabs(int prec, const void *x, int N, double *out);
+int um_reduce_maxvecnorm(int prec, const void *const v[3], int N, double *out);
+/* sqrt3_quirk=1 reproduces the widening in reducemaxvecdiff2.cl:24 */
+int um_reduce_maxvecdiff(int prec, const void *const a[3], const void *const b[3], int N,
+                         int sqrt3_quirk, double *out);
+
+/* ---- demag: opencl/conv_demag.go, mag/demagkernel.go ---------------------- */
+/* NewDemag(inputSize, PBC, DemagKernel(...)), opencl/conv_demag.go:27 with the
+ * brute-force tensor of mag/demagkernel.go:115 computed on the GPU in FP64.
+ * accuracy = DemagAccuracy (engine/demag.go:49). */
+int um_demag_create(int prec, int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
+                    double accuracy, void **plan);
+/* NewDemag from a caller-supplied FP64 host tensor (xx,yy,zz,yz,xz,xy), each of
+ * the padded size P (x fastest) -- e.g. the Newell tensor or an OVF cache. */
+int um_demag_create_from_kernel(int prec, int Nx, int Ny, int Nz, const int P[3],
+                                const double *const K[6], void **plan);
+/* (*DemagConvolution).Exec(B, m, vol, Msat), opencl/conv_demag.go:43 */
+int um_demag_exec(void *plan, void *const B[3], const void *const m[3], const void *vol,
+                  const void *Ms, double Ms_mul);
+/* (*DemagConvolution).Free, opencl/conv_demag.go:204 */
+int um_demag_destroy(void *plan);
+/* padded size of a plan and the kernel spectrum component c (0..5) copied to
+ * host (H x (Py/2+1) x (Pz/2+1) values of the plan's precision) */
+int um_demag_info(void *plan, int P[3]);
+int um_demag_kspec(void *plan, int c, void *host_out);
+/* CalcDemagKernel, mag/demagkernel.go:115-336, on the GPU in FP64; K[c] are
+ * host buffers of the padded size P = padSize(N, pbc). */
+int um_demag_kernel_bf(int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
+                       double accuracy, double *const K[6]);
+
+/* ---- simulation handle: engine/ effective field, torque, steppers --------- */
+/* Parameter ids (the .mx3 quantity names) for um_sim_set_scalar / _vector */
+enum {
+  UM_P_MSAT = 0, UM_P_KU1, UM_P_KU2, UM_P_KC1, UM_P_KC2, UM_P_KC3, UM_P_ALPHA,
+  UM_P_TEMP, UM_P_ANISU, UM_P_ANISC1, UM_P_ANISC2, UM_P_BEXT, UM_P_COUNT
+};
+/* SetSolver ids, engine/run.go:78-90 */
+enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45DP = 5 };
+
+/* A simulation owns m, the parameters, the demag plan and the stepper state on
+ * one HIP stream (engine/magnetization.go, parameter.go, run.go). */
+int um_sim_create(int prec, int Nx, int Ny, int Nz, double cx, double cy, double cz,
+                  int pbcx, int pbcy, int pbcz, void **sim);
+int um_sim_destroy(void *sim);
+/* Msat, Ku1, ... (engine/parameter.go): host array (per cell, NULL = uniform) x mul */
+int um_sim_set_scalar(void *sim, int id, const void *host_arr, double mul);
+int um_sim_set_vector(void *sim, int id, const void *ax, const void *ay, const void *az,
+                      double mx, double my, double mz);
+/* regions (engine/regions.go), host uint8 per cell or NULL (all region 0) */
+int um_sim_set_regions(void *sim, const uint8_t *host_regions);
+/* lex2 / din2 symmetric LUTs (engine/exchange.go:106-177), 32896 values each;
+ * dLUT NULL = no interfacial DMI (AddExchange instead of AddDMI) */
+int um_sim_set_exchange_luts(void *sim, const void *aLUT, const void *dLUT);
+/* B_ext per region (engine/excitation.go:41-53), 3 x 256 values, NULL clears */
+int um_sim_set_bext_lut(void *sim, const void *lx, const void *ly, const void *lz);
+/* FrozenSpins per region (engine/torque.go:348-352), 256 values, NULL clears */
+int um_sim_set_frozen_lut(void *sim, const void *lut);
+/* geometry (engine/geom.go): per-cell volume fraction, NULL = full */
+int um_sim_set_geometry(void *sim, const void *vol);
+/* EnableDemag, OpenBC, DoPrecess, and the MaxVecDiff sqrt(3) widening in Heun */
+int um_sim_set_flags(void *sim, int enable_demag, int openbc, int precess, int heun_sqrt3);
+int um_sim_set_demag_accuracy(void *sim, double accuracy);
+/* ThermSeed (engine/temperature.go:138); stream separates ensemble members */
+int um_sim_set_thermal_seed(void *sim, uint64_t seed, uint32_t stream);
+/* m.SetArray (engine/magnetization.go:45), normalises; and m readback */
+int um_sim_set_m(void *sim, const void *mx, const void *my, const void *mz);
+int um_sim_get_m(void *sim, void *mx, void *my, void *mz);
+/* device pointers of m (resident state, for callers that keep data on the GPU) */
+int um_sim_m_device(void *sim, void *ptrs[3]);
+/* B_eff, torque and B_demag quantities (effectivefield.go:14, torque.go:124, demag.go:42) */
+int um_sim_beff(void *sim, void *bx, void *by, void *bz);
+int um_sim_torque(void *sim, void *tx, void *ty, void *tz);
+int um_sim_demag_field(void *sim, void *bx, void *by, void *bz);
+/* SetSolver, engine/run.go:92 */
+int um_sim_set_solver(void *sim, int solver);
+/* dt, FixDt, MaxErr, RelErr, MinSlope, Headroom, MinDt, MaxDt, GammaLL */
+int um_sim_set_solver_params(void *sim, const double knobs[9]);
+/* t, dt, step, NUndone, NEval, LastErr, PeakErr, LastTorque */
+int um_sim_get_state(void *sim, double out[8]);
+/* one stepper.Step(); Steps(n) (run.go:197); Run(seconds) (run.go:190) */
+int um_sim_step(void *sim);
+int um_sim_steps(void *sim, int n);
+int um_sim_run(void *sim, double seconds);
+/* m.Average() (engine/average.go:28) and energy terms (energy.go):
+ * 0 demag, 1 exchange(+DMI), 2 Zeeman, 3 uniaxial K1 */
+int um_sim_average_m(void *sim, double out[3]);
+int um_sim_energy(void *sim, int term, double *out);
+/* the HIP stream the simulation enqueues on */
+int um_sim_stream(void *sim, void **hip_stream);
+/* per-evaluation timing with HIP events on the simulation stream (profiling
+ * on resets the counters): out_ms = total ms in {demag FFT path, fused
+ * field+torque kernel, unused}, n_evals = evaluations timed */
+int um_sim_set_profiling(void *sim, int on);
+int um_sim_kernel_times(void *sim, double out_ms[3], int *n_evals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,425 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances (written per test): FP64 GPU vs FP64 oracle to ~1e-12 relative;
+FP32 GPU vs FP64 oracle to 1e-5 relative for fields (normwise: max|diff| /
+max|field|); integer work (Philox) bit-exact.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from cases import GAMMA, MU0, config1_m, random_unit, relerr, setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    from umagnus_amd import umhip
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    umhip.init(0)
+    _say("device:", umhip.device_info())
+    return umhip
+
+
+def _t(a, dtype):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, dtype)).cuda()
+
+
+def _h(t):
+    return t.detach().cpu().numpy()
+
+
+# ---- demag tensor (FP64 on the GPU) --------------------------------------------------------
+@pytest.mark.parametrize("N,cs", [((3, 4, 1), (1e-9, 2e-9, 0.5e-9)), ((3, 4, 2), (1e-9, 2e-9, 3e-9)),
+                                  ((6, 5, 3), (2e-9, 3e-9, 1.5e-9)), ((16, 8, 7), (2e-9, 2e-9, 2e-9))])
+def test_demag_tensor_vs_oracle(gpu, oracle, N, cs):
+    _say("tensor", N)
+    Kg = gpu.demag_kernel_bf(N, cs)
+    Ko = oracle.demag_kernel_bf(N, cs)
+    for c in range(6):
+        scale = max(np.abs(Ko[c]).max(), 1e-300)
+        assert np.abs(Kg[c] - Ko[c]).max() <= 1e-14 * scale, c
+
+
+# ---- demag convolution ---------------------------------------------------------------
+DEMAG_GOLDEN = [
+    ((3, 4, 1), (1e-9, 2e-9, 0.5e-9), (-0.15768295526, -0.05676037073, -0.78447151184), 1e-6),
+    ((3, 4, 2), (1e-9, 2e-9, 3e-9), (-0.52710835138, -0.20121671756, -0.27112555503), 1e-6),
+    ((2, 2, 2), (1e-9, 1e-9, 1e-9), (-1 / 3, -1 / 3, -1 / 3), 5e-4),
+    ((5, 5, 5), (1e-9, 1e-9, 1e-9), (-1 / 3, -1 / 3, -1 / 3), 5e-4),
+]
+
+
+@pytest.mark.parametrize("N,cs,expect,tol", DEMAG_GOLDEN)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_demag_golden(gpu, N, cs, expect, tol, prec):
+    """test/demagSmall.mx3, demagSmall3D.mx3, demagcube.mx3, demagodd.mx3."""
+    _say("demag golden", N, prec)
+    dt = np.float32 if prec == 0 else np.float64
+    conv = gpu.DemagConvolution(N, cs, precision=prec)
+    for c in range(3):
+        m = np.zeros((3, N[2], N[1], N[0]))
+        m[c] = 1
+        mt = [_t(m[k], dt) for k in range(3)]
+        B = [_t(np.zeros_like(m[0]), dt) for _ in range(3)]
+        conv.Exec(B, mt, None, 1 / MU0)
+        avg = [float(_h(B[k]).mean()) for k in range(3)]
+        for k in range(3):
+            want = expect[c] if k == c else 0.0
+            assert abs(avg[k] - want) < tol, (c, k, avg[k], want)
+
+
+CONV_SIZES = [(3, 4, 1), (16, 8, 1), (20, 12, 1), (5, 4, 2), (6, 5, 3), (8, 6, 4), (7, 3, 5), (16, 12, 6),
+              (24, 10, 7), (48, 20, 3), (96, 12, 2), (64, 64, 1), (32, 24, 8)]
+
+
+@pytest.mark.parametrize("N", CONV_SIZES)
+@pytest.mark.parametrize("prec", [1, 0])
+def test_demag_conv_vs_oracle(gpu, oracle, N, prec):
+    """B_demag of a random m with per-cell Ms and geometry vs the FP64 oracle."""
+    _say("demag conv", N, prec)
+    cs = (2e-9, 3e-9, 1.5e-9)
+    rng = np.random.default_rng(sum(N) + 7 * prec)
+    m = random_unit((N[2], N[1], N[0]), sum(N))
+    Ms = rng.uniform(5e5, 1e6, (N[2], N[1], N[0]))
+    vol = rng.uniform(0.2, 1.0, (N[2], N[1], N[0]))
+    ref = oracle.Demag(N, cs, dtype=np.float64).exec(m, Ms, vol)
+    dt = np.float32 if prec == 0 else np.float64
+    conv = gpu.DemagConvolution(N, cs, precision=prec)
+    B = [_t(np.zeros_like(m[0]), dt) for _ in range(3)]
+    conv.Exec(B, [_t(m[k], dt) for k in range(3)], _t(vol, dt), _t(Ms, dt))
+    got = np.stack([_h(b) for b in B])
+    tol = 1e-12 if prec == 1 else 2e-6
+    assert relerr(got, ref) <= tol, relerr(got, ref)
+
+
+def test_demag_kspec_vs_oracle(gpu, oracle):
+    N, cs = (12, 10, 3), (2e-9, 3e-9, 1.5e-9)
+    conv = gpu.DemagConvolution(N, cs, precision=1)
+    d = oracle.Demag(N, cs, dtype=np.float64)
+    for c in range(6):
+        got = conv.kspec(c)
+        want = np.zeros_like(got)
+        assert oracle.fn("demag_kspec", np.float64)(__import__("ctypes").c_void_p(d.h), c, oracle.ptr(want)) == 0
+        assert relerr(got, want) < 1e-12, c
+
+
+# ---- unfused field terms vs oracle ----------------------------------------------------------
+@pytest.mark.parametrize("prec", [1, 0])
+def test_field_terms_vs_oracle(gpu, oracle, prec):
+    _say("field terms", prec)
+    import ctypes as C
+    dt = np.float32 if prec == 0 else np.float64
+    tol = 1e-13 if prec == 1 else 2e-6
+    N = (13, 9, 4)
+    shape = (N[2], N[1], N[0])
+    n = N[0] * N[1] * N[2]
+    cs = (2e-9, 2.5e-9, 3e-9)
+    rng = np.random.default_rng(11)
+    m = random_unit(shape, 3).astype(dt)
+    m[:, 1, 2, 3] = 0  # a hole: missing neighbour handling
+    regions = rng.integers(0, 3, shape).astype(np.uint8)
+    Ms = rng.uniform(5e5, 1e6, shape).astype(dt)
+    from umagnus_amd.params import exchange_lut
+    aLUT = exchange_lut({0: 1.3e-11, 1: 2e-11, 2: 1e-11}, dtype=dt)
+    dLUT = exchange_lut({0: 3e-3, 1: -1e-3, 2: 2e-3}, dtype=dt)
+    R = oracle._rt(dt)
+    p = oracle.ptr
+    for pbc in (0, 1 | 2, 4):
+        for openbc in (0, 1):
+            Bo = np.zeros((3,) + shape, dt)
+            oracle.fn("adddmi", dt)(p(Bo[0]), p(Bo[1]), p(Bo[2]), p(m[0]), p(m[1]), p(m[2]), p(Ms), R(1.0),
+                                    p(aLUT), p(dLUT), p(regions), R(cs[0]), R(cs[1]), R(cs[2]), N[0], N[1], N[2],
+                                    pbc, openbc)
+            Bg = [_t(np.zeros(shape), dt) for _ in range(3)]
+            gpu.AddDMI(Bg, [_t(m[k], dt) for k in range(3)], _t(aLUT, dt), _t(dLUT, dt), _t(Ms, dt),
+                       _t(regions, np.uint8), cs, pbc=pbc, OpenBC=bool(openbc))
+            assert relerr(np.stack([_h(b) for b in Bg]), Bo) <= tol, ("dmi", pbc, openbc)
+        Bo = np.zeros((3,) + shape, dt)
+        w = [dt(2 / c**2) for c in cs]
+        oracle.fn("addexchange", dt)(p(Bo[0]), p(Bo[1]), p(Bo[2]), p(m[0]), p(m[1]), p(m[2]), p(Ms), R(1.0),
+                                     p(aLUT), p(regions), R(w[0]), R(w[1]), R(w[2]), N[0], N[1], N[2], pbc)
+        Bg = [_t(np.zeros(shape), dt) for _ in range(3)]
+        gpu.AddExchange(Bg, [_t(m[k], dt) for k in range(3)], _t(aLUT, dt), _t(Ms, dt), _t(regions, np.uint8), cs,
+                        pbc=pbc)
+        assert relerr(np.stack([_h(b) for b in Bg]), Bo) <= tol, ("exchange", pbc)
+    # anisotropies: per-cell K1, uniform K2 / axes
+    K1 = rng.uniform(1e5, 8e5, shape).astype(dt)
+    Bo = np.zeros((3,) + shape, dt)
+    oracle.fn("adduniaxial2", dt)(p(Bo[0]), p(Bo[1]), p(Bo[2]), p(m[0]), p(m[1]), p(m[2]), p(Ms), R(1.0), p(K1),
+                                  R(1.0), None, R(-2e5), None, R(0.3), None, R(-0.2), None, R(0.9), n)
+    Bg = [_t(np.zeros(shape), dt) for _ in range(3)]
+    gpu.AddUniaxialAnisotropy2(Bg, [_t(m[k], dt) for k in range(3)], _t(Ms, dt), _t(K1, dt), -2e5, (0.3, -0.2, 0.9))
+    assert relerr(np.stack([_h(b) for b in Bg]), Bo) <= tol, "uniaxial"
+    Bo = np.zeros((3,) + shape, dt)
+    oracle.fn("addcubic2", dt)(p(Bo[0]), p(Bo[1]), p(Bo[2]), p(m[0]), p(m[1]), p(m[2]), p(Ms), R(1.0), None,
+                               R(4e5), None, R(-1e5), None, R(2e4), None, R(1.0), None, R(1.0), None, R(0.0), None,
+                               R(-1.0), None, R(1.0), None, R(0.0), n)
+    Bg = [_t(np.zeros(shape), dt) for _ in range(3)]
+    gpu.AddCubicAnisotropy2(Bg, [_t(m[k], dt) for k in range(3)], _t(Ms, dt), 4e5, -1e5, 2e4, (1.0, 1.0, 0.0),
+                            (-1.0, 1.0, 0.0))
+    assert relerr(np.stack([_h(b) for b in Bg]), Bo) <= tol, "cubic"
+    # LL torque, no-precession torque
+    H = rng.standard_normal((3,) + shape).astype(dt)
+    alpha = rng.uniform(0, 1, shape).astype(dt)
+    To = np.zeros((3,) + shape, dt)
+    oracle.fn("lltorque2", dt)(p(To[0]), p(To[1]), p(To[2]), p(m[0]), p(m[1]), p(m[2]), p(H[0]), p(H[1]),
+                               p(H[2]), p(alpha), R(1.0), n)
+    Tg = [_t(np.zeros(shape), dt) for _ in range(3)]
+    gpu.LLTorque(Tg, [_t(m[k], dt) for k in range(3)], [_t(H[k], dt) for k in range(3)], _t(alpha, dt))
+    assert relerr(np.stack([_h(b) for b in Tg]), To) <= tol, "lltorque"
+    To = np.zeros((3,) + shape, dt)
+    oracle.fn("llnoprecess", dt)(p(To[0]), p(To[1]), p(To[2]), p(m[0]), p(m[1]), p(m[2]), p(H[0]), p(H[1]),
+                                 p(H[2]), n)
+    gpu.LLNoPrecess(Tg, [_t(m[k], dt) for k in range(3)], [_t(H[k], dt) for k in range(3)])
+    assert relerr(np.stack([_h(b) for b in Tg]), To) <= tol, "llnoprecess"
+    # region LUT add and zero mask
+    lut = [rng.standard_normal(256).astype(dt) for _ in range(3)]
+    Do = H.copy()
+    oracle.fn("regionaddv", dt)(p(Do[0]), p(Do[1]), p(Do[2]), p(lut[0]), p(lut[1]), p(lut[2]), p(regions), n)
+    Dg = [_t(H[k], dt) for k in range(3)]
+    gpu.RegionAddV(Dg, [_t(l, dt) for l in lut], _t(regions, np.uint8))
+    assert np.array_equal(np.stack([_h(b) for b in Dg]), Do), "regionaddv"
+    mask = np.zeros(256, dt)
+    mask[1] = 1
+    zo = H[0].copy()
+    oracle.fn("zeromask", dt)(p(zo), p(mask), p(regions), n)
+    zg = _t(H[0], dt)
+    gpu.ZeroMask(zg, _t(mask, dt), _t(regions, np.uint8))
+    assert np.array_equal(_h(zg), zo), "zeromask"
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_vector_ops_and_reductions(gpu, oracle, prec):
+    """Madd, Normalize, VecNorm, Div and opencl/reduce_test.go:46-177 known answers."""
+    import torch
+    dt = np.float32 if prec == 0 else np.float64
+    rng = np.random.default_rng(5)
+    n = 1000
+    a, b, c = [rng.standard_normal(n).astype(dt) for _ in range(3)]
+    d = _t(np.zeros(n), dt)
+    gpu.Madd(d, [_t(a, dt), _t(b, dt), _t(c, dt)], [1.0, 0.25, -3.0])
+    want = (dt(1.0) * a) + (dt(0.25) * b + dt(-3.0) * c)
+    assert np.abs(_h(d) - want).max() <= 4 * np.finfo(dt).eps * np.abs(want).max()
+    v = [_t(x, dt) for x in (a, b, c)]
+    gpu.Normalize(v)
+    nrm = np.sqrt(sum(_h(x) ** 2 for x in v))
+    assert np.abs(nrm - 1).max() < 8 * np.finfo(dt).eps
+    # known answers
+    in1 = _t(np.arange(1000), dt)
+    assert gpu.Sum(in1) == 499500
+    assert gpu.Dot(_t([1, 2, 3, 4, 5], dt), _t([5, 4, 3, -1, 2], dt)) == 5 + 8 + 9 - 4 + 10
+    assert gpu.MaxAbs(in1) == 999
+    assert gpu.MaxAbs(_t(-np.arange(100000) / 100, dt)) == float(dt(999.99))
+    N = 32
+    A = [_t(np.full(N, x), dt) for x in (1, 2, 3)]
+    B = [_t(np.full(N, x), dt) for x in (4, 5, 6)]
+    assert gpu.MaxVecDiff(A, B) == math.sqrt(27)
+    B[0][5], B[1][5], B[2][5] = 6, 7, 9
+    assert gpu.MaxVecDiff(A, B) == math.sqrt(25 + 25 + 36)
+    assert abs(gpu.MaxVecDiff(A, B, sqrt3_quirk=True) - math.sqrt(3 * 86)) < 1e-12
+    assert gpu.MaxVecNorm(A) == math.sqrt(14)
+    A[0][5], A[1][5], A[2][5] = 6, 7, 8
+    assert gpu.MaxVecNorm(A) == math.sqrt(36 + 49 + 64)
+    nan = _t(np.ones(N), dt)
+    nan[7] = float("nan")
+    assert math.isnan(gpu.MaxVecNorm([nan, _t(np.ones(N), dt), _t(np.ones(N), dt)]))
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_thermal_field_vs_oracle(gpu, oracle, prec):
+    """Philox normals in the kernel equal the oracle's; B = eta sqrt(k alpha T / Ms)."""
+    dt = np.float32 if prec == 0 else np.float64
+    n = 4096
+    seed, step, stream = 0x1234ABCD5678, 17, 3
+    k = 2 * 1.380650424e-23 / (GAMMA * 8e-27 * 1e-14)
+    Bg = [_t(np.zeros(n), dt) for _ in range(3)]
+    gpu.ThermalField(Bg, seed, step, stream, k, 8e5, 300.0, 0.02)
+    eta = oracle.thermal_normals(n, seed, step, stream, dtype=np.float64)
+    want = eta * math.sqrt(k * 0.02 * 300.0 / 8e5)
+    got = np.stack([_h(b) for b in Bg]).astype(np.float64)
+    tol = 1e-12 if prec == 1 else 2e-6
+    assert np.abs(got - want).max() <= tol * np.abs(want).max()
+
+
+# ---- simulation level: effective field and torque ---------------------------------------------
+def _both(oracle, gpu, N, cs, prec, **params):
+    dt = np.float32 if prec == 0 else np.float64
+    so = oracle.Sim(N, cs, dtype=np.float64)
+    sg = gpu.Sim(N, cs, dtype=dt)
+    setup(so, oracle, dtype=np.float64, **params)
+    setup(sg, gpu, dtype=dt, **params)
+    return so, sg
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_sim_beff_torque_config2_like(gpu, oracle, prec):
+    """Config-2 physics (DMI, Ku1 || z, alpha 0.1, random m) on a small 3D grid."""
+    N, cs = (32, 24, 6), (2e-9, 2e-9, 2e-9)
+    so, sg = _both(oracle, gpu, N, cs, prec, Ms=1.1e6, Aex=1.6e-11, Dind=3e-3, Ku1=8e5, alpha=0.1)
+    m = random_unit((N[2], N[1], N[0]), 21)
+    so.set_m(m)
+    sg.set_m(m)
+    tol = 1e-12 if prec == 1 else 1e-5
+    b_o, b_g = so.beff(), sg.beff()
+    assert relerr(b_g, b_o) <= tol, relerr(b_g, b_o)
+    t_o, t_g = so.torque(), sg.torque()
+    assert relerr(t_g, t_o) <= tol, relerr(t_g, t_o)
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_sim_beff_regions_cubic_thermal(gpu, oracle, prec):
+    """Regions (harmonic-mean exchange), cubic anisotropy, region B_ext, frozen spins, T > 0."""
+    N, cs = (20, 16, 3), (2e-9, 2.5e-9, 3e-9)
+    rng = np.random.default_rng(8)
+    regions = rng.integers(0, 4, (N[2], N[1], N[0])).astype(np.uint8)
+    Ms = rng.uniform(6e5, 1e6, (N[2], N[1], N[0]))
+    lut = [np.zeros(256), np.zeros(256), np.zeros(256)]
+    lut[0][1], lut[2][2] = 0.05, -0.02
+    frozen = np.zeros(256)
+    frozen[3] = 1
+    so, sg = _both(oracle, gpu, N, cs, prec, Ms=Ms, Aex={0: 1e-11, 1: 2e-11, 2: 1.5e-11, 3: 1.2e-11},
+                   Kc=(4e5, -1e5, 0.0), c1=(1, 1, 0), c2=(-1, 1, 0), alpha=0.05, regions=regions,
+                   bext_lut=lut, frozen=frozen, temp=300.0, seed=99)
+    for s in (so, sg):
+        s.set_solver_params(dt=1e-14)
+    m = random_unit((N[2], N[1], N[0]), 5)
+    so.set_m(m)
+    sg.set_m(m)
+    tol = 1e-11 if prec == 1 else 1e-5
+    assert relerr(sg.beff(), so.beff()) <= tol
+    assert relerr(sg.torque(), so.torque()) <= tol
+
+
+# ---- steppers ---------------------------------------------------------------------------------
+def _macrospin(mod, solver, fixdt, t_end, dtype, alpha=0.1, Bz=0.1):
+    s = mod.Sim((1, 1, 1), (1e-9, 1e-9, 1e-9), dtype=dtype)
+    setup(s, mod, Ms=8e5, Aex=0.0, alpha=alpha, Bext=(0.0, 0.0, Bz), demag=False, dtype=dtype)
+    th0, ph0 = 1.2, 0.3
+    s.set_m(np.array([math.sin(th0) * math.cos(ph0), math.sin(th0) * math.sin(ph0), math.cos(th0)]).reshape(3, 1, 1, 1))
+    s.set_solver(solver)
+    s.set_solver_params(dt=1e-14, fixdt=fixdt)
+    s.run(t_end)
+    t_end = s.state()["t"]
+    g = GAMMA / (1 + alpha**2)
+    th = 2 * math.atan(math.tan(th0 / 2) * math.exp(-g * alpha * Bz * t_end))
+    ph = ph0 + g * Bz * t_end
+    want = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    return s.get_m().reshape(3), want
+
+
+@pytest.mark.parametrize("solver,fixdt,tol", [(5, 0.0, 2e-5), (5, 1e-13, 1e-7), (4, 1e-13, 1e-6), (2, 1e-14, 1e-4)])
+def test_macrospin_analytic(gpu, solver, fixdt, tol):
+    m, want = _macrospin(gpu, solver, fixdt, 0.2e-9, np.float64)
+    assert np.abs(m - want).max() < tol
+    m, want = _macrospin(gpu, solver, fixdt, 0.2e-9, np.float32)
+    assert np.abs(m - want).max() < max(tol, 2e-5)
+
+
+def test_gammaLL_heun_golden(gpu):
+    """test/gammaLL.mx3:6-28."""
+    for dt in (np.float64, np.float32):
+        N, c = 16, 5e-9
+        s = gpu.Sim((N, N, 1), (c, c, c), dtype=dt)
+        setup(s, gpu, Ms=1 / MU0, Aex=10e-12, alpha=0.001, Bext=(0.0, 0.0, 1.0), dtype=dt)
+        s.set_solver(gpu.SOLVER_HEUN)
+        s.set_solver_params(dt=1e-15, maxerr=1e-4)
+        s.set_m(np.ones((3, 1, N, N)))
+        s.run(0.1e-9)
+        s.set_solver_params(dt=s.state()["dt"], maxerr=1e-4, gammaLL=2 * GAMMA)
+        s.run(0.1e-9)
+        got = s.average_m()
+        assert np.abs(got - np.array([-0.286423, -0.746625, 0.5970549])).max() < 1.2e-4, (dt, got)
+
+
+def test_rk4_golden(gpu):
+    """test/rk4.mx3:6-28 (RK4 relax, fixed-step reversal, adaptive reversal)."""
+    for dt in (np.float64, np.float32):
+        s = gpu.Sim((128, 32, 1), (500e-9 / 128, 125e-9 / 32, 3e-9), dtype=dt)
+        setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=3.0, dtype=dt)
+        m = np.zeros((3, 1, 32, 128))
+        m[0], m[1] = 1, 0.1
+        s.set_m(m)
+        s.set_solver(gpu.SOLVER_RK4)
+        s.set_solver_params(dt=1e-15)
+        s.run(1e-9)
+        s.set_scalar(gpu.P_ALPHA, 0.02)
+        s.set_vector(gpu.P_BEXT, (-24.6e-3, 4.3e-3, 0.0))
+        s.set_solver_params(dt=s.state()["dt"], fixdt=1e-14)
+        s.run(0.1e-9)
+        got = s.average_m()
+        assert np.abs(got - np.array([0.59293, 0.63278, -0.08152])).max() < 1e-3, (dt, got)
+        s.set_solver_params(dt=1e-14, fixdt=0.0)
+        s.run(0.4e-9)
+        got = s.average_m()
+        assert np.abs(got - np.array([-0.8740, -0.2713, 0.01795])).max() < 1e-3, (dt, got)
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_config1_trajectory_vs_oracle(gpu, oracle, prec):
+    """Config 1 (128x32x1, RK45DP adaptive, MaxErr 1e-5): <m>(t) within 1e-4 of the FP64 oracle."""
+    N, cs = (128, 32, 1), (3.90625e-9, 3.90625e-9, 3e-9)
+    so, sg = _both(oracle, gpu, N, cs, prec, Ms=8e5, Aex=1.3e-11, alpha=0.02, Bext=(-24.6e-3, 4.3e-3, 0.0))
+    m = config1_m(1)
+    so.set_m(m)
+    sg.set_m(m)
+    for k in range(4):
+        so.run(0.025e-9)
+        sg.run(0.025e-9)
+        d = np.abs(sg.average_m() - so.average_m()).max()
+        _say("t", so.state()["t"], "d<m>", d, "steps", so.state()["nsteps"], sg.state()["nsteps"])
+        assert d < (1e-9 if prec == 1 else 1e-4)
+
+
+# ---- full-size properties (512 x 512 x 32) -----------------------------------------------------
+def test_config2_full_size_properties(gpu):
+    """Config 2 at full size: FP32 field within 1e-5 of the FP64 GPU path, demag
+    linearity and symmetry (<D u, v> = <u, D v>), unit |m| after RK45DP steps."""
+    import torch
+    N, cs = (512, 512, 32), (2e-9, 2e-9, 2e-9)
+    m = random_unit((N[2], N[1], N[0]), 2).astype(np.float32)
+    params = dict(Ms=1.1e6, Aex=1.6e-11, Dind=3e-3, Ku1=8e5, alpha=0.1)
+    s64 = gpu.Sim(N, cs, dtype=np.float64)
+    setup(s64, gpu, dtype=np.float64, **params)
+    s64.set_m(m)
+    b64 = s64.beff()
+    del s64
+    torch.cuda.empty_cache()
+    s32 = gpu.Sim(N, cs, dtype=np.float32)
+    setup(s32, gpu, dtype=np.float32, **params)
+    s32.set_m(m)
+    b32 = s32.beff()
+    e = relerr(b32, b64)
+    _say("config2 FP32 vs FP64 Beff relerr", e)
+    assert e <= 1e-5
+    # demag symmetry on the FP32 path: <D u, v> = <u, D v>
+    conv = gpu.DemagConvolution(N, cs, precision=0)
+    u = [torch.randn(N[2], N[1], N[0], device="cuda") for _ in range(3)]
+    v = [torch.randn(N[2], N[1], N[0], device="cuda") for _ in range(3)]
+    Du = [torch.zeros_like(u[0]) for _ in range(3)]
+    Dv = [torch.zeros_like(u[0]) for _ in range(3)]
+    conv.Exec(Du, u, None, 1 / MU0)
+    conv.Exec(Dv, v, None, 1 / MU0)
+    a = sum(float((Du[c].double() * v[c].double()).sum()) for c in range(3))
+    b = sum(float((u[c].double() * Dv[c].double()).sum()) for c in range(3))
+    assert abs(a - b) <= 1e-5 * max(abs(a), abs(b))
+    u2 = [2 * x for x in u]
+    D2 = [torch.zeros_like(u[0]) for _ in range(3)]
+    conv.Exec(D2, u2, None, 1 / MU0)
+    assert max(float((D2[c] - 2 * Du[c]).abs().max()) for c in range(3)) <= 1e-6 * max(float(Du[c].abs().max()) for c in range(3))
+    del conv, Du, Dv, D2, u, v, u2
+    s32.set_solver_params(fixdt=1e-14)
+    s32.steps(3)
+    mm = s32.get_m()
+    assert np.abs(np.linalg.norm(mm, axis=0) - 1).max() < 1e-6

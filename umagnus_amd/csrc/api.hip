@@ -1,0 +1,593 @@
+// C ABI of libumhip.so (declared in include/umhip.h).
+#include <cstring>
+#include <mutex>
+
+#include "field.hpp"
+#include "sim.hpp"
+#include "vec.hpp"
+
+namespace um {
+
+static thread_local std::string g_err;
+void set_last_error(const std::string &msg) { g_err = msg; }
+
+static std::mutex g_mu;
+static bool g_inited = false;
+static int g_device = 0;
+static hipStream_t g_own_stream = nullptr;
+static hipStream_t g_stream = nullptr;
+
+void ensure_init() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_inited) return;
+  int n = 0;
+  UM_HIP(hipGetDeviceCount(&n));
+  UM_REQUIRE(n > 0, UM_EHIP, "no HIP device");
+  UM_HIP(hipSetDevice(g_device));
+  UM_HIP(hipStreamCreateWithFlags(&g_own_stream, hipStreamNonBlocking));
+  g_stream = g_own_stream;
+  g_inited = true;
+}
+hipStream_t lib_stream() {
+  ensure_init();
+  return g_stream;
+}
+
+// host launchers defined in field.hip
+template <typename T>
+void add_exchange(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const uint8_t *regions, T wx,
+                  T wy, T wz, Mesh g, hipStream_t s);
+template <typename T>
+void add_dmi(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const T *dLUT,
+             const uint8_t *regions, T cx, T cy, T cz, Mesh g, int openbc, hipStream_t s);
+template <typename T>
+void add_uniaxial2(void *const B[3], const void *const m[3], Par<T> Ms, Par<T> K1, Par<T> K2, const Par<T> u[3],
+                   int N, hipStream_t s);
+template <typename T>
+void add_cubic2(void *const B[3], const void *const m[3], Par<T> Ms, Par<T> K1, Par<T> K2, Par<T> K3,
+                const Par<T> c1[3], const Par<T> c2[3], int N, hipStream_t s);
+template <typename T>
+void region_addv(void *const d[3], const void *const l[3], const uint8_t *regions, int N, hipStream_t s);
+template <typename T>
+void zeromask(void *dst, const void *lut, const uint8_t *regions, int N, hipStream_t s);
+template <typename T>
+void lltorque(void *const t[3], const void *const m[3], const void *const B[3], Par<T> alpha, int N,
+              hipStream_t s);
+template <typename T>
+void llnoprecess(void *const t[3], const void *const m[3], const void *const B[3], int N, hipStream_t s);
+template <typename T>
+void thermal_field(void *const B[3], uint64_t seed, uint64_t step, uint32_t stream, T k, Par<T> Ms, Par<T> temp,
+                   Par<T> alpha, int N, hipStream_t s);
+
+static void check_prec(int prec) { UM_REQUIRE(prec == UM_F32 || prec == UM_F64, UM_EINVAL, "prec must be UM_F32/UM_F64"); }
+static void check3(const void *const p[3], const char *what) {
+  UM_REQUIRE(p && p[0] && p[1] && p[2], UM_EINVAL, std::string(what) + ": NULL component");
+}
+
+// single-use scratch for reductions on the library stream
+struct RedScratch {
+  DevBuf red, partial, sum;
+  unsigned long long *h = nullptr;
+  double *hs = nullptr;
+  RedScratch() {
+    red.alloc(4 * sizeof(unsigned long long));
+    partial.alloc(RED_BLOCKS * sizeof(double));
+    sum.alloc(sizeof(double));
+    UM_HIP(hipHostMalloc((void **)&h, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+    UM_HIP(hipHostMalloc((void **)&hs, sizeof(double), hipHostMallocDefault));
+  }
+  ~RedScratch() {
+    if (h) (void)hipHostFree(h);
+    if (hs) (void)hipHostFree(hs);
+  }
+};
+static RedScratch &scratch() {
+  static RedScratch *r = new RedScratch();
+  return *r;
+}
+
+static double read_max_slot(hipStream_t s) {
+  RedScratch &r = scratch();
+  UM_HIP(hipMemcpyAsync(r.h, r.red.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  UM_HIP(hipStreamSynchronize(s));
+  double v;
+  memcpy(&v, &r.h[0], 8);
+  return v;
+}
+template <typename T>
+static double err_max(int mode, const void *const a[3], const void *const b[3], long N) {
+  hipStream_t s = lib_stream();
+  RedScratch &r = scratch();
+  fill_bytes(r.red.p, r.red.bytes, s);
+  ErrArgs<T> e{};
+  e.mode = mode;
+  for (int c = 0; c < 3; c++) {
+    e.k[0][c] = (const T *)a[c];
+    if (b) e.k[1][c] = (const T *)b[c];
+  }
+  e.N = N;
+  e.res = (unsigned long long *)r.red.p;
+  launch_err(e, s);
+  return read_max_slot(s);
+}
+template <typename T>
+static double sum_wdot(const void *a, const void *b, long N) {
+  hipStream_t s = lib_stream();
+  RedScratch &r = scratch();
+  const T *aa[1] = {(const T *)a};
+  const T *bb[1] = {(const T *)b};
+  launch_wdot<T>(aa, b ? bb : nullptr, 1, nullptr, T(1), N, r.partial.as<double>(), r.sum.as<double>(), s);
+  UM_HIP(hipMemcpyAsync(r.hs, r.sum.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  UM_HIP(hipStreamSynchronize(s));
+  return *r.hs;
+}
+
+}  // namespace um
+
+using namespace um;
+
+extern "C" {
+
+int um_init(int device) {
+  return guard([&] {
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      UM_REQUIRE(!g_inited || device == g_device, UM_ESTATE, "um_init: already initialised on another device");
+      g_device = device;
+    }
+    ensure_init();
+  });
+}
+const char *um_last_error(void) { return g_err.c_str(); }
+int um_device_info(char *name, int name_len, int *cu_count, size_t *mem_bytes) {
+  return guard([&] {
+    ensure_init();
+    hipDeviceProp_t p;
+    UM_HIP(hipGetDeviceProperties(&p, g_device));
+    if (name && name_len > 0) {
+      std::string s = std::string(p.name) + " (" + p.gcnArchName + ")";
+      strncpy(name, s.c_str(), name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (mem_bytes) *mem_bytes = p.totalGlobalMem;
+  });
+}
+int um_set_stream(void *s) {
+  return guard([&] {
+    ensure_init();
+    g_stream = (hipStream_t)s;  // NULL selects the HIP null stream
+  });
+}
+int um_sync(void) { return guard([&] { UM_HIP(hipStreamSynchronize(lib_stream())); }); }
+int um_malloc(void **ptr, size_t bytes) {
+  return guard([&] {
+    ensure_init();
+    UM_REQUIRE(ptr, UM_EINVAL, "um_malloc: NULL out pointer");
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) throw Error(UM_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  });
+}
+int um_free(void *ptr) { return guard([&] { if (ptr) UM_HIP(hipFree(ptr)); }); }
+int um_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  return guard([&] {
+    UM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, lib_stream()));
+    UM_HIP(hipStreamSynchronize(lib_stream()));
+  });
+}
+int um_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  return guard([&] {
+    UM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, lib_stream()));
+    UM_HIP(hipStreamSynchronize(lib_stream()));
+  });
+}
+int um_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  return guard([&] { UM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, lib_stream())); });
+}
+int um_zero(void *dst, size_t bytes) { return guard([&] { fill_bytes(dst, bytes, lib_stream()); }); }
+int um_memset(int prec, void *dst, double value, size_t n) {
+  return guard([&] {
+    check_prec(prec);
+    if (prec == UM_F32) fill_value<float>((float *)dst, (float)value, n, lib_stream());
+    else fill_value<double>((double *)dst, value, n, lib_stream());
+  });
+}
+
+#define DISPATCH(prec, CALL_F, CALL_D) \
+  do {                                 \
+    check_prec(prec);                  \
+    if ((prec) == UM_F32) { CALL_F; }  \
+    else { CALL_D; }                   \
+  } while (0)
+
+int um_add_exchange(int prec, void *const B[3], const void *const m[3], const void *Ms, double Ms_mul,
+                    const void *aLUT, const uint8_t *regions, double wx, double wy, double wz, int Nx, int Ny,
+                    int Nz, int pbc) {
+  return guard([&] {
+    check3(B, "B"); check3(m, "m");
+    UM_REQUIRE(aLUT && Nx > 0 && Ny > 0 && Nz > 0, UM_EINVAL, "um_add_exchange: bad arguments");
+    Mesh g{Nx, Ny, Nz, pbc};
+    DISPATCH(prec,
+             add_exchange<float>(B, m, par<float>(Ms, Ms_mul), (const float *)aLUT, regions, (float)wx, (float)wy,
+                                 (float)wz, g, lib_stream()),
+             add_exchange<double>(B, m, par<double>(Ms, Ms_mul), (const double *)aLUT, regions, wx, wy, wz, g,
+                                  lib_stream()));
+  });
+}
+int um_add_dmi(int prec, void *const B[3], const void *const m[3], const void *Ms, double Ms_mul, const void *aLUT,
+               const void *dLUT, const uint8_t *regions, double cx, double cy, double cz, int Nx, int Ny, int Nz,
+               int pbc, int openbc) {
+  return guard([&] {
+    check3(B, "B"); check3(m, "m");
+    UM_REQUIRE(aLUT && dLUT && Nx > 0 && Ny > 0 && Nz > 0, UM_EINVAL, "um_add_dmi: bad arguments");
+    Mesh g{Nx, Ny, Nz, pbc};
+    DISPATCH(prec,
+             add_dmi<float>(B, m, par<float>(Ms, Ms_mul), (const float *)aLUT, (const float *)dLUT, regions,
+                            (float)cx, (float)cy, (float)cz, g, openbc, lib_stream()),
+             add_dmi<double>(B, m, par<double>(Ms, Ms_mul), (const double *)aLUT, (const double *)dLUT, regions,
+                             cx, cy, cz, g, openbc, lib_stream()));
+  });
+}
+int um_add_uniaxial2(int prec, void *const B[3], const void *const m[3], const void *Ms, double Ms_mul,
+                     const void *K1, double K1_mul, const void *K2, double K2_mul, const void *const u[3],
+                     const double u_mul[3], int N) {
+  return guard([&] {
+    check3(B, "B"); check3(m, "m");
+    UM_REQUIRE(u && u_mul && N >= 0, UM_EINVAL, "um_add_uniaxial2: bad arguments");
+    auto go = [&](auto zero) {
+      using T = decltype(zero);
+      Par<T> up[3];
+      for (int c = 0; c < 3; c++) up[c] = par<T>(u[c], u_mul[c]);
+      add_uniaxial2<T>(B, m, par<T>(Ms, Ms_mul), par<T>(K1, K1_mul), par<T>(K2, K2_mul), up, N, lib_stream());
+    };
+    DISPATCH(prec, go(0.0f), go(0.0));
+  });
+}
+int um_add_cubic2(int prec, void *const B[3], const void *const m[3], const void *Ms, double Ms_mul, const void *K1,
+                  double K1_mul, const void *K2, double K2_mul, const void *K3, double K3_mul,
+                  const void *const c1[3], const double c1_mul[3], const void *const c2[3], const double c2_mul[3],
+                  int N) {
+  return guard([&] {
+    check3(B, "B"); check3(m, "m");
+    UM_REQUIRE(c1 && c2 && c1_mul && c2_mul && N >= 0, UM_EINVAL, "um_add_cubic2: bad arguments");
+    auto go = [&](auto zero) {
+      using T = decltype(zero);
+      Par<T> a[3], b[3];
+      for (int c = 0; c < 3; c++) { a[c] = par<T>(c1[c], c1_mul[c]); b[c] = par<T>(c2[c], c2_mul[c]); }
+      add_cubic2<T>(B, m, par<T>(Ms, Ms_mul), par<T>(K1, K1_mul), par<T>(K2, K2_mul), par<T>(K3, K3_mul), a, b, N,
+                    lib_stream());
+    };
+    DISPATCH(prec, go(0.0f), go(0.0));
+  });
+}
+int um_region_addv(int prec, void *const dst[3], const void *const lut[3], const uint8_t *regions, int N) {
+  return guard([&] {
+    check3(dst, "dst"); check3(lut, "lut");
+    DISPATCH(prec, region_addv<float>(dst, lut, regions, N, lib_stream()),
+             region_addv<double>(dst, lut, regions, N, lib_stream()));
+  });
+}
+int um_zeromask(int prec, void *dst, const void *maskLUT, const uint8_t *regions, int N) {
+  return guard([&] {
+    UM_REQUIRE(dst && maskLUT, UM_EINVAL, "um_zeromask: NULL argument");
+    DISPATCH(prec, zeromask<float>(dst, maskLUT, regions, N, lib_stream()),
+             zeromask<double>(dst, maskLUT, regions, N, lib_stream()));
+  });
+}
+int um_lltorque(int prec, void *const t[3], const void *const m[3], const void *const B[3], const void *alpha,
+                double alpha_mul, int N) {
+  return guard([&] {
+    check3(t, "t"); check3(m, "m"); check3(B, "B");
+    DISPATCH(prec, lltorque<float>(t, m, B, par<float>(alpha, alpha_mul), N, lib_stream()),
+             lltorque<double>(t, m, B, par<double>(alpha, alpha_mul), N, lib_stream()));
+  });
+}
+int um_llnoprecess(int prec, void *const t[3], const void *const m[3], const void *const B[3], int N) {
+  return guard([&] {
+    check3(t, "t"); check3(m, "m"); check3(B, "B");
+    DISPATCH(prec, llnoprecess<float>(t, m, B, N, lib_stream()), llnoprecess<double>(t, m, B, N, lib_stream()));
+  });
+}
+int um_thermal_field(int prec, void *const B[3], uint64_t seed, uint64_t step, uint32_t stream, double k,
+                     const void *Ms, double Ms_mul, const void *temp, double temp_mul, const void *alpha,
+                     double alpha_mul, int N) {
+  return guard([&] {
+    check3(B, "B");
+    DISPATCH(prec,
+             thermal_field<float>(B, seed, step, stream, (float)k, par<float>(Ms, Ms_mul), par<float>(temp, temp_mul),
+                                  par<float>(alpha, alpha_mul), N, lib_stream()),
+             thermal_field<double>(B, seed, step, stream, k, par<double>(Ms, Ms_mul), par<double>(temp, temp_mul),
+                                   par<double>(alpha, alpha_mul), N, lib_stream()));
+  });
+}
+
+int um_madd(int prec, void *dst, int nsrc, const void *const *src, const double *fac, int N) {
+  return guard([&] {
+    UM_REQUIRE(dst && src && fac && nsrc >= 1 && nsrc <= 8, UM_EINVAL, "um_madd: bad arguments");
+    for (int j = 0; j < nsrc; j++) UM_REQUIRE(src[j], UM_EINVAL, "um_madd: NULL source");
+    auto go = [&](auto zero) {
+      using T = decltype(zero);
+      StageArgs<T> a{};
+      a.nsrc = nsrc;
+      a.ncomp = 1;
+      a.group3 = 1;
+      a.normalize = 0;
+      for (int j = 0; j < nsrc; j++) {
+        a.src[j][0] = (const T *)src[j];
+        a.fac[j] = (T)fac[j];
+      }
+      a.dst[0] = (T *)dst;
+      a.vol = nullptr;
+      a.N = N;
+      launch_stage(a, lib_stream());
+    };
+    DISPATCH(prec, go(0.0f), go(0.0));
+  });
+}
+int um_normalize(int prec, void *const v[3], const void *vol, int N) {
+  return guard([&] {
+    check3(v, "v");
+    auto go = [&](auto zero) {
+      using T = decltype(zero);
+      StageArgs<T> a{};
+      a.nsrc = 1;
+      a.ncomp = 3;
+      a.normalize = 1;
+      for (int c = 0; c < 3; c++) { a.src[0][c] = (const T *)v[c]; a.dst[c] = (T *)v[c]; }
+      a.fac[0] = T(1);
+      a.vol = (const T *)vol;
+      a.N = N;
+      launch_stage(a, lib_stream());
+    };
+    DISPATCH(prec, go(0.0f), go(0.0));
+  });
+}
+int um_vecnorm(int prec, void *dst, const void *const a[3], int N) {
+  return guard([&] {
+    check3(a, "a");
+    DISPATCH(prec, launch_vecnorm<float>((float *)dst, (const float *const *)a, N, lib_stream()),
+             launch_vecnorm<double>((double *)dst, (const double *const *)a, N, lib_stream()));
+  });
+}
+int um_div(int prec, void *dst, const void *a, const void *b, int N) {
+  return guard([&] {
+    DISPATCH(prec, launch_div<float>((float *)dst, (const float *)a, (const float *)b, N, lib_stream()),
+             launch_div<double>((double *)dst, (const double *)a, (const double *)b, N, lib_stream()));
+  });
+}
+int um_mul(int prec, void *dst, const void *a, const void *b, int N) {
+  return guard([&] {
+    DISPATCH(prec, launch_mul<float>((float *)dst, (const float *)a, (const float *)b, N, lib_stream()),
+             launch_mul<double>((double *)dst, (const double *)a, (const double *)b, N, lib_stream()));
+  });
+}
+
+// ---- reductions ---------------------------------------------------------------------
+int um_reduce_sum(int prec, const void *x, int N, double *out) {
+  return guard([&] {
+    UM_REQUIRE(x && out, UM_EINVAL, "um_reduce_sum: NULL argument");
+    DISPATCH(prec, *out = sum_wdot<float>(x, nullptr, N), *out = sum_wdot<double>(x, nullptr, N));
+  });
+}
+int um_reduce_dot(int prec, const void *a, const void *b, int N, double *out) {
+  return guard([&] {
+    UM_REQUIRE(a && b && out, UM_EINVAL, "um_reduce_dot: NULL argument");
+    DISPATCH(prec, *out = sum_wdot<float>(a, b, N), *out = sum_wdot<double>(a, b, N));
+  });
+}
+int um_reduce_maxabs(int prec, const void *x, int N, double *out) {
+  return guard([&] {
+    UM_REQUIRE(x && out, UM_EINVAL, "um_reduce_maxabs: NULL argument");
+    hipStream_t s = lib_stream();
+    RedScratch &r = scratch();
+    fill_bytes(r.red.p, r.red.bytes, s);
+    DISPATCH(prec, launch_maxabs<float>((const float *)x, N, (unsigned long long *)r.red.p, s),
+             launch_maxabs<double>((const double *)x, N, (unsigned long long *)r.red.p, s));
+    *out = read_max_slot(s);
+  });
+}
+int um_reduce_maxvecnorm(int prec, const void *const v[3], int N, double *out) {
+  return guard([&] {
+    check3(v, "v");
+    UM_REQUIRE(out, UM_EINVAL, "NULL out");
+    DISPATCH(prec, *out = std::sqrt(err_max<float>(2, v, nullptr, N)),
+             *out = std::sqrt(err_max<double>(2, v, nullptr, N)));
+  });
+}
+int um_reduce_maxvecdiff(int prec, const void *const a[3], const void *const b[3], int N, int sqrt3_quirk,
+                         double *out) {
+  return guard([&] {
+    check3(a, "a"); check3(b, "b");
+    UM_REQUIRE(out, UM_EINVAL, "NULL out");
+    double m2 = 0;
+    DISPATCH(prec, m2 = err_max<float>(1, a, b, N), m2 = err_max<double>(1, a, b, N));
+    *out = std::sqrt(sqrt3_quirk ? 3.0 * m2 : m2);
+  });
+}
+
+// ---- demag ----------------------------------------------------------------------------
+int um_demag_create(int prec, int Nx, int Ny, int Nz, const int pbc[3], const double cs[3], double accuracy,
+                    void **plan) {
+  return guard([&] {
+    check_prec(prec);
+    UM_REQUIRE(plan && cs && Nx > 0 && Ny > 0 && Nz > 0, UM_EINVAL, "um_demag_create: bad arguments");
+    const int zero[3] = {0, 0, 0};
+    const int *pb = pbc ? pbc : zero;
+    const int N[3] = {Nx, Ny, Nz};
+    int P[3];
+    pad_size(N, pb, P);
+    hipStream_t s = lib_stream();
+    const size_t np = (size_t)P[0] * P[1] * P[2];
+    DevBuf K(sizeof(double) * 6 * np);
+    double *Kd[6];
+    for (int c = 0; c < 6; c++) Kd[c] = K.as<double>() + c * np;
+    demag_kernel_bf_device(N, pb, cs, accuracy, Kd, P, s);
+    *plan = demag_create(prec, N, P, Kd, s);
+  });
+}
+int um_demag_create_from_kernel(int prec, int Nx, int Ny, int Nz, const int P[3], const double *const K[6],
+                                void **plan) {
+  return guard([&] {
+    check_prec(prec);
+    UM_REQUIRE(plan && P && K && Nx > 0 && Ny > 0 && Nz > 0, UM_EINVAL, "um_demag_create_from_kernel: bad arguments");
+    UM_REQUIRE(P[0] >= Nx && P[1] >= Ny && P[2] >= Nz, UM_EINVAL, "padded size smaller than the grid");
+    hipStream_t s = lib_stream();
+    const size_t np = (size_t)P[0] * P[1] * P[2];
+    DevBuf Kb(sizeof(double) * 6 * np);
+    double *Kd[6];
+    for (int c = 0; c < 6; c++) {
+      Kd[c] = Kb.as<double>() + c * np;
+      UM_REQUIRE(K[c], UM_EINVAL, "NULL tensor component");
+      UM_HIP(hipMemcpyAsync(Kd[c], K[c], sizeof(double) * np, hipMemcpyHostToDevice, s));
+    }
+    const int N[3] = {Nx, Ny, Nz};
+    *plan = demag_create(prec, N, P, Kd, s);
+  });
+}
+int um_demag_exec(void *plan, void *const B[3], const void *const m[3], const void *vol, const void *Ms,
+                  double Ms_mul) {
+  return guard([&] {
+    UM_REQUIRE(plan, UM_EINVAL, "NULL plan");
+    check3(B, "B"); check3(m, "m");
+    demag_exec((DemagBase *)plan, B, m, vol, Ms, Ms_mul, lib_stream());
+  });
+}
+int um_demag_destroy(void *plan) {
+  return guard([&] {
+    if (plan) {
+      UM_HIP(hipStreamSynchronize(lib_stream()));
+      demag_destroy((DemagBase *)plan);
+    }
+  });
+}
+int um_demag_info(void *plan, int P[3]) {
+  return guard([&] {
+    UM_REQUIRE(plan && P, UM_EINVAL, "NULL argument");
+    int prec;
+    demag_info((DemagBase *)plan, P, &prec);
+  });
+}
+int um_demag_kspec(void *plan, int c, void *host_out) {
+  return guard([&] {
+    UM_REQUIRE(plan && host_out, UM_EINVAL, "NULL argument");
+    UM_HIP(hipStreamSynchronize(lib_stream()));
+    demag_kspec((DemagBase *)plan, c, host_out);
+  });
+}
+int um_demag_kernel_bf(int Nx, int Ny, int Nz, const int pbc[3], const double cs[3], double accuracy,
+                       double *const K[6]) {
+  return guard([&] {
+    UM_REQUIRE(cs && K && Nx > 0 && Ny > 0 && Nz > 0, UM_EINVAL, "um_demag_kernel_bf: bad arguments");
+    const int zero[3] = {0, 0, 0};
+    const int *pb = pbc ? pbc : zero;
+    const int N[3] = {Nx, Ny, Nz};
+    int P[3];
+    pad_size(N, pb, P);
+    hipStream_t s = lib_stream();
+    const size_t np = (size_t)P[0] * P[1] * P[2];
+    DevBuf Kb(sizeof(double) * 6 * np);
+    double *Kd[6];
+    for (int c = 0; c < 6; c++) Kd[c] = Kb.as<double>() + c * np;
+    demag_kernel_bf_device(N, pb, cs, accuracy, Kd, P, s);
+    for (int c = 0; c < 6; c++) UM_HIP(hipMemcpyAsync(K[c], Kd[c], sizeof(double) * np, hipMemcpyDeviceToHost, s));
+    UM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// ---- simulation handle ---------------------------------------------------------------
+#define SIM(h) ((SimBase *)(h))
+#define SIMCHK(h) UM_REQUIRE((h) != nullptr, UM_EINVAL, "NULL simulation handle")
+
+int um_sim_create(int prec, int Nx, int Ny, int Nz, double cx, double cy, double cz, int pbcx, int pbcy, int pbcz,
+                  void **sim) {
+  return guard([&] {
+    check_prec(prec);
+    UM_REQUIRE(sim, UM_EINVAL, "NULL out pointer");
+    const int N[3] = {Nx, Ny, Nz};
+    const double cs[3] = {cx, cy, cz};
+    const int pbc[3] = {pbcx, pbcy, pbcz};
+    *sim = sim_create(prec, N, cs, pbc);
+  });
+}
+int um_sim_destroy(void *sim) { return guard([&] { delete SIM(sim); }); }
+int um_sim_set_scalar(void *sim, int id, const void *arr, double mul) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->set_scalar(id, arr, mul); });
+}
+int um_sim_set_vector(void *sim, int id, const void *ax, const void *ay, const void *az, double mx, double my,
+                      double mz) {
+  return guard([&] {
+    SIMCHK(sim);
+    const void *a[3] = {ax, ay, az};
+    const double m[3] = {mx, my, mz};
+    SIM(sim)->set_vector(id, a, m);
+  });
+}
+int um_sim_set_regions(void *sim, const uint8_t *r) { return guard([&] { SIMCHK(sim); SIM(sim)->set_regions(r); }); }
+int um_sim_set_exchange_luts(void *sim, const void *a, const void *d) {
+  return guard([&] {
+    SIMCHK(sim);
+    UM_REQUIRE(a, UM_EINVAL, "NULL exchange LUT");
+    SIM(sim)->set_exchange_luts(a, d);
+  });
+}
+int um_sim_set_bext_lut(void *sim, const void *lx, const void *ly, const void *lz) {
+  return guard([&] {
+    SIMCHK(sim);
+    const void *l[3] = {lx, ly, lz};
+    SIM(sim)->set_bext_lut(l);
+  });
+}
+int um_sim_set_frozen_lut(void *sim, const void *lut) { return guard([&] { SIMCHK(sim); SIM(sim)->set_frozen_lut(lut); }); }
+int um_sim_set_geometry(void *sim, const void *vol) { return guard([&] { SIMCHK(sim); SIM(sim)->set_geometry(vol); }); }
+int um_sim_set_flags(void *sim, int d, int ob, int pr, int hs) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->set_flags(d, ob, pr, hs); });
+}
+int um_sim_set_demag_accuracy(void *sim, double a) { return guard([&] { SIMCHK(sim); SIM(sim)->set_demag_accuracy(a); }); }
+int um_sim_set_thermal_seed(void *sim, uint64_t seed, uint32_t stream) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->set_thermal_seed(seed, stream); });
+}
+int um_sim_set_m(void *sim, const void *mx, const void *my, const void *mz) {
+  return guard([&] {
+    SIMCHK(sim);
+    UM_REQUIRE(mx && my && mz, UM_EINVAL, "NULL m component");
+    const void *m[3] = {mx, my, mz};
+    SIM(sim)->set_m(m);
+  });
+}
+int um_sim_get_m(void *sim, void *mx, void *my, void *mz) {
+  return guard([&] {
+    SIMCHK(sim);
+    UM_REQUIRE(mx && my && mz, UM_EINVAL, "NULL m component");
+    void *m[3] = {mx, my, mz};
+    SIM(sim)->get_m(m);
+  });
+}
+int um_sim_m_device(void *sim, void *ptrs[3]) { return guard([&] { SIMCHK(sim); SIM(sim)->m_device(ptrs); }); }
+int um_sim_beff(void *sim, void *bx, void *by, void *bz) {
+  return guard([&] { SIMCHK(sim); void *o[3] = {bx, by, bz}; SIM(sim)->beff(o); });
+}
+int um_sim_torque(void *sim, void *tx, void *ty, void *tz) {
+  return guard([&] { SIMCHK(sim); void *o[3] = {tx, ty, tz}; SIM(sim)->torque(o); });
+}
+int um_sim_demag_field(void *sim, void *bx, void *by, void *bz) {
+  return guard([&] { SIMCHK(sim); void *o[3] = {bx, by, bz}; SIM(sim)->demag_field(o); });
+}
+int um_sim_set_solver(void *sim, int s) { return guard([&] { SIMCHK(sim); SIM(sim)->set_solver(s); }); }
+int um_sim_set_solver_params(void *sim, const double k[9]) {
+  return guard([&] { SIMCHK(sim); UM_REQUIRE(k, UM_EINVAL, "NULL knobs"); SIM(sim)->set_solver_params(k); });
+}
+int um_sim_get_state(void *sim, double out[8]) { return guard([&] { SIMCHK(sim); SIM(sim)->get_state(out); }); }
+int um_sim_step(void *sim) { return guard([&] { SIMCHK(sim); SIM(sim)->step(); }); }
+int um_sim_steps(void *sim, int n) { return guard([&] { SIMCHK(sim); SIM(sim)->steps(n); }); }
+int um_sim_run(void *sim, double seconds) { return guard([&] { SIMCHK(sim); SIM(sim)->run(seconds); }); }
+int um_sim_average_m(void *sim, double out[3]) { return guard([&] { SIMCHK(sim); SIM(sim)->average_m(out); }); }
+int um_sim_energy(void *sim, int term, double *out) {
+  return guard([&] { SIMCHK(sim); UM_REQUIRE(out, UM_EINVAL, "NULL out"); *out = SIM(sim)->energy(term); });
+}
+int um_sim_stream(void *sim, void **s) { return guard([&] { SIMCHK(sim); *s = (void *)SIM(sim)->stream(); }); }
+int um_sim_set_profiling(void *sim, int on) { return guard([&] { SIMCHK(sim); SIM(sim)->set_profiling(on); }); }
+int um_sim_kernel_times(void *sim, double out_ms[3], int *n_evals) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->kernel_times(out_ms, n_evals); });
+}
+
+}  // extern "C"

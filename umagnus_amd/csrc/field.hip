@@ -1,0 +1,294 @@
+// Field-term kernels: unfused entry points (opencl/exchange.go, dmi.go,
+// anisotropy.go, region.go, zeromask.go, lltorque.go, temperature.go) and the
+// fused effective-field + torque pass.
+#include "field.hpp"
+
+namespace um {
+
+// ---- unfused stencil terms ---------------------------------------------------
+template <typename T>
+__global__ void k_addexchange(T *Bx, T *By, T *Bz, const T *mx, const T *my, const T *mz, Par<T> Ms,
+                              const T *aLUT, const uint8_t *regions, T wx, T wy, T wz, Mesh g) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  const int iz = blockIdx.z;
+  if (ix >= g.Nx || iy >= g.Ny) return;
+  const T *m[3] = {mx, my, mz};
+  const long I = g.idx(ix, iy, iz);
+  const V3<T> m0 = load3(m, I);
+  if (is0(m0)) return;
+  const V3<T> B = exchange_cell(g, ix, iy, iz, I, m0, m, regions, aLUT, wx, wy, wz);
+  const T invMs = inv_msat(Ms, I);
+  Bx[I] += B.x * invMs;
+  By[I] += B.y * invMs;
+  Bz[I] += B.z * invMs;
+}
+
+template <typename T>
+__global__ void k_adddmi(T *Bx, T *By, T *Bz, const T *mx, const T *my, const T *mz, Par<T> Ms,
+                         const T *aLUT, const T *dLUT, const uint8_t *regions, T cx, T cy, T cz, Mesh g,
+                         int openbc) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  const int iz = blockIdx.z;
+  if (ix >= g.Nx || iy >= g.Ny) return;
+  const T *m[3] = {mx, my, mz};
+  const long I = g.idx(ix, iy, iz);
+  const V3<T> m0 = load3(m, I);
+  if (is0(m0)) return;
+  const V3<T> h = dmi_cell(g, ix, iy, iz, I, m0, m, regions, aLUT, dLUT, cx, cy, cz, openbc);
+  const T invMs = inv_msat(Ms, I);
+  Bx[I] += h.x * invMs;
+  By[I] += h.y * invMs;
+  Bz[I] += h.z * invMs;
+}
+
+template <typename T>
+__global__ void k_adduniaxial2(T *Bx, T *By, T *Bz, const T *mx, const T *my, const T *mz, Par<T> Ms,
+                               Par<T> K1, Par<T> K2, Par<T> ux, Par<T> uy, Par<T> uz, int N) {
+  const Par<T> u[3] = {ux, uy, uz};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const V3<T> B = uniaxial_cell(i, v3(mx[i], my[i], mz[i]), Ms, K1, K2, u);
+    Bx[i] += B.x;
+    By[i] += B.y;
+    Bz[i] += B.z;
+  }
+}
+
+template <typename T>
+struct CubicArgs {
+  T *B[3];
+  const T *m[3];
+  Par<T> Ms, K1, K2, K3, c1[3], c2[3];
+  int N;
+};
+template <typename T>
+__global__ void k_addcubic2(CubicArgs<T> a) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
+    const V3<T> B = cubic_cell(i, load3(a.m, i), a.Ms, a.K1, a.K2, a.K3, a.c1, a.c2);
+    a.B[0][i] += B.x;
+    a.B[1][i] += B.y;
+    a.B[2][i] += B.z;
+  }
+}
+
+template <typename T>
+__global__ void k_regionaddv(T *dx, T *dy, T *dz, const T *lx, const T *ly, const T *lz, const uint8_t *regions,
+                             int N) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const int r = regions ? regions[i] : 0;
+    dx[i] += lx[r];
+    dy[i] += ly[r];
+    dz[i] += lz[r];
+  }
+}
+
+template <typename T>
+__global__ void k_zeromask(T *dst, const T *maskLUT, const uint8_t *regions, int N) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+    if (maskLUT[regions ? regions[i] : 0] != T(0)) dst[i] = T(0);
+}
+
+template <typename T>
+__global__ void k_lltorque(T *tx, T *ty, T *tz, const T *mx, const T *my, const T *mz, const T *hx, const T *hy,
+                           const T *hz, Par<T> alpha, int N) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const V3<T> t = lltorque_cell(v3(mx[i], my[i], mz[i]), v3(hx[i], hy[i], hz[i]), alpha.at(i));
+    tx[i] = t.x;
+    ty[i] = t.y;
+    tz[i] = t.z;
+  }
+}
+
+template <typename T>
+__global__ void k_llnoprecess(T *tx, T *ty, T *tz, const T *mx, const T *my, const T *mz, const T *hx,
+                              const T *hy, const T *hz, int N) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const V3<T> m = v3(mx[i], my[i], mz[i]);
+    const V3<T> t = vcross(m, vcross(m, v3(hx[i], hy[i], hz[i])));
+    tx[i] = -t.x;
+    ty[i] = -t.y;
+    tz[i] = -t.z;
+  }
+}
+
+template <typename T>
+__global__ void k_thermal(T *Bx, T *By, T *Bz, uint64_t seed, uint64_t step, uint32_t stream, T k, Par<T> Ms,
+                          Par<T> temp, Par<T> alpha, int N) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const V3<T> b = thermal_cell(i, seed, step, stream, k, Ms, temp, alpha);
+    Bx[i] = b.x;
+    By[i] = b.y;
+    Bz[i] = b.z;
+  }
+}
+
+// ---- fused effective field + torque ------------------------------------------------
+// B = B_demag + B_exch(+DMI) + B_anis + B_ext + B_therm in the order of
+// SetEffectiveField (engine/effectivefield.go:14-24), then tau = LLTorque(m, B)
+// (lltorque2.cl) and FreezeSpins (engine/torque.go:348-352), one pass per cell.
+template <typename T>
+__global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  const int iz = blockIdx.z;
+  const Mesh &g = a.g;
+  if (ix >= g.Nx || iy >= g.Ny) return;
+  const long I = g.idx(ix, iy, iz);
+  const V3<T> m0 = load3(a.m, I);
+  V3<T> B = a.bd[0] ? load3(a.bd, I) : v3(T(0), T(0), T(0));
+  if (a.exch_mode != 0 && !is0(m0)) {
+    const V3<T> h = (a.exch_mode == 1)
+                        ? exchange_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.wx, a.wy, a.wz)
+                        : dmi_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.dLUT, a.cx, a.cy, a.cz, a.openbc);
+    const T invMs = inv_msat(a.Ms, I);
+    B.x += h.x * invMs;
+    B.y += h.y * invMs;
+    B.z += h.z * invMs;
+  }
+  if (a.has_uniax) {
+    const V3<T> h = uniaxial_cell(I, m0, a.Ms, a.Ku1, a.Ku2, a.u);
+    B.x += h.x; B.y += h.y; B.z += h.z;
+  }
+  if (a.has_cubic) {
+    const V3<T> h = cubic_cell(I, m0, a.Ms, a.Kc1, a.Kc2, a.Kc3, a.c1, a.c2);
+    B.x += h.x; B.y += h.y; B.z += h.z;
+  }
+  if (a.bextLUT[0]) {
+    const int r = a.regions ? a.regions[I] : 0;
+    B.x += a.bextLUT[0][r];
+    B.y += a.bextLUT[1][r];
+    B.z += a.bextLUT[2][r];
+  }
+  if (a.has_bext) {
+    B.x += a.bext[0].at(I);
+    B.y += a.bext[1].at(I);
+    B.z += a.bext[2].at(I);
+  }
+  if (a.has_therm) {
+    const V3<T> h = thermal_cell(I, a.seed, a.step, a.stream, a.kth, a.Ms, a.temp, a.alpha);
+    B.x += h.x; B.y += h.y; B.z += h.z;
+  }
+  if (a.beff[0]) {
+    a.beff[0][I] = B.x;
+    a.beff[1][I] = B.y;
+    a.beff[2][I] = B.z;
+  }
+  if (a.tau[0]) {
+    V3<T> t;
+    if (a.precess) {
+      t = lltorque_cell(m0, B, a.alpha.at(I));
+    } else {
+      const V3<T> c = vcross(m0, vcross(m0, B));
+      t = v3(-c.x, -c.y, -c.z);
+    }
+    if (a.frozenLUT && a.frozenLUT[a.regions ? a.regions[I] : 0] != T(0)) t = v3(T(0), T(0), T(0));
+    a.tau[0][I] = t.x;
+    a.tau[1][I] = t.y;
+    a.tau[2][I] = t.z;
+  }
+}
+
+template <typename T>
+void launch_field_torque(const FieldArgs<T> &a, hipStream_t s) {
+  dim3 blk(64, 4, 1);
+  dim3 grd((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.g.Nz);
+  hipLaunchKernelGGL(k_field_torque<T>, grd, blk, 0, s, a);
+  UM_HIP(hipGetLastError());
+}
+template void launch_field_torque<float>(const FieldArgs<float> &, hipStream_t);
+template void launch_field_torque<double>(const FieldArgs<double> &, hipStream_t);
+
+// ---- host launchers for the unfused entry points --------------------------------------
+static inline dim3 grid3(const Mesh &g) { return dim3((g.Nx + 63) / 64, (g.Ny + 3) / 4, g.Nz); }
+
+template <typename T>
+void add_exchange(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const uint8_t *regions, T wx,
+                  T wy, T wz, Mesh g, hipStream_t s) {
+  hipLaunchKernelGGL(k_addexchange<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2],
+                     (const T *)m[0], (const T *)m[1], (const T *)m[2], Ms, aLUT, regions, wx, wy, wz, g);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void add_dmi(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const T *dLUT,
+             const uint8_t *regions, T cx, T cy, T cz, Mesh g, int openbc, hipStream_t s) {
+  hipLaunchKernelGGL(k_adddmi<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2], (const T *)m[0],
+                     (const T *)m[1], (const T *)m[2], Ms, aLUT, dLUT, regions, cx, cy, cz, g, openbc);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void add_uniaxial2(void *const B[3], const void *const m[3], Par<T> Ms, Par<T> K1, Par<T> K2, const Par<T> u[3],
+                   int N, hipStream_t s) {
+  hipLaunchKernelGGL(k_adduniaxial2<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)B[0], (T *)B[1],
+                     (T *)B[2], (const T *)m[0], (const T *)m[1], (const T *)m[2], Ms, K1, K2, u[0], u[1], u[2], N);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void add_cubic2(void *const B[3], const void *const m[3], Par<T> Ms, Par<T> K1, Par<T> K2, Par<T> K3,
+                const Par<T> c1[3], const Par<T> c2[3], int N, hipStream_t s) {
+  CubicArgs<T> a{};
+  for (int c = 0; c < 3; c++) {
+    a.B[c] = (T *)B[c];
+    a.m[c] = (const T *)m[c];
+    a.c1[c] = c1[c];
+    a.c2[c] = c2[c];
+  }
+  a.Ms = Ms; a.K1 = K1; a.K2 = K2; a.K3 = K3; a.N = N;
+  hipLaunchKernelGGL(k_addcubic2<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, a);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void region_addv(void *const d[3], const void *const l[3], const uint8_t *regions, int N, hipStream_t s) {
+  hipLaunchKernelGGL(k_regionaddv<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)d[0], (T *)d[1],
+                     (T *)d[2], (const T *)l[0], (const T *)l[1], (const T *)l[2], regions, N);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void zeromask(void *dst, const void *lut, const uint8_t *regions, int N, hipStream_t s) {
+  hipLaunchKernelGGL(k_zeromask<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)dst, (const T *)lut,
+                     regions, N);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void lltorque(void *const t[3], const void *const m[3], const void *const B[3], Par<T> alpha, int N,
+              hipStream_t s) {
+  hipLaunchKernelGGL(k_lltorque<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)t[0], (T *)t[1],
+                     (T *)t[2], (const T *)m[0], (const T *)m[1], (const T *)m[2], (const T *)B[0],
+                     (const T *)B[1], (const T *)B[2], alpha, N);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void llnoprecess(void *const t[3], const void *const m[3], const void *const B[3], int N, hipStream_t s) {
+  hipLaunchKernelGGL(k_llnoprecess<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)t[0], (T *)t[1],
+                     (T *)t[2], (const T *)m[0], (const T *)m[1], (const T *)m[2], (const T *)B[0],
+                     (const T *)B[1], (const T *)B[2], N);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void thermal_field(void *const B[3], uint64_t seed, uint64_t step, uint32_t stream, T k, Par<T> Ms, Par<T> temp,
+                   Par<T> alpha, int N, hipStream_t s) {
+  hipLaunchKernelGGL(k_thermal<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)B[0], (T *)B[1],
+                     (T *)B[2], seed, step, stream, k, Ms, temp, alpha, N);
+  UM_HIP(hipGetLastError());
+}
+
+#define UM_INST(T)                                                                                            \
+  template void add_exchange<T>(void *const[3], const void *const[3], Par<T>, const T *, const uint8_t *, T, T, \
+                                T, Mesh, hipStream_t);                                                        \
+  template void add_dmi<T>(void *const[3], const void *const[3], Par<T>, const T *, const T *, const uint8_t *, \
+                           T, T, T, Mesh, int, hipStream_t);                                                  \
+  template void add_uniaxial2<T>(void *const[3], const void *const[3], Par<T>, Par<T>, Par<T>, const Par<T>[3], \
+                                 int, hipStream_t);                                                           \
+  template void add_cubic2<T>(void *const[3], const void *const[3], Par<T>, Par<T>, Par<T>, Par<T>,            \
+                              const Par<T>[3], const Par<T>[3], int, hipStream_t);                            \
+  template void region_addv<T>(void *const[3], const void *const[3], const uint8_t *, int, hipStream_t);       \
+  template void zeromask<T>(void *, const void *, const uint8_t *, int, hipStream_t);                         \
+  template void lltorque<T>(void *const[3], const void *const[3], const void *const[3], Par<T>, int,          \
+                            hipStream_t);                                                                     \
+  template void llnoprecess<T>(void *const[3], const void *const[3], const void *const[3], int, hipStream_t); \
+  template void thermal_field<T>(void *const[3], uint64_t, uint64_t, uint32_t, T, Par<T>, Par<T>, Par<T>, int, \
+                                 hipStream_t);
+UM_INST(float)
+UM_INST(double)
+
+}  // namespace um

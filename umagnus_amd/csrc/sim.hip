@@ -1,0 +1,658 @@
+// Simulation handle: the engine-level restatement (engine/effectivefield.go,
+// torque.go, rk45dp.go, heun.go, rk4.go, stepcontroller.go, run.go) driving
+// the fused kernels.  Stage combinations are fused with normalisation and the
+// step-control norms with the error estimate, so a step reads back one
+// 24-byte result instead of several reductions.
+#include <cmath>
+#include <cstring>
+
+#include "field.hpp"
+#include "sim.hpp"
+#include "vec.hpp"
+
+namespace um {
+
+static constexpr double KB = 1.380650424e-23;  // mag/constants.go:9
+static constexpr int NREGION = 256;
+static constexpr int SYMLUT = NREGION * (NREGION + 1) / 2;
+
+template <typename T>
+struct Sim : SimBase {
+  int N[3];
+  long n;
+  double cs[3];
+  int pbc[3], pbccode;
+  hipStream_t st = nullptr;
+
+  struct PSlot {
+    DevBuf arr;
+    double mul = 0;
+    bool nonzero = false;   // host-side knowledge: any non-zero value
+  };
+  PSlot p[UM_P_COUNT][3];
+  DevBuf regions, aLUT, dLUT, bextLUT, frozenLUT, vol;
+  bool has_regions = false, has_dmi = false, has_bextlut = false, has_frozen = false, has_vol = false;
+  int enable_demag = 1, openbc = 0, precess = 1, heun_sqrt3 = 1;
+  double demag_acc = 6.0;
+  DemagBase *demag = nullptr;
+  uint64_t seed = 0;
+  uint32_t tstream = 0;
+
+  DevBuf bufs[10];          // m, m0, k1..k6, bd, spare
+  T *m, *m0, *k[7], *bd, *spare;  // k[0]=k1 .. k[5]=k6
+  DevBuf red;               // 4 x u64 reduction slots
+  DevBuf partial, dsum;     // deterministic sums
+  unsigned long long *hres = nullptr;  // pinned readback
+  double *hsum = nullptr;
+
+  // solver state, engine/run.go:13-43
+  int solver = UM_SOLVER_RK45DP;
+  double Time = 0, dt = 1e-15, fixdt = 0, maxerr = 1e-5, relerr = 1e-4, minslope = 1e8, headroom = 0.8;
+  double mindt = 0, maxdt = 0, gammaLL = 1.7595e11, alarm = 0;
+  double lastErr = 0, peakErr = 0, lastTorque = 0;
+  long nsteps = 0, nundone = 0, nevals = 0;
+  bool k1_valid = false;
+
+  // profiling (HIP events around the demag and field kernels)
+  int profiling = 0;
+  hipEvent_t ev[4] = {};
+  double t_demag = 0, t_field = 0, t_other = 0;
+  int prof_evals = 0;
+
+  Sim(const int Nin[3], const double csin[3], const int pbcin[3]) {
+    ensure_init();
+    for (int i = 0; i < 3; i++) { N[i] = Nin[i]; cs[i] = csin[i]; pbc[i] = pbcin[i]; }
+    UM_REQUIRE(N[0] > 0 && N[1] > 0 && N[2] > 0, UM_EINVAL, "grid size must be positive");
+    n = (long)N[0] * N[1] * N[2];
+    UM_REQUIRE(n < (1L << 31), UM_EUNSUP, "grid too large for 32-bit cell indices");
+    pbccode = (pbc[0] ? 1 : 0) | (pbc[1] ? 2 : 0) | (pbc[2] ? 4 : 0);
+    UM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const size_t vb = sizeof(T) * 3 * n;
+    for (auto &b : bufs) { b.alloc(vb); fill_bytes(b.p, vb, st); }
+    m = bufs[0].as<T>(); m0 = bufs[1].as<T>();
+    for (int i = 0; i < 6; i++) k[i] = bufs[2 + i].as<T>();
+    k[6] = nullptr;
+    bd = bufs[8].as<T>(); spare = bufs[9].as<T>();
+    red.alloc(4 * sizeof(unsigned long long));
+    partial.alloc(RED_BLOCKS * sizeof(double));
+    dsum.alloc(sizeof(double));
+    UM_HIP(hipHostMalloc((void **)&hres, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+    UM_HIP(hipHostMalloc((void **)&hsum, sizeof(double), hipHostMallocDefault));
+    aLUT.alloc(sizeof(T) * SYMLUT);
+    dLUT.alloc(sizeof(T) * SYMLUT);
+    fill_bytes(aLUT.p, aLUT.bytes, st);
+    fill_bytes(dLUT.p, dLUT.bytes, st);
+    for (auto &e : ev) UM_HIP(hipEventCreate(&e));
+    UM_HIP(hipStreamSynchronize(st));
+  }
+  ~Sim() override {
+    if (st) (void)hipStreamSynchronize(st);
+    if (demag) demag_destroy(demag);
+    if (hres) (void)hipHostFree(hres);
+    if (hsum) (void)hipHostFree(hsum);
+    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  hipStream_t stream() override { return st; }
+
+  // ---- parameters --------------------------------------------------------------
+  void upload(DevBuf &b, const void *host, size_t bytes) {
+    if (b.bytes != bytes) b.alloc(bytes);
+    UM_HIP(hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, st));
+    UM_HIP(hipStreamSynchronize(st));
+  }
+  static bool any_nonzero(const T *a, long cnt) {
+    for (long i = 0; i < cnt; i++) if (a[i] != T(0)) return true;
+    return false;
+  }
+  void set_slot(PSlot &s, const void *arr, double mul) {
+    s.mul = mul;
+    if (arr) {
+      upload(s.arr, arr, sizeof(T) * n);
+      s.nonzero = (mul != 0) && any_nonzero((const T *)arr, n);
+    } else {
+      s.arr.release();
+      s.nonzero = (mul != 0);
+    }
+  }
+  void set_scalar(int id, const void *arr, double mul) override {
+    UM_REQUIRE(id >= 0 && id < UM_P_COUNT, UM_EINVAL, "parameter id");
+    set_slot(p[id][0], arr, mul);
+    if (id == UM_P_MSAT) invalidate_fsal();
+  }
+  void set_vector(int id, const void *const a[3], const double mul[3]) override {
+    UM_REQUIRE(id >= 0 && id < UM_P_COUNT, UM_EINVAL, "parameter id");
+    for (int c = 0; c < 3; c++) set_slot(p[id][c], a[c], mul[c]);
+  }
+  Par<T> P(int id, int c = 0) const { return Par<T>{(const T *)p[id][c].arr.p, (T)p[id][c].mul}; }
+  bool nz(int id, int nc = 1) const {
+    for (int c = 0; c < nc; c++) if (p[id][c].nonzero) return true;
+    return false;
+  }
+  void set_regions(const uint8_t *r) override {
+    if (r) { upload(regions, r, (size_t)n); has_regions = true; }
+    else { regions.release(); has_regions = false; }
+  }
+  void set_exchange_luts(const void *a, const void *d) override {
+    upload(aLUT, a, sizeof(T) * SYMLUT);
+    has_dmi = false;
+    if (d) {
+      upload(dLUT, d, sizeof(T) * SYMLUT);
+      has_dmi = any_nonzero((const T *)d, SYMLUT);
+    }
+  }
+  void set_bext_lut(const void *const l[3]) override {
+    if (!l[0]) { bextLUT.release(); has_bextlut = false; return; }
+    std::vector<T> h(3 * NREGION);
+    for (int c = 0; c < 3; c++) memcpy(h.data() + c * NREGION, l[c], sizeof(T) * NREGION);
+    upload(bextLUT, h.data(), sizeof(T) * 3 * NREGION);
+    has_bextlut = any_nonzero(h.data(), 3 * NREGION);
+  }
+  void set_frozen_lut(const void *l) override {
+    if (!l) { frozenLUT.release(); has_frozen = false; return; }
+    upload(frozenLUT, l, sizeof(T) * NREGION);
+    has_frozen = any_nonzero((const T *)l, NREGION);
+  }
+  void set_geometry(const void *v) override {
+    if (v) { upload(vol, v, sizeof(T) * n); has_vol = true; }
+    else { vol.release(); has_vol = false; }
+  }
+  void set_flags(int d, int ob, int pr, int hs) override {
+    enable_demag = d; openbc = ob; precess = pr; heun_sqrt3 = hs;
+  }
+  void set_demag_accuracy(double a) override {
+    if (a != demag_acc && demag) { demag_destroy(demag); demag = nullptr; }
+    demag_acc = a;
+  }
+  void set_thermal_seed(uint64_t s, uint32_t strm) override { seed = s; tstream = strm; }
+  void invalidate_fsal() { k1_valid = false; }
+
+  // ---- m --------------------------------------------------------------------------
+  void set_m(const void *const mh[3]) override {
+    for (int c = 0; c < 3; c++)
+      UM_HIP(hipMemcpyAsync(m + c * n, mh[c], sizeof(T) * n, hipMemcpyHostToDevice, st));
+    normalize(m);
+    UM_HIP(hipStreamSynchronize(st));
+    k1_valid = false;
+  }
+  void get_m(void *const mh[3]) override {
+    for (int c = 0; c < 3; c++)
+      UM_HIP(hipMemcpyAsync(mh[c], m + c * n, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+    UM_HIP(hipStreamSynchronize(st));
+  }
+  void m_device(void *ptrs[3]) override {
+    for (int c = 0; c < 3; c++) ptrs[c] = m + c * n;
+  }
+  void normalize(T *v) {
+    StageArgs<T> a{};
+    a.nsrc = 1;
+    a.ncomp = 3;
+    for (int c = 0; c < 3; c++) { a.src[0][c] = v + c * n; a.dst[c] = v + c * n; }
+    a.fac[0] = T(1);
+    a.normalize = 1;
+    a.vol = (const T *)vol.p;
+    a.N = n;
+    launch_stage(a, st);
+  }
+
+  // ---- effective field and torque ------------------------------------------------------
+  void ensure_demag() {
+    if (demag || !enable_demag) return;
+    int P[3];
+    pad_size(N, pbc, P);
+    const size_t np = (size_t)P[0] * P[1] * P[2];
+    DevBuf K(sizeof(double) * 6 * np);
+    double *Kd[6];
+    for (int c = 0; c < 6; c++) Kd[c] = K.as<double>() + c * np;
+    int P2[3];
+    demag_kernel_bf_device(N, pbc, cs, demag_acc, Kd, P2, st);
+    demag = demag_create(sizeof(T) == 4 ? UM_F32 : UM_F64, N, P2, Kd, st);
+    UM_HIP(hipStreamSynchronize(st));
+  }
+  FieldArgs<T> field_args(const T *mm) {
+    FieldArgs<T> a{};
+    a.g = Mesh{N[0], N[1], N[2], pbccode};
+    a.wx = (T)(2 / (cs[0] * cs[0]));
+    a.wy = (T)(2 / (cs[1] * cs[1]));
+    a.wz = (T)(2 / (cs[2] * cs[2]));
+    a.cx = (T)cs[0]; a.cy = (T)cs[1]; a.cz = (T)cs[2];
+    for (int c = 0; c < 3; c++) a.m[c] = mm + c * n;
+    a.regions = has_regions ? (const uint8_t *)regions.p : nullptr;
+    a.Ms = P(UM_P_MSAT);
+    a.aLUT = aLUT.as<T>();
+    a.dLUT = dLUT.as<T>();
+    a.exch_mode = has_dmi ? 2 : 1;
+    a.openbc = openbc;
+    a.has_uniax = nz(UM_P_KU1) || nz(UM_P_KU2);
+    a.Ku1 = P(UM_P_KU1); a.Ku2 = P(UM_P_KU2);
+    a.has_cubic = nz(UM_P_KC1) || nz(UM_P_KC2) || nz(UM_P_KC3);
+    a.Kc1 = P(UM_P_KC1); a.Kc2 = P(UM_P_KC2); a.Kc3 = P(UM_P_KC3);
+    for (int c = 0; c < 3; c++) {
+      a.u[c] = P(UM_P_ANISU, c);
+      a.c1[c] = P(UM_P_ANISC1, c);
+      a.c2[c] = P(UM_P_ANISC2, c);
+      a.bext[c] = P(UM_P_BEXT, c);
+      a.bextLUT[c] = has_bextlut ? bextLUT.as<T>() + c * NREGION : nullptr;
+    }
+    a.has_bext = nz(UM_P_BEXT, 3);
+    a.has_therm = nz(UM_P_TEMP);
+    a.temp = P(UM_P_TEMP);
+    a.kth = (T)(2 * KB / (gammaLL * cs[0] * cs[1] * cs[2] * dt));
+    a.seed = seed;
+    a.step = (uint64_t)nsteps;
+    a.stream = tstream;
+    a.alpha = P(UM_P_ALPHA);
+    a.precess = precess;
+    a.frozenLUT = has_frozen ? frozenLUT.as<T>() : nullptr;
+    return a;
+  }
+  // B_demag into bd (SetDemagField, engine/demag.go:42-65)
+  void compute_demag(const T *mm) {
+    if (!enable_demag) return;
+    ensure_demag();
+    void *B[3] = {bd, bd + n, bd + 2 * n};
+    const void *mv[3] = {mm, mm + n, mm + 2 * n};
+    demag_exec(demag, B, mv, vol.p, p[UM_P_MSAT][0].arr.p, p[UM_P_MSAT][0].mul, st);
+  }
+  // torqueFn (engine/run.go:127-130): SetTorque into dst
+  void torque_fn(T *dst) {
+    if (profiling) UM_HIP(hipEventRecord(ev[0], st));
+    compute_demag(m);
+    if (profiling) UM_HIP(hipEventRecord(ev[1], st));
+    FieldArgs<T> a = field_args(m);
+    for (int c = 0; c < 3; c++) {
+      a.bd[c] = enable_demag ? bd + c * n : nullptr;
+      a.tau[c] = dst + c * n;
+      a.beff[c] = nullptr;
+    }
+    launch_field_torque(a, st);
+    if (profiling) {
+      UM_HIP(hipEventRecord(ev[2], st));
+      UM_HIP(hipEventSynchronize(ev[2]));
+      float d1 = 0, d2 = 0;
+      UM_HIP(hipEventElapsedTime(&d1, ev[0], ev[1]));
+      UM_HIP(hipEventElapsedTime(&d2, ev[1], ev[2]));
+      t_demag += d1;
+      t_field += d2;
+      prof_evals++;
+    }
+    nevals++;
+  }
+  void beff(void *const out[3]) override {
+    compute_demag(m);
+    FieldArgs<T> a = field_args(m);
+    for (int c = 0; c < 3; c++) {
+      a.bd[c] = enable_demag ? bd + c * n : nullptr;
+      a.tau[c] = nullptr;
+      a.beff[c] = spare + c * n;
+    }
+    launch_field_torque(a, st);
+    copy_out(spare, out);
+  }
+  void torque(void *const out[3]) override {
+    torque_fn(spare);
+    nevals--;
+    copy_out(spare, out);
+  }
+  void demag_field(void *const out[3]) override {
+    if (enable_demag) {
+      compute_demag(m);
+      copy_out(bd, out);
+    } else {
+      fill_bytes(spare, sizeof(T) * 3 * n, st);
+      copy_out(spare, out);
+    }
+  }
+  void copy_out(const T *src, void *const out[3]) {
+    for (int c = 0; c < 3; c++)
+      UM_HIP(hipMemcpyAsync(out[c], src + c * n, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+    UM_HIP(hipStreamSynchronize(st));
+  }
+
+  // ---- solver ------------------------------------------------------------------------
+  void set_solver(int s) override {
+    UM_REQUIRE(s == UM_SOLVER_HEUN || s == UM_SOLVER_RK4 || s == UM_SOLVER_RK45DP, UM_EUNSUP,
+               "solver " + std::to_string(s) + " not available (Heun=2, RK4=4, RK45DP=5)");
+    solver = s;
+    k1_valid = false;
+  }
+  void set_solver_params(const double kk[9]) override {
+    dt = kk[0]; fixdt = kk[1]; maxerr = kk[2]; relerr = kk[3]; minslope = kk[4];
+    headroom = kk[5]; mindt = kk[6]; maxdt = kk[7];
+    if (kk[8] != gammaLL) k1_valid = false;
+    gammaLL = kk[8];
+  }
+  void get_state(double o[8]) override {
+    o[0] = Time; o[1] = dt; o[2] = (double)nsteps; o[3] = (double)nundone; o[4] = (double)nevals;
+    o[5] = lastErr; o[6] = peakErr; o[7] = lastTorque;
+  }
+  void set_last_err(double err) {
+    lastErr = err;
+    if (err > peakErr) peakErr = err;
+  }
+  // adaptDt, engine/run.go:150-187
+  void adapt(double corr) {
+    if (fixdt != 0) { dt = fixdt; return; }
+    if (std::isnan(corr)) corr = 1;
+    UM_REQUIRE(corr != 0, UM_ESTATE, "Time step too small, check if parameters are sensible");
+    corr *= headroom;
+    if (corr > 2) corr = 2;
+    if (corr < 0.5) corr = 0.5;
+    dt *= corr;
+    if (mindt != 0 && dt < mindt) dt = mindt;
+    if (maxdt != 0 && dt > maxdt) dt = maxdt;
+    UM_REQUIRE(dt != 0, UM_ESTATE, "time step too small");
+    if (Time < alarm && Time + dt > alarm) dt = alarm - Time;
+    UM_REQUIRE(dt > 0, UM_ESTATE, "Time step too small");
+  }
+  // m_out = normalize(sum fac_j src_j)
+  void stage(T *dst, int ns, const T *const *src, const T *fac, bool norm, bool group3 = false) {
+    StageArgs<T> a{};
+    a.nsrc = ns;
+    a.ncomp = 3;
+    a.group3 = group3;
+    a.normalize = norm;
+    for (int j = 0; j < ns; j++) {
+      for (int c = 0; c < 3; c++) a.src[j][c] = src[j] + c * n;
+      a.fac[j] = fac[j];
+    }
+    for (int c = 0; c < 3; c++) a.dst[c] = dst + c * n;
+    a.vol = (const T *)vol.p;
+    a.N = n;
+    launch_stage(a, st);
+  }
+  // fused norms -> host (one readback)
+  void err_reduce(int mode, const T *const *src, int nsrc, const T *e, double out[3]) {
+    fill_bytes(red.p, red.bytes, st);
+    ErrArgs<T> a{};
+    a.mode = mode;
+    for (int j = 0; j < nsrc; j++)
+      for (int c = 0; c < 3; c++) a.k[j][c] = src[j] + c * n;
+    if (e) for (int j = 0; j < 6; j++) a.e[j] = e[j];
+    a.N = n;
+    a.res = (unsigned long long *)red.p;
+    launch_err(a, st);
+    UM_HIP(hipMemcpyAsync(hres, red.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    UM_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < 3; i++) memcpy(&out[i], &hres[i], 8);
+  }
+
+  // RK45DP.Step, engine/rk45dp.go:12-98, with integralController (stepcontroller.go:35-97)
+  void step_rk45dp() {
+    if (fixdt != 0) dt = fixdt;
+    T *k1 = k[0];
+    if (!k1_valid) { torque_fn(k1); k1_valid = true; }
+    if (nz(UM_P_TEMP)) torque_fn(k1);
+    const double t0 = Time;
+    std::swap(m, m0);   // m0 <- current magnetization; m is overwritten by every stage
+    const T h = (T)(dt * gammaLL);
+    T *k2 = k[1], *k3 = k[2], *k4 = k[3], *k5 = k[4], *k6 = k[5];
+    {
+      const T *s[] = {m0, k1};
+      const T f[] = {T(1), T(1. / 5.) * h};
+      Time = t0 + (1. / 5.) * dt;
+      stage(m, 2, s, f, true);
+      torque_fn(k2);
+    }
+    {
+      const T *s[] = {m0, k1, k2};
+      const T f[] = {T(1), T(3. / 40.) * h, T(9. / 40.) * h};
+      Time = t0 + (3. / 10.) * dt;
+      stage(m, 3, s, f, true, true);
+      torque_fn(k3);
+    }
+    {
+      const T *s[] = {m0, k1, k2, k3};
+      const T f[] = {T(1), T(44. / 45.) * h, T(-56. / 15.) * h, T(32. / 9.) * h};
+      Time = t0 + (4. / 5.) * dt;
+      stage(m, 4, s, f, true);
+      torque_fn(k4);
+    }
+    {
+      const T *s[] = {m0, k1, k2, k3, k4};
+      const T f[] = {T(1), T(19372. / 6561.) * h, T(-25360. / 2187.) * h, T(64448. / 6561.) * h,
+                     T(-212. / 729.) * h};
+      Time = t0 + (8. / 9.) * dt;
+      stage(m, 5, s, f, true);
+      torque_fn(k5);
+    }
+    {
+      const T *s[] = {m0, k1, k2, k3, k4, k5};
+      const T f[] = {T(1), T(9017. / 3168.) * h, T(-355. / 33.) * h, T(46732. / 5247.) * h,
+                     T(49. / 176.) * h, T(-5103. / 18656.) * h};
+      Time = t0 + dt;
+      stage(m, 6, s, f, true);
+      torque_fn(k6);
+    }
+    T *k7 = k2;
+    {
+      const T *s[] = {m0, k1, k3, k4, k5, k6};
+      const T f[] = {T(1), T(35. / 384.) * h, T(500. / 1113.) * h, T(125. / 192.) * h,
+                     T(-2187. / 6784.) * h, T(11. / 84.) * h};
+      Time = t0 + dt;
+      stage(m, 6, s, f, true);
+      torque_fn(k7);
+    }
+    const T e[6] = {T(35. / 384. - 5179. / 57600.), T(500. / 1113. - 7571. / 16695.),
+                    T(125. / 192. - 393. / 640.), T(-2187. / 6784. - -92097. / 339200.),
+                    T(11. / 84. - 187. / 2100.), T(0. - 1. / 40.)};
+    const T *src[6] = {k1, k3, k4, k5, k6, k7};
+    double r[3];
+    err_reduce(0, src, 6, e, r);
+    const double err = std::sqrt(r[0]) * (double)h;
+    const double maxdm = std::sqrt(r[1]);
+    if (err < maxerr || dt <= mindt || fixdt != 0) {
+      int fail = 0;
+      double rlerr = 0;
+      if (maxdm < minslope) fail = 0;
+      else { rlerr = r[2]; fail = 1; }
+      if (fail == 0 || relerr <= 0.0 || rlerr < relerr || dt <= mindt || fixdt != 0) {
+        set_last_err(err);
+        lastTorque = maxdm;
+        nsteps++;
+        Time = t0 + dt;
+        if (fail == 0) adapt(std::pow(maxerr / err, 1. / 5));
+        else adapt(std::pow(relerr / rlerr, 1. / 5));
+        std::swap(k[0], k[1]);   // FSAL: k1 <- k7
+        k1_valid = true;
+      } else {
+        Time = t0;
+        std::swap(m, m0);
+        nundone++;
+        adapt(std::pow(relerr / rlerr, 1. / 6));
+      }
+    } else {
+      Time = t0;
+      std::swap(m, m0);
+      nundone++;
+      adapt(std::pow(maxerr / err, 1. / 6));
+    }
+  }
+
+  // Heun.Step, engine/heun.go:15-59
+  void step_heun() {
+    T *dy0 = k[0], *dy = k[1];
+    if (fixdt != 0) dt = fixdt;
+    const T h = (T)(dt * gammaLL);
+    UM_REQUIRE(h > 0, UM_ESTATE, "dt must be positive");
+    torque_fn(dy0);
+    std::swap(m, m0);
+    {
+      const T *s[] = {m0, dy0};
+      const T f[] = {T(1), h};
+      stage(m, 2, s, f, false);   // stage 1 is not normalised (heun.go:32)
+    }
+    Time += dt;
+    torque_fn(dy);
+    double r[3];
+    const T *src[2] = {dy0, dy};
+    err_reduce(1, src, 2, nullptr, r);
+    double err = std::sqrt(heun_sqrt3 ? 3.0 * r[0] : r[0]) * (double)h;
+    if (err < maxerr || dt <= mindt || fixdt != 0) {
+      const T *s[] = {m, dy, dy0};
+      const T f[] = {T(1), T(0.5) * h, T(-0.5) * h};
+      stage(m, 3, s, f, true, true);
+      nsteps++;
+      adapt(std::pow(maxerr / err, 1. / 2.));
+      set_last_err(err);
+      const T *sn[1] = {dy};
+      double r2[3];
+      err_reduce(2, sn, 1, nullptr, r2);
+      lastTorque = std::sqrt(r2[0]);
+    } else {
+      Time -= dt;
+      std::swap(m, m0);
+      nundone++;
+      adapt(std::pow(maxerr / err, 1. / 3.));
+    }
+  }
+
+  // RK4.Step, engine/rk4.go:11-69 with simpleController (stepcontroller.go:13-32)
+  void step_rk4() {
+    if (fixdt != 0) dt = fixdt;
+    const double t0 = Time;
+    T *k1 = k[0], *k2 = k[1], *k3 = k[2], *k4 = k[3];
+    const T h = (T)(dt * gammaLL);
+    torque_fn(k1);
+    std::swap(m, m0);
+    Time = t0 + (1. / 2.) * dt;
+    { const T *s[] = {m0, k1}; const T f[] = {T(1), T(1. / 2.) * h}; stage(m, 2, s, f, true); }
+    torque_fn(k2);
+    { const T *s[] = {m0, k2}; const T f[] = {T(1), T(1. / 2.) * h}; stage(m, 2, s, f, true); }
+    torque_fn(k3);
+    Time = t0 + dt;
+    { const T *s[] = {m0, k3}; const T f[] = {T(1), T(1.) * h}; stage(m, 2, s, f, true); }
+    torque_fn(k4);
+    double r[3];
+    const T *src[2] = {k1, k4};
+    err_reduce(1, src, 2, nullptr, r);
+    const double err = std::sqrt(r[0]) * (double)h;
+    bool ok;
+    if (err < maxerr || dt <= mindt || fixdt != 0) {
+      nsteps++;
+      adapt(std::pow(maxerr / err, 1. / 4.));
+      set_last_err(err);
+      ok = true;
+    } else {
+      nundone++;
+      adapt(std::pow(maxerr / err, 1. / 5.));
+      ok = false;
+    }
+    if (ok) {
+      const T *s[] = {m0, k1, k2, k3, k4};
+      const T f[] = {T(1), T(1. / 6.) * h, T(1. / 3.) * h, T(1. / 3.) * h, T(1. / 6.) * h};
+      stage(m, 5, s, f, true);
+      const T *sn[1] = {k4};
+      double r2[3];
+      err_reduce(2, sn, 1, nullptr, r2);
+      lastTorque = std::sqrt(r2[0]);
+    } else {
+      Time = t0;
+      std::swap(m, m0);
+    }
+  }
+
+  void step() override {
+    switch (solver) {
+      case UM_SOLVER_HEUN: step_heun(); break;
+      case UM_SOLVER_RK4: step_rk4(); break;
+      default: step_rk45dp(); break;
+    }
+  }
+  // Steps(n), engine/run.go:197-200; RunWhile drops the FSAL torque (:207)
+  void steps(int cnt) override {
+    k1_valid = false;
+    const long stop = nsteps + cnt;
+    while (nsteps < stop) step();
+    UM_HIP(hipStreamSynchronize(st));
+  }
+  // Run(seconds), engine/run.go:190-194
+  void run(double seconds) override {
+    k1_valid = false;
+    const double stop = Time + seconds;
+    alarm = stop;
+    while (Time < stop) step();
+    UM_HIP(hipStreamSynchronize(st));
+  }
+
+  // ---- outputs -----------------------------------------------------------------------
+  double wdot(const T *const *a, const T *const *b, int nc, const T *w, T wmul) {
+    launch_wdot<T>(a, b, nc, w, wmul, n, partial.as<double>(), dsum.as<double>(), st);
+    UM_HIP(hipMemcpyAsync(hsum, dsum.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    UM_HIP(hipStreamSynchronize(st));
+    return *hsum;
+  }
+  // sAverageMagnet, engine/average.go:28-40
+  void average_m(double out[3]) override {
+    const T *w = (const T *)vol.p;
+    double denom = (double)n;
+    if (w) {
+      const T *a1[1] = {w};
+      denom = wdot(a1, nullptr, 1, nullptr, T(1));
+    }
+    for (int c = 0; c < 3; c++) {
+      const T *a1[1] = {m + c * n};
+      out[c] = wdot(a1, nullptr, 1, w, T(1)) / denom;
+    }
+  }
+  // E = pref * V * sum(Ms vol m . B_term), engine/energy.go, demag.go:146, exchange.go:79, zeeman.go:15
+  double energy(int term) override {
+    UM_REQUIRE(term >= 0 && term <= 3, UM_EINVAL, "energy term 0..3");
+    fill_bytes(spare, sizeof(T) * 3 * n, st);
+    void *B[3] = {spare, spare + n, spare + 2 * n};
+    const void *mv[3] = {m, m + n, m + 2 * n};
+    if (term == 0) {
+      if (enable_demag) { compute_demag(m); B[0] = bd; B[1] = bd + n; B[2] = bd + 2 * n; }
+    } else if (term == 1) {
+      FieldArgs<T> a = field_args(m);
+      // exchange only: zero every other term
+      a.has_uniax = a.has_cubic = a.has_bext = a.has_therm = 0;
+      for (int c = 0; c < 3; c++) { a.bextLUT[c] = nullptr; a.bd[c] = nullptr; a.tau[c] = nullptr; a.beff[c] = spare + c * n; }
+      launch_field_torque(a, st);
+    } else if (term == 2) {
+      FieldArgs<T> a = field_args(m);
+      a.exch_mode = 0;
+      a.has_uniax = a.has_cubic = a.has_therm = 0;
+      for (int c = 0; c < 3; c++) { a.bd[c] = nullptr; a.tau[c] = nullptr; a.beff[c] = spare + c * n; }
+      launch_field_torque(a, st);
+    } else {
+      FieldArgs<T> a = field_args(m);
+      a.exch_mode = 0;
+      a.has_cubic = a.has_bext = a.has_therm = 0;
+      a.Ku2 = Par<T>{nullptr, T(0)};
+      for (int c = 0; c < 3; c++) { a.bextLUT[c] = nullptr; a.bd[c] = nullptr; a.tau[c] = nullptr; a.beff[c] = spare + c * n; }
+      launch_field_torque(a, st);
+    }
+    (void)mv;
+    // weight = Ms * vol per cell
+    DevBuf w(sizeof(T) * n);
+    const T *msa = (const T *)p[UM_P_MSAT][0].arr.p;
+    if (msa && has_vol) launch_mul<T>(w.as<T>(), msa, (const T *)vol.p, n, st);
+    else if (msa) UM_HIP(hipMemcpyAsync(w.p, msa, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+    else if (has_vol) UM_HIP(hipMemcpyAsync(w.p, vol.p, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+    else fill_value<T>(w.as<T>(), T(1), n, st);
+    const T *a3[3] = {m, m + n, m + 2 * n};
+    const T *b3[3] = {(const T *)B[0], (const T *)B[1], (const T *)B[2]};
+    const double s = wdot(a3, b3, 3, w.as<T>(), (T)p[UM_P_MSAT][0].mul);
+    const double V = cs[0] * cs[1] * cs[2];
+    return (term == 2 ? -1.0 : -0.5) * V * s;
+  }
+
+  void set_profiling(int on) override {
+    profiling = on;
+    t_demag = t_field = t_other = 0;
+    prof_evals = 0;
+  }
+  void kernel_times(double ms[3], int *ne) override {
+    ms[0] = t_demag; ms[1] = t_field; ms[2] = t_other;
+    *ne = prof_evals;
+  }
+};
+
+SimBase *sim_create(int prec, const int N[3], const double cs[3], const int pbc[3]) {
+  if (prec == UM_F32) return new Sim<float>(N, cs, pbc);
+  UM_REQUIRE(prec == UM_F64, UM_EINVAL, "prec must be UM_F32 or UM_F64");
+  return new Sim<double>(N, cs, pbc);
+}
+
+}  // namespace um

@@ -1,0 +1,53 @@
+// Simulation handle: engine-level state (m, parameters, demag plan, solver
+// state) and the steppers, restating engine/rk45dp.go, heun.go, rk4.go,
+// stepcontroller.go and run.go on top of the fused kernels.
+#pragma once
+#include "common.hpp"
+
+namespace um {
+
+struct DemagBase;
+DemagBase *demag_create(int prec, const int N[3], const int P[3], double *const Kdev[6], hipStream_t s);
+void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3], double accuracy,
+                            double *const Kdev[6], int P[3], hipStream_t s);
+void pad_size(const int N[3], const int pbc[3], int P[3]);
+void demag_exec(DemagBase *d, void *const B[3], const void *const m[3], const void *vol, const void *Ms,
+                double Ms_mul, hipStream_t s);
+void demag_destroy(DemagBase *d);
+void demag_kspec(DemagBase *d, int c, void *host);
+void demag_info(DemagBase *d, int P[3], int *prec);
+
+struct SimBase {
+  virtual ~SimBase() {}
+  virtual void set_scalar(int id, const void *arr, double mul) = 0;
+  virtual void set_vector(int id, const void *const a[3], const double mul[3]) = 0;
+  virtual void set_regions(const uint8_t *r) = 0;
+  virtual void set_exchange_luts(const void *a, const void *d) = 0;
+  virtual void set_bext_lut(const void *const l[3]) = 0;
+  virtual void set_frozen_lut(const void *l) = 0;
+  virtual void set_geometry(const void *vol) = 0;
+  virtual void set_flags(int demag, int openbc, int precess, int heun_sqrt3) = 0;
+  virtual void set_demag_accuracy(double a) = 0;
+  virtual void set_thermal_seed(uint64_t seed, uint32_t stream) = 0;
+  virtual void set_m(const void *const m[3]) = 0;
+  virtual void get_m(void *const m[3]) = 0;
+  virtual void m_device(void *p[3]) = 0;
+  virtual void beff(void *const out[3]) = 0;
+  virtual void torque(void *const out[3]) = 0;
+  virtual void demag_field(void *const out[3]) = 0;
+  virtual void set_solver(int s) = 0;
+  virtual void set_solver_params(const double k[9]) = 0;
+  virtual void get_state(double o[8]) = 0;
+  virtual void step() = 0;
+  virtual void steps(int n) = 0;
+  virtual void run(double seconds) = 0;
+  virtual void average_m(double out[3]) = 0;
+  virtual double energy(int term) = 0;
+  virtual hipStream_t stream() = 0;
+  virtual void set_profiling(int on) = 0;
+  virtual void kernel_times(double ms[3], int *nevals) = 0;
+};
+
+SimBase *sim_create(int prec, const int N[3], const double cs[3], const int pbc[3]);
+
+}  // namespace um

@@ -1,0 +1,291 @@
+// Vector arithmetic (opencl/madd.go, normalize.go, vecnorm.go), RK stage and
+// error kernels (engine/rk45dp.go, heun.go, rk4.go, stepcontroller.go) and
+// reductions (opencl/reduce.go) accumulated in FP64.
+#include "vec.hpp"
+
+namespace um {
+
+// ---- fills (kernel based, so they may sit inside captured graphs) ----------
+__global__ void k_fill_bytes(uint32_t *dst, size_t nwords) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = 0u;
+}
+__global__ void k_fill_tail(unsigned char *dst, size_t n) {
+  if (threadIdx.x < n) dst[threadIdx.x] = 0;
+}
+void fill_bytes(void *dst, size_t bytes, hipStream_t s) {
+  const size_t nw = bytes / 4;
+  if (nw) {
+    hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for((long)nw, 256, 8192)), dim3(256), 0, s, (uint32_t *)dst, nw);
+    UM_HIP(hipGetLastError());
+  }
+  if (bytes % 4) {
+    hipLaunchKernelGGL(k_fill_tail, dim3(1), dim3(64), 0, s, (unsigned char *)dst + nw * 4, bytes % 4);
+    UM_HIP(hipGetLastError());
+  }
+}
+template <typename T>
+__global__ void k_fill_value(T *dst, T v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = v;
+}
+template <typename T>
+void fill_value(T *dst, T v, size_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_value<T>, dim3(blocks_for((long)n, 256, 8192)), dim3(256), 0, s, dst, v, n);
+  UM_HIP(hipGetLastError());
+}
+template void fill_value<float>(float *, float, size_t, hipStream_t);
+template void fill_value<double>(double *, double, size_t, hipStream_t);
+template void fill_value<unsigned long long>(unsigned long long *, unsigned long long, size_t, hipStream_t);
+
+// ---- stage: madd + normalize ---------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_stage(StageArgs<T> a) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
+    T r[3] = {T(0), T(0), T(0)};
+    const int nc = a.ncomp;
+    for (int c = 0; c < nc; c++) {
+      T acc;
+      if (a.group3 && a.nsrc == 3) {
+        acc = (a.fac[0] * a.src[0][c][i]) + (a.fac[1] * a.src[1][c][i] + a.fac[2] * a.src[2][c][i]);
+      } else {
+        acc = a.fac[0] * a.src[0][c][i];
+        for (int k = 1; k < a.nsrc; k++) acc = acc + a.fac[k] * a.src[k][c][i];
+      }
+      r[c] = acc;
+    }
+    if (a.normalize && nc == 3) {
+      // normalize2.cl:8-23
+      const T v = a.vol ? a.vol[i] : T(1);
+      T X = v * r[0], Y = v * r[1], Z = v * r[2];
+      const T l = sqrt(X * X + Y * Y + Z * Z);
+      const T inv = (l != T(0)) ? T(1) / l : T(0);
+      if (v == T(0)) {
+        r[0] = r[1] = r[2] = T(0);
+      } else {
+        r[0] = X * inv;
+        r[1] = Y * inv;
+        r[2] = Z * inv;
+      }
+    }
+    for (int c = 0; c < nc; c++) a.dst[c][i] = r[c];
+  }
+}
+template <typename T>
+void launch_stage(const StageArgs<T> &a, hipStream_t s) {
+  UM_REQUIRE(a.nsrc >= 1 && a.nsrc <= 8, UM_EINVAL, "madd: 1..8 sources");
+  UM_REQUIRE(a.ncomp == 1 || a.ncomp == 3, UM_EINVAL, "stage: ncomp must be 1 or 3");
+  hipLaunchKernelGGL(k_stage<T>, dim3(blocks_for(a.N, 256, 16384)), dim3(256), 0, s, a);
+  UM_HIP(hipGetLastError());
+}
+template void launch_stage<float>(const StageArgs<float> &, hipStream_t);
+template void launch_stage<double>(const StageArgs<double> &, hipStream_t);
+
+// ---- block max helper -------------------------------------------------------------
+// max that keeps NaN (fmax would drop it): a NaN error must reject the step
+__device__ __forceinline__ double nmax(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// max-combine non-negative doubles by their bit pattern (NaN propagates as the largest)
+__device__ __forceinline__ void block_max_to(double v, unsigned long long *dst) {
+  __shared__ double sh[16];
+  v = wave_max(v);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sh[0];
+    for (int i = 1; i < (int)(blockDim.x + 63) / 64; i++) m = nmax(m, sh[i]);
+    if (m != m) m = __longlong_as_double(0x7ff8000000000000LL);
+    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
+  __syncthreads();
+}
+
+// ---- fused error reductions --------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_err(ErrArgs<T> a) {
+  double m0 = 0, m1 = 0, m2 = 0;
+  bool nan = false;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
+    if (a.mode == 0) {
+      T E[3], K7[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        // Madd6(Err, k1,k3,k4,k5,k6,k7, e...), engine/rk45dp.go:95
+        T acc = a.e[0] * a.k[0][c][i];
+#pragma unroll
+        for (int j = 1; j < 6; j++) acc = acc + a.e[j] * a.k[j][c][i];
+        E[c] = acc;
+        K7[c] = a.k[5][c][i];
+      }
+      const double e2 = (double)E[0] * E[0] + (double)E[1] * E[1] + (double)E[2] * E[2];
+      const double k2 = (double)K7[0] * K7[0] + (double)K7[1] * K7[1] + (double)K7[2] * K7[2];
+      // VecNorm(errnorm)/VecNorm(ddtnorm), 0 where |k7| = 0 (div.cl:9)
+      const double rel = (k2 == 0.0) ? 0.0 : sqrt(e2) / sqrt(k2);
+      nan |= (e2 != e2);
+      m0 = fmax(m0, e2);
+      m1 = fmax(m1, k2);
+      m2 = fmax(m2, rel);
+    } else if (a.mode == 1) {
+      const double dx = (double)(a.k[0][0][i] - a.k[1][0][i]);
+      const double dy = (double)(a.k[0][1][i] - a.k[1][1][i]);
+      const double dz = (double)(a.k[0][2][i] - a.k[1][2][i]);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      nan |= (d2 != d2);
+      m0 = fmax(m0, d2);
+    } else {
+      const double x = a.k[0][0][i], y = a.k[0][1][i], z = a.k[0][2][i];
+      const double d2 = x * x + y * y + z * z;
+      nan |= (d2 != d2);
+      m0 = fmax(m0, d2);
+    }
+  }
+  if (nan) m0 = __longlong_as_double(0x7ff8000000000000LL);
+  block_max_to(m0, a.res);
+  if (a.mode == 0) {
+    block_max_to(m1, a.res + 1);
+    block_max_to(m2, a.res + 2);
+  }
+}
+template <typename T>
+void launch_err(const ErrArgs<T> &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_err<T>, dim3(blocks_for(a.N, 256, 2048)), dim3(256), 0, s, a);
+  UM_HIP(hipGetLastError());
+}
+template void launch_err<float>(const ErrArgs<float> &, hipStream_t);
+template void launch_err<double>(const ErrArgs<double> &, hipStream_t);
+
+// ---- deterministic weighted dot sums ------------------------------------------------
+template <typename T>
+struct WdotArgs {
+  const T *a[3];
+  const T *b[3];
+  int ncomp;
+  const T *w;
+  T wmul;
+  long N;
+  double *partial;
+};
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_wdot(WdotArgs<T> a) {
+  double acc = 0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
+    double d = 0;
+    for (int c = 0; c < a.ncomp; c++) d += (double)a.a[c][i] * (a.b[c] ? (double)a.b[c][i] : 1.0);
+    const double w = a.w ? (double)a.wmul * (double)a.w[i] : (double)a.wmul;
+    acc += w * d;
+  }
+  __shared__ double sh[RED_THREADS / 64];
+  acc = wave_sum(acc);
+  if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int k = 0; k < RED_THREADS / 64; k++) t += sh[k];
+    a.partial[blockIdx.x] = t;
+  }
+}
+// Kahan-compensated final sum over the fixed partials, one thread order.
+__global__ void k_final_sum(const double *partial, int n, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0, c = 0;
+  for (int i = 0; i < n; i++) {
+    const double y = partial[i] - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  *out = s;
+}
+template <typename T>
+void launch_wdot(const T *const *a, const T *const *b, int ncomp, const T *w, T wmul, long N, double *partial,
+                 double *out, hipStream_t s) {
+  WdotArgs<T> args{};
+  for (int c = 0; c < ncomp; c++) {
+    args.a[c] = a[c];
+    args.b[c] = b ? b[c] : nullptr;
+  }
+  args.ncomp = ncomp;
+  args.w = w;
+  args.wmul = wmul;
+  args.N = N;
+  args.partial = partial;
+  hipLaunchKernelGGL(k_wdot<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, args);
+  UM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(64), 0, s, (const double *)partial, RED_BLOCKS, out);
+  UM_HIP(hipGetLastError());
+}
+template void launch_wdot<float>(const float *const *, const float *const *, int, const float *, float, long,
+                                 double *, double *, hipStream_t);
+template void launch_wdot<double>(const double *const *, const double *const *, int, const double *, double, long,
+                                  double *, double *, hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_maxabs(const T *x, long N, unsigned long long *res) {
+  double m = 0;
+  bool nan = false;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const double v = fabs((double)x[i]);
+    nan |= (v != v);
+    m = fmax(m, v);
+  }
+  if (nan) m = __longlong_as_double(0x7ff8000000000000LL);
+  block_max_to(m, res);
+}
+template <typename T>
+void launch_maxabs(const T *x, long N, unsigned long long *res, hipStream_t s) {
+  hipLaunchKernelGGL(k_maxabs<T>, dim3(blocks_for(N, 256, 2048)), dim3(256), 0, s, x, N, res);
+  UM_HIP(hipGetLastError());
+}
+template void launch_maxabs<float>(const float *, long, unsigned long long *, hipStream_t);
+template void launch_maxabs<double>(const double *, long, unsigned long long *, hipStream_t);
+
+// ---- pointwise helpers (vecnorm.cl, div.cl, mul.cl) ---------------------------------
+template <typename T>
+__global__ void k_vecnorm(T *dst, const T *ax, const T *ay, const T *az, long N) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+    dst[i] = sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+}
+template <typename T>
+__global__ void k_div(T *dst, const T *a, const T *b, long N) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+    dst[i] = (b[i] == T(0)) ? T(0) : (a[i] / b[i]);
+}
+template <typename T>
+__global__ void k_mul(T *dst, const T *a, const T *b, long N) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+    dst[i] = a[i] * b[i];
+}
+template <typename T>
+void launch_vecnorm(T *dst, const T *const a[3], long N, hipStream_t s) {
+  hipLaunchKernelGGL(k_vecnorm<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, dst, a[0], a[1], a[2], N);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_div(T *dst, const T *a, const T *b, long N, hipStream_t s) {
+  hipLaunchKernelGGL(k_div<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, dst, a, b, N);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_mul(T *dst, const T *a, const T *b, long N, hipStream_t s) {
+  hipLaunchKernelGGL(k_mul<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, dst, a, b, N);
+  UM_HIP(hipGetLastError());
+}
+template void launch_vecnorm<float>(float *, const float *const[3], long, hipStream_t);
+template void launch_vecnorm<double>(double *, const double *const[3], long, hipStream_t);
+template void launch_div<float>(float *, const float *, const float *, long, hipStream_t);
+template void launch_div<double>(double *, const double *, const double *, long, hipStream_t);
+template void launch_mul<float>(float *, const float *, const float *, long, hipStream_t);
+template void launch_mul<double>(double *, const double *, const double *, long, hipStream_t);
+
+}  // namespace um

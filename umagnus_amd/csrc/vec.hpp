@@ -1,0 +1,56 @@
+// Vector arithmetic, RK stage/error kernels and reductions.
+#pragma once
+#include "common.hpp"
+
+namespace um {
+
+constexpr int RED_BLOCKS = 1024;   // fixed partial count: deterministic sums
+constexpr int RED_THREADS = 256;
+
+template <typename T> struct Vec3P { T *c[3]; };
+template <typename T> struct CVec3P { const T *c[3]; };
+
+// dst = sum_k fac[k] * src[k] (nsrc <= 8), then optionally normalise the
+// three components with vol (normalize2.cl) -- one pass per cell.
+template <typename T>
+struct StageArgs {
+  int nsrc;
+  int ncomp;             // 3 (vectors) or 1 (one component array)
+  int group3;            // madd3 grouping (madd3.cl:12)
+  int normalize;
+  const T *src[8][3];
+  T fac[8];
+  T *dst[3];
+  const T *vol;
+  long N;
+};
+template <typename T> void launch_stage(const StageArgs<T> &a, hipStream_t s);
+
+// Fused RK error reductions.  Writes into res[0..2] (as FP64 bit patterns,
+// max-combined with atomics; res must be zeroed first):
+//  RK45DP: Err = sum e_j k_j;  res = {max|Err|^2, max|k7|^2, max |Err|/|k7|}
+//  diff:   res[0] = max|a - b|^2   (Heun's MaxVecDiff, RK4's k1 - k4)
+//  norm:   res[0] = max|a|^2
+template <typename T>
+struct ErrArgs {
+  int mode;              // 0 rk45dp, 1 diff, 2 norm
+  const T *k[6][3];      // rk45dp: k1 k3 k4 k5 k6 k7; diff: a b; norm: a
+  T e[6];
+  long N;
+  unsigned long long *res;
+};
+template <typename T> void launch_err(const ErrArgs<T> &a, hipStream_t s);
+
+// Deterministic FP64 sums: out[0] = sum_i w_i * (a_i . b_i) over ncomp
+// components (w nullable = 1, b nullable = 1).  partial: RED_BLOCKS doubles.
+template <typename T>
+void launch_wdot(const T *const *a, const T *const *b, int ncomp, const T *w, T wmul, long N, double *partial,
+                 double *out, hipStream_t s);
+// max |x_i| over N (result as FP64 bits, res zeroed by the caller)
+template <typename T> void launch_maxabs(const T *x, long N, unsigned long long *res, hipStream_t s);
+
+template <typename T> void launch_vecnorm(T *dst, const T *const a[3], long N, hipStream_t s);
+template <typename T> void launch_div(T *dst, const T *a, const T *b, long N, hipStream_t s);
+template <typename T> void launch_mul(T *dst, const T *a, const T *b, long N, hipStream_t s);
+
+}  // namespace um

@@ -1,0 +1,487 @@
+"""ctypes binding of libumhip.so (include/umhip.h).
+
+Two layers, as in the reference:
+  * the exported API of package opencl (opencl/*.go) -- one function per
+    reference function (AddExchange, AddDMI, LLTorque, Madd, MaxVecNorm,
+    DemagConvolution.Exec, ...) operating on device arrays; device arrays are
+    torch tensors on cuda:0 (torch is only the allocator/stream here);
+  * the engine-level simulation handle (Sim): parameters, effective field,
+    torque and the RK45DP / Heun / RK4 steppers on host numpy arrays.
+
+There is no CPU fallback: importing works anywhere, but every call needs the
+built library and a GPU, and raises otherwise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libumhip.so")
+_lib = None
+
+F32, F64 = 0, 1
+P_MSAT, P_KU1, P_KU2, P_KC1, P_KC2, P_KC3, P_ALPHA, P_TEMP, P_ANISU, P_ANISC1, P_ANISC2, P_BEXT = range(12)
+SOLVER_HEUN, SOLVER_RK4, SOLVER_RK45DP = 2, 4, 5
+NREGION = 256
+SYMLUT = NREGION * (NREGION + 1) // 2
+
+# every symbol declared in include/umhip.h
+EXPORTS = [
+    "um_init", "um_last_error", "um_device_info", "um_set_stream", "um_sync", "um_malloc", "um_free",
+    "um_memcpy_h2d", "um_memcpy_d2h", "um_memcpy_d2d", "um_zero", "um_memset",
+    "um_add_exchange", "um_add_dmi", "um_add_uniaxial2", "um_add_cubic2", "um_region_addv", "um_zeromask",
+    "um_lltorque", "um_llnoprecess", "um_thermal_field", "um_madd", "um_normalize", "um_vecnorm", "um_div",
+    "um_mul", "um_reduce_sum", "um_reduce_dot", "um_reduce_maxabs", "um_reduce_maxvecnorm",
+    "um_reduce_maxvecdiff", "um_demag_create", "um_demag_create_from_kernel", "um_demag_exec",
+    "um_demag_destroy", "um_demag_info", "um_demag_kspec", "um_demag_kernel_bf",
+    "um_sim_create", "um_sim_destroy", "um_sim_set_scalar", "um_sim_set_vector", "um_sim_set_regions",
+    "um_sim_set_exchange_luts", "um_sim_set_bext_lut", "um_sim_set_frozen_lut", "um_sim_set_geometry",
+    "um_sim_set_flags", "um_sim_set_demag_accuracy", "um_sim_set_thermal_seed", "um_sim_set_m",
+    "um_sim_get_m", "um_sim_m_device", "um_sim_beff", "um_sim_torque", "um_sim_demag_field",
+    "um_sim_set_solver", "um_sim_set_solver_params", "um_sim_get_state", "um_sim_step", "um_sim_steps",
+    "um_sim_run", "um_sim_average_m", "um_sim_energy", "um_sim_stream", "um_sim_set_profiling",
+    "um_sim_kernel_times",
+]
+
+
+class UmError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load libumhip.so (no GPU needed to load; calls need one)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise UmError(f"{path} is not built: run `python -m umagnus_amd.build` "
+                          "(there is no CPU fallback)")
+        lib = C.CDLL(path)
+        lib.um_last_error.restype = C.c_char_p
+        for name in EXPORTS:
+            f = getattr(lib, name)
+            if name != "um_last_error":
+                f.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise UmError(f"libumhip error {rc}: {load().um_last_error().decode()}")
+
+
+def _call(name, *args):
+    _check(getattr(load(), name)(*args))
+
+
+_initialised = False
+
+
+def init(device=0):
+    """Init(gpu) and bind the library's unfused ops to torch's current stream."""
+    global _initialised
+    if _initialised:
+        return
+    import torch
+    if not torch.cuda.is_available():
+        raise UmError("no GPU visible: libumhip has no CPU fallback")
+    torch.cuda.set_device(device)
+    _call("um_init", C.c_int(device))
+    _call("um_set_stream", C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    _initialised = True
+
+
+def device_info():
+    init()
+    name = C.create_string_buffer(256)
+    cu = C.c_int()
+    mem = C.c_size_t()
+    _call("um_device_info", name, 256, C.byref(cu), C.byref(mem))
+    return name.value.decode(), cu.value, mem.value
+
+
+def sync():
+    _call("um_sync")
+
+
+# ---- device-array helpers (torch tensors) ---------------------------------------------
+def _prec(t):
+    import torch
+    if t.dtype == torch.float32:
+        return F32
+    if t.dtype == torch.float64:
+        return F64
+    raise UmError(f"unsupported dtype {t.dtype}")
+
+
+def _dp(t):
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous(), "device arrays must be contiguous cuda tensors"
+    return C.c_void_p(t.data_ptr())
+
+
+def _vec(ts):
+    return (C.c_void_p * 3)(*[t.data_ptr() for t in ts])
+
+
+def _par(p):
+    """MSlice convention: (tensor or None, multiplier)."""
+    if p is None:
+        return None, C.c_double(0.0)
+    if isinstance(p, (int, float)):
+        return None, C.c_double(float(p))
+    return _dp(p), C.c_double(1.0)
+
+
+# ---- opencl package mirror (opencl/*.go) ----------------------------------------------
+def AddExchange(B, m, aLUT, Msat, regions, cellsize, pbc=0):
+    """opencl.AddExchange (opencl/exchange.go:16); wx = float32(2/c^2)."""
+    init()
+    Nz, Ny, Nx = m[0].shape
+    ms, msm = _par(Msat)
+    w = [np.float32(2 / (c * c)) if _prec(m[0]) == F32 else 2 / (c * c) for c in cellsize]
+    _call("um_add_exchange", _prec(m[0]), _vec(B), _vec(m), ms, msm, _dp(aLUT), _dp(regions),
+          C.c_double(float(w[0])), C.c_double(float(w[1])), C.c_double(float(w[2])), Nx, Ny, Nz, pbc)
+
+
+def AddDMI(B, m, aLUT, dLUT, Msat, regions, cellsize, pbc=0, OpenBC=False):
+    """opencl.AddDMI (opencl/dmi.go:15)."""
+    init()
+    Nz, Ny, Nx = m[0].shape
+    ms, msm = _par(Msat)
+    _call("um_add_dmi", _prec(m[0]), _vec(B), _vec(m), ms, msm, _dp(aLUT), _dp(dLUT), _dp(regions),
+          *[C.c_double(c) for c in cellsize], Nx, Ny, Nz, pbc, int(OpenBC))
+
+
+def AddUniaxialAnisotropy2(B, m, Msat, k1, k2, u):
+    """opencl.AddUniaxialAnisotropy2 (opencl/anisotropy.go:153)."""
+    init()
+    ms, msm = _par(Msat)
+    a1, m1 = _par(k1)
+    a2, m2 = _par(k2)
+    up = [_par(x) for x in u]
+    uarr = (C.c_void_p * 3)(*[p[0].value if p[0] is not None else None for p in up])
+    umul = (C.c_double * 3)(*[p[1].value for p in up])
+    _call("um_add_uniaxial2", _prec(m[0]), _vec(B), _vec(m), ms, msm, a1, m1, a2, m2, uarr, umul, m[0].numel())
+
+
+def AddCubicAnisotropy2(B, m, Msat, k1, k2, k3, c1, c2):
+    """opencl.AddCubicAnisotropy2 (opencl/anisotropy.go:12)."""
+    init()
+    ms, msm = _par(Msat)
+    ks = [_par(k) for k in (k1, k2, k3)]
+    cs = []
+    for c in (c1, c2):
+        pp = [_par(x) for x in c]
+        cs.append(((C.c_void_p * 3)(*[p[0].value if p[0] is not None else None for p in pp]),
+                   (C.c_double * 3)(*[p[1].value for p in pp])))
+    _call("um_add_cubic2", _prec(m[0]), _vec(B), _vec(m), ms, msm, ks[0][0], ks[0][1], ks[1][0], ks[1][1],
+          ks[2][0], ks[2][1], cs[0][0], cs[0][1], cs[1][0], cs[1][1], m[0].numel())
+
+
+def RegionAddV(dst, lut, regions):
+    init()
+    _call("um_region_addv", _prec(dst[0]), _vec(dst), _vec(lut), _dp(regions), dst[0].numel())
+
+
+def ZeroMask(dst, maskLUT, regions):
+    init()
+    _call("um_zeromask", _prec(dst), _dp(dst), _dp(maskLUT), _dp(regions), dst.numel())
+
+
+def LLTorque(torque, m, B, alpha):
+    init()
+    a, am = _par(alpha)
+    _call("um_lltorque", _prec(m[0]), _vec(torque), _vec(m), _vec(B), a, am, m[0].numel())
+
+
+def LLNoPrecess(torque, m, B):
+    init()
+    _call("um_llnoprecess", _prec(m[0]), _vec(torque), _vec(m), _vec(B), m[0].numel())
+
+
+def ThermalField(B, seed, step, stream, k, Msat, Temp, Alpha):
+    """SetTemperature (opencl/temperature.go:14) with in-place Philox normals."""
+    init()
+    ms, msm = _par(Msat)
+    t, tm = _par(Temp)
+    a, am = _par(Alpha)
+    _call("um_thermal_field", _prec(B[0]), _vec(B), C.c_uint64(seed), C.c_uint64(step), C.c_uint32(stream),
+          C.c_double(k), ms, msm, t, tm, a, am, B[0].numel())
+
+
+def Madd(dst, srcs, facs):
+    """Madd2..Madd7 (opencl/madd.go:111-489), one component array."""
+    init()
+    arr = (C.c_void_p * len(srcs))(*[s.data_ptr() for s in srcs])
+    f = (C.c_double * len(facs))(*facs)
+    _call("um_madd", _prec(dst), _dp(dst), len(srcs), arr, f, dst.numel())
+
+
+def Normalize(v, vol=None):
+    init()
+    _call("um_normalize", _prec(v[0]), _vec(v), _dp(vol), v[0].numel())
+
+
+def VecNorm(dst, a):
+    init()
+    _call("um_vecnorm", _prec(dst), _dp(dst), _vec(a), dst.numel())
+
+
+def Div(dst, a, b):
+    init()
+    _call("um_div", _prec(dst), _dp(dst), _dp(a), _dp(b), dst.numel())
+
+
+def Mul(dst, a, b):
+    init()
+    _call("um_mul", _prec(dst), _dp(dst), _dp(a), _dp(b), dst.numel())
+
+
+def _red(name, prec, *args):
+    out = C.c_double()
+    _call(name, prec, *args, C.byref(out))
+    return out.value
+
+
+def Sum(x):
+    init()
+    return _red("um_reduce_sum", _prec(x), _dp(x), x.numel())
+
+
+def Dot(a, b):
+    init()
+    return _red("um_reduce_dot", _prec(a), _dp(a), _dp(b), a.numel())
+
+
+def MaxAbs(x):
+    init()
+    return _red("um_reduce_maxabs", _prec(x), _dp(x), x.numel())
+
+
+def MaxVecNorm(v):
+    init()
+    return _red("um_reduce_maxvecnorm", _prec(v[0]), _vec(v), v[0].numel())
+
+
+def MaxVecDiff(a, b, sqrt3_quirk=False):
+    init()
+    return _red("um_reduce_maxvecdiff", _prec(a[0]), _vec(a), _vec(b), a[0].numel(), int(sqrt3_quirk))
+
+
+class DemagConvolution:
+    """opencl.DemagConvolution (opencl/conv_demag.go:14-23): NewDemag + Exec."""
+
+    def __init__(self, N, cellsize, pbc=(0, 0, 0), accuracy=6.0, precision=F32, K=None):
+        init()
+        self.h = C.c_void_p()
+        self.N = tuple(N)
+        self.precision = precision
+        if K is None:
+            _call("um_demag_create", precision, N[0], N[1], N[2], (C.c_int * 3)(*pbc),
+                  (C.c_double * 3)(*cellsize), C.c_double(accuracy), C.byref(self.h))
+        else:
+            P = (K[0].shape[2], K[0].shape[1], K[0].shape[0])
+            ks = [np.ascontiguousarray(k, np.float64) for k in K]
+            arr = (C.c_void_p * 6)(*[k.ctypes.data for k in ks])
+            _call("um_demag_create_from_kernel", precision, N[0], N[1], N[2], (C.c_int * 3)(*P), arr,
+                  C.byref(self.h))
+
+    def Exec(self, B, m, vol, Msat):
+        ms, msm = _par(Msat)
+        _call("um_demag_exec", self.h, _vec(B), _vec(m), _dp(vol), ms, msm)
+
+    def padded_size(self):
+        P = (C.c_int * 3)()
+        _call("um_demag_info", self.h, P)
+        return tuple(P)
+
+    def kspec(self, c):
+        P = self.padded_size()
+        shape = (P[2] // 2 + 1, P[1] // 2 + 1, P[0] // 2 + 1)
+        out = np.zeros(shape, np.float32 if self.precision == F32 else np.float64)
+        _call("um_demag_kspec", self.h, c, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def Free(self):
+        if self.h:
+            _call("um_demag_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.Free()
+        except Exception:
+            pass
+
+
+def demag_kernel_bf(N, cellsize, pbc=(0, 0, 0), accuracy=6.0):
+    """CalcDemagKernel (mag/demagkernel.go:115) on the GPU in FP64 -> 6 host arrays."""
+    init()
+    P = []
+    for i in range(3):
+        if pbc[i]:
+            P.append(N[i])
+        elif i != 2 or N[i] > 5:
+            P.append(2 * N[i])
+        else:
+            P.append(2 * N[i] - 1)
+    K = [np.zeros((P[2], P[1], P[0]), np.float64) for _ in range(6)]
+    arr = (C.c_void_p * 6)(*[k.ctypes.data for k in K])
+    _call("um_demag_kernel_bf", N[0], N[1], N[2], (C.c_int * 3)(*pbc), (C.c_double * 3)(*cellsize),
+          C.c_double(accuracy), arr)
+    return K
+
+
+# ---- simulation handle (engine/) -------------------------------------------------------
+class Sim:
+    """Engine-level simulation on one GPU stream; host numpy arrays in and out."""
+
+    def __init__(self, N, cellsize, pbc=(0, 0, 0), dtype=np.float32):
+        load()
+        init()
+        self.dtype = np.dtype(dtype)
+        self.prec = F32 if self.dtype == np.float32 else F64
+        self.N = tuple(N)
+        self.n = N[0] * N[1] * N[2]
+        self.shape = (N[2], N[1], N[0])
+        self.h = C.c_void_p()
+        _call("um_sim_create", self.prec, N[0], N[1], N[2], C.c_double(cellsize[0]), C.c_double(cellsize[1]),
+              C.c_double(cellsize[2]), pbc[0], pbc[1], pbc[2], C.byref(self.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                _call("um_sim_destroy", self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _a(self, v):
+        return None if v is None else np.ascontiguousarray(v, self.dtype)
+
+    @staticmethod
+    def _p(a):
+        return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+    def set_scalar(self, pid, value):
+        if value is None or np.isscalar(value):
+            _call("um_sim_set_scalar", self.h, pid, None, C.c_double(0.0 if value is None else float(value)))
+        else:
+            a = self._a(value)
+            _call("um_sim_set_scalar", self.h, pid, self._p(a), C.c_double(1.0))
+
+    def set_vector(self, pid, value):
+        if value is None:
+            value = (0.0, 0.0, 0.0)
+        if all(np.isscalar(v) for v in value):
+            _call("um_sim_set_vector", self.h, pid, None, None, None, *[C.c_double(float(v)) for v in value])
+        else:
+            a = [self._a(v) for v in value]
+            _call("um_sim_set_vector", self.h, pid, self._p(a[0]), self._p(a[1]), self._p(a[2]),
+                  C.c_double(1), C.c_double(1), C.c_double(1))
+
+    def set_regions(self, regions):
+        r = None if regions is None else np.ascontiguousarray(regions, np.uint8)
+        _call("um_sim_set_regions", self.h, self._p(r))
+
+    def set_exchange_luts(self, aLUT, dLUT=None):
+        a, d = self._a(aLUT), self._a(dLUT)
+        _call("um_sim_set_exchange_luts", self.h, self._p(a), self._p(d))
+
+    def set_bext_lut(self, lut3):
+        if lut3 is None:
+            _call("um_sim_set_bext_lut", self.h, None, None, None)
+            return
+        ls = [self._a(l) for l in lut3]
+        _call("um_sim_set_bext_lut", self.h, self._p(ls[0]), self._p(ls[1]), self._p(ls[2]))
+
+    def set_frozen_lut(self, lut):
+        _call("um_sim_set_frozen_lut", self.h, self._p(self._a(lut)))
+
+    def set_geometry(self, vol):
+        _call("um_sim_set_geometry", self.h, self._p(self._a(vol)))
+
+    def set_flags(self, enable_demag=1, openbc=0, precess=1, heun_sqrt3=1):
+        _call("um_sim_set_flags", self.h, int(enable_demag), int(openbc), int(precess), int(heun_sqrt3))
+
+    def set_demag_accuracy(self, acc):
+        _call("um_sim_set_demag_accuracy", self.h, C.c_double(acc))
+
+    def set_thermal_seed(self, seed, stream=0):
+        _call("um_sim_set_thermal_seed", self.h, C.c_uint64(seed), C.c_uint32(stream))
+
+    def set_m(self, m):
+        m = np.ascontiguousarray(m, self.dtype).reshape(3, -1)
+        _call("um_sim_set_m", self.h, self._p(m[0]), self._p(m[1]), self._p(m[2]))
+
+    def _get3(self, name):
+        out = np.zeros((3,) + self.shape, self.dtype)
+        _call(name, self.h, self._p(out[0]), self._p(out[1]), self._p(out[2]))
+        return out
+
+    def get_m(self):
+        return self._get3("um_sim_get_m")
+
+    def beff(self):
+        return self._get3("um_sim_beff")
+
+    def torque(self):
+        return self._get3("um_sim_torque")
+
+    def demag_field(self):
+        return self._get3("um_sim_demag_field")
+
+    def set_solver(self, solver):
+        _call("um_sim_set_solver", self.h, int(solver))
+
+    def set_solver_params(self, dt=1e-15, fixdt=0.0, maxerr=1e-5, relerr=1e-4, minslope=1e8, headroom=0.8,
+                          mindt=0.0, maxdt=0.0, gammaLL=1.7595e11):
+        k = (C.c_double * 9)(dt, fixdt, maxerr, relerr, minslope, headroom, mindt, maxdt, gammaLL)
+        _call("um_sim_set_solver_params", self.h, k)
+
+    def state(self):
+        o = (C.c_double * 8)()
+        _call("um_sim_get_state", self.h, o)
+        keys = ("t", "dt", "nsteps", "nundone", "nevals", "lasterr", "peakerr", "lasttorque")
+        return dict(zip(keys, list(o)))
+
+    def step(self):
+        _call("um_sim_step", self.h)
+
+    def steps(self, n):
+        _call("um_sim_steps", self.h, int(n))
+
+    def run(self, seconds):
+        _call("um_sim_run", self.h, C.c_double(seconds))
+
+    def average_m(self):
+        o = (C.c_double * 3)()
+        _call("um_sim_average_m", self.h, o)
+        return np.array(list(o))
+
+    def energy(self, term):
+        o = C.c_double()
+        _call("um_sim_energy", self.h, int(term), C.byref(o))
+        return o.value
+
+    def stream(self):
+        s = C.c_void_p()
+        _call("um_sim_stream", self.h, C.byref(s))
+        return s.value
+
+    def m_device(self):
+        p = (C.c_void_p * 3)()
+        _call("um_sim_m_device", self.h, p)
+        return [p[i] for i in range(3)]
+
+    def set_profiling(self, on):
+        _call("um_sim_set_profiling", self.h, int(on))
+
+    def kernel_times(self):
+        ms = (C.c_double * 3)()
+        ne = C.c_int()
+        _call("um_sim_kernel_times", self.h, ms, C.byref(ne))
+        return list(ms), ne.value
