@@ -206,9 +206,10 @@ int um_sim_average_m(void *sim, double out[3]);
 int um_sim_energy(void *sim, int term, double *out);
 /* the HIP stream the simulation enqueues on */
 int um_sim_stream(void *sim, void **hip_stream);
-/* per-evaluation timing with HIP events on the simulation stream (profiling
- * on resets the counters): out_ms = total ms in {demag FFT path, fused
- * field+torque kernel, unused}, n_evals = evaluations timed */
+/* timing with HIP events on the simulation stream, read at the per-step
+ * synchronisation (profiling on resets the counters): out_ms = total ms in
+ * {demag FFT path, fused field+torque kernel, whole steps}, n_evals =
+ * evaluations timed */
 int um_sim_set_profiling(void *sim, int on);
 int um_sim_kernel_times(void *sim, double out_ms[3], int *n_evals);
 

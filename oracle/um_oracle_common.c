@@ -78,7 +78,8 @@ int um_or_demag_kernel_bf(const int N[3], const int pbc[3], const double cs[3],
   if (!(size[0] > 0 && size[1] > 0 && size[2] > 0)) return -1;
   if (!(cs[0] > 0 && cs[1] > 0 && cs[2] > 0) || !(accuracy > 0)) return -1;
   const long P = (long)size[0] * size[1] * size[2];
-  /* array[s][d] for d >= s, mapped onto K[] */
+  /* array[s][d] for d >= s, mapped onto K[].  The loop nest is parallel over
+   * (s, z): every (s, z) writes its own entries (no PBC images). */
   double *arr[3][3] = {{K[0], K[5], K[4]}, {NULL, K[1], K[3]}, {NULL, NULL, K[2]}};
   for (int c = 0; c < 6; c++) memset(K[c], 0, sizeof(double) * P);
 
@@ -89,11 +90,13 @@ int um_or_demag_kernel_bf(const int N[3], const int pbc[3], const double cs[3],
   if (cs[2] < L) L = cs[2];
 
 #define AT(a, z, y, x) (a)[((long)(z) * size[1] + (y)) * size[0] + (x)]
+  const int nzr = r2[2] - r1[2] + 1;
 #pragma omp parallel for schedule(dynamic)
-  for (int s = 0; s < 3; s++) {
+  for (int sz = 0; sz < 3 * nzr; sz++) {
+    const int s = sz / nzr, z = r1[2] + sz % nzr;
     int u = s, v = (s + 1) % 3, w = (s + 2) % 3;
     double R[3], R2[3], pole[3];
-    for (int z = r1[2]; z <= r2[2]; z++) {
+    {
       int zw = wrap_(z, size[2]);
       if (zw > size[2] / 2) continue;
       R[2] = (double)z * cs[2];

@@ -442,6 +442,37 @@ static void fft_line(cx *x, cx *y, int n, int sign, const double *twr, const dou
     int R = f[s];
     int nR = n / R;
     int tstep = n / (p * R);
+    if (R == 2 || R == 4) {
+      /* same Stockham indexing; radix-2/4 butterflies written out */
+      for (int i = 0; i < nR; i++) {
+        int k = i % p;
+        for (int j = 0; j < R; j++) {
+          cx v = a[i + j * nR];
+          int m = j * k * tstep;
+          double wr = twr[m], wi = sign < 0 ? twi[m] : -twi[m];
+          u[j].re = (real)(v.re * (real)wr - v.im * (real)wi);
+          u[j].im = (real)(v.re * (real)wi + v.im * (real)wr);
+        }
+        if (R == 2) {
+          t[0].re = u[0].re + u[1].re; t[0].im = u[0].im + u[1].im;
+          t[1].re = u[0].re - u[1].re; t[1].im = u[0].im - u[1].im;
+        } else {
+          /* y1 = (a0 - a2) + sign*i*(a1 - a3) */
+          cx s02 = {u[0].re + u[2].re, u[0].im + u[2].im}, d02 = {u[0].re - u[2].re, u[0].im - u[2].im};
+          cx s13 = {u[1].re + u[3].re, u[1].im + u[3].im}, d13 = {u[1].re - u[3].re, u[1].im - u[3].im};
+          cx r = sign < 0 ? (cx){d13.im, -d13.re} : (cx){-d13.im, d13.re};
+          t[0].re = s02.re + s13.re; t[0].im = s02.im + s13.im;
+          t[2].re = s02.re - s13.re; t[2].im = s02.im - s13.im;
+          t[1].re = d02.re + r.re; t[1].im = d02.im + r.im;
+          t[3].re = d02.re - r.re; t[3].im = d02.im - r.im;
+        }
+        int j0 = (i - k) * R + k;
+        for (int j = 0; j < R; j++) b[j0 + j * p] = t[j];
+      }
+      cx *tmp = a; a = b; b = tmp;
+      p *= R;
+      continue;
+    }
     for (int i = 0; i < nR; i++) {
       int k = i % p;
       for (int j = 0; j < R; j++) {

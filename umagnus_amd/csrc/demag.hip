@@ -18,13 +18,39 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
-#include "fft_device.hpp"
+#include "fft_ct.hpp"
 #include "sim.hpp"
 
 namespace um {
 
 static constexpr double MU0 = 4 * M_PI * 1e-7;
+
+// Data movement between HBM and LDS uses fixed-trip, fully unrolled slots
+// (slot s of thread t handles flat index g = t + s*FFT_NT, g < nl*n <= FFT_NT*EPT)
+// so every global load of a work-group is in flight before the first LDS store.
+//
+// Every pass kernel is a template on (CN, CNL): CN > 0 instantiates it for one
+// transform length and line count (compile-time LDS offsets, twiddles copied
+// into LDS); CN == 0 is the run-time generic version used for other sizes.
+
+template <int C>
+__device__ __forceinline__ int cdiv(int x, const FastDiv &f) {
+  if constexpr (C > 0) return x / C;
+  else return (int)f.div(x);
+}
+
+// FFT over the LDS lines: compile-time engine when the size is fixed.
+template <typename T, int CN, int CNL, int DIR, int MAPEM, int TWS>
+__device__ __forceinline__ void run_fft(C2<T> *lds, const C2<T> *twl, int nl, const FastDiv &fnl, const FftLen &f) {
+  if constexpr (CN > 0) {
+    ct_fft<T, CN, CNL, DIR, MAPEM, TWS>(lds, twl);
+  } else {
+    if (MAPEM) lds_fft<T, DIR, 1>(lds, nl, fnl, 1, nl, f);
+    else lds_fft<T, DIR, 0>(lds, nl, f.fn, f.n, 1, f);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // X forward: rows of real data -> half spectrum (packed two-for-one R2C).
@@ -32,12 +58,13 @@ static constexpr double MU0 = 4 * M_PI * 1e-7;
 template <typename T>
 struct XArgs {
   FftLen f;               // complex length n = Px/2
-  const C2<T> *tw2;       // exp(-2 pi i k / Px), k in [0, n)
+  const C2<T> *tw2;       // exp(-2 pi i k / Px), k in [0, Px)
   int mode;               // 0: mu0*Ms*vol*m rows of Nx valid reals; 1: raw real rows of Px
   int nl;                 // lines per work-group
+  FastDiv fhalf;          // n/2 + 1
   int Nx;                 // valid reals per input row
   int rows;               // input rows per component
-  int Ry;                 // input rows per z-plane (Ny, or Py for mode 1)
+  FastDiv fRy;            // input rows per z-plane (Ny, or Py for mode 1)
   int Py;                 // spectrum rows per z-plane
   int Hp;                 // spectrum row pitch
   long in_pitch;          // input row pitch (reals)
@@ -47,53 +74,83 @@ struct XArgs {
   C2<T> *out[3];
 };
 
-template <typename T>
+template <typename T, int CN, int CNL>
 __global__ __launch_bounds__(FFT_NT) void k_xfwd(XArgs<T> a) {
+  constexpr int EPT = FftEpt<T>::value;
   extern __shared__ __align__(16) unsigned char smem_[];
   C2<T> *lds = (C2<T> *)smem_;
   const int c = blockIdx.y;
-  const int n = a.f.n, nl = a.nl;
+  const int n = CN > 0 ? CN : a.f.n, nl = CN > 0 ? CNL : a.nl, total = nl * n;
+  C2<T> *twl = lds + nl * n;   // W_{2n}^k, k < 2n (compile-time path only)
+  if constexpr (CN > 0) {
+    for (int g = threadIdx.x; g < 2 * CN; g += FFT_NT) twl[g] = a.tw2[g];
+  }
   const int row0 = blockIdx.x * nl;
   const T *src = a.src[c];
-  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
-    const int l = g / n, e = g - l * n;
-    const int r = row0 + l;
-    T v0 = T(0), v1 = T(0);
-    if (r < a.rows) {
-      const long base = (long)r * a.in_pitch;
-      const int x0 = 2 * e, x1 = 2 * e + 1;
-      if (a.mode == 0) {
-        if (x0 < a.Nx) {
-          const long i = base + x0;
-          const T bsat = T(MU0) * a.Ms.at(i);
-          v0 = bsat * (a.vol ? a.vol[i] : T(1)) * src[i];
+  const int Nx = a.Nx;
+  // uniform Ms and no geometry: one factor mu0*Ms (copypadmul2.cl:15-19 with vol = 1)
+  const bool uniform = (a.mode != 0) || (a.Ms.arr == nullptr && a.vol == nullptr);
+  const T fac = (a.mode != 0) ? T(1) : T(MU0) * a.Ms.mul;
+  const bool pairs = ((Nx & 1) == 0) || a.mode != 0;   // rows start 8-byte aligned
+  constexpr int SL = CN > 0 ? (CN * CNL + FFT_NT - 1) / FFT_NT : EPT;
+  constexpr int BATCH = SL > EPT / 2 ? EPT / 2 : SL;
+#pragma unroll
+  for (int s0 = 0; s0 < SL; s0 += BATCH) {
+    C2<T> v[BATCH];
+#pragma unroll
+    for (int s = 0; s < BATCH; s++) {
+      const int g = threadIdx.x + (s0 + s) * FFT_NT;
+      v[s] = C2<T>{T(0), T(0)};
+      if (s0 + s < SL && g < total) {
+        const int l = cdiv<CN>(g, a.f.fn), e = g - l * n;
+        const int r = row0 + l;
+        if (r < a.rows) {
+          const long i0 = (long)r * a.in_pitch + 2 * e;
+          if (pairs && 2 * e + 1 < (a.mode != 0 ? 2 * n : Nx)) {
+            v[s] = *(const C2<T> *)(src + i0);
+          } else {
+            if (a.mode != 0 || 2 * e < Nx) v[s].x = src[i0];
+            if (a.mode != 0 || 2 * e + 1 < Nx) v[s].y = src[i0 + 1];
+          }
         }
-        if (x1 < a.Nx) {
-          const long i = base + x1;
-          const T bsat = T(MU0) * a.Ms.at(i);
-          v1 = bsat * (a.vol ? a.vol[i] : T(1)) * src[i];
-        }
-      } else {
-        v0 = src[base + x0];
-        v1 = src[base + x1];
       }
     }
-    lds[l * n + e] = C2<T>{v0, v1};
+#pragma unroll
+    for (int s = 0; s < BATCH; s++) {
+      const int g = threadIdx.x + (s0 + s) * FFT_NT;
+      if (s0 + s < SL && g < total) {
+        C2<T> w = v[s];
+        if (uniform) {
+          w.x = fac * w.x;
+          w.y = fac * w.y;
+        } else {
+          const int l = cdiv<CN>(g, a.f.fn), e = g - l * n;
+          const int r = row0 + l;
+          if (r < a.rows) {
+            const long i0 = (long)r * a.in_pitch + 2 * e;
+            if (2 * e < Nx) w.x = (T(MU0) * a.Ms.at(i0)) * (a.vol ? a.vol[i0] : T(1)) * w.x;
+            if (2 * e + 1 < Nx) w.y = (T(MU0) * a.Ms.at(i0 + 1)) * (a.vol ? a.vol[i0 + 1] : T(1)) * w.y;
+          }
+        }
+        lds[g] = w;
+      }
+    }
   }
   __syncthreads();
-  lds_fft<T, -1, 0>(lds, nl, n, 1, a.f);
+  run_fft<T, CN, CNL, -1, 0, 2>(lds, twl, nl, a.f.fn, a.f);
   // two-for-one post-processing: X[k] = E[k] + W^k O[k], X[n-k] = conj(E[k] - W^k O[k])
+  const C2<T> *tw2 = CN > 0 ? twl : a.tw2;
   const int half = n / 2 + 1;
   for (int g = threadIdx.x; g < nl * half; g += FFT_NT) {
-    const int l = g / half, k = g - l * half;
+    const int l = cdiv<(CN > 0 ? CN / 2 + 1 : 0)>(g, a.fhalf), k = g - l * half;
     const int r = row0 + l;
     if (r >= a.rows) continue;
     const C2<T> A = lds[l * n + k];
     const C2<T> Bc = cconj(lds[l * n + (k == 0 ? 0 : n - k)]);
     const C2<T> E = C2<T>{T(0.5) * (A.x + Bc.x), T(0.5) * (A.y + Bc.y)};
     const C2<T> O = C2<T>{T(0.5) * (A.y - Bc.y), T(-0.5) * (A.x - Bc.x)};
-    const C2<T> t = cmul(a.tw2[k], O);
-    const int y = r % a.Ry, z = r / a.Ry;
+    const C2<T> t = cmul(tw2[k], O);
+    const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
     C2<T> *orow = a.out[c] + ((long)z * a.Py + y) * a.Hp;
     orow[k] = cadd(E, t);
     if (n - k != k) orow[n - k] = cconj(csub(E, t));
@@ -107,55 +164,87 @@ template <typename T>
 struct XiArgs {
   FftLen f;
   const C2<T> *tw2;
-  int nl, Nx, rows, Ry, Py, Hp;
+  int nl, Nx, rows, Py, Hp;
+  FastDiv fhalf, fRy;
   const C2<T> *in[3];
   T *dst[3];
 };
 
-template <typename T>
+template <typename T, int CN, int CNL>
 __global__ __launch_bounds__(FFT_NT) void k_xinv(XiArgs<T> a) {
+  constexpr int EPT = FftEpt<T>::value;
   extern __shared__ __align__(16) unsigned char smem_[];
   C2<T> *lds = (C2<T> *)smem_;
   const int c = blockIdx.y;
-  const int n = a.f.n, nl = a.nl;
+  const int n = CN > 0 ? CN : a.f.n, nl = CN > 0 ? CNL : a.nl;
+  C2<T> *twl = lds + nl * n;
+  if constexpr (CN > 0) {
+    for (int g = threadIdx.x; g < 2 * CN; g += FFT_NT) twl[g] = a.tw2[g];
+  }
+  const C2<T> *tw2 = a.tw2;   // pre-processing reads the global table (once per element)
   const int row0 = blockIdx.x * nl;
   const int half = n / 2 + 1;
-  for (int g = threadIdx.x; g < nl * half; g += FFT_NT) {
-    const int l = g / half, k = g - l * half;
-    const int r = row0 + l;
-    C2<T> zk = C2<T>{T(0), T(0)}, znk = zk;
-    if (r < a.rows) {
-      const int y = r % a.Ry, z = r / a.Ry;
-      const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
-      const C2<T> Xk = irow[k], Xnk = irow[n - k];
-      // Z'[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k])
-      C2<T> s = cadd(Xk, cconj(Xnk)), d = csub(Xk, cconj(Xnk));
-      C2<T> w = cconj(a.tw2[k]);
-      C2<T> t = cmul(w, d);
-      zk = C2<T>{s.x - t.y, s.y + t.x};
-      if (k != 0 && n - k != k) {
-        s = cadd(Xnk, cconj(Xk));
-        d = csub(Xnk, cconj(Xk));
-        w = cconj(a.tw2[n - k]);
-        t = cmul(w, d);
-        znk = C2<T>{s.x - t.y, s.y + t.x};
+  const int total = nl * half;
+  constexpr int S = CN > 0 ? (CNL * (CN / 2 + 1) + FFT_NT - 1) / FFT_NT : EPT / 2 + 1;
+  for (int g0 = 0; g0 < total; g0 += S * FFT_NT) {
+    C2<T> xk[S], xn[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const int g = g0 + threadIdx.x + s * FFT_NT;
+      xk[s] = C2<T>{T(0), T(0)};
+      xn[s] = xk[s];
+      if (g < total) {
+        const int l = cdiv<(CN > 0 ? CN / 2 + 1 : 0)>(g, a.fhalf), k = g - l * half;
+        const int r = row0 + l;
+        if (r < a.rows) {
+          const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+          const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
+          xk[s] = irow[k];
+          xn[s] = irow[n - k];
+        }
       }
     }
-    lds[l * n + k] = zk;
-    if (k != 0 && n - k != k) lds[l * n + n - k] = znk;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const int g = g0 + threadIdx.x + s * FFT_NT;
+      if (g < total) {
+        const int l = cdiv<(CN > 0 ? CN / 2 + 1 : 0)>(g, a.fhalf), k = g - l * half;
+        const C2<T> Xk = xk[s], Xnk = xn[s];
+        // Z'[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k])
+        C2<T> sm = cadd(Xk, cconj(Xnk)), d = csub(Xk, cconj(Xnk));
+        C2<T> t = cmul(cconj(tw2[k]), d);
+        lds[l * n + k] = C2<T>{sm.x - t.y, sm.y + t.x};
+        if (k != 0 && n - k != k) {
+          sm = cadd(Xnk, cconj(Xk));
+          d = csub(Xnk, cconj(Xk));
+          t = cmul(cconj(tw2[n - k]), d);
+          lds[l * n + n - k] = C2<T>{sm.x - t.y, sm.y + t.x};
+        }
+      }
+    }
   }
   __syncthreads();
-  lds_fft<T, +1, 0>(lds, nl, n, 1, a.f);
+  run_fft<T, CN, CNL, +1, 0, 2>(lds, twl, nl, a.f.fn, a.f);
   T *dst = a.dst[c];
-  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
-    const int l = g / n, e = g - l * n;
-    const int r = row0 + l;
-    if (r >= a.rows) continue;
-    const C2<T> v = lds[l * n + e];
-    const long base = (long)r * a.Nx;
-    const int x0 = 2 * e, x1 = 2 * e + 1;
-    if (x0 < a.Nx) dst[base + x0] = v.x;
-    if (x1 < a.Nx) dst[base + x1] = v.y;
+  const int Nx = a.Nx;
+  constexpr int SL = CN > 0 ? (CN * CNL + FFT_NT - 1) / FFT_NT : EPT;
+#pragma unroll
+  for (int s = 0; s < SL; s++) {
+    const int g = threadIdx.x + s * FFT_NT;
+    if (g < nl * n) {
+      const int l = cdiv<CN>(g, a.f.fn), e = g - l * n;
+      const int r = row0 + l;
+      if (r < a.rows) {
+        const C2<T> v = lds[g];
+        const long base = (long)r * Nx;
+        if (2 * e + 1 < Nx && (Nx & 1) == 0) {
+          *(C2<T> *)(dst + base + 2 * e) = v;
+        } else {
+          if (2 * e < Nx) dst[base + 2 * e] = v.x;
+          if (2 * e + 1 < Nx) dst[base + 2 * e + 1] = v.y;
+        }
+      }
+    }
   }
 }
 
@@ -166,6 +255,7 @@ template <typename T>
 struct ColArgs {
   FftLen f;
   int nl;          // columns per work-group
+  FastDiv fnl;
   int H;           // valid kx columns
   int nvalid;      // input elements that may be non-zero
   int nout;        // output elements stored
@@ -174,26 +264,46 @@ struct ColArgs {
   C2<T> *buf[3];
 };
 
-template <typename T, int DIR>
+template <typename T, int DIR, int CN, int CNL>
 __global__ __launch_bounds__(FFT_NT) void k_colfft(ColArgs<T> a) {
+  constexpr int EPT = FftEpt<T>::value;
   extern __shared__ __align__(16) unsigned char smem_[];
   C2<T> *lds = (C2<T> *)smem_;
-  const int n = a.f.n, nl = a.nl;
+  const int n = CN > 0 ? CN : a.f.n, nl = CN > 0 ? CNL : a.nl, total = nl * n;
+  C2<T> *twl = lds + nl * n;
+  if constexpr (CN > 0) {
+    const C2<T> *tw = (const C2<T> *)a.f.tw;
+    for (int g = threadIdx.x; g < CN; g += FFT_NT) twl[g] = tw[g];
+  }
   const int kx0 = blockIdx.x * nl;
   C2<T> *base = a.buf[blockIdx.z] + (long)blockIdx.y * a.pstride;
-  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
-    const int l = g % nl, e = g / nl;
-    const int kx = kx0 + l;
-    C2<T> v = C2<T>{T(0), T(0)};
-    if (kx < a.H && e < a.nvalid) v = base[(long)e * a.estride + kx];
-    lds[e * nl + l] = v;
+  constexpr int SL = CN > 0 ? (CN * CNL + FFT_NT - 1) / FFT_NT : EPT;
+  C2<T> v[SL];
+#pragma unroll
+  for (int s = 0; s < SL; s++) {
+    const int g = threadIdx.x + s * FFT_NT;
+    v[s] = C2<T>{T(0), T(0)};
+    if (g < total) {
+      const int e = cdiv<CNL>(g, a.fnl), l = g - e * nl;
+      const int kx = kx0 + l;
+      if (kx < a.H && e < a.nvalid) v[s] = base[(long)e * a.estride + kx];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < SL; s++) {
+    const int g = threadIdx.x + s * FFT_NT;
+    if (g < total) lds[g] = v[s];   // element-major: g = e*nl + l
   }
   __syncthreads();
-  lds_fft<T, DIR, 1>(lds, nl, 1, nl, a.f);
-  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
-    const int l = g % nl, e = g / nl;
-    const int kx = kx0 + l;
-    if (kx < a.H && e < a.nout) base[(long)e * a.estride + kx] = lds[e * nl + l];
+  run_fft<T, CN, CNL, DIR, 1, 1>(lds, twl, nl, a.fnl, a.f);
+#pragma unroll
+  for (int s = 0; s < SL; s++) {
+    const int g = threadIdx.x + s * FFT_NT;
+    if (g < total) {
+      const int e = cdiv<CNL>(g, a.fnl), l = g - e * nl;
+      const int kx = kx0 + l;
+      if (kx < a.H && e < a.nout) base[(long)e * a.estride + kx] = lds[g];
+    }
   }
 }
 
@@ -206,6 +316,7 @@ template <typename T>
 struct ConvArgs {
   FftLen f;        // length along the convolved axis (Pz in 3D, Py in 2D)
   int B;           // kx columns per work-group (lines = 3B)
+  FastDiv fB, f3B;
   int H;           // valid kx columns
   int nvalid;      // non-zero input elements along the axis (Nz or Ny)
   long estride;    // element stride along the axis (complex)
@@ -217,59 +328,111 @@ struct ConvArgs {
   C2<T> *buf[3];
 };
 
-template <typename T>
+constexpr int CONV_QK = 6;   // tensor prefetch slots per thread: B*(n/2+1) <= FFT_NT*CONV_QK
+
+template <typename T, int CN, int CB>
 __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
+  constexpr int EPT = FftEpt<T>::value;
   extern __shared__ __align__(16) unsigned char smem_[];
   C2<T> *lds = (C2<T> *)smem_;
-  const int n = a.f.n, B = a.B, nl = 3 * B;
+  const int n = CN > 0 ? CN : a.f.n, B = CN > 0 ? CB : a.B, nl = 3 * B, total = nl * n;
+  C2<T> *twl = lds + nl * n;
+  if constexpr (CN > 0) {
+    const C2<T> *tw = (const C2<T> *)a.f.tw;
+    for (int g = threadIdx.x; g < CN; g += FFT_NT) twl[g] = tw[g];
+  }
   const int kx0 = blockIdx.x * B;
   const int ky = blockIdx.y;  // 3D only
   const long off = (long)ky * a.pstride;
-  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
-    const int l = g % nl, e = g / nl;
-    const int c = l / B, b = l - c * B;
-    const int kx = kx0 + b;
-    C2<T> v = C2<T>{T(0), T(0)};
-    if (kx < a.H && e < a.nvalid) v = a.buf[c][off + (long)e * a.estride + kx];
-    lds[e * nl + l] = v;
+  const int nh = n / 2 + 1;   // unique points along the axis: k and n-k share |K|
+  constexpr int QK = CN > 0 ? (CB * (CN / 2 + 1) + FFT_NT - 1) / FFT_NT : CONV_QK;
+  // 1. tensor prefetch (independent of the data, so it overlaps the loads below)
+  T kv[QK][6];
+#pragma unroll
+  for (int q = 0; q < QK; q++) {
+    const int g = threadIdx.x + q * FFT_NT;
+#pragma unroll
+    for (int j = 0; j < 6; j++) kv[q][j] = T(0);
+    if (g < B * nh) {
+      const int e = cdiv<CB>(g, a.fB), b = g - e * B;
+      const int kx = kx0 + b;
+      if (kx < a.H) {
+        if (!a.is2d) {
+          int jy = ky;
+          T sYZ = T(1), sXY = T(1);
+          if (jy > a.Py / 2) { jy = a.Py - jy; sYZ = -sYZ; sXY = -sXY; }
+          const long I = ((long)e * a.KHy + jy) * a.H + kx;
+          kv[q][0] = a.K[0][I]; kv[q][1] = a.K[1][I]; kv[q][2] = a.K[2][I];
+          kv[q][3] = a.K[3][I] * sYZ; kv[q][4] = a.K[4][I]; kv[q][5] = a.K[5][I] * sXY;
+        } else {
+          const long I = (long)e * a.H + kx;
+          kv[q][0] = a.K[0][I]; kv[q][1] = a.K[1][I]; kv[q][2] = a.K[2][I]; kv[q][5] = a.K[5][I];
+        }
+      }
+    }
+  }
+  // 2. spectrum loads
+  constexpr int SL = CN > 0 ? (3 * CB * CN + FFT_NT - 1) / FFT_NT : EPT;
+  C2<T> v[SL];
+#pragma unroll
+  for (int s = 0; s < SL; s++) {
+    const int g = threadIdx.x + s * FFT_NT;
+    v[s] = C2<T>{T(0), T(0)};
+    if (g < total) {
+      const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
+      const int c = cdiv<CB>(l, a.fB), b = l - c * B;
+      const int kx = kx0 + b;
+      if (kx < a.H && e < a.nvalid) v[s] = a.buf[c][off + (long)e * a.estride + kx];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < SL; s++) {
+    const int g = threadIdx.x + s * FFT_NT;
+    if (g < total) lds[g] = v[s];
   }
   __syncthreads();
-  lds_fft<T, -1, 1>(lds, nl, 1, nl, a.f);
-  for (int g = threadIdx.x; g < B * n; g += FFT_NT) {
-    const int b = g % B, e = g / B;
-    const int kx = kx0 + b;
-    if (kx >= a.H) continue;
-    C2<T> *p = lds + e * nl + b;
+  run_fft<T, CN, 3 * CB, -1, 1, 1>(lds, twl, nl, a.f3B, a.f);
+  // 3. tensor multiply at k and at its mirror n-k (signs of the odd components flip)
+  auto mul = [&](C2<T> *p, T xx, T yy, T zz, T yz, T xz, T xy) {
     const C2<T> M0 = p[0], M1 = p[B], M2 = p[2 * B];
     if (!a.is2d) {
-      int jy = ky, jz = e;
-      T sYZ = T(1), sXZ = T(1), sXY = T(1);
-      if (jy > a.Py / 2) { jy = a.Py - jy; sYZ = -sYZ; sXY = -sXY; }
-      if (jz > a.Pz / 2) { jz = a.Pz - jz; sYZ = -sYZ; sXZ = -sXZ; }
-      const long I = ((long)jz * a.KHy + jy) * a.H + kx;
-      const T xx = a.K[0][I], yy = a.K[1][I], zz = a.K[2][I];
-      const T yz = a.K[3][I] * sYZ, xz = a.K[4][I] * sXZ, xy = a.K[5][I] * sXY;
       p[0] = C2<T>{M0.x * xx + M1.x * xy + M2.x * xz, M0.y * xx + M1.y * xy + M2.y * xz};
       p[B] = C2<T>{M0.x * xy + M1.x * yy + M2.x * yz, M0.y * xy + M1.y * yy + M2.y * yz};
       p[2 * B] = C2<T>{M0.x * xz + M1.x * yz + M2.x * zz, M0.y * xz + M1.y * yz + M2.y * zz};
     } else {
-      int jy = e;
-      T fxy = T(1);
-      if (jy > a.Py / 2) { jy = a.Py - jy; fxy = -fxy; }
-      const long I = (long)jy * a.H + kx;
-      const T xx = a.K[0][I], yy = a.K[1][I], zz = a.K[2][I], xy = fxy * a.K[5][I];
       p[0] = C2<T>{M0.x * xx + M1.x * xy, M0.y * xx + M1.y * xy};
       p[B] = C2<T>{M0.x * xy + M1.x * yy, M0.y * xy + M1.y * yy};
       p[2 * B] = C2<T>{M2.x * zz, M2.y * zz};
     }
+  };
+#pragma unroll
+  for (int q = 0; q < QK; q++) {
+    const int g = threadIdx.x + q * FFT_NT;
+    if (g < B * nh) {
+      const int e = cdiv<CB>(g, a.fB), b = g - e * B;
+      if (kx0 + b < a.H) {
+        const T *k = kv[q];
+        mul(lds + e * nl + b, k[0], k[1], k[2], k[3], k[4], k[5]);
+        const int e2 = n - e;
+        if (e != 0 && e2 != e) {
+          // 3D mirror in z flips yz and xz (kernmulrsymm3d.cl:67-71); 2D mirror in y flips xy
+          if (!a.is2d) mul(lds + e2 * nl + b, k[0], k[1], k[2], -k[3], -k[4], k[5]);
+          else mul(lds + e2 * nl + b, k[0], k[1], k[2], k[3], k[4], -k[5]);
+        }
+      }
+    }
   }
   __syncthreads();
-  lds_fft<T, +1, 1>(lds, nl, 1, nl, a.f);
-  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
-    const int l = g % nl, e = g / nl;
-    const int c = l / B, b = l - c * B;
-    const int kx = kx0 + b;
-    if (kx < a.H && e < a.nvalid) a.buf[c][off + (long)e * a.estride + kx] = lds[e * nl + l];
+  run_fft<T, CN, 3 * CB, +1, 1, 1>(lds, twl, nl, a.f3B, a.f);
+#pragma unroll
+  for (int s = 0; s < SL; s++) {
+    const int g = threadIdx.x + s * FFT_NT;
+    if (g < total) {
+      const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
+      const int c = cdiv<CB>(l, a.fB), b = l - c * B;
+      const int kx = kx0 + b;
+      if (kx < a.H && e < a.nvalid) a.buf[c][off + (long)e * a.estride + kx] = lds[g];
+    }
   }
 }
 
@@ -440,7 +603,14 @@ struct TwTable {
     UM_REQUIRE((int)r.size() <= FFT_MAXPASS, UM_EUNSUP, "FFT length too long");
     len.n = n;
     len.np = (n > 1) ? (int)r.size() : 0;
-    for (int i = 0; i < len.np; i++) len.rad[i] = r[i];
+    len.fn.init(n > 0 ? n : 1);
+    int pp = 1;
+    for (int i = 0; i < len.np; i++) {
+      len.rad[i] = r[i];
+      len.fp[i].init(pp);
+      len.fnr[i].init(n / r[i]);
+      pp *= r[i];
+    }
     std::vector<C2<T>> h(n > 0 ? n : 1);
     for (int m = 0; m < n; m++) {
       const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
@@ -452,6 +622,114 @@ struct TwTable {
     len.tw = buf.p;
   }
 };
+
+// ---- launch dispatch: compile-time instantiations for the hot sizes ------------
+// Lines per work-group of each specialised length (LDS budget ~40-72 KiB so two
+// or more work-groups share a CU); other lengths use the generic kernels.
+template <typename T> struct CtTable;
+template <> struct CtTable<float> {
+  static int x(int n) { switch (n) { case 64: return 32; case 128: return 16; case 256: return 16; case 512: return 8; case 1024: return 4; } return 0; }
+  static int col(int n) { switch (n) { case 64: return 32; case 128: return 16; case 256: return 16; case 512: return 8; case 1024: return 8; case 2048: return 4; } return 0; }
+  static int conv3(int n) { switch (n) { case 16: return 16; case 32: return 16; case 64: return 16; } return 0; }
+  static int conv2(int n) { switch (n) { case 64: return 16; case 128: return 16; case 256: return 8; case 512: return 4; } return 0; }
+};
+template <> struct CtTable<double> {
+  static int x(int n) { switch (n) { case 256: return 8; case 512: return 4; } return 0; }
+  static int col(int n) { switch (n) { case 512: return 4; case 1024: return 4; } return 0; }
+  static int conv3(int n) { switch (n) { case 16: return 8; case 64: return 8; } return 0; }
+  static int conv2(int n) { switch (n) { case 256: return 4; case 512: return 2; } return 0; }
+};
+
+template <typename K>
+static void allow_lds(K kern, size_t bytes) {
+  if (bytes > 65536) UM_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+#define UM_CT_CASE(KERN, N, NL, LDSB)                                        \
+  if (n == N && nl == NL) {                                                  \
+    allow_lds(KERN, LDSB);                                                   \
+    hipLaunchKernelGGL(KERN, grid, dim3(FFT_NT), LDSB, s, a);                \
+    UM_HIP(hipGetLastError());                                               \
+    return;                                                                  \
+  }
+
+template <typename T>
+static void launch_xfwd(const XArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s) {
+  const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + 2 * n);
+  if constexpr (std::is_same<T, float>::value) {
+    UM_CT_CASE((k_xfwd<float, 64, 32>), 64, 32, ct)
+    UM_CT_CASE((k_xfwd<float, 128, 16>), 128, 16, ct)
+    UM_CT_CASE((k_xfwd<float, 256, 16>), 256, 16, ct)
+    UM_CT_CASE((k_xfwd<float, 512, 8>), 512, 8, ct)
+    UM_CT_CASE((k_xfwd<float, 1024, 4>), 1024, 4, ct)
+  } else {
+    UM_CT_CASE((k_xfwd<double, 256, 8>), 256, 8, ct)
+    UM_CT_CASE((k_xfwd<double, 512, 4>), 512, 4, ct)
+  }
+  hipLaunchKernelGGL((k_xfwd<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+static void launch_xinv(const XiArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s) {
+  const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + 2 * n);
+  if constexpr (std::is_same<T, float>::value) {
+    UM_CT_CASE((k_xinv<float, 64, 32>), 64, 32, ct)
+    UM_CT_CASE((k_xinv<float, 128, 16>), 128, 16, ct)
+    UM_CT_CASE((k_xinv<float, 256, 16>), 256, 16, ct)
+    UM_CT_CASE((k_xinv<float, 512, 8>), 512, 8, ct)
+    UM_CT_CASE((k_xinv<float, 1024, 4>), 1024, 4, ct)
+  } else {
+    UM_CT_CASE((k_xinv<double, 256, 8>), 256, 8, ct)
+    UM_CT_CASE((k_xinv<double, 512, 4>), 512, 4, ct)
+  }
+  hipLaunchKernelGGL((k_xinv<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
+  UM_HIP(hipGetLastError());
+}
+template <typename T, int DIR>
+static void launch_col(const ColArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s) {
+  const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + n);
+  if constexpr (std::is_same<T, float>::value) {
+    UM_CT_CASE((k_colfft<float, DIR, 64, 32>), 64, 32, ct)
+    UM_CT_CASE((k_colfft<float, DIR, 128, 16>), 128, 16, ct)
+    UM_CT_CASE((k_colfft<float, DIR, 256, 16>), 256, 16, ct)
+    UM_CT_CASE((k_colfft<float, DIR, 512, 8>), 512, 8, ct)
+    UM_CT_CASE((k_colfft<float, DIR, 1024, 8>), 1024, 8, ct)
+    UM_CT_CASE((k_colfft<float, DIR, 2048, 4>), 2048, 4, ct)
+  } else {
+    UM_CT_CASE((k_colfft<double, DIR, 512, 4>), 512, 4, ct)
+    UM_CT_CASE((k_colfft<double, DIR, 1024, 4>), 1024, 4, ct)
+  }
+  hipLaunchKernelGGL((k_colfft<T, DIR, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+static void launch_conv(const ConvArgs<T> &a, int n, int B, dim3 grid, hipStream_t s) {
+  const int nl = B;  // UM_CT_CASE compares (n, nl)
+  const size_t ct = sizeof(C2<T>) * ((size_t)3 * B * n + n);
+  if (!a.is2d) {
+    if constexpr (std::is_same<T, float>::value) {
+      UM_CT_CASE((k_conv<float, 16, 16>), 16, 16, ct)
+      UM_CT_CASE((k_conv<float, 32, 16>), 32, 16, ct)
+      UM_CT_CASE((k_conv<float, 64, 16>), 64, 16, ct)
+    } else {
+      UM_CT_CASE((k_conv<double, 16, 8>), 16, 8, ct)
+      UM_CT_CASE((k_conv<double, 64, 8>), 64, 8, ct)
+    }
+  } else {
+    if constexpr (std::is_same<T, float>::value) {
+      UM_CT_CASE((k_conv<float, 64, 16>), 64, 16, ct)
+      UM_CT_CASE((k_conv<float, 128, 16>), 128, 16, ct)
+      UM_CT_CASE((k_conv<float, 256, 8>), 256, 8, ct)
+      UM_CT_CASE((k_conv<float, 512, 4>), 512, 4, ct)
+    } else {
+      UM_CT_CASE((k_conv<double, 256, 4>), 256, 4, ct)
+      UM_CT_CASE((k_conv<double, 512, 2>), 512, 2, ct)
+    }
+  }
+  hipLaunchKernelGGL((k_conv<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * 3 * B * n, s, a);
+  UM_HIP(hipGetLastError());
+}
+#undef UM_CT_CASE
 
 struct DemagBase {
   virtual ~DemagBase() {}
@@ -469,7 +747,6 @@ struct DemagPlan : DemagBase {
   DevBuf spec[3];               // per component: Nz*Py*Hp complex
   DevBuf kern;                  // 6 x H*KHy*KHz of T
   int nl_x = 1, nl_y = 1, B_conv = 1;
-  size_t lds_x = 0, lds_y = 0, lds_conv = 0;
   long kcount = 0;
 
   static int lines_for(int n) {
@@ -493,14 +770,20 @@ struct DemagPlan : DemagBase {
     tx2.make(P[0], s);
     ty.make(P[1], s);
     tz.make(P[2], s);
-    nl_x = lines_for(P[0] / 2);
-    lds_x = sizeof(C2<T>) * nl_x * (P[0] / 2);
-    nl_y = std::min(lines_for(P[1]), 64);
-    lds_y = sizeof(C2<T>) * nl_y * P[1];
+    nl_x = CtTable<T>::x(P[0] / 2);
+    if (!nl_x) nl_x = lines_for(P[0] / 2);
+    nl_y = CtTable<T>::col(P[1]);
+    if (!nl_y) nl_y = std::min(lines_for(P[1]), 64);
     const int convlen = is2d ? P[1] : P[2];
-    B_conv = std::min(lines_for(convlen) / 3, 64);
-    UM_REQUIRE(B_conv >= 1, UM_EUNSUP, "convolution axis too long");
-    lds_conv = sizeof(C2<T>) * 3 * B_conv * convlen;
+    B_conv = is2d ? CtTable<T>::conv2(convlen) : CtTable<T>::conv3(convlen);
+    if (!B_conv) {
+      const int bmax = std::min(lines_for(convlen) / 3, 32);
+      UM_REQUIRE(bmax >= 1, UM_EUNSUP, "convolution axis too long");
+      B_conv = 1;
+      while (B_conv * 2 <= bmax) B_conv *= 2;
+    }
+    UM_REQUIRE(B_conv * (convlen / 2 + 1) <= FFT_NT * CONV_QK, UM_EUNSUP, "convolution prefetch slots exceeded");
+    UM_REQUIRE(3 * B_conv * convlen <= FFT_NT * FftEpt<T>::value, UM_EUNSUP, "convolution tile exceeds LDS budget");
     const size_t sc = is2d ? (size_t)N[1] * Hp : (size_t)N[2] * P[1] * Hp;
     for (int c = 0; c < 3; c++) spec[c].alloc(sizeof(C2<T>) * sc);
     kcount = (long)H * KHy * KHz;
@@ -531,27 +814,28 @@ struct DemagPlan : DemagBase {
       }
       XArgs<double> xa{};
       xa.f = dx.len; xa.tw2 = (const double2 *)dx2.buf.p; xa.mode = 1;
-      xa.nl = capd / (P[0] / 2); xa.Nx = P[0]; xa.rows = P[1] * P[2]; xa.Ry = P[1]; xa.Py = P[1]; xa.Hp = Hpd;
+      xa.nl = capd / (P[0] / 2); xa.fhalf.init(P[0] / 4 + 1); xa.Nx = P[0]; xa.rows = P[1] * P[2];
+      xa.fRy.init(P[1]); xa.Py = P[1]; xa.Hp = Hpd;
       xa.in_pitch = P[0];
       xa.src[0] = Kdev[c]; xa.Ms = Par<double>{nullptr, 1.0}; xa.vol = nullptr;
       xa.out[0] = (double2 *)sp.p;
       dim3 gx((xa.rows + xa.nl - 1) / xa.nl, 1);
-      hipLaunchKernelGGL(k_xfwd<double>, gx, dim3(FFT_NT), sizeof(double2) * xa.nl * (P[0] / 2), s, xa);
+      hipLaunchKernelGGL((k_xfwd<double, 0, 0>), gx, dim3(FFT_NT), sizeof(double2) * xa.nl * (P[0] / 2), s, xa);
       UM_HIP(hipGetLastError());
       if (P[1] > 1) {
         ColArgs<double> ca{};
-        ca.f = dy.len; ca.nl = std::min(capd / P[1], 64); ca.H = H; ca.nvalid = P[1]; ca.nout = P[1];
+        ca.f = dy.len; ca.nl = std::min(capd / P[1], 64); ca.fnl.init(ca.nl); ca.H = H; ca.nvalid = P[1]; ca.nout = P[1];
         ca.estride = Hpd; ca.pstride = (long)P[1] * Hpd; ca.buf[0] = (double2 *)sp.p;
         dim3 g((H + ca.nl - 1) / ca.nl, P[2], 1);
-        hipLaunchKernelGGL((k_colfft<double, -1>), g, dim3(FFT_NT), sizeof(double2) * ca.nl * P[1], s, ca);
+        hipLaunchKernelGGL((k_colfft<double, -1, 0, 0>), g, dim3(FFT_NT), sizeof(double2) * ca.nl * P[1], s, ca);
         UM_HIP(hipGetLastError());
       }
       if (P[2] > 1) {
         ColArgs<double> ca{};
-        ca.f = dz.len; ca.nl = std::min(capd / P[2], 64); ca.H = H; ca.nvalid = P[2]; ca.nout = P[2];
+        ca.f = dz.len; ca.nl = std::min(capd / P[2], 64); ca.fnl.init(ca.nl); ca.H = H; ca.nvalid = P[2]; ca.nout = P[2];
         ca.estride = (long)P[1] * Hpd; ca.pstride = Hpd; ca.buf[0] = (double2 *)sp.p;
         dim3 g((H + ca.nl - 1) / ca.nl, P[1], 1);
-        hipLaunchKernelGGL((k_colfft<double, -1>), g, dim3(FFT_NT), sizeof(double2) * ca.nl * P[2], s, ca);
+        hipLaunchKernelGGL((k_colfft<double, -1, 0, 0>), g, dim3(FFT_NT), sizeof(double2) * ca.nl * P[2], s, ca);
         UM_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL(k_extract_kspec<T>, dim3(blocks_for(kcount, 256, 4096)), dim3(256), 0, s,
@@ -576,54 +860,50 @@ struct DemagPlan : DemagBase {
     // X forward
     {
       XArgs<T> a{};
-      a.f = tx.len; a.tw2 = (const C2<T> *)tx2.buf.p; a.mode = 0; a.nl = nl_x; a.Nx = Nx;
-      a.rows = Ny * Nz; a.Ry = Ny; a.Py = is2d ? Ny : P[1]; a.Hp = Hp; a.in_pitch = Nx;
+      a.f = tx.len; a.tw2 = (const C2<T> *)tx2.buf.p; a.mode = 0; a.nl = nl_x; a.fhalf.init(P[0] / 4 + 1);
+      a.Nx = Nx; a.rows = Ny * Nz; a.fRy.init(Ny); a.Py = is2d ? Ny : P[1]; a.Hp = Hp; a.in_pitch = Nx;
       for (int c = 0; c < 3; c++) { a.src[c] = (const T *)mv[c]; a.out[c] = sp[c]; }
       a.Ms = Par<T>{(const T *)Msa, (T)Ms_mul}; a.vol = (const T *)vol;
       dim3 g((a.rows + nl_x - 1) / nl_x, 3);
-      hipLaunchKernelGGL(k_xfwd<T>, g, dim3(FFT_NT), lds_x, s, a);
-      UM_HIP(hipGetLastError());
+      launch_xfwd<T>(a, P[0] / 2, nl_x, g, s);
     }
     const T *K[6];
     for (int c = 0; c < 6; c++) K[c] = kern.as<T>() + c * kcount;
     if (!is2d) {
       ColArgs<T> ya{};
-      ya.f = ty.len; ya.nl = nl_y; ya.H = H; ya.estride = Hp; ya.pstride = (long)P[1] * Hp;
+      ya.f = ty.len; ya.nl = nl_y; ya.fnl.init(nl_y); ya.H = H; ya.estride = Hp; ya.pstride = (long)P[1] * Hp;
       for (int c = 0; c < 3; c++) ya.buf[c] = sp[c];
       dim3 gy((H + nl_y - 1) / nl_y, Nz, 3);
       ya.nvalid = Ny; ya.nout = P[1];
-      hipLaunchKernelGGL((k_colfft<T, -1>), gy, dim3(FFT_NT), lds_y, s, ya);
-      UM_HIP(hipGetLastError());
+      launch_col<T, -1>(ya, P[1], nl_y, gy, s);
       ConvArgs<T> ca{};
-      ca.f = tz.len; ca.B = B_conv; ca.H = H; ca.nvalid = Nz; ca.estride = (long)P[1] * Hp; ca.pstride = Hp;
+      ca.f = tz.len; ca.B = B_conv; ca.fB.init(B_conv); ca.f3B.init(3 * B_conv); ca.H = H; ca.nvalid = Nz;
+      ca.estride = (long)P[1] * Hp; ca.pstride = Hp;
       ca.is2d = 0; ca.Py = P[1]; ca.Pz = P[2]; ca.KHy = KHy;
       for (int c = 0; c < 6; c++) ca.K[c] = K[c];
       for (int c = 0; c < 3; c++) ca.buf[c] = sp[c];
       dim3 gc((H + B_conv - 1) / B_conv, P[1]);
-      hipLaunchKernelGGL(k_conv<T>, gc, dim3(FFT_NT), lds_conv, s, ca);
-      UM_HIP(hipGetLastError());
+      launch_conv<T>(ca, P[2], B_conv, gc, s);
       ya.nvalid = P[1]; ya.nout = Ny;
-      hipLaunchKernelGGL((k_colfft<T, +1>), gy, dim3(FFT_NT), lds_y, s, ya);
-      UM_HIP(hipGetLastError());
+      launch_col<T, +1>(ya, P[1], nl_y, gy, s);
     } else {
       ConvArgs<T> ca{};
-      ca.f = ty.len; ca.B = B_conv; ca.H = H; ca.nvalid = Ny; ca.estride = Hp; ca.pstride = 0;
+      ca.f = ty.len; ca.B = B_conv; ca.fB.init(B_conv); ca.f3B.init(3 * B_conv); ca.H = H; ca.nvalid = Ny;
+      ca.estride = Hp; ca.pstride = 0;
       ca.is2d = 1; ca.Py = P[1]; ca.Pz = 1; ca.KHy = KHy;
       for (int c = 0; c < 6; c++) ca.K[c] = K[c];
       for (int c = 0; c < 3; c++) ca.buf[c] = sp[c];
       dim3 gc((H + B_conv - 1) / B_conv, 1);
-      hipLaunchKernelGGL(k_conv<T>, gc, dim3(FFT_NT), lds_conv, s, ca);
-      UM_HIP(hipGetLastError());
+      launch_conv<T>(ca, P[1], B_conv, gc, s);
     }
     // X inverse
     {
       XiArgs<T> a{};
       a.f = tx.len; a.tw2 = (const C2<T> *)tx2.buf.p; a.nl = nl_x; a.Nx = Nx; a.rows = Ny * Nz;
-      a.Ry = Ny; a.Py = is2d ? Ny : P[1]; a.Hp = Hp;
+      a.fhalf.init(P[0] / 4 + 1); a.fRy.init(Ny); a.Py = is2d ? Ny : P[1]; a.Hp = Hp;
       for (int c = 0; c < 3; c++) { a.in[c] = sp[c]; a.dst[c] = (T *)Bv[c]; }
       dim3 g((a.rows + nl_x - 1) / nl_x, 3);
-      hipLaunchKernelGGL(k_xinv<T>, g, dim3(FFT_NT), lds_x, s, a);
-      UM_HIP(hipGetLastError());
+      launch_xinv<T>(a, P[0] / 2, nl_x, g, s);
     }
   }
 

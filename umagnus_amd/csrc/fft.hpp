@@ -14,10 +14,29 @@ namespace um {
 constexpr int FFT_NT = 256;      // threads per FFT work-group
 constexpr int FFT_MAXPASS = 12;
 
+// Unsigned division by a run-time constant with a multiply-high (round-up
+// method of Granlund & Montgomery); exact for numerators below 2^31.
+struct FastDiv {
+  uint32_t d = 1, m = 1, s = 0;
+  FastDiv() = default;
+  explicit FastDiv(uint32_t dd) { init(dd); }
+  __host__ void init(uint32_t dd) {
+    d = dd;
+    s = 0;
+    while ((1u << s) < d) s++;
+    m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << s) - d)) / d + 1);
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t x) const { return (__umulhi(x, m) + x) >> s; }
+  __device__ __forceinline__ uint32_t mod(uint32_t x) const { return x - div(x) * d; }
+};
+
 struct FftLen {
   int n;                         // transform length
   int np;                        // number of radix passes
   int rad[FFT_MAXPASS];          // radices in pass order
+  FastDiv fp[FFT_MAXPASS];       // p = product of the radices before pass s
+  FastDiv fnr[FFT_MAXPASS];      // n / rad[s]
+  FastDiv fn;                    // n
   const void *tw;                // exp(-2 pi i m / n), m in [0, n), precision T
 };
 

@@ -155,8 +155,8 @@ __device__ __forceinline__ void dft(C *a) {
 // butterflies of one line (line-major LDS, es=1); both keep LDS access unit-stride
 // across a wavefront when loading.
 template <typename T, int R, int DIR, int MAPEM>
-__device__ __forceinline__ void fft_pass(C2<T> *lds, int nl, int n, int p, int ls, int es,
-                                         const C2<T> *__restrict__ tw) {
+__device__ __forceinline__ void fft_pass(C2<T> *lds, int nl, const FastDiv &fnl, int n, int p, const FastDiv &fp,
+                                         const FastDiv &fnR, int ls, int es, const C2<T> *__restrict__ tw) {
   constexpr int EPT = FftEpt<T>::value;
   constexpr int S = (EPT + R - 1) / R;
   const int tid = threadIdx.x;
@@ -171,8 +171,8 @@ __device__ __forceinline__ void fft_pass(C2<T> *lds, int nl, int n, int p, int l
     li[s] = -1;
     if (g < total) {
       int l, i;
-      if (MAPEM) { l = g % nl; i = g / nl; }
-      else { l = g / nR; i = g - l * nR; }
+      if (MAPEM) { i = (int)fnl.div(g); l = g - i * nl; }
+      else { l = (int)fnR.div(g); i = g - l * nR; }
       li[s] = l;
       ii[s] = i;
 #pragma unroll
@@ -184,7 +184,7 @@ __device__ __forceinline__ void fft_pass(C2<T> *lds, int nl, int n, int p, int l
   for (int s = 0; s < S; s++) {
     if (li[s] >= 0) {
       const int l = li[s], i = ii[s];
-      const int k = i % p;
+      const int k = (int)fp.mod(i);
       if (k != 0) {
 #pragma unroll
         for (int j = 1; j < R; j++) {
@@ -203,19 +203,19 @@ __device__ __forceinline__ void fft_pass(C2<T> *lds, int nl, int n, int p, int l
 }
 
 template <typename T, int DIR, int MAPEM>
-__device__ __forceinline__ void lds_fft(C2<T> *lds, int nl, int ls, int es, const FftLen &f) {
+__device__ __forceinline__ void lds_fft(C2<T> *lds, int nl, const FastDiv &fnl, int ls, int es, const FftLen &f) {
   const C2<T> *tw = (const C2<T> *)f.tw;
   int p = 1;
   for (int s = 0; s < f.np; s++) {
     const int R = f.rad[s];
     switch (R) {
-      case 2: fft_pass<T, 2, DIR, MAPEM>(lds, nl, f.n, p, ls, es, tw); break;
-      case 3: fft_pass<T, 3, DIR, MAPEM>(lds, nl, f.n, p, ls, es, tw); break;
-      case 4: fft_pass<T, 4, DIR, MAPEM>(lds, nl, f.n, p, ls, es, tw); break;
-      case 5: fft_pass<T, 5, DIR, MAPEM>(lds, nl, f.n, p, ls, es, tw); break;
-      case 7: fft_pass<T, 7, DIR, MAPEM>(lds, nl, f.n, p, ls, es, tw); break;
-      case 8: fft_pass<T, 8, DIR, MAPEM>(lds, nl, f.n, p, ls, es, tw); break;
-      default: fft_pass<T, 16, DIR, MAPEM>(lds, nl, f.n, p, ls, es, tw); break;
+      case 2: fft_pass<T, 2, DIR, MAPEM>(lds, nl, fnl, f.n, p, f.fp[s], f.fnr[s], ls, es, tw); break;
+      case 3: fft_pass<T, 3, DIR, MAPEM>(lds, nl, fnl, f.n, p, f.fp[s], f.fnr[s], ls, es, tw); break;
+      case 4: fft_pass<T, 4, DIR, MAPEM>(lds, nl, fnl, f.n, p, f.fp[s], f.fnr[s], ls, es, tw); break;
+      case 5: fft_pass<T, 5, DIR, MAPEM>(lds, nl, fnl, f.n, p, f.fp[s], f.fnr[s], ls, es, tw); break;
+      case 7: fft_pass<T, 7, DIR, MAPEM>(lds, nl, fnl, f.n, p, f.fp[s], f.fnr[s], ls, es, tw); break;
+      case 8: fft_pass<T, 8, DIR, MAPEM>(lds, nl, fnl, f.n, p, f.fp[s], f.fnr[s], ls, es, tw); break;
+      default: fft_pass<T, 16, DIR, MAPEM>(lds, nl, fnl, f.n, p, f.fp[s], f.fnr[s], ls, es, tw); break;
     }
     p *= R;
   }

@@ -53,9 +53,13 @@ struct Sim : SimBase {
   long nsteps = 0, nundone = 0, nevals = 0;
   bool k1_valid = false;
 
-  // profiling (HIP events around the demag and field kernels)
+  // profiling: HIP events around the demag and field kernels of every
+  // evaluation, recorded without synchronising and read after the step's
+  // own readback has drained the stream
   int profiling = 0;
-  hipEvent_t ev[4] = {};
+  std::vector<hipEvent_t> evpool;
+  int evused = 0;
+  hipEvent_t step_ev[2] = {};
   double t_demag = 0, t_field = 0, t_other = 0;
   int prof_evals = 0;
 
@@ -82,7 +86,7 @@ struct Sim : SimBase {
     dLUT.alloc(sizeof(T) * SYMLUT);
     fill_bytes(aLUT.p, aLUT.bytes, st);
     fill_bytes(dLUT.p, dLUT.bytes, st);
-    for (auto &e : ev) UM_HIP(hipEventCreate(&e));
+    for (auto &e : step_ev) UM_HIP(hipEventCreate(&e));
     UM_HIP(hipStreamSynchronize(st));
   }
   ~Sim() override {
@@ -90,7 +94,8 @@ struct Sim : SimBase {
     if (demag) demag_destroy(demag);
     if (hres) (void)hipHostFree(hres);
     if (hsum) (void)hipHostFree(hsum);
-    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : evpool) (void)hipEventDestroy(e);
+    for (auto &e : step_ev) if (e) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
   }
   hipStream_t stream() override { return st; }
@@ -255,10 +260,37 @@ struct Sim : SimBase {
     demag_exec(demag, B, mv, vol.p, p[UM_P_MSAT][0].arr.p, p[UM_P_MSAT][0].mul, st);
   }
   // torqueFn (engine/run.go:127-130): SetTorque into dst
+  hipEvent_t next_event() {
+    if (evused == (int)evpool.size()) {
+      hipEvent_t e;
+      UM_HIP(hipEventCreate(&e));
+      evpool.push_back(e);
+    }
+    return evpool[evused++];
+  }
+  // read the events of the evaluations enqueued so far (stream must be drained)
+  void prof_collect() {
+    if (!profiling) return;
+    for (int i = 0; i + 2 < evused; i += 3) {
+      float d1 = 0, d2 = 0;
+      UM_HIP(hipEventElapsedTime(&d1, evpool[i], evpool[i + 1]));
+      UM_HIP(hipEventElapsedTime(&d2, evpool[i + 1], evpool[i + 2]));
+      t_demag += d1;
+      t_field += d2;
+      prof_evals++;
+    }
+    evused = 0;
+  }
   void torque_fn(T *dst) {
-    if (profiling) UM_HIP(hipEventRecord(ev[0], st));
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (profiling) {
+      e0 = next_event();
+      e1 = next_event();
+      e2 = next_event();
+      UM_HIP(hipEventRecord(e0, st));
+    }
     compute_demag(m);
-    if (profiling) UM_HIP(hipEventRecord(ev[1], st));
+    if (profiling) UM_HIP(hipEventRecord(e1, st));
     FieldArgs<T> a = field_args(m);
     for (int c = 0; c < 3; c++) {
       a.bd[c] = enable_demag ? bd + c * n : nullptr;
@@ -266,16 +298,7 @@ struct Sim : SimBase {
       a.beff[c] = nullptr;
     }
     launch_field_torque(a, st);
-    if (profiling) {
-      UM_HIP(hipEventRecord(ev[2], st));
-      UM_HIP(hipEventSynchronize(ev[2]));
-      float d1 = 0, d2 = 0;
-      UM_HIP(hipEventElapsedTime(&d1, ev[0], ev[1]));
-      UM_HIP(hipEventElapsedTime(&d2, ev[1], ev[2]));
-      t_demag += d1;
-      t_field += d2;
-      prof_evals++;
-    }
+    if (profiling) UM_HIP(hipEventRecord(e2, st));
     nevals++;
   }
   void beff(void *const out[3]) override {
@@ -293,6 +316,7 @@ struct Sim : SimBase {
     torque_fn(spare);
     nevals--;
     copy_out(spare, out);
+    prof_collect();
   }
   void demag_field(void *const out[3]) override {
     if (enable_demag) {
@@ -553,10 +577,19 @@ struct Sim : SimBase {
   }
 
   void step() override {
+    if (profiling) UM_HIP(hipEventRecord(step_ev[0], st));
     switch (solver) {
       case UM_SOLVER_HEUN: step_heun(); break;
       case UM_SOLVER_RK4: step_rk4(); break;
       default: step_rk45dp(); break;
+    }
+    if (profiling) {
+      UM_HIP(hipEventRecord(step_ev[1], st));
+      UM_HIP(hipEventSynchronize(step_ev[1]));
+      float d = 0;
+      UM_HIP(hipEventElapsedTime(&d, step_ev[0], step_ev[1]));
+      t_other += d;  // whole step; the stage/reduction share is this minus the evaluations
+      prof_collect();
     }
   }
   // Steps(n), engine/run.go:197-200; RunWhile drops the FSAL torque (:207)
@@ -639,7 +672,9 @@ struct Sim : SimBase {
   }
 
   void set_profiling(int on) override {
+    UM_HIP(hipStreamSynchronize(st));
     profiling = on;
+    evused = 0;
     t_demag = t_field = t_other = 0;
     prof_evals = 0;
   }
