@@ -342,8 +342,11 @@ __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
     for (int g = threadIdx.x; g < CN; g += FFT_NT) twl[g] = tw[g];
   }
   const int kx0 = blockIdx.x * B;
-  const int ky = blockIdx.y;  // 3D only
-  const long off = (long)ky * a.pstride;
+  // 3D: this work-group handles ky and its mirror Py-ky, which use the same
+  // tensor values (kernmulrsymm3d.cl:62-66 flips the yz and xy signs); 2D: one tile
+  const int kya = blockIdx.y;
+  const int kyb = a.Py - kya;
+  const int ntiles = (!a.is2d && kya != 0 && kyb != kya) ? 2 : 1;
   const int nh = n / 2 + 1;   // unique points along the axis: k and n-k share |K|
   constexpr int QK = CN > 0 ? (CB * (CN / 2 + 1) + FFT_NT - 1) / FFT_NT : CONV_QK;
   // 1. tensor prefetch (independent of the data, so it overlaps the loads below)
@@ -358,12 +361,9 @@ __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
       const int kx = kx0 + b;
       if (kx < a.H) {
         if (!a.is2d) {
-          int jy = ky;
-          T sYZ = T(1), sXY = T(1);
-          if (jy > a.Py / 2) { jy = a.Py - jy; sYZ = -sYZ; sXY = -sXY; }
-          const long I = ((long)e * a.KHy + jy) * a.H + kx;
+          const long I = ((long)e * a.KHy + kya) * a.H + kx;
           kv[q][0] = a.K[0][I]; kv[q][1] = a.K[1][I]; kv[q][2] = a.K[2][I];
-          kv[q][3] = a.K[3][I] * sYZ; kv[q][4] = a.K[4][I]; kv[q][5] = a.K[5][I] * sXY;
+          kv[q][3] = a.K[3][I]; kv[q][4] = a.K[4][I]; kv[q][5] = a.K[5][I];
         } else {
           const long I = (long)e * a.H + kx;
           kv[q][0] = a.K[0][I]; kv[q][1] = a.K[1][I]; kv[q][2] = a.K[2][I]; kv[q][5] = a.K[5][I];
@@ -371,28 +371,23 @@ __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
       }
     }
   }
-  // 2. spectrum loads
+  // 2. spectrum loads of a tile
   constexpr int SL = CN > 0 ? (3 * CB * CN + FFT_NT - 1) / FFT_NT : EPT;
   C2<T> v[SL];
+  auto load = [&](int ky) {
+    const long off = (long)ky * a.pstride;
 #pragma unroll
-  for (int s = 0; s < SL; s++) {
-    const int g = threadIdx.x + s * FFT_NT;
-    v[s] = C2<T>{T(0), T(0)};
-    if (g < total) {
-      const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
-      const int c = cdiv<CB>(l, a.fB), b = l - c * B;
-      const int kx = kx0 + b;
-      if (kx < a.H && e < a.nvalid) v[s] = a.buf[c][off + (long)e * a.estride + kx];
+    for (int s = 0; s < SL; s++) {
+      const int g = threadIdx.x + s * FFT_NT;
+      v[s] = C2<T>{T(0), T(0)};
+      if (g < total) {
+        const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
+        const int c = cdiv<CB>(l, a.fB), b = l - c * B;
+        const int kx = kx0 + b;
+        if (kx < a.H && e < a.nvalid) v[s] = a.buf[c][off + (long)e * a.estride + kx];
+      }
     }
-  }
-#pragma unroll
-  for (int s = 0; s < SL; s++) {
-    const int g = threadIdx.x + s * FFT_NT;
-    if (g < total) lds[g] = v[s];
-  }
-  __syncthreads();
-  run_fft<T, CN, 3 * CB, -1, 1, 1>(lds, twl, nl, a.f3B, a.f);
-  // 3. tensor multiply at k and at its mirror n-k (signs of the odd components flip)
+  };
   auto mul = [&](C2<T> *p, T xx, T yy, T zz, T yz, T xz, T xy) {
     const C2<T> M0 = p[0], M1 = p[B], M2 = p[2 * B];
     if (!a.is2d) {
@@ -405,34 +400,50 @@ __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
       p[2 * B] = C2<T>{M2.x * zz, M2.y * zz};
     }
   };
+  for (int t = 0; t < ntiles; t++) {
+    const int ky = (t == 0) ? kya : kyb;
+    load(ky);
 #pragma unroll
-  for (int q = 0; q < QK; q++) {
-    const int g = threadIdx.x + q * FFT_NT;
-    if (g < B * nh) {
-      const int e = cdiv<CB>(g, a.fB), b = g - e * B;
-      if (kx0 + b < a.H) {
-        const T *k = kv[q];
-        mul(lds + e * nl + b, k[0], k[1], k[2], k[3], k[4], k[5]);
-        const int e2 = n - e;
-        if (e != 0 && e2 != e) {
-          // 3D mirror in z flips yz and xz (kernmulrsymm3d.cl:67-71); 2D mirror in y flips xy
-          if (!a.is2d) mul(lds + e2 * nl + b, k[0], k[1], k[2], -k[3], -k[4], k[5]);
-          else mul(lds + e2 * nl + b, k[0], k[1], k[2], k[3], k[4], -k[5]);
+    for (int s = 0; s < SL; s++) {
+      const int g = threadIdx.x + s * FFT_NT;
+      if (g < total) lds[g] = v[s];
+    }
+    __syncthreads();
+    run_fft<T, CN, 3 * CB, -1, 1, 1>(lds, twl, nl, a.f3B, a.f);
+    // 3. tensor multiply at k and at its mirror n-k (signs of the odd components flip)
+    const T sy = (t == 0) ? T(1) : T(-1);   // ky mirror: yz and xy change sign
+#pragma unroll
+    for (int q = 0; q < QK; q++) {
+      const int g = threadIdx.x + q * FFT_NT;
+      if (g < B * nh) {
+        const int e = cdiv<CB>(g, a.fB), b = g - e * B;
+        if (kx0 + b < a.H) {
+          const T *k = kv[q];
+          const T yz = sy * k[3], xy = sy * k[5];
+          mul(lds + e * nl + b, k[0], k[1], k[2], yz, k[4], xy);
+          const int e2 = n - e;
+          if (e != 0 && e2 != e) {
+            // 3D mirror in z flips yz and xz (kernmulrsymm3d.cl:67-71); 2D mirror in y flips xy
+            if (!a.is2d) mul(lds + e2 * nl + b, k[0], k[1], k[2], -yz, -k[4], xy);
+            else mul(lds + e2 * nl + b, k[0], k[1], k[2], k[3], k[4], -k[5]);
+          }
         }
       }
     }
-  }
-  __syncthreads();
-  run_fft<T, CN, 3 * CB, +1, 1, 1>(lds, twl, nl, a.f3B, a.f);
+    __syncthreads();
+    run_fft<T, CN, 3 * CB, +1, 1, 1>(lds, twl, nl, a.f3B, a.f);
+    const long off = (long)ky * a.pstride;
 #pragma unroll
-  for (int s = 0; s < SL; s++) {
-    const int g = threadIdx.x + s * FFT_NT;
-    if (g < total) {
-      const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
-      const int c = cdiv<CB>(l, a.fB), b = l - c * B;
-      const int kx = kx0 + b;
-      if (kx < a.H && e < a.nvalid) a.buf[c][off + (long)e * a.estride + kx] = lds[g];
+    for (int s = 0; s < SL; s++) {
+      const int g = threadIdx.x + s * FFT_NT;
+      if (g < total) {
+        const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
+        const int c = cdiv<CB>(l, a.fB), b = l - c * B;
+        const int kx = kx0 + b;
+        if (kx < a.H && e < a.nvalid) a.buf[c][off + (long)e * a.estride + kx] = lds[g];
+      }
     }
+    __syncthreads();
   }
 }
 
@@ -774,6 +785,7 @@ struct DemagPlan : DemagBase {
     if (!nl_x) nl_x = lines_for(P[0] / 2);
     nl_y = CtTable<T>::col(P[1]);
     if (!nl_y) nl_y = std::min(lines_for(P[1]), 64);
+
     const int convlen = is2d ? P[1] : P[2];
     B_conv = is2d ? CtTable<T>::conv2(convlen) : CtTable<T>::conv3(convlen);
     if (!B_conv) {
@@ -882,7 +894,7 @@ struct DemagPlan : DemagBase {
       ca.is2d = 0; ca.Py = P[1]; ca.Pz = P[2]; ca.KHy = KHy;
       for (int c = 0; c < 6; c++) ca.K[c] = K[c];
       for (int c = 0; c < 3; c++) ca.buf[c] = sp[c];
-      dim3 gc((H + B_conv - 1) / B_conv, P[1]);
+      dim3 gc((H + B_conv - 1) / B_conv, P[1] / 2 + 1);   // ky and Py-ky per work-group
       launch_conv<T>(ca, P[2], B_conv, gc, s);
       ya.nvalid = P[1]; ya.nout = Ny;
       launch_col<T, +1>(ya, P[1], nl_y, gy, s);
