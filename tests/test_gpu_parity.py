@@ -80,7 +80,9 @@ def test_demag_golden(gpu, N, cs, expect, tol, prec):
 
 
 CONV_SIZES = [(3, 4, 1), (16, 8, 1), (20, 12, 1), (5, 4, 2), (6, 5, 3), (8, 6, 4), (7, 3, 5), (16, 12, 6),
-              (24, 10, 7), (48, 20, 3), (96, 12, 2), (64, 64, 1), (32, 24, 8)]
+              (24, 10, 7), (48, 20, 3), (96, 12, 2), (64, 64, 1), (32, 24, 8),
+              # register Z pass (Pz = 2 Nz, Nz in 8/16/32): partial kx groups, unpaired ky rows
+              (70, 5, 8), (40, 9, 16), (12, 8, 32)]
 
 
 @pytest.mark.parametrize("N", CONV_SIZES)

@@ -308,6 +308,80 @@ __global__ __launch_bounds__(FFT_NT) void k_colfft(ColArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Register four-step column pass for P = A*A (A in 8, 16, 32): C adjacent kx
+// columns per work-group, A lanes per column.  With n = n1 + A n2 and
+// k = k2 + A k1,
+//   X[k2 + A k1] = sum_n1 W_A^(n1 k1) W_P^(n1 k2) sum_n2 x[n1 + A n2] W_A^(n2 k2):
+// lane n1 transforms its strided A elements in registers, twiddles them,
+// one LDS transpose hands lane k2 the A values over n1, and a second
+// register transform gives the outputs k2 + A k1.  HALF: the forward input is
+// zero for n >= P/2 (n2 >= A/2, never loaded) and the inverse keeps only
+// k < P/2 (k1 < A/2).  One LDS round trip per element instead of one per
+// radix pass.
+// ---------------------------------------------------------------------------
+constexpr int COLY_PAD = 8;   // row padding of the transpose (2-way = minimal b64 bank sharing)
+
+// Work-group index permuted so that the tiles one XCD runs (linear ids w,
+// w+8, w+16, ... are dispatched to the same XCD) are adjacent in memory and
+// share their 128-byte lines in that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int w, int total) {
+  const int q = total >> 3, full = q << 3;
+  return w >= full ? w : (w & 7) * q + (w >> 3);
+}
+
+template <typename T, int A, int DIR, int C, bool HALF>
+__global__ __launch_bounds__(A * C) void k_coly(ColArgs<T> a) {
+  constexpr int P = A * A, S = A * C + COLY_PAD;
+  extern __shared__ __align__(16) unsigned char smem_[];
+  C2<T> *lds = (C2<T> *)smem_;
+  const int ngx = gridDim.x, ngy = gridDim.y;
+  const int w = xcd_tile(blockIdx.x + ngx * (blockIdx.y + ngy * blockIdx.z), ngx * ngy * gridDim.z);
+  const int gx = w % ngx, gyz = w / ngx, gy = gyz % ngy, gz = gyz / ngy;
+  const int col = threadIdx.x % C, i = threadIdx.x / C;
+  const int kx = gx * C + col;
+  const bool valid = kx < a.H;
+  // uniform plane base + one 32-bit lane offset (row i, column kx)
+  C2<T> *ub = a.buf[gz] + (long)gy * a.pstride;
+  const uint32_t loff = (uint32_t)(((long)i * a.estride + min(kx, a.H - 1)) * sizeof(C2<T>));
+  const long rstep = (long)A * a.estride;   // rows n2 -> n2 + 1
+  const C2<T> *tw = (const C2<T> *)a.f.tw;   // W_P^m, m < P
+
+  C2<T> v[A];
+  constexpr int NIN = (HALF && DIR < 0) ? A / 2 : A;
+#pragma unroll
+  for (int n2 = 0; n2 < A; n2++) {
+    v[n2] = C2<T>{T(0), T(0)};
+    if (n2 < NIN) {
+      const auto *q = at_off<const T>(ub + n2 * rstep, loff);
+      v[n2] = C2<T>{q[0], q[1]};
+    }
+  }
+  dft<A, DIR>(v);
+  pin(v);
+#pragma unroll
+  for (int k2 = 1; k2 < A; k2++) {
+    C2<T> t = tw[i * k2];
+    if (DIR > 0) t.y = -t.y;
+    v[k2] = cmul(v[k2], t);
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < A; k2++) lds[k2 * S + threadIdx.x] = v[k2];
+  __syncthreads();
+#pragma unroll
+  for (int n1 = 0; n1 < A; n1++) v[n1] = lds[i * S + n1 * C + col];
+  dft<A, DIR>(v);
+  constexpr int NOUT = (HALF && DIR > 0) ? A / 2 : A;
+  if (valid) {
+#pragma unroll
+    for (int k1 = 0; k1 < NOUT; k1++) {
+      auto *q = at_off<T>(ub + k1 * rstep, loff);
+      q[0] = v[k1].x;
+      q[1] = v[k1].y;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Fused convolution pass along the last axis: forward FFT of the three
 // components, symmetric tensor multiply, inverse FFT (kernmulrsymm3d.cl:27-91,
 // kernmulrsymm2dxy.cl / 2dz.cl for 2D).
@@ -445,6 +519,198 @@ __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
     }
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// Register Z pass (3D with Pz = 2 Nz, Nz in {8, 16, 32}): one lane per
+// (component, kx, ky) column.  The Nz non-zero inputs give the even and odd
+// halves of the Pz-point spectrum,
+//   X[2m] = DFT_Nz(x)[m],   X[2m+1] = DFT_Nz(x W_Pz^n)[m],
+// both transformed in registers with compile-time twiddles.  The three
+// components of a column meet in LDS only for the tensor multiply, FR
+// frequencies per round; the inverse runs in registers again,
+//   y[n] = IDFT_Nz(Y_even)[n] + W_Pz^-n IDFT_Nz(Y_odd)[n],   n < Nz,
+// and only the Nz planes that hold data are stored.  A work-group is three
+// waves, one per component; lanes 0-31 take ky and lanes 32-63 its mirror
+// Py-ky, which read the same tensor values (kernmulrsymm3d.cl:62-66).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ConvZArgs {
+  int H;           // valid kx columns
+  int Py, KHy;     // padded y size and Py/2+1
+  long zstride;    // complex elements between z planes of a component
+  long ystride;    // complex elements between ky rows
+  const T *K[6];   // xx yy zz yz xz xy, each H x KHy x (Nz+1)
+  C2<T> *buf[3];
+};
+
+constexpr int CONVZ_NT = 192;
+constexpr int CONVZ_COLS = 32;
+
+// exp(dir * 2 pi i j / n); constant once the caller's loops are unrolled
+template <typename T>
+__device__ __forceinline__ C2<T> cexpi(int j, int n, int dir) {
+  const double ang = 2.0 * M_PI * (double)j / (double)n;
+  return C2<T>{T(__builtin_cos(ang)), T(dir * __builtin_sin(ang))};
+}
+
+// Visiting order of the Pz frequencies: 0, Nz, then the mirror pairs (p, Pz-p)
+// next to each other so a pair shares its tensor loads.
+template <int NZ>
+__device__ __forceinline__ constexpr int zorder(int i) {
+  return i == 0 ? 0 : i == 1 ? NZ : (i & 1) ? 2 * NZ - i / 2 : i / 2;
+}
+
+template <typename T, int NZ, int FR, int G>
+__global__ __launch_bounds__(CONVZ_NT) void k_convz(ConvZArgs<T> a) {
+  constexpr int PZ = 2 * NZ, KZ = NZ + 1;
+  static_assert(PZ % FR == 0 && FR % G == 0, "rounds must tile the spectrum");
+  __shared__ C2<T> xch[FR * 3 * 64];             // exchange: [freq][component][lane]
+  __shared__ T tens[KZ * 6 * CONVZ_COLS];        // tensor slab: [kz][element][kx]
+  const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  const int t = l >> 5;
+  const int kx0 = blockIdx.x * CONVZ_COLS;
+  const int kx = kx0 + (l & 31);
+  const int kya = blockIdx.y;
+  const bool mirror = kya != 0 && 2 * kya != a.Py;
+  const bool valid = kx < a.H && (t == 0 || mirror);
+  // every lane loads in-bounds data (clamped column / row); only valid lanes store
+  const int kxc = min(kx, a.H - 1);
+  const int ky = (t && mirror) ? a.Py - kya : kya;
+  // Addresses are a wave-uniform base (plane n, scalar registers) plus one
+  // per-lane 32-bit byte offset shared by every load and store of the lane.
+  C2<T> *plane0 = a.buf[c];
+  const uint32_t loff = (uint32_t)(((long)ky * a.ystride + kxc) * sizeof(C2<T>));
+
+  // 0. tensor slab of this work-group (32 kx x 6 elements x Nz+1 kz) into LDS
+  // (fixed trip count, unrolled: every load of the slab is in flight at once)
+  {
+    constexpr int TS = KZ * 6 * CONVZ_COLS, TQ = (TS + CONVZ_NT - 1) / CONVZ_NT;
+    const long kzs = (long)a.KHy * a.H;
+    const int x = min(kx0 + (int)(threadIdx.x & (CONVZ_COLS - 1)), a.H - 1);
+    T tv[TQ];
+#pragma unroll
+    for (int q = 0; q < TQ; q++) {
+      const int g = threadIdx.x + q * CONVZ_NT;
+      if (TS % CONVZ_NT == 0 || g < TS) {
+        const int ez = g / CONVZ_COLS, e = ez % 6, kz = ez / 6;
+        tv[q] = a.K[e][kz * kzs + (long)kya * a.H + x];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < TQ; q++) {
+      const int g = threadIdx.x + q * CONVZ_NT;
+      if (TS % CONVZ_NT == 0 || g < TS) tens[g] = tv[q];
+    }
+  }
+
+  // 1. forward: even and odd halves of the spectrum
+  C2<T> E[NZ], O[NZ];
+#pragma unroll
+  for (int n = 0; n < NZ; n++) {
+    const auto *p = at_off<const T>(plane0 + n * a.zstride, loff);
+    E[n] = C2<T>{p[0], p[1]};
+  }
+#pragma unroll
+  for (int n = 0; n < NZ; n++) O[n] = n == 0 ? E[0] : cmul(E[n], cexpi<T>(n, PZ, -1));
+  // one transform at a time: interleaving the two raises the register peak
+  dft<NZ, -1>(E);
+  pin(E);
+  dft<NZ, -1>(O);
+  pin(O);
+
+  // 2. tensor multiply: component c of K . (X0, X1, X2) for each frequency.
+  // Row c of the symmetric tensor: x: (xx, xy, xz), y: (xy, yy, yz), z: (xz, yz, zz).
+  const int e0 = c == 0 ? 0 : c == 1 ? 5 : 4;
+  const int e1 = c == 0 ? 5 : c == 1 ? 1 : 3;
+  const int e2 = c == 0 ? 4 : c == 1 ? 3 : 2;
+  // the ky mirror flips xy and yz; the kz mirror flips xz and yz
+  const T sy = t ? T(-1) : T(1);
+  auto fy = [&](int e) { return (e == 5 || e == 3) ? sy : T(1); };
+  auto fz = [&](int e) { return (e == 4 || e == 3) ? T(-1) : T(1); };
+  const T y0 = fy(e0), y1 = fy(e1), y2 = fy(e2);
+  const T z0 = y0 * fz(e0), z1 = y1 * fz(e1), z2 = y2 * fz(e2);
+  const T *tl = tens + (l & (CONVZ_COLS - 1));
+  C2<T> *xw = xch + c * 64 + l;
+#pragma unroll
+  for (int r = 0; r < PZ / FR; r++) {
+#pragma unroll
+    for (int j = 0; j < FR; j++) {
+      const int k = zorder<NZ>(r * FR + j);
+      xw[j * 192] = (k & 1) ? O[k >> 1] : E[k >> 1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j0 = 0; j0 < FR; j0 += G) {
+      // groups of G frequencies: bounded look-ahead keeps the live registers low
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = j0; j < j0 + G; j++) {
+        const int k = zorder<NZ>(r * FR + j);
+        const int kz = k <= NZ ? k : PZ - k;
+        const C2<T> X0 = xch[(j * 3 + 0) * 64 + l];
+        const C2<T> X1 = xch[(j * 3 + 1) * 64 + l];
+        const C2<T> X2 = xch[(j * 3 + 2) * 64 + l];
+        const T *tk = tl + kz * 6 * CONVZ_COLS;
+        const T k0 = tk[e0 * CONVZ_COLS] * (k <= NZ ? y0 : z0);
+        const T k1 = tk[e1 * CONVZ_COLS] * (k <= NZ ? y1 : z1);
+        const T k2 = tk[e2 * CONVZ_COLS] * (k <= NZ ? y2 : z2);
+        C2<T> Y{X0.x * k0 + X1.x * k1 + X2.x * k2, X0.y * k0 + X1.y * k1 + X2.y * k2};
+        // keep the product here: otherwise it sinks into the guarded store
+        // block below, and all LDS reads of the kernel stay live at once
+        asm volatile("" : "+v"(Y.x), "+v"(Y.y));
+        if (k & 1) O[k >> 1] = Y;
+        else E[k >> 1] = Y;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+  }
+
+  // 3. inverse and store of the Nz data planes
+  dft<NZ, +1>(E);
+  pin(E);
+  dft<NZ, +1>(O);
+  pin(O);
+  if (valid) {
+#pragma unroll
+    for (int n = 0; n < NZ; n++) {
+      const C2<T> o = n == 0 ? O[0] : cmul(O[n], cexpi<T>(n, PZ, +1));
+      const C2<T> y = cadd(E[n], o);
+      auto *p = at_off<T>(plane0 + n * a.zstride, loff);
+      p[0] = y.x;
+      p[1] = y.y;
+    }
+  }
+}
+
+template <typename T> struct ConvZTable;
+template <> struct ConvZTable<float> {
+  static constexpr int FR = 8, G = 4;
+  static bool has(int nz) { return nz == 8 || nz == 16 || nz == 32; }
+};
+template <> struct ConvZTable<double> {
+  static constexpr int FR = 4, G = 2;
+  static bool has(int nz) { return nz == 8 || nz == 16; }
+};
+
+template <typename T>
+static void launch_convz(const ConvZArgs<T> &a, int Nz, hipStream_t s) {
+  constexpr int FR = ConvZTable<T>::FR, G = ConvZTable<T>::G;
+  const dim3 grid((a.H + CONVZ_COLS - 1) / CONVZ_COLS, a.Py / 2 + 1);
+  switch (Nz) {
+    case 8: hipLaunchKernelGGL((k_convz<T, 8, FR, G>), grid, dim3(CONVZ_NT), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_convz<T, 16, FR, G>), grid, dim3(CONVZ_NT), 0, s, a); break;
+    case 32:
+      if constexpr (std::is_same<T, float>::value) {
+        hipLaunchKernelGGL((k_convz<T, 32, FR, G>), grid, dim3(CONVZ_NT), 0, s, a);
+        break;
+      }
+      [[fallthrough]];
+    default: throw Error(UM_EUNSUP, "register Z pass: unsupported Nz");
+  }
+  UM_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -713,6 +979,38 @@ static void launch_col(const ColArgs<T> &a, int n, int nl, dim3 grid, hipStream_
   hipLaunchKernelGGL((k_colfft<T, DIR, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
   UM_HIP(hipGetLastError());
 }
+// Register four-step Y pass when P = A*A (A = 8, 16, 32 in FP32; 8, 16 in FP64)
+// and the column is either half zero-padded or full; false: use launch_col.
+template <typename T, int DIR>
+static bool launch_coly(const ColArgs<T> &a, int P, int nz, hipStream_t s) {
+  const int nin = a.nvalid, nout = a.nout;
+  int half;
+  if (DIR < 0 && nin * 2 == P && nout == P) half = 1;
+  else if (DIR > 0 && nin == P && nout * 2 == P) half = 1;
+  else if (nin == P && nout == P) half = 0;
+  else return false;
+  auto go = [&](auto kern, int A, int C) {
+    const size_t lds = sizeof(C2<T>) * (size_t)A * (A * C + COLY_PAD);
+    allow_lds(kern, lds);
+    const dim3 grid((a.H + C - 1) / C, nz, 3);
+    hipLaunchKernelGGL(kern, grid, dim3(A * C), lds, s, a);
+    UM_HIP(hipGetLastError());
+    return true;
+  };
+#define UM_COLY(A, C)                                                                            \
+  if (P == A * A) return half ? go(k_coly<T, A, DIR, C, true>, A, C) : go(k_coly<T, A, DIR, C, false>, A, C);
+  if constexpr (std::is_same<T, float>::value) {
+    UM_COLY(32, 8)
+    UM_COLY(16, 16)
+    UM_COLY(8, 32)
+  } else {
+    UM_COLY(16, 16)
+    UM_COLY(8, 32)
+  }
+#undef UM_COLY
+  return false;
+}
+
 template <typename T>
 static void launch_conv(const ConvArgs<T> &a, int n, int B, dim3 grid, hipStream_t s) {
   const int nl = B;  // UM_CT_CASE compares (n, nl)
@@ -758,6 +1056,7 @@ struct DemagPlan : DemagBase {
   DevBuf spec[3];               // per component: Nz*Py*Hp complex
   DevBuf kern;                  // 6 x H*KHy*KHz of T
   int nl_x = 1, nl_y = 1, B_conv = 1;
+  bool zreg = false;            // register Z pass (Pz = 2 Nz, Nz in ConvZTable)
   long kcount = 0;
 
   static int lines_for(int n) {
@@ -786,6 +1085,7 @@ struct DemagPlan : DemagBase {
     nl_y = CtTable<T>::col(P[1]);
     if (!nl_y) nl_y = std::min(lines_for(P[1]), 64);
 
+    zreg = !is2d && P[2] == 2 * N[2] && ConvZTable<T>::has(N[2]);
     const int convlen = is2d ? P[1] : P[2];
     B_conv = is2d ? CtTable<T>::conv2(convlen) : CtTable<T>::conv3(convlen);
     if (!B_conv) {
@@ -887,17 +1187,25 @@ struct DemagPlan : DemagBase {
       for (int c = 0; c < 3; c++) ya.buf[c] = sp[c];
       dim3 gy((H + nl_y - 1) / nl_y, Nz, 3);
       ya.nvalid = Ny; ya.nout = P[1];
-      launch_col<T, -1>(ya, P[1], nl_y, gy, s);
-      ConvArgs<T> ca{};
-      ca.f = tz.len; ca.B = B_conv; ca.fB.init(B_conv); ca.f3B.init(3 * B_conv); ca.H = H; ca.nvalid = Nz;
-      ca.estride = (long)P[1] * Hp; ca.pstride = Hp;
-      ca.is2d = 0; ca.Py = P[1]; ca.Pz = P[2]; ca.KHy = KHy;
-      for (int c = 0; c < 6; c++) ca.K[c] = K[c];
-      for (int c = 0; c < 3; c++) ca.buf[c] = sp[c];
-      dim3 gc((H + B_conv - 1) / B_conv, P[1] / 2 + 1);   // ky and Py-ky per work-group
-      launch_conv<T>(ca, P[2], B_conv, gc, s);
+      if (!launch_coly<T, -1>(ya, P[1], Nz, s)) launch_col<T, -1>(ya, P[1], nl_y, gy, s);
+      if (zreg) {
+        ConvZArgs<T> za{};
+        za.H = H; za.Py = P[1]; za.KHy = KHy; za.zstride = (long)P[1] * Hp; za.ystride = Hp;
+        for (int c = 0; c < 6; c++) za.K[c] = K[c];
+        for (int c = 0; c < 3; c++) za.buf[c] = sp[c];
+        launch_convz<T>(za, Nz, s);
+      } else {
+        ConvArgs<T> ca{};
+        ca.f = tz.len; ca.B = B_conv; ca.fB.init(B_conv); ca.f3B.init(3 * B_conv); ca.H = H; ca.nvalid = Nz;
+        ca.estride = (long)P[1] * Hp; ca.pstride = Hp;
+        ca.is2d = 0; ca.Py = P[1]; ca.Pz = P[2]; ca.KHy = KHy;
+        for (int c = 0; c < 6; c++) ca.K[c] = K[c];
+        for (int c = 0; c < 3; c++) ca.buf[c] = sp[c];
+        dim3 gc((H + B_conv - 1) / B_conv, P[1] / 2 + 1);   // ky and Py-ky per work-group
+        launch_conv<T>(ca, P[2], B_conv, gc, s);
+      }
       ya.nvalid = P[1]; ya.nout = Ny;
-      launch_col<T, +1>(ya, P[1], nl_y, gy, s);
+      if (!launch_coly<T, +1>(ya, P[1], Nz, s)) launch_col<T, +1>(ya, P[1], nl_y, gy, s);
     } else {
       ConvArgs<T> ca{};
       ca.f = ty.len; ca.B = B_conv; ca.fB.init(B_conv); ca.f3B.init(3 * B_conv); ca.H = H; ca.nvalid = Ny;

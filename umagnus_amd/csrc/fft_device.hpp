@@ -81,6 +81,27 @@ __device__ __forceinline__ void dft16(C *a) {
     a[k + 8] = csub(e[k], o[k]);
   }
 }
+template <int DIR, typename C>
+__device__ __forceinline__ void dft32(C *a) {
+  using S = decltype(a[0].x);
+  C e[16], o[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) { e[j] = a[2 * j]; o[j] = a[2 * j + 1]; }
+  dft16<DIR>(e);
+  dft16<DIR>(o);
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    if (k == 8) {
+      o[k] = crot<DIR>(o[k]);
+    } else if (k > 0) {
+      // W_32^k, folded to constants once the loop is unrolled
+      const double ang = 2.0 * M_PI * k / 32.0;
+      o[k] = cmul_cs<DIR>(o[k], S(__builtin_cos(ang)), S(__builtin_sin(ang)));
+    }
+    a[k] = cadd(e[k], o[k]);
+    a[k + 16] = csub(e[k], o[k]);
+  }
+}
 // cos / sin of 2 pi m / R for the odd radices
 template <int R> struct OddTw;
 template <> struct OddTw<3> {
@@ -146,7 +167,33 @@ __device__ __forceinline__ void dft(C *a) {
   else if constexpr (R == 4) dft4<DIR>(a[0], a[1], a[2], a[3]);
   else if constexpr (R == 8) dft8<DIR>(a);
   else if constexpr (R == 16) dft16<DIR>(a);
+  else if constexpr (R == 32) dft32<DIR>(a);
   else dft_odd<R, DIR>(a);
+}
+
+// ---- register-level helpers ------------------------------------------------
+// base + byte offset: a uniform base and a zero-extended 32-bit lane offset
+// map onto the scalar-base + vector-offset form of the global loads
+// (the base passes through readfirstlane so that it stays in scalar registers
+// instead of being re-associated with the lane offset into 64-bit VGPR pairs)
+template <typename V, typename B>
+__device__ __forceinline__ __attribute__((address_space(1))) V *at_off(B *base, uint32_t off) {
+  using Ch = typename std::conditional<std::is_const<V>::value, const char, char>::type;
+  using GCh = __attribute__((address_space(1))) Ch;
+  const uint64_t u = (uint64_t)base;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (__attribute__((address_space(1))) V *)((GCh *)(((uint64_t)hi << 32) | lo) + off);
+}
+
+// Materialise an unrolled array at this point: its computation may neither
+// sink past the call (e.g. into a guarded store block) nor interleave with
+// the code after it, which bounds the register peak of long unrolled FFTs.
+template <typename CT, int N>
+__device__ __forceinline__ void pin(CT (&v)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i++) asm volatile("" : "+v"(v[i].x), "+v"(v[i].y));
+  __builtin_amdgcn_sched_barrier(0);
 }
 
 // ---- one Stockham pass over nl lines in LDS ---------------------------------
