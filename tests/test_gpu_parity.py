@@ -304,6 +304,27 @@ def test_sim_beff_regions_cubic_thermal(gpu, oracle, prec):
     assert relerr(sg.torque(), so.torque()) <= tol
 
 
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("N,openbc", [((24, 12, 4), 1), ((16, 8, 1), 0), ((12, 20, 3), 0), ((8, 6, 2), 1)])
+def test_sim_beff_dmi_regions_vacuum(gpu, oracle, prec, N, openbc):
+    """Exchange + interfacial DMI with a region map (per-pair A and D), vacuum cells
+    (m = 0: the free-boundary branches of dmi.cl) and open / closed edges."""
+    _say("dmi regions", N, openbc, prec)
+    cs = (2e-9, 2.5e-9, 1.5e-9)
+    rng = np.random.default_rng(N[0] * 7 + openbc)
+    shape = (N[2], N[1], N[0])
+    regions = rng.integers(0, 3, shape).astype(np.uint8)
+    so, sg = _both(oracle, gpu, N, cs, prec, Ms=9e5, Aex={0: 1e-11, 1: 2e-11, 2: 1.5e-11},
+                   Dind={0: 2e-3, 1: -1e-3, 2: 3e-3}, Ku1=5e5, alpha=0.1, regions=regions, openbc=openbc)
+    m = random_unit(shape, N[1])
+    m[:, rng.random(shape) < 0.15] = 0.0
+    so.set_m(m)
+    sg.set_m(m)
+    tol = 1e-12 if prec == 1 else 1e-5
+    assert relerr(sg.beff(), so.beff()) <= tol
+    assert relerr(sg.torque(), so.torque()) <= tol
+
+
 # ---- steppers ---------------------------------------------------------------------------------
 def _macrospin(mod, solver, fixdt, t_end, dtype, alpha=0.1, Bz=0.1):
     s = mod.Sim((1, 1, 1), (1e-9, 1e-9, 1e-9), dtype=dtype)

@@ -189,8 +189,58 @@ __global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
   }
 }
 
+// four x cells per thread; consecutive threads take consecutive x groups
+template <typename T, bool REG, int EX>
+__global__ __launch_bounds__(256) void k_field_torque4(FieldArgs<T> a) {
+  const int ngx = a.g.Nx >> 2;
+  const long gid = blockIdx.x * 256L + threadIdx.x;
+  const long rows = (long)a.g.Ny * a.g.Nz;
+  if (gid >= rows * ngx) return;
+  const long row = gid / ngx;
+  const int gx = (int)(gid - row * ngx);
+  const int iy = (int)(row % a.g.Ny), iz = (int)(row / a.g.Ny);
+  const long I0 = a.g.idx(gx * 4, iy, iz);
+  T bdv[3][4];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (a.bd[0]) ld4(a.bd[c] + I0, bdv[c]);
+    else bdv[c][0] = bdv[c][1] = bdv[c][2] = bdv[c][3] = T(0);
+  }
+  field_group4<T, REG, EX>(a, gx * 4, iy, iz, bdv);
+}
+
+// whether the four-cell kernel applies: FP32 (in FP64 the four cells need
+// twice the registers and the one-cell kernel keeps a higher occupancy),
+// Nx % 4 == 0 and every per-cell array aligned to four cells
+template <typename T>
+static bool field4_ok(const FieldArgs<T> &a) {
+  if (sizeof(T) != 4 || a.g.Nx % 4) return false;
+  auto al = [](const void *p) { return ((uintptr_t)p & (4 * sizeof(T) - 1)) == 0; };
+  for (int c = 0; c < 3; c++) {
+    if (!al(a.m[c])) return false;
+    if (a.bd[0] && !al(a.bd[c])) return false;
+    if (a.tau[0] && !al(a.tau[c])) return false;
+    if (a.beff[0] && !al(a.beff[c])) return false;
+  }
+  return !a.regions || ((uintptr_t)a.regions & 3) == 0;
+}
+
 template <typename T>
 void launch_field_torque(const FieldArgs<T> &a, hipStream_t s) {
+  if (field4_ok(a)) {
+    const long n4 = (long)a.g.Nx / 4 * a.g.Ny * a.g.Nz;
+    const dim3 grid((unsigned)((n4 + 255) / 256));
+    const bool reg = a.regions != nullptr;
+#define UM_F4(R, E) hipLaunchKernelGGL((k_field_torque4<T, R, E>), grid, dim3(256), 0, s, a)
+    if constexpr (sizeof(T) == 4) {
+      if (a.exch_mode == 0) { if (reg) UM_F4(true, 0); else UM_F4(false, 0); }
+      else if (a.exch_mode == 1) { if (reg) UM_F4(true, 1); else UM_F4(false, 1); }
+      else { if (reg) UM_F4(true, 2); else UM_F4(false, 2); }
+    }
+#undef UM_F4
+    UM_HIP(hipGetLastError());
+    return;
+  }
   dim3 blk(64, 4, 1);
   dim3 grd((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.g.Nz);
   hipLaunchKernelGGL(k_field_torque<T>, grd, blk, 0, s, a);

@@ -252,6 +252,250 @@ struct FieldArgs {
   const T *frozenLUT;    // FrozenSpins per region or nullptr
 };
 
+// ---- four consecutive x cells per thread ----------------------------------------
+// The same per-cell arithmetic as the one-cell kernels, with the stencil's m
+// values loaded as 16-byte vectors (the x neighbours of the inner cells come
+// from the thread's own registers) and, without a region map, the exchange /
+// DMI lookup tables read once.  Needs Nx % 4 == 0.
+template <typename T>
+__device__ __forceinline__ void ld4(const T *p, T (&v)[4]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 f = *(const float4 *)p;
+    v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+  } else {
+    const double2 f0 = ((const double2 *)p)[0], f1 = ((const double2 *)p)[1];
+    v[0] = f0.x; v[1] = f0.y; v[2] = f1.x; v[3] = f1.y;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void st4(T *p, const T (&v)[4]) {
+  if constexpr (sizeof(T) == 4) {
+    *(float4 *)p = float4{v[0], v[1], v[2], v[3]};
+  } else {
+    ((double2 *)p)[0] = double2{v[0], v[1]};
+    ((double2 *)p)[1] = double2{v[2], v[3]};
+  }
+}
+__device__ __forceinline__ void ld4u8(const uint8_t *p, int (&r)[4]) {
+  const uint32_t u = *(const uint32_t *)p;
+  r[0] = u & 255; r[1] = (u >> 8) & 255; r[2] = (u >> 16) & 255; r[3] = u >> 24;
+}
+
+// exchange from gathered neighbours (order x-1, x+1, y-1, y+1, z-1, z+1), exchange.cl:5-87
+template <typename T, bool REG>
+__device__ __forceinline__ V3<T> exchange_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb, int r0, const int (&rn)[6],
+                                             const T *aLUT, T A0, T wx, T wy, T wz) {
+  V3<T> B = v3(T(0), T(0), T(0));
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    if (j >= nnb) break;
+    V3<T> n = mn[j];
+    if (is0(n)) n = m0;
+    const T aj = REG ? aLUT[symidx(r0, rn[j])] : A0;
+    const T w = j < 2 ? wx : j < 4 ? wy : wz;
+    const T s = w * aj;
+    B.x += s * (n.x - m0.x);
+    B.y += s * (n.y - m0.y);
+    B.z += s * (n.z - m0.z);
+  }
+  return B;
+}
+
+// exchange + interfacial DMI from gathered neighbours, dmi.cl:8-148; an in-plane
+// neighbour outside the mesh (inside[j] false) is passed as zero
+template <typename T, bool REG>
+__device__ __forceinline__ V3<T> dmi_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb, int r0, const int (&rn)[6],
+                                        const T *aLUT, const T *dLUT, T A0, T D0, T cx, T cy, T cz, int openbc) {
+  V3<T> h = v3(T(0), T(0), T(0));
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int axis = j >> 1;
+    const T sgn = (j & 1) ? T(1) : T(-1);
+    const T c = axis == 0 ? cx : cy;
+    V3<T> n = mn[j];
+    const int rj = is0(n) ? r0 : rn[j];
+    const T A = REG ? aLUT[symidx(r0, rj)] : A0;
+    const T D = REG ? dLUT[symidx(r0, rj)] : D0;
+    if (!is0(n) || !openbc) {
+      if (is0(n)) {
+        const T f = sgn * c * (T(0.5) * D / A);
+        if (axis == 0) {
+          n.x = m0.x - (f * m0.z);
+          n.y = m0.y;
+          n.z = m0.z + (f * m0.x);
+        } else {
+          n.x = m0.x;
+          n.y = m0.y - (f * m0.z);
+          n.z = m0.z + (f * m0.y);
+        }
+      }
+      const T e = (T(2.0) * A / (c * c));
+      h.x += e * (n.x - m0.x);
+      h.y += e * (n.y - m0.y);
+      h.z += e * (n.z - m0.z);
+      const T dc = D / c;
+      if (axis == 0) {
+        h.x += dc * (sgn * n.z);
+        h.z -= dc * (sgn * n.x);
+      } else {
+        h.y += dc * (sgn * n.z);
+        h.z -= dc * (sgn * n.y);
+      }
+    }
+  }
+  if (nnb == 6) {
+#pragma unroll
+    for (int j = 4; j < 6; j++) {
+      V3<T> n = mn[j];
+      if (is0(n)) n = m0;
+      const T A = REG ? aLUT[symidx(r0, rn[j])] : A0;
+      const T e = (T(2.0) * A / (cz * cz));
+      h.x += e * (n.x - m0.x);
+      h.y += e * (n.y - m0.y);
+      h.z += e * (n.z - m0.z);
+    }
+  }
+  return h;
+}
+
+// Effective field and torque of cells (ix0..ix0+3, iy, iz); bdv: demag field
+// of those cells (zero when absent).  Same term order as k_field_torque.
+template <typename T, bool REG, int EX>
+__device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int iy, int iz, const T (&bdv)[3][4]) {
+  const Mesh &g = a.g;
+  const long I0 = g.idx(ix0, iy, iz);
+  T mc[3][4];
+#pragma unroll
+  for (int c = 0; c < 3; c++) ld4(a.m[c] + I0, mc[c]);
+  int rc[4] = {0, 0, 0, 0};
+  if (REG) ld4u8(a.regions + I0, rc);
+  // stencil neighbours
+  const int nnb = g.Nz != 1 ? 6 : 4;
+  T mxl[3] = {T(0), T(0), T(0)}, mxr[3] = {T(0), T(0), T(0)};
+  T myl[3][4], myh[3][4], mzl[3][4], mzh[3][4];
+  int rxl = 0, rxr = 0, ryl[4] = {0, 0, 0, 0}, ryh[4] = {0, 0, 0, 0}, rzl[4] = {0, 0, 0, 0}, rzh[4] = {0, 0, 0, 0};
+  T A0 = T(0), D0 = T(0);
+  if (EX != 0) {
+    const long iL = g.idx(g.lx(ix0 - 1), iy, iz), iR = g.idx(g.hx(ix0 + 4), iy, iz);
+    const long iYl = g.idx(ix0, g.ly(iy - 1), iz), iYh = g.idx(ix0, g.hy(iy + 1), iz);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      mxl[c] = a.m[c][iL];
+      mxr[c] = a.m[c][iR];
+      ld4(a.m[c] + iYl, myl[c]);
+      ld4(a.m[c] + iYh, myh[c]);
+    }
+    if (REG) {
+      rxl = a.regions[iL];
+      rxr = a.regions[iR];
+      ld4u8(a.regions + iYl, ryl);
+      ld4u8(a.regions + iYh, ryh);
+    }
+    if (nnb == 6) {
+      const long iZl = g.idx(ix0, iy, g.lz(iz - 1)), iZh = g.idx(ix0, iy, g.hz(iz + 1));
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        ld4(a.m[c] + iZl, mzl[c]);
+        ld4(a.m[c] + iZh, mzh[c]);
+      }
+      if (REG) {
+        ld4u8(a.regions + iZl, rzl);
+        ld4u8(a.regions + iZh, rzh);
+      }
+    }
+    if (!REG) {
+      A0 = a.aLUT[0];
+      if (EX == 2) D0 = a.dLUT[0];
+    }
+  }
+  const bool inxl = (ix0 - 1 >= 0) || (g.pbc & 1), inxr = (ix0 + 4 < g.Nx) || (g.pbc & 1);
+  const bool inyl = (iy - 1 >= 0) || (g.pbc & 2), inyh = (iy + 1 < g.Ny) || (g.pbc & 2);
+  T ob[3][4], ot[3][4];
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    const long I = I0 + w;
+    const V3<T> m0 = v3(mc[0][w], mc[1][w], mc[2][w]);
+    V3<T> B = v3(bdv[0][w], bdv[1][w], bdv[2][w]);
+    if (EX != 0 && !is0(m0)) {
+      V3<T> mn[6];
+      int rn[6];
+      mn[0] = w > 0 ? v3(mc[0][w - 1], mc[1][w - 1], mc[2][w - 1]) : v3(mxl[0], mxl[1], mxl[2]);
+      rn[0] = w > 0 ? rc[w - 1] : rxl;
+      mn[1] = w < 3 ? v3(mc[0][w + 1], mc[1][w + 1], mc[2][w + 1]) : v3(mxr[0], mxr[1], mxr[2]);
+      rn[1] = w < 3 ? rc[w + 1] : rxr;
+      mn[2] = v3(myl[0][w], myl[1][w], myl[2][w]);
+      rn[2] = ryl[w];
+      mn[3] = v3(myh[0][w], myh[1][w], myh[2][w]);
+      rn[3] = ryh[w];
+      if (nnb == 6) {
+        mn[4] = v3(mzl[0][w], mzl[1][w], mzl[2][w]);
+        rn[4] = rzl[w];
+        mn[5] = v3(mzh[0][w], mzh[1][w], mzh[2][w]);
+        rn[5] = rzh[w];
+      } else {
+        mn[4] = mn[5] = m0;
+        rn[4] = rn[5] = rc[w];
+      }
+      V3<T> h;
+      if (EX == 1) {
+        h = exchange_nb<T, REG>(m0, mn, nnb, rc[w], rn, a.aLUT, A0, a.wx, a.wy, a.wz);
+      } else {
+        const V3<T> zero = v3(T(0), T(0), T(0));
+        if (w == 0 && !inxl) mn[0] = zero;
+        if (w == 3 && !inxr) mn[1] = zero;
+        if (!inyl) mn[2] = zero;
+        if (!inyh) mn[3] = zero;
+        h = dmi_nb<T, REG>(m0, mn, nnb, rc[w], rn, a.aLUT, a.dLUT, A0, D0, a.cx, a.cy, a.cz, a.openbc);
+      }
+      const T invMs = inv_msat(a.Ms, I);
+      B.x += h.x * invMs;
+      B.y += h.y * invMs;
+      B.z += h.z * invMs;
+    }
+    if (a.has_uniax) {
+      const V3<T> h = uniaxial_cell(I, m0, a.Ms, a.Ku1, a.Ku2, a.u);
+      B.x += h.x; B.y += h.y; B.z += h.z;
+    }
+    if (a.has_cubic) {
+      const V3<T> h = cubic_cell(I, m0, a.Ms, a.Kc1, a.Kc2, a.Kc3, a.c1, a.c2);
+      B.x += h.x; B.y += h.y; B.z += h.z;
+    }
+    if (a.bextLUT[0]) {
+      const int r = rc[w];
+      B.x += a.bextLUT[0][r];
+      B.y += a.bextLUT[1][r];
+      B.z += a.bextLUT[2][r];
+    }
+    if (a.has_bext) {
+      B.x += a.bext[0].at(I);
+      B.y += a.bext[1].at(I);
+      B.z += a.bext[2].at(I);
+    }
+    if (a.has_therm) {
+      const V3<T> h = thermal_cell(I, a.seed, a.step, a.stream, a.kth, a.Ms, a.temp, a.alpha);
+      B.x += h.x; B.y += h.y; B.z += h.z;
+    }
+    ob[0][w] = B.x; ob[1][w] = B.y; ob[2][w] = B.z;
+    V3<T> t;
+    if (a.precess) {
+      t = lltorque_cell(m0, B, a.alpha.at(I));
+    } else {
+      const V3<T> c = vcross(m0, vcross(m0, B));
+      t = v3(-c.x, -c.y, -c.z);
+    }
+    if (a.frozenLUT && a.frozenLUT[rc[w]] != T(0)) t = v3(T(0), T(0), T(0));
+    ot[0][w] = t.x; ot[1][w] = t.y; ot[2][w] = t.z;
+  }
+  if (a.beff[0]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) st4(a.beff[c] + I0, ob[c]);
+  }
+  if (a.tau[0]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) st4(a.tau[c] + I0, ot[c]);
+  }
+}
+
 template <typename T> void launch_field_torque(const FieldArgs<T> &a, hipStream_t s);
 
 }  // namespace um

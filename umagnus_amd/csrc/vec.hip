@@ -72,10 +72,113 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<T> a) {
     for (int c = 0; c < nc; c++) a.dst[c][i] = r[c];
   }
 }
+// Same arithmetic, 16 bytes per access: each thread owns W = 16/sizeof(T)
+// consecutive cells of every component, and the source count, component
+// count, madd3 grouping and normalisation are compile-time.
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; static constexpr int n = 4; };
+template <> struct Vec16<double> { using type = double2; static constexpr int n = 2; };
+
+template <typename T, int NS, int NC, bool G3, bool NORM>
+__global__ __launch_bounds__(256) void k_stage_v(StageArgs<T> a) {
+  using V = typename Vec16<T>::type;
+  constexpr int W = Vec16<T>::n;
+  const long g = blockIdx.x * 256L + threadIdx.x;
+  if (g * W >= a.N) return;
+  T r[3][W];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    T x[NS][W];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const V v = ((const V *)a.src[k][c])[g];
+      __builtin_memcpy(x[k], &v, sizeof(V));
+    }
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      T acc;
+      if (G3) {
+        acc = (a.fac[0] * x[0][w]) + (a.fac[1] * x[1][w] + a.fac[2] * x[2][w]);
+      } else {
+        acc = a.fac[0] * x[0][w];
+#pragma unroll
+        for (int k = 1; k < NS; k++) acc = acc + a.fac[k] * x[k][w];
+      }
+      r[c][w] = acc;
+    }
+  }
+  if (NORM) {
+    T vv[W];
+    if (a.vol) {
+      const V v = ((const V *)a.vol)[g];
+      __builtin_memcpy(vv, &v, sizeof(V));
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; w++) vv[w] = T(1);
+    }
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      const T v = vv[w];
+      const T X = v * r[0][w], Y = v * r[1][w], Z = v * r[2][w];
+      const T l = sqrt(X * X + Y * Y + Z * Z);
+      const T inv = (l != T(0)) ? T(1) / l : T(0);
+      if (v == T(0)) {
+        r[0][w] = r[1][w] = r[2][w] = T(0);
+      } else {
+        r[0][w] = X * inv;
+        r[1][w] = Y * inv;
+        r[2][w] = Z * inv;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    V v;
+    __builtin_memcpy(&v, r[c], sizeof(V));
+    ((V *)a.dst[c])[g] = v;
+  }
+}
+
+template <typename T>
+static bool stage_vectorisable(const StageArgs<T> &a) {
+  constexpr int W = Vec16<T>::n;
+  if (a.N % W) return false;
+  auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+  for (int c = 0; c < a.ncomp; c++) {
+    if (!al(a.dst[c])) return false;
+    for (int k = 0; k < a.nsrc; k++)
+      if (!al(a.src[k][c])) return false;
+  }
+  return !(a.normalize && a.vol && !al(a.vol));
+}
+
 template <typename T>
 void launch_stage(const StageArgs<T> &a, hipStream_t s) {
   UM_REQUIRE(a.nsrc >= 1 && a.nsrc <= 8, UM_EINVAL, "madd: 1..8 sources");
   UM_REQUIRE(a.ncomp == 1 || a.ncomp == 3, UM_EINVAL, "stage: ncomp must be 1 or 3");
+  if (a.N > 0 && stage_vectorisable(a)) {
+    const long groups = a.N / Vec16<T>::n;
+    const dim3 grid((unsigned)((groups + 255) / 256));
+    const bool g3 = a.group3 && a.nsrc == 3, nrm = a.normalize && a.ncomp == 3;
+#define UM_STAGE(NS)                                                                          \
+  case NS:                                                                                    \
+    if (a.ncomp == 1) hipLaunchKernelGGL((k_stage_v<T, NS, 1, false, false>), grid, dim3(256), 0, s, a); \
+    else if (nrm) hipLaunchKernelGGL((k_stage_v<T, NS, 3, false, true>), grid, dim3(256), 0, s, a);      \
+    else hipLaunchKernelGGL((k_stage_v<T, NS, 3, false, false>), grid, dim3(256), 0, s, a);             \
+    break;
+    if (g3) {
+      if (a.ncomp == 1) hipLaunchKernelGGL((k_stage_v<T, 3, 1, true, false>), grid, dim3(256), 0, s, a);
+      else if (nrm) hipLaunchKernelGGL((k_stage_v<T, 3, 3, true, true>), grid, dim3(256), 0, s, a);
+      else hipLaunchKernelGGL((k_stage_v<T, 3, 3, true, false>), grid, dim3(256), 0, s, a);
+    } else {
+      switch (a.nsrc) {
+        UM_STAGE(1) UM_STAGE(2) UM_STAGE(3) UM_STAGE(4) UM_STAGE(5) UM_STAGE(6) UM_STAGE(7) UM_STAGE(8)
+      }
+    }
+#undef UM_STAGE
+    UM_HIP(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_stage<T>, dim3(blocks_for(a.N, 256, 16384)), dim3(256), 0, s, a);
   UM_HIP(hipGetLastError());
 }
@@ -157,8 +260,81 @@ __global__ __launch_bounds__(256) void k_err(ErrArgs<T> a) {
     block_max_to(m2, a.res + 2);
   }
 }
+// Same reductions with 16-byte accesses (W cells per access), mode compile-time.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_err_v(ErrArgs<T> a) {
+  using V = typename Vec16<T>::type;
+  constexpr int W = Vec16<T>::n, NK = MODE == 0 ? 6 : MODE == 1 ? 2 : 1;
+  double m0 = 0, m1 = 0, m2 = 0;
+  bool nan = false;
+  const long groups = a.N / W;
+#pragma unroll 2
+  for (long g = blockIdx.x * 256L + threadIdx.x; g < groups; g += (long)gridDim.x * 256) {
+    T x[NK][3][W];
+#pragma unroll
+    for (int j = 0; j < NK; j++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const V v = ((const V *)a.k[j][c])[g];
+        __builtin_memcpy(x[j][c], &v, sizeof(V));
+      }
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      if (MODE == 0) {
+        T E[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          T acc = a.e[0] * x[0][c][w];
+#pragma unroll
+          for (int j = 1; j < 6; j++) acc = acc + a.e[j] * x[j][c][w];
+          E[c] = acc;
+        }
+        const double e2 = (double)E[0] * E[0] + (double)E[1] * E[1] + (double)E[2] * E[2];
+        const double k2 = (double)x[5][0][w] * x[5][0][w] + (double)x[5][1][w] * x[5][1][w] +
+                          (double)x[5][2][w] * x[5][2][w];
+        const double rel = (k2 == 0.0) ? 0.0 : sqrt(e2) / sqrt(k2);
+        nan |= (e2 != e2);
+        m0 = fmax(m0, e2);
+        m1 = fmax(m1, k2);
+        m2 = fmax(m2, rel);
+      } else if (MODE == 1) {
+        const double dx = (double)(x[0][0][w] - x[1][0][w]);
+        const double dy = (double)(x[0][1][w] - x[1][1][w]);
+        const double dz = (double)(x[0][2][w] - x[1][2][w]);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        nan |= (d2 != d2);
+        m0 = fmax(m0, d2);
+      } else {
+        const double X = x[0][0][w], Y = x[0][1][w], Z = x[0][2][w];
+        const double d2 = X * X + Y * Y + Z * Z;
+        nan |= (d2 != d2);
+        m0 = fmax(m0, d2);
+      }
+    }
+  }
+  if (nan) m0 = __longlong_as_double(0x7ff8000000000000LL);
+  block_max_to(m0, a.res);
+  if (MODE == 0) {
+    block_max_to(m1, a.res + 1);
+    block_max_to(m2, a.res + 2);
+  }
+}
+
 template <typename T>
 void launch_err(const ErrArgs<T> &a, hipStream_t s) {
+  constexpr int W = Vec16<T>::n;
+  const int nk = a.mode == 0 ? 6 : a.mode == 1 ? 2 : 1;
+  bool vec = a.N > 0 && a.N % W == 0;
+  for (int j = 0; j < nk && vec; j++)
+    for (int c = 0; c < 3; c++) vec = vec && ((uintptr_t)a.k[j][c] & 15) == 0;
+  if (vec) {
+    const dim3 grid(blocks_for(a.N / W, 256, 2048));
+    if (a.mode == 0) hipLaunchKernelGGL((k_err_v<T, 0>), grid, dim3(256), 0, s, a);
+    else if (a.mode == 1) hipLaunchKernelGGL((k_err_v<T, 1>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_err_v<T, 2>), grid, dim3(256), 0, s, a);
+    UM_HIP(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_err<T>, dim3(blocks_for(a.N, 256, 2048)), dim3(256), 0, s, a);
   UM_HIP(hipGetLastError());
 }
