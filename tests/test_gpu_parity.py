@@ -82,7 +82,11 @@ def test_demag_golden(gpu, N, cs, expect, tol, prec):
 CONV_SIZES = [(3, 4, 1), (16, 8, 1), (20, 12, 1), (5, 4, 2), (6, 5, 3), (8, 6, 4), (7, 3, 5), (16, 12, 6),
               (24, 10, 7), (48, 20, 3), (96, 12, 2), (64, 64, 1), (32, 24, 8),
               # register Z pass (Pz = 2 Nz, Nz in 8/16/32): partial kx groups, unpaired ky rows
-              (70, 5, 8), (40, 9, 16), (12, 8, 32)]
+              (70, 5, 8), (40, 9, 16), (12, 8, 32),
+              # four-step Y pass (Py = 64, 256, 1024), with even Nz (two planes per work-group) and odd Nz
+              (12, 32, 4), (10, 128, 3), (6, 512, 2), (9, 512, 1),
+              # four-step X passes (Nx = 128 .. 1024)
+              (128, 6, 3), (256, 5, 2), (512, 3, 2), (1024, 2, 1)]
 
 
 @pytest.mark.parametrize("N", CONV_SIZES)
@@ -305,10 +309,13 @@ def test_sim_beff_regions_cubic_thermal(gpu, oracle, prec):
 
 
 @pytest.mark.parametrize("prec", [1, 0])
-@pytest.mark.parametrize("N,openbc", [((24, 12, 4), 1), ((16, 8, 1), 0), ((12, 20, 3), 0), ((8, 6, 2), 1)])
+@pytest.mark.parametrize("N,openbc", [((24, 12, 4), 1), ((16, 8, 1), 0), ((12, 20, 3), 0), ((8, 6, 2), 1),
+                                      ((64, 12, 3), 1), ((128, 10, 1), 0)])
 def test_sim_beff_dmi_regions_vacuum(gpu, oracle, prec, N, openbc):
     """Exchange + interfacial DMI with a region map (per-pair A and D), vacuum cells
-    (m = 0: the free-boundary branches of dmi.cl) and open / closed edges."""
+    (m = 0: the free-boundary branches of dmi.cl) and open / closed edges.  Nx = 64
+    and 128 in FP32 take the torque through the fused X-inverse + field kernel
+    (with a partly filled last row block)."""
     _say("dmi regions", N, openbc, prec)
     cs = (2e-9, 2.5e-9, 1.5e-9)
     rng = np.random.default_rng(N[0] * 7 + openbc)

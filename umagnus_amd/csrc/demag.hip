@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "fft_ct.hpp"
+#include "field.hpp"
 #include "sim.hpp"
 
 namespace um {
@@ -249,6 +250,83 @@ __global__ __launch_bounds__(FFT_NT) void k_xinv(XiArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Register four-step X inverse (FP32, zero-padded rows: Px = 2 Nx, n = Px/2
+// complex = A*B with B = A or 2A).  A group of B lanes owns G = B/A rows.
+// Step 1: lane (h, n1) transforms z[n1 + A n2] over n2 (B points) in
+// registers, twiddles by W_n^(n1 k2) and writes to LDS; step 2: lane k2
+// transforms over n1 (A points) for each of the G rows, giving Z[k2 + B k1].
+// The LDS transpose rows are padded to A+1 entries, so both the writes and
+// the stride-(A+1) reads are bank-conflict free.
+// ---------------------------------------------------------------------------
+constexpr int XF4_NT = 128;
+
+template <int N> struct X4Shape;
+template <> struct X4Shape<64> { static constexpr int A = 8, B = 8; };
+template <> struct X4Shape<128> { static constexpr int A = 8, B = 16; };
+template <> struct X4Shape<256> { static constexpr int A = 16, B = 16; };
+template <> struct X4Shape<512> { static constexpr int A = 16, B = 32; };
+template <> struct X4Shape<1024> { static constexpr int A = 32, B = 32; };
+
+// inverse: half spectrum X[k], k = 0..n -> real rows, x < Nx = n:
+// Z'[k] = (X[k] + conj X[n-k]) + i W_2n^-k (X[k] - conj X[n-k]), z = IDFT_n(Z'),
+// x[2j] = Re z[j], x[2j+1] = Im z[j] for j < n/2
+template <typename T, int N>
+__global__ __launch_bounds__(XF4_NT) void k_xi4(XiArgs<T> a) {
+  constexpr int A = X4Shape<N>::A, B = X4Shape<N>::B, G = B / A;
+  constexpr int GROUPS = XF4_NT / B, ROWS = GROUPS * G, TS = B * (A + 1);
+  __shared__ C2<T> lds[ROWS * TS];
+  const int c = blockIdx.y;
+  const int grp = threadIdx.x / B, l = threadIdx.x % B;
+  const int row0 = blockIdx.x * ROWS + grp * G;
+  {
+    const int h = l / A, n1 = l % A, r = row0 + h;
+    C2<T> v[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
+    if (r < a.rows) {
+      const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+      const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
+      C2<T> xk[B], xn[B];
+#pragma unroll
+      for (int n2 = 0; n2 < B; n2++) {
+        const int k = n1 + A * n2;
+        xk[n2] = irow[k];
+        xn[n2] = irow[N - k];
+      }
+#pragma unroll
+      for (int n2 = 0; n2 < B; n2++) {
+        const int k = n1 + A * n2;
+        const C2<T> sm = cadd(xk[n2], cconj(xn[n2])), d = csub(xk[n2], cconj(xn[n2]));
+        const C2<T> t = cmul(cconj(a.tw2[k]), d);
+        v[n2] = C2<T>{sm.x - t.y, sm.y + t.x};
+      }
+    }
+    dft<B, +1>(v);
+#pragma unroll
+    for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(a.tw2[2 * n1 * k2]));   // W_n^-(n1 k2)
+    C2<T> *t = lds + (grp * G + h) * TS + n1;
+#pragma unroll
+    for (int k2 = 0; k2 < B; k2++) t[k2 * (A + 1)] = v[k2];
+  }
+  __syncthreads();
+  T *dst = a.dst[c];
+#pragma unroll
+  for (int h = 0; h < G; h++) {
+    C2<T> u[A];
+    const C2<T> *t = lds + (grp * G + h) * TS + l * (A + 1);
+#pragma unroll
+    for (int n1 = 0; n1 < A; n1++) u[n1] = t[n1];
+    dft<A, +1>(u);
+    const int r = row0 + h;
+    if (r < a.rows) {
+      T *orow = dst + (long)r * N;
+#pragma unroll
+      for (int k1 = 0; k1 < A / 2; k1++) *(float2 *)(orow + 2 * (l + B * k1)) = float2{u[k1].x, u[k1].y};
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Column pass along y (or z): nl adjacent kx columns per work-group.
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -319,7 +397,14 @@ __global__ __launch_bounds__(FFT_NT) void k_colfft(ColArgs<T> a) {
 // k < P/2 (k1 < A/2).  One LDS round trip per element instead of one per
 // radix pass.
 // ---------------------------------------------------------------------------
-constexpr int COLY_PAD = 8;   // row padding of the transpose (2-way = minimal b64 bank sharing)
+// columns per work-group of the P = 1024 pass (build-time tunable)
+#ifndef UM_COLY_C32
+#define UM_COLY_C32 16
+#endif
+// z planes per work-group of the FP32 Y passes (build-time tunable)
+#ifndef UM_COLY_ZB
+#define UM_COLY_ZB 1
+#endif
 
 // Work-group index permuted so that the tiles one XCD runs (linear ids w,
 // w+8, w+16, ... are dispatched to the same XCD) are adjacent in memory and
@@ -329,9 +414,11 @@ __device__ __forceinline__ int xcd_tile(int w, int total) {
   return w >= full ? w : (w & 7) * q + (w >> 3);
 }
 
-template <typename T, int A, int DIR, int C, bool HALF>
+template <typename T, int A, int DIR, int C, bool HALF, int ZB>
 __global__ __launch_bounds__(A * C) void k_coly(ColArgs<T> a) {
-  constexpr int P = A * A, S = A * C + COLY_PAD;
+  // transpose row stride A*C + C: a wave's 64 b64 accesses then share each
+  // LDS bank pair exactly twice (the minimum for 8-byte lanes)
+  constexpr int P = A * A, S = A * C + C;
   extern __shared__ __align__(16) unsigned char smem_[];
   C2<T> *lds = (C2<T> *)smem_;
   const int ngx = gridDim.x, ngy = gridDim.y;
@@ -341,42 +428,53 @@ __global__ __launch_bounds__(A * C) void k_coly(ColArgs<T> a) {
   const int kx = gx * C + col;
   const bool valid = kx < a.H;
   // uniform plane base + one 32-bit lane offset (row i, column kx)
-  C2<T> *ub = a.buf[gz] + (long)gy * a.pstride;
   const uint32_t loff = (uint32_t)(((long)i * a.estride + min(kx, a.H - 1)) * sizeof(C2<T>));
   const long rstep = (long)A * a.estride;   // rows n2 -> n2 + 1
   const C2<T> *tw = (const C2<T> *)a.f.tw;   // W_P^m, m < P
-
-  C2<T> v[A];
   constexpr int NIN = (HALF && DIR < 0) ? A / 2 : A;
-#pragma unroll
-  for (int n2 = 0; n2 < A; n2++) {
-    v[n2] = C2<T>{T(0), T(0)};
-    if (n2 < NIN) {
-      const auto *q = at_off<const T>(ub + n2 * rstep, loff);
-      v[n2] = C2<T>{q[0], q[1]};
-    }
-  }
-  dft<A, DIR>(v);
-  pin(v);
-#pragma unroll
-  for (int k2 = 1; k2 < A; k2++) {
-    C2<T> t = tw[i * k2];
-    if (DIR > 0) t.y = -t.y;
-    v[k2] = cmul(v[k2], t);
-  }
-#pragma unroll
-  for (int k2 = 0; k2 < A; k2++) lds[k2 * S + threadIdx.x] = v[k2];
-  __syncthreads();
-#pragma unroll
-  for (int n1 = 0; n1 < A; n1++) v[n1] = lds[i * S + n1 * C + col];
-  dft<A, DIR>(v);
   constexpr int NOUT = (HALF && DIR > 0) ? A / 2 : A;
-  if (valid) {
+
+  // ZB planes per work-group; the loads of plane p+1 are issued before plane p
+  // is transformed, so they overlap its arithmetic and LDS exchange
+  C2<T> nxt[NIN];
+  auto load = [&](int plane) {
+    C2<T> *ub = a.buf[gz] + (long)plane * a.pstride;
 #pragma unroll
-    for (int k1 = 0; k1 < NOUT; k1++) {
-      auto *q = at_off<T>(ub + k1 * rstep, loff);
-      q[0] = v[k1].x;
-      q[1] = v[k1].y;
+    for (int n2 = 0; n2 < NIN; n2++) {
+      const auto *q = at_off<const T>(ub + n2 * rstep, loff);
+      nxt[n2] = C2<T>{q[0], q[1]};
+    }
+  };
+  load(gy * ZB);
+#pragma unroll
+  for (int p = 0; p < ZB; p++) {
+    C2<T> v[A];
+#pragma unroll
+    for (int n2 = 0; n2 < A; n2++) v[n2] = n2 < NIN ? nxt[n2] : C2<T>{T(0), T(0)};
+    if (p + 1 < ZB) load(gy * ZB + p + 1);
+    dft<A, DIR>(v);
+    pin(v);
+#pragma unroll
+    for (int k2 = 1; k2 < A; k2++) {
+      C2<T> t = tw[i * k2];
+      if (DIR > 0) t.y = -t.y;
+      v[k2] = cmul(v[k2], t);
+    }
+    if (p > 0) __syncthreads();   // the previous plane's transpose reads are done
+#pragma unroll
+    for (int k2 = 0; k2 < A; k2++) lds[k2 * S + threadIdx.x] = v[k2];
+    __syncthreads();
+#pragma unroll
+    for (int n1 = 0; n1 < A; n1++) v[n1] = lds[i * S + n1 * C + col];
+    dft<A, DIR>(v);
+    if (valid) {
+      C2<T> *ub = a.buf[gz] + (long)(gy * ZB + p) * a.pstride;
+#pragma unroll
+      for (int k1 = 0; k1 < NOUT; k1++) {
+        auto *q = at_off<T>(ub + k1 * rstep, loff);
+        q[0] = v[k1].x;
+        q[1] = v[k1].y;
+      }
     }
   }
 }
@@ -686,8 +784,15 @@ __global__ __launch_bounds__(CONVZ_NT) void k_convz(ConvZArgs<T> a) {
 }
 
 template <typename T> struct ConvZTable;
+// frequencies per LDS exchange round and per scheduling group (build-time tunables)
+#ifndef UM_CONVZ_FR
+#define UM_CONVZ_FR 8
+#endif
+#ifndef UM_CONVZ_G
+#define UM_CONVZ_G 4
+#endif
 template <> struct ConvZTable<float> {
-  static constexpr int FR = 8, G = 4;
+  static constexpr int FR = UM_CONVZ_FR, G = UM_CONVZ_G;
   static bool has(int nz) { return nz == 8 || nz == 16 || nz == 32; }
 };
 template <> struct ConvZTable<double> {
@@ -930,6 +1035,28 @@ static void allow_lds(K kern, size_t bytes) {
     return;                                                                  \
   }
 
+// four-step X inverse: FP32, n in 64..1024, rows of exactly Nx = n reals (Px = 2 Nx)
+template <typename T>
+static bool x4_ok(int n, int Nx) {
+  return std::is_same<T, float>::value && Nx == n && (n == 64 || n == 128 || n == 256 || n == 512 || n == 1024);
+}
+template <typename T>
+static bool launch_xi4(const XiArgs<T> &a, int n, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (!x4_ok<T>(n, a.Nx)) return false;
+    auto go = [&](auto kern, int rows_per_wg) {
+      hipLaunchKernelGGL(kern, dim3((a.rows + rows_per_wg - 1) / rows_per_wg, 3), dim3(XF4_NT), 0, s, a);
+      UM_HIP(hipGetLastError());
+      return true;
+    };
+#define UM_XI4(NN) \
+    if (n == NN) return go(k_xi4<T, NN>, XF4_NT / X4Shape<NN>::B * (X4Shape<NN>::B / X4Shape<NN>::A));
+    UM_XI4(64) UM_XI4(128) UM_XI4(256) UM_XI4(512) UM_XI4(1024)
+#undef UM_XI4
+  }
+  return false;
+}
+
 template <typename T>
 static void launch_xfwd(const XArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s) {
   const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + 2 * n);
@@ -948,6 +1075,7 @@ static void launch_xfwd(const XArgs<T> &a, int n, int nl, dim3 grid, hipStream_t
 }
 template <typename T>
 static void launch_xinv(const XiArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s) {
+  if (launch_xi4<T>(a, n, s)) return;
   const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + 2 * n);
   if constexpr (std::is_same<T, float>::value) {
     UM_CT_CASE((k_xinv<float, 64, 32>), 64, 32, ct)
@@ -989,18 +1117,25 @@ static bool launch_coly(const ColArgs<T> &a, int P, int nz, hipStream_t s) {
   else if (DIR > 0 && nin == P && nout * 2 == P) half = 1;
   else if (nin == P && nout == P) half = 0;
   else return false;
-  auto go = [&](auto kern, int A, int C) {
-    const size_t lds = sizeof(C2<T>) * (size_t)A * (A * C + COLY_PAD);
+  // planes per work-group (software-pipelined): FP32 forward pass only (the
+  // inverse loads twice as many rows and would spill)
+  const int zb = (sizeof(T) == 4 && DIR < 0 && nz % UM_COLY_ZB == 0) ? UM_COLY_ZB : 1;
+  auto go = [&](auto kern, int A, int C, int ZB) {
+    const size_t lds = sizeof(C2<T>) * (size_t)A * (A * C + C);
     allow_lds(kern, lds);
-    const dim3 grid((a.H + C - 1) / C, nz, 3);
+    const dim3 grid((a.H + C - 1) / C, nz / ZB, 3);
     hipLaunchKernelGGL(kern, grid, dim3(A * C), lds, s, a);
     UM_HIP(hipGetLastError());
     return true;
   };
-#define UM_COLY(A, C)                                                                            \
-  if (P == A * A) return half ? go(k_coly<T, A, DIR, C, true>, A, C) : go(k_coly<T, A, DIR, C, false>, A, C);
+#define UM_COLY(A, C)                                                                                         \
+  if (P == A * A) {                                                                                           \
+    if constexpr (DIR < 0 && UM_COLY_ZB > 1)                                                                  \
+      if (half && zb == UM_COLY_ZB) return go(k_coly<T, A, DIR, C, true, UM_COLY_ZB>, A, C, UM_COLY_ZB);     \
+    return half ? go(k_coly<T, A, DIR, C, true, 1>, A, C, 1) : go(k_coly<T, A, DIR, C, false, 1>, A, C, 1);  \
+  }
   if constexpr (std::is_same<T, float>::value) {
-    UM_COLY(32, 8)
+    UM_COLY(32, UM_COLY_C32)
     UM_COLY(16, 16)
     UM_COLY(8, 32)
   } else {
@@ -1040,6 +1175,117 @@ static void launch_conv(const ConvArgs<T> &a, int n, int B, dim3 grid, hipStream
 }
 #undef UM_CT_CASE
 
+// ---------------------------------------------------------------------------
+// X inverse fused with the four-cell effective field + torque (FP32): a
+// work-group inverts R rows of all three components (3R lines in LDS), then
+// evaluates the field of those R x Nx cells with B_demag read from LDS.
+// ---------------------------------------------------------------------------
+// rows per work-group for X length n (complex): 3 R n = 3072 lines' elements
+static int xf_rows(int n) {
+  switch (n) { case 64: return 16; case 128: return 8; case 256: return 4; case 512: return 2; case 1024: return 1; }
+  return 0;
+}
+
+template <typename T, int CN, int R, bool REG, int EX>
+__global__ __launch_bounds__(FFT_NT) void k_xinv_field(XiArgs<T> a, FieldArgs<T> f) {
+  constexpr int NL = 3 * R, HALF = CN / 2 + 1, TOT = NL * HALF, S = (TOT + FFT_NT - 1) / FFT_NT;
+  extern __shared__ __align__(16) unsigned char smem_[];
+  C2<T> *lds = (C2<T> *)smem_;
+  C2<T> *twl = lds + NL * CN;
+  for (int g = threadIdx.x; g < 2 * CN; g += FFT_NT) twl[g] = a.tw2[g];
+  const int row0 = xcd_tile(blockIdx.x, gridDim.x) * R;   // neighbouring row blocks share an XCD
+  // this thread's four-cell group (R*Nx/4 <= FFT_NT): its m and stencil loads
+  // go out first and overlap the spectrum loads and the FFT
+  const int ngx = f.g.Nx >> 2;
+  const int gr = threadIdx.x / ngx, ggx = threadIdx.x - gr * ngx, grow = row0 + gr;
+  const bool gvalid = gr < R && grow < a.rows;
+  const int giz = gvalid ? (int)a.fRy.div(grow) : 0, giy = gvalid ? grow - giz * (int)a.fRy.d : 0;
+  G4In<T> q;
+  if (gvalid) g4_load<T, REG, EX>(f, 4 * ggx, giy, giz, q);
+  // Z'[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k]); line l = c*R + r
+  C2<T> xk[S], xn[S];
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const int g = threadIdx.x + s * FFT_NT;
+    xk[s] = C2<T>{T(0), T(0)};
+    xn[s] = xk[s];
+    if (TOT % FFT_NT == 0 || g < TOT) {
+      const int l = g / HALF, k = g - l * HALF;
+      const int c = l / R, r = row0 + (l - c * R);
+      if (r < a.rows) {
+        const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+        const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
+        xk[s] = irow[k];
+        xn[s] = irow[CN - k];
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const int g = threadIdx.x + s * FFT_NT;
+    if (TOT % FFT_NT == 0 || g < TOT) {
+      const int l = g / HALF, k = g - l * HALF;
+      const C2<T> Xk = xk[s], Xnk = xn[s];
+      C2<T> sm = cadd(Xk, cconj(Xnk)), d = csub(Xk, cconj(Xnk));
+      C2<T> t = cmul(cconj(a.tw2[k]), d);
+      lds[l * CN + k] = C2<T>{sm.x - t.y, sm.y + t.x};
+      if (k != 0 && CN - k != k) {
+        sm = cadd(Xnk, cconj(Xk));
+        d = csub(Xnk, cconj(Xk));
+        t = cmul(cconj(a.tw2[CN - k]), d);
+        lds[l * CN + CN - k] = C2<T>{sm.x - t.y, sm.y + t.x};
+      }
+    }
+  }
+  __syncthreads();
+  ct_fft<T, CN, NL, +1, 0, 2>(lds, twl);
+  // field + torque of the group; the real output x of line l is lds[l*CN + x/2]
+  if (gvalid) {
+    T bdv[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const C2<T> *b = lds + (c * R + gr) * CN + 2 * ggx;
+      const C2<T> u0 = b[0], u1 = b[1];
+      bdv[c][0] = u0.x; bdv[c][1] = u0.y; bdv[c][2] = u1.x; bdv[c][3] = u1.y;
+    }
+    g4_compute<T, REG, EX>(f, 4 * ggx, giy, giz, q, bdv);
+  }
+}
+
+template <typename T>
+static bool field4_aligned(const FieldArgs<T> &a) {
+  if (a.g.Nx % 4) return false;
+  auto al = [](const void *p) { return ((uintptr_t)p & (4 * sizeof(T) - 1)) == 0; };
+  for (int c = 0; c < 3; c++) {
+    if (!al(a.m[c])) return false;
+    if (a.tau[0] && !al(a.tau[c])) return false;
+    if (a.beff[0] && !al(a.beff[c])) return false;
+  }
+  return !a.regions || ((uintptr_t)a.regions & 3) == 0;
+}
+
+template <typename T>
+static void launch_xinv_field(const XiArgs<T> &a, const FieldArgs<T> &f, int n, int R, hipStream_t s) {
+  const dim3 grid((a.rows + R - 1) / R);
+  const size_t lds = sizeof(C2<T>) * ((size_t)3 * R * n + 2 * n);
+  const bool reg = f.regions != nullptr;
+  auto go = [&](auto kern) {
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, grid, dim3(FFT_NT), lds, s, a, f);
+    UM_HIP(hipGetLastError());
+  };
+#define UM_XF(CN, RR)                                                                   \
+  if (n == CN) {                                                                        \
+    if (f.exch_mode == 0) { if (reg) go(k_xinv_field<T, CN, RR, true, 0>); else go(k_xinv_field<T, CN, RR, false, 0>); } \
+    else if (f.exch_mode == 1) { if (reg) go(k_xinv_field<T, CN, RR, true, 1>); else go(k_xinv_field<T, CN, RR, false, 1>); } \
+    else { if (reg) go(k_xinv_field<T, CN, RR, true, 2>); else go(k_xinv_field<T, CN, RR, false, 2>); } \
+    return;                                                                             \
+  }
+  UM_XF(64, 16) UM_XF(128, 8) UM_XF(256, 4) UM_XF(512, 2) UM_XF(1024, 1)
+#undef UM_XF
+  throw Error(UM_EUNSUP, "fused X inverse: unsupported length");
+}
+
 struct DemagBase {
   virtual ~DemagBase() {}
   int prec = UM_F32;
@@ -1047,6 +1293,8 @@ struct DemagBase {
   virtual void exec(void *const B[3], const void *const m[3], const void *vol, const void *Ms,
                     double Ms_mul, hipStream_t s) = 0;
   virtual void kspec(int c, void *host) = 0;
+  virtual bool exec_field(const void *const m[3], const void *vol, const void *Ms, double Ms_mul, const void *fa,
+                          hipStream_t s) = 0;
 };
 
 template <typename T>
@@ -1165,8 +1413,9 @@ struct DemagPlan : DemagBase {
       fprintf(stderr, "umhip: demag kernel imaginary part %g (tensor lacks symmetry)\n", maximg * sqrt(scale));
   }
 
-  void exec(void *const Bv[3], const void *const mv[3], const void *vol, const void *Msa,
-            double Ms_mul, hipStream_t s) override {
+  // X-fwd through Y-inv (3D) or the 2D Y pass: leaves the convolved half
+  // spectrum (rows y < Ny) in spec[] for the X inverse.
+  void spectral(const void *const mv[3], const void *vol, const void *Msa, double Ms_mul, hipStream_t s) {
     const int Nx = N[0], Ny = N[1], Nz = N[2];
     C2<T> *sp[3] = {spec[0].as<C2<T>>(), spec[1].as<C2<T>>(), spec[2].as<C2<T>>()};
     // X forward
@@ -1216,14 +1465,47 @@ struct DemagPlan : DemagBase {
       dim3 gc((H + B_conv - 1) / B_conv, 1);
       launch_conv<T>(ca, P[1], B_conv, gc, s);
     }
-    // X inverse
-    {
-      XiArgs<T> a{};
-      a.f = tx.len; a.tw2 = (const C2<T> *)tx2.buf.p; a.nl = nl_x; a.Nx = Nx; a.rows = Ny * Nz;
-      a.fhalf.init(P[0] / 4 + 1); a.fRy.init(Ny); a.Py = is2d ? Ny : P[1]; a.Hp = Hp;
-      for (int c = 0; c < 3; c++) { a.in[c] = sp[c]; a.dst[c] = (T *)Bv[c]; }
-      dim3 g((a.rows + nl_x - 1) / nl_x, 3);
-      launch_xinv<T>(a, P[0] / 2, nl_x, g, s);
+  }
+
+  XiArgs<T> xinv_args(int nl) const {
+    XiArgs<T> a{};
+    a.f = tx.len; a.tw2 = (const C2<T> *)tx2.buf.p; a.nl = nl; a.Nx = N[0]; a.rows = N[1] * N[2];
+    a.fhalf.init(P[0] / 4 + 1); a.fRy.init(N[1]); a.Py = is2d ? N[1] : P[1]; a.Hp = Hp;
+    for (int c = 0; c < 3; c++) a.in[c] = spec[c].as<C2<T>>();
+    return a;
+  }
+
+  void exec(void *const Bv[3], const void *const mv[3], const void *vol, const void *Msa,
+            double Ms_mul, hipStream_t s) override {
+    spectral(mv, vol, Msa, Ms_mul, s);
+    XiArgs<T> a = xinv_args(nl_x);
+    for (int c = 0; c < 3; c++) a.dst[c] = (T *)Bv[c];
+    dim3 g((a.rows + nl_x - 1) / nl_x, 3);
+    launch_xinv<T>(a, P[0] / 2, nl_x, g, s);
+  }
+
+  // Demag convolution fused with the effective field and torque: the X
+  // inverse of all three components of a few rows stays in LDS and feeds the
+  // four-cell field kernel, so B_demag never goes to HBM.  false: not
+  // available for this size / precision (the caller runs exec + field).
+  bool exec_field(const void *const mv[3], const void *vol, const void *Msa, double Ms_mul, const void *fav,
+                  hipStream_t s) override {
+#ifndef UM_FUSED_FIELD
+    // measured slower than the separate X inverse + four-cell field kernels at
+    // 512x512x32 (the fused kernel is VALU- and latency-bound), so opt-in only
+    return false;
+#endif
+    if constexpr (!std::is_same<T, float>::value) {
+      return false;
+    } else {
+      const FieldArgs<T> &f = *(const FieldArgs<T> *)fav;
+      const int n = P[0] / 2, R = xf_rows(n);
+      // one four-cell group per thread
+      if (!R || !field4_aligned(f) || R * (N[0] / 4) > FFT_NT) return false;
+      spectral(mv, vol, Msa, Ms_mul, s);
+      const XiArgs<T> a = xinv_args(R);
+      launch_xinv_field(a, f, n, R, s);
+      return true;
     }
   }
 
@@ -1279,6 +1561,10 @@ void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3]
 void demag_exec(DemagBase *d, void *const B[3], const void *const m[3], const void *vol, const void *Ms,
                 double Ms_mul, hipStream_t s) {
   d->exec(B, m, vol, Ms, Ms_mul, s);
+}
+bool demag_exec_field(DemagBase *d, const void *const m[3], const void *vol, const void *Ms, double Ms_mul,
+                      const void *fa, hipStream_t s) {
+  return d->exec_field(m, vol, Ms, Ms_mul, fa, s);
 }
 void demag_destroy(DemagBase *d) { delete d; }
 void demag_kspec(DemagBase *d, int c, void *host) { d->kspec(c, host); }

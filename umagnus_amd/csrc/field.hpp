@@ -358,94 +358,107 @@ __device__ __forceinline__ V3<T> dmi_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb,
   return h;
 }
 
-// Effective field and torque of cells (ix0..ix0+3, iy, iz); bdv: demag field
-// of those cells (zero when absent).  Same term order as k_field_torque.
+// Inputs of a four-cell group (ix0..ix0+3, iy, iz): its m and region bytes and
+// those of the stencil neighbours.  Loading them is split from the arithmetic
+// so a caller can issue the loads early and overlap them with other work.
+template <typename T>
+struct G4In {
+  T mc[3][4], mxl[3], mxr[3], myl[3][4], myh[3][4], mzl[3][4], mzh[3][4];
+  int rc[4], rxl, rxr, ryl[4], ryh[4], rzl[4], rzh[4];
+  T A0, D0;
+};
+
 template <typename T, bool REG, int EX>
-__device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int iy, int iz, const T (&bdv)[3][4]) {
+__device__ __forceinline__ void g4_load(const FieldArgs<T> &a, int ix0, int iy, int iz, G4In<T> &q) {
   const Mesh &g = a.g;
   const long I0 = g.idx(ix0, iy, iz);
-  T mc[3][4];
 #pragma unroll
-  for (int c = 0; c < 3; c++) ld4(a.m[c] + I0, mc[c]);
-  int rc[4] = {0, 0, 0, 0};
-  if (REG) ld4u8(a.regions + I0, rc);
-  // stencil neighbours
-  const int nnb = g.Nz != 1 ? 6 : 4;
-  T mxl[3] = {T(0), T(0), T(0)}, mxr[3] = {T(0), T(0), T(0)};
-  T myl[3][4], myh[3][4], mzl[3][4], mzh[3][4];
-  int rxl = 0, rxr = 0, ryl[4] = {0, 0, 0, 0}, ryh[4] = {0, 0, 0, 0}, rzl[4] = {0, 0, 0, 0}, rzh[4] = {0, 0, 0, 0};
-  T A0 = T(0), D0 = T(0);
-  if (EX != 0) {
-    const long iL = g.idx(g.lx(ix0 - 1), iy, iz), iR = g.idx(g.hx(ix0 + 4), iy, iz);
-    const long iYl = g.idx(ix0, g.ly(iy - 1), iz), iYh = g.idx(ix0, g.hy(iy + 1), iz);
+  for (int c = 0; c < 3; c++) ld4(a.m[c] + I0, q.mc[c]);
+#pragma unroll
+  for (int w = 0; w < 4; w++) q.rc[w] = q.ryl[w] = q.ryh[w] = q.rzl[w] = q.rzh[w] = 0;
+  q.rxl = q.rxr = 0;
+  q.A0 = q.D0 = T(0);
+  if (REG) ld4u8(a.regions + I0, q.rc);
+  if (EX == 0) return;
+  const long iL = g.idx(g.lx(ix0 - 1), iy, iz), iR = g.idx(g.hx(ix0 + 4), iy, iz);
+  const long iYl = g.idx(ix0, g.ly(iy - 1), iz), iYh = g.idx(ix0, g.hy(iy + 1), iz);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    q.mxl[c] = a.m[c][iL];
+    q.mxr[c] = a.m[c][iR];
+    ld4(a.m[c] + iYl, q.myl[c]);
+    ld4(a.m[c] + iYh, q.myh[c]);
+  }
+  if (REG) {
+    q.rxl = a.regions[iL];
+    q.rxr = a.regions[iR];
+    ld4u8(a.regions + iYl, q.ryl);
+    ld4u8(a.regions + iYh, q.ryh);
+  }
+  if (g.Nz != 1) {
+    const long iZl = g.idx(ix0, iy, g.lz(iz - 1)), iZh = g.idx(ix0, iy, g.hz(iz + 1));
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      mxl[c] = a.m[c][iL];
-      mxr[c] = a.m[c][iR];
-      ld4(a.m[c] + iYl, myl[c]);
-      ld4(a.m[c] + iYh, myh[c]);
+      ld4(a.m[c] + iZl, q.mzl[c]);
+      ld4(a.m[c] + iZh, q.mzh[c]);
     }
     if (REG) {
-      rxl = a.regions[iL];
-      rxr = a.regions[iR];
-      ld4u8(a.regions + iYl, ryl);
-      ld4u8(a.regions + iYh, ryh);
-    }
-    if (nnb == 6) {
-      const long iZl = g.idx(ix0, iy, g.lz(iz - 1)), iZh = g.idx(ix0, iy, g.hz(iz + 1));
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        ld4(a.m[c] + iZl, mzl[c]);
-        ld4(a.m[c] + iZh, mzh[c]);
-      }
-      if (REG) {
-        ld4u8(a.regions + iZl, rzl);
-        ld4u8(a.regions + iZh, rzh);
-      }
-    }
-    if (!REG) {
-      A0 = a.aLUT[0];
-      if (EX == 2) D0 = a.dLUT[0];
+      ld4u8(a.regions + iZl, q.rzl);
+      ld4u8(a.regions + iZh, q.rzh);
     }
   }
+  if (!REG) {
+    q.A0 = a.aLUT[0];
+    if (EX == 2) q.D0 = a.dLUT[0];
+  }
+}
+
+// Effective field and torque of the group from its loaded inputs; bdv: demag
+// field of the four cells (zero when absent).  Same term order as k_field_torque.
+template <typename T, bool REG, int EX>
+__device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int iy, int iz, const G4In<T> &q,
+                                           const T (&bdv)[3][4]) {
+  const Mesh &g = a.g;
+  const long I0 = g.idx(ix0, iy, iz);
+  const int nnb = g.Nz != 1 ? 6 : 4;
   const bool inxl = (ix0 - 1 >= 0) || (g.pbc & 1), inxr = (ix0 + 4 < g.Nx) || (g.pbc & 1);
   const bool inyl = (iy - 1 >= 0) || (g.pbc & 2), inyh = (iy + 1 < g.Ny) || (g.pbc & 2);
   T ob[3][4], ot[3][4];
 #pragma unroll
   for (int w = 0; w < 4; w++) {
     const long I = I0 + w;
-    const V3<T> m0 = v3(mc[0][w], mc[1][w], mc[2][w]);
+    const V3<T> m0 = v3(q.mc[0][w], q.mc[1][w], q.mc[2][w]);
     V3<T> B = v3(bdv[0][w], bdv[1][w], bdv[2][w]);
     if (EX != 0 && !is0(m0)) {
       V3<T> mn[6];
       int rn[6];
-      mn[0] = w > 0 ? v3(mc[0][w - 1], mc[1][w - 1], mc[2][w - 1]) : v3(mxl[0], mxl[1], mxl[2]);
-      rn[0] = w > 0 ? rc[w - 1] : rxl;
-      mn[1] = w < 3 ? v3(mc[0][w + 1], mc[1][w + 1], mc[2][w + 1]) : v3(mxr[0], mxr[1], mxr[2]);
-      rn[1] = w < 3 ? rc[w + 1] : rxr;
-      mn[2] = v3(myl[0][w], myl[1][w], myl[2][w]);
-      rn[2] = ryl[w];
-      mn[3] = v3(myh[0][w], myh[1][w], myh[2][w]);
-      rn[3] = ryh[w];
+      mn[0] = w > 0 ? v3(q.mc[0][w - 1], q.mc[1][w - 1], q.mc[2][w - 1]) : v3(q.mxl[0], q.mxl[1], q.mxl[2]);
+      rn[0] = w > 0 ? q.rc[w - 1] : q.rxl;
+      mn[1] = w < 3 ? v3(q.mc[0][w + 1], q.mc[1][w + 1], q.mc[2][w + 1]) : v3(q.mxr[0], q.mxr[1], q.mxr[2]);
+      rn[1] = w < 3 ? q.rc[w + 1] : q.rxr;
+      mn[2] = v3(q.myl[0][w], q.myl[1][w], q.myl[2][w]);
+      rn[2] = q.ryl[w];
+      mn[3] = v3(q.myh[0][w], q.myh[1][w], q.myh[2][w]);
+      rn[3] = q.ryh[w];
       if (nnb == 6) {
-        mn[4] = v3(mzl[0][w], mzl[1][w], mzl[2][w]);
-        rn[4] = rzl[w];
-        mn[5] = v3(mzh[0][w], mzh[1][w], mzh[2][w]);
-        rn[5] = rzh[w];
+        mn[4] = v3(q.mzl[0][w], q.mzl[1][w], q.mzl[2][w]);
+        rn[4] = q.rzl[w];
+        mn[5] = v3(q.mzh[0][w], q.mzh[1][w], q.mzh[2][w]);
+        rn[5] = q.rzh[w];
       } else {
         mn[4] = mn[5] = m0;
-        rn[4] = rn[5] = rc[w];
+        rn[4] = rn[5] = q.rc[w];
       }
       V3<T> h;
       if (EX == 1) {
-        h = exchange_nb<T, REG>(m0, mn, nnb, rc[w], rn, a.aLUT, A0, a.wx, a.wy, a.wz);
+        h = exchange_nb<T, REG>(m0, mn, nnb, q.rc[w], rn, a.aLUT, q.A0, a.wx, a.wy, a.wz);
       } else {
         const V3<T> zero = v3(T(0), T(0), T(0));
         if (w == 0 && !inxl) mn[0] = zero;
         if (w == 3 && !inxr) mn[1] = zero;
         if (!inyl) mn[2] = zero;
         if (!inyh) mn[3] = zero;
-        h = dmi_nb<T, REG>(m0, mn, nnb, rc[w], rn, a.aLUT, a.dLUT, A0, D0, a.cx, a.cy, a.cz, a.openbc);
+        h = dmi_nb<T, REG>(m0, mn, nnb, q.rc[w], rn, a.aLUT, a.dLUT, q.A0, q.D0, a.cx, a.cy, a.cz, a.openbc);
       }
       const T invMs = inv_msat(a.Ms, I);
       B.x += h.x * invMs;
@@ -461,7 +474,7 @@ __device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int
       B.x += h.x; B.y += h.y; B.z += h.z;
     }
     if (a.bextLUT[0]) {
-      const int r = rc[w];
+      const int r = q.rc[w];
       B.x += a.bextLUT[0][r];
       B.y += a.bextLUT[1][r];
       B.z += a.bextLUT[2][r];
@@ -483,7 +496,7 @@ __device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int
       const V3<T> c = vcross(m0, vcross(m0, B));
       t = v3(-c.x, -c.y, -c.z);
     }
-    if (a.frozenLUT && a.frozenLUT[rc[w]] != T(0)) t = v3(T(0), T(0), T(0));
+    if (a.frozenLUT && a.frozenLUT[q.rc[w]] != T(0)) t = v3(T(0), T(0), T(0));
     ot[0][w] = t.x; ot[1][w] = t.y; ot[2][w] = t.z;
   }
   if (a.beff[0]) {
@@ -494,6 +507,13 @@ __device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int
 #pragma unroll
     for (int c = 0; c < 3; c++) st4(a.tau[c] + I0, ot[c]);
   }
+}
+
+template <typename T, bool REG, int EX>
+__device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int iy, int iz, const T (&bdv)[3][4]) {
+  G4In<T> q;
+  g4_load<T, REG, EX>(a, ix0, iy, iz, q);
+  g4_compute<T, REG, EX>(a, ix0, iy, iz, q, bdv);
 }
 
 template <typename T> void launch_field_torque(const FieldArgs<T> &a, hipStream_t s);

@@ -289,15 +289,22 @@ struct Sim : SimBase {
       e2 = next_event();
       UM_HIP(hipEventRecord(e0, st));
     }
-    compute_demag(m);
-    if (profiling) UM_HIP(hipEventRecord(e1, st));
     FieldArgs<T> a = field_args(m);
     for (int c = 0; c < 3; c++) {
       a.bd[c] = enable_demag ? bd + c * n : nullptr;
       a.tau[c] = dst + c * n;
       a.beff[c] = nullptr;
     }
-    launch_field_torque(a, st);
+    bool fused = false;
+    if (enable_demag) {
+      // one pipeline: B_demag stays on chip (the field share is timed with the demag)
+      ensure_demag();
+      const void *mv[3] = {m, m + n, m + 2 * n};
+      fused = demag_exec_field(demag, mv, vol.p, p[UM_P_MSAT][0].arr.p, p[UM_P_MSAT][0].mul, &a, st);
+      if (!fused) compute_demag(m);
+    }
+    if (profiling) UM_HIP(hipEventRecord(e1, st));
+    if (!fused) launch_field_torque(a, st);
     if (profiling) UM_HIP(hipEventRecord(e2, st));
     nevals++;
   }

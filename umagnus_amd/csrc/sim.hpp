@@ -13,6 +13,10 @@ void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3]
 void pad_size(const int N[3], const int pbc[3], int P[3]);
 void demag_exec(DemagBase *d, void *const B[3], const void *const m[3], const void *vol, const void *Ms,
                 double Ms_mul, hipStream_t s);
+// Demag + effective field + torque in one pipeline; fa: FieldArgs of the
+// plan's precision (bd unused).  false when the plan has no fused path.
+bool demag_exec_field(DemagBase *d, const void *const m[3], const void *vol, const void *Ms, double Ms_mul,
+                      const void *fa, hipStream_t s);
 void demag_destroy(DemagBase *d);
 void demag_kspec(DemagBase *d, int c, void *host);
 void demag_info(DemagBase *d, int P[3], int *prec);
