@@ -52,6 +52,21 @@ def algorithmic_bytes(N, s=4):
     return b_eval, b_step
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_512x512x32_f32_rk45dp.md")
+
+
+def measured_traffic(N):
+    """HBM bytes per evaluation from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a copy kernel), for the default
+    workload only; None when absent."""
+    if tuple(N) != (512, 512, 32) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_eval:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def seeded_m(N, seed=2):
     rng = np.random.default_rng(seed)
     m = rng.standard_normal((3, N[2], N[1], N[0])).astype(np.float32)
@@ -157,13 +172,13 @@ def main():
         },
         "roofline": {
             "kernel": "one effective-field + torque evaluation: demag FFT passes (X-fwd R2C, Y-fwd, "
-                      "Z fwd+tensor+inv, Y-inv, X-inv C2R) + fused field/torque kernel",
+                      "Z fwd+tensor+inv, Y-inv, X-inv C2R) + four-cell field/torque kernel",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(N),
             "bytes_per_launch": b_eval,
             "avg_launch_ms": round(eval_s * 1e3, 4),
             "demag_ms": round(t_demag / max(n_evals, 1), 4),
