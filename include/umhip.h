@@ -213,6 +213,31 @@ int um_sim_stream(void *sim, void **hip_stream);
 int um_sim_set_profiling(void *sim, int on);
 int um_sim_kernel_times(void *sim, double out_ms[3], int *n_evals);
 
+/* ---- HIP graphs and ensembles (no counterpart in the reference, whose steps
+ * are enqueued call by call; SURVEY-listed um_graph_* of the boundary) ------- */
+/* With FixDt set, um_sim_steps runs one step eagerly, then replays a captured
+ * two-step HIP graph (no host synchronisation per step: the step statistics
+ * and the thermal Philox step counter stay on the device) and reads the
+ * statistics back once.  Results are identical to eager stepping.  The graph
+ * is re-captured after any parameter or solver change.  Adaptive steps
+ * (FixDt = 0) always run eagerly. */
+int um_sim_set_graph(void *sim, int on);
+/* Ensemble stepping, the batched form of the reference's job queue
+ * (cmd/uMagNUS/queue.go:86-151): n steps of every simulation in sims[], each on
+ * its own stream, with the graph replays of the members interleaved so small
+ * grids run concurrently.  Members in graph mode with FixDt replay graphs;
+ * others step eagerly in turn. */
+int um_sims_steps(int nsims, void *const *sims, int n);
+/* Generic capture of the library stream (um_set_stream; a stream other than
+ * the NULL stream, which HIP cannot capture): every um_* call between begin
+ * and end is recorded instead of run; launch replays it on the library
+ * stream.  Calls that read results back to the host (um_reduce_*, copies to
+ * host) fail while capturing. */
+int um_graph_begin(void);
+int um_graph_end(void **graph);
+int um_graph_launch(void *graph);
+int um_graph_destroy(void *graph);
+
 #ifdef __cplusplus
 }
 #endif

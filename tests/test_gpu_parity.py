@@ -453,3 +453,96 @@ def test_config2_full_size_properties(gpu):
     s32.steps(3)
     mm = s32.get_m()
     assert np.abs(np.linalg.norm(mm, axis=0) - 1).max() < 1e-6
+
+
+# ---- HIP graph replay and ensembles ----------------------------------------------------------
+def _graph_pair(gpu, N, solver, prec, temp, seed=7, stream=0):
+    dt = np.float32 if prec == 0 else np.float64
+    sims = []
+    for _ in range(2):
+        s = gpu.Sim(N, (2e-9, 2e-9, 2e-9), dtype=dt)
+        setup(s, gpu, Ms=8e5, Aex=1.3e-11, Dind=1e-3, Ku1=2e5, alpha=0.05, Bext=(0.0, 0.01, 0.0),
+              temp=temp, seed=seed, dtype=dt)
+        s.set_thermal_seed(seed, stream)
+        s.set_solver(solver)
+        s.set_solver_params(dt=1e-14, fixdt=2e-14)
+        s.set_m(random_unit((N[2], N[1], N[0]), 31))
+        sims.append(s)
+    sims[1].set_graph(1)
+    return sims
+
+
+def _same_state(a, b):
+    sa, sb = a.state(), b.state()
+    for key in ("t", "nsteps", "nevals", "lasterr", "peakerr", "lasttorque"):
+        if key in sa:
+            assert sa[key] == sb[key], (key, sa[key], sb[key])
+    assert np.array_equal(a.get_m(), b.get_m())
+
+
+@pytest.mark.parametrize("solver", [5, 2, 4])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_graph_steps_match_eager(gpu, solver, prec):
+    """Fixed-dt steps replayed from a captured two-step graph are bit-identical to
+    eager steps (m and the step statistics), across chunks of odd length (the
+    pointer parity changes, forcing a re-capture), a parameter change between
+    chunks (stale graph) and T > 0 (the Philox step counter read on the device)."""
+    _say("graph", solver, prec)
+    eager, graph = _graph_pair(gpu, (32, 16, 2), solver, prec, temp=300.0)
+    for chunk in (5, 4, 7, 2):
+        eager.steps(chunk)
+        graph.steps(chunk)
+        _same_state(eager, graph)
+    for s in (eager, graph):
+        s.set_scalar(gpu.P_ALPHA, 0.2)
+    eager.steps(6)
+    graph.steps(6)
+    _same_state(eager, graph)
+
+
+def test_sims_steps_ensemble(gpu):
+    """um_sims_steps over members of different sizes (Heun, T = 300 K, one thermal
+    stream per member) equals stepping each member on its own."""
+    sizes = [(32, 32, 1), (48, 32, 3), (64, 16, 2), (16, 48, 4)]
+    batch, single = [], []
+    for i, N in enumerate(sizes):
+        a, b = _graph_pair(gpu, N, 2, 0, temp=300.0, seed=11, stream=i)
+        single.append(a)
+        batch.append(b)
+    for chunk in (9, 6):
+        gpu.sims_steps(batch, chunk)
+        for s in single:
+            s.steps(chunk)
+        for a, b in zip(single, batch):
+            _same_state(a, b)
+
+
+def test_generic_graph_capture(gpu):
+    """um_graph_begin/end/launch on the library stream: a captured Madd chain,
+    replayed three times with eager work and synchronisation in between, equals
+    the same chain run eagerly each time."""
+    import ctypes as C
+    import torch
+    s = torch.cuda.Stream()
+    gpu._call("um_set_stream", C.c_void_p(s.cuda_stream))
+    try:
+        n = 4099
+        x = torch.rand(n, device="cuda", dtype=torch.float32)
+        y = torch.rand(n, device="cuda", dtype=torch.float32)
+        acc = torch.zeros(n, device="cuda", dtype=torch.float32)
+        ref = acc.clone()
+        torch.cuda.synchronize()
+        with gpu.Graph() as g:
+            gpu.Madd(acc, [acc, x, y], [1.0, 0.5, -0.25])
+            gpu.Madd(acc, [acc, y], [0.75, 0.125])
+        for _ in range(3):
+            g.launch()
+            gpu._call("um_sync")
+            gpu.Madd(ref, [ref, x, y], [1.0, 0.5, -0.25])
+            gpu.Madd(ref, [ref, y], [0.75, 0.125])
+            gpu._call("um_sync")
+            assert torch.equal(acc, ref)
+            x.mul_(0.5)   # eager work between replays
+            torch.cuda.synchronize()
+    finally:
+        gpu._call("um_set_stream", C.c_void_p(torch.cuda.current_stream().cuda_stream))

@@ -1,4 +1,5 @@
 // C ABI of libumhip.so (declared in include/umhip.h).
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 
@@ -588,6 +589,57 @@ int um_sim_stream(void *sim, void **s) { return guard([&] { SIMCHK(sim); *s = (v
 int um_sim_set_profiling(void *sim, int on) { return guard([&] { SIMCHK(sim); SIM(sim)->set_profiling(on); }); }
 int um_sim_kernel_times(void *sim, double out_ms[3], int *n_evals) {
   return guard([&] { SIMCHK(sim); SIM(sim)->kernel_times(out_ms, n_evals); });
+}
+
+int um_sim_set_graph(void *sim, int on) { return guard([&] { SIMCHK(sim); SIM(sim)->set_graph(on); }); }
+
+int um_sims_steps(int nsims, void *const *sims, int n) {
+  return guard([&] {
+    UM_REQUIRE(nsims >= 0 && n >= 0 && (nsims == 0 || sims), UM_EINVAL, "um_sims_steps: bad arguments");
+    for (int i = 0; i < nsims; i++) SIMCHK(sims[i]);
+    std::vector<long> stop(nsims), reps(nsims);
+    long most = 0;
+    for (int i = 0; i < nsims; i++) {
+      stop[i] = SIM(sims[i])->step_count() + n;
+      reps[i] = SIM(sims[i])->graph_prepare(stop[i]);
+      most = std::max(most, reps[i]);
+    }
+    // round-robin replays: every member's stream has work queued at once
+    for (long r = 0; r < most; r++)
+      for (int i = 0; i < nsims; i++)
+        if (r < reps[i]) SIM(sims[i])->graph_replay();
+    for (int i = 0; i < nsims; i++) SIM(sims[i])->graph_finish(stop[i]);
+  });
+}
+
+int um_graph_begin(void) {
+  return guard([&] {
+    ensure_init();
+    UM_HIP(hipStreamBeginCapture(lib_stream(), hipStreamCaptureModeRelaxed));
+  });
+}
+int um_graph_end(void **graph) {
+  return guard([&] {
+    UM_REQUIRE(graph, UM_EINVAL, "graph out-pointer is NULL");
+    hipGraph_t g = nullptr;
+    UM_HIP(hipStreamEndCapture(lib_stream(), &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    UM_HIP(e);
+    *graph = (void *)ex;
+  });
+}
+int um_graph_launch(void *graph) {
+  return guard([&] {
+    UM_REQUIRE(graph, UM_EINVAL, "graph is NULL");
+    UM_HIP(hipGraphLaunch((hipGraphExec_t)graph, lib_stream()));
+  });
+}
+int um_graph_destroy(void *graph) {
+  return guard([&] {
+    if (graph) UM_HIP(hipGraphExecDestroy((hipGraphExec_t)graph));
+  });
 }
 
 }  // extern "C"

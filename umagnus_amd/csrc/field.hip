@@ -166,7 +166,8 @@ __global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
     B.z += a.bext[2].at(I);
   }
   if (a.has_therm) {
-    const V3<T> h = thermal_cell(I, a.seed, a.step, a.stream, a.kth, a.Ms, a.temp, a.alpha);
+    const uint64_t stp = a.step_dev ? (uint64_t)*a.step_dev : a.step;
+    const V3<T> h = thermal_cell(I, a.seed, stp, a.stream, a.kth, a.Ms, a.temp, a.alpha);
     B.x += h.x; B.y += h.y; B.z += h.z;
   }
   if (a.beff[0]) {

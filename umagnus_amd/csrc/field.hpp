@@ -246,6 +246,7 @@ struct FieldArgs {
   Par<T> temp;
   T kth;                 // 2 kB / (gamma V dt)
   uint64_t seed, step;
+  const unsigned long long *step_dev;   // when set, the Philox step counter is read here (graph replays)
   uint32_t stream;
   Par<T> alpha;
   int precess;
@@ -485,7 +486,8 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
       B.z += a.bext[2].at(I);
     }
     if (a.has_therm) {
-      const V3<T> h = thermal_cell(I, a.seed, a.step, a.stream, a.kth, a.Ms, a.temp, a.alpha);
+      const uint64_t stp = a.step_dev ? (uint64_t)*a.step_dev : a.step;
+      const V3<T> h = thermal_cell(I, a.seed, stp, a.stream, a.kth, a.Ms, a.temp, a.alpha);
       B.x += h.x; B.y += h.y; B.z += h.z;
     }
     ob[0][w] = B.x; ob[1][w] = B.y; ob[2][w] = B.z;

@@ -53,6 +53,21 @@ struct Sim : SimBase {
   long nsteps = 0, nundone = 0, nevals = 0;
   bool k1_valid = false;
 
+  // graph replay of fixed-dt steps: two steps are captured once (the pointer
+  // swaps of a step undo themselves after two) and replayed; the step
+  // statistics and the Philox step counter live on the device meanwhile
+  int graph_mode = 0;
+  bool capturing = false;
+  long gen = 0;                  // bumped by every setter: a captured graph is stale
+  DevBuf gstat;                  // {lastErr, peakErr, lastTorque, step counter}
+  double *hstat = nullptr;       // pinned mirror
+  hipGraphExec_t gexec = nullptr;
+  long g_gen = -1;
+  int g_solver = -1;
+  double g_dt = 0;
+  const void *g_ptrs[4] = {};
+  long g_evals = 0;
+
   // profiling: HIP events around the demag and field kernels of every
   // evaluation, recorded without synchronising and read after the step's
   // own readback has drained the stream
@@ -87,11 +102,15 @@ struct Sim : SimBase {
     fill_bytes(aLUT.p, aLUT.bytes, st);
     fill_bytes(dLUT.p, dLUT.bytes, st);
     for (auto &e : step_ev) UM_HIP(hipEventCreate(&e));
+    gstat.alloc(4 * sizeof(double));
+    UM_HIP(hipHostMalloc((void **)&hstat, 4 * sizeof(double), hipHostMallocDefault));
     UM_HIP(hipStreamSynchronize(st));
   }
   ~Sim() override {
     if (st) (void)hipStreamSynchronize(st);
     if (demag) demag_destroy(demag);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (hstat) (void)hipHostFree(hstat);
     if (hres) (void)hipHostFree(hres);
     if (hsum) (void)hipHostFree(hsum);
     for (auto &e : evpool) (void)hipEventDestroy(e);
@@ -121,11 +140,13 @@ struct Sim : SimBase {
     }
   }
   void set_scalar(int id, const void *arr, double mul) override {
+    gen++;
     UM_REQUIRE(id >= 0 && id < UM_P_COUNT, UM_EINVAL, "parameter id");
     set_slot(p[id][0], arr, mul);
     if (id == UM_P_MSAT) invalidate_fsal();
   }
   void set_vector(int id, const void *const a[3], const double mul[3]) override {
+    gen++;
     UM_REQUIRE(id >= 0 && id < UM_P_COUNT, UM_EINVAL, "parameter id");
     for (int c = 0; c < 3; c++) set_slot(p[id][c], a[c], mul[c]);
   }
@@ -135,10 +156,12 @@ struct Sim : SimBase {
     return false;
   }
   void set_regions(const uint8_t *r) override {
+    gen++;
     if (r) { upload(regions, r, (size_t)n); has_regions = true; }
     else { regions.release(); has_regions = false; }
   }
   void set_exchange_luts(const void *a, const void *d) override {
+    gen++;
     upload(aLUT, a, sizeof(T) * SYMLUT);
     has_dmi = false;
     if (d) {
@@ -147,6 +170,7 @@ struct Sim : SimBase {
     }
   }
   void set_bext_lut(const void *const l[3]) override {
+    gen++;
     if (!l[0]) { bextLUT.release(); has_bextlut = false; return; }
     std::vector<T> h(3 * NREGION);
     for (int c = 0; c < 3; c++) memcpy(h.data() + c * NREGION, l[c], sizeof(T) * NREGION);
@@ -154,22 +178,26 @@ struct Sim : SimBase {
     has_bextlut = any_nonzero(h.data(), 3 * NREGION);
   }
   void set_frozen_lut(const void *l) override {
+    gen++;
     if (!l) { frozenLUT.release(); has_frozen = false; return; }
     upload(frozenLUT, l, sizeof(T) * NREGION);
     has_frozen = any_nonzero((const T *)l, NREGION);
   }
   void set_geometry(const void *v) override {
+    gen++;
     if (v) { upload(vol, v, sizeof(T) * n); has_vol = true; }
     else { vol.release(); has_vol = false; }
   }
   void set_flags(int d, int ob, int pr, int hs) override {
+    gen++;
     enable_demag = d; openbc = ob; precess = pr; heun_sqrt3 = hs;
   }
   void set_demag_accuracy(double a) override {
+    gen++;
     if (a != demag_acc && demag) { demag_destroy(demag); demag = nullptr; }
     demag_acc = a;
   }
-  void set_thermal_seed(uint64_t s, uint32_t strm) override { seed = s; tstream = strm; }
+  void set_thermal_seed(uint64_t s, uint32_t strm) override { gen++; seed = s; tstream = strm; }
   void invalidate_fsal() { k1_valid = false; }
 
   // ---- m --------------------------------------------------------------------------
@@ -245,6 +273,7 @@ struct Sim : SimBase {
     a.kth = (T)(2 * KB / (gammaLL * cs[0] * cs[1] * cs[2] * dt));
     a.seed = seed;
     a.step = (uint64_t)nsteps;
+    a.step_dev = capturing ? (const unsigned long long *)(gstat.as<double>() + 3) : nullptr;
     a.stream = tstream;
     a.alpha = P(UM_P_ALPHA);
     a.precess = precess;
@@ -342,12 +371,14 @@ struct Sim : SimBase {
 
   // ---- solver ------------------------------------------------------------------------
   void set_solver(int s) override {
+    gen++;
     UM_REQUIRE(s == UM_SOLVER_HEUN || s == UM_SOLVER_RK4 || s == UM_SOLVER_RK45DP, UM_EUNSUP,
                "solver " + std::to_string(s) + " not available (Heun=2, RK4=4, RK45DP=5)");
     solver = s;
     k1_valid = false;
   }
   void set_solver_params(const double kk[9]) override {
+    gen++;
     dt = kk[0]; fixdt = kk[1]; maxerr = kk[2]; relerr = kk[3]; minslope = kk[4];
     headroom = kk[5]; mindt = kk[6]; maxdt = kk[7];
     if (kk[8] != gammaLL) k1_valid = false;
@@ -403,6 +434,10 @@ struct Sim : SimBase {
     a.N = n;
     a.res = (unsigned long long *)red.p;
     launch_err(a, st);
+    if (capturing) {   // the statistics are folded on the device (step_stats); decisions are fixed by FixDt
+      out[0] = out[1] = out[2] = 0;
+      return;
+    }
     UM_HIP(hipMemcpyAsync(hres, red.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     UM_HIP(hipStreamSynchronize(st));
     for (int i = 0; i < 3; i++) memcpy(&out[i], &hres[i], 8);
@@ -478,8 +513,11 @@ struct Sim : SimBase {
       if (maxdm < minslope) fail = 0;
       else { rlerr = r[2]; fail = 1; }
       if (fail == 0 || relerr <= 0.0 || rlerr < relerr || dt <= mindt || fixdt != 0) {
-        set_last_err(err);
-        lastTorque = maxdm;
+        if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), (double)h, 0, st);
+        else {
+          set_last_err(err);
+          lastTorque = maxdm;
+        }
         nsteps++;
         Time = t0 + dt;
         if (fail == 0) adapt(std::pow(maxerr / err, 1. / 5));
@@ -519,17 +557,19 @@ struct Sim : SimBase {
     const T *src[2] = {dy0, dy};
     err_reduce(1, src, 2, nullptr, r);
     double err = std::sqrt(heun_sqrt3 ? 3.0 * r[0] : r[0]) * (double)h;
+    if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), (double)h, heun_sqrt3 ? 3 : 1, st);
     if (err < maxerr || dt <= mindt || fixdt != 0) {
       const T *s[] = {m, dy, dy0};
       const T f[] = {T(1), T(0.5) * h, T(-0.5) * h};
       stage(m, 3, s, f, true, true);
       nsteps++;
       adapt(std::pow(maxerr / err, 1. / 2.));
-      set_last_err(err);
+      if (!capturing) set_last_err(err);
       const T *sn[1] = {dy};
       double r2[3];
       err_reduce(2, sn, 1, nullptr, r2);
-      lastTorque = std::sqrt(r2[0]);
+      if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), 0.0, 2, st);
+      else lastTorque = std::sqrt(r2[0]);
     } else {
       Time -= dt;
       std::swap(m, m0);
@@ -558,11 +598,12 @@ struct Sim : SimBase {
     const T *src[2] = {k1, k4};
     err_reduce(1, src, 2, nullptr, r);
     const double err = std::sqrt(r[0]) * (double)h;
+    if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), (double)h, 1, st);
     bool ok;
     if (err < maxerr || dt <= mindt || fixdt != 0) {
       nsteps++;
       adapt(std::pow(maxerr / err, 1. / 4.));
-      set_last_err(err);
+      if (!capturing) set_last_err(err);
       ok = true;
     } else {
       nundone++;
@@ -576,7 +617,8 @@ struct Sim : SimBase {
       const T *sn[1] = {k4};
       double r2[3];
       err_reduce(2, sn, 1, nullptr, r2);
-      lastTorque = std::sqrt(r2[0]);
+      if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), 0.0, 2, st);
+      else lastTorque = std::sqrt(r2[0]);
     } else {
       Time = t0;
       std::swap(m, m0);
@@ -590,6 +632,7 @@ struct Sim : SimBase {
       case UM_SOLVER_RK4: step_rk4(); break;
       default: step_rk45dp(); break;
     }
+    if (capturing) launch_step_count((unsigned long long *)(gstat.as<double>() + 3), st);
     if (profiling) {
       UM_HIP(hipEventRecord(step_ev[1], st));
       UM_HIP(hipEventSynchronize(step_ev[1]));
@@ -603,6 +646,74 @@ struct Sim : SimBase {
   void steps(int cnt) override {
     k1_valid = false;
     const long stop = nsteps + cnt;
+    long reps = graph_prepare(stop);
+    while (reps-- > 0) graph_replay();
+    graph_finish(stop);
+  }
+
+  // ---- graph replay (FixDt only) ---------------------------------------------------------
+  void set_graph(int on) override { graph_mode = on; }
+  long step_count() const override { return nsteps; }
+  bool graph_usable() const { return graph_mode && fixdt != 0 && !profiling; }
+  // one eager step (FSAL setup), then capture two steps if the cached graph is
+  // stale; returns how many two-step replays fit before `stop`
+  long graph_prepare(long stop) override {
+    if (!graph_usable() || nsteps >= stop) return 0;
+    step();
+    if (nsteps + 2 > stop) return 0;
+    const void *ptrs[4] = {m, m0, k[0], k[1]};
+    bool fresh = gexec && g_gen == gen && g_solver == solver && g_dt == fixdt;
+    for (int i = 0; i < 4 && fresh; i++) fresh = (g_ptrs[i] == ptrs[i]);
+    if (!fresh) capture2();
+    hstat[0] = lastErr; hstat[1] = peakErr; hstat[2] = lastTorque;
+    const unsigned long long ns = (unsigned long long)nsteps;
+    std::memcpy(&hstat[3], &ns, 8);
+    UM_HIP(hipMemcpyAsync(gstat.p, hstat, 4 * sizeof(double), hipMemcpyHostToDevice, st));
+    return (stop - nsteps) / 2;
+  }
+  void capture2() {
+    if (gexec) { UM_HIP(hipGraphExecDestroy(gexec)); gexec = nullptr; }
+    ensure_demag();
+    UM_HIP(hipStreamSynchronize(st));
+    const double t0 = Time;
+    const long s0 = nsteps, e0 = nevals, u0 = nundone;
+    hipGraph_t g = nullptr;
+    UM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    capturing = true;
+    try {
+      step();
+      step();
+    } catch (...) {
+      capturing = false;
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(st, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      throw;
+    }
+    capturing = false;
+    UM_HIP(hipStreamEndCapture(st, &g));
+    hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    UM_HIP(e);
+    g_evals = nevals - e0;
+    Time = t0; nsteps = s0; nevals = e0; nundone = u0;   // capture executed nothing
+    g_gen = gen; g_solver = solver; g_dt = fixdt;
+    const void *ptrs[4] = {m, m0, k[0], k[1]};
+    for (int i = 0; i < 4; i++) g_ptrs[i] = ptrs[i];
+  }
+  void graph_replay() override {
+    UM_HIP(hipGraphLaunch(gexec, st));
+    for (int i = 0; i < 2; i++) Time = Time + dt;
+    nsteps += 2;
+    nevals += g_evals;
+  }
+  // device statistics back to the host, then the remaining steps eagerly
+  void graph_finish(long stop) override {
+    if (graph_usable() && gexec) {
+      UM_HIP(hipMemcpyAsync(hstat, gstat.p, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+      UM_HIP(hipStreamSynchronize(st));
+      lastErr = hstat[0]; peakErr = hstat[1]; lastTorque = hstat[2];
+    }
     while (nsteps < stop) step();
     UM_HIP(hipStreamSynchronize(st));
   }

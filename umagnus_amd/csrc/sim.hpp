@@ -49,6 +49,11 @@ struct SimBase {
   virtual double energy(int term) = 0;
   virtual hipStream_t stream() = 0;
   virtual void set_profiling(int on) = 0;
+  virtual void set_graph(int on) = 0;
+  virtual long graph_prepare(long stop) = 0;
+  virtual void graph_replay() = 0;
+  virtual void graph_finish(long stop) = 0;
+  virtual long step_count() const = 0;
   virtual void kernel_times(double ms[3], int *nevals) = 0;
 };
 

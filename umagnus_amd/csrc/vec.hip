@@ -348,6 +348,31 @@ void launch_err(const ErrArgs<T> &a, hipStream_t s) {
 template void launch_err<float>(const ErrArgs<float> &, hipStream_t);
 template void launch_err<double>(const ErrArgs<double> &, hipStream_t);
 
+// ---- step statistics on the device ----------------------------------------------------
+__global__ void k_step_stats(const unsigned long long *res, double *gs, double h, int mode) {
+  if (threadIdx.x != 0) return;
+  const double r0 = __longlong_as_double((long long)res[0]);
+  if (mode == 2) {
+    gs[2] = sqrt(r0);
+    return;
+  }
+  const double err = sqrt(mode == 3 ? 3.0 * r0 : r0) * h;
+  gs[0] = err;
+  if (err > gs[1]) gs[1] = err;
+  if (mode == 0) gs[2] = sqrt(__longlong_as_double((long long)res[1]));
+}
+__global__ void k_step_count(unsigned long long *counter) {
+  if (threadIdx.x == 0) *counter += 1ull;
+}
+void launch_step_stats(const unsigned long long *res, double *gs, double h, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_stats, dim3(1), dim3(64), 0, s, res, gs, h, mode);
+  UM_HIP(hipGetLastError());
+}
+void launch_step_count(unsigned long long *counter, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, s, counter);
+  UM_HIP(hipGetLastError());
+}
+
 // ---- deterministic weighted dot sums ------------------------------------------------
 template <typename T>
 struct WdotArgs {

@@ -41,6 +41,13 @@ struct ErrArgs {
 };
 template <typename T> void launch_err(const ErrArgs<T> &a, hipStream_t s);
 
+// Step statistics folded on the device (graph replays): gs = {lastErr, peakErr,
+// lastTorque, step counter}.  mode 0: RK45DP (err = sqrt(res0) h, torque =
+// sqrt(res1)); 1 / 3: err = sqrt(mode * res0) h (Heun's sqrt3 widening = 3);
+// 2: torque = sqrt(res0).
+void launch_step_stats(const unsigned long long *res, double *gs, double h, int mode, hipStream_t s);
+void launch_step_count(unsigned long long *counter, hipStream_t s);
+
 // Deterministic FP64 sums: out[0] = sum_i w_i * (a_i . b_i) over ncomp
 // components (w nullable = 1, b nullable = 1).  partial: RED_BLOCKS doubles.
 template <typename T>

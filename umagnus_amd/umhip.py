@@ -41,7 +41,8 @@ EXPORTS = [
     "um_sim_get_m", "um_sim_m_device", "um_sim_beff", "um_sim_torque", "um_sim_demag_field",
     "um_sim_set_solver", "um_sim_set_solver_params", "um_sim_get_state", "um_sim_step", "um_sim_steps",
     "um_sim_run", "um_sim_average_m", "um_sim_energy", "um_sim_stream", "um_sim_set_profiling",
-    "um_sim_kernel_times",
+    "um_sim_kernel_times", "um_sim_set_graph", "um_sims_steps", "um_graph_begin", "um_graph_end",
+    "um_graph_launch", "um_graph_destroy",
 ]
 
 
@@ -485,3 +486,42 @@ class Sim:
         ne = C.c_int()
         _call("um_sim_kernel_times", self.h, ms, C.byref(ne))
         return list(ms), ne.value
+
+    def set_graph(self, on):
+        """Replay fixed-dt steps as a captured HIP graph (um_sim_set_graph)."""
+        _call("um_sim_set_graph", self.h, int(on))
+
+
+def sims_steps(sims, n):
+    """n steps of every simulation in `sims`, graph replays interleaved across
+    their streams (um_sims_steps)."""
+    arr = (C.c_void_p * len(sims))(*[s.h for s in sims])
+    _call("um_sims_steps", len(sims), arr, int(n))
+
+
+class Graph:
+    """Capture of the library stream: `with Graph() as g: <um_* calls>`, then
+    g.launch() replays them (um_graph_begin/end/launch/destroy)."""
+
+    def __init__(self):
+        self.h = None
+
+    def __enter__(self):
+        _call("um_graph_begin")
+        return self
+
+    def __exit__(self, et, ev, tb):
+        h = C.c_void_p()
+        rc = load().um_graph_end(C.byref(h))
+        if et is None and rc != 0:
+            raise UmError(f"libumhip error {rc}: {load().um_last_error().decode()}")
+        self.h = h.value
+        return False
+
+    def launch(self):
+        _call("um_graph_launch", C.c_void_p(self.h))
+
+    def __del__(self):
+        if self.h is not None and _lib is not None:
+            _lib.um_graph_destroy(C.c_void_p(self.h))
+            self.h = None
