@@ -6,12 +6,15 @@ seeded uniform-random unit m, RK45DP at fixed dt = 1e-14 s.  A step is one
 RK45DP step (six effective-field + torque evaluations, FSAL) over the whole
 grid with m resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batched]
 
 Prints one JSON line.  `value` is Mcell*steps/s; `roofline` prices one
 effective-field + torque evaluation (the demag FFT passes plus the fused
 field/torque kernel, timed with HIP events on the simulation stream over the
 timed region) at its algorithmic HBM bytes against the 8 TB/s HBM3E peak.
+`batched` compares a 64-member thermal Heun ensemble stepped together
+(graph replays on per-member streams) with the same members stepped one
+after another.
 """
 import argparse
 import json
@@ -85,6 +88,60 @@ def configure(sim, mod, dtype):
     sim.set_solver_params(dt=1e-14, fixdt=1e-14)
 
 
+def ensemble_members(count=64, seed=4):
+    """Grid sizes of a config-4-like ensemble: (Nx, Ny) from {32..128}, Nz 1..4."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice([32, 48, 64, 96, 128])), int(rng.choice([32, 48, 64, 96, 128])),
+             int(rng.integers(1, 5))) for _ in range(count)]
+
+
+def batched(steps=100):
+    """Ensemble throughput: Heun at FixDt 5e-15 s with T = 300 K (one Philox
+    stream per member), 64 members stepped one after another (eager, the
+    reference's queue order) vs together through graph replays (um_sims_steps)."""
+    import torch
+    from umagnus_amd import umhip
+    from umagnus_amd.params import exchange_lut
+    sizes = ensemble_members()
+    cells = sum(a * b * c for a, b, c in sizes)
+
+    def make(graph):
+        sims = []
+        for i, N in enumerate(sizes):
+            s = umhip.Sim(N, (2e-9,) * 3, dtype=np.float32)
+            s.set_scalar(umhip.P_MSAT, 8e5)
+            s.set_scalar(umhip.P_ALPHA, 0.02)
+            s.set_scalar(umhip.P_TEMP, 300.0)
+            s.set_exchange_luts(exchange_lut(1.3e-11, dtype=np.float32), None)
+            s.set_thermal_seed(5, i)
+            s.set_solver(umhip.SOLVER_HEUN)
+            s.set_solver_params(dt=5e-15, fixdt=5e-15)
+            s.set_m(seeded_m(N, 3 + i))
+            s.set_graph(graph)
+            s.steps(2)
+            sims.append(s)
+        return sims
+
+    res = {}
+    for mode in ("plain", "batched"):
+        sims = make(mode == "batched")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if mode == "batched":
+            umhip.sims_steps(sims, steps)
+        else:
+            for s in sims:
+                s.steps(steps)
+        torch.cuda.synchronize()
+        res[mode] = cells * steps / (time.perf_counter() - t0) / 1e6
+        del sims
+        torch.cuda.empty_cache()
+    return {"workload": f"{len(sizes)} instances, {cells} cells in total, (Nx, Ny) in {{32..128}}, Nz 1..4, "
+                        "2 nm, Heun FixDt 5e-15 s, T = 300 K, FP32",
+            "steps": steps, "unit": "Mcell·steps/s", "plain": round(res["plain"], 2),
+            "value": round(res["batched"], 2), "speedup": round(res["batched"] / res["plain"], 3)}
+
+
 def cpu_baseline(N, cs):
     """The FP32 oracle (C, OpenMP) on the host: one RK45DP step of the same
     workload after one warm-up step (setup and tensor excluded)."""
@@ -114,6 +171,7 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -185,9 +243,11 @@ def main():
             "field_torque_ms": round(t_field / max(n_evals, 1), 4),
         },
     }
+    del sim
+    torch.cuda.empty_cache()
+    if not args.no_batched:
+        out["batched"] = batched()
     if not args.no_cpu_baseline:
-        del sim
-        torch.cuda.empty_cache()
         out["cpu_baseline"] = cpu_baseline(N, cs)
     print(json.dumps(out), flush=True)
 

@@ -546,3 +546,13 @@ def test_generic_graph_capture(gpu):
             torch.cuda.synchronize()
     finally:
         gpu._call("um_set_stream", C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_exchange_inhomo_gpu(gpu, prec):
+    """test/exchange_inhomo.mx3 on the HIP path: OVF-loaded m, four regions with
+    harmonic-mean Aex; E_exch equals OOMMF's value to 1e-5, for both Msat maps."""
+    from test_ovf import E_EXCH_OOMMF, run_exchange_inhomo
+    dt = np.float32 if prec == 0 else np.float64
+    for e in run_exchange_inhomo(gpu.Sim, gpu, dt):
+        assert abs(e - E_EXCH_OOMMF) <= 1e-5 * E_EXCH_OOMMF, e
