@@ -154,7 +154,7 @@ enum {
   UM_P_TEMP, UM_P_ANISU, UM_P_ANISC1, UM_P_ANISC2, UM_P_BEXT, UM_P_COUNT
 };
 /* SetSolver ids, engine/run.go:78-90 */
-enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45DP = 5 };
+enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK23 = 3, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45DP = 5 };
 
 /* A simulation owns m, the parameters, the demag plan and the stepper state on
  * one HIP stream (engine/magnetization.go, parameter.go, run.go). */
@@ -201,9 +201,16 @@ int um_sim_step(void *sim);
 int um_sim_steps(void *sim, int n);
 int um_sim_run(void *sim, double seconds);
 /* m.Average() (engine/average.go:28) and energy terms (energy.go):
- * 0 demag, 1 exchange(+DMI), 2 Zeeman, 3 uniaxial K1 */
+ * 0 demag, 1 exchange(+DMI), 2 Zeeman, 3 uniaxial K1, 4 uniaxial K2,
+ * 5..7 cubic Kc1..Kc3 (anisotropy.go:163-207), 8 thermal (temperature.go:129),
+ * 9 total (GetTotalEnergy, energy.go:27-34) */
 int um_sim_average_m(void *sim, double out[3]);
 int um_sim_energy(void *sim, int term, double *out);
+/* Relax (engine/relax.go:24-100): RK23 without precession and without the
+ * thermal field, stepping while the total energy decreases, then while the
+ * torque decreases (torque_threshold <= 0) or until max |torque| is below it,
+ * tightening MaxErr; solver, MaxErr, FixDt and precession are restored. */
+int um_sim_relax(void *sim, double torque_threshold);
 /* the HIP stream the simulation enqueues on */
 int um_sim_stream(void *sim, void **hip_stream);
 /* timing with HIP events on the simulation stream, read at the per-step

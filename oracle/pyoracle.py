@@ -18,7 +18,7 @@ NREGION = 256
 SYMLUT = NREGION * (NREGION + 1) // 2
 
 P_MSAT, P_KU1, P_KU2, P_KC1, P_KC2, P_KC3, P_ALPHA, P_TEMP, P_ANISU, P_ANISC1, P_ANISC2, P_BEXT = range(12)
-SOLVER_HEUN, SOLVER_RK4, SOLVER_RK45DP = 2, 4, 5
+SOLVER_HEUN, SOLVER_RK23, SOLVER_RK4, SOLVER_RK45DP = 2, 3, 4, 5
 
 
 def build():
@@ -288,6 +288,10 @@ class Sim:
 
     def run(self, seconds):
         self._f("sim_run")(self._h(), C.c_double(seconds))
+
+    def relax(self, torque_threshold=-1.0):
+        """Relax(), engine/relax.go:24-100."""
+        self._f("sim_relax")(self._h(), C.c_double(torque_threshold))
 
     def average_m(self):
         o = (C.c_double * 3)()

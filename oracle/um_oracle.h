@@ -117,7 +117,7 @@ enum {
   UM_P_MSAT = 0, UM_P_KU1, UM_P_KU2, UM_P_KC1, UM_P_KC2, UM_P_KC3, UM_P_ALPHA,
   UM_P_TEMP, UM_P_ANISU, UM_P_ANISC1, UM_P_ANISC2, UM_P_BEXT, UM_P_COUNT
 };
-enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45DP = 5 };
+enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK23 = 3, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45DP = 5 };
 
 #define UM_OR_SIM_DECL(SFX, real)                                                          \
   void *um_or_sim_new##SFX(int Nx, int Ny, int Nz, double cx, double cy, double cz,      \
@@ -149,7 +149,10 @@ enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45DP = 5 };
   void um_or_sim_step##SFX(void *h);                                                       \
   void um_or_sim_steps##SFX(void *h, int n);                                               \
   void um_or_sim_run##SFX(void *h, double seconds);                                        \
+  void um_or_sim_relax##SFX(void *h, double torque_threshold);                              \
   void um_or_sim_average_m##SFX(void *h, double out[3]);                                   \
+  /* energy terms: 0 demag, 1 exchange(+DMI), 2 Zeeman, 3 uniaxial K1, 4 uniaxial K2,      \
+   * 5..7 cubic Kc1..Kc3, 8 thermal, 9 total (engine/energy.go:27-34) */                   \
   double um_or_sim_energy##SFX(void *h, int term);
 
 UM_OR_SIM_DECL(_f32, float)

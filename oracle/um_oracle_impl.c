@@ -852,6 +852,7 @@ typedef struct {
   double lastErr, peakErr, lastTorque;
   long nsteps, nundone, nevals;
   real *k1; int k1_valid;
+  int relaxing;                      /* engine/relax.go:22: no thermal field while relaxing */
 } sim_t;
 
 #define MU0_D (4 * M_PI * 1e-7)
@@ -990,6 +991,8 @@ static int param_nonzero(sim_t *s, int id, int nc) {
   return 0;
 }
 #define PA(id, c) s->p[id][c].arr, (real)s->p[id][c].mul
+#define PA_ARR(id) ((const real *)s->p[id][0].arr)
+#define PA_MUL(id) ((real)s->p[id][0].mul)
 
 /* SetDemagField, engine/demag.go:42-65 (NoDemagSpins not modelled) */
 static void set_demag(sim_t *s, real *B) {
@@ -1000,6 +1003,20 @@ static void set_demag(sim_t *s, real *B) {
 }
 
 /* SetEffectiveField, engine/effectivefield.go:14-24 */
+/* B += B_therm of the current step (temperature.go:55-127, temperature2.cl) */
+static void add_thermal(sim_t *s, real *B) {
+  int n = s->n;
+  real *eta = s->tmp[15];
+  OR(thermal_normals)(eta, eta + n, eta + 2 * n, n, s->seed, (uint64_t)s->nsteps, s->stream);
+  double V = s->cs[0] * s->cs[1] * s->cs[2];
+  real k2 = (real)(2 * KB_D / (s->gammaLL * V * s->dt));
+  real *bt = s->tmp[14];
+  for (int c = 0; c < 3; c++) {
+    OR(settemperature2)(bt + c * n, eta + c * n, k2, PA(UM_P_MSAT, 0), PA(UM_P_TEMP, 0), PA(UM_P_ALPHA, 0), n);
+    for (int i = 0; i < n; i++) B[c * n + i] += bt[c * n + i];
+  }
+}
+
 static void set_beff(sim_t *s, real *B) {
   int n = s->n;
   real *m = s->m;
@@ -1029,18 +1046,8 @@ static void set_beff(sim_t *s, real *B) {
   if (param_nonzero(s, UM_P_BEXT, 3))
     for (int c = 0; c < 3; c++)
       for (int i = 0; i < n; i++) B[c * n + i] += amul(PA(UM_P_BEXT, c), i);
-  /* B_therm.AddTo, engine/temperature.go:55-127 */
-  if (param_nonzero(s, UM_P_TEMP, 1)) {
-    real *eta = s->tmp[15];
-    OR(thermal_normals)(eta, eta + n, eta + 2 * n, n, s->seed, (uint64_t)s->nsteps, s->stream);
-    double V = s->cs[0] * s->cs[1] * s->cs[2];
-    real k2 = (real)(2 * KB_D / (s->gammaLL * V * s->dt));
-    real *bt = s->tmp[14];
-    for (int c = 0; c < 3; c++) {
-      OR(settemperature2)(bt + c * n, eta + c * n, k2, PA(UM_P_MSAT, 0), PA(UM_P_TEMP, 0), PA(UM_P_ALPHA, 0), n);
-      for (int i = 0; i < n; i++) B[c * n + i] += bt[c * n + i];
-    }
-  }
+  /* B_therm.AddTo, engine/temperature.go:55-127; skipped while relaxing (effectivefield.go:20) */
+  if (param_nonzero(s, UM_P_TEMP, 1) && !s->relaxing) add_thermal(s, B);
 }
 
 /* SetTorque, engine/torque.go:124-144, FreezeSpins :348-352 */
@@ -1305,10 +1312,46 @@ static void step_rk4(sim_t *s) {
   }
 }
 
+/* RK23 (Bogacki-Shampine).Step, engine/rk23.go:21-84, with integralController
+ * (accept order 3, reject order 4, FSAL) */
+static void step_rk23(sim_t *s) {
+  real *m = s->m;
+  if (s->fixdt != 0) s->dt = s->fixdt;
+  if (!s->k1_valid) { torque_fn(s, s->k1); s->k1_valid = 1; }
+  if (param_nonzero(s, UM_P_TEMP, 1)) torque_fn(s, s->k1);   /* FSAL is not used with temperature */
+  double t0 = s->Time;
+  real *m0 = s->tmp[0], *k2 = s->tmp[1], *k3 = s->tmp[2], *k4 = s->tmp[3];
+  real *k1 = s->k1;
+  copy3(s, m0, m);
+  real h = (real)(s->dt * s->gammaLL);
+  s->Time = t0 + (1. / 2.) * s->dt;
+  { const real *src[] = {m, k1}; real fac[] = {1, (real)(1. / 2.) * h}; madd_vec(s, m, 2, src, fac); }
+  normalize_m(s); torque_fn(s, k2);
+  s->Time = t0 + (3. / 4.) * s->dt;
+  { const real *src[] = {m0, k2}; real fac[] = {1, (real)(3. / 4.) * h}; madd_vec(s, m, 2, src, fac); }
+  normalize_m(s); torque_fn(s, k3);
+  {
+    const real *src[] = {m0, k1, k2, k3};
+    real fac[] = {1, (real)(2. / 9.) * h, (real)(1. / 3.) * h, (real)(4. / 9.) * h};
+    madd_vec(s, m, 4, src, fac);
+  }
+  normalize_m(s);
+  s->Time = t0 + s->dt;
+  torque_fn(s, k4);
+  real *Err = k2;   /* re-use k2 */
+  {
+    const real *src[] = {k1, k2, k3, k4};
+    real fac[] = {(real)(7. / 24. - 2. / 9.), (real)(1. / 4. - 1. / 3.), (real)(1. / 3. - 4. / 9.), (real)(1. / 8.)};
+    madd_vec(s, Err, 4, src, fac);
+  }
+  integral_controller(s, Err, k4, k1, m0, t0, (double)h, 3, 4, 1);
+}
+
 void OR(sim_step)(void *h) {
   sim_t *s = (sim_t *)h;
   switch (s->solver) {
   case UM_SOLVER_HEUN: step_heun(s); break;
+  case UM_SOLVER_RK23: step_rk23(s); break;
   case UM_SOLVER_RK4: step_rk4(s); break;
   default: step_rk45dp(s); break;
   }
@@ -1327,6 +1370,64 @@ void OR(sim_run)(void *h, double seconds) {
   double stop = s->Time + seconds;
   s->alarm = stop;
   while (s->Time < stop) OR(sim_step)(h);
+}
+
+/* relaxSteps, engine/relax.go:104-112: n steps, Time restored */
+static void relax_steps(sim_t *s, int n) {
+  double t0 = s->Time;
+  long stop = s->nsteps + n;
+  while (s->nsteps < stop) OR(sim_step)(s);
+  s->Time = t0;
+}
+static double selfdot3(sim_t *s, const real *a) {
+  double d = 0;
+  for (long i = 0; i < 3L * s->n; i++) d += (double)a[i] * (double)a[i];
+  return d;
+}
+/* Relax, engine/relax.go:24-100 */
+void OR(sim_relax)(void *h, double torque_threshold) {
+  sim_t *s = (sim_t *)h;
+  int prev_solver = s->solver, prev_precess = s->precess;
+  double prev_maxerr = s->maxerr, prev_fixdt = s->fixdt;
+  s->solver = UM_SOLVER_RK23;
+  s->k1_valid = 0;
+  s->fixdt = 0;
+  s->precess = 0;
+  s->relaxing = 1;
+  const int N = 3;
+  relax_steps(s, N);
+  double E0 = OR(sim_energy)(h, 9);
+  relax_steps(s, N);
+  double E1 = OR(sim_energy)(h, 9);
+  while (E1 < E0) {
+    relax_steps(s, N);
+    E0 = E1;
+    E1 = OR(sim_energy)(h, 9);
+  }
+  if (torque_threshold > 0) {
+    for (;;) {
+      while (maxvecnorm3(s, s->k1) > torque_threshold) relax_steps(s, N);
+      s->maxerr /= sqrt(2.0);
+      if (s->maxerr < 1e-9) break;
+    }
+  } else {
+    double T0 = 0, T1 = selfdot3(s, s->k1);
+    while (s->maxerr > 1e-9) {
+      s->maxerr /= sqrt(2.0);
+      relax_steps(s, N);
+      T0 = T1; T1 = selfdot3(s, s->k1);
+      while (T1 < T0) {
+        relax_steps(s, N);
+        T0 = T1; T1 = selfdot3(s, s->k1);
+      }
+    }
+  }
+  s->solver = prev_solver;
+  s->maxerr = prev_maxerr;
+  s->fixdt = prev_fixdt;
+  s->precess = prev_precess;
+  s->relaxing = 0;
+  s->k1_valid = 0;
 }
 void OR(sim_average_m)(void *h, double out[3]) {
   sim_t *s = (sim_t *)h;
@@ -1368,11 +1469,36 @@ double OR(sim_energy)(void *h, int term) {
       OR(regionaddv)(B, B + n, B + 2 * n, s->bextLUT[0], s->bextLUT[1], s->bextLUT[2], s->regions, n);
     for (int c = 0; c < 3; c++)
       for (int i = 0; i < n; i++) B[c * n + i] += amul(PA(UM_P_BEXT, c), i);
-  } else if (term == 3) {
-    OR(adduniaxial2)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), PA(UM_P_KU1, 0),
-                     (const real *)NULL, (real)0, PA(UM_P_ANISU, 0), PA(UM_P_ANISU, 1), PA(UM_P_ANISU, 2), n);
+  } else if (term == 3 || term == 4) {
+    /* AddAnisotropyEnergyDensity, engine/anisotropy.go:163-190: K1 and K2 separately */
+    if (term == 3)
+      OR(adduniaxial2)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), PA(UM_P_KU1, 0),
+                       (const real *)NULL, (real)0, PA(UM_P_ANISU, 0), PA(UM_P_ANISU, 1), PA(UM_P_ANISU, 2), n);
+    else
+      OR(adduniaxial2)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), (const real *)NULL, (real)0,
+                       PA(UM_P_KU2, 0), PA(UM_P_ANISU, 0), PA(UM_P_ANISU, 1), PA(UM_P_ANISU, 2), n);
+  } else if (term >= 5 && term <= 7) {
+    /* cubic Kc1, Kc2, Kc3 one at a time (anisotropy.go:192-207) */
+    const real *ka = NULL, *kb = NULL, *kc = NULL;
+    real ma = 0, mb = 0, mc = 0;
+    if (term == 5) { ka = PA_ARR(UM_P_KC1); ma = PA_MUL(UM_P_KC1); }
+    if (term == 6) { kb = PA_ARR(UM_P_KC2); mb = PA_MUL(UM_P_KC2); }
+    if (term == 7) { kc = PA_ARR(UM_P_KC3); mc = PA_MUL(UM_P_KC3); }
+    OR(addcubic2)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), ka, ma, kb, mb, kc, mc,
+                  PA(UM_P_ANISC1, 0), PA(UM_P_ANISC1, 1), PA(UM_P_ANISC1, 2), PA(UM_P_ANISC2, 0),
+                  PA(UM_P_ANISC2, 1), PA(UM_P_ANISC2, 2), n);
+  } else if (term == 8) {
+    /* GetThermalEnergy, engine/temperature.go:129-135 */
+    if (!param_nonzero(s, UM_P_TEMP, 1) || s->relaxing) return 0;
+    add_thermal(s, B);
+  } else if (term == 9) {
+    /* GetTotalEnergy, engine/energy.go:27-34 */
+    double tot = 0;
+    for (int t = 0; t <= 8; t++) tot += OR(sim_energy)(h, t);
+    return tot;
   }
-  double pref = (term == 2) ? -1.0 : -0.5;
+  static const double prefs[9] = {-0.5, -0.5, -1.0, -0.5, -0.25, -0.25, -1. / 6., -1. / 8., -1.0};
+  double pref = prefs[term];
   for (int i = 0; i < n; i++) {
     double ms = (double)amul(PA(UM_P_MSAT, 0), i) * (s->vol ? (double)s->vol[i] : 1.0);
     for (int c = 0; c < 3; c++) e += ms * (double)m[c * n + i] * (double)B[c * n + i];

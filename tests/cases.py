@@ -61,3 +61,57 @@ def setup(sim, mod, *, Ms, Aex, alpha, Dind=0.0, Ku1=0.0, Ku2=0.0, anisU=(0, 0, 
 def relerr(got, want):
     """max |got - want| / max |want| (normwise relative error of a field)."""
     return float(np.abs(np.asarray(got, np.float64) - want).max() / max(np.abs(want).max(), 1e-300))
+
+
+# ---- initial states and shapes of the reference's scripts (engine/config.go, shape.go) ----------
+def coords(N, cs):
+    """Index2Coord (engine/util.go:132-140): cell centres, origin at the grid centre."""
+    ix, iy, iz = (np.arange(n) for n in N)
+    x = cs[0] * (ix - 0.5 * (N[0] - 1))
+    y = cs[1] * (iy - 0.5 * (N[1] - 1))
+    z = cs[2] * (iz - 0.5 * (N[2] - 1))
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    return X, Y, Z
+
+
+def uniform_m(N, v):
+    m = np.zeros((3, N[2], N[1], N[0]))
+    for c in range(3):
+        m[c] = v[c]
+    return m
+
+
+def vortex(N, cs, circ, pol):
+    """Vortex(circ, pol), engine/config.go:63-73 (NaN at r = 0 -> (0, 0, pol))."""
+    X, Y, _ = coords(N, cs)
+    r2 = X * X + Y * Y
+    r = np.sqrt(r2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        m = np.stack([-Y * circ / r, X * circ / r, 1.5 * pol * np.exp(-r2 / (2 * cs[0] ** 2))])
+    bad = ~np.isfinite(m).all(axis=0)
+    m[:, bad] = np.array([0.0, 0.0, float(pol)])[:, None]
+    return m
+
+
+def two_domain(N, cs, m1, mwall, m2):
+    """TwoDomain, engine/config.go:144-159: m1 for x < 0, m2 for x >= 0, Gaussian wall."""
+    X, _, _ = coords(N, cs)
+    ww = 2 * cs[0]
+    gauss = np.exp(-(X / ww) ** 2)
+    m = np.empty((3,) + X.shape)
+    for c in range(3):
+        base = np.where(X < 0, m1[c], m2[c])
+        m[c] = (1 - gauss) * base + gauss * mwall[c]
+    return m
+
+
+def ellipse_geom(N, cs, dx, dy):
+    """SetGeom(Ellipse(dx, dy)) without edge smoothing (geom.go:74-140): 1 inside, 0 outside."""
+    X, Y, _ = coords(N, cs)
+    return ((X / dx) ** 2 + (Y / dy) ** 2 <= 0.25).astype(np.float64)
+
+
+def cylinder_geom(N, cs, diam, height):
+    """SetGeom(Cylinder(diam, height)), engine/shape.go:61-66."""
+    X, Y, Z = coords(N, cs)
+    return ((np.abs(Z) <= height / 2) & ((X / diam) ** 2 + (Y / diam) ** 2 <= 0.25)).astype(np.float64)

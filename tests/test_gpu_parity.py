@@ -556,3 +556,119 @@ def test_exchange_inhomo_gpu(gpu, prec):
     dt = np.float32 if prec == 0 else np.float64
     for e in run_exchange_inhomo(gpu.Sim, gpu, dt):
         assert abs(e - E_EXCH_OOMMF) <= 1e-5 * E_EXCH_OOMMF, e
+
+
+# ---- RK23, relax and energies -------------------------------------------------------------------
+@pytest.mark.parametrize("prec", [1, 0])
+def test_energy_terms_vs_oracle(gpu, oracle, prec):
+    """All energy terms (demag, exchange+DMI, Zeeman, K1, K2, Kc1..3, thermal, total)
+    on a mixed configuration, vs the FP64 oracle."""
+    N, cs = (12, 10, 3), (2e-9, 3e-9, 2.5e-9)
+    params = dict(Ms=7e5, Aex=1.1e-11, Dind=1e-3, alpha=0.1, Ku1=3e5, Ku2=-1e5, anisU=(0.3, -0.5, 0.8),
+                  Kc=(2e5, -3e4, 1e4), c1=(1, 1, 0), c2=(-1, 1, 0), Bext=(0.01, -0.02, 0.03), temp=250.0, seed=4)
+    so, sg = _both(oracle, gpu, N, cs, prec, **params)
+    for s in (so, sg):
+        s.set_solver_params(dt=1e-14)
+    m = random_unit((N[2], N[1], N[0]), 12)
+    so.set_m(m)
+    sg.set_m(m)
+    tol = 1e-11 if prec == 1 else 1e-5
+    for t in range(10):
+        eo, eg = so.energy(t), sg.energy(t)
+        assert abs(eg - eo) <= tol * max(abs(eo), 1e-30), (t, eg, eo)
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_rk23_trajectory_vs_oracle(gpu, oracle, prec):
+    """Adaptive RK23 (engine/rk23.go) on config-1 physics: <m>(t) vs the FP64 oracle."""
+    N, cs = (64, 16, 1), (3.90625e-9, 3.90625e-9, 3e-9)
+    so, sg = _both(oracle, gpu, N, cs, prec, Ms=8e5, Aex=1.3e-11, alpha=0.02, Bext=(-24.6e-3, 4.3e-3, 0.0))
+    for s in (so, sg):
+        s.set_solver(gpu.SOLVER_RK23)
+    m = config1_m(1)[:, :, :16, :64]
+    so.set_m(m)
+    sg.set_m(m)
+    for _ in range(3):
+        so.run(0.02e-9)
+        sg.run(0.02e-9)
+        d = np.abs(sg.average_m() - so.average_m()).max()
+        assert d < (1e-9 if prec == 1 else 1e-4), d
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_standard_problem4_relax_and_run(gpu, prec):
+    """test/standardproblem4.mx3: relax() then B_ext = (-24.6, 4.3, 0) mT for 1 ns;
+    <m> within the reference's TOL = 1e-5 after each."""
+    _say("sp4", prec)
+    from cases import uniform_m
+    dt = np.float32 if prec == 0 else np.float64
+    N, cs = (128, 32, 1), (500e-9 / 128, 125e-9 / 32, 3e-9)
+    s = gpu.Sim(N, cs, dtype=dt)
+    setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=0.02, dtype=dt)
+    s.set_m(uniform_m(N, (1, 0.1, 0)))
+    s.relax()
+    assert np.abs(s.average_m() - np.array([0.9669684171676636, 0.1252832127904892, 0.0])).max() < 1e-5
+    s.set_vector(gpu.P_BEXT, (-24.6e-3, 4.3e-3, 0.0))
+    s.run(1e-9)
+    got = s.average_m()
+    assert np.abs(got - np.array([-0.9846124053001404, 0.12604689081287384, 0.04327124357223511])).max() < 1e-5, got
+
+
+def test_cubic_anisotropy_relax(gpu):
+    """test/cubicanisotropy.mx3: relaxed <m> in a cylinder with Kc1 = 0.2e6 against an
+    in-plane field, vs OOMMF (tol 1e-3)."""
+    from cases import cylinder_geom, uniform_m
+    N, cs = (64, 64, 1), (4e-9, 4e-9, 2e-9)
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    setup(s, gpu, Ms=1100e3, Aex=13e-12, alpha=1.0, Kc=(0.2e6, 0.0, 0.0), c1=(100, 0, 0), c2=(0, 1, 0),
+          vol=cylinder_geom(N, cs, 64 * 4e-9, 64 * 4e-9), dtype=np.float32)
+    s.set_m(uniform_m(N, (1, 0, 0)))
+    for b, want in [(0.0, (1, 0, 0)), (0.01, (0.9994, 0.0270, 0)), (0.03, (0.9967, 0.0776, 0)),
+                    (0.10, (0.9711, 0.2367, 0)), (0.30, (0.7040, 0.7040, 0))]:
+        s.set_vector(gpu.P_BEXT, (b, b, 0.0))
+        s.relax()
+        got = s.average_m()
+        assert np.abs(got - np.array(want)).max() < 1e-3, (b, got)
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_energy_mx3_oommf(gpu, prec):
+    """test/energy.mx3: energy differences between the relaxed vortex state of an
+    ellipse and its relaxed state in 100 mT, vs OOMMF (the reference's tolerances)."""
+    _say("energy.mx3", prec)
+    from cases import ellipse_geom, vortex
+    dt = np.float32 if prec == 0 else np.float64
+    l, w, d, c = 512e-9, 256e-9, 40e-9, 4e-9
+    vol = l * w * d * (math.pi / 4)
+    N, cs = (128, 64, 1), (c, c, d)
+    s = gpu.Sim(N, cs, dtype=dt)
+    setup(s, gpu, Ms=860e3, Aex=13e-12, Ku1=50.0, anisU=(1, 0, 0), alpha=3.0,
+          vol=ellipse_geom(N, cs, l, w), dtype=dt)
+    s.set_m(vortex(N, cs, 1, 1))
+    terms = {"tot": 9, "exc": 1, "dem": 0, "zee": 2}
+    s.relax()
+    E1 = {k: s.energy(t) for k, t in terms.items()}
+    E1["ani"] = s.energy(3) + s.energy(4)
+    s.set_vector(gpu.P_BEXT, (100e-3, 0.0, 0.0))
+    s.relax()
+    E2 = {k: s.energy(t) for k, t in terms.items()}
+    E2["ani"] = s.energy(3) + s.energy(4)
+    oommf = {"tot": (12274.249330 - (-56507.645615)) * vol, "exc": (5038.693945 - 71.438905) * vol,
+             "ani": (19.517039 - 0.271210) * vol, "dem": (7216.038345 - 29186.031229) * vol,
+             "zee": (0 - (-85765.386961)) * vol}
+    tols = {"ani": 1e-21, "zee": 1e-18, "exc": 1e-18, "dem": 1e-18, "tot": 1e-18}
+    for k in ("ani", "zee", "exc", "dem", "tot"):
+        assert abs((E1[k] - E2[k]) - oommf[k]) < tols[k], (k, E1[k] - E2[k], oommf[k])
+
+
+def test_dmi_bloch_to_neel_relax(gpu):
+    """test/dmi.mx3: interfacial DMI turns a Bloch wall into a Neel wall; relaxed
+    <m> = (0.059, 0.035) within 0.01."""
+    from cases import two_domain
+    N, cs = (128, 128, 1), (250e-9 / 128, 250e-9 / 128, 0.6e-9)
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    setup(s, gpu, Ms=1100e3, Aex=16e-12, Dind=0.1e-3, Ku1=1.27e6, anisU=(0, 0, 1), alpha=3.0, dtype=np.float32)
+    s.set_m(two_domain(N, cs, (0, 0, -1), (1, 1, 0), (0, 0, 1)))
+    s.relax()
+    got = s.average_m()
+    assert abs(got[0] - 0.059) < 0.01 and abs(got[1] - 0.035) < 0.01, got

@@ -445,3 +445,70 @@ def test_rk4_trajectory(oracle):
     s.run(0.4e-9)
     got = s.average_m()
     assert np.abs(got - np.array([-0.8740, -0.2713, 0.01795])).max() < 1e-3, got
+
+
+# ---- RK23, relax and the energy terms ------------------------------------------------------------
+from cases import random_unit, setup, uniform_m  # noqa: E402
+def test_rk23_macrospin_and_order(oracle):
+    """Bogacki-Shampine (engine/rk23.go): adaptive run matches the analytic macrospin,
+    and fixed steps converge at third order."""
+    import math
+    from cases import GAMMA
+    def run(fixdt, t_end, maxerr=1e-5):
+        s = oracle.Sim((1, 1, 1), (1e-9,) * 3, dtype=np.float64)
+        setup(s, oracle, Ms=8e5, Aex=0.0, alpha=0.1, Bext=(0.0, 0.0, 0.1), demag=False, dtype=np.float64)
+        th0, ph0 = 1.2, 0.3
+        s.set_m(np.array([math.sin(th0) * math.cos(ph0), math.sin(th0) * math.sin(ph0), math.cos(th0)]).reshape(3, 1, 1, 1))
+        s.set_solver(oracle.SOLVER_RK23)
+        s.set_solver_params(dt=1e-14, fixdt=fixdt, maxerr=maxerr)
+        s.run(t_end)
+        t = s.state()["t"]
+        g = GAMMA / (1 + 0.1 ** 2)
+        th = 2 * math.atan(math.tan(th0 / 2) * math.exp(-g * 0.1 * 0.1 * t))
+        ph = ph0 + g * 0.1 * t
+        want = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+        return np.abs(s.get_m().reshape(3) - want).max()
+    assert run(0.0, 0.2e-9) < 1e-4
+    e1, e2 = run(4e-13, 0.2e-9), run(2e-13, 0.2e-9)
+    assert 2 ** 2.6 < e1 / e2 < 2 ** 3.4, (e1, e2)
+
+
+def test_sp4_relax_oracle(oracle):
+    """test/standardproblem4.mx3:17-20: relax() from uniform(1, .1, 0) gives
+    <m> = (0.96697, 0.12528, 0) within 1e-5, in FP64 and FP32."""
+    want = np.array([0.9669684171676636, 0.1252832127904892, 0.0])
+    for dt in (np.float64, np.float32):
+        s = oracle.Sim((128, 32, 1), (500e-9 / 128, 125e-9 / 32, 3e-9), dtype=dt)
+        setup(s, oracle, Ms=800e3, Aex=13e-12, alpha=0.02, dtype=dt)
+        s.set_m(uniform_m((128, 32, 1), (1, 0.1, 0)))
+        s.relax()
+        assert np.abs(s.average_m() - want).max() < 1e-5, (dt, s.average_m())
+        assert s.state()["t"] == 0.0   # relax does not advance time
+
+
+def test_energy_terms_closed_form(oracle):
+    """Anisotropy energies with the reference's prefactors (anisotropy.go:163-207)
+    equal the closed-form densities -K1 (m.u)^2, -K2 (m.u)^4, Kc1 (a2b2+b2c2+c2a2),
+    Kc2 a2b2c2, Kc3 (a4b4+b4c4+c4a4); Zeeman -V sum Ms m.B; the total is the sum."""
+    N, cs = (6, 5, 2), (2e-9, 3e-9, 4e-9)
+    V = cs[0] * cs[1] * cs[2]
+    m = random_unit((N[2], N[1], N[0]), 9)
+    u = np.array([0.3, -0.5, 0.8]); u /= np.linalg.norm(u)
+    c1 = np.array([1.0, 1.0, 0.0]) / np.sqrt(2); c2 = np.array([-1.0, 1.0, 0.0]) / np.sqrt(2); c3 = np.cross(c1, c2)
+    s = oracle.Sim(N, cs, dtype=np.float64)
+    setup(s, oracle, Ms=7e5, Aex=1e-11, alpha=0.1, Ku1=3e5, Ku2=-1e5, anisU=tuple(u), Kc=(2e5, -3e4, 1e4),
+          c1=tuple(c1), c2=tuple(c2), Bext=(0.01, -0.02, 0.03), demag=False, dtype=np.float64)
+    s.set_m(m)
+    mu = np.einsum("c...,c->...", m, u)
+    a, b, c = (np.einsum("c...,c->...", m, v) for v in (c1, c2, c3))
+    want = {3: -3e5 * (mu ** 2).sum() * V, 4: 1e5 * (mu ** 4).sum() * V,
+            5: 2e5 * (a * a * b * b + b * b * c * c + c * c * a * a).sum() * V,
+            6: -3e4 * (a * a * b * b * c * c).sum() * V,
+            7: 1e4 * (a ** 4 * b ** 4 + b ** 4 * c ** 4 + c ** 4 * a ** 4).sum() * V,
+            2: -V * 7e5 * (m[0] * 0.01 - m[1] * 0.02 + m[2] * 0.03).sum()}
+    for t, w in want.items():
+        e = s.energy(t)
+        assert abs(e - w) <= 1e-9 * abs(w), (t, e, w)
+    assert s.energy(8) == 0.0
+    parts = sum(s.energy(t) for t in range(9))
+    assert abs(s.energy(9) - parts) <= 1e-12 * abs(parts)

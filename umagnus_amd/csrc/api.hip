@@ -591,6 +591,10 @@ int um_sim_kernel_times(void *sim, double out_ms[3], int *n_evals) {
   return guard([&] { SIMCHK(sim); SIM(sim)->kernel_times(out_ms, n_evals); });
 }
 
+int um_sim_relax(void *sim, double torque_threshold) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->relax(torque_threshold); });
+}
+
 int um_sim_set_graph(void *sim, int on) { return guard([&] { SIMCHK(sim); SIM(sim)->set_graph(on); }); }
 
 int um_sims_steps(int nsims, void *const *sims, int n) {

@@ -47,6 +47,7 @@ struct Sim : SimBase {
 
   // solver state, engine/run.go:13-43
   int solver = UM_SOLVER_RK45DP;
+  int relaxing = 0;              // engine/relax.go:22: no thermal field while relaxing
   double Time = 0, dt = 1e-15, fixdt = 0, maxerr = 1e-5, relerr = 1e-4, minslope = 1e8, headroom = 0.8;
   double mindt = 0, maxdt = 0, gammaLL = 1.7595e11, alarm = 0;
   double lastErr = 0, peakErr = 0, lastTorque = 0;
@@ -65,7 +66,7 @@ struct Sim : SimBase {
   long g_gen = -1;
   int g_solver = -1;
   double g_dt = 0;
-  const void *g_ptrs[4] = {};
+  const void *g_ptrs[8] = {};
   long g_evals = 0;
 
   // profiling: HIP events around the demag and field kernels of every
@@ -268,7 +269,7 @@ struct Sim : SimBase {
       a.bextLUT[c] = has_bextlut ? bextLUT.as<T>() + c * NREGION : nullptr;
     }
     a.has_bext = nz(UM_P_BEXT, 3);
-    a.has_therm = nz(UM_P_TEMP);
+    a.has_therm = nz(UM_P_TEMP) && !relaxing;
     a.temp = P(UM_P_TEMP);
     a.kth = (T)(2 * KB / (gammaLL * cs[0] * cs[1] * cs[2] * dt));
     a.seed = seed;
@@ -372,7 +373,7 @@ struct Sim : SimBase {
   // ---- solver ------------------------------------------------------------------------
   void set_solver(int s) override {
     gen++;
-    UM_REQUIRE(s == UM_SOLVER_HEUN || s == UM_SOLVER_RK4 || s == UM_SOLVER_RK45DP, UM_EUNSUP,
+    UM_REQUIRE(s == UM_SOLVER_HEUN || s == UM_SOLVER_RK23 || s == UM_SOLVER_RK4 || s == UM_SOLVER_RK45DP, UM_EUNSUP,
                "solver " + std::to_string(s) + " not available (Heun=2, RK4=4, RK45DP=5)");
     solver = s;
     k1_valid = false;
@@ -538,6 +539,77 @@ struct Sim : SimBase {
     }
   }
 
+  // RK23 (Bogacki-Shampine).Step, engine/rk23.go:21-84, integralController
+  // (accept order 3, reject order 4, FSAL k1 <- k4)
+  void step_rk23() {
+    if (fixdt != 0) dt = fixdt;
+    T *k1 = k[0];
+    if (!k1_valid) { torque_fn(k1); k1_valid = true; }
+    if (nz(UM_P_TEMP)) torque_fn(k1);   // FSAL is not used with temperature
+    const double t0 = Time;
+    std::swap(m, m0);
+    const T h = (T)(dt * gammaLL);
+    T *k2 = k[1], *k3 = k[2], *k4 = k[3];
+    {
+      const T *s[] = {m0, k1};
+      const T f[] = {T(1), T(1. / 2.) * h};
+      Time = t0 + (1. / 2.) * dt;
+      stage(m, 2, s, f, true);
+      torque_fn(k2);
+    }
+    {
+      const T *s[] = {m0, k2};
+      const T f[] = {T(1), T(3. / 4.) * h};
+      Time = t0 + (3. / 4.) * dt;
+      stage(m, 2, s, f, true);
+      torque_fn(k3);
+    }
+    {
+      const T *s[] = {m0, k1, k2, k3};
+      const T f[] = {T(1), T(2. / 9.) * h, T(1. / 3.) * h, T(4. / 9.) * h};
+      stage(m, 4, s, f, true);
+    }
+    Time = t0 + dt;
+    torque_fn(k4);
+    // Err = sum e_j k_j over k1..k4 (the two trailing zero terms leave the sum
+    // unchanged); the controller's delM is the last source, k4
+    const T e[6] = {T(7. / 24. - 2. / 9.), T(1. / 4. - 1. / 3.), T(1. / 3. - 4. / 9.), T(1. / 8.), T(0), T(0)};
+    const T *src[6] = {k1, k2, k3, k4, k4, k4};
+    double r[3];
+    err_reduce(0, src, 6, e, r);
+    const double err = std::sqrt(r[0]) * (double)h;
+    const double maxdm = std::sqrt(r[1]);
+    if (err < maxerr || dt <= mindt || fixdt != 0) {
+      int fail = 0;
+      double rlerr = 0;
+      if (maxdm < minslope) fail = 0;
+      else { rlerr = r[2]; fail = 1; }
+      if (fail == 0 || relerr <= 0.0 || rlerr < relerr || dt <= mindt || fixdt != 0) {
+        if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), (double)h, 0, st);
+        else {
+          set_last_err(err);
+          lastTorque = maxdm;
+        }
+        nsteps++;
+        Time = t0 + dt;
+        if (fail == 0) adapt(std::pow(maxerr / err, 1. / 3));
+        else adapt(std::pow(relerr / rlerr, 1. / 3));
+        std::swap(k[0], k[3]);   // FSAL: k1 <- k4
+        k1_valid = true;
+      } else {
+        Time = t0;
+        std::swap(m, m0);
+        nundone++;
+        adapt(std::pow(relerr / rlerr, 1. / 4));
+      }
+    } else {
+      Time = t0;
+      std::swap(m, m0);
+      nundone++;
+      adapt(std::pow(maxerr / err, 1. / 4));
+    }
+  }
+
   // Heun.Step, engine/heun.go:15-59
   void step_heun() {
     T *dy0 = k[0], *dy = k[1];
@@ -629,6 +701,7 @@ struct Sim : SimBase {
     if (profiling) UM_HIP(hipEventRecord(step_ev[0], st));
     switch (solver) {
       case UM_SOLVER_HEUN: step_heun(); break;
+      case UM_SOLVER_RK23: step_rk23(); break;
       case UM_SOLVER_RK4: step_rk4(); break;
       default: step_rk45dp(); break;
     }
@@ -661,9 +734,9 @@ struct Sim : SimBase {
     if (!graph_usable() || nsteps >= stop) return 0;
     step();
     if (nsteps + 2 > stop) return 0;
-    const void *ptrs[4] = {m, m0, k[0], k[1]};
+    const void *ptrs[8] = {m, m0, k[0], k[1], k[2], k[3], k[4], k[5]};
     bool fresh = gexec && g_gen == gen && g_solver == solver && g_dt == fixdt;
-    for (int i = 0; i < 4 && fresh; i++) fresh = (g_ptrs[i] == ptrs[i]);
+    for (int i = 0; i < 8 && fresh; i++) fresh = (g_ptrs[i] == ptrs[i]);
     if (!fresh) capture2();
     hstat[0] = lastErr; hstat[1] = peakErr; hstat[2] = lastTorque;
     const unsigned long long ns = (unsigned long long)nsteps;
@@ -698,8 +771,8 @@ struct Sim : SimBase {
     g_evals = nevals - e0;
     Time = t0; nsteps = s0; nevals = e0; nundone = u0;   // capture executed nothing
     g_gen = gen; g_solver = solver; g_dt = fixdt;
-    const void *ptrs[4] = {m, m0, k[0], k[1]};
-    for (int i = 0; i < 4; i++) g_ptrs[i] = ptrs[i];
+    const void *ptrs[8] = {m, m0, k[0], k[1], k[2], k[3], k[4], k[5]};
+    for (int i = 0; i < 8; i++) g_ptrs[i] = ptrs[i];
   }
   void graph_replay() override {
     UM_HIP(hipGraphLaunch(gexec, st));
@@ -746,35 +819,41 @@ struct Sim : SimBase {
       out[c] = wdot(a1, nullptr, 1, w, T(1)) / denom;
     }
   }
-  // E = pref * V * sum(Ms vol m . B_term), engine/energy.go, demag.go:146, exchange.go:79, zeeman.go:15
+  // E = pref * V * sum(Ms vol m . B_term), engine/energy.go, demag.go:146, exchange.go:79,
+  // zeeman.go:15, anisotropy.go:163-216 (K1 -1/2, K2 -1/4, Kc1 -1/4, Kc2 -1/6, Kc3 -1/8),
+  // temperature.go:129 (thermal, zero while relaxing); term 9 = GetTotalEnergy
   double energy(int term) override {
-    UM_REQUIRE(term >= 0 && term <= 3, UM_EINVAL, "energy term 0..3");
+    UM_REQUIRE(term >= 0 && term <= 9, UM_EINVAL, "energy term 0..9");
+    if (term == 9) {
+      double tot = 0;
+      for (int t = 0; t <= 8; t++) tot += energy(t);
+      return tot;
+    }
+    if (term == 8 && (!nz(UM_P_TEMP) || relaxing)) return 0.0;
+    static const double pref[9] = {-0.5, -0.5, -1.0, -0.5, -0.25, -0.25, -1. / 6., -1. / 8., -1.0};
     fill_bytes(spare, sizeof(T) * 3 * n, st);
-    void *B[3] = {spare, spare + n, spare + 2 * n};
-    const void *mv[3] = {m, m + n, m + 2 * n};
+    const T *B = spare;
     if (term == 0) {
-      if (enable_demag) { compute_demag(m); B[0] = bd; B[1] = bd + n; B[2] = bd + 2 * n; }
-    } else if (term == 1) {
-      FieldArgs<T> a = field_args(m);
-      // exchange only: zero every other term
-      a.has_uniax = a.has_cubic = a.has_bext = a.has_therm = 0;
-      for (int c = 0; c < 3; c++) { a.bextLUT[c] = nullptr; a.bd[c] = nullptr; a.tau[c] = nullptr; a.beff[c] = spare + c * n; }
-      launch_field_torque(a, st);
-    } else if (term == 2) {
-      FieldArgs<T> a = field_args(m);
-      a.exch_mode = 0;
-      a.has_uniax = a.has_cubic = a.has_therm = 0;
-      for (int c = 0; c < 3; c++) { a.bd[c] = nullptr; a.tau[c] = nullptr; a.beff[c] = spare + c * n; }
-      launch_field_torque(a, st);
+      if (enable_demag) { compute_demag(m); B = bd; }
     } else {
       FieldArgs<T> a = field_args(m);
-      a.exch_mode = 0;
-      a.has_cubic = a.has_bext = a.has_therm = 0;
-      a.Ku2 = Par<T>{nullptr, T(0)};
-      for (int c = 0; c < 3; c++) { a.bextLUT[c] = nullptr; a.bd[c] = nullptr; a.tau[c] = nullptr; a.beff[c] = spare + c * n; }
+      // only the requested term: switch every other one off
+      if (term != 1) a.exch_mode = 0;
+      if (term != 2) {
+        a.has_bext = 0;
+        for (int c = 0; c < 3; c++) a.bextLUT[c] = nullptr;
+      }
+      a.has_uniax = (term == 3 || term == 4) ? a.has_uniax : 0;
+      if (term == 3) a.Ku2 = Par<T>{nullptr, T(0)};
+      if (term == 4) a.Ku1 = Par<T>{nullptr, T(0)};
+      a.has_cubic = (term >= 5 && term <= 7) ? a.has_cubic : 0;
+      if (term == 5) { a.Kc2 = Par<T>{nullptr, T(0)}; a.Kc3 = Par<T>{nullptr, T(0)}; }
+      if (term == 6) { a.Kc1 = Par<T>{nullptr, T(0)}; a.Kc3 = Par<T>{nullptr, T(0)}; }
+      if (term == 7) { a.Kc1 = Par<T>{nullptr, T(0)}; a.Kc2 = Par<T>{nullptr, T(0)}; }
+      if (term != 8) a.has_therm = 0;
+      for (int c = 0; c < 3; c++) { a.bd[c] = nullptr; a.tau[c] = nullptr; a.beff[c] = spare + c * n; }
       launch_field_torque(a, st);
     }
-    (void)mv;
     // weight = Ms * vol per cell
     DevBuf w(sizeof(T) * n);
     const T *msa = (const T *)p[UM_P_MSAT][0].arr.p;
@@ -783,10 +862,83 @@ struct Sim : SimBase {
     else if (has_vol) UM_HIP(hipMemcpyAsync(w.p, vol.p, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
     else fill_value<T>(w.as<T>(), T(1), n, st);
     const T *a3[3] = {m, m + n, m + 2 * n};
-    const T *b3[3] = {(const T *)B[0], (const T *)B[1], (const T *)B[2]};
-    const double s = wdot(a3, b3, 3, w.as<T>(), (T)p[UM_P_MSAT][0].mul);
+    const T *b3[3] = {B, B + n, B + 2 * n};
+    const double sum = wdot(a3, b3, 3, w.as<T>(), (T)p[UM_P_MSAT][0].mul);
     const double V = cs[0] * cs[1] * cs[2];
-    return (term == 2 ? -1.0 : -0.5) * V * s;
+    return pref[term] * V * sum;
+  }
+
+  // relaxSteps, engine/relax.go:104-112: n steps, Time restored
+  void relax_steps(int cnt) {
+    const double t0 = Time;
+    const long stop = nsteps + cnt;
+    while (nsteps < stop) step();
+    Time = t0;
+  }
+  double k1_maxnorm() {
+    const T *src[1] = {k[0]};
+    double r[3];
+    err_reduce(2, src, 1, nullptr, r);
+    return std::sqrt(r[0]);
+  }
+  double k1_dot() {
+    const T *a3[3] = {k[0], k[0] + n, k[0] + 2 * n};
+    return wdot(a3, a3, 3, nullptr, T(1));
+  }
+  // Relax, engine/relax.go:24-100
+  void relax(double torque_threshold) override {
+    gen++;
+    const int prev_solver = solver, prev_precess = precess;
+    const double prev_maxerr = maxerr, prev_fixdt = fixdt;
+    solver = UM_SOLVER_RK23;
+    k1_valid = false;
+    fixdt = 0;
+    precess = 0;
+    relaxing = 1;
+    try {
+      const int NR = 3;   // energy every 3 steps
+      relax_steps(NR);
+      double E0 = energy(9);
+      relax_steps(NR);
+      double E1 = energy(9);
+      while (E1 < E0) {
+        relax_steps(NR);
+        E0 = E1;
+        E1 = energy(9);
+      }
+      if (torque_threshold > 0) {
+        for (;;) {
+          while (k1_maxnorm() > torque_threshold) relax_steps(NR);
+          maxerr /= std::sqrt(2.0);
+          if (maxerr < 1e-9) break;
+        }
+      } else {
+        double T0 = 0, T1 = k1_dot();
+        while (maxerr > 1e-9) {
+          maxerr /= std::sqrt(2.0);
+          relax_steps(NR);
+          T0 = T1;
+          T1 = k1_dot();
+          while (T1 < T0) {
+            relax_steps(NR);
+            T0 = T1;
+            T1 = k1_dot();
+          }
+        }
+      }
+    } catch (...) {
+      solver = prev_solver; maxerr = prev_maxerr; fixdt = prev_fixdt; precess = prev_precess;
+      relaxing = 0; k1_valid = false; gen++;
+      throw;
+    }
+    solver = prev_solver;
+    maxerr = prev_maxerr;
+    fixdt = prev_fixdt;
+    precess = prev_precess;
+    relaxing = 0;
+    k1_valid = false;
+    gen++;
+    UM_HIP(hipStreamSynchronize(st));
   }
 
   void set_profiling(int on) override {

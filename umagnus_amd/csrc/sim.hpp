@@ -47,6 +47,7 @@ struct SimBase {
   virtual void run(double seconds) = 0;
   virtual void average_m(double out[3]) = 0;
   virtual double energy(int term) = 0;
+  virtual void relax(double torque_threshold) = 0;
   virtual hipStream_t stream() = 0;
   virtual void set_profiling(int on) = 0;
   virtual void set_graph(int on) = 0;

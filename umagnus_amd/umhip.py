@@ -22,7 +22,7 @@ _lib = None
 
 F32, F64 = 0, 1
 P_MSAT, P_KU1, P_KU2, P_KC1, P_KC2, P_KC3, P_ALPHA, P_TEMP, P_ANISU, P_ANISC1, P_ANISC2, P_BEXT = range(12)
-SOLVER_HEUN, SOLVER_RK4, SOLVER_RK45DP = 2, 4, 5
+SOLVER_HEUN, SOLVER_RK23, SOLVER_RK4, SOLVER_RK45DP = 2, 3, 4, 5
 NREGION = 256
 SYMLUT = NREGION * (NREGION + 1) // 2
 
@@ -42,7 +42,7 @@ EXPORTS = [
     "um_sim_set_solver", "um_sim_set_solver_params", "um_sim_get_state", "um_sim_step", "um_sim_steps",
     "um_sim_run", "um_sim_average_m", "um_sim_energy", "um_sim_stream", "um_sim_set_profiling",
     "um_sim_kernel_times", "um_sim_set_graph", "um_sims_steps", "um_graph_begin", "um_graph_end",
-    "um_graph_launch", "um_graph_destroy",
+    "um_graph_launch", "um_graph_destroy", "um_sim_relax",
 ]
 
 
@@ -486,6 +486,10 @@ class Sim:
         ne = C.c_int()
         _call("um_sim_kernel_times", self.h, ms, C.byref(ne))
         return list(ms), ne.value
+
+    def relax(self, torque_threshold=-1.0):
+        """Relax(), engine/relax.go:24-100 (um_sim_relax)."""
+        _call("um_sim_relax", self.h, C.c_double(torque_threshold))
 
     def set_graph(self, on):
         """Replay fixed-dt steps as a captured HIP graph (um_sim_set_graph)."""
