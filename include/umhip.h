@@ -129,6 +129,14 @@ int um_reduce_maxvecdiff(int prec, const void *const a[3], const void *const b[3
  * accuracy = DemagAccuracy (engine/demag.go:49). */
 int um_demag_create(int prec, int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
                     double accuracy, void **plan);
+/* NewDemag with the Newell tensor: newellDemagConv (engine/demag.go:134-142)
+ * with mag.NewellDemagKernel (mag/newellDemagKernel.go:1006, CalcNewellDemagKernel
+ * :1108-1324) computed on the GPU in FP64.  asymptotic_radius: far-field
+ * expansion beyond this many cells (reference default 32, negative: never);
+ * zero_self_demag != 0 adds 1/3 to the diagonal self terms (default 0).
+ * Open boundaries only (UM_EUNSUP for PBC). */
+int um_demag_create_newell(int prec, int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
+                           int asymptotic_radius, int zero_self_demag, void **plan);
 /* NewDemag from a caller-supplied FP64 host tensor (xx,yy,zz,yz,xz,xy), each of
  * the padded size P (x fastest) -- e.g. the Newell tensor or an OVF cache. */
 int um_demag_create_from_kernel(int prec, int Nx, int Ny, int Nz, const int P[3],
@@ -146,6 +154,10 @@ int um_demag_kspec(void *plan, int c, void *host_out);
  * host buffers of the padded size P = padSize(N, pbc). */
 int um_demag_kernel_bf(int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
                        double accuracy, double *const K[6]);
+/* CalcNewellDemagKernel, mag/newellDemagKernel.go:1108-1324, on the GPU in
+ * FP64 into host buffers K[c] of the padded size. */
+int um_demag_kernel_newell(int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
+                           int asymptotic_radius, int zero_self_demag, double *const K[6]);
 
 /* ---- simulation handle: engine/ effective field, torque, steppers --------- */
 /* Parameter ids (the .mx3 quantity names) for um_sim_set_scalar / _vector */
@@ -179,6 +191,10 @@ int um_sim_set_geometry(void *sim, const void *vol);
 /* EnableDemag, OpenBC, DoPrecess, and the MaxVecDiff sqrt(3) widening in Heun */
 int um_sim_set_flags(void *sim, int enable_demag, int openbc, int precess, int heun_sqrt3);
 int um_sim_set_demag_accuracy(void *sim, double accuracy);
+/* EnableNewellDemag (engine/demag.go:26-56): enable=1 selects the Newell tensor
+ * (with asymptotic_radius and zero_self_demag as in um_demag_create_newell) in
+ * place of the brute-force one; demag itself stays switched by enable_demag. */
+int um_sim_set_demag_newell(void *sim, int enable, int asymptotic_radius, int zero_self_demag);
 /* ThermSeed (engine/temperature.go:138); stream separates ensemble members */
 int um_sim_set_thermal_seed(void *sim, uint64_t seed, uint32_t stream);
 /* m.SetArray (engine/magnetization.go:45), normalises; and m readback */

@@ -91,6 +91,51 @@ def demag_kernel_bf(N, cellsize, pbc=(0, 0, 0), accuracy=6.0):
     return K
 
 
+def demag_kernel_newell(N, cellsize, pbc=(0, 0, 0), asymptotic_radius=32, zero_self_demag=0):
+    """Newell tensor (mag/newellDemagKernel.go:1108), same layout as demag_kernel_bf."""
+    P = pad_size(N, pbc)
+    K = [np.zeros((P[2], P[1], P[0]), np.float64) for _ in range(6)]
+    arr = (C.c_void_p * 6)(*[ptr(k) for k in K])
+    rc = lib().um_or_demag_kernel_newell((C.c_int * 3)(*N), (C.c_int * 3)(*pbc),
+                                         (C.c_double * 3)(*cellsize), int(asymptotic_radius),
+                                         int(zero_self_demag), arr)
+    assert rc == 0, rc
+    return K
+
+
+def _d(name, *args):
+    f = getattr(lib(), name)
+    f.restype = C.c_double
+    return f(*[C.c_double(a) for a in args])
+
+
+def newell_f(x, y, z):
+    return _d("um_or_newell_f", x, y, z)
+
+
+def newell_g(x, y, z):
+    return _d("um_or_newell_g", x, y, z)
+
+
+def self_demag_nx(x, y, z):
+    return _d("um_or_self_demag_nx", x, y, z)
+
+
+def newell_cell(cellsize):
+    d = (C.c_double * 3)()
+    lib().um_or_newell_cell((C.c_double * 3)(*cellsize), d)
+    return tuple(d)
+
+
+def newell_element(comp, d, R, far=False):
+    """Kernel element -N (comp 0..5 = xx yy zz yz xz xy) at offset R (normalised
+    cell units d): exact 27-point form, or the far-field expansion."""
+    f = lib().um_or_newell_asymptotic if far else lib().um_or_newell_exact
+    f.restype = C.c_double
+    v = f(int(comp), (C.c_double * 3)(*d), *[C.c_double(r) for r in R])
+    return -v if far else v
+
+
 def symmidx(i, j):
     return lib().um_or_symmidx(i, j)
 
@@ -241,6 +286,9 @@ class Sim:
 
     def set_demag_accuracy(self, acc):
         self._f("sim_set_demag_accuracy")(self._h(), C.c_double(acc))
+
+    def set_demag_newell(self, enable=1, asymptotic_radius=32, zero_self_demag=0):
+        self._f("sim_set_demag_newell")(self._h(), int(enable), int(asymptotic_radius), int(zero_self_demag))
 
     def set_thermal_seed(self, seed, stream=0):
         self._f("sim_set_thermal_seed")(self._h(), C.c_uint64(seed), C.c_uint32(stream))

@@ -41,6 +41,26 @@ int um_or_demag_kernel_bf(const int N[3], const int pbc[3], const double cs[3],
 /* padSize, mag/demagkernel.go:404-420 */
 void um_or_pad_size(const int N[3], const int pbc[3], int P[3]);
 
+/* Newell demag tensor, mag/newellDemagKernel.go:1108-1324 (open boundaries;
+ * returns -2 for periodic ones).  Same output layout as um_or_demag_kernel_bf.
+ * [closed: prism demag factors (Aharoni), block sums, far-field expansion vs
+ *  the exact form; golden: test/standardproblem4_newell-3d.mx3,
+ *  standardproblem4b_newell.mx3] */
+int um_or_demag_kernel_newell(const int N[3], const int pbc[3], const double cs[3],
+                              int asymptotic_radius, int zero_self_demag, double *K[6]);
+/* pieces of it, for the tests: Newell's f and g, SelfDemagNx, the 27-point
+ * sums CalculateSDA00/01, the normalised cell edges, and one tensor element
+ * (xx yy zz yz xz xy = 0..5) from the exact form or the far-field expansion,
+ * both as kernel values (-N) for the exact one and N for the expansion. */
+double um_or_newell_f(double x, double y, double z);
+double um_or_newell_g(double x, double y, double z);
+double um_or_self_demag_nx(double x, double y, double z);
+double um_or_newell_sda00(double x, double y, double z, double dx, double dy, double dz);
+double um_or_newell_sda01(double x, double y, double z, double dx, double dy, double dz);
+void um_or_newell_cell(const double cs[3], double d[3]);
+double um_or_newell_exact(int comp, const double d[3], double Rx, double Ry, double Rz);
+double um_or_newell_asymptotic(int comp, const double d[3], double Rx, double Ry, double Rz);
+
 /* symmidx, engine/exchange.go:181-187 */
 int um_or_symmidx(int i, int j);
 /* exchAverage, engine/exchange.go:193-202 (float32 arithmetic, as Go does) */
@@ -136,6 +156,9 @@ enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK23 = 3, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45
   void um_or_sim_set_flags##SFX(void *h, int enable_demag, int openbc, int precess,        \
         int heun_sqrt3);                                                                   \
   void um_or_sim_set_demag_accuracy##SFX(void *h, double acc);                             \
+  /* EnableNewellDemag (engine/demag.go:26-56): 1 = Newell tensor, 0 = brute force */      \
+  void um_or_sim_set_demag_newell##SFX(void *h, int enable, int asymptotic_radius,         \
+        int zero_self_demag);                                                              \
   void um_or_sim_set_thermal_seed##SFX(void *h, uint64_t seed, uint32_t stream);           \
   void um_or_sim_set_m##SFX(void *h, const real *mx, const real *my, const real *mz);      \
   void um_or_sim_get_m##SFX(void *h, real *mx, real *my, real *mz);                        \

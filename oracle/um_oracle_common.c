@@ -119,7 +119,8 @@ int um_or_demag_kernel_bf(const int N[3], const int pbc[3], const double cs[3],
           int nz = (int)(fmax(cs[2] / maxSize, 1) + 0.5);
           nv *= 2;
           nw *= 2;
-          double scale = 1 / (double)(nv * nw * nx * ny * nz);
+          /* Go's int is 64-bit: the sub-cell count overflows 32 bits at accuracy ~40 */
+          double scale = 1 / (double)((long long)nv * nw * nx * ny * nz);
           double surface = cs[v] * cs[w];
           double charge = surface * scale;
           double pu1 = cs[u] / 2.;

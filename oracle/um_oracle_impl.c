@@ -843,6 +843,7 @@ typedef struct {
   real *vol;
   int enable_demag, openbc, precess, heun_sqrt3, has_dmi;
   double demag_acc;
+  int newell, newell_arad, newell_zsd; /* EnableNewellDemag and its two settings */
   void *demag;
   uint64_t seed; uint32_t stream;
   real *m, *tmp[16];
@@ -868,6 +869,7 @@ void *OR(sim_new)(int Nx, int Ny, int Nz, double cx, double cy, double cz, int p
   s->pbccode = (pbcx ? 1 : 0) | (pbcy ? 2 : 0) | (pbcz ? 4 : 0);
   s->enable_demag = 1; s->precess = 1; s->heun_sqrt3 = 1;
   s->demag_acc = 6.0;
+  s->newell_arad = 32;               /* asymptotic_radius, engine/demag.go:27 */
   s->aLUT = (real *)calloc(UM_OR_SYMLUT, sizeof(real));
   s->dLUT = (real *)calloc(UM_OR_SYMLUT, sizeof(real));
   s->p[UM_P_ANISU][0].mul = 0; s->p[UM_P_ANISU][1].mul = 0; s->p[UM_P_ANISU][2].mul = 0;
@@ -963,6 +965,23 @@ void OR(sim_set_demag_accuracy)(void *h, double acc) {
   s->demag_acc = acc;
   if (s->demag) { OR(demag_free)(s->demag); s->demag = NULL; }
 }
+void OR(sim_set_demag_newell)(void *h, int enable, int asymptotic_radius, int zero_self_demag) {
+  sim_t *s = (sim_t *)h;
+  s->newell = enable; s->newell_arad = asymptotic_radius; s->newell_zsd = zero_self_demag;
+  if (s->demag) { OR(demag_free)(s->demag); s->demag = NULL; }
+}
+static void *sim_newell_plan(sim_t *s) {
+  int P[3];
+  um_or_pad_size(s->N, s->pbc, P);
+  long np = (long)P[0] * P[1] * P[2];
+  double *K[6];
+  for (int c = 0; c < 6; c++) K[c] = (double *)malloc(sizeof(double) * np);
+  void *h = NULL;
+  if (um_or_demag_kernel_newell(s->N, s->pbc, s->cs, s->newell_arad, s->newell_zsd, K) == 0)
+    h = OR(demag_new_from_kernel)(s->N, P, K);
+  for (int c = 0; c < 6; c++) free(K[c]);
+  return h;
+}
 void OR(sim_set_thermal_seed)(void *h, uint64_t seed, uint32_t stream) {
   sim_t *s = (sim_t *)h;
   s->seed = seed; s->stream = stream;
@@ -998,7 +1017,8 @@ static int param_nonzero(sim_t *s, int id, int nc) {
 static void set_demag(sim_t *s, real *B) {
   int n = s->n;
   if (!s->enable_demag) { memset(B, 0, sizeof(real) * 3 * n); return; }
-  if (!s->demag) s->demag = OR(demag_new)(s->N, s->pbc, s->cs, s->demag_acc, 0);
+  if (!s->demag)
+    s->demag = s->newell ? sim_newell_plan(s) : OR(demag_new)(s->N, s->pbc, s->cs, s->demag_acc, 0);
   OR(demag_exec)(s->demag, B, B + n, B + 2 * n, s->m, s->m + n, s->m + 2 * n, s->vol, PA(UM_P_MSAT, 0));
 }
 

@@ -426,6 +426,44 @@ int um_demag_create(int prec, int Nx, int Ny, int Nz, const int pbc[3], const do
     *plan = demag_create(prec, N, P, Kd, s);
   });
 }
+int um_demag_create_newell(int prec, int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
+                           int asymptotic_radius, int zero_self_demag, void **plan) {
+  return guard([&] {
+    check_prec(prec);
+    UM_REQUIRE(plan && cs && Nx > 0 && Ny > 0 && Nz > 0, UM_EINVAL, "um_demag_create_newell: bad arguments");
+    const int zero[3] = {0, 0, 0};
+    const int *pb = pbc ? pbc : zero;
+    const int N[3] = {Nx, Ny, Nz};
+    int P[3];
+    pad_size(N, pb, P);
+    hipStream_t s = lib_stream();
+    const size_t np = (size_t)P[0] * P[1] * P[2];
+    DevBuf K(sizeof(double) * 6 * np);
+    double *Kd[6];
+    for (int c = 0; c < 6; c++) Kd[c] = K.as<double>() + c * np;
+    demag_kernel_newell_device(N, pb, cs, asymptotic_radius, zero_self_demag, Kd, P, s);
+    *plan = demag_create(prec, N, P, Kd, s);
+  });
+}
+int um_demag_kernel_newell(int Nx, int Ny, int Nz, const int pbc[3], const double cs[3], int asymptotic_radius,
+                           int zero_self_demag, double *const K[6]) {
+  return guard([&] {
+    UM_REQUIRE(cs && K && Nx > 0 && Ny > 0 && Nz > 0, UM_EINVAL, "um_demag_kernel_newell: bad arguments");
+    const int zero[3] = {0, 0, 0};
+    const int *pb = pbc ? pbc : zero;
+    const int N[3] = {Nx, Ny, Nz};
+    int P[3];
+    pad_size(N, pb, P);
+    hipStream_t s = lib_stream();
+    const size_t np = (size_t)P[0] * P[1] * P[2];
+    DevBuf Kb(sizeof(double) * 6 * np);
+    double *Kd[6];
+    for (int c = 0; c < 6; c++) Kd[c] = Kb.as<double>() + c * np;
+    demag_kernel_newell_device(N, pb, cs, asymptotic_radius, zero_self_demag, Kd, P, s);
+    for (int c = 0; c < 6; c++) UM_HIP(hipMemcpyAsync(K[c], Kd[c], sizeof(double) * np, hipMemcpyDeviceToHost, s));
+    UM_HIP(hipStreamSynchronize(s));
+  });
+}
 int um_demag_create_from_kernel(int prec, int Nx, int Ny, int Nz, const int P[3], const double *const K[6],
                                 void **plan) {
   return guard([&] {
@@ -544,6 +582,13 @@ int um_sim_set_flags(void *sim, int d, int ob, int pr, int hs) {
   return guard([&] { SIMCHK(sim); SIM(sim)->set_flags(d, ob, pr, hs); });
 }
 int um_sim_set_demag_accuracy(void *sim, double a) { return guard([&] { SIMCHK(sim); SIM(sim)->set_demag_accuracy(a); }); }
+int um_sim_set_demag_newell(void *sim, int enable, int asymptotic_radius, int zero_self_demag) {
+  return guard([&] {
+    SIMCHK(sim);
+    UM_REQUIRE(zero_self_demag >= 0, UM_EINVAL, "zero_self_demag must be >= 0");
+    SIM(sim)->set_demag_newell(enable, asymptotic_radius, zero_self_demag);
+  });
+}
 int um_sim_set_thermal_seed(void *sim, uint64_t seed, uint32_t stream) {
   return guard([&] { SIMCHK(sim); SIM(sim)->set_thermal_seed(seed, stream); });
 }

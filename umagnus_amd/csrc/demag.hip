@@ -868,7 +868,8 @@ __global__ void k_demag_bf(BfArgs a) {
     const int nz = (int)(fmax(cs[2] / maxSize, 1.0) + 0.5);
     nv *= 2;
     nw *= 2;
-    const double scale = 1 / (double)(nv * nw * nx * ny * nz);
+    // 64-bit count as Go's int: 32 bits overflow at DemagAccuracy ~40
+    const double scale = 1 / (double)((long long)nv * nw * nx * ny * nz);
     const double surface = cs[v] * cs[w];
     const double charge = surface * scale;
     const double pu1 = cs[u] / 2.;
@@ -1524,6 +1525,8 @@ void pad_size(const int N[3], const int pbc[3], int P[3]) {
   }
 }
 
+void demag_kernel_mirror(const int P[3], double *const K[6], hipStream_t s);
+
 // Brute-force tensor on the GPU into device arrays Kdev[6] of size P (FP64).
 void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3], double accuracy,
                             double *const Kdev[6], int P[3], hipStream_t s) {
@@ -1550,12 +1553,21 @@ void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3]
   const long total = (long)a.ext[0] * a.ext[1] * a.ext[2] * 3;
   hipLaunchKernelGGL(k_demag_bf, dim3(blocks_for(total, 64, 1 << 16)), dim3(64), 0, s, a);
   UM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_demag_mirror, dim3(blocks_for(np, 256, 1 << 16)), dim3(256), 0, s, a);
-  UM_HIP(hipGetLastError());
+  demag_kernel_mirror(P, Kdev, s);
   if (P[2] == 1) {
     fill_value<double>(Kdev[3], 0.0, np, s);
     fill_value<double>(Kdev[4], 0.0, np, s);
   }
+}
+
+// Fill the octants with a coordinate above P/2 from their mirror images.
+void demag_kernel_mirror(const int P[3], double *const K[6], hipStream_t s) {
+  BfArgs a{};
+  for (int i = 0; i < 3; i++) a.P[i] = P[i];
+  for (int c = 0; c < 6; c++) a.K[c] = K[c];
+  const size_t np = (size_t)P[0] * P[1] * P[2];
+  hipLaunchKernelGGL(k_demag_mirror, dim3(blocks_for(np, 256, 1 << 16)), dim3(256), 0, s, a);
+  UM_HIP(hipGetLastError());
 }
 
 void demag_exec(DemagBase *d, void *const B[3], const void *const m[3], const void *vol, const void *Ms,

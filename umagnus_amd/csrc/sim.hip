@@ -33,6 +33,7 @@ struct Sim : SimBase {
   DevBuf regions, aLUT, dLUT, bextLUT, frozenLUT, vol;
   bool has_regions = false, has_dmi = false, has_bextlut = false, has_frozen = false, has_vol = false;
   int enable_demag = 1, openbc = 0, precess = 1, heun_sqrt3 = 1;
+  int newell = 0, newell_arad = 32, newell_zsd = 0;   // asymptotic_radius default, engine/demag.go:27
   double demag_acc = 6.0;
   DemagBase *demag = nullptr;
   uint64_t seed = 0;
@@ -198,6 +199,17 @@ struct Sim : SimBase {
     if (a != demag_acc && demag) { demag_destroy(demag); demag = nullptr; }
     demag_acc = a;
   }
+  // EnableNewellDemag with asymptotic_radius and zero_self_demag (engine/demag.go:26-28)
+  void set_demag_newell(int enable, int arad, int zsd) override {
+    gen++;
+    enable = enable != 0;
+    if ((enable != newell || arad != newell_arad || zsd != newell_zsd) && demag) {
+      UM_HIP(hipStreamSynchronize(st));
+      demag_destroy(demag);
+      demag = nullptr;
+    }
+    newell = enable; newell_arad = arad; newell_zsd = zsd;
+  }
   void set_thermal_seed(uint64_t s, uint32_t strm) override { gen++; seed = s; tstream = strm; }
   void invalidate_fsal() { k1_valid = false; }
 
@@ -239,7 +251,8 @@ struct Sim : SimBase {
     double *Kd[6];
     for (int c = 0; c < 6; c++) Kd[c] = K.as<double>() + c * np;
     int P2[3];
-    demag_kernel_bf_device(N, pbc, cs, demag_acc, Kd, P2, st);
+    if (newell) demag_kernel_newell_device(N, pbc, cs, newell_arad, newell_zsd, Kd, P2, st);
+    else demag_kernel_bf_device(N, pbc, cs, demag_acc, Kd, P2, st);
     demag = demag_create(sizeof(T) == 4 ? UM_F32 : UM_F64, N, P2, Kd, st);
     UM_HIP(hipStreamSynchronize(st));
   }

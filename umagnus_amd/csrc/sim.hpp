@@ -10,6 +10,9 @@ struct DemagBase;
 DemagBase *demag_create(int prec, const int N[3], const int P[3], double *const Kdev[6], hipStream_t s);
 void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3], double accuracy,
                             double *const Kdev[6], int P[3], hipStream_t s);
+// Newell tensor (mag/newellDemagKernel.go:1108-1324) into Kdev[6], FP64
+void demag_kernel_newell_device(const int N[3], const int pbc[3], const double cs[3], int asymptotic_radius,
+                                int zero_self_demag, double *const Kdev[6], int P[3], hipStream_t s);
 void pad_size(const int N[3], const int pbc[3], int P[3]);
 void demag_exec(DemagBase *d, void *const B[3], const void *const m[3], const void *vol, const void *Ms,
                 double Ms_mul, hipStream_t s);
@@ -32,6 +35,7 @@ struct SimBase {
   virtual void set_geometry(const void *vol) = 0;
   virtual void set_flags(int demag, int openbc, int precess, int heun_sqrt3) = 0;
   virtual void set_demag_accuracy(double a) = 0;
+  virtual void set_demag_newell(int enable, int asymptotic_radius, int zero_self_demag) = 0;
   virtual void set_thermal_seed(uint64_t seed, uint32_t stream) = 0;
   virtual void set_m(const void *const m[3]) = 0;
   virtual void get_m(void *const m[3]) = 0;

@@ -35,6 +35,7 @@ EXPORTS = [
     "um_mul", "um_reduce_sum", "um_reduce_dot", "um_reduce_maxabs", "um_reduce_maxvecnorm",
     "um_reduce_maxvecdiff", "um_demag_create", "um_demag_create_from_kernel", "um_demag_exec",
     "um_demag_destroy", "um_demag_info", "um_demag_kspec", "um_demag_kernel_bf",
+    "um_demag_create_newell", "um_demag_kernel_newell", "um_sim_set_demag_newell",
     "um_sim_create", "um_sim_destroy", "um_sim_set_scalar", "um_sim_set_vector", "um_sim_set_regions",
     "um_sim_set_exchange_luts", "um_sim_set_bext_lut", "um_sim_set_frozen_lut", "um_sim_set_geometry",
     "um_sim_set_flags", "um_sim_set_demag_accuracy", "um_sim_set_thermal_seed", "um_sim_set_m",
@@ -275,12 +276,16 @@ def MaxVecDiff(a, b, sqrt3_quirk=False):
 class DemagConvolution:
     """opencl.DemagConvolution (opencl/conv_demag.go:14-23): NewDemag + Exec."""
 
-    def __init__(self, N, cellsize, pbc=(0, 0, 0), accuracy=6.0, precision=F32, K=None):
+    def __init__(self, N, cellsize, pbc=(0, 0, 0), accuracy=6.0, precision=F32, K=None, newell=False,
+                 asymptotic_radius=32, zero_self_demag=0):
         init()
         self.h = C.c_void_p()
         self.N = tuple(N)
         self.precision = precision
-        if K is None:
+        if K is None and newell:
+            _call("um_demag_create_newell", precision, N[0], N[1], N[2], (C.c_int * 3)(*pbc),
+                  (C.c_double * 3)(*cellsize), int(asymptotic_radius), int(zero_self_demag), C.byref(self.h))
+        elif K is None:
             _call("um_demag_create", precision, N[0], N[1], N[2], (C.c_int * 3)(*pbc),
                   (C.c_double * 3)(*cellsize), C.c_double(accuracy), C.byref(self.h))
         else:
@@ -318,9 +323,7 @@ class DemagConvolution:
             pass
 
 
-def demag_kernel_bf(N, cellsize, pbc=(0, 0, 0), accuracy=6.0):
-    """CalcDemagKernel (mag/demagkernel.go:115) on the GPU in FP64 -> 6 host arrays."""
-    init()
+def _padded(N, pbc):
     P = []
     for i in range(3):
         if pbc[i]:
@@ -330,9 +333,24 @@ def demag_kernel_bf(N, cellsize, pbc=(0, 0, 0), accuracy=6.0):
         else:
             P.append(2 * N[i] - 1)
     K = [np.zeros((P[2], P[1], P[0]), np.float64) for _ in range(6)]
-    arr = (C.c_void_p * 6)(*[k.ctypes.data for k in K])
+    return K, (C.c_void_p * 6)(*[k.ctypes.data for k in K])
+
+
+def demag_kernel_bf(N, cellsize, pbc=(0, 0, 0), accuracy=6.0):
+    """CalcDemagKernel (mag/demagkernel.go:115) on the GPU in FP64 -> 6 host arrays."""
+    init()
+    K, arr = _padded(N, pbc)
     _call("um_demag_kernel_bf", N[0], N[1], N[2], (C.c_int * 3)(*pbc), (C.c_double * 3)(*cellsize),
           C.c_double(accuracy), arr)
+    return K
+
+
+def demag_kernel_newell(N, cellsize, pbc=(0, 0, 0), asymptotic_radius=32, zero_self_demag=0):
+    """CalcNewellDemagKernel (mag/newellDemagKernel.go:1108) on the GPU in FP64 -> 6 host arrays."""
+    init()
+    K, arr = _padded(N, pbc)
+    _call("um_demag_kernel_newell", N[0], N[1], N[2], (C.c_int * 3)(*pbc), (C.c_double * 3)(*cellsize),
+          int(asymptotic_radius), int(zero_self_demag), arr)
     return K
 
 
@@ -410,6 +428,10 @@ class Sim:
 
     def set_demag_accuracy(self, acc):
         _call("um_sim_set_demag_accuracy", self.h, C.c_double(acc))
+
+    def set_demag_newell(self, enable=1, asymptotic_radius=32, zero_self_demag=0):
+        """EnableNewellDemag = enable (engine/demag.go:26-56)."""
+        _call("um_sim_set_demag_newell", self.h, int(enable), int(asymptotic_radius), int(zero_self_demag))
 
     def set_thermal_seed(self, seed, stream=0):
         _call("um_sim_set_thermal_seed", self.h, C.c_uint64(seed), C.c_uint32(stream))
