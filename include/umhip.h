@@ -191,6 +191,29 @@ int um_sim_set_geometry(void *sim, const void *vol);
 /* EnableDemag, OpenBC, DoPrecess, and the MaxVecDiff sqrt(3) widening in Heun */
 int um_sim_set_flags(void *sim, int enable_demag, int openbc, int precess, int heun_sqrt3);
 int um_sim_set_demag_accuracy(void *sim, double accuracy);
+/* ---- discrete adjoint of the fixed-step RK4 path ------------------------------
+ * No counterpart in the reference (it has no differentiation); these serve
+ * callers that need gradients of a loss of the final magnetization: inverse
+ * design and parameter fitting.  Valid for SetSolver(4) with FixDt > 0,
+ * precession on, no thermal field, no cubic anisotropy, no frozen spins, no
+ * geometry, and Msat > 0 in every cell.
+ *
+ * um_sim_tape_begin: subsequent um_sim_step/steps/run calls record their
+ *   stages (up to max_steps steps).  checkpoint_every = 0 keeps the whole
+ *   trajectory in device memory (28 values per cell per step); K > 0 keeps m
+ *   every K steps and K steps of stage data, recomputing the rest during the
+ *   backward sweep (K ~ sqrt(max_steps) minimises memory).
+ * um_sim_backward: given dm = dL/dm at the last recorded step (3 device arrays
+ *   of the handle's precision), writes dL/dm at the first recorded step into
+ *   dm0 (device, may be NULL), dL/dMsat per cell into dms (device, N values,
+ *   may be NULL) and dL/dB_ext of a uniform field into dbext (host, 3 doubles,
+ *   may be NULL).  May be called repeatedly with other cotangents.
+ * um_sim_set_m is refused while the tape records; any parameter change
+ * invalidates the tape (UM_ESTATE). */
+int um_sim_tape_begin(void *sim, int max_steps, int checkpoint_every);
+int um_sim_tape_end(void *sim);
+int um_sim_tape_steps(void *sim, long long *steps);
+int um_sim_backward(void *sim, const void *const dm[3], void *const dm0[3], void *dms, double dbext[3]);
 /* EnableNewellDemag (engine/demag.go:26-56): enable=1 selects the Newell tensor
  * (with asymptotic_radius and zero_self_demag as in um_demag_create_newell) in
  * place of the brute-force one; demag itself stays switched by enable_demag. */

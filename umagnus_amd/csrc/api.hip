@@ -582,6 +582,24 @@ int um_sim_set_flags(void *sim, int d, int ob, int pr, int hs) {
   return guard([&] { SIMCHK(sim); SIM(sim)->set_flags(d, ob, pr, hs); });
 }
 int um_sim_set_demag_accuracy(void *sim, double a) { return guard([&] { SIMCHK(sim); SIM(sim)->set_demag_accuracy(a); }); }
+int um_sim_tape_begin(void *sim, int max_steps, int checkpoint_every) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->tape_begin(max_steps, checkpoint_every); });
+}
+int um_sim_tape_end(void *sim) { return guard([&] { SIMCHK(sim); SIM(sim)->tape_end(); }); }
+int um_sim_tape_steps(void *sim, long long *steps) {
+  return guard([&] {
+    SIMCHK(sim);
+    UM_REQUIRE(steps, UM_EINVAL, "NULL steps");
+    *steps = SIM(sim)->tape_steps();
+  });
+}
+int um_sim_backward(void *sim, const void *const dm[3], void *const dm0[3], void *dms, double dbext[3]) {
+  return guard([&] {
+    SIMCHK(sim);
+    UM_REQUIRE(dm, UM_EINVAL, "NULL cotangent");
+    SIM(sim)->backward(dm, dm0, dms, dbext);
+  });
+}
 int um_sim_set_demag_newell(void *sim, int enable, int asymptotic_radius, int zero_self_demag) {
   return guard([&] {
     SIMCHK(sim);

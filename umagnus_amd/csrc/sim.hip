@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "adjoint.hpp"
 #include "field.hpp"
 #include "sim.hpp"
 #include "vec.hpp"
@@ -13,6 +14,7 @@
 namespace um {
 
 static constexpr double KB = 1.380650424e-23;  // mag/constants.go:9
+static constexpr double MU0_S = 4 * M_PI * 1e-7;   // mag/constants.go:8
 static constexpr int NREGION = 256;
 static constexpr int SYMLUT = NREGION * (NREGION + 1) / 2;
 
@@ -215,6 +217,7 @@ struct Sim : SimBase {
 
   // ---- m --------------------------------------------------------------------------
   void set_m(const void *const mh[3]) override {
+    UM_REQUIRE(!tape_on, UM_ESTATE, "m cannot be set while the adjoint tape records (end it first)");
     for (int c = 0; c < 3; c++)
       UM_HIP(hipMemcpyAsync(m + c * n, mh[c], sizeof(T) * n, hipMemcpyHostToDevice, st));
     normalize(m);
@@ -422,8 +425,10 @@ struct Sim : SimBase {
     UM_REQUIRE(dt > 0, UM_ESTATE, "Time step too small");
   }
   // m_out = normalize(sum fac_j src_j)
-  void stage(T *dst, int ns, const T *const *src, const T *fac, bool norm, bool group3 = false) {
+  void stage(T *dst, int ns, const T *const *src, const T *fac, bool norm, bool group3 = false,
+             T *nrm = nullptr) {
     StageArgs<T> a{};
+    a.nrm = nrm;
     a.nsrc = ns;
     a.ncomp = 3;
     a.group3 = group3;
@@ -710,7 +715,203 @@ struct Sim : SimBase {
     }
   }
 
+  // ---- discrete adjoint of fixed-step RK4 ------------------------------------------------
+  // While the tape records, every step stores its stage inputs, the norms of
+  // the stage sums before normalisation and the stage demag fields in a
+  // segment tape of K steps; m at every K-th step is kept as a checkpoint.
+  // The reverse sweep walks the segments backwards, recomputing a segment from
+  // its checkpoint when the tape no longer holds it.  K = the step capacity
+  // (default) keeps the whole trajectory in HBM and recomputes nothing;
+  // K ~ sqrt(steps) bounds the memory to ~2 sqrt(steps) states.
+  bool tape_on = false;
+  long tape_K = 0, tape_cap = 0, tape_n = 0, seg_start = -1, tape_gen = -1;
+  DevBuf tape_ckpt, tape_seg;
+  // segment tape: M[0..K] (3n each), then per step 25n: U[3] (3n), NRM[4] (n), BD[4] (3n)
+  T *tM(long k) { return tape_seg.as<T>() + (size_t)k * 3 * n; }
+  T *tStep(long k) { return tape_seg.as<T>() + (size_t)(tape_K + 1) * 3 * n + (size_t)k * 25 * n; }
+  T *tU(long k, int s) { return tStep(k) + (size_t)s * 3 * n; }
+  T *tN(long k, int s) { return tStep(k) + 9 * n + (size_t)s * n; }
+  T *tBD(long k, int s) { return tStep(k) + 13 * n + (size_t)s * 3 * n; }
+
+  // torqueFn at an explicit input, the demag field into bd_out (kept for the adjoint)
+  void torque_at(const T *u, T *bd_out, T *dst) {
+    FieldArgs<T> a = field_args(u);
+    if (enable_demag) {
+      ensure_demag();
+      void *B[3] = {bd_out, bd_out + n, bd_out + 2 * n};
+      const void *mv[3] = {u, u + n, u + 2 * n};
+      demag_exec(demag, B, mv, vol.p, p[UM_P_MSAT][0].arr.p, p[UM_P_MSAT][0].mul, st);
+    }
+    for (int c = 0; c < 3; c++) {
+      a.bd[c] = enable_demag ? bd_out + c * n : nullptr;
+      a.tau[c] = dst + c * n;
+      a.beff[c] = nullptr;
+    }
+    launch_field_torque(a, st);
+    nevals++;
+  }
+  // RK4.Step (engine/rk4.go:11-69) with FixDt on tape slot k: M[k] -> M[k+1]
+  void step_rk4_taped(long kk) {
+    const T h = (T)(fixdt * gammaLL);
+    T *k1 = k[0], *k2 = k[1], *k3 = k[2], *k4 = k[3];
+    const T *u1 = tM(kk);
+    torque_at(u1, tBD(kk, 0), k1);
+    { const T *s_[] = {u1, k1}; const T f[] = {T(1), T(1. / 2.) * h}; stage(tU(kk, 0), 2, s_, f, true, false, tN(kk, 0)); }
+    torque_at(tU(kk, 0), tBD(kk, 1), k2);
+    { const T *s_[] = {u1, k2}; const T f[] = {T(1), T(1. / 2.) * h}; stage(tU(kk, 1), 2, s_, f, true, false, tN(kk, 1)); }
+    torque_at(tU(kk, 1), tBD(kk, 2), k3);
+    { const T *s_[] = {u1, k3}; const T f[] = {T(1), T(1.) * h}; stage(tU(kk, 2), 2, s_, f, true, false, tN(kk, 2)); }
+    torque_at(tU(kk, 2), tBD(kk, 3), k4);
+    const T *s_[] = {u1, k1, k2, k3, k4};
+    const T f[] = {T(1), T(1. / 6.) * h, T(1. / 3.) * h, T(1. / 3.) * h, T(1. / 6.) * h};
+    stage(tM(kk + 1), 5, s_, f, true, false, tN(kk, 3));
+  }
+  void copy3(T *dst, const T *src) {
+    UM_HIP(hipMemcpyAsync(dst, src, sizeof(T) * 3 * n, hipMemcpyDeviceToDevice, st));
+  }
+  void step_tape() {
+    UM_REQUIRE(tape_n < tape_cap, UM_ESTATE, "adjoint tape full (raise max_steps)");
+    UM_REQUIRE(gen == tape_gen, UM_ESTATE, "parameters changed while the adjoint tape records");
+    const long kk = tape_n % tape_K;
+    if (kk == 0) {
+      copy3(tape_ckpt.as<T>() + (size_t)(tape_n / tape_K) * 3 * n, m);
+      copy3(tM(0), m);
+      seg_start = tape_n;
+    }
+    dt = fixdt;
+    step_rk4_taped(kk);
+    copy3(m, tM(kk + 1));
+    Time += dt;
+    nsteps++;
+    tape_n++;
+  }
+  void tape_begin(int cap, int every) override {
+    UM_REQUIRE(cap > 0 && every >= 0, UM_EINVAL, "tape: max_steps > 0, checkpoint_every >= 0");
+    UM_REQUIRE(solver == UM_SOLVER_RK4 && fixdt > 0, UM_EUNSUP, "the adjoint needs the RK4 solver with FixDt > 0");
+    UM_REQUIRE(!nz(UM_P_TEMP), UM_EUNSUP, "no adjoint with a thermal field");
+    UM_REQUIRE(!(nz(UM_P_KC1) || nz(UM_P_KC2) || nz(UM_P_KC3)), UM_EUNSUP, "no adjoint of cubic anisotropy");
+    UM_REQUIRE(precess && !has_frozen && !has_vol && !relaxing, UM_EUNSUP,
+               "the adjoint needs precession on, no frozen spins and no geometry");
+    tape_end();
+    tape_K = every > 0 ? std::min<long>(every, cap) : cap;
+    const long nck = (cap + tape_K - 1) / tape_K;
+    tape_ckpt.alloc(sizeof(T) * 3 * n * nck);
+    tape_seg.alloc(sizeof(T) * ((size_t)(tape_K + 1) * 3 * n + (size_t)tape_K * 25 * n));
+    tape_cap = cap;
+    tape_n = 0;
+    seg_start = -1;
+    tape_on = true;
+    gen++;
+    tape_gen = gen;
+  }
+  void tape_end() override {
+    if (tape_seg.p || tape_ckpt.p) UM_HIP(hipStreamSynchronize(st));
+    tape_on = false;
+    tape_n = tape_cap = 0;
+    seg_start = -1;
+    tape_ckpt.release();
+    tape_seg.release();
+  }
+  long tape_steps() const override { return tape_n; }
+  // one step of the reverse sweep (slot kk): lm -> cotangent of the step's m0
+  void adjoint_step(long kk, T *&lm, T *&lacc, T *lv5, T *&lk, T *&lk2, T *w, T *D, double *gms, double *part) {
+    const T h = (T)(fixdt * gammaLL);
+    AdjFinalArgs<T> fa{};
+    for (int c = 0; c < 3; c++) {
+      fa.m1[c] = tM(kk + 1) + c * n;
+      fa.lm[c] = lm + c * n;
+      fa.lv5[c] = lv5 + c * n;
+      fa.lacc[c] = lacc + c * n;
+      fa.lk[c] = lk + c * n;
+    }
+    fa.nrm = tN(kk, 3);
+    fa.c4 = T(1. / 6.) * h;
+    fa.N = n;
+    launch_adj_final(fa, st);
+    for (int sidx = 4; sidx >= 1; sidx--) {
+      const T *u = sidx == 1 ? tM(kk) : tU(kk, sidx - 2);
+      AdjArgs<T> a{};
+      a.f = field_args(u);
+      for (int c = 0; c < 3; c++) {
+        a.u[c] = u + c * n;
+        a.lk[c] = lk + c * n;
+        a.w[c] = w + c * n;
+        a.bd[c] = enable_demag ? tBD(kk, sidx - 1) + c * n : nullptr;
+        a.D[c] = nullptr;
+        a.lv5[c] = lv5 + c * n;
+        a.lacc[c] = lacc + c * n;
+        a.lknext[c] = sidx == 1 ? nullptr : lk2 + c * n;
+      }
+      a.nrm = sidx == 1 ? nullptr : tN(kk, sidx - 2);
+      a.c5 = sidx == 2 ? T(1. / 6.) * h : T(1. / 3.) * h;
+      a.cs = sidx == 4 ? T(1.) * h : T(1. / 2.) * h;
+      a.mu0 = (T)MU0_S;
+      a.gms = gms;
+      a.gbext = part;
+      a.N = n;
+      launch_adj_gb(a, st);
+      if (enable_demag) {
+        // D = K * (Ms w) = K * gB: the forward plan with the mu0 Ms input scaling undone
+        void *B[3] = {D, D + n, D + 2 * n};
+        const void *mv[3] = {w, w + n, w + 2 * n};
+        demag_exec(demag, B, mv, nullptr, p[UM_P_MSAT][0].arr.p, p[UM_P_MSAT][0].mul / MU0_S, st);
+        for (int c = 0; c < 3; c++) a.D[c] = D + c * n;
+      }
+      launch_adj_stage(a, st);
+      std::swap(lk, lk2);
+    }
+    std::swap(lm, lacc);
+  }
+  void backward(const void *const gm[3], void *const gm0[3], void *gms_out, double *gbext_out) override {
+    UM_REQUIRE(tape_on && tape_n > 0, UM_ESTATE, "backward: no recorded steps (tape_begin, then steps)");
+    UM_REQUIRE(gen == tape_gen, UM_ESTATE, "parameters changed since the adjoint tape was started");
+    for (int c = 0; c < 3; c++) UM_REQUIRE(gm[c], UM_EINVAL, "backward: NULL cotangent");
+    if (enable_demag) ensure_demag();
+    const size_t vb = sizeof(T) * 3 * n;
+    DevBuf b_lm(vb), b_lacc(vb), b_lv5(vb), b_lk(vb), b_lk2(vb), b_w(vb), b_D;
+    if (enable_demag) b_D.alloc(vb);
+    DevBuf b_gms, b_part;
+    const long nb = adj_blocks(n);
+    if (gms_out) { b_gms.alloc(sizeof(double) * n); fill_bytes(b_gms.p, b_gms.bytes, st); }
+    if (gbext_out) { b_part.alloc(sizeof(double) * 3 * nb); fill_bytes(b_part.p, b_part.bytes, st); }
+    T *lm = b_lm.as<T>(), *lacc = b_lacc.as<T>(), *lk = b_lk.as<T>(), *lk2 = b_lk2.as<T>();
+    for (int c = 0; c < 3; c++)
+      UM_HIP(hipMemcpyAsync(lm + c * n, gm[c], sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+    const long K = tape_K, nseg = (tape_n + K - 1) / K;
+    for (long sg = nseg - 1; sg >= 0; sg--) {
+      const long s0 = sg * K, s1 = std::min(s0 + K, tape_n);
+      if (seg_start != s0) {
+        copy3(tM(0), tape_ckpt.as<T>() + (size_t)sg * 3 * n);
+        for (long t = s0; t < s1; t++) step_rk4_taped(t - s0);
+        seg_start = s0;
+      }
+      for (long t = s1 - 1; t >= s0; t--)
+        adjoint_step(t - s0, lm, lacc, b_lv5.as<T>(), lk, lk2, b_w.as<T>(), b_D.as<T>(), b_gms.as<double>(),
+                     b_part.as<double>());
+    }
+    if (gm0)
+      for (int c = 0; c < 3; c++)
+        if (gm0[c]) UM_HIP(hipMemcpyAsync(gm0[c], lm + c * n, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+    if (gms_out) launch_adj_cast<T>((T *)gms_out, b_gms.as<double>(), n, st);
+    if (gbext_out) {
+      std::vector<double> h(3 * nb);
+      UM_HIP(hipMemcpyAsync(h.data(), b_part.p, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, st));
+      UM_HIP(hipStreamSynchronize(st));
+      for (int c = 0; c < 3; c++) {
+        double t = 0;
+        for (long b = 0; b < nb; b++) t += h[3 * b + c];
+        gbext_out[c] = t;
+      }
+    }
+    UM_HIP(hipStreamSynchronize(st));
+  }
+
   void step() override {
+    if (tape_on) {
+      UM_REQUIRE(solver == UM_SOLVER_RK4 && fixdt > 0, UM_ESTATE, "the adjoint tape needs RK4 with FixDt");
+      step_tape();
+      return;
+    }
     if (profiling) UM_HIP(hipEventRecord(step_ev[0], st));
     switch (solver) {
       case UM_SOLVER_HEUN: step_heun(); break;
@@ -740,7 +941,7 @@ struct Sim : SimBase {
   // ---- graph replay (FixDt only) ---------------------------------------------------------
   void set_graph(int on) override { graph_mode = on; }
   long step_count() const override { return nsteps; }
-  bool graph_usable() const { return graph_mode && fixdt != 0 && !profiling; }
+  bool graph_usable() const { return graph_mode && fixdt != 0 && !profiling && !tape_on; }
   // one eager step (FSAL setup), then capture two steps if the cached graph is
   // stale; returns how many two-step replays fit before `stop`
   long graph_prepare(long stop) override {

@@ -36,6 +36,11 @@ struct SimBase {
   virtual void set_flags(int demag, int openbc, int precess, int heun_sqrt3) = 0;
   virtual void set_demag_accuracy(double a) = 0;
   virtual void set_demag_newell(int enable, int asymptotic_radius, int zero_self_demag) = 0;
+  // discrete RK4 adjoint: record steps, then a reverse sweep from dL/dm_final
+  virtual void tape_begin(int max_steps, int checkpoint_every) = 0;
+  virtual void tape_end() = 0;
+  virtual long tape_steps() const = 0;
+  virtual void backward(const void *const gm[3], void *const gm0[3], void *gms, double *gbext) = 0;
   virtual void set_thermal_seed(uint64_t seed, uint32_t stream) = 0;
   virtual void set_m(const void *const m[3]) = 0;
   virtual void get_m(void *const m[3]) = 0;

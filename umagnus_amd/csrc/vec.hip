@@ -68,6 +68,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<T> a) {
         r[1] = Y * inv;
         r[2] = Z * inv;
       }
+      if (a.nrm) a.nrm[i] = l;
     }
     for (int c = 0; c < nc; c++) a.dst[c][i] = r[c];
   }
@@ -123,11 +124,13 @@ __global__ __launch_bounds__(256) void k_stage_v(StageArgs<T> a) {
 #pragma unroll
       for (int w = 0; w < W; w++) vv[w] = T(1);
     }
+    T ln[W];
 #pragma unroll
     for (int w = 0; w < W; w++) {
       const T v = vv[w];
       const T X = v * r[0][w], Y = v * r[1][w], Z = v * r[2][w];
       const T l = sqrt(X * X + Y * Y + Z * Z);
+      ln[w] = l;
       const T inv = (l != T(0)) ? T(1) / l : T(0);
       if (v == T(0)) {
         r[0][w] = r[1][w] = r[2][w] = T(0);
@@ -136,6 +139,11 @@ __global__ __launch_bounds__(256) void k_stage_v(StageArgs<T> a) {
         r[1][w] = Y * inv;
         r[2][w] = Z * inv;
       }
+    }
+    if (a.nrm) {
+      V v;
+      __builtin_memcpy(&v, ln, sizeof(V));
+      st_nt<V>(a.nrm, g, v);
     }
   }
 #pragma unroll
@@ -156,6 +164,7 @@ static bool stage_vectorisable(const StageArgs<T> &a) {
     for (int k = 0; k < a.nsrc; k++)
       if (!al(a.src[k][c])) return false;
   }
+  if (a.normalize && a.nrm && !al(a.nrm)) return false;
   return !(a.normalize && a.vol && !al(a.vol));
 }
 

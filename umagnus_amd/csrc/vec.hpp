@@ -22,6 +22,7 @@ struct StageArgs {
   T fac[8];
   T *dst[3];
   const T *vol;
+  T *nrm;                // when normalising: |sum| per cell (before the scaling) or nullptr
   long N;
 };
 template <typename T> void launch_stage(const StageArgs<T> &a, hipStream_t s);

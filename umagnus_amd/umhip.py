@@ -36,6 +36,7 @@ EXPORTS = [
     "um_reduce_maxvecdiff", "um_demag_create", "um_demag_create_from_kernel", "um_demag_exec",
     "um_demag_destroy", "um_demag_info", "um_demag_kspec", "um_demag_kernel_bf",
     "um_demag_create_newell", "um_demag_kernel_newell", "um_sim_set_demag_newell",
+    "um_sim_tape_begin", "um_sim_tape_end", "um_sim_tape_steps", "um_sim_backward",
     "um_sim_create", "um_sim_destroy", "um_sim_set_scalar", "um_sim_set_vector", "um_sim_set_regions",
     "um_sim_set_exchange_luts", "um_sim_set_bext_lut", "um_sim_set_frozen_lut", "um_sim_set_geometry",
     "um_sim_set_flags", "um_sim_set_demag_accuracy", "um_sim_set_thermal_seed", "um_sim_set_m",
@@ -512,6 +513,45 @@ class Sim:
     def relax(self, torque_threshold=-1.0):
         """Relax(), engine/relax.go:24-100 (um_sim_relax)."""
         _call("um_sim_relax", self.h, C.c_double(torque_threshold))
+
+    # ---- discrete RK4 adjoint ------------------------------------------------------
+    def tape_begin(self, max_steps, checkpoint_every=0):
+        """Record the following fixed-step RK4 steps for backward() (um_sim_tape_begin)."""
+        _call("um_sim_tape_begin", self.h, int(max_steps), int(checkpoint_every))
+
+    def tape_end(self):
+        _call("um_sim_tape_end", self.h)
+
+    def tape_steps(self):
+        o = C.c_longlong()
+        _call("um_sim_tape_steps", self.h, C.byref(o))
+        return o.value
+
+    def backward(self, dm):
+        """Reverse sweep from dL/dm_final (host array, shape (3, Nz, Ny, Nx)).
+        Returns (dL/dm0, dL/dMsat per cell, dL/dB_ext uniform) as host arrays."""
+        n = int(np.prod(self.N))
+        it = self.dtype.itemsize
+        dm = np.ascontiguousarray(dm, self.dtype).reshape(3, n)
+        bufs = []
+        try:
+            for _ in range(7):
+                p = C.c_void_p()
+                _call("um_malloc", C.byref(p), C.c_size_t(n * it))
+                bufs.append(p)
+            for c in range(3):
+                _call("um_memcpy_h2d", bufs[c], dm[c].ctypes.data_as(C.c_void_p), C.c_size_t(n * it))
+            gb = (C.c_double * 3)()
+            _call("um_sim_backward", self.h, (C.c_void_p * 3)(*bufs[:3]), (C.c_void_p * 3)(*bufs[3:6]),
+                  bufs[6], gb)
+            out = np.zeros((4, n), self.dtype)
+            for c in range(4):
+                _call("um_memcpy_d2h", out[c].ctypes.data_as(C.c_void_p), bufs[3 + c], C.c_size_t(n * it))
+        finally:
+            for p in bufs:
+                _call("um_free", p)
+        shape = (self.N[2], self.N[1], self.N[0])
+        return out[:3].reshape((3,) + shape), out[3].reshape(shape), np.array(list(gb))
 
     def set_graph(self, on):
         """Replay fixed-dt steps as a captured HIP graph (um_sim_set_graph)."""
