@@ -6,7 +6,7 @@ seeded uniform-random unit m, RK45DP at fixed dt = 1e-14 s.  A step is one
 RK45DP step (six effective-field + torque evaluations, FSAL) over the whole
 grid with m resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batched]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batched] [--no-adjoint]
 
 Prints one JSON line.  `value` is Mcell*steps/s; `roofline` prices one
 effective-field + torque evaluation (the demag FFT passes plus the fused
@@ -14,7 +14,8 @@ field/torque kernel, timed with HIP events on the simulation stream over the
 timed region) at its algorithmic HBM bytes against the 8 TB/s HBM3E peak.
 `batched` compares a 64-member thermal Heun ensemble stepped together
 (graph replays on per-member streams) with the same members stepped one
-after another.
+after another; `adjoint` times the gradient of a 2000-step RK4 film run
+(taped forward + reverse sweep) against the plain forward run.
 """
 import argparse
 import json
@@ -142,6 +143,79 @@ def batched(steps=100):
             "value": round(res["batched"], 2), "speedup": round(res["batched"] / res["plain"], 3)}
 
 
+def adjoint_workload(seed=6):
+    """Differentiation workload: 256 x 256 x 1 film of 2 x 2 x 1 nm cells, Ms 8e5
+    A/m (per-cell map), Aex 1.3e-11 J/m, Dind 2e-3 J/m^2, alpha 0.3, seeded
+    random m, RK4 at FixDt 1e-13 s; loss = sum |m_final - target|^2 with a
+    seeded random target texture."""
+    from umagnus_amd.params import exchange_lut
+    N, cs = (256, 256, 1), (2e-9, 2e-9, 1e-9)
+    shape = (N[2], N[1], N[0])
+    rng = np.random.default_rng(seed)
+    m0 = rng.standard_normal((3,) + shape)
+    m0 /= np.linalg.norm(m0, axis=0)
+    tgt = rng.standard_normal((3,) + shape)
+    tgt /= np.linalg.norm(tgt, axis=0)
+    Ms = np.full(shape, 8e5)
+
+    def make(mod, dtype, Ms_map=Ms, bext=(0.0, 0.0, 0.0)):
+        s = mod.Sim(N, cs, dtype=dtype)
+        s.set_scalar(mod.P_MSAT, Ms_map)
+        s.set_scalar(mod.P_ALPHA, 0.3)
+        s.set_vector(mod.P_BEXT, bext)
+        s.set_exchange_luts(exchange_lut(1.3e-11, dtype=dtype), exchange_lut(2e-3, dtype=dtype))
+        s.set_solver(mod.SOLVER_RK4)
+        s.set_solver_params(dt=1e-13, fixdt=1e-13)
+        s.set_m(m0)
+        return s
+    return N, m0, tgt, Ms, make
+
+
+def adjoint(steps=2000):
+    """Cost of the gradient (taped forward + reverse sweep) against a plain
+    forward run of the same steps (graph replays), FP32, with sqrt(steps)
+    checkpointing and with the whole trajectory kept in HBM."""
+    import torch
+    from umagnus_amd import umhip
+    N, m0, tgt, Ms, make = adjoint_workload()
+    sim = make(umhip, np.float32)
+    sim.set_graph(1)
+    sim.steps(2)                          # plans, graph capture
+    sim.set_m(m0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.steps(steps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    sim.set_graph(0)
+    res, fwd = {}, {}
+    for name, every in (("sqrt_checkpoint", int(math.ceil(math.sqrt(steps)))), ("full_tape", 0)):
+        sim.set_m(m0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.tape_begin(steps, every)
+        sim.steps(steps)
+        mN = sim.get_m()
+        fwd[name] = time.perf_counter() - t0
+        _, gms, gb = sim.backward(2 * (mN - tgt))
+        sim.tape_end()
+        torch.cuda.synchronize()
+        res[name] = time.perf_counter() - t0
+        ok = bool(np.isfinite(gms).all() and np.isfinite(gb).all())
+    return {"workload": "256x256x1 cells of 2x2x1 nm, Ms 8e5 (per-cell map), Aex 1.3e-11, Dind 2e-3, "
+                        "alpha 0.3, RK4 FixDt 1e-13 s, FP32; loss sum |m - target|^2; gradients wrt "
+                        "per-cell Ms, uniform B_ext and m0",
+            "steps": steps, "forward_s": round(t_fwd, 4),
+            "gradient_s": round(res["sqrt_checkpoint"], 4),
+            "taped_forward_s": round(fwd["sqrt_checkpoint"], 4),
+            "reverse_sweep_s": round(res["sqrt_checkpoint"] - fwd["sqrt_checkpoint"], 4),
+            "cost_vs_forward": round(res["sqrt_checkpoint"] / t_fwd, 3),
+            "checkpoint_every": int(math.ceil(math.sqrt(steps))),
+            "full_tape_gradient_s": round(res["full_tape"], 4),
+            "full_tape_cost_vs_forward": round(res["full_tape"] / t_fwd, 3),
+            "finite": ok}
+
+
 def cpu_baseline(N, cs):
     """The FP32 oracle (C, OpenMP) on the host: one RK45DP step of the same
     workload after one warm-up step (setup and tensor excluded)."""
@@ -172,6 +246,7 @@ def main():
     ap.add_argument("--nz", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -247,6 +322,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batched:
         out["batched"] = batched()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, cs)
     print(json.dumps(out), flush=True)

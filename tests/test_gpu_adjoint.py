@@ -244,3 +244,56 @@ def test_autograd_function(gpu):
     L, _, gms, gb = _gpu_grads(gpu, N, cs, Ms0, bext0, m0, target, nsteps, np.float64)
     assert abs(float(loss.detach()) - L) <= 1e-12 * L
     assert np.array_equal(Ms.grad.numpy(), gms) and np.array_equal(bext.grad.numpy(), gb)
+
+
+def test_differentiation_workload_fd(gpu):
+    """The 256 x 256 x 1 film, 2000 RK4 steps: FP64 adjoint (sqrt(N)
+    checkpoints) vs central differences on 64 random cells, the perturbed runs
+    stepped together as one ensemble.  Central differences with h = 1e-4 Ms
+    carry their own O(h^2) truncation error (measured: 1.6e-4 normwise over the
+    64 cells, up to 3e-3 on a cell with a small gradient), so the per-cell
+    check is against the Richardson combination of h and h/2 (error O(h^4)):
+    the adjoint matches it to ~5e-6.  The FP32 adjoint follows the FP64 one."""
+    import bench
+    N, m0, tgt, Ms, make = bench.adjoint_workload()
+    steps = 2000
+    grads = {}
+    for dt in (np.float64, np.float32):
+        s = make(gpu, dt)
+        s.tape_begin(steps, 45)
+        s.steps(steps)
+        _, gms, gb = s.backward(2 * (s.get_m() - tgt))
+        s.tape_end()
+        grads[dt] = (gms, gb)
+        del s
+        _say("adjoint done", dt)
+    gms, gb = grads[np.float64]
+    rng = np.random.default_rng(64)
+    flat = rng.choice(gms.size, 64, replace=False)
+    sims = []
+    for i in flat:
+        for d in (1e-4, -1e-4, 5e-5, -5e-5):
+            Mp = Ms.copy().reshape(-1)
+            Mp[i] += d * Ms.reshape(-1)[i]
+            s = make(gpu, np.float64, Ms_map=Mp.reshape(Ms.shape))
+            s.set_graph(1)
+            sims.append(s)
+    gpu.sims_steps(sims, steps)
+    _say("fd runs done")
+    g = gms.reshape(-1)[flat]
+    L = np.array([_loss(s.get_m(), tgt) for s in sims]).reshape(64, 4)
+    h = 1e-4 * Ms.reshape(-1)[flat]
+    fd1 = (L[:, 0] - L[:, 1]) / (2 * h)
+    fd2 = (L[:, 2] - L[:, 3]) / h
+    rich = (4 * fd2 - fd1) / 3
+    e_norm = np.linalg.norm(g - fd1) / np.linalg.norm(fd1)
+    e_cell = np.abs(g - rich) / np.abs(rich)
+    _say("normwise vs FD(h=1e-4): %.2e  max per-cell vs Richardson: %.2e  max per-cell vs FD(h): %.2e"
+         % (e_norm, e_cell.max(), (np.abs(g - fd1) / np.abs(fd1)).max()))
+    assert e_norm <= 5e-4, e_norm
+    assert (np.abs(g - rich) <= 2e-5 * np.abs(rich) + 1e-7 * np.abs(gms).max()).all(), e_cell.max()
+    g32, gb32 = grads[np.float32]
+    e32 = np.abs(g32 - gms).max() / np.abs(gms).max()
+    eb32 = np.abs(gb32 - gb).max() / np.abs(gb).max()
+    _say("fp32 vs fp64: Ms %.2e  Bext %.2e" % (e32, eb32))
+    assert e32 <= 2e-2 and eb32 <= 2e-2

@@ -1,9 +1,11 @@
 // Kernels of the discrete RK4 adjoint (see adjoint.hpp for the linearisation).
-// Per stage: k_adj_gb turns the torque cotangent into the field cotangent
-// (stored as gB/Ms, the input of both transposed field operators), the demag
-// plan convolves Ms*w, and k_adj_stage gathers the transposed exchange/DMI
-// stencil, the demag and anisotropy terms and the direct torque term, applies
-// the transposed normalisation and forms the next stage's torque cotangent.
+// Per step: k_adj_final transposes the final normalisation and turns the
+// cotangent on k4 into stage 4's field cotangent w = gB/Ms (the input of both
+// transposed field operators).  Per stage the demag plan convolves Ms*w, and
+// k_adj_stage gathers the transposed exchange/DMI stencil, the demag and
+// anisotropy terms and the direct torque term, transposes the stage
+// normalisation, forms the previous stage's torque cotangent and, from it,
+// that stage's w.
 #include "adjoint.hpp"
 
 namespace um {
@@ -108,40 +110,28 @@ __device__ __forceinline__ void lin_pair(const FieldArgs<T> &f, int ix, int iy, 
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(ADJ_NT) void k_adj_gb(AdjArgs<T> a) {
-  const long i = blockIdx.x * (long)ADJ_NT + threadIdx.x;
-  double s[3] = {0.0, 0.0, 0.0};
-  if (i < a.N) {
-    const V3<T> u = load3(a.u, i), l = load3(a.lk, i);
-    const V3<T> gB = adj_gb(u, l, a.f.alpha.at(i));
-    st3(a.w, i, vscale(gB, inv_msat(a.f.Ms, i)));
-    s[0] = gB.x; s[1] = gB.y; s[2] = gB.z;
-  }
-  if (!a.gbext) return;
-  // block sum of gB in FP64: wave64 butterfly, then across the 4 waves in LDS;
-  // each block owns its 3 partials, so the sums stay in a fixed order
-  __shared__ double part[ADJ_NT / 64][3];
+// add the block's sum of s[0..2] to its own three partials (fixed order: no
+// atomics, so repeated sweeps give identical sums); all threads must call it
+__device__ __forceinline__ void block_sum3(double *part, const double (&s)[3]) {
+  __shared__ double ws[ADJ_NT / 64][3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     double v = s[c];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][c] = v;
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][c] = v;
   }
   __syncthreads();
   if (threadIdx.x < 3) {
     double t = 0.0;
 #pragma unroll
-    for (int wv = 0; wv < ADJ_NT / 64; wv++) t += part[wv][threadIdx.x];
-    a.gbext[blockIdx.x * 3 + threadIdx.x] += t;
+    for (int wv = 0; wv < ADJ_NT / 64; wv++) t += ws[wv][threadIdx.x];
+    part[blockIdx.x * 3 + threadIdx.x] += t;
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(ADJ_NT) void k_adj_stage(AdjArgs<T> a) {
-  const long i = blockIdx.x * (long)ADJ_NT + threadIdx.x;
-  if (i >= a.N) return;
+__device__ __forceinline__ void adj_stage_cell(const AdjArgs<T> &a, long i, double (&sn)[3]) {
   const FieldArgs<T> &f = a.f;
   const int ix = (int)(i % f.g.Nx);
   const int iy = (int)((i / f.g.Nx) % f.g.Ny);
@@ -195,7 +185,13 @@ __global__ __launch_bounds__(ADJ_NT) void k_adj_stage(AdjArgs<T> a) {
     st3(a.lacc, i, vadd(load3((const T *const *)a.lacc, i), lv));
     if (a.lknext[0]) {
       const V3<T> l5 = load3(a.lv5, i);
-      st3(a.lknext, i, v3(a.c5 * l5.x + a.cs * lv.x, a.c5 * l5.y + a.cs * lv.y, a.c5 * l5.z + a.cs * lv.z));
+      const V3<T> ln = v3(a.c5 * l5.x + a.cs * lv.x, a.c5 * l5.y + a.cs * lv.y, a.c5 * l5.z + a.cs * lv.z);
+      st3(a.lknext, i, ln);
+      if (a.wnext[0]) {
+        const V3<T> gBn = adj_gb(load3(a.unext, i), ln, alpha);
+        st3(a.wnext, i, vscale(gBn, inv));
+        sn[0] = gBn.x; sn[1] = gBn.y; sn[2] = gBn.z;
+      }
     }
   } else {
     st3(a.lacc, i, vadd(load3((const T *const *)a.lacc, i), gm));
@@ -203,16 +199,31 @@ __global__ __launch_bounds__(ADJ_NT) void k_adj_stage(AdjArgs<T> a) {
 }
 
 template <typename T>
+__global__ __launch_bounds__(ADJ_NT) void k_adj_stage(AdjArgs<T> a) {
+  const long i = blockIdx.x * (long)ADJ_NT + threadIdx.x;
+  double sn[3] = {0.0, 0.0, 0.0};
+  if (i < a.N) adj_stage_cell(a, i, sn);
+  if (a.gbext && a.wnext[0]) block_sum3(a.gbext, sn);
+}
+
+template <typename T>
 __global__ __launch_bounds__(ADJ_NT) void k_adj_final(AdjFinalArgs<T> a) {
   const long i = blockIdx.x * (long)ADJ_NT + threadIdx.x;
-  if (i >= a.N) return;
-  const V3<T> m = load3(a.m1, i), l = load3(a.lm, i);
-  const T r = T(1) / a.nrm[i];
-  const T ml = vdot(m, l);
-  const V3<T> lv = v3((l.x - m.x * ml) * r, (l.y - m.y * ml) * r, (l.z - m.z * ml) * r);
-  st3(a.lv5, i, lv);
-  st3(a.lacc, i, lv);
-  st3(a.lk, i, vscale(lv, a.c4));
+  double s[3] = {0.0, 0.0, 0.0};
+  if (i < a.N) {
+    const V3<T> m = load3(a.m1, i), l = load3(a.lm, i);
+    const T r = T(1) / a.nrm[i];
+    const T ml = vdot(m, l);
+    const V3<T> lv = v3((l.x - m.x * ml) * r, (l.y - m.y * ml) * r, (l.z - m.z * ml) * r);
+    st3(a.lv5, i, lv);
+    st3(a.lacc, i, lv);
+    const V3<T> lk = vscale(lv, a.c4);
+    st3(a.lk, i, lk);
+    const V3<T> gB = adj_gb(load3(a.u4, i), lk, a.alpha.at(i));
+    st3(a.w, i, vscale(gB, inv_msat(a.Ms, i)));
+    s[0] = gB.x; s[1] = gB.y; s[2] = gB.z;
+  }
+  if (a.gbext) block_sum3(a.gbext, s);
 }
 
 template <typename T>
@@ -221,11 +232,6 @@ __global__ void k_adj_cast(T *dst, const double *src, long N) {
   if (i < N) dst[i] = (T)src[i];
 }
 
-template <typename T>
-void launch_adj_gb(const AdjArgs<T> &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_gb<T>, dim3((unsigned)adj_blocks(a.N)), dim3(ADJ_NT), 0, s, a);
-  UM_HIP(hipGetLastError());
-}
 template <typename T>
 void launch_adj_stage(const AdjArgs<T> &a, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_stage<T>, dim3((unsigned)adj_blocks(a.N)), dim3(ADJ_NT), 0, s, a);
@@ -243,7 +249,6 @@ void launch_adj_cast(T *dst, const double *src, long N, hipStream_t s) {
 }
 
 #define UM_ADJ_INST(T)                                                        \
-  template void launch_adj_gb<T>(const AdjArgs<T> &, hipStream_t);           \
   template void launch_adj_stage<T>(const AdjArgs<T> &, hipStream_t);        \
   template void launch_adj_final<T>(const AdjFinalArgs<T> &, hipStream_t);   \
   template void launch_adj_cast<T>(T *, const double *, long, hipStream_t);

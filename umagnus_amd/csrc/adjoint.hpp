@@ -23,7 +23,7 @@ struct AdjArgs {
   FieldArgs<T> f;        // parameters and mesh of the forward evaluation (f.m unused)
   const T *u[3];         // stage input m (normalised)
   const T *lk[3];        // cotangent on the stage torque
-  T *w[3];               // kernel A: out gB / Ms; kernel B: in
+  T *w[3];               // gB / Ms of this stage (written by the previous kernel)
   const T *bd[3];        // demag field of the stage (forward tape) or nullptr
   const T *D[3];         // K * gB or nullptr
   const T *nrm;          // |v| of the stage before normalisation (nullptr: stage 1)
@@ -31,6 +31,8 @@ struct AdjArgs {
   T *lacc[3];            // cotangent of the step's m0 (accumulated)
   T *lknext[3];          // cotangent on the previous stage's torque (nullptr: stage 1)
   T c5, cs;              // lknext = c5 lv5 + cs lv
+  const T *unext[3];     // input m of the previous stage: with lknext it gives
+  T *wnext[3];           //   that stage's gB / Ms, written here (nullptr: stage 1)
   T mu0;
   double *gms;           // per-cell dL/dMs accumulator or nullptr
   double *gbext;         // per-block partial sums of gB (3 per block) or nullptr
@@ -38,17 +40,18 @@ struct AdjArgs {
 };
 
 // final step normalisation m1 = v5/|v5|: lv5 = (lm - m1 (m1.lm))/|v5|,
-// lacc = lv5, lk = c4 lv5 (the cotangent on k4)
+// lacc = lv5, lk = c4 lv5 (the cotangent on k4), and stage 4's gB / Ms into w
 template <typename T>
 struct AdjFinalArgs {
-  const T *m1[3], *lm[3];
+  const T *m1[3], *lm[3], *u4[3];
   const T *nrm;
-  T *lv5[3], *lacc[3], *lk[3];
+  T *lv5[3], *lacc[3], *lk[3], *w[3];
   T c4;
+  Par<T> Ms, alpha;
+  double *gbext;         // per-block partial sums of gB or nullptr
   long N;
 };
 
-template <typename T> void launch_adj_gb(const AdjArgs<T> &a, hipStream_t s);
 template <typename T> void launch_adj_stage(const AdjArgs<T> &a, hipStream_t s);
 template <typename T> void launch_adj_final(const AdjFinalArgs<T> &a, hipStream_t s);
 // blocks used by the gB partial sums (3 doubles each)

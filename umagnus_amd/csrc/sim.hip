@@ -814,22 +814,30 @@ struct Sim : SimBase {
   }
   long tape_steps() const override { return tape_n; }
   // one step of the reverse sweep (slot kk): lm -> cotangent of the step's m0
-  void adjoint_step(long kk, T *&lm, T *&lacc, T *lv5, T *&lk, T *&lk2, T *w, T *D, double *gms, double *part) {
+  // stage input m of stage s (1..4) of tape slot kk
+  const T *stage_u(long kk, int s) { return s == 1 ? tM(kk) : tU(kk, s - 2); }
+  void adjoint_step(long kk, T *&lm, T *&lacc, T *lv5, T *&lk, T *&lk2, T *&w, T *&w2, T *D, double *gms,
+                    double *part) {
     const T h = (T)(fixdt * gammaLL);
     AdjFinalArgs<T> fa{};
     for (int c = 0; c < 3; c++) {
       fa.m1[c] = tM(kk + 1) + c * n;
       fa.lm[c] = lm + c * n;
+      fa.u4[c] = stage_u(kk, 4) + c * n;
       fa.lv5[c] = lv5 + c * n;
       fa.lacc[c] = lacc + c * n;
       fa.lk[c] = lk + c * n;
+      fa.w[c] = w + c * n;
     }
     fa.nrm = tN(kk, 3);
     fa.c4 = T(1. / 6.) * h;
+    fa.Ms = P(UM_P_MSAT);
+    fa.alpha = P(UM_P_ALPHA);
+    fa.gbext = part;
     fa.N = n;
     launch_adj_final(fa, st);
     for (int sidx = 4; sidx >= 1; sidx--) {
-      const T *u = sidx == 1 ? tM(kk) : tU(kk, sidx - 2);
+      const T *u = stage_u(kk, sidx);
       AdjArgs<T> a{};
       a.f = field_args(u);
       for (int c = 0; c < 3; c++) {
@@ -841,6 +849,8 @@ struct Sim : SimBase {
         a.lv5[c] = lv5 + c * n;
         a.lacc[c] = lacc + c * n;
         a.lknext[c] = sidx == 1 ? nullptr : lk2 + c * n;
+        a.unext[c] = sidx == 1 ? nullptr : stage_u(kk, sidx - 1) + c * n;
+        a.wnext[c] = sidx == 1 ? nullptr : w2 + c * n;
       }
       a.nrm = sidx == 1 ? nullptr : tN(kk, sidx - 2);
       a.c5 = sidx == 2 ? T(1. / 6.) * h : T(1. / 3.) * h;
@@ -849,7 +859,6 @@ struct Sim : SimBase {
       a.gms = gms;
       a.gbext = part;
       a.N = n;
-      launch_adj_gb(a, st);
       if (enable_demag) {
         // D = K * (Ms w) = K * gB: the forward plan with the mu0 Ms input scaling undone
         void *B[3] = {D, D + n, D + 2 * n};
@@ -859,6 +868,7 @@ struct Sim : SimBase {
       }
       launch_adj_stage(a, st);
       std::swap(lk, lk2);
+      std::swap(w, w2);
     }
     std::swap(lm, lacc);
   }
@@ -868,13 +878,14 @@ struct Sim : SimBase {
     for (int c = 0; c < 3; c++) UM_REQUIRE(gm[c], UM_EINVAL, "backward: NULL cotangent");
     if (enable_demag) ensure_demag();
     const size_t vb = sizeof(T) * 3 * n;
-    DevBuf b_lm(vb), b_lacc(vb), b_lv5(vb), b_lk(vb), b_lk2(vb), b_w(vb), b_D;
+    DevBuf b_lm(vb), b_lacc(vb), b_lv5(vb), b_lk(vb), b_lk2(vb), b_w(vb), b_w2(vb), b_D;
     if (enable_demag) b_D.alloc(vb);
     DevBuf b_gms, b_part;
     const long nb = adj_blocks(n);
     if (gms_out) { b_gms.alloc(sizeof(double) * n); fill_bytes(b_gms.p, b_gms.bytes, st); }
     if (gbext_out) { b_part.alloc(sizeof(double) * 3 * nb); fill_bytes(b_part.p, b_part.bytes, st); }
     T *lm = b_lm.as<T>(), *lacc = b_lacc.as<T>(), *lk = b_lk.as<T>(), *lk2 = b_lk2.as<T>();
+    T *w = b_w.as<T>(), *w2 = b_w2.as<T>();
     for (int c = 0; c < 3; c++)
       UM_HIP(hipMemcpyAsync(lm + c * n, gm[c], sizeof(T) * n, hipMemcpyDeviceToDevice, st));
     const long K = tape_K, nseg = (tape_n + K - 1) / K;
@@ -886,7 +897,7 @@ struct Sim : SimBase {
         seg_start = s0;
       }
       for (long t = s1 - 1; t >= s0; t--)
-        adjoint_step(t - s0, lm, lacc, b_lv5.as<T>(), lk, lk2, b_w.as<T>(), b_D.as<T>(), b_gms.as<double>(),
+        adjoint_step(t - s0, lm, lacc, b_lv5.as<T>(), lk, lk2, w, w2, b_D.as<T>(), b_gms.as<double>(),
                      b_part.as<double>());
     }
     if (gm0)
