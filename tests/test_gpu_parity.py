@@ -86,7 +86,10 @@ CONV_SIZES = [(3, 4, 1), (16, 8, 1), (20, 12, 1), (5, 4, 2), (6, 5, 3), (8, 6, 4
               # four-step Y pass (Py = 64, 256, 1024), with even Nz (two planes per work-group) and odd Nz
               (12, 32, 4), (10, 128, 3), (6, 512, 2), (9, 512, 1),
               # four-step X passes (Nx = 128 .. 1024)
-              (128, 6, 3), (256, 5, 2), (512, 3, 2), (1024, 2, 1)]
+              (128, 6, 3), (256, 5, 2), (512, 3, 2), (1024, 2, 1),
+              # small grids: X rows and 2D conv columns split over more work-groups
+              (128, 32, 1), (256, 256, 1), (512, 64, 1), (1024, 8, 1), (32, 256, 1), (256, 64, 8),
+              (256, 256, 8), (128, 128, 16), (64, 64, 64), (1536, 128, 1), (3072, 32, 1), (3072, 64, 1)]
 
 
 @pytest.mark.parametrize("N", CONV_SIZES)

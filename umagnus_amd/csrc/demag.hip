@@ -501,6 +501,9 @@ struct ConvArgs {
 };
 
 constexpr int CONV_QK = 6;   // tensor prefetch slots per thread: B*(n/2+1) <= FFT_NT*CONV_QK
+// a pass launching fewer work-groups than this (3 per CU on 256 CUs) switches
+// to a variant with fewer lines per work-group when one is compiled in
+constexpr long SMALL_GRID_WG = 768;
 
 template <typename T, int CN, int CB>
 __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
@@ -1067,6 +1070,16 @@ static void launch_xfwd(const XArgs<T> &a, int n, int nl, dim3 grid, hipStream_t
     UM_CT_CASE((k_xfwd<float, 256, 16>), 256, 16, ct)
     UM_CT_CASE((k_xfwd<float, 512, 8>), 512, 8, ct)
     UM_CT_CASE((k_xfwd<float, 1024, 4>), 1024, 4, ct)
+    // few rows (small grids): fewer rows per work-group, more work-groups
+    UM_CT_CASE((k_xfwd<float, 64, 8>), 64, 8, ct)
+    UM_CT_CASE((k_xfwd<float, 64, 1>), 64, 1, ct)
+    UM_CT_CASE((k_xfwd<float, 128, 4>), 128, 4, ct)
+    UM_CT_CASE((k_xfwd<float, 128, 1>), 128, 1, ct)
+    UM_CT_CASE((k_xfwd<float, 256, 4>), 256, 4, ct)
+    UM_CT_CASE((k_xfwd<float, 256, 1>), 256, 1, ct)
+    UM_CT_CASE((k_xfwd<float, 512, 2>), 512, 2, ct)
+    UM_CT_CASE((k_xfwd<float, 512, 1>), 512, 1, ct)
+    UM_CT_CASE((k_xfwd<float, 1024, 1>), 1024, 1, ct)
   } else {
     UM_CT_CASE((k_xfwd<double, 256, 8>), 256, 8, ct)
     UM_CT_CASE((k_xfwd<double, 512, 4>), 512, 4, ct)
@@ -1166,6 +1179,14 @@ static void launch_conv(const ConvArgs<T> &a, int n, int B, dim3 grid, hipStream
       UM_CT_CASE((k_conv<float, 128, 16>), 128, 16, ct)
       UM_CT_CASE((k_conv<float, 256, 8>), 256, 8, ct)
       UM_CT_CASE((k_conv<float, 512, 4>), 512, 4, ct)
+      // narrow spectra (small grids): fewer columns per work-group
+      UM_CT_CASE((k_conv<float, 64, 4>), 64, 4, ct)
+      UM_CT_CASE((k_conv<float, 64, 1>), 64, 1, ct)
+      UM_CT_CASE((k_conv<float, 128, 4>), 128, 4, ct)
+      UM_CT_CASE((k_conv<float, 128, 1>), 128, 1, ct)
+      UM_CT_CASE((k_conv<float, 256, 2>), 256, 2, ct)
+      UM_CT_CASE((k_conv<float, 256, 1>), 256, 1, ct)
+      UM_CT_CASE((k_conv<float, 512, 1>), 512, 1, ct)
     } else {
       UM_CT_CASE((k_conv<double, 256, 4>), 256, 4, ct)
       UM_CT_CASE((k_conv<double, 512, 2>), 512, 2, ct)
@@ -1331,12 +1352,27 @@ struct DemagPlan : DemagBase {
     tz.make(P[2], s);
     nl_x = CtTable<T>::x(P[0] / 2);
     if (!nl_x) nl_x = lines_for(P[0] / 2);
+    // small grids: trade rows per work-group for work-groups while the X pass
+    // would launch fewer than ~3 per CU (the smaller variants are compiled in)
+    if (std::is_same<T, float>::value && CtTable<T>::x(P[0] / 2)) {
+      const int rows = N[1] * N[2];
+      for (int nl : {nl_x / 4, 1}) {
+        if (nl < 1 || 3L * ((rows + nl_x - 1) / nl_x) >= SMALL_GRID_WG) break;
+        nl_x = nl;
+      }
+    }
     nl_y = CtTable<T>::col(P[1]);
     if (!nl_y) nl_y = std::min(lines_for(P[1]), 64);
 
     zreg = !is2d && P[2] == 2 * N[2] && ConvZTable<T>::has(N[2]);
     const int convlen = is2d ? P[1] : P[2];
     B_conv = is2d ? CtTable<T>::conv2(convlen) : CtTable<T>::conv3(convlen);
+    if (is2d && B_conv && std::is_same<T, float>::value) {
+      for (int b : {B_conv / 4, 1}) {   // narrow spectra: more work-groups, as for the X pass
+        if (b < 1 || (H + B_conv - 1) / B_conv >= SMALL_GRID_WG) break;
+        B_conv = b;
+      }
+    }
     if (!B_conv) {
       const int bmax = std::min(lines_for(convlen) / 3, 32);
       UM_REQUIRE(bmax >= 1, UM_EUNSUP, "convolution axis too long");

@@ -228,7 +228,10 @@ static bool field4_ok(const FieldArgs<T> &a) {
 
 template <typename T>
 void launch_field_torque(const FieldArgs<T> &a, hipStream_t s) {
-  if (field4_ok(a)) {
+  // four cells per thread pays on large grids; below ~512 work-groups the
+  // one-cell kernel's four times more work-groups hide the latency better
+  const long ncell = (long)a.g.Nx * a.g.Ny * a.g.Nz;
+  if (ncell >= 512L * 256 * 4 && field4_ok(a)) {
     const long n4 = (long)a.g.Nx / 4 * a.g.Ny * a.g.Nz;
     const dim3 grid((unsigned)((n4 + 255) / 256));
     const bool reg = a.regions != nullptr;

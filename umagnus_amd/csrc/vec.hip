@@ -172,7 +172,9 @@ template <typename T>
 void launch_stage(const StageArgs<T> &a, hipStream_t s) {
   UM_REQUIRE(a.nsrc >= 1 && a.nsrc <= 8, UM_EINVAL, "madd: 1..8 sources");
   UM_REQUIRE(a.ncomp == 1 || a.ncomp == 3, UM_EINVAL, "stage: ncomp must be 1 or 3");
-  if (a.N > 0 && stage_vectorisable(a)) {
+  // 16-byte accesses pay on large arrays; small ones keep one cell per thread
+  // for four times more work-groups
+  if (a.N >= 512L * 256 * Vec16<T>::n && stage_vectorisable(a)) {
     const long groups = a.N / Vec16<T>::n;
     const dim3 grid((unsigned)((groups + 255) / 256));
     const bool g3 = a.group3 && a.nsrc == 3, nrm = a.normalize && a.ncomp == 3;
