@@ -7,6 +7,7 @@ RK45DP step (six effective-field + torque evaluations, FSAL) over the whole
 grid with m resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batched] [--no-adjoint]
+                    [--no-precision]
 
 Prints one JSON line.  `value` is Mcell*steps/s; `roofline` prices one
 effective-field + torque evaluation (the demag FFT passes plus the fused
@@ -15,7 +16,9 @@ timed region) at its algorithmic HBM bytes against the 8 TB/s HBM3E peak.
 `batched` compares a 64-member thermal Heun ensemble stepped together
 (graph replays on per-member streams) with the same members stepped one
 after another; `adjoint` times the gradient of a 2000-step RK4 film run
-(taped forward + reverse sweep) against the plain forward run.
+(taped forward + reverse sweep) against the plain forward run; `precision`
+compares the FP32 path with the all-FP64 one on a 1024x1024x8 Voronoi-grain
+film with the Newell tensor.
 """
 import argparse
 import json
@@ -216,6 +219,67 @@ def adjoint(steps=2000):
             "finite": ok}
 
 
+def precision_workload(N=(1024, 1024, 8), seed=9, alpha=0.1):
+    """Precision workload: 2 nm cubes, 4 regions from a seeded Voronoi map in
+    the film plane (columnar grains), Ms uniform in [6e5, 1e6] A/m and Aex
+    uniform in [1e-11, 2e-11] J/m per region, seeded random m, Newell tensor,
+    RK45DP with MaxErr 1e-5.  With alpha = 0 (the reference default) the
+    random state precesses chaotically and per-cell differences of any two runs
+    grow to O(1) within ~10 ps, so the comparison uses alpha = 0.1."""
+    from umagnus_amd.params import exchange_lut
+    rng = np.random.default_rng(seed)
+    seeds = rng.uniform(0, 1, (4, 2)) * np.array([N[0], N[1]])
+    ms_r = rng.uniform(6e5, 1e6, 4)
+    aex_r = rng.uniform(1e-11, 2e-11, 4)
+    yy, xx = np.meshgrid(np.arange(N[1]) + 0.5, np.arange(N[0]) + 0.5, indexing="ij")
+    d2 = (xx[None] - seeds[:, 0, None, None]) ** 2 + (yy[None] - seeds[:, 1, None, None]) ** 2
+    plane = np.argmin(d2, axis=0).astype(np.uint8)
+    regions = np.broadcast_to(plane, (N[2], N[1], N[0])).copy()
+    m0 = seeded_m(N, seed + 1)
+
+    def make(mod, dtype):
+        s = mod.Sim(N, (2e-9,) * 3, dtype=dtype)
+        s.set_regions(regions)
+        s.set_scalar(mod.P_MSAT, ms_r[regions].astype(np.float64))
+        s.set_scalar(mod.P_ALPHA, alpha)
+        s.set_exchange_luts(exchange_lut({r: float(aex_r[r]) for r in range(4)}, dtype=dtype), None)
+        s.set_demag_newell(1)
+        s.set_solver(mod.SOLVER_RK45DP)
+        s.set_solver_params(dt=1e-15, fixdt=0.0, maxerr=1e-5)
+        s.set_m(m0)
+        return s
+    return make
+
+
+def precision(times=(1e-12, 1e-11, 5e-11)):
+    """FP32 path (FP64 Newell tensor, FP64 error norms) vs the all-FP64 path on
+    the precision workload: per-cell m error and relative total-energy error
+    at each of `times` of adaptive RK45DP, the two runs advanced side by side."""
+    import torch
+    from umagnus_amd import umhip
+    make = precision_workload()
+    s32, s64 = make(umhip, np.float32), make(umhip, np.float64)
+    rows, t_run = [], {32: 0.0, 64: 0.0}
+    for t in times:
+        for key, s in ((32, s32), (64, s64)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s.run(t - s.state()["t"])
+            torch.cuda.synchronize()
+            t_run[key] += time.perf_counter() - t0
+        m32, m64 = s32.get_m().astype(np.float64), s64.get_m()
+        d = np.linalg.norm(m32 - m64, axis=0)
+        E32, E64 = s32.energy(9), s64.energy(9)
+        rows.append({"t": t, "steps_f32": int(s32.state()["nsteps"]), "steps_f64": int(s64.state()["nsteps"]),
+                     "max_m_error": float(d.max()), "rms_m_error": float(np.sqrt((d ** 2).mean())),
+                     "rel_energy_error": float(abs(E32 - E64) / abs(E64))})
+    del s32, s64
+    torch.cuda.empty_cache()
+    return {"workload": "1024x1024x8 cells of 2 nm, 4 Voronoi regions (Ms in [6e5, 1e6], Aex in [1e-11, 2e-11]), "
+                        "Newell tensor, alpha 0.1, RK45DP MaxErr 1e-5, seeded random m",
+            "checkpoints": rows, "run_s_f32": round(t_run[32], 3), "run_s_f64": round(t_run[64], 3)}
+
+
 def cpu_baseline(N, cs):
     """The FP32 oracle (C, OpenMP) on the host: one RK45DP step of the same
     workload after one warm-up step (setup and tensor excluded)."""
@@ -247,6 +311,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -324,6 +389,8 @@ def main():
         out["batched"] = batched()
     if not args.no_adjoint:
         out["adjoint"] = adjoint()
+    if not args.no_precision:
+        out["precision"] = precision()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, cs)
     print(json.dumps(out), flush=True)

@@ -163,3 +163,23 @@ def test_sim_newell_switch(gpu, oracle):
         so.set_demag_newell(newell)
         Bg, Bo = s.demag_field(), so.demag_field()
         assert np.abs(Bg - Bo).max() <= 1e-10 * np.abs(Bo).max(), newell
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_voronoi_grains_newell_vs_oracle(gpu, oracle, prec):
+    """The precision workload's physics at 96 x 80 x 8: 4 Voronoi regions with
+    their own Ms and Aex, Newell tensor, alpha 0.1, random m: B_eff vs the FP64
+    oracle, then 1 ps of adaptive RK45DP (<m> and the total energy)."""
+    import bench
+    _say("voronoi", prec)
+    N = (96, 80, 8)
+    make = bench.precision_workload(N, seed=3)
+    dt = np.float32 if prec == 0 else np.float64
+    sg, so = make(gpu, dt), make(oracle, np.float64)
+    Bg, Bo = sg.beff(), so.beff()
+    assert np.abs(Bg - Bo).max() <= (1e-5 if prec == 0 else 1e-11) * np.abs(Bo).max()
+    for s in (sg, so):
+        s.run(1e-12)
+    assert np.abs(sg.average_m() - so.average_m()).max() <= (1e-4 if prec == 0 else 1e-9)
+    eg, eo = sg.energy(9), so.energy(9)
+    assert abs(eg - eo) <= (1e-4 if prec == 0 else 1e-9) * abs(eo), (eg, eo)
