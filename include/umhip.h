@@ -103,6 +103,23 @@ int um_thermal_field(int prec, void *const B[3], uint64_t seed, uint64_t step,
                      int N);
 
 /* ---- vector arithmetic: opencl/madd.go, normalize.go, vecnorm.go ---------- */
+/* AddDMIBulk, opencl/dmibulk.go:14 (dmibulk.cl) */
+int um_add_dmi_bulk(int prec, void *const B[3], const void *const m[3], const void *Ms, double Ms_mul,
+                    const void *aLUT, const void *dLUT, const uint8_t *regions, double cx, double cy, double cz,
+                    int Nx, int Ny, int Nz, int pbc, int openbc);
+/* AddZhangLiTorque, opencl/zhangli.go:12 (zhangli2.cl): tau += torque */
+int um_add_zhangli(int prec, void *const tau[3], const void *const m[3], const void *Ms, double Ms_mul,
+                   const void *const J[3], const double J_mul[3], const void *alpha, double alpha_mul,
+                   const void *xi, double xi_mul, const void *pol, double pol_mul, double cx, double cy,
+                   double cz, int Nx, int Ny, int Nz, int pbc);
+/* AddSlonczewskiTorque2, opencl/slonczewski.go:12 (slonczewski2.cl): tau += torque;
+ * mesh_thickness = Nz*cz, flp = +1 (fixed layer on top) or -1 */
+int um_add_slonczewski(int prec, void *const tau[3], const void *const m[3], const void *Ms, double Ms_mul,
+                       const void *Jz, double Jz_mul, const void *const fixedP[3], const double fixedP_mul[3],
+                       const void *alpha, double alpha_mul, const void *pol, double pol_mul, const void *lambda,
+                       double lambda_mul, const void *epsPrime, double epsPrime_mul, const void *thickness,
+                       double thickness_mul, double mesh_thickness, double flp, int N);
+
 /* Madd2..Madd7, opencl/madd.go:111-489: dst = sum fac[k]*src[k], nsrc in 1..7 */
 int um_madd(int prec, void *dst, int nsrc, const void *const *src, const double *fac, int N);
 /* Normalize, opencl/normalize.go:13 */
@@ -163,7 +180,12 @@ int um_demag_kernel_newell(int Nx, int Ny, int Nz, const int pbc[3], const doubl
 /* Parameter ids (the .mx3 quantity names) for um_sim_set_scalar / _vector */
 enum {
   UM_P_MSAT = 0, UM_P_KU1, UM_P_KU2, UM_P_KC1, UM_P_KC2, UM_P_KC3, UM_P_ALPHA,
-  UM_P_TEMP, UM_P_ANISU, UM_P_ANISC1, UM_P_ANISC2, UM_P_BEXT, UM_P_COUNT
+  UM_P_TEMP, UM_P_ANISU, UM_P_ANISC1, UM_P_ANISC2, UM_P_BEXT,
+  /* spin-transfer torque, engine/torque.go:11-20: xi, Pol (default 1), Lambda
+   * (default 1), EpsilonPrime, FreeLayerThickness (0: mesh thickness), J
+   * (vector, A/m^2), FixedLayer (vector) */
+  UM_P_XI, UM_P_POL, UM_P_LAMBDA, UM_P_EPSPRIME, UM_P_FREELAYER, UM_P_J, UM_P_FIXEDLAYER,
+  UM_P_COUNT
 };
 /* SetSolver ids, engine/run.go:78-90 */
 enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK23 = 3, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45DP = 5 };
@@ -191,6 +213,13 @@ int um_sim_set_geometry(void *sim, const void *vol);
 /* EnableDemag, OpenBC, DoPrecess, and the MaxVecDiff sqrt(3) widening in Heun */
 int um_sim_set_flags(void *sim, int enable_demag, int openbc, int precess, int heun_sqrt3);
 int um_sim_set_demag_accuracy(void *sim, double accuracy);
+/* Dbulk (engine/exchange.go:49-66): the bulk DMI LUT (same layout as the
+ * exchange LUTs; NULL: none).  Refused at evaluation time together with a
+ * non-zero interfacial Dind, as the reference does. */
+int um_sim_set_dmi_bulk(void *sim, const void *lut);
+/* DisableZhangLiTorque, DisableSlonczewskiTorque and FixedLayerPosition
+ * (+1 FIXEDLAYER_TOP, -1 FIXEDLAYER_BOTTOM), engine/torque.go:26-29, 368-371 */
+int um_sim_set_stt_flags(void *sim, int disable_zhangli, int disable_slonczewski, double fixed_layer_sign);
 /* ---- discrete adjoint of the fixed-step RK4 path ------------------------------
  * No counterpart in the reference (it has no differentiation); these serve
  * callers that need gradients of a loss of the final magnetization: inverse

@@ -17,7 +17,9 @@ _lib = None
 NREGION = 256
 SYMLUT = NREGION * (NREGION + 1) // 2
 
-P_MSAT, P_KU1, P_KU2, P_KC1, P_KC2, P_KC3, P_ALPHA, P_TEMP, P_ANISU, P_ANISC1, P_ANISC2, P_BEXT = range(12)
+(P_MSAT, P_KU1, P_KU2, P_KC1, P_KC2, P_KC3, P_ALPHA, P_TEMP, P_ANISU, P_ANISC1, P_ANISC2, P_BEXT,
+ P_XI, P_POL, P_LAMBDA, P_EPSPRIME, P_FREELAYER, P_J, P_FIXEDLAYER) = range(19)
+VECTOR_PARAMS = (P_ANISU, P_ANISC1, P_ANISC2, P_BEXT, P_J, P_FIXEDLAYER)
 SOLVER_HEUN, SOLVER_RK23, SOLVER_RK4, SOLVER_RK45DP = 2, 3, 4, 5
 
 
@@ -286,6 +288,15 @@ class Sim:
 
     def set_demag_accuracy(self, acc):
         self._f("sim_set_demag_accuracy")(self._h(), C.c_double(acc))
+
+    def set_dmi_bulk(self, lut):
+        """Dbulk LUT (engine/exchange.go:49-66); None: none."""
+        v = None if lut is None else np.ascontiguousarray(lut, self.dtype)
+        self._f("sim_set_dmi_bulk")(self._h(), ptr(v))
+
+    def set_stt_flags(self, disable_zhangli=0, disable_slonczewski=0, fixed_layer_top=True):
+        self._f("sim_set_stt_flags")(self._h(), int(disable_zhangli), int(disable_slonczewski),
+                                     C.c_double(1.0 if fixed_layer_top else -1.0))
 
     def set_demag_newell(self, enable=1, asymptotic_radius=32, zero_self_demag=0):
         self._f("sim_set_demag_newell")(self._h(), int(enable), int(asymptotic_radius), int(zero_self_demag))

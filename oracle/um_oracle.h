@@ -94,6 +94,23 @@ double um_or_adapt_dt(double dt, double corr, double fixdt, double headroom,
   void um_or_regionaddv##SFX(real *dx, real *dy, real *dz, const real *lx,              \
         const real *ly, const real *lz, const uint8_t *regions, int N);                  \
   void um_or_zeromask##SFX(real *dst, const real *maskLUT, const uint8_t *regions, int N);\
+  void um_or_adddmibulk##SFX(real *Hx, real *Hy, real *Hz, const real *mx, const real *my, \
+        const real *mz, const real *Ms_, real Ms_mul, const real *aLUT,                  \
+        const real *dLUT, const uint8_t *regions, real cx, real cy, real cz,             \
+        int Nx, int Ny, int Nz, int pbc, int openbc);                                    \
+  void um_or_addzhangli##SFX(real *tx, real *ty, real *tz, const real *mx,              \
+        const real *my, const real *mz, const real *Ms_, real Ms_mul, const real *jx_,   \
+        real jx_mul, const real *jy_, real jy_mul, const real *jz_, real jz_mul,         \
+        const real *alpha_, real alpha_mul, const real *xi_, real xi_mul,                \
+        const real *pol_, real pol_mul, real cx, real cy, real cz, int Nx, int Ny,       \
+        int Nz, int pbc);                                                                \
+  void um_or_addslonczewski##SFX(real *tx, real *ty, real *tz, const real *mx,          \
+        const real *my, const real *mz, const real *Ms_, real Ms_mul, const real *jz_,   \
+        real jz_mul, const real *px_, real px_mul, const real *py_, real py_mul,         \
+        const real *pz_, real pz_mul, const real *alpha_, real alpha_mul,                \
+        const real *pol_, real pol_mul, const real *lambda_, real lambda_mul,            \
+        const real *eps_, real eps_mul, const real *thick_, real thick_mul,              \
+        real meshThickness, real flp, int N);                                            \
   void um_or_lltorque2##SFX(real *tx, real *ty, real *tz, const real *mx,               \
         const real *my, const real *mz, const real *hx, const real *hy, const real *hz,  \
         const real *alpha_, real alpha_mul, int N);                                      \
@@ -135,7 +152,11 @@ UM_OR_DECL(_f64, double)
 /* Parameter ids used by um_or_sim_set_* (names follow the .mx3 quantities). */
 enum {
   UM_P_MSAT = 0, UM_P_KU1, UM_P_KU2, UM_P_KC1, UM_P_KC2, UM_P_KC3, UM_P_ALPHA,
-  UM_P_TEMP, UM_P_ANISU, UM_P_ANISC1, UM_P_ANISC2, UM_P_BEXT, UM_P_COUNT
+  UM_P_TEMP, UM_P_ANISU, UM_P_ANISC1, UM_P_ANISC2, UM_P_BEXT,
+  /* spin-transfer torque (engine/torque.go:11-20): xi, Pol, Lambda, EpsilonPrime,
+   * FreeLayerThickness, J (vector, A/m^2), FixedLayer (vector) */
+  UM_P_XI, UM_P_POL, UM_P_LAMBDA, UM_P_EPSPRIME, UM_P_FREELAYER, UM_P_J, UM_P_FIXEDLAYER,
+  UM_P_COUNT
 };
 enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK23 = 3, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45DP = 5 };
 
@@ -156,6 +177,10 @@ enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK23 = 3, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45
   void um_or_sim_set_flags##SFX(void *h, int enable_demag, int openbc, int precess,        \
         int heun_sqrt3);                                                                   \
   void um_or_sim_set_demag_accuracy##SFX(void *h, double acc);                             \
+  /* bulk DMI LUT (Dbulk, engine/exchange.go:49-66); NULL: none */                         \
+  void um_or_sim_set_dmi_bulk##SFX(void *h, const real *lut);                               \
+  /* DisableZhangLiTorque, DisableSlonczewskiTorque, FixedLayerPosition sign (+1 top) */   \
+  void um_or_sim_set_stt_flags##SFX(void *h, int disable_zl, int disable_sl, double flp);   \
   /* EnableNewellDemag (engine/demag.go:26-56): 1 = Newell tensor, 0 = brute force */      \
   void um_or_sim_set_demag_newell##SFX(void *h, int enable, int asymptotic_radius,         \
         int zero_self_demag);                                                              \

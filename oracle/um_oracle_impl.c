@@ -287,6 +287,158 @@ void OR(zeromask)(real *dst, const real *maskLUT, const uint8_t *regions, int N)
     if (maskLUT[regions ? regions[i] : 0] != (real)0) dst[i] = 0;
 }
 
+/* ---- exchange + bulk DMI, kernels_src/cl/dmibulk.cl:31-235 -------------- */
+/* One neighbour along `axis` (0 x, 1 y, 2 z) on side sgn (-1 lower, +1 upper):
+ * missing neighbours are extrapolated from the boundary condition unless
+ * OpenBC; the DMI term couples the two components transverse to the axis. */
+static void dmibulk_nb(v3 *h, v3 m0, v3 n, int present, int r0, int rn, int axis, real sgn, real c,
+                       const real *aLUT, const real *dLUT, int openbc) {
+  int r1 = present ? rn : r0;
+  real A = aLUT[symidx(r0, r1)];
+  real D = dLUT[symidx(r0, r1)];
+  real D_2A = D / ((real)2.0 * A);
+  if (!present && openbc) return;
+  if (!present) {
+    real f = sgn * c * D_2A;
+    if (axis == 0) { n.x = m0.x; n.y = m0.y - (f * m0.z); n.z = m0.z + (f * m0.y); }
+    else if (axis == 1) { n.x = m0.x + (f * m0.z); n.y = m0.y; n.z = m0.z - (f * m0.x); }
+    else { n.x = m0.x - (f * m0.y); n.y = m0.y + (f * m0.x); n.z = m0.z; }
+  }
+  real e = ((real)2.0 * A / (c * c));
+  h->x += e * (n.x - m0.x); h->y += e * (n.y - m0.y); h->z += e * (n.z - m0.z);
+  real dc = D / c;
+  if (axis == 0) { h->y += dc * (sgn * n.z); h->z -= dc * (sgn * n.y); }
+  else if (axis == 1) { h->x -= dc * (sgn * n.z); h->z += dc * (sgn * n.x); }
+  else { h->x += dc * (sgn * n.y); h->y -= dc * (sgn * n.x); }
+}
+
+void OR(adddmibulk)(real *Hx, real *Hy, real *Hz, const real *mx, const real *my,
+                    const real *mz, const real *Ms_, real Ms_mul, const real *aLUT,
+                    const real *dLUT, const uint8_t *regions, real cx, real cy, real cz,
+                    int Nx, int Ny, int Nz, int pbc, int openbc) {
+  const int px = pbc & 1, py = pbc & 2, pz = pbc & 4;
+#pragma omp parallel for collapse(2)
+  for (int iz = 0; iz < Nz; iz++)
+    for (int iy = 0; iy < Ny; iy++)
+      for (int ix = 0; ix < Nx; ix++) {
+        long I = IDX(ix, iy, iz);
+        v3 h = mk(0, 0, 0);
+        v3 m0 = mk(mx[I], my[I], mz[I]);
+        int r0 = regions ? regions[I] : 0;
+        if (is0(m0)) continue;
+        const int nax = Nz != 1 ? 3 : 2;
+        for (int axis = 0; axis < nax; axis++)
+          for (int side = 0; side < 2; side++) {
+            const int sg = side ? 1 : -1;
+            int jx = ix, jy = iy, jz = iz, inside, p;
+            int NN = axis == 0 ? Nx : axis == 1 ? Ny : Nz;
+            int *jj = axis == 0 ? &jx : axis == 1 ? &jy : &jz;
+            p = axis == 0 ? px : axis == 1 ? py : pz;
+            inside = (*jj + sg >= 0 && *jj + sg < NN) || p;
+            *jj = sg > 0 ? hclamp(*jj + 1, NN, p) : lclamp(*jj - 1, NN, p);
+            long J = IDX(jx, jy, jz);
+            v3 n = mk(0, 0, 0);
+            if (inside) n = mk(mx[J], my[J], mz[J]);
+            int rn = regions ? regions[J] : 0;
+            real c = axis == 0 ? cx : axis == 1 ? cy : cz;
+            dmibulk_nb(&h, m0, n, !is0(n), r0, rn, axis, (real)sg, c, aLUT, dLUT, openbc);
+          }
+        real invMs = inv_msat(Ms_, Ms_mul, (int)I);
+        Hx[I] += h.x * invMs;
+        Hy[I] += h.y * invMs;
+        Hz[I] += h.z * invMs;
+      }
+}
+
+/* ---- Zhang-Li torque, kernels_src/cl/zhangli2.cl:1-58 -------------------- */
+#define UM_MUB 9.2740091523E-24
+#define UM_QE 1.60217646E-19
+#define UM_GAMMA0 1.7595e11
+#define UM_HBAR 1.05457173E-34
+void OR(addzhangli)(real *tx, real *ty, real *tz, const real *mx, const real *my,
+                    const real *mz, const real *Ms_, real Ms_mul, const real *jx_, real jx_mul,
+                    const real *jy_, real jy_mul, const real *jz_, real jz_mul,
+                    const real *alpha_, real alpha_mul, const real *xi_, real xi_mul,
+                    const real *pol_, real pol_mul, real cx, real cy, real cz, int Nx, int Ny,
+                    int Nz, int pbc) {
+  const int px = pbc & 1, py = pbc & 2, pz = pbc & 4;
+  const real pre = (real)(UM_MUB / (2 * UM_QE * UM_GAMMA0));
+#pragma omp parallel for collapse(2)
+  for (int iz = 0; iz < Nz; iz++)
+    for (int iy = 0; iy < Ny; iy++)
+      for (int ix = 0; ix < Nx; ix++) {
+        long i = IDX(ix, iy, iz);
+        real alpha = amul(alpha_, alpha_mul, (int)i);
+        real xi = amul(xi_, xi_mul, (int)i);
+        real pol = amul(pol_, pol_mul, (int)i);
+        real invMs = inv_msat(Ms_, Ms_mul, (int)i);
+        real b = invMs * pre / ((real)1.0 + xi * xi);
+        v3 J = mk(pol * amul(jx_, jx_mul, (int)i), pol * amul(jy_, jy_mul, (int)i),
+                  pol * amul(jz_, jz_mul, (int)i));
+        v3 hs = mk(0, 0, 0);   /* (u . grad) m, central differences over clamped neighbours */
+        if (J.x != (real)0) {
+          long a = IDX(hclamp(ix + 1, Nx, px), iy, iz), c = IDX(lclamp(ix - 1, Nx, px), iy, iz);
+          real f = (b / cx) * J.x;
+          hs.x += f * (mx[a] - mx[c]); hs.y += f * (my[a] - my[c]); hs.z += f * (mz[a] - mz[c]);
+        }
+        if (J.y != (real)0) {
+          long a = IDX(ix, hclamp(iy + 1, Ny, py), iz), c = IDX(ix, lclamp(iy - 1, Ny, py), iz);
+          real f = (b / cy) * J.y;
+          hs.x += f * (mx[a] - mx[c]); hs.y += f * (my[a] - my[c]); hs.z += f * (mz[a] - mz[c]);
+        }
+        if (J.z != (real)0) {
+          long a = IDX(ix, iy, hclamp(iz + 1, Nz, pz)), c = IDX(ix, iy, lclamp(iz - 1, Nz, pz));
+          real f = (b / cz) * J.z;
+          hs.x += f * (mx[a] - mx[c]); hs.y += f * (my[a] - my[c]); hs.z += f * (mz[a] - mz[c]);
+        }
+        v3 m = mk(mx[i], my[i], mz[i]);
+        v3 mh = cross3(m, hs), mmh = cross3(m, mh);
+        real g = (real)-1.0 / ((real)1.0 + alpha * alpha);
+        real c1 = (real)1.0 + xi * alpha, c2 = xi - alpha;
+        tx[i] += g * (c1 * mmh.x + c2 * mh.x);
+        ty[i] += g * (c1 * mmh.y + c2 * mh.y);
+        tz[i] += g * (c1 * mmh.z + c2 * mh.z);
+      }
+}
+
+/* ---- Slonczewski torque, kernels_src/cl/slonczewski2.cl:4-61 ------------- */
+/* A cell with J == 0 or Ms == 0 gets no torque.  (The reference returns from
+ * its grid-stride loop there, which also skips the thread's later cells; the
+ * per-cell skip is what the loop intends.) */
+void OR(addslonczewski)(real *tx, real *ty, real *tz, const real *mx, const real *my,
+                        const real *mz, const real *Ms_, real Ms_mul, const real *jz_, real jz_mul,
+                        const real *px_, real px_mul, const real *py_, real py_mul,
+                        const real *pz_, real pz_mul, const real *alpha_, real alpha_mul,
+                        const real *pol_, real pol_mul, const real *lambda_, real lambda_mul,
+                        const real *eps_, real eps_mul, const real *thick_, real thick_mul,
+                        real meshThickness, real flp, int N) {
+#pragma omp parallel for
+  for (int i = 0; i < N; i++) {
+    v3 m = mk(mx[i], my[i], mz[i]);
+    real J = amul(jz_, jz_mul, i);
+    v3 p = normalized(mk(amul(px_, px_mul, i), amul(py_, py_mul, i), amul(pz_, pz_mul, i)));
+    real Ms = amul(Ms_, Ms_mul, i);
+    real alpha = amul(alpha_, alpha_mul, i);
+    real pol = amul(pol_, pol_mul, i);
+    real lambda = amul(lambda_, lambda_mul, i);
+    real epsPrime = amul(eps_, eps_mul, i);
+    real thickness = amul(thick_, thick_mul, i);
+    if (thickness == (real)0.0) thickness = meshThickness;
+    thickness *= flp;
+    if (J == (real)0.0 || Ms == (real)0.0) continue;
+    real beta = (real)(UM_HBAR / UM_QE) * (J / (thickness * Ms));
+    real l2 = lambda * lambda;
+    real eps = pol * l2 / ((l2 + (real)1.0) + (l2 - (real)1.0) * dot3(p, m));
+    real A = beta * eps, B = beta * epsPrime;
+    real gilb = (real)1.0 / ((real)1.0 + alpha * alpha);
+    real f1 = gilb * (A + alpha * B), f2 = gilb * (B - alpha * A);
+    v3 pxm = cross3(p, m), mpm = cross3(m, pxm);
+    tx[i] += f1 * mpm.x + f2 * pxm.x;
+    ty[i] += f1 * mpm.y + f2 * pxm.y;
+    tz[i] += f1 * mpm.z + f2 * pxm.z;
+  }
+}
+
 /* ---- LL torque, kernels_src/cl/lltorque2.cl:1-25 ------------------------ */
 void OR(lltorque2)(real *tx, real *ty, real *tz, const real *mx, const real *my,
                    const real *mz, const real *hx, const real *hy, const real *hz,
@@ -842,6 +994,8 @@ typedef struct {
   real *frozenLUT;
   real *vol;
   int enable_demag, openbc, precess, heun_sqrt3, has_dmi;
+  real *dbLUT; int has_dmibulk;      /* bulk DMI (Dbulk) */
+  int disable_zl, disable_sl; double flp;   /* STT switches, engine/torque.go:26-29 */
   double demag_acc;
   int newell, newell_arad, newell_zsd; /* EnableNewellDemag and its two settings */
   void *demag;
@@ -870,6 +1024,10 @@ void *OR(sim_new)(int Nx, int Ny, int Nz, double cx, double cy, double cz, int p
   s->enable_demag = 1; s->precess = 1; s->heun_sqrt3 = 1;
   s->demag_acc = 6.0;
   s->newell_arad = 32;               /* asymptotic_radius, engine/demag.go:27 */
+  s->dbLUT = (real *)calloc(UM_OR_SYMLUT, sizeof(real));
+  s->flp = 1.0;                      /* FIXEDLAYER_TOP, engine/torque.go:368-371 */
+  s->p[UM_P_POL][0].mul = 1;         /* Pol = 1, Lambda = 1 by default (torque.go:73-74) */
+  s->p[UM_P_LAMBDA][0].mul = 1;
   s->aLUT = (real *)calloc(UM_OR_SYMLUT, sizeof(real));
   s->dLUT = (real *)calloc(UM_OR_SYMLUT, sizeof(real));
   s->p[UM_P_ANISU][0].mul = 0; s->p[UM_P_ANISU][1].mul = 0; s->p[UM_P_ANISU][2].mul = 0;
@@ -887,7 +1045,7 @@ void OR(sim_free)(void *h) {
   sim_t *s = (sim_t *)h;
   if (!s) return;
   for (int i = 0; i < UM_P_COUNT; i++) for (int c = 0; c < 3; c++) free(s->p[i][c].arr);
-  free(s->regions); free(s->aLUT); free(s->dLUT); free(s->frozenLUT); free(s->vol);
+  free(s->regions); free(s->aLUT); free(s->dLUT); free(s->dbLUT); free(s->frozenLUT); free(s->vol);
   for (int c = 0; c < 3; c++) free(s->bextLUT[c]);
   if (s->demag) OR(demag_free)(s->demag);
   free(s->m); free(s->k1);
@@ -937,6 +1095,20 @@ void OR(sim_set_exchange_luts)(void *h, const real *aLUT, const real *dLUT) {
   } else {
     memset(s->dLUT, 0, sizeof(real) * UM_OR_SYMLUT);
   }
+}
+void OR(sim_set_dmi_bulk)(void *h, const real *lut) {
+  sim_t *s = (sim_t *)h;
+  s->has_dmibulk = 0;
+  if (lut) {
+    memcpy(s->dbLUT, lut, sizeof(real) * UM_OR_SYMLUT);
+    for (int i = 0; i < UM_OR_SYMLUT; i++) if (lut[i] != 0) s->has_dmibulk = 1;
+  } else {
+    memset(s->dbLUT, 0, sizeof(real) * UM_OR_SYMLUT);
+  }
+}
+void OR(sim_set_stt_flags)(void *h, int disable_zl, int disable_sl, double flp) {
+  sim_t *s = (sim_t *)h;
+  s->disable_zl = disable_zl; s->disable_sl = disable_sl; s->flp = flp;
 }
 void OR(sim_set_bext_lut)(void *h, const real *lx, const real *ly, const real *lz) {
   sim_t *s = (sim_t *)h;
@@ -1037,21 +1209,31 @@ static void add_thermal(sim_t *s, real *B) {
   }
 }
 
-static void set_beff(sim_t *s, real *B) {
+/* AddExchangeField, engine/exchange.go:49-66: exchange, + interfacial or + bulk DMI */
+static void add_exchange_field(sim_t *s, real *B) {
   int n = s->n;
   real *m = s->m;
-  set_demag(s, B);
-  /* AddExchangeField, engine/exchange.go:49-66 */
-  if (!s->has_dmi) {
+  if (s->has_dmi) {
+    OR(adddmi)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), s->aLUT, s->dLUT, s->regions,
+               (real)s->cs[0], (real)s->cs[1], (real)s->cs[2], s->N[0], s->N[1], s->N[2], s->pbccode,
+               s->openbc);
+  } else if (s->has_dmibulk) {
+    OR(adddmibulk)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), s->aLUT, s->dbLUT,
+                   s->regions, (real)s->cs[0], (real)s->cs[1], (real)s->cs[2], s->N[0], s->N[1], s->N[2],
+                   s->pbccode, s->openbc);
+  } else {
     real wx = (real)(2 / (s->cs[0] * s->cs[0])), wy = (real)(2 / (s->cs[1] * s->cs[1])),
          wz = (real)(2 / (s->cs[2] * s->cs[2]));
     OR(addexchange)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), s->aLUT, s->regions,
                     wx, wy, wz, s->N[0], s->N[1], s->N[2], s->pbccode);
-  } else {
-    OR(adddmi)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), s->aLUT, s->dLUT, s->regions,
-               (real)s->cs[0], (real)s->cs[1], (real)s->cs[2], s->N[0], s->N[1], s->N[2], s->pbccode,
-               s->openbc);
   }
+}
+
+static void set_beff(sim_t *s, real *B) {
+  int n = s->n;
+  real *m = s->m;
+  set_demag(s, B);
+  add_exchange_field(s, B);
   /* AddAnisotropyField, engine/anisotropy.go:156-161 */
   if (param_nonzero(s, UM_P_KU1, 1) || param_nonzero(s, UM_P_KU2, 1))
     OR(adduniaxial2)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), PA(UM_P_KU1, 0),
@@ -1079,6 +1261,19 @@ static void torque_fn(sim_t *s, real *dst) {
                   PA(UM_P_ALPHA, 0), n);
   else
     OR(llnoprecess)(dst, dst + n, dst + 2 * n, s->m, s->m + n, s->m + 2 * n, dst, dst + n, dst + 2 * n, n);
+  /* AddSTTorque, engine/torque.go:147-196 (J != 0) */
+  if (param_nonzero(s, UM_P_J, 3)) {
+    if (!s->disable_zl)
+      OR(addzhangli)(dst, dst + n, dst + 2 * n, s->m, s->m + n, s->m + 2 * n, PA(UM_P_MSAT, 0),
+                     PA(UM_P_J, 0), PA(UM_P_J, 1), PA(UM_P_J, 2), PA(UM_P_ALPHA, 0), PA(UM_P_XI, 0),
+                     PA(UM_P_POL, 0), (real)s->cs[0], (real)s->cs[1], (real)s->cs[2], s->N[0], s->N[1],
+                     s->N[2], s->pbccode);
+    if (!s->disable_sl && param_nonzero(s, UM_P_FIXEDLAYER, 3))
+      OR(addslonczewski)(dst, dst + n, dst + 2 * n, s->m, s->m + n, s->m + 2 * n, PA(UM_P_MSAT, 0),
+                         PA(UM_P_J, 2), PA(UM_P_FIXEDLAYER, 0), PA(UM_P_FIXEDLAYER, 1), PA(UM_P_FIXEDLAYER, 2),
+                         PA(UM_P_ALPHA, 0), PA(UM_P_POL, 0), PA(UM_P_LAMBDA, 0), PA(UM_P_EPSPRIME, 0),
+                         PA(UM_P_FREELAYER, 0), (real)(s->N[2] * s->cs[2]), (real)s->flp, n);
+  }
   if (s->frozenLUT)
     for (int c = 0; c < 3; c++) OR(zeromask)(dst + c * n, s->frozenLUT, s->regions, n);
   s->nevals++;
@@ -1474,16 +1669,7 @@ double OR(sim_energy)(void *h, int term) {
   memset(B, 0, sizeof(real) * 3 * n);
   if (term == 0) { set_demag(s, B); }
   else if (term == 1) {
-    if (!s->has_dmi) {
-      real wx = (real)(2 / (s->cs[0] * s->cs[0])), wy = (real)(2 / (s->cs[1] * s->cs[1])),
-           wz = (real)(2 / (s->cs[2] * s->cs[2]));
-      OR(addexchange)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), s->aLUT, s->regions,
-                      wx, wy, wz, s->N[0], s->N[1], s->N[2], s->pbccode);
-    } else {
-      OR(adddmi)(B, B + n, B + 2 * n, m, m + n, m + 2 * n, PA(UM_P_MSAT, 0), s->aLUT, s->dLUT, s->regions,
-                 (real)s->cs[0], (real)s->cs[1], (real)s->cs[2], s->N[0], s->N[1], s->N[2], s->pbccode,
-                 s->openbc);
-    }
+    add_exchange_field(s, B);
   } else if (term == 2) {
     if (s->bextLUT[0])
       OR(regionaddv)(B, B + n, B + 2 * n, s->bextLUT[0], s->bextLUT[1], s->bextLUT[2], s->regions, n);

@@ -42,6 +42,12 @@ template <typename T>
 void add_dmi(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const T *dLUT,
              const uint8_t *regions, T cx, T cy, T cz, Mesh g, int openbc, hipStream_t s);
 template <typename T>
+void add_dmi_bulk(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const T *dLUT,
+                  const uint8_t *regions, T cx, T cy, T cz, Mesh g, int openbc, hipStream_t s);
+template <typename T>
+void add_stt(void *const t[3], const void *const m[3], Par<T> Ms, Par<T> alpha, const SttPar<T> &p, T cx, T cy,
+             T cz, Mesh g, int zl, int sl, hipStream_t s);
+template <typename T>
 void add_uniaxial2(void *const B[3], const void *const m[3], Par<T> Ms, Par<T> K1, Par<T> K2, const Par<T> u[3],
                    int N, hipStream_t s);
 template <typename T>
@@ -227,6 +233,66 @@ int um_add_dmi(int prec, void *const B[3], const void *const m[3], const void *M
                             (float)cx, (float)cy, (float)cz, g, openbc, lib_stream()),
              add_dmi<double>(B, m, par<double>(Ms, Ms_mul), (const double *)aLUT, (const double *)dLUT, regions,
                              cx, cy, cz, g, openbc, lib_stream()));
+  });
+}
+int um_add_dmi_bulk(int prec, void *const B[3], const void *const m[3], const void *Ms, double Ms_mul,
+                    const void *aLUT, const void *dLUT, const uint8_t *regions, double cx, double cy, double cz,
+                    int Nx, int Ny, int Nz, int pbc, int openbc) {
+  return guard([&] {
+    check3(B, "B"); check3(m, "m");
+    UM_REQUIRE(aLUT && dLUT && Nx > 0 && Ny > 0 && Nz > 0, UM_EINVAL, "um_add_dmi_bulk: bad arguments");
+    Mesh g{Nx, Ny, Nz, pbc};
+    DISPATCH(prec,
+             add_dmi_bulk<float>(B, m, par<float>(Ms, Ms_mul), (const float *)aLUT, (const float *)dLUT, regions,
+                                 (float)cx, (float)cy, (float)cz, g, openbc, lib_stream()),
+             add_dmi_bulk<double>(B, m, par<double>(Ms, Ms_mul), (const double *)aLUT, (const double *)dLUT,
+                                  regions, cx, cy, cz, g, openbc, lib_stream()));
+  });
+}
+int um_add_zhangli(int prec, void *const tau[3], const void *const m[3], const void *Ms, double Ms_mul,
+                   const void *const J[3], const double J_mul[3], const void *alpha, double alpha_mul,
+                   const void *xi, double xi_mul, const void *pol, double pol_mul, double cx, double cy,
+                   double cz, int Nx, int Ny, int Nz, int pbc) {
+  return guard([&] {
+    check3(tau, "tau"); check3(m, "m");
+    UM_REQUIRE(J && J_mul && Nx > 0 && Ny > 0 && Nz > 0, UM_EINVAL, "um_add_zhangli: bad arguments");
+    Mesh g{Nx, Ny, Nz, pbc};
+    auto go = [&](auto zero) {
+      using T = decltype(zero);
+      SttPar<T> p{};
+      for (int c = 0; c < 3; c++) p.J[c] = par<T>(J[c], J_mul[c]);
+      p.xi = par<T>(xi, xi_mul);
+      p.pol = par<T>(pol, pol_mul);
+      add_stt<T>(tau, m, par<T>(Ms, Ms_mul), par<T>(alpha, alpha_mul), p, (T)cx, (T)cy, (T)cz, g, 1, 0,
+                 lib_stream());
+    };
+    DISPATCH(prec, go(0.0f), go(0.0));
+  });
+}
+int um_add_slonczewski(int prec, void *const tau[3], const void *const m[3], const void *Ms, double Ms_mul,
+                       const void *Jz, double Jz_mul, const void *const fixedP[3], const double fixedP_mul[3],
+                       const void *alpha, double alpha_mul, const void *pol, double pol_mul, const void *lambda,
+                       double lambda_mul, const void *epsPrime, double epsPrime_mul, const void *thickness,
+                       double thickness_mul, double mesh_thickness, double flp, int N) {
+  return guard([&] {
+    check3(tau, "tau"); check3(m, "m");
+    UM_REQUIRE(fixedP && fixedP_mul && N > 0, UM_EINVAL, "um_add_slonczewski: bad arguments");
+    Mesh g{N, 1, 1, 0};
+    auto go = [&](auto zero) {
+      using T = decltype(zero);
+      SttPar<T> p{};
+      p.J[2] = par<T>(Jz, Jz_mul);
+      for (int c = 0; c < 3; c++) p.fixp[c] = par<T>(fixedP[c], fixedP_mul[c]);
+      p.pol = par<T>(pol, pol_mul);
+      p.lambda = par<T>(lambda, lambda_mul);
+      p.epsp = par<T>(epsPrime, epsPrime_mul);
+      p.thick = par<T>(thickness, thickness_mul);
+      p.mesh_thick = (T)mesh_thickness;
+      p.flp = (T)flp;
+      add_stt<T>(tau, m, par<T>(Ms, Ms_mul), par<T>(alpha, alpha_mul), p, T(1), T(1), T(1), g, 0, 1,
+                 lib_stream());
+    };
+    DISPATCH(prec, go(0.0f), go(0.0));
   });
 }
 int um_add_uniaxial2(int prec, void *const B[3], const void *const m[3], const void *Ms, double Ms_mul,
@@ -598,6 +664,16 @@ int um_sim_backward(void *sim, const void *const dm[3], void *const dm0[3], void
     SIMCHK(sim);
     UM_REQUIRE(dm, UM_EINVAL, "NULL cotangent");
     SIM(sim)->backward(dm, dm0, dms, dbext);
+  });
+}
+int um_sim_set_dmi_bulk(void *sim, const void *lut) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->set_dmi_bulk(lut); });
+}
+int um_sim_set_stt_flags(void *sim, int disable_zhangli, int disable_slonczewski, double fixed_layer_sign) {
+  return guard([&] {
+    SIMCHK(sim);
+    UM_REQUIRE(fixed_layer_sign == 1.0 || fixed_layer_sign == -1.0, UM_EINVAL, "fixed layer sign must be +1 or -1");
+    SIM(sim)->set_stt_flags(disable_zhangli, disable_slonczewski, fixed_layer_sign);
   });
 }
 int um_sim_set_demag_newell(void *sim, int enable, int asymptotic_radius, int zero_self_demag) {

@@ -1277,6 +1277,7 @@ __global__ __launch_bounds__(FFT_NT) void k_xinv_field(XiArgs<T> a, FieldArgs<T>
 template <typename T>
 static bool field4_aligned(const FieldArgs<T> &a) {
   if (a.g.Nx % 4) return false;
+  if (a.exch_mode == 3 || a.has_zl || a.has_sl) return false;   // terms of the one-cell kernel only
   auto al = [](const void *p) { return ((uintptr_t)p & (4 * sizeof(T) - 1)) == 0; };
   for (int c = 0; c < 3; c++) {
     if (!al(a.m[c])) return false;

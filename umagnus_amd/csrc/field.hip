@@ -44,6 +44,47 @@ __global__ void k_adddmi(T *Bx, T *By, T *Bz, const T *mx, const T *my, const T 
 }
 
 template <typename T>
+__global__ void k_adddmibulk(T *Bx, T *By, T *Bz, const T *mx, const T *my, const T *mz, Par<T> Ms,
+                             const T *aLUT, const T *dLUT, const uint8_t *regions, T cx, T cy, T cz, Mesh g,
+                             int openbc) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  const int iz = blockIdx.z;
+  if (ix >= g.Nx || iy >= g.Ny) return;
+  const T *m[3] = {mx, my, mz};
+  const long I = g.idx(ix, iy, iz);
+  const V3<T> m0 = load3(m, I);
+  if (is0(m0)) return;
+  const V3<T> h = dmibulk_cell(g, ix, iy, iz, I, m0, m, regions, aLUT, dLUT, cx, cy, cz, openbc);
+  const T invMs = inv_msat(Ms, I);
+  Bx[I] += h.x * invMs;
+  By[I] += h.y * invMs;
+  Bz[I] += h.z * invMs;
+}
+
+// tau += Zhang-Li torque (zhangli2.cl) and/or Slonczewski torque (slonczewski2.cl)
+template <typename T>
+__global__ void k_addstt(T *tx, T *ty, T *tz, const T *mx, const T *my, const T *mz, Par<T> Ms, Par<T> alpha,
+                         SttPar<T> p, T cx, T cy, T cz, Mesh g, int zl, int sl) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  const int iz = blockIdx.z;
+  if (ix >= g.Nx || iy >= g.Ny) return;
+  const T *m[3] = {mx, my, mz};
+  const long I = g.idx(ix, iy, iz);
+  const V3<T> m0 = load3(m, I);
+  V3<T> t = v3(T(0), T(0), T(0));
+  if (zl) t = zhangli_cell(g, ix, iy, iz, I, m0, m, Ms, alpha, p, cx, cy, cz);
+  if (sl) {
+    const V3<T> z = slonczewski_cell(I, m0, Ms, alpha, p);
+    t.x += z.x; t.y += z.y; t.z += z.z;
+  }
+  tx[I] += t.x;
+  ty[I] += t.y;
+  tz[I] += t.z;
+}
+
+template <typename T>
 __global__ void k_adduniaxial2(T *Bx, T *By, T *Bz, const T *mx, const T *my, const T *mz, Par<T> Ms,
                                Par<T> K1, Par<T> K2, Par<T> ux, Par<T> uy, Par<T> uz, int N) {
   const Par<T> u[3] = {ux, uy, uz};
@@ -138,9 +179,11 @@ __global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
   const V3<T> m0 = load3(a.m, I);
   V3<T> B = a.bd[0] ? load3(a.bd, I) : v3(T(0), T(0), T(0));
   if (a.exch_mode != 0 && !is0(m0)) {
-    const V3<T> h = (a.exch_mode == 1)
-                        ? exchange_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.wx, a.wy, a.wz)
-                        : dmi_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.dLUT, a.cx, a.cy, a.cz, a.openbc);
+    const V3<T> h =
+        (a.exch_mode == 1)   ? exchange_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.wx, a.wy, a.wz)
+        : (a.exch_mode == 2) ? dmi_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.dLUT, a.cx, a.cy, a.cz, a.openbc)
+                             : dmibulk_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.dLUT, a.cx, a.cy, a.cz,
+                                            a.openbc);
     const T invMs = inv_msat(a.Ms, I);
     B.x += h.x * invMs;
     B.y += h.y * invMs;
@@ -183,6 +226,15 @@ __global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
       const V3<T> c = vcross(m0, vcross(m0, B));
       t = v3(-c.x, -c.y, -c.z);
     }
+    // AddSTTorque (engine/torque.go:147-196), before FreezeSpins
+    if (a.has_zl) {
+      const V3<T> z = zhangli_cell(g, ix, iy, iz, I, m0, a.m, a.Ms, a.alpha, a.stt, a.cx, a.cy, a.cz);
+      t.x += z.x; t.y += z.y; t.z += z.z;
+    }
+    if (a.has_sl) {
+      const V3<T> z = slonczewski_cell(I, m0, a.Ms, a.alpha, a.stt);
+      t.x += z.x; t.y += z.y; t.z += z.z;
+    }
     if (a.frozenLUT && a.frozenLUT[a.regions ? a.regions[I] : 0] != T(0)) t = v3(T(0), T(0), T(0));
     a.tau[0][I] = t.x;
     a.tau[1][I] = t.y;
@@ -216,6 +268,7 @@ __global__ __launch_bounds__(256) void k_field_torque4(FieldArgs<T> a) {
 template <typename T>
 static bool field4_ok(const FieldArgs<T> &a) {
   if (sizeof(T) != 4 || a.g.Nx % 4) return false;
+  if (a.exch_mode == 3 || a.has_zl || a.has_sl) return false;   // bulk DMI and STT: one-cell kernel
   auto al = [](const void *p) { return ((uintptr_t)p & (4 * sizeof(T) - 1)) == 0; };
   for (int c = 0; c < 3; c++) {
     if (!al(a.m[c])) return false;
@@ -268,6 +321,21 @@ void add_dmi(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT,
              const uint8_t *regions, T cx, T cy, T cz, Mesh g, int openbc, hipStream_t s) {
   hipLaunchKernelGGL(k_adddmi<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2], (const T *)m[0],
                      (const T *)m[1], (const T *)m[2], Ms, aLUT, dLUT, regions, cx, cy, cz, g, openbc);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void add_dmi_bulk(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const T *dLUT,
+                  const uint8_t *regions, T cx, T cy, T cz, Mesh g, int openbc, hipStream_t s) {
+  hipLaunchKernelGGL(k_adddmibulk<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2],
+                     (const T *)m[0], (const T *)m[1], (const T *)m[2], Ms, aLUT, dLUT, regions, cx, cy, cz, g,
+                     openbc);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void add_stt(void *const t[3], const void *const m[3], Par<T> Ms, Par<T> alpha, const SttPar<T> &p, T cx, T cy,
+             T cz, Mesh g, int zl, int sl, hipStream_t s) {
+  hipLaunchKernelGGL(k_addstt<T>, grid3(g), dim3(64, 4), 0, s, (T *)t[0], (T *)t[1], (T *)t[2], (const T *)m[0],
+                     (const T *)m[1], (const T *)m[2], Ms, alpha, p, cx, cy, cz, g, zl, sl);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
@@ -331,6 +399,10 @@ void thermal_field(void *const B[3], uint64_t seed, uint64_t step, uint32_t stre
                                 T, Mesh, hipStream_t);                                                        \
   template void add_dmi<T>(void *const[3], const void *const[3], Par<T>, const T *, const T *, const uint8_t *, \
                            T, T, T, Mesh, int, hipStream_t);                                                  \
+  template void add_dmi_bulk<T>(void *const[3], const void *const[3], Par<T>, const T *, const T *,           \
+                                const uint8_t *, T, T, T, Mesh, int, hipStream_t);                             \
+  template void add_stt<T>(void *const[3], const void *const[3], Par<T>, Par<T>, const SttPar<T> &, T, T, T,   \
+                           Mesh, int, int, hipStream_t);                                                      \
   template void add_uniaxial2<T>(void *const[3], const void *const[3], Par<T>, Par<T>, Par<T>, const Par<T>[3], \
                                  int, hipStream_t);                                                           \
   template void add_cubic2<T>(void *const[3], const void *const[3], Par<T>, Par<T>, Par<T>, Par<T>,            \

@@ -133,6 +133,110 @@ __device__ __forceinline__ V3<T> dmi_cell(const Mesh &g, int ix, int iy, int iz,
   return h;
 }
 
+// ---- exchange + bulk DMI, kernels_src/cl/dmibulk.cl:31-235 -------------------
+// A neighbour outside the mesh (or with m = 0) is extrapolated from the bulk
+// DMI boundary condition unless OpenBC; z neighbours only in 3D.
+template <typename T>
+__device__ __forceinline__ V3<T> dmibulk_cell(const Mesh &g, int ix, int iy, int iz, long I, V3<T> m0,
+                                              const T *const *m, const uint8_t *regions, const T *aLUT,
+                                              const T *dLUT, T cx, T cy, T cz, int openbc) {
+  const int r0 = regions ? regions[I] : 0;
+  V3<T> h = v3(T(0), T(0), T(0));
+  auto nb = [&](long i_, bool inside, int axis, T sgn, T c) {
+    V3<T> n = v3(T(0), T(0), T(0));
+    if (inside) n = load3(m, i_);
+    const bool present = !is0(n);
+    const int rn = present ? (regions ? regions[i_] : 0) : r0;
+    const T A = aLUT[symidx(r0, rn)];
+    const T D = dLUT[symidx(r0, rn)];
+    const T D_2A = D / (T(2.0) * A);
+    if (!present && openbc) return;
+    if (!present) {
+      const T f = sgn * c * D_2A;
+      if (axis == 0) { n.x = m0.x; n.y = m0.y - (f * m0.z); n.z = m0.z + (f * m0.y); }
+      else if (axis == 1) { n.x = m0.x + (f * m0.z); n.y = m0.y; n.z = m0.z - (f * m0.x); }
+      else { n.x = m0.x - (f * m0.y); n.y = m0.y + (f * m0.x); n.z = m0.z; }
+    }
+    const T e = (T(2.0) * A / (c * c));
+    h.x += e * (n.x - m0.x);
+    h.y += e * (n.y - m0.y);
+    h.z += e * (n.z - m0.z);
+    const T dc = D / c;
+    if (axis == 0) { h.y += dc * (sgn * n.z); h.z -= dc * (sgn * n.y); }
+    else if (axis == 1) { h.x -= dc * (sgn * n.z); h.z += dc * (sgn * n.x); }
+    else { h.x += dc * (sgn * n.y); h.y -= dc * (sgn * n.x); }
+  };
+  nb(g.idx(g.lx(ix - 1), iy, iz), (ix - 1 >= 0) || (g.pbc & 1), 0, T(-1), cx);
+  nb(g.idx(g.hx(ix + 1), iy, iz), (ix + 1 < g.Nx) || (g.pbc & 1), 0, T(1), cx);
+  nb(g.idx(ix, g.ly(iy - 1), iz), (iy - 1 >= 0) || (g.pbc & 2), 1, T(-1), cy);
+  nb(g.idx(ix, g.hy(iy + 1), iz), (iy + 1 < g.Ny) || (g.pbc & 2), 1, T(1), cy);
+  if (g.Nz != 1) {
+    nb(g.idx(ix, iy, g.lz(iz - 1)), (iz - 1 >= 0) || (g.pbc & 4), 2, T(-1), cz);
+    nb(g.idx(ix, iy, g.hz(iz + 1)), (iz + 1 < g.Nz) || (g.pbc & 4), 2, T(1), cz);
+  }
+  return h;
+}
+
+// ---- spin-transfer torques -----------------------------------------------------
+constexpr double UM_MUB = 9.2740091523E-24;   // Bohr magneton, J/T (constants.clh:7)
+constexpr double UM_QE = 1.60217646E-19;      // electron charge, C (:6)
+constexpr double UM_GAMMA0 = 1.7595e11;       // rad/Ts (:8)
+constexpr double UM_HBAR = 1.05457173E-34;    // J s (:9)
+
+template <typename T>
+struct SttPar {
+  Par<T> J[3], xi, pol, lambda, epsp, thick, fixp[3];
+  T mesh_thick, flp;   // mesh thickness Nz*cz; +1 fixed layer on top, -1 below
+};
+
+// Zhang-Li torque (zhangli2.cl:17-57): (u.grad) m from central differences over
+// clamped (or periodic) neighbours, in T
+template <typename T>
+__device__ __forceinline__ V3<T> zhangli_cell(const Mesh &g, int ix, int iy, int iz, long i, V3<T> m0,
+                                              const T *const *m, const Par<T> &Ms, const Par<T> &alpha_,
+                                              const SttPar<T> &p, T cx, T cy, T cz) {
+  const T alpha = alpha_.at(i), xi = p.xi.at(i), pol = p.pol.at(i);
+  const T invMs = inv_msat(Ms, i);
+  const T b = invMs * T(UM_MUB / (2 * UM_QE * UM_GAMMA0)) / (T(1.0) + xi * xi);
+  const V3<T> J = v3(pol * p.J[0].at(i), pol * p.J[1].at(i), pol * p.J[2].at(i));
+  V3<T> hs = v3(T(0), T(0), T(0));
+  auto add = [&](long a_, long c_, T f) {
+    const V3<T> ma = load3(m, a_), mc = load3(m, c_);
+    hs.x += f * (ma.x - mc.x);
+    hs.y += f * (ma.y - mc.y);
+    hs.z += f * (ma.z - mc.z);
+  };
+  if (J.x != T(0)) add(g.idx(g.hx(ix + 1), iy, iz), g.idx(g.lx(ix - 1), iy, iz), (b / cx) * J.x);
+  if (J.y != T(0)) add(g.idx(ix, g.hy(iy + 1), iz), g.idx(ix, g.ly(iy - 1), iz), (b / cy) * J.y);
+  if (J.z != T(0)) add(g.idx(ix, iy, g.hz(iz + 1)), g.idx(ix, iy, g.lz(iz - 1)), (b / cz) * J.z);
+  const V3<T> mh = vcross(m0, hs), mmh = vcross(m0, mh);
+  const T gg = T(-1.0) / (T(1.0) + alpha * alpha);
+  const T c1 = T(1.0) + xi * alpha, c2 = xi - alpha;
+  return v3(gg * (c1 * mmh.x + c2 * mh.x), gg * (c1 * mmh.y + c2 * mh.y), gg * (c1 * mmh.z + c2 * mh.z));
+}
+
+// Slonczewski torque (slonczewski2.cl:22-60); no torque where J or Ms is zero
+template <typename T>
+__device__ __forceinline__ V3<T> slonczewski_cell(long i, V3<T> m, const Par<T> &Ms_, const Par<T> &alpha_,
+                                                  const SttPar<T> &p) {
+  const T J = p.J[2].at(i);
+  const V3<T> pp = vnormalized(v3(p.fixp[0].at(i), p.fixp[1].at(i), p.fixp[2].at(i)));
+  const T Ms = Ms_.at(i);
+  const T alpha = alpha_.at(i), pol = p.pol.at(i), lambda = p.lambda.at(i), epsp = p.epsp.at(i);
+  T thickness = p.thick.at(i);
+  if (thickness == T(0.0)) thickness = p.mesh_thick;
+  thickness *= p.flp;
+  if (J == T(0.0) || Ms == T(0.0)) return v3(T(0), T(0), T(0));
+  const T beta = T(UM_HBAR / UM_QE) * (J / (thickness * Ms));
+  const T l2 = lambda * lambda;
+  const T eps = pol * l2 / ((l2 + T(1.0)) + (l2 - T(1.0)) * vdot(pp, m));
+  const T A = beta * eps, B = beta * epsp;
+  const T gilb = T(1.0) / (T(1.0) + alpha * alpha);
+  const T f1 = gilb * (A + alpha * B), f2 = gilb * (B - alpha * A);
+  const V3<T> pxm = vcross(pp, m), mpm = vcross(m, pxm);
+  return v3(f1 * mpm.x + f2 * pxm.x, f1 * mpm.y + f2 * pxm.y, f1 * mpm.z + f2 * pxm.z);
+}
+
 // ---- uniaxial anisotropy, kernels_src/cl/uniaxialanisotropy2.cl:17-35 -------
 template <typename T>
 __device__ __forceinline__ V3<T> uniaxial_cell(long i, V3<T> m, const Par<T> &Ms, const Par<T> &K1_,
@@ -233,7 +337,7 @@ struct FieldArgs {
   const uint8_t *regions;
   Par<T> Ms;
   const T *aLUT, *dLUT;
-  int exch_mode;         // 0 none, 1 exchange, 2 exchange + interfacial DMI
+  int exch_mode;         // 0 none, 1 exchange, 2 exchange + interfacial DMI, 3 exchange + bulk DMI (dLUT = Dbulk)
   int openbc;
   int has_uniax;
   Par<T> Ku1, Ku2, u[3];
@@ -251,6 +355,8 @@ struct FieldArgs {
   Par<T> alpha;
   int precess;
   const T *frozenLUT;    // FrozenSpins per region or nullptr
+  int has_zl, has_sl;    // Zhang-Li / Slonczewski torque on (one-cell kernel only)
+  SttPar<T> stt;
 };
 
 // ---- four consecutive x cells per thread ----------------------------------------

@@ -32,8 +32,11 @@ struct Sim : SimBase {
     bool nonzero = false;   // host-side knowledge: any non-zero value
   };
   PSlot p[UM_P_COUNT][3];
-  DevBuf regions, aLUT, dLUT, bextLUT, frozenLUT, vol;
-  bool has_regions = false, has_dmi = false, has_bextlut = false, has_frozen = false, has_vol = false;
+  DevBuf regions, aLUT, dLUT, dbLUT, bextLUT, frozenLUT, vol;
+  bool has_regions = false, has_dmi = false, has_dmibulk = false, has_bextlut = false, has_frozen = false,
+       has_vol = false;
+  int disable_zl = 0, disable_sl = 0;   // DisableZhangLiTorque / DisableSlonczewskiTorque (engine/torque.go:26-27)
+  double flp = 1.0;                     // FIXEDLAYER_TOP (engine/torque.go:368-371)
   int enable_demag = 1, openbc = 0, precess = 1, heun_sqrt3 = 1;
   int newell = 0, newell_arad = 32, newell_zsd = 0;   // asymptotic_radius default, engine/demag.go:27
   double demag_acc = 6.0;
@@ -103,8 +106,12 @@ struct Sim : SimBase {
     UM_HIP(hipHostMalloc((void **)&hsum, sizeof(double), hipHostMallocDefault));
     aLUT.alloc(sizeof(T) * SYMLUT);
     dLUT.alloc(sizeof(T) * SYMLUT);
+    dbLUT.alloc(sizeof(T) * SYMLUT);
     fill_bytes(aLUT.p, aLUT.bytes, st);
     fill_bytes(dLUT.p, dLUT.bytes, st);
+    fill_bytes(dbLUT.p, dbLUT.bytes, st);
+    // Pol = 1 and Lambda = 1 by default (engine/torque.go:73-74)
+    for (int id : {UM_P_POL, UM_P_LAMBDA}) { p[id][0].mul = 1; p[id][0].nonzero = true; }
     for (auto &e : step_ev) UM_HIP(hipEventCreate(&e));
     gstat.alloc(4 * sizeof(double));
     UM_HIP(hipHostMalloc((void **)&hstat, 4 * sizeof(double), hipHostMallocDefault));
@@ -172,6 +179,18 @@ struct Sim : SimBase {
       upload(dLUT, d, sizeof(T) * SYMLUT);
       has_dmi = any_nonzero((const T *)d, SYMLUT);
     }
+  }
+  void set_dmi_bulk(const void *l) override {
+    gen++;
+    has_dmibulk = false;
+    if (l) {
+      upload(dbLUT, l, sizeof(T) * SYMLUT);
+      has_dmibulk = any_nonzero((const T *)l, SYMLUT);
+    }
+  }
+  void set_stt_flags(int zl, int sl, double f) override {
+    gen++;
+    disable_zl = zl; disable_sl = sl; flp = f;
   }
   void set_bext_lut(const void *const l[3]) override {
     gen++;
@@ -270,8 +289,10 @@ struct Sim : SimBase {
     a.regions = has_regions ? (const uint8_t *)regions.p : nullptr;
     a.Ms = P(UM_P_MSAT);
     a.aLUT = aLUT.as<T>();
-    a.dLUT = dLUT.as<T>();
-    a.exch_mode = has_dmi ? 2 : 1;
+    // AddExchangeField, engine/exchange.go:49-66
+    UM_REQUIRE(!(has_dmi && has_dmibulk), UM_EINVAL, "Cannot have induced and interfacial DMI at the same time");
+    a.dLUT = has_dmibulk ? dbLUT.as<T>() : dLUT.as<T>();
+    a.exch_mode = has_dmi ? 2 : has_dmibulk ? 3 : 1;
     a.openbc = openbc;
     a.has_uniax = nz(UM_P_KU1) || nz(UM_P_KU2);
     a.Ku1 = P(UM_P_KU1); a.Ku2 = P(UM_P_KU2);
@@ -295,6 +316,22 @@ struct Sim : SimBase {
     a.alpha = P(UM_P_ALPHA);
     a.precess = precess;
     a.frozenLUT = has_frozen ? frozenLUT.as<T>() : nullptr;
+    // AddSTTorque, engine/torque.go:147-196
+    const bool has_j = nz(UM_P_J, 3);
+    UM_REQUIRE(!has_j || nz(UM_P_POL), UM_EINVAL, "spin polarization should not be 0");
+    a.has_zl = has_j && !disable_zl;
+    a.has_sl = has_j && !disable_sl && nz(UM_P_FIXEDLAYER, 3);
+    for (int c = 0; c < 3; c++) {
+      a.stt.J[c] = P(UM_P_J, c);
+      a.stt.fixp[c] = P(UM_P_FIXEDLAYER, c);
+    }
+    a.stt.xi = P(UM_P_XI);
+    a.stt.pol = P(UM_P_POL);
+    a.stt.lambda = P(UM_P_LAMBDA);
+    a.stt.epsp = P(UM_P_EPSPRIME);
+    a.stt.thick = P(UM_P_FREELAYER);
+    a.stt.mesh_thick = (T)(N[2] * cs[2]);
+    a.stt.flp = (T)flp;
     return a;
   }
   // B_demag into bd (SetDemagField, engine/demag.go:42-65)
@@ -792,6 +829,7 @@ struct Sim : SimBase {
     UM_REQUIRE(!(nz(UM_P_KC1) || nz(UM_P_KC2) || nz(UM_P_KC3)), UM_EUNSUP, "no adjoint of cubic anisotropy");
     UM_REQUIRE(precess && !has_frozen && !has_vol && !relaxing, UM_EUNSUP,
                "the adjoint needs precession on, no frozen spins and no geometry");
+    UM_REQUIRE(!has_dmibulk && !nz(UM_P_J, 3), UM_EUNSUP, "no adjoint of bulk DMI or spin-transfer torques");
     tape_end();
     tape_K = every > 0 ? std::min<long>(every, cap) : cap;
     const long nck = (cap + tape_K - 1) / tape_K;

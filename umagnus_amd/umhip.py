@@ -21,7 +21,8 @@ LIB_PATH = os.path.join(_HERE, "libumhip.so")
 _lib = None
 
 F32, F64 = 0, 1
-P_MSAT, P_KU1, P_KU2, P_KC1, P_KC2, P_KC3, P_ALPHA, P_TEMP, P_ANISU, P_ANISC1, P_ANISC2, P_BEXT = range(12)
+(P_MSAT, P_KU1, P_KU2, P_KC1, P_KC2, P_KC3, P_ALPHA, P_TEMP, P_ANISU, P_ANISC1, P_ANISC2, P_BEXT,
+ P_XI, P_POL, P_LAMBDA, P_EPSPRIME, P_FREELAYER, P_J, P_FIXEDLAYER) = range(19)
 SOLVER_HEUN, SOLVER_RK23, SOLVER_RK4, SOLVER_RK45DP = 2, 3, 4, 5
 NREGION = 256
 SYMLUT = NREGION * (NREGION + 1) // 2
@@ -37,6 +38,7 @@ EXPORTS = [
     "um_demag_destroy", "um_demag_info", "um_demag_kspec", "um_demag_kernel_bf",
     "um_demag_create_newell", "um_demag_kernel_newell", "um_sim_set_demag_newell",
     "um_sim_tape_begin", "um_sim_tape_end", "um_sim_tape_steps", "um_sim_backward",
+    "um_add_dmi_bulk", "um_add_zhangli", "um_add_slonczewski", "um_sim_set_dmi_bulk", "um_sim_set_stt_flags",
     "um_sim_create", "um_sim_destroy", "um_sim_set_scalar", "um_sim_set_vector", "um_sim_set_regions",
     "um_sim_set_exchange_luts", "um_sim_set_bext_lut", "um_sim_set_frozen_lut", "um_sim_set_geometry",
     "um_sim_set_flags", "um_sim_set_demag_accuracy", "um_sim_set_thermal_seed", "um_sim_set_m",
@@ -156,6 +158,36 @@ def AddDMI(B, m, aLUT, dLUT, Msat, regions, cellsize, pbc=0, OpenBC=False):
     ms, msm = _par(Msat)
     _call("um_add_dmi", _prec(m[0]), _vec(B), _vec(m), ms, msm, _dp(aLUT), _dp(dLUT), _dp(regions),
           *[C.c_double(c) for c in cellsize], Nx, Ny, Nz, pbc, int(OpenBC))
+
+
+def AddDMIBulk(B, m, aLUT, dLUT, Msat, regions, cellsize, pbc=0, OpenBC=False):
+    """opencl.AddDMIBulk (opencl/dmibulk.go:14)."""
+    init()
+    Nz, Ny, Nx = m[0].shape
+    ms, msm = _par(Msat)
+    _call("um_add_dmi_bulk", _prec(m[0]), _vec(B), _vec(m), ms, msm, _dp(aLUT), _dp(dLUT), _dp(regions),
+          *[C.c_double(c) for c in cellsize], Nx, Ny, Nz, pbc, int(OpenBC))
+
+
+def AddZhangLiTorque(torque, m, Msat, J, alpha, xi, pol, cellsize, pbc=0):
+    """opencl.AddZhangLiTorque (opencl/zhangli.go:12); J: three parameters."""
+    init()
+    Nz, Ny, Nx = m[0].shape
+    js = [_par(j) for j in J]
+    _call("um_add_zhangli", _prec(m[0]), _vec(torque), _vec(m), *_par(Msat),
+          (C.c_void_p * 3)(*[j[0] for j in js]), (C.c_double * 3)(*[j[1].value for j in js]),
+          *_par(alpha), *_par(xi), *_par(pol), *[C.c_double(c) for c in cellsize], Nx, Ny, Nz, pbc)
+
+
+def AddSlonczewskiTorque2(torque, m, Msat, Jz, fixedP, alpha, pol, lam, eps_prime, thickness, mesh_thickness,
+                          flp=1.0):
+    """opencl.AddSlonczewskiTorque2 (opencl/slonczewski.go:12); fixedP: three parameters."""
+    init()
+    ps = [_par(q) for q in fixedP]
+    _call("um_add_slonczewski", _prec(m[0]), _vec(torque), _vec(m), *_par(Msat), *_par(Jz),
+          (C.c_void_p * 3)(*[q[0] for q in ps]), (C.c_double * 3)(*[q[1].value for q in ps]),
+          *_par(alpha), *_par(pol), *_par(lam), *_par(eps_prime), *_par(thickness),
+          C.c_double(mesh_thickness), C.c_double(flp), m[0].numel())
 
 
 def AddUniaxialAnisotropy2(B, m, Msat, k1, k2, u):
@@ -429,6 +461,18 @@ class Sim:
 
     def set_demag_accuracy(self, acc):
         _call("um_sim_set_demag_accuracy", self.h, C.c_double(acc))
+
+    def set_dmi_bulk(self, lut):
+        """Dbulk LUT (engine/exchange.go:49-66); None: none."""
+        if lut is None:
+            _call("um_sim_set_dmi_bulk", self.h, None)
+        else:
+            lut = np.ascontiguousarray(lut, self.dtype)
+            _call("um_sim_set_dmi_bulk", self.h, lut.ctypes.data_as(C.c_void_p))
+
+    def set_stt_flags(self, disable_zhangli=0, disable_slonczewski=0, fixed_layer_top=True):
+        _call("um_sim_set_stt_flags", self.h, int(disable_zhangli), int(disable_slonczewski),
+              C.c_double(1.0 if fixed_layer_top else -1.0))
 
     def set_demag_newell(self, enable=1, asymptotic_radius=32, zero_self_demag=0):
         """EnableNewellDemag = enable (engine/demag.go:26-56)."""
