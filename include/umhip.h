@@ -279,6 +279,11 @@ int um_sim_energy(void *sim, int term, double *out);
  * torque decreases (torque_threshold <= 0) or until max |torque| is below it,
  * tightening MaxErr; solver, MaxErr, FixDt and precession are restored. */
 int um_sim_relax(void *sim, double torque_threshold);
+/* Minimize(), engine/minimizer.go:56-160: steepest descent with Barzilai-Borwein
+ * steps on the precession-free torque until the last `samples` max|dm| are all
+ * below stop_max_dm (MinimizerStop, default 1e-6; MinimizerSamples, default 10).
+ * Time does not advance; LastErr reports the sampled max|dm|. */
+int um_sim_minimize(void *sim, double stop_max_dm, int samples);
 /* the HIP stream the simulation enqueues on */
 int um_sim_stream(void *sim, void **hip_stream);
 /* timing with HIP events on the simulation stream, read at the per-step

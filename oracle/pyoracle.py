@@ -348,6 +348,9 @@ class Sim:
     def run(self, seconds):
         self._f("sim_run")(self._h(), C.c_double(seconds))
 
+    def minimize(self, stop=1e-6, samples=10):
+        self._f("sim_minimize")(self._h(), C.c_double(stop), int(samples))
+
     def relax(self, torque_threshold=-1.0):
         """Relax(), engine/relax.go:24-100."""
         self._f("sim_relax")(self._h(), C.c_double(torque_threshold))

@@ -177,6 +177,7 @@ enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK23 = 3, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45
   void um_or_sim_set_flags##SFX(void *h, int enable_demag, int openbc, int precess,        \
         int heun_sqrt3);                                                                   \
   void um_or_sim_set_demag_accuracy##SFX(void *h, double acc);                             \
+  void um_or_sim_minimize##SFX(void *h, double stop, int samples);                         \
   /* bulk DMI LUT (Dbulk, engine/exchange.go:49-66); NULL: none */                         \
   void um_or_sim_set_dmi_bulk##SFX(void *h, const real *lut);                               \
   /* DisableZhangLiTorque, DisableSlonczewskiTorque, FixedLayerPosition sign (+1 top) */   \

@@ -1600,6 +1600,58 @@ static double selfdot3(sim_t *s, const real *a) {
   return d;
 }
 /* Relax, engine/relax.go:24-100 */
+/* Minimize, engine/minimizer.go:56-160: steepest descent (minimize.cl) with
+ * Barzilai-Borwein step lengths alternating on NSteps parity, torque without
+ * precession and without thermal noise; stops when the last `samples` max|dm|
+ * are all <= stop.  Time does not advance. */
+void OR(sim_minimize)(void *h, double stop, int samples) {
+  sim_t *s = (sim_t *)h;
+  int n = s->n;
+  int prev_precess = s->precess;
+  s->precess = 0;
+  s->relaxing = 1;
+  double *ring = (double *)calloc(samples, sizeof(double));
+  int count = 0, tail = 0;
+  real *k = s->tmp[0], *k0 = s->tmp[1], *m0 = s->tmp[2], *dm = s->tmp[3], *dk = s->tmp[4];
+  real hh = (real)1e-4;
+  torque_fn(s, k);
+  for (;;) {
+    double rmax = ring[0];
+    for (int i = 1; i < count; i++) if (ring[i] > rmax) rmax = ring[i];
+    if (!(count < samples || rmax > stop)) break;
+    memcpy(m0, s->m, sizeof(real) * 3 * n);
+    /* minimize.cl:11-20 */
+    for (int i = 0; i < n; i++) {
+      real t[3] = {k[i], k[n + i], k[2 * n + i]};
+      real t2 = hh * hh * (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+      real div = (real)4.0 + t2;
+      for (int c = 0; c < 3; c++)
+        s->m[c * n + i] = (((real)4.0 - t2) * m0[c * n + i] + (real)4.0 * hh * t[c]) / div;
+    }
+    memcpy(k0, k, sizeof(real) * 3 * n);
+    torque_fn(s, k);
+    s->lastTorque = OR(maxvecnorm)(k, k + n, k + 2 * n, n);
+    for (int i = 0; i < 3 * n; i++) { dm[i] = s->m[i] - m0[i]; dk[i] = -k[i] + k0[i]; }
+    ring[tail] = OR(maxvecnorm)(dm, dm + n, dm + 2 * n, n);
+    tail = (tail + 1) % samples;
+    if (count < samples) count++;
+    rmax = ring[0];
+    for (int i = 1; i < count; i++) if (ring[i] > rmax) rmax = ring[i];
+    s->lastErr = rmax;
+    if (rmax > s->peakErr) s->peakErr = rmax;
+    real nom, dv;
+    if (s->nsteps % 2 == 0) { nom = (real)OR(dot)(dm, dm, 3 * n); dv = (real)OR(dot)(dm, dk, 3 * n); }
+    else { nom = (real)OR(dot)(dm, dk, 3 * n); dv = (real)OR(dot)(dk, dk, 3 * n); }
+    hh = dv != (real)0 ? nom / dv : (real)1e-4;
+    normalize_m(s);
+    s->nsteps++;
+  }
+  free(ring);
+  s->precess = prev_precess;
+  s->relaxing = 0;
+  s->k1_valid = 0;
+}
+
 void OR(sim_relax)(void *h, double torque_threshold) {
   sim_t *s = (sim_t *)h;
   int prev_solver = s->solver, prev_precess = s->precess;

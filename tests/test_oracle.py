@@ -512,3 +512,23 @@ def test_energy_terms_closed_form(oracle):
     assert s.energy(8) == 0.0
     parts = sum(s.energy(t) for t in range(9))
     assert abs(s.energy(9) - parts) <= 1e-12 * abs(parts)
+
+
+def test_minimize_stoner_wohlfarth(oracle):
+    """Minimize() (engine/minimizer.go) on a uniform film without demag: uniaxial
+    easy axis x, field B along y below the anisotropy field 2K1/Ms; the energy
+    minimum has m_y = B Ms / (2 K1) exactly.  Time does not advance.  (The start
+    avoids 45 degrees, where the energy along the descent path has an
+    inflection and the first Barzilai-Borwein step is unbounded.)"""
+    N, cs = (4, 4, 1), (2e-9, 2e-9, 2e-9)
+    Ms, K1 = 1.1e6, 0.5e6
+    for B in (0.0, 0.03, 0.3):
+        s = oracle.Sim(N, cs, dtype=np.float64)
+        setup(s, oracle, Ms=Ms, Aex=13e-12, alpha=1.0, Ku1=K1, anisU=(1, 0, 0), Bext=(0, B, 0), demag=False,
+              dtype=np.float64)
+        s.set_m(uniform_m(N, (1, 0.3, 0.01)))
+        s.minimize()
+        m = s.average_m()
+        assert abs(m[1] - B * Ms / (2 * K1)) < 1e-5, (B, m)
+        assert abs(m[2]) < 1e-5
+        assert s.state()["t"] == 0.0

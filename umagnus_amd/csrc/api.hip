@@ -666,6 +666,9 @@ int um_sim_backward(void *sim, const void *const dm[3], void *const dm0[3], void
     SIM(sim)->backward(dm, dm0, dms, dbext);
   });
 }
+int um_sim_minimize(void *sim, double stop_max_dm, int samples) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->minimize(stop_max_dm, samples); });
+}
 int um_sim_set_dmi_bulk(void *sim, const void *lut) {
   return guard([&] { SIMCHK(sim); SIM(sim)->set_dmi_bulk(lut); });
 }

@@ -3,6 +3,7 @@
 // the fused kernels.  Stage combinations are fused with normalisation and the
 // step-control norms with the error estimate, so a step reads back one
 // 24-byte result instead of several reductions.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -1202,6 +1203,61 @@ struct Sim : SimBase {
     k1_valid = false;
     gen++;
     UM_HIP(hipStreamSynchronize(st));
+  }
+
+  // Minimize, engine/minimizer.go:56-160 (Exl et al., JAP 115, 17D118): steepest
+  // descent on the precession-free torque with Barzilai-Borwein step lengths,
+  // until the last `samples` max|dm| are all below `stop`.  Time does not advance.
+  void minimize(double stop, int samples) override {
+    UM_REQUIRE(samples > 0, UM_EINVAL, "MinimizerSamples must be positive");
+    gen++;
+    const int prev_precess = precess;
+    precess = 0;
+    relaxing = 1;
+    auto restore = [&] {
+      precess = prev_precess;
+      relaxing = 0;
+      k1_valid = false;
+      gen++;
+    };
+    try {
+      std::vector<double> ring(samples, 0.0);
+      int count = 0, tail = 0;
+      auto ring_max = [&] {
+        double mx = ring[0];
+        for (int i = 1; i < count; i++) mx = std::max(mx, ring[i]);
+        return mx;
+      };
+      DevBuf part(sizeof(double) * 5 * RED_BLOCKS), res(sizeof(double) * 5);
+      double hres5[5];
+      T h = T(1e-4);
+      T *ka = k[0], *kb = k[1];
+      torque_fn(ka);
+      while (count < samples || ring_max() > stop) {
+        launch_min_step<T>(m, m0, ka, h, n, st);
+        torque_fn(kb);
+        launch_min_stats<T>(m, m0, kb, ka, n, part.as<double>(), res.as<double>(), st);
+        UM_HIP(hipMemcpyAsync(hres5, res.p, sizeof(hres5), hipMemcpyDeviceToHost, st));
+        UM_HIP(hipStreamSynchronize(st));
+        lastTorque = std::sqrt(hres5[4]);
+        ring[tail] = std::sqrt(hres5[0]);
+        tail = (tail + 1) % samples;
+        count = std::min(count + 1, samples);
+        set_last_err(ring_max());
+        T nom, div;
+        if (nsteps % 2 == 0) { nom = (T)hres5[1]; div = (T)hres5[2]; }
+        else { nom = (T)hres5[2]; div = (T)hres5[3]; }
+        h = (div != T(0)) ? nom / div : T(1e-4);
+        normalize(m);
+        nsteps++;
+        std::swap(ka, kb);
+      }
+      if (ka != k[0]) std::swap(k[0], k[1]);
+    } catch (...) {
+      restore();
+      throw;
+    }
+    restore();
   }
 
   void set_profiling(int on) override {

@@ -37,6 +37,7 @@ struct SimBase {
   virtual void set_demag_accuracy(double a) = 0;
   virtual void set_demag_newell(int enable, int asymptotic_radius, int zero_self_demag) = 0;
   virtual void set_dmi_bulk(const void *lut) = 0;
+  virtual void minimize(double stop_max_dm, int samples) = 0;
   virtual void set_stt_flags(int disable_zl, int disable_sl, double flp) = 0;
   // discrete RK4 adjoint: record steps, then a reverse sweep from dL/dm_final
   virtual void tape_begin(int max_steps, int checkpoint_every) = 0;

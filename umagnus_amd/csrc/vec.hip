@@ -414,6 +414,99 @@ __global__ __launch_bounds__(RED_THREADS) void k_wdot(WdotArgs<T> a) {
     a.partial[blockIdx.x] = t;
   }
 }
+// ---- steepest-descent minimizer ------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_min_step(T *m, T *m0, const T *k, T h, long N) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    T v[3], t[3];
+    for (int c = 0; c < 3; c++) { v[c] = m[c * N + i]; t[c] = k[c * N + i]; }
+    const T t2 = h * h * (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    const T div = T(4.0) + t2;
+    for (int c = 0; c < 3; c++) {
+      m0[c * N + i] = v[c];
+      m[c * N + i] = ((T(4.0) - t2) * v[c] + T(4.0) * h * t[c]) / div;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_min_stats(const T *m, const T *m0, const T *k, const T *k0,
+                                                          long N, double *partial) {
+  // v: max|dm|^2, sum dm.dm, sum dm.dk, sum dk.dk, max|k|^2
+  double v[5] = {0, 0, 0, 0, 0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    double n2 = 0, k2 = 0;
+    for (int c = 0; c < 3; c++) {
+      const double dm = (double)(m[c * N + i] - m0[c * N + i]);
+      const double dk = (double)(k0[c * N + i] - k[c * N + i]);
+      const double kk = (double)k[c * N + i];
+      n2 += dm * dm;
+      v[2] += dm * dk;
+      v[3] += dk * dk;
+      k2 += kk * kk;
+    }
+    v[1] += n2;
+    v[0] = fmax(v[0], n2);
+    v[4] = fmax(v[4], k2);
+  }
+  __shared__ double sh[5][RED_THREADS / 64];
+#pragma unroll
+  for (int j = 0; j < 5; j++) {
+    double x = v[j];
+    const bool mx = (j == 0 || j == 4);
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double o = __shfl_xor(x, off, 64);
+      x = mx ? fmax(x, o) : x + o;
+    }
+    if (threadIdx.x % 64 == 0) sh[j][threadIdx.x / 64] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < 5; j++) {
+      const bool mx = (j == 0 || j == 4);
+      double r = 0;
+      for (int w = 0; w < RED_THREADS / 64; w++) r = mx ? fmax(r, sh[j][w]) : r + sh[j][w];
+      partial[5 * blockIdx.x + j] = r;
+    }
+  }
+}
+__global__ void k_min_final(const double *partial, int n, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double mx = 0, mk = 0, s[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+  for (int i = 0; i < n; i++) {
+    mx = fmax(mx, partial[5 * i]);
+    mk = fmax(mk, partial[5 * i + 4]);
+    for (int j = 0; j < 3; j++) {   // Kahan
+      const double y = partial[5 * i + 1 + j] - c[j];
+      const double t = s[j] + y;
+      c[j] = (t - s[j]) - y;
+      s[j] = t;
+    }
+  }
+  out[0] = mx;
+  for (int j = 0; j < 3; j++) out[1 + j] = s[j];
+  out[4] = mk;
+}
+template <typename T>
+void launch_min_step(T *m, T *m0, const T *k, T h, long N, hipStream_t s) {
+  hipLaunchKernelGGL(k_min_step<T>, dim3(blocks_for(N, 256, 16384)), dim3(256), 0, s, m, m0, k, h, N);
+  UM_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_min_stats(const T *m, const T *m0, const T *k, const T *k0, long N, double *partial, double *out,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_min_stats<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, m, m0, k, k0, N, partial);
+  UM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(64), 0, s, (const double *)partial, RED_BLOCKS, out);
+  UM_HIP(hipGetLastError());
+}
+template void launch_min_step<float>(float *, float *, const float *, float, long, hipStream_t);
+template void launch_min_step<double>(double *, double *, const double *, double, long, hipStream_t);
+template void launch_min_stats<float>(const float *, const float *, const float *, const float *, long, double *,
+                                      double *, hipStream_t);
+template void launch_min_stats<double>(const double *, const double *, const double *, const double *, long,
+                                       double *, double *, hipStream_t);
+
 // Kahan-compensated final sum over the fixed partials, one thread order.
 __global__ void k_final_sum(const double *partial, int n, double *out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;

@@ -27,6 +27,15 @@ struct StageArgs {
 };
 template <typename T> void launch_stage(const StageArgs<T> &a, hipStream_t s);
 
+// Minimizer step (minimize.cl): m0 <- m, m <- ((4 - h^2|k|^2) m + 4 h k) / (4 + h^2|k|^2)
+template <typename T> void launch_min_step(T *m, T *m0, const T *k, T h, long N, hipStream_t s);
+// Minimizer statistics (engine/minimizer.go:78-95) with dm = m - m0, dk = k0 - k:
+// out = {max|dm|^2, dm.dm, dm.dk, dk.dk, max|k|^2} in FP64 (fixed-order partials:
+// 5*RED_BLOCKS doubles)
+template <typename T>
+void launch_min_stats(const T *m, const T *m0, const T *k, const T *k0, long N, double *partial, double *out,
+                      hipStream_t s);
+
 // Fused RK error reductions.  Writes into res[0..2] (as FP64 bit patterns,
 // max-combined with atomics; res must be zeroed first):
 //  RK45DP: Err = sum e_j k_j;  res = {max|Err|^2, max|k7|^2, max |Err|/|k7|}

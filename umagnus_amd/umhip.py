@@ -39,6 +39,7 @@ EXPORTS = [
     "um_demag_create_newell", "um_demag_kernel_newell", "um_sim_set_demag_newell",
     "um_sim_tape_begin", "um_sim_tape_end", "um_sim_tape_steps", "um_sim_backward",
     "um_add_dmi_bulk", "um_add_zhangli", "um_add_slonczewski", "um_sim_set_dmi_bulk", "um_sim_set_stt_flags",
+    "um_sim_minimize",
     "um_sim_create", "um_sim_destroy", "um_sim_set_scalar", "um_sim_set_vector", "um_sim_set_regions",
     "um_sim_set_exchange_luts", "um_sim_set_bext_lut", "um_sim_set_frozen_lut", "um_sim_set_geometry",
     "um_sim_set_flags", "um_sim_set_demag_accuracy", "um_sim_set_thermal_seed", "um_sim_set_m",
@@ -553,6 +554,10 @@ class Sim:
         ne = C.c_int()
         _call("um_sim_kernel_times", self.h, ms, C.byref(ne))
         return list(ms), ne.value
+
+    def minimize(self, stop=1e-6, samples=10):
+        """Minimize(), engine/minimizer.go:120-160 (um_sim_minimize)."""
+        _call("um_sim_minimize", self.h, C.c_double(stop), int(samples))
 
     def relax(self, torque_threshold=-1.0):
         """Relax(), engine/relax.go:24-100 (um_sim_relax)."""
