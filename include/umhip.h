@@ -143,7 +143,10 @@ int um_reduce_maxvecdiff(int prec, const void *const a[3], const void *const b[3
 /* ---- demag: opencl/conv_demag.go, mag/demagkernel.go ---------------------- */
 /* NewDemag(inputSize, PBC, DemagKernel(...)), opencl/conv_demag.go:27 with the
  * brute-force tensor of mag/demagkernel.go:115 computed on the GPU in FP64.
- * accuracy = DemagAccuracy (engine/demag.go:49). */
+ * accuracy = DemagAccuracy (engine/demag.go:49).  pbc: SetPBC repetitions per
+ * axis (NULL or 0: open); a periodic axis is not padded (P = N, odd Nx
+ * included) and its tensor sums the periodic images of kernelRanges
+ * (mag/demagkernel.go:339-352). */
 int um_demag_create(int prec, int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
                     double accuracy, void **plan);
 /* NewDemag with the Newell tensor: newellDemagConv (engine/demag.go:134-142)
@@ -151,7 +154,8 @@ int um_demag_create(int prec, int Nx, int Ny, int Nz, const int pbc[3], const do
  * :1108-1324) computed on the GPU in FP64.  asymptotic_radius: far-field
  * expansion beyond this many cells (reference default 32, negative: never);
  * zero_self_demag != 0 adds 1/3 to the diagonal self terms (default 0).
- * Open boundaries only (UM_EUNSUP for PBC). */
+ * pbc as for um_demag_create; as in the reference the origin element holds
+ * the self term alone (the images summed into it are discarded, :1300-1310). */
 int um_demag_create_newell(int prec, int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
                            int asymptotic_radius, int zero_self_demag, void **plan);
 /* NewDemag from a caller-supplied FP64 host tensor (xx,yy,zz,yz,xz,xy), each of
@@ -175,6 +179,23 @@ int um_demag_kernel_bf(int Nx, int Ny, int Nz, const int pbc[3], const double cs
  * FP64 into host buffers K[c] of the padded size. */
 int um_demag_kernel_newell(int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
                            int asymptotic_radius, int zero_self_demag, double *const K[6]);
+
+/* ---- demag tensor cache: mag/demagkernel.go:17-109, newellDemagKernel.go:1006-1106 ----
+ * The -cache flag (engine/gofiles.go:17): with a directory set, um_demag_create,
+ * um_demag_create_newell and the simulation handle read the tensor from
+ * <dir>/uMagNUSkernel_<N>_<pbc>_<cellsize>_<accuracy | zero_self_demag>_<ij>.ovf
+ * (the reference's names and OVF2 "binary 4" files, so caches are shared) and
+ * write it there after computing it when a file is missing or unreadable.
+ * With a cache the float32 tensor of the files is used (also when freshly
+ * computed); without one (NULL or "", the default) the FP64 tensor is. */
+int um_set_kernel_cache(const char *dir);
+/* the file prefix of a plan's cache entry (fmt.Sprint spelling) into out */
+int um_kernel_cache_basename(const char *dir, const int N[3], const int pbc[3], const double cs[3], int newell,
+                             double accuracy, int zero_self_demag, char *out, int out_len);
+/* LoadKernel (mag/demagkernel.go:97): a scalar OVF1/OVF2 field of size P into K */
+int um_load_kernel(const char *fname, const int P[3], double *K);
+/* SaveKernel (mag/demagkernel.go:102): K of size P as an OVF2 binary 4 scalar field */
+int um_save_kernel(const char *fname, const int P[3], const double cs[3], const char *name, const double *K);
 
 /* ---- simulation handle: engine/ effective field, torque, steppers --------- */
 /* Parameter ids (the .mx3 quantity names) for um_sim_set_scalar / _vector */

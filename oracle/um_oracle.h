@@ -34,15 +34,18 @@ void um_or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t 
 
 /* Brute-force demag tensor, mag/demagkernel.go:115-336.  Writes the six
  * real-space components (xx,yy,zz,yz,xz,xy) of size P = padSize(N, pbc) in
- * FP64.  In 2D (P[2]==1) the yz and xz outputs are zero.  Returns 0 on
- * success.  [golden: test/demagSmall*.mx3, demagcube.mx3, demagodd.mx3] */
+ * FP64; with PBC each element sums its periodic images (kernelRanges).  In
+ * 2D (P[2]==1) the yz and xz outputs are zero.  Returns 0 on success.
+ * [golden: test/demagSmall*.mx3, demagcube.mx3, demagodd.mx3, demag2Dpbc*.mx3,
+ *  demagrodpbc.mx3; closed: sum of open-boundary images] */
 int um_or_demag_kernel_bf(const int N[3], const int pbc[3], const double cs[3],
                           double accuracy, double *K[6]);
 /* padSize, mag/demagkernel.go:404-420 */
 void um_or_pad_size(const int N[3], const int pbc[3], int P[3]);
 
-/* Newell demag tensor, mag/newellDemagKernel.go:1108-1324 (open boundaries;
- * returns -2 for periodic ones).  Same output layout as um_or_demag_kernel_bf.
+/* Newell demag tensor, mag/newellDemagKernel.go:1108-1324, with periodic
+ * images as the brute-force one (the origin keeps the self term alone).
+ * Same output layout as um_or_demag_kernel_bf.
  * [closed: prism demag factors (Aharoni), block sums, far-field expansion vs
  *  the exact form; golden: test/standardproblem4_newell-3d.mx3,
  *  standardproblem4b_newell.mx3] */

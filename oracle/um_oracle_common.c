@@ -70,7 +70,10 @@ static int wrap_(int number, int max) {
 /* CalcDemagKernel, mag/demagkernel.go:115-336.  The FP64 accumulation order of
  * the integration loops follows the reference exactly; the reference then
  * stores float32(B[d]) (:267), which the FP32 oracle does when it consumes
- * this array.  Component order of K: xx, yy, zz, yz, xz, xy. */
+ * this array.  With PBC the reference adds float32(B) of every periodic image
+ * into its float32 array; here the images are summed in FP64 (same order) and
+ * rounded once, which differs by at most (images) x 2^-24 relative.
+ * Component order of K: xx, yy, zz, yz, xz, xy. */
 int um_or_demag_kernel_bf(const int N[3], const int pbc[3], const double cs[3],
                           double accuracy, double *K[6]) {
   int size[3];
@@ -79,7 +82,10 @@ int um_or_demag_kernel_bf(const int N[3], const int pbc[3], const double cs[3],
   if (!(cs[0] > 0 && cs[1] > 0 && cs[2] > 0) || !(accuracy > 0)) return -1;
   const long P = (long)size[0] * size[1] * size[2];
   /* array[s][d] for d >= s, mapped onto K[].  The loop nest is parallel over
-   * (s, z): every (s, z) writes its own entries (no PBC images). */
+   * (s, zw): every task owns the elements of one wrapped z plane and visits
+   * the offsets z of the range that wrap onto it in ascending order, so each
+   * element sums its periodic images in the reference's loop order (z, y, x
+   * ascending, :165-268). */
   double *arr[3][3] = {{K[0], K[5], K[4]}, {NULL, K[1], K[3]}, {NULL, NULL, K[2]}};
   for (int c = 0; c < 6; c++) memset(K[c], 0, sizeof(double) * P);
 
@@ -90,15 +96,15 @@ int um_or_demag_kernel_bf(const int N[3], const int pbc[3], const double cs[3],
   if (cs[2] < L) L = cs[2];
 
 #define AT(a, z, y, x) (a)[((long)(z) * size[1] + (y)) * size[0] + (x)]
-  const int nzr = r2[2] - r1[2] + 1;
+  const int nzw = size[2] / 2 + 1;
 #pragma omp parallel for schedule(dynamic)
-  for (int sz = 0; sz < 3 * nzr; sz++) {
-    const int s = sz / nzr, z = r1[2] + sz % nzr;
+  for (int sz = 0; sz < 3 * nzw; sz++) {
+    const int s = sz / nzw, zt = sz % nzw;
     int u = s, v = (s + 1) % 3, w = (s + 2) % 3;
     double R[3], R2[3], pole[3];
-    {
+    for (int z = r1[2]; z <= r2[2]; z++) {
       int zw = wrap_(z, size[2]);
-      if (zw > size[2] / 2) continue;
+      if (zw != zt) continue;
       R[2] = (double)z * cs[2];
       for (int y = r1[1]; y <= r2[1]; y++) {
         int yw = wrap_(y, size[1]);

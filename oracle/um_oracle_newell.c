@@ -531,15 +531,19 @@ int um_or_demag_kernel_newell(const int N[3], const int pbc[3], const double cs[
                               int asymptotic_radius, int zero_self_demag, double *K[6]) {
   int size[3];
   for (int i = 0; i < 3; i++)
-    if (pbc[i] != 0) return -2;   /* periodic images: not modelled */
+    if (pbc[i] < 0) return -1;
   um_or_pad_size(N, pbc, size);
   if (!(size[0] > 0 && size[1] > 0 && size[2] > 0)) return -1;
   if (!(cs[0] > 0 && cs[1] > 0 && cs[2] > 0) || zero_self_demag < 0) return -1;
   const long P = (long)size[0] * size[1] * size[2];
   for (int c = 0; c < 6; c++) memset(K[c], 0, sizeof(double) * P);
+  /* kernelRanges (mag/demagkernel.go:339-352) */
   int r1[3], r2[3];
-  for (int c = 0; c < 3; c++) { r1[c] = -(size[c] - 1) / 2; r2[c] = (size[c] - 1) / 2; }
-  if (size[2] == 1) r2[2] = 0;
+  for (int c = 0; c < 3; c++) {
+    if (pbc[c] == 0) { r1[c] = -(size[c] - 1) / 2; r2[c] = (size[c] - 1) / 2; }
+    else { r1[c] = -(size[c] * pbc[c] - 1); r2[c] = size[c] * pbc[c] - 1; }
+  }
+  if (size[2] == 1 && pbc[2] == 0) r2[2] = 0;
 
   double d[3];
   um_or_newell_cell(cs, d);
@@ -554,27 +558,32 @@ int um_or_demag_kernel_newell(const int N[3], const int pbc[3], const double cs[
   asym_comp_init(&far[5], 1, 0, 1, 2, d);
 
 #define AT(c, z, y, x) K[c][((long)(z) * size[1] + (y)) * size[0] + (x)]
-  const int nzr = r2[2] - r1[2] + 1;
+  /* The reference runs one goroutine per offset z (:1165); with PBC the
+   * goroutines of the z images of one plane add into the same float32
+   * elements unsynchronised.  Here one task owns each wrapped plane and sums
+   * its images in ascending z (then y, x) in FP64. */
+  const int nzw = size[2] / 2 + 1;
 #pragma omp parallel for schedule(dynamic)
-  for (int iz = 0; iz < nzr; iz++) {
-    const int z = r1[2] + iz;
-    const int zw = (z % size[2] + size[2]) % size[2];
-    if (zw > size[2] / 2) continue;
-    double R[3];
-    R[2] = (double)z * d[2];
-    for (int y = r1[1]; y <= r2[1]; y++) {
-      const int yw = (y % size[1] + size[1]) % size[1];
-      if (yw > size[1] / 2) continue;
-      R[1] = (double)y * d[1];
-      for (int x = r1[0]; x <= r2[0]; x++) {
-        const int xw = (x % size[0] + size[0]) % size[0];
-        if (xw > size[0] / 2) continue;
-        R[0] = (double)x * d[0];
-        const int farfield = arad >= 0.0 && R[0] * R[0] > arad2 - R[2] * R[2] - R[1] * R[1];
-        for (int c = 0; c < 6; c++) {
-          const double v = farfield ? -1.0 * asym_comp_eval(&far[c], R)
-                                    : um_or_newell_exact(c, d, R[0], R[1], R[2]);
-          AT(c, zw, yw, xw) += v;
+  for (int zt = 0; zt < nzw; zt++) {
+    for (int z = r1[2]; z <= r2[2]; z++) {
+      const int zw = (z % size[2] + size[2]) % size[2];
+      if (zw != zt) continue;
+      double R[3];
+      R[2] = (double)z * d[2];
+      for (int y = r1[1]; y <= r2[1]; y++) {
+        const int yw = (y % size[1] + size[1]) % size[1];
+        if (yw > size[1] / 2) continue;
+        R[1] = (double)y * d[1];
+        for (int x = r1[0]; x <= r2[0]; x++) {
+          const int xw = (x % size[0] + size[0]) % size[0];
+          if (xw > size[0] / 2) continue;
+          R[0] = (double)x * d[0];
+          const int farfield = arad >= 0.0 && R[0] * R[0] > arad2 - R[2] * R[2] - R[1] * R[1];
+          for (int c = 0; c < 6; c++) {
+            const double v = farfield ? -1.0 * asym_comp_eval(&far[c], R)
+                                      : um_or_newell_exact(c, d, R[0], R[1], R[2]);
+            AT(c, zw, yw, xw) += v;
+          }
         }
       }
     }
@@ -594,7 +603,8 @@ int um_or_demag_kernel_newell(const int N[3], const int pbc[3], const double cs[
         AT(4, z, y, x) = sx * sz * AT(4, z2, y2, x2);
         AT(5, z, y, x) = sx * sy * AT(5, z2, y2, x2);
       }
-  /* self term (:1300-1310) */
+  /* self term (:1300-1310); with PBC the images summed into the origin
+   * element are discarded with it, as in the reference */
   double third = zero_self_demag != 0 ? 1.0 / 3.0 : 0.0;
   AT(0, 0, 0, 0) = -um_or_self_demag_nx(d[0], d[1], d[2]) + third;
   AT(1, 0, 0, 0) = -um_or_self_demag_nx(d[1], d[2], d[0]) + third;

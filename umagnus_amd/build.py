@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libumhip.so")
-SOURCES = ["api.hip", "demag.hip", "newell.hip", "field.hip", "vec.hip", "sim.hip", "adjoint.hip"]
+SOURCES = ["api.hip", "demag.hip", "newell.hip", "kcache.hip", "field.hip", "vec.hip", "sim.hip", "adjoint.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("UMHIP_ARCH", "gfx950")
 FLAGS = [

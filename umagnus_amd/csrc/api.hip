@@ -488,7 +488,7 @@ int um_demag_create(int prec, int Nx, int Ny, int Nz, const int pbc[3], const do
     DevBuf K(sizeof(double) * 6 * np);
     double *Kd[6];
     for (int c = 0; c < 6; c++) Kd[c] = K.as<double>() + c * np;
-    demag_kernel_bf_device(N, pb, cs, accuracy, Kd, P, s);
+    demag_tensor(N, pb, cs, 0, accuracy, 0, 0, Kd, P, s);
     *plan = demag_create(prec, N, P, Kd, s);
   });
 }
@@ -507,8 +507,36 @@ int um_demag_create_newell(int prec, int Nx, int Ny, int Nz, const int pbc[3], c
     DevBuf K(sizeof(double) * 6 * np);
     double *Kd[6];
     for (int c = 0; c < 6; c++) Kd[c] = K.as<double>() + c * np;
-    demag_kernel_newell_device(N, pb, cs, asymptotic_radius, zero_self_demag, Kd, P, s);
+    demag_tensor(N, pb, cs, 1, 0.0, asymptotic_radius, zero_self_demag, Kd, P, s);
     *plan = demag_create(prec, N, P, Kd, s);
+  });
+}
+int um_set_kernel_cache(const char *dir) { return guard([&] { set_kernel_cache(dir); }); }
+int um_kernel_cache_basename(const char *dir, const int N[3], const int pbc[3], const double cs[3], int newell,
+                             double accuracy, int zero_self_demag, char *out, int out_len) {
+  return guard([&] {
+    UM_REQUIRE(dir && N && pbc && cs && out && out_len > 0, UM_EINVAL, "um_kernel_cache_basename: bad arguments");
+    const std::string b = kernel_cache_basename(dir, N, pbc, cs, newell, accuracy, zero_self_demag);
+    UM_REQUIRE((int)b.size() < out_len, UM_EINVAL, "um_kernel_cache_basename: buffer too small");
+    std::memcpy(out, b.c_str(), b.size() + 1);
+  });
+}
+int um_load_kernel(const char *fname, const int P[3], double *K) {
+  return guard([&] {
+    UM_REQUIRE(fname && P && K, UM_EINVAL, "um_load_kernel: bad arguments");
+    int sz[3], nc;
+    std::vector<double> v;
+    ovf_read(fname, sz, &nc, v);
+    UM_REQUIRE(nc == 1 && sz[0] == P[0] && sz[1] == P[1] && sz[2] == P[2], UM_EINVAL,
+               "um_load_kernel: not a scalar field of the requested size");
+    std::memcpy(K, v.data(), sizeof(double) * v.size());
+  });
+}
+int um_save_kernel(const char *fname, const int P[3], const double cs[3], const char *name, const double *K) {
+  return guard([&] {
+    UM_REQUIRE(fname && P && cs && name && K && P[0] > 0 && P[1] > 0 && P[2] > 0, UM_EINVAL,
+               "um_save_kernel: bad arguments");
+    ovf_write_scalar(fname, P, cs, name, K);
   });
 }
 int um_demag_kernel_newell(int Nx, int Ny, int Nz, const int pbc[3], const double cs[3], int asymptotic_radius,

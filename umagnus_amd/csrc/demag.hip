@@ -250,6 +250,72 @@ __global__ __launch_bounds__(FFT_NT) void k_xinv(XiArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// X axis of odd length (periodic along x with odd Nx, so Px = Nx is odd and
+// the two-for-one packing does not apply): each row is a full complex
+// transform of length Px with zero imaginary parts, of which the H = Px/2+1
+// non-redundant outputs are stored; the inverse rebuilds the Hermitian
+// spectrum X[Px-k] = conj X[k] and keeps the real parts.  Small grids only
+// (nl*Px <= FFT_NT*EPT per work-group), generic radix passes.
+// a.f: FftLen of length Px; a.fhalf: H; a.nl lines per work-group.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(FFT_NT) void k_xfwd_odd(XArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_[];
+  C2<T> *lds = (C2<T> *)smem_;
+  const int c = blockIdx.y;
+  const int n = a.f.n, nl = a.nl, half = n / 2 + 1;
+  const int row0 = blockIdx.x * nl;
+  const T *src = a.src[c];
+  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
+    const int l = (int)a.f.fn.div(g), e = g - l * n;
+    const int r = row0 + l;
+    T v = T(0);
+    if (r < a.rows && e < a.Nx) {
+      const long i = (long)r * a.in_pitch + e;
+      v = src[i];
+      if (a.mode == 0) v = (T(MU0) * a.Ms.at(i)) * (a.vol ? a.vol[i] : T(1)) * v;
+    }
+    lds[g] = C2<T>{v, T(0)};
+  }
+  __syncthreads();
+  lds_fft<T, -1, 0>(lds, nl, a.f.fn, n, 1, a.f);
+  for (int g = threadIdx.x; g < nl * half; g += FFT_NT) {
+    const int l = (int)a.fhalf.div(g), k = g - l * half;
+    const int r = row0 + l;
+    if (r >= a.rows) continue;
+    const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+    a.out[c][((long)z * a.Py + y) * a.Hp + k] = lds[l * n + k];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(FFT_NT) void k_xinv_odd(XiArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_[];
+  C2<T> *lds = (C2<T> *)smem_;
+  const int c = blockIdx.y;
+  const int n = a.f.n, nl = a.nl, half = n / 2 + 1;
+  const int row0 = blockIdx.x * nl;
+  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
+    const int l = (int)a.f.fn.div(g), k = g - l * n;
+    const int r = row0 + l;
+    C2<T> v = C2<T>{T(0), T(0)};
+    if (r < a.rows) {
+      const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+      const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
+      v = k < half ? irow[k] : cconj(irow[n - k]);
+    }
+    lds[g] = v;
+  }
+  __syncthreads();
+  lds_fft<T, +1, 0>(lds, nl, a.f.fn, n, 1, a.f);
+  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
+    const int l = (int)a.f.fn.div(g), e = g - l * n;
+    const int r = row0 + l;
+    if (r < a.rows && e < a.Nx) a.dst[c][(long)r * a.Nx + e] = lds[g].x;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Register four-step X inverse (FP32, zero-padded rows: Px = 2 Nx, n = Px/2
 // complex = A*B with B = A or 2A).  A group of B lanes owns G = B/A rows.
 // Step 1: lane (h, n1) transforms z[n1 + A n2] over n2 (B points) in
@@ -823,94 +889,110 @@ static void launch_convz(const ConvZArgs<T> &a, int Nz, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Tensor generation in FP64: brute-force face-charge integration,
-// mag/demagkernel.go:115-336.  One thread per destination cell and source
-// direction; the summation order follows the reference loop nest and FP
-// contraction is disabled so the FP64 values match a sequential evaluation.
+// mag/demagkernel.go:115-336.  One thread per wrapped destination element
+// (xw, yw, zw) with every coordinate <= P/2 and per source direction; the
+// thread visits the periodic images of that element (every offset of the
+// kernelRanges box that wraps onto it, :339-352) in the reference loop order,
+// z outer, y, x inner, each ascending, and sums their FP64 fields in that
+// order.  Without PBC an element has at most one image.  FP contraction is
+// disabled so the FP64 values match a sequential evaluation.
 // ---------------------------------------------------------------------------
 struct BfArgs {
   int P[3];
-  int r1[3], ext[3];     // loop start and extent per axis
+  int r1[3], r2[3];      // image offsets per axis: r1 <= x <= r2
+  int H[3];              // wrapped destinations per axis: 0..P/2
   double cs[3], L, accuracy;
   double *K[6];          // xx yy zz yz xz xy
 };
 
-__device__ __forceinline__ int bf_wrap(int x, int m) {
-  while (x < 0) x += m;
-  while (x >= m) x -= m;
-  return x;
-}
 __device__ __forceinline__ double bf_delta(int d) {
   if (d < 0) d = -d;
   if (d > 0) d -= 1;
   return (double)d;
 }
 
-__global__ void k_demag_bf(BfArgs a) {
+// Field at offset (x, y, z) of a source cell magnetised along s: two face
+// charge sheets of nv x nw points against an nx x ny x nz destination grid
+// (:186-262).
+__device__ void bf_image(const BfArgs &a, int s, int x, int y, int z, double B[3]) {
 #pragma clang fp contract(off)
-  const long total = (long)a.ext[0] * a.ext[1] * a.ext[2] * 3;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-    const int s = (int)(t % 3);
-    long q = t / 3;
-    const int x = a.r1[0] + (int)(q % a.ext[0]);
-    q /= a.ext[0];
-    const int y = a.r1[1] + (int)(q % a.ext[1]);
-    const int z = a.r1[2] + (int)(q / a.ext[1]);
-    const int xw = bf_wrap(x, a.P[0]), yw = bf_wrap(y, a.P[1]), zw = bf_wrap(z, a.P[2]);
-    if (xw > a.P[0] / 2 || yw > a.P[1] / 2 || zw > a.P[2] / 2) continue;
-    const int u = s, v = (s + 1) % 3, w = (s + 2) % 3;
-    const double *cs = a.cs;
-    double R[3] = {(double)x * cs[0], (double)y * cs[1], (double)z * cs[2]};
-    const double dx = bf_delta(x) * cs[0], dy = bf_delta(y) * cs[1], dz = bf_delta(z) * cs[2];
-    double d = sqrt(dx * dx + dy * dy + dz * dz);
-    if (d == 0) d = a.L;
-    const double maxSize = d / a.accuracy;
-    int nv = (int)(fmax(cs[v] / maxSize, 1.0) + 0.5);
-    int nw = (int)(fmax(cs[w] / maxSize, 1.0) + 0.5);
-    const int nx = (int)(fmax(cs[0] / maxSize, 1.0) + 0.5);
-    const int ny = (int)(fmax(cs[1] / maxSize, 1.0) + 0.5);
-    const int nz = (int)(fmax(cs[2] / maxSize, 1.0) + 0.5);
-    nv *= 2;
-    nw *= 2;
-    // 64-bit count as Go's int: 32 bits overflow at DemagAccuracy ~40
-    const double scale = 1 / (double)((long long)nv * nw * nx * ny * nz);
-    const double surface = cs[v] * cs[w];
-    const double charge = surface * scale;
-    const double pu1 = cs[u] / 2.;
-    const double pu2 = -pu1;
-    double B0 = 0, B1 = 0, B2 = 0;
-    double pole[3];
-    for (int i = 0; i < nv; i++) {
-      pole[v] = -(cs[v] / 2.) + cs[v] / (double)(2 * nv) + (double)i * (cs[v] / (double)nv);
-      for (int j = 0; j < nw; j++) {
-        pole[w] = -(cs[w] / 2.) + cs[w] / (double)(2 * nw) + (double)j * (cs[w] / (double)nw);
-        for (int al = 0; al < nx; al++) {
-          const double rx = R[0] - cs[0] / 2 + cs[0] / (double)(2 * nx) + (cs[0] / (double)nx) * (double)al;
-          for (int be = 0; be < ny; be++) {
-            const double ry = R[1] - cs[1] / 2 + cs[1] / (double)(2 * ny) + (cs[1] / (double)ny) * (double)be;
-            for (int ga = 0; ga < nz; ga++) {
-              const double rz = R[2] - cs[2] / 2 + cs[2] / (double)(2 * nz) + (cs[2] / (double)nz) * (double)ga;
-              pole[u] = pu1;
-              double X = rx - pole[0], Y = ry - pole[1], Z = rz - pole[2];
-              double r = sqrt(X * X + Y * Y + Z * Z);
-              double qr = charge / (4 * M_PI * r * r * r);
-              const double bx = X * qr, by = Y * qr, bz = Z * qr;
-              pole[u] = pu2;
-              X = rx - pole[0]; Y = ry - pole[1]; Z = rz - pole[2];
-              r = sqrt(X * X + Y * Y + Z * Z);
-              qr = -charge / (4 * M_PI * r * r * r);
-              B0 += (bx + X * qr);
-              B1 += (by + Y * qr);
-              B2 += (bz + Z * qr);
-            }
+  const int u = s, v = (s + 1) % 3, w = (s + 2) % 3;
+  const double *cs = a.cs;
+  const double R[3] = {(double)x * cs[0], (double)y * cs[1], (double)z * cs[2]};
+  const double dx = bf_delta(x) * cs[0], dy = bf_delta(y) * cs[1], dz = bf_delta(z) * cs[2];
+  double d = sqrt(dx * dx + dy * dy + dz * dz);
+  if (d == 0) d = a.L;
+  const double maxSize = d / a.accuracy;
+  int nv = (int)(fmax(cs[v] / maxSize, 1.0) + 0.5);
+  int nw = (int)(fmax(cs[w] / maxSize, 1.0) + 0.5);
+  const int nx = (int)(fmax(cs[0] / maxSize, 1.0) + 0.5);
+  const int ny = (int)(fmax(cs[1] / maxSize, 1.0) + 0.5);
+  const int nz = (int)(fmax(cs[2] / maxSize, 1.0) + 0.5);
+  nv *= 2;
+  nw *= 2;
+  // 64-bit count as Go's int: 32 bits overflow at DemagAccuracy ~40
+  const double scale = 1 / (double)((long long)nv * nw * nx * ny * nz);
+  const double surface = cs[v] * cs[w];
+  const double charge = surface * scale;
+  const double pu1 = cs[u] / 2.;
+  const double pu2 = -pu1;
+  double B0 = 0, B1 = 0, B2 = 0;
+  double pole[3];
+  for (int i = 0; i < nv; i++) {
+    pole[v] = -(cs[v] / 2.) + cs[v] / (double)(2 * nv) + (double)i * (cs[v] / (double)nv);
+    for (int j = 0; j < nw; j++) {
+      pole[w] = -(cs[w] / 2.) + cs[w] / (double)(2 * nw) + (double)j * (cs[w] / (double)nw);
+      for (int al = 0; al < nx; al++) {
+        const double rx = R[0] - cs[0] / 2 + cs[0] / (double)(2 * nx) + (cs[0] / (double)nx) * (double)al;
+        for (int be = 0; be < ny; be++) {
+          const double ry = R[1] - cs[1] / 2 + cs[1] / (double)(2 * ny) + (cs[1] / (double)ny) * (double)be;
+          for (int ga = 0; ga < nz; ga++) {
+            const double rz = R[2] - cs[2] / 2 + cs[2] / (double)(2 * nz) + (cs[2] / (double)nz) * (double)ga;
+            pole[u] = pu1;
+            double X = rx - pole[0], Y = ry - pole[1], Z = rz - pole[2];
+            double r = sqrt(X * X + Y * Y + Z * Z);
+            double qr = charge / (4 * M_PI * r * r * r);
+            const double bx = X * qr, by = Y * qr, bz = Z * qr;
+            pole[u] = pu2;
+            X = rx - pole[0]; Y = ry - pole[1]; Z = rz - pole[2];
+            r = sqrt(X * X + Y * Y + Z * Z);
+            qr = -charge / (4 * M_PI * r * r * r);
+            B0 += (bx + X * qr);
+            B1 += (by + Y * qr);
+            B2 += (bz + Z * qr);
           }
         }
       }
     }
+  }
+  B[0] = B0; B[1] = B1; B[2] = B2;
+}
+
+__global__ void k_demag_bf(BfArgs a) {
+#pragma clang fp contract(off)
+  const long total = (long)a.H[0] * a.H[1] * a.H[2] * 3;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int s = (int)(t % 3);
+    long q = t / 3;
+    const int xw = (int)(q % a.H[0]);
+    q /= a.H[0];
+    const int yw = (int)(q % a.H[1]);
+    const int zw = (int)(q / a.H[1]);
+    double S0 = 0, S1 = 0, S2 = 0;
+    for (int z = image_first(zw, a.P[2], a.r1[2]); z <= a.r2[2]; z += a.P[2])
+      for (int y = image_first(yw, a.P[1], a.r1[1]); y <= a.r2[1]; y += a.P[1])
+        for (int x = image_first(xw, a.P[0], a.r1[0]); x <= a.r2[0]; x += a.P[0]) {
+          double B[3];
+          bf_image(a, s, x, y, z, B);
+          S0 += B[0];
+          S1 += B[1];
+          S2 += B[2];
+        }
     const long I = ((long)zw * a.P[1] + yw) * a.P[0] + xw;
     // array[s][d] for d >= s: s=0 -> xx, xy, xz; s=1 -> yy, yz; s=2 -> zz
-    if (s == 0) { a.K[0][I] = B0; a.K[5][I] = B1; a.K[4][I] = B2; }
-    else if (s == 1) { a.K[1][I] = B1; a.K[3][I] = B2; }
-    else { a.K[2][I] = B2; }
+    if (s == 0) { a.K[0][I] = S0; a.K[5][I] = S1; a.K[4][I] = S2; }
+    else if (s == 1) { a.K[1][I] = S1; a.K[3][I] = S2; }
+    else { a.K[2][I] = S2; }
   }
 }
 
@@ -1328,6 +1410,7 @@ struct DemagPlan : DemagBase {
   DevBuf kern;                  // 6 x H*KHy*KHz of T
   int nl_x = 1, nl_y = 1, B_conv = 1;
   bool zreg = false;            // register Z pass (Pz = 2 Nz, Nz in ConvZTable)
+  bool xodd = false;            // odd Px (periodic along x, odd Nx): full-length X passes
   long kcount = 0;
 
   static int lines_for(int n) {
@@ -1339,7 +1422,7 @@ struct DemagPlan : DemagBase {
   void setup(const int Nin[3], const int Pin[3], hipStream_t s) {
     for (int i = 0; i < 3; i++) { N[i] = Nin[i]; P[i] = Pin[i]; }
     prec = sizeof(T) == 4 ? UM_F32 : UM_F64;
-    UM_REQUIRE(P[0] % 2 == 0, UM_EUNSUP, "odd padded X size (PBC with odd Nx) not supported");
+    xodd = (P[0] % 2) != 0;
     is2d = (N[2] == 1);
     UM_REQUIRE(!is2d || P[2] == 1, UM_EUNSUP, "2D grid with Z padding");
     H = P[0] / 2 + 1;
@@ -1351,11 +1434,11 @@ struct DemagPlan : DemagBase {
     tx2.make(P[0], s);
     ty.make(P[1], s);
     tz.make(P[2], s);
-    nl_x = CtTable<T>::x(P[0] / 2);
+    nl_x = xodd ? lines_for(P[0]) : CtTable<T>::x(P[0] / 2);
     if (!nl_x) nl_x = lines_for(P[0] / 2);
     // small grids: trade rows per work-group for work-groups while the X pass
     // would launch fewer than ~3 per CU (the smaller variants are compiled in)
-    if (std::is_same<T, float>::value && CtTable<T>::x(P[0] / 2)) {
+    if (std::is_same<T, float>::value && !xodd && CtTable<T>::x(P[0] / 2)) {
       const int rows = N[1] * N[2];
       for (int nl : {nl_x / 4, 1}) {
         if (nl < 1 || 3L * ((rows + nl_x - 1) / nl_x) >= SMALL_GRID_WG) break;
@@ -1402,7 +1485,8 @@ struct DemagPlan : DemagBase {
     DevBuf mx(sizeof(unsigned long long));
     UM_HIP(hipMemsetAsync(mx.p, 0, 8, s));
     const int capd = FFT_NT * FftEpt<double>::value;
-    UM_REQUIRE(P[0] / 2 <= capd && P[1] <= capd && P[2] <= capd, UM_EUNSUP, "tensor FFT too long for FP64 LDS budget");
+    UM_REQUIRE((xodd ? P[0] : P[0] / 2) <= capd && P[1] <= capd && P[2] <= capd, UM_EUNSUP,
+               "tensor FFT too long for FP64 LDS budget");
     const double scale = 1.0 / ((double)P[0] * P[1] * P[2]);
     for (int c = 0; c < 6; c++) {
       T *out = kern.as<T>() + c * kcount;
@@ -1417,8 +1501,14 @@ struct DemagPlan : DemagBase {
       xa.in_pitch = P[0];
       xa.src[0] = Kdev[c]; xa.Ms = Par<double>{nullptr, 1.0}; xa.vol = nullptr;
       xa.out[0] = (double2 *)sp.p;
-      dim3 gx((xa.rows + xa.nl - 1) / xa.nl, 1);
-      hipLaunchKernelGGL((k_xfwd<double, 0, 0>), gx, dim3(FFT_NT), sizeof(double2) * xa.nl * (P[0] / 2), s, xa);
+      if (xodd) {
+        xa.f = dx2.len; xa.nl = capd / P[0]; xa.fhalf.init(H);
+        dim3 gx((xa.rows + xa.nl - 1) / xa.nl, 1);
+        hipLaunchKernelGGL(k_xfwd_odd<double>, gx, dim3(FFT_NT), sizeof(double2) * xa.nl * P[0], s, xa);
+      } else {
+        dim3 gx((xa.rows + xa.nl - 1) / xa.nl, 1);
+        hipLaunchKernelGGL((k_xfwd<double, 0, 0>), gx, dim3(FFT_NT), sizeof(double2) * xa.nl * (P[0] / 2), s, xa);
+      }
       UM_HIP(hipGetLastError());
       if (P[1] > 1) {
         ColArgs<double> ca{};
@@ -1464,7 +1554,13 @@ struct DemagPlan : DemagBase {
       for (int c = 0; c < 3; c++) { a.src[c] = (const T *)mv[c]; a.out[c] = sp[c]; }
       a.Ms = Par<T>{(const T *)Msa, (T)Ms_mul}; a.vol = (const T *)vol;
       dim3 g((a.rows + nl_x - 1) / nl_x, 3);
-      launch_xfwd<T>(a, P[0] / 2, nl_x, g, s);
+      if (xodd) {
+        a.f = tx2.len; a.fhalf.init(H);
+        hipLaunchKernelGGL(k_xfwd_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
+        UM_HIP(hipGetLastError());
+      } else {
+        launch_xfwd<T>(a, P[0] / 2, nl_x, g, s);
+      }
     }
     const T *K[6];
     for (int c = 0; c < 6; c++) K[c] = kern.as<T>() + c * kcount;
@@ -1519,7 +1615,13 @@ struct DemagPlan : DemagBase {
     XiArgs<T> a = xinv_args(nl_x);
     for (int c = 0; c < 3; c++) a.dst[c] = (T *)Bv[c];
     dim3 g((a.rows + nl_x - 1) / nl_x, 3);
-    launch_xinv<T>(a, P[0] / 2, nl_x, g, s);
+    if (xodd) {
+      a.f = tx2.len; a.fhalf.init(H);
+      hipLaunchKernelGGL(k_xinv_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
+      UM_HIP(hipGetLastError());
+    } else {
+      launch_xinv<T>(a, P[0] / 2, nl_x, g, s);
+    }
   }
 
   // Demag convolution fused with the effective field and torque: the X
@@ -1539,7 +1641,7 @@ struct DemagPlan : DemagBase {
       const FieldArgs<T> &f = *(const FieldArgs<T> *)fav;
       const int n = P[0] / 2, R = xf_rows(n);
       // one four-cell group per thread
-      if (!R || !field4_aligned(f) || R * (N[0] / 4) > FFT_NT) return false;
+      if (xodd || !R || !field4_aligned(f) || R * (N[0] / 4) > FFT_NT) return false;
       spectral(mv, vol, Msa, Ms_mul, s);
       const XiArgs<T> a = xinv_args(R);
       launch_xinv_field(a, f, n, R, s);
@@ -1564,22 +1666,35 @@ void pad_size(const int N[3], const int pbc[3], int P[3]) {
 
 void demag_kernel_mirror(const int P[3], double *const K[6], hipStream_t s);
 
+// kernelRanges, mag/demagkernel.go:339-352 (P: the padded size)
+void kernel_ranges(const int P[3], const int pbc[3], int r1[3], int r2[3]) {
+  for (int c = 0; c < 3; c++) {
+    if (pbc[c] == 0) {
+      r1[c] = -(P[c] - 1) / 2;
+      r2[c] = (P[c] - 1) / 2;
+    } else {
+      r1[c] = -(P[c] * pbc[c] - 1);
+      r2[c] = P[c] * pbc[c] - 1;
+    }
+  }
+  if (P[2] == 1 && pbc[2] == 0) r2[2] = 0;
+}
+
 // Brute-force tensor on the GPU into device arrays Kdev[6] of size P (FP64).
 void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3], double accuracy,
                             double *const Kdev[6], int P[3], hipStream_t s) {
   for (int i = 0; i < 3; i++)
-    UM_REQUIRE(pbc[i] == 0, UM_EUNSUP, "periodic demag tensor not supported yet");
+    UM_REQUIRE(pbc[i] >= 0 && pbc[i] <= (1 << 16), UM_EINVAL, "PBC repetitions out of range");
   UM_REQUIRE(cs[0] > 0 && cs[1] > 0 && cs[2] > 0 && accuracy > 0, UM_EINVAL, "bad cell size / accuracy");
   pad_size(N, pbc, P);
   BfArgs a{};
+  kernel_ranges(P, pbc, a.r1, a.r2);
   for (int i = 0; i < 3; i++) {
     a.P[i] = P[i];
-    a.r1[i] = -(P[i] - 1) / 2;
-    const int r2 = (P[i] - 1) / 2;
-    a.ext[i] = r2 - a.r1[i] + 1;
+    a.H[i] = P[i] / 2 + 1;
+    UM_REQUIRE((long)P[i] * pbc[i] < (1L << 30), UM_EINVAL, "PBC image range too large");
     a.cs[i] = cs[i];
   }
-  if (P[2] == 1) { a.r1[2] = 0; a.ext[2] = 1; }
   a.L = std::min(cs[0], std::min(cs[1], cs[2]));
   a.accuracy = accuracy;
   const size_t np = (size_t)P[0] * P[1] * P[2];
@@ -1587,7 +1702,7 @@ void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3]
     a.K[c] = Kdev[c];
     fill_value<double>(Kdev[c], 0.0, np, s);
   }
-  const long total = (long)a.ext[0] * a.ext[1] * a.ext[2] * 3;
+  const long total = (long)a.H[0] * a.H[1] * a.H[2] * 3;
   hipLaunchKernelGGL(k_demag_bf, dim3(blocks_for(total, 64, 1 << 16)), dim3(64), 0, s, a);
   UM_HIP(hipGetLastError());
   demag_kernel_mirror(P, Kdev, s);

@@ -354,7 +354,9 @@ __device__ inline double far_eval(const NewellFar &F, const double R[3]) {
 }
 
 struct NewellArgs {
-  int P[3], ext[3];
+  int P[3];
+  int r1[3], r2[3];      // image offsets per axis (kernelRanges)
+  int H[3];              // wrapped destinations per axis: 0..P/2
   double d[3];
   double scale;          // -1 / (4 pi V): Newell's sums are 4 pi V N, the kernel is -N
   double arad, arad2;    // far field where |R|^2 > arad2 (arad >= 0)
@@ -363,35 +365,51 @@ struct NewellArgs {
   double *K[6];
 };
 
+// One thread per wrapped destination element; its periodic images (the
+// offsets of the kernelRanges box that wrap onto it) are summed in FP64 in the
+// reference loop order, z outer, y, x inner.  Each image takes the asymptotic
+// expansion beyond the radius and the exact 27-term sums inside it
+// (:1161-1246).  The origin element is the self term alone: the reference
+// overwrites it after the image sums (:1300-1310).
 __global__ void __launch_bounds__(64) k_demag_newell(NewellArgs a) {
 #pragma clang fp contract(off)
-  const long total = (long)a.ext[0] * a.ext[1] * a.ext[2];
+  const long total = (long)a.H[0] * a.H[1] * a.H[2];
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int x = (int)(t % a.ext[0]);
-  const int y = (int)((t / a.ext[0]) % a.ext[1]);
-  const int z = (int)(t / ((long)a.ext[0] * a.ext[1]));
-  const long I = ((long)z * a.P[1] + y) * a.P[0] + x;
-  if (x == 0 && y == 0 && z == 0) {
+  const int xw = (int)(t % a.H[0]);
+  const int yw = (int)((t / a.H[0]) % a.H[1]);
+  const int zw = (int)(t / ((long)a.H[0] * a.H[1]));
+  const long I = ((long)zw * a.P[1] + yw) * a.P[0] + xw;
+  if (xw == 0 && yw == 0 && zw == 0) {
     a.K[0][I] = a.self[0]; a.K[1][I] = a.self[1]; a.K[2][I] = a.self[2];
     a.K[3][I] = 0.0; a.K[4][I] = 0.0; a.K[5][I] = 0.0;
     return;
   }
-  const double R[3] = {(double)x * a.d[0], (double)y * a.d[1], (double)z * a.d[2]};
   const double dx = a.d[0], dy = a.d[1], dz = a.d[2];
-  const bool far = a.arad >= 0.0 && R[0] * R[0] > a.arad2 - R[2] * R[2] - R[1] * R[1];
-  if (far) {
+  double S[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll 1
-    for (int c = 0; c < 6; c++) a.K[c][I] = -1.0 * far_eval(a.far[c], R);
-    return;
-  }
-  // CalculateSDA00/11/22 and 01/02/12 with their argument permutations
-  a.K[0][I] = second_diff27<false>(R[0], R[1], R[2], dx, dy, dz) * a.scale;
-  a.K[1][I] = second_diff27<false>(R[1], R[0], R[2], dy, dx, dz) * a.scale;
-  a.K[2][I] = second_diff27<false>(R[2], R[1], R[0], dz, dy, dx) * a.scale;
-  a.K[3][I] = (R[1] == 0.0 || R[2] == 0.0) ? 0.0 : second_diff27<true>(R[1], R[2], R[0], dy, dz, dx) * a.scale;
-  a.K[4][I] = (R[0] == 0.0 || R[2] == 0.0) ? 0.0 : second_diff27<true>(R[0], R[2], R[1], dx, dz, dy) * a.scale;
-  a.K[5][I] = (R[0] == 0.0 || R[1] == 0.0) ? 0.0 : second_diff27<true>(R[0], R[1], R[2], dx, dy, dz) * a.scale;
+  for (int z = image_first(zw, a.P[2], a.r1[2]); z <= a.r2[2]; z += a.P[2])
+#pragma unroll 1
+    for (int y = image_first(yw, a.P[1], a.r1[1]); y <= a.r2[1]; y += a.P[1])
+#pragma unroll 1
+      for (int x = image_first(xw, a.P[0], a.r1[0]); x <= a.r2[0]; x += a.P[0]) {
+        const double R[3] = {(double)x * dx, (double)y * dy, (double)z * dz};
+        const bool far = a.arad >= 0.0 && R[0] * R[0] > a.arad2 - R[2] * R[2] - R[1] * R[1];
+        if (far) {
+#pragma unroll 1
+          for (int c = 0; c < 6; c++) S[c] += -1.0 * far_eval(a.far[c], R);
+          continue;
+        }
+        // CalculateSDA00/11/22 and 01/02/12 with their argument permutations
+        S[0] += second_diff27<false>(R[0], R[1], R[2], dx, dy, dz) * a.scale;
+        S[1] += second_diff27<false>(R[1], R[0], R[2], dy, dx, dz) * a.scale;
+        S[2] += second_diff27<false>(R[2], R[1], R[0], dz, dy, dx) * a.scale;
+        S[3] += (R[1] == 0.0 || R[2] == 0.0) ? 0.0 : second_diff27<true>(R[1], R[2], R[0], dy, dz, dx) * a.scale;
+        S[4] += (R[0] == 0.0 || R[2] == 0.0) ? 0.0 : second_diff27<true>(R[0], R[2], R[1], dx, dz, dy) * a.scale;
+        S[5] += (R[0] == 0.0 || R[1] == 0.0) ? 0.0 : second_diff27<true>(R[0], R[1], R[2], dx, dy, dz) * a.scale;
+      }
+#pragma unroll
+  for (int c = 0; c < 6; c++) a.K[c][I] = S[c];
 }
 
 // ---- cell normalisation -------------------------------------------------------
@@ -455,18 +473,19 @@ void demag_kernel_mirror(const int P[3], double *const K[6], hipStream_t s);
 void demag_kernel_newell_device(const int N[3], const int pbc[3], const double cs[3], int asymptotic_radius,
                                 int zero_self_demag, double *const Kdev[6], int P[3], hipStream_t s) {
   for (int i = 0; i < 3; i++)
-    UM_REQUIRE(pbc[i] == 0, UM_EUNSUP, "periodic Newell demag tensor not supported yet");
+    UM_REQUIRE(pbc[i] >= 0 && pbc[i] <= (1 << 16), UM_EINVAL, "PBC repetitions out of range");
   UM_REQUIRE(cs[0] > 0 && cs[1] > 0 && cs[2] > 0, UM_EINVAL, "bad cell size");
   UM_REQUIRE(zero_self_demag >= 0, UM_EINVAL, "zero_self_demag must be >= 0");
   pad_size(N, pbc, P);
   NewellArgs a{};
+  kernel_ranges(P, pbc, a.r1, a.r2);
   for (int i = 0; i < 3; i++) {
     a.P[i] = P[i];
-    a.ext[i] = (P[i] - 1) / 2 + 1;   // offsets 0..(P-1)/2 (kernelRanges :339-352)
+    a.H[i] = P[i] / 2 + 1;
+    UM_REQUIRE((long)P[i] * pbc[i] < (1L << 30), UM_EINVAL, "PBC image range too large");
     a.K[i] = Kdev[i];
     a.K[i + 3] = Kdev[i + 3];
   }
-  if (P[2] == 1) a.ext[2] = 1;
   newell_cell(cs, a.d);
   const double V = a.d[0] * a.d[1] * a.d[2];
   a.scale = -1.0 / (4.0 * M_PI * V);
@@ -487,7 +506,7 @@ void demag_kernel_newell_device(const int N[3], const int pbc[3], const double c
 
   const size_t np = (size_t)P[0] * P[1] * P[2];
   for (int c = 0; c < 6; c++) fill_value<double>(Kdev[c], 0.0, np, s);
-  const long total = (long)a.ext[0] * a.ext[1] * a.ext[2];
+  const long total = (long)a.H[0] * a.H[1] * a.H[2];
   UM_REQUIRE(total < (1L << 31), UM_EINVAL, "Newell tensor too large");
   hipLaunchKernelGGL(k_demag_newell, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, s, a);
   UM_HIP(hipGetLastError());

@@ -274,8 +274,7 @@ struct Sim : SimBase {
     double *Kd[6];
     for (int c = 0; c < 6; c++) Kd[c] = K.as<double>() + c * np;
     int P2[3];
-    if (newell) demag_kernel_newell_device(N, pbc, cs, newell_arad, newell_zsd, Kd, P2, st);
-    else demag_kernel_bf_device(N, pbc, cs, demag_acc, Kd, P2, st);
+    demag_tensor(N, pbc, cs, newell, demag_acc, newell_arad, newell_zsd, Kd, P2, st);
     demag = demag_create(sizeof(T) == 4 ? UM_F32 : UM_F64, N, P2, Kd, st);
     UM_HIP(hipStreamSynchronize(st));
   }

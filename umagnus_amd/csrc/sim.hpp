@@ -14,6 +14,21 @@ void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3]
 void demag_kernel_newell_device(const int N[3], const int pbc[3], const double cs[3], int asymptotic_radius,
                                 int zero_self_demag, double *const Kdev[6], int P[3], hipStream_t s);
 void pad_size(const int N[3], const int pbc[3], int P[3]);
+// The tensor of a plan (mag.DemagKernel / NewellDemagKernel): from the OVF
+// cache when a cache directory is set, else computed on the GPU (kcache.hip)
+void demag_tensor(const int N[3], const int pbc[3], const double cs[3], int newell, double accuracy,
+                  int asymptotic_radius, int zero_self_demag, double *const Kdev[6], int P[3], hipStream_t s);
+void set_kernel_cache(const char *dir);
+std::string kernel_cache_basename(const std::string &dir, const int N[3], const int pbc[3], const double cs[3],
+                                  int newell, double accuracy, int zero_self_demag);
+void ovf_read(const std::string &fname, int size[3], int *ncomp, std::vector<double> &vals);
+void ovf_write_scalar(const std::string &fname, const int size[3], const double cs[3], const std::string &name,
+                      const double *K);
+// kernelRanges (mag/demagkernel.go:339-352): tensor offsets r1..r2 per axis
+void kernel_ranges(const int P[3], const int pbc[3], int r1[3], int r2[3]);
+// first offset >= r1 that wraps onto w (0 <= w < P, r1 <= 0); the periodic
+// images of w are image_first, +P, +2P, ... up to r2
+__host__ __device__ __forceinline__ int image_first(int w, int P, int r1) { return w - P * ((w - r1) / P); }
 void demag_exec(DemagBase *d, void *const B[3], const void *const m[3], const void *vol, const void *Ms,
                 double Ms_mul, hipStream_t s);
 // Demag + effective field + torque in one pipeline; fa: FieldArgs of the

@@ -142,9 +142,42 @@ def _hdr(out, key, *values):
     out.write(f"# {key}: " + " ".join(_fmt(v) for v in values) + "\n")
 
 
+def go_float(v):
+    """Go's %v of a float64 (strconv 'g', precision -1): the shortest decimal
+    that reads back to v, in exponent form (two exponent digits at least) when
+    the decimal exponent is below -4 or at least 6; integral values carry no
+    '.0'."""
+    import decimal
+    import math
+    if v != v:
+        return "NaN"
+    if math.isinf(v):
+        return "+Inf" if v > 0 else "-Inf"
+    if v == 0:
+        return "-0" if math.copysign(1.0, v) < 0 else "0"
+    sign, digits, e = decimal.Decimal(repr(v)).as_tuple()   # repr: shortest round-trip digits
+    digits = list(digits)
+    while len(digits) > 1 and digits[-1] == 0:
+        digits.pop()
+        e += 1
+    while len(digits) > 1 and digits[0] == 0:
+        digits.pop(0)
+    d = "".join(map(str, digits))
+    exp = len(d) - 1 + e
+    out = "-" if sign else ""
+    if exp < -4 or exp >= 6:
+        out += d[0] + ("." + d[1:] if len(d) > 1 else "") + ("e-%02d" % -exp if exp < 0 else "e+%02d" % exp)
+    elif exp >= 0:
+        ip = d[:exp + 1].ljust(exp + 1, "0")
+        out += ip + ("." + d[exp + 1:] if len(d) > exp + 1 else "")
+    else:
+        out += "0." + "0" * (-exp - 1) + d
+    return out
+
+
 def _fmt(v):
-    # shortest round-trip form, like Go's %v
-    return repr(v) if isinstance(v, float) else str(v)
+    # fmt.Fprintln's spelling of an operand
+    return go_float(v) if isinstance(v, float) else str(v)
 
 
 def write(path, data, cellsize, name="m", unit="1", time=0.0, fmt="binary 4", ovf=2):

@@ -39,7 +39,7 @@ EXPORTS = [
     "um_demag_create_newell", "um_demag_kernel_newell", "um_sim_set_demag_newell",
     "um_sim_tape_begin", "um_sim_tape_end", "um_sim_tape_steps", "um_sim_backward",
     "um_add_dmi_bulk", "um_add_zhangli", "um_add_slonczewski", "um_sim_set_dmi_bulk", "um_sim_set_stt_flags",
-    "um_sim_minimize",
+    "um_sim_minimize", "um_set_kernel_cache", "um_kernel_cache_basename", "um_load_kernel", "um_save_kernel",
     "um_sim_create", "um_sim_destroy", "um_sim_set_scalar", "um_sim_set_vector", "um_sim_set_regions",
     "um_sim_set_exchange_luts", "um_sim_set_bext_lut", "um_sim_set_frozen_lut", "um_sim_set_geometry",
     "um_sim_set_flags", "um_sim_set_demag_accuracy", "um_sim_set_thermal_seed", "um_sim_set_m",
@@ -386,6 +386,35 @@ def demag_kernel_newell(N, cellsize, pbc=(0, 0, 0), asymptotic_radius=32, zero_s
     _call("um_demag_kernel_newell", N[0], N[1], N[2], (C.c_int * 3)(*pbc), (C.c_double * 3)(*cellsize),
           int(asymptotic_radius), int(zero_self_demag), arr)
     return K
+
+
+def set_kernel_cache(directory):
+    """The -cache flag (engine/gofiles.go:17): demag plans read and write their
+    tensor as OVF files in `directory` (None or "" disables caching)."""
+    _call("um_set_kernel_cache", (directory or "").encode())
+
+
+def kernel_cache_basename(directory, N, cellsize, pbc=(0, 0, 0), newell=False, accuracy=6.0, zero_self_demag=0):
+    """File prefix of a cache entry (mag/demagkernel.go:40, newellDemagKernel.go:1023)."""
+    buf = C.create_string_buffer(4096)
+    _call("um_kernel_cache_basename", directory.encode(), (C.c_int * 3)(*N), (C.c_int * 3)(*pbc),
+          (C.c_double * 3)(*cellsize), int(newell), C.c_double(accuracy), int(zero_self_demag), buf, 4096)
+    return buf.value.decode()
+
+
+def load_kernel(fname, P):
+    """LoadKernel (mag/demagkernel.go:97): scalar OVF field of size P (x, y, z) -> (Pz, Py, Px) float64."""
+    K = np.zeros((P[2], P[1], P[0]), np.float64)
+    _call("um_load_kernel", str(fname).encode(), (C.c_int * 3)(*P), K.ctypes.data_as(C.c_void_p))
+    return K
+
+
+def save_kernel(fname, K, cellsize, name):
+    """SaveKernel (mag/demagkernel.go:102): (Pz, Py, Px) array as an OVF2 binary 4 scalar field."""
+    K = np.ascontiguousarray(K, np.float64)
+    P = (K.shape[2], K.shape[1], K.shape[0])
+    _call("um_save_kernel", str(fname).encode(), (C.c_int * 3)(*P), (C.c_double * 3)(*cellsize),
+          name.encode(), K.ctypes.data_as(C.c_void_p))
 
 
 # ---- simulation handle (engine/) -------------------------------------------------------
