@@ -92,17 +92,23 @@ def configure(sim, mod, dtype):
     sim.set_solver_params(dt=1e-14, fixdt=1e-14)
 
 
-def ensemble_members(count=64, seed=4):
-    """Grid sizes of a config-4-like ensemble: (Nx, Ny) from {32..128}, Nz 1..4."""
+ENSEMBLE_SIZES = (32, 48, 64, 96, 128, 192, 256)
+
+
+def ensemble_members(count=256, seed=4):
+    """Grid sizes of the ensemble: Nx, Ny drawn from {32, 48, 64, 96, 128, 192,
+    256}, Nz from 1..4."""
     rng = np.random.default_rng(seed)
-    return [(int(rng.choice([32, 48, 64, 96, 128])), int(rng.choice([32, 48, 64, 96, 128])),
-             int(rng.integers(1, 5))) for _ in range(count)]
+    return [(int(rng.choice(ENSEMBLE_SIZES)), int(rng.choice(ENSEMBLE_SIZES)), int(rng.integers(1, 5)))
+            for _ in range(count)]
 
 
-def batched(steps=100):
-    """Ensemble throughput: Heun at FixDt 5e-15 s with T = 300 K (one Philox
-    stream per member), 64 members stepped one after another (eager, the
-    reference's queue order) vs together through graph replays (um_sims_steps)."""
+def batched(steps=40):
+    """Ensemble throughput: 256 members, Heun at FixDt 5e-15 s with T = 300 K
+    (one Philox stream per member), stepped one after another (eager, the
+    reference's queue order) vs together through graph replays (um_sims_steps),
+    the latter also with autotuned launch schedules.  A bounded number of
+    steps per member (the ensemble's 20000 steps repeat the same step)."""
     import torch
     from umagnus_amd import umhip
     from umagnus_amd.params import exchange_lut
@@ -127,11 +133,16 @@ def batched(steps=100):
         return sims
 
     res = {}
-    for mode in ("plain", "batched"):
-        sims = make(mode == "batched")
+    for mode in ("plain", "batched", "tuned"):
+        if mode == "tuned":
+            umhip.set_autotune(1)
+        try:
+            sims = make(mode != "plain")
+        finally:
+            umhip.set_autotune(0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        if mode == "batched":
+        if mode != "plain":
             umhip.sims_steps(sims, steps)
         else:
             for s in sims:
@@ -140,10 +151,49 @@ def batched(steps=100):
         res[mode] = cells * steps / (time.perf_counter() - t0) / 1e6
         del sims
         torch.cuda.empty_cache()
-    return {"workload": f"{len(sizes)} instances, {cells} cells in total, (Nx, Ny) in {{32..128}}, Nz 1..4, "
-                        "2 nm, Heun FixDt 5e-15 s, T = 300 K, FP32",
+    return {"workload": f"{len(sizes)} instances, {cells} cells in total, Nx, Ny in {set(ENSEMBLE_SIZES)}, "
+                        "Nz 1..4, 2 nm, Heun FixDt 5e-15 s, T = 300 K, FP32",
             "steps": steps, "unit": "Mcell·steps/s", "plain": round(res["plain"], 2),
-            "value": round(res["batched"], 2), "speedup": round(res["batched"] / res["plain"], 3)}
+            "value": round(res["batched"], 2), "speedup": round(res["batched"] / res["plain"], 3),
+            "tuned": round(res["tuned"], 2), "tuned_speedup": round(res["tuned"] / res["plain"], 3)}
+
+
+def autotune(shapes=((512, 512, 32), (192, 96, 3), (96, 96, 2), (256, 256, 1), (48, 192, 4)), reps=20):
+    """Demag convolution time per call with the static launch tables vs the
+    autotuned schedule (FP32), per shape."""
+    import torch
+    from umagnus_amd import umhip
+    rows = []
+    for N in shapes:
+        n = N[0] * N[1] * N[2]
+        m = [torch.rand(N[2], N[1], N[0], device="cuda") for _ in range(3)]
+        B = [torch.empty_like(m[0]) for _ in range(3)]
+        Ms = torch.full_like(m[0], 8e5)
+        row = {"N": list(N)}
+        for mode in (0, 1):
+            umhip.set_autotune(mode)
+            try:
+                conv = umhip.DemagConvolution(N, (2e-9,) * 3, precision=umhip.F32)
+            finally:
+                umhip.set_autotune(0)
+            conv.Exec(B, m, None, Ms)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                conv.Exec(B, m, None, Ms)
+            e1.record()
+            torch.cuda.synchronize()
+            key = "static" if mode == 0 else "tuned"
+            row[key + "_us"] = round(e0.elapsed_time(e1) / reps * 1e3, 2)
+            row[key + "_schedule"] = conv.schedule()
+            conv.Free()
+        row["speedup"] = round(row["static_us"] / row["tuned_us"], 3)
+        rows.append(row)
+        del m, B, Ms
+        torch.cuda.empty_cache()
+    return {"what": "demag convolution per call, FP32; schedule = nl_x nl_xi four_step nl_y B_conv "
+                    "sched_x sched_y sched_z", "shapes": rows}
 
 
 def adjoint_workload(seed=6):
@@ -312,6 +362,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -391,6 +442,8 @@ def main():
         out["adjoint"] = adjoint()
     if not args.no_precision:
         out["precision"] = precision()
+    if not args.no_autotune:
+        out["autotune"] = autotune()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, cs)
     print(json.dumps(out), flush=True)

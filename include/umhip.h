@@ -180,6 +180,19 @@ int um_demag_kernel_bf(int Nx, int Ny, int Nz, const int pbc[3], const double cs
 int um_demag_kernel_newell(int Nx, int Ny, int Nz, const int pbc[3], const double cs[3],
                            int asymptotic_radius, int zero_self_demag, double *const K[6]);
 
+/* ---- autotuner (no counterpart in the reference, whose VkFFT plans are fixed) ----
+ * Launch schedules of the demag passes picked per shape by timing: rows per
+ * work-group of the X passes, the four-step X inverse, lines of the generic
+ * Y pass, columns of the convolution pass, and the pass order of the generic
+ * Stockham engine (radix schedule).  mode 0: static tables (default); 1: time
+ * every new shape when its plan is created (results kept per precision and
+ * size until the next call); 2: apply known results only.  file (or NULL):
+ * the known results are read from it now and new ones appended, so later
+ * processes reuse them. */
+int um_set_autotune(int mode, const char *file);
+/* the schedule of a plan: nl_x nl_xi four_step nl_y B_conv sched_x sched_y sched_z */
+int um_demag_schedule(void *plan, int out[8]);
+
 /* ---- demag tensor cache: mag/demagkernel.go:17-109, newellDemagKernel.go:1006-1106 ----
  * The -cache flag (engine/gofiles.go:17): with a directory set, um_demag_create,
  * um_demag_create_newell and the simulation handle read the tensor from

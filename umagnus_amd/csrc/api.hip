@@ -511,6 +511,18 @@ int um_demag_create_newell(int prec, int Nx, int Ny, int Nz, const int pbc[3], c
     *plan = demag_create(prec, N, P, Kd, s);
   });
 }
+int um_set_autotune(int mode, const char *file) {
+  return guard([&] {
+    UM_REQUIRE(mode >= 0 && mode <= 2, UM_EINVAL, "um_set_autotune: mode must be 0, 1 or 2");
+    set_autotune(mode, file);
+  });
+}
+int um_demag_schedule(void *plan, int out[8]) {
+  return guard([&] {
+    UM_REQUIRE(plan && out, UM_EINVAL, "um_demag_schedule: bad arguments");
+    demag_schedule((DemagBase *)plan, out);
+  });
+}
 int um_set_kernel_cache(const char *dir) { return guard([&] { set_kernel_cache(dir); }); }
 int um_kernel_cache_basename(const char *dir, const int N[3], const int pbc[3], const double cs[3], int newell,
                              double accuracy, int zero_self_demag, char *out, int out_len) {

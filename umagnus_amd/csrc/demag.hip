@@ -15,8 +15,10 @@
 // zero padding) -> Y-fwd -> Z (fwd FFT, 6-component symmetric tensor
 // multiply, inverse FFT, all in LDS) -> Y-inv -> X-inv C2R (writes the
 // unpadded field).  2D (Nz==1): X-fwd -> Y (fwd, 4-component multiply, inv) -> X-inv.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <type_traits>
 
@@ -1041,20 +1043,30 @@ __global__ void k_extract_kspec(const double2 *__restrict__ spec, T *__restrict_
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-std::vector<int> fft_radices(int n) {
-  std::vector<int> r;
+// Radix schedule of the run-time (generic) Stockham engine: the order and
+// size of the passes.  sched 0: odd radices first, then the power of two in
+// balanced passes of at most 16; 1: the power-of-two passes first; 2 and 3:
+// odd first, power-of-two passes of at most 8 / 4 (more passes, fewer values
+// per thread).  The compiled fixed-length kernels have their own schedules.
+constexpr int FFT_NSCHED = 4;
+std::vector<int> fft_radices(int n, int sched) {
+  std::vector<int> odd;
   int m = n;
   for (int p : {7, 5, 3}) {
-    while (m % p == 0) { r.push_back(p); m /= p; }
+    while (m % p == 0) { odd.push_back(p); m /= p; }
   }
   int e2 = 0;
   while (m % 2 == 0) { m /= 2; e2++; }
   if (m != 1) throw Error(UM_EUNSUP, "FFT length " + std::to_string(n) + " has a prime factor above 7");
+  const int maxlog = sched == 2 ? 3 : sched == 3 ? 2 : 4;
+  std::vector<int> two;
   if (e2 > 0) {
-    const int np = (e2 + 3) / 4;
+    const int np = (e2 + maxlog - 1) / maxlog;
     const int base = e2 / np, rem = e2 % np;
-    for (int i = 0; i < np; i++) r.push_back(1 << (base + (i < rem ? 1 : 0)));
+    for (int i = 0; i < np; i++) two.push_back(1 << (base + (i < rem ? 1 : 0)));
   }
+  std::vector<int> r = sched == 1 ? two : odd;
+  for (int v : (sched == 1 ? odd : two)) r.push_back(v);
   if (r.empty()) r.push_back(1);
   return r;
 }
@@ -1066,12 +1078,14 @@ template <typename T>
 struct TwTable {
   DevBuf buf;
   FftLen len;
-  void make(int n, hipStream_t s) {
-    std::vector<int> r = (n > 1) ? fft_radices(n) : std::vector<int>{};
+  int sched = 0;
+  // pass order of the generic engine (the twiddle table does not depend on it)
+  void schedule(int sc) {
+    const int n = len.n;
+    std::vector<int> r = (n > 1) ? fft_radices(n, sc) : std::vector<int>{};
     UM_REQUIRE((int)r.size() <= FFT_MAXPASS, UM_EUNSUP, "FFT length too long");
-    len.n = n;
+    sched = sc;
     len.np = (n > 1) ? (int)r.size() : 0;
-    len.fn.init(n > 0 ? n : 1);
     int pp = 1;
     for (int i = 0; i < len.np; i++) {
       len.rad[i] = r[i];
@@ -1079,6 +1093,11 @@ struct TwTable {
       len.fnr[i].init(n / r[i]);
       pp *= r[i];
     }
+  }
+  void make(int n, hipStream_t s) {
+    len.n = n;
+    len.fn.init(n > 0 ? n : 1);
+    schedule(0);
     std::vector<C2<T>> h(n > 0 ? n : 1);
     for (int m = 0; m < n; m++) {
       const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
@@ -1170,8 +1189,8 @@ static void launch_xfwd(const XArgs<T> &a, int n, int nl, dim3 grid, hipStream_t
   UM_HIP(hipGetLastError());
 }
 template <typename T>
-static void launch_xinv(const XiArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s) {
-  if (launch_xi4<T>(a, n, s)) return;
+static void launch_xinv(const XiArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s, bool four_step = true) {
+  if (four_step && launch_xi4<T>(a, n, s)) return;
   const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + 2 * n);
   if constexpr (std::is_same<T, float>::value) {
     UM_CT_CASE((k_xinv<float, 64, 32>), 64, 32, ct)
@@ -1400,7 +1419,67 @@ struct DemagBase {
   virtual void kspec(int c, void *host) = 0;
   virtual bool exec_field(const void *const m[3], const void *vol, const void *Ms, double Ms_mul, const void *fa,
                           hipStream_t s) = 0;
+  virtual void schedule(int out[DEMAG_NKNOB]) const = 0;
+  virtual void tune(hipStream_t s) = 0;
 };
+
+// ---- autotuner ---------------------------------------------------------------
+// Launch schedules of a plan are picked per shape by timing the candidates of
+// each pass on the plan's own buffers (HIP events, best of a few launches),
+// pass by pass in pipeline order.  Results are kept per (precision, N, P)
+// until the next set_autotune call and, with a tuning file, appended to it and
+// read back by later calls and processes.  Off by default: the static tables are then used, and results do
+// not depend on timing noise.  Mode 2 applies known results (from the
+// file) without timing anything.
+static std::mutex g_tune_mu;
+static int g_tune_mode = 0;
+static std::string g_tune_file;
+static std::map<std::string, std::vector<int>> g_tune_memo;
+
+void set_autotune(int mode, const char *file) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_mode = mode;
+  g_tune_file = file ? file : "";
+  g_tune_memo.clear();   // the known results are those of the file
+  if (g_tune_file.empty()) return;
+  FILE *f = std::fopen(g_tune_file.c_str(), "r");
+  if (!f) return;
+  char line[512];
+  while (std::fgets(line, sizeof line, f)) {
+    char *colon = std::strchr(line, ':');
+    if (!colon) continue;
+    std::vector<int> v;
+    char *q = colon + 1;
+    for (;;) {
+      char *end = nullptr;
+      const long x = std::strtol(q, &end, 10);
+      if (end == q) break;
+      v.push_back((int)x);
+      q = end;
+    }
+    if ((int)v.size() == DEMAG_NKNOB) g_tune_memo[std::string(line, colon)] = v;
+  }
+  std::fclose(f);
+}
+static bool tune_lookup(const std::string &key, std::vector<int> &v, int *mode) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  *mode = g_tune_mode;
+  auto it = g_tune_memo.find(key);
+  if (it == g_tune_memo.end()) return false;
+  v = it->second;
+  return true;
+}
+static void tune_store(const std::string &key, const std::vector<int> &v) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_memo[key] = v;
+  if (g_tune_file.empty()) return;
+  if (FILE *f = std::fopen(g_tune_file.c_str(), "a")) {
+    std::fprintf(f, "%s:", key.c_str());
+    for (int x : v) std::fprintf(f, " %d", x);
+    std::fprintf(f, "\n");
+    std::fclose(f);
+  }
+}
 
 template <typename T>
 struct DemagPlan : DemagBase {
@@ -1408,7 +1487,11 @@ struct DemagPlan : DemagBase {
   TwTable<T> tx, ty, tz, tx2;   // tx: length Px/2, tx2: length Px (post-processing), ty, tz
   DevBuf spec[3];               // per component: Nz*Py*Hp complex
   DevBuf kern;                  // 6 x H*KHy*KHz of T
-  int nl_x = 1, nl_y = 1, B_conv = 1;
+  // launch schedule (static tables, or the autotuner's choice): X forward and
+  // inverse rows per work-group, four-step X inverse on/off, generic Y
+  // lines, convolution columns, and the generic engine's radix schedules
+  int nl_x = 1, nl_xi = 1, nl_y = 1, B_conv = 1;
+  bool xi4 = true;
   bool zreg = false;            // register Z pass (Pz = 2 Nz, Nz in ConvZTable)
   bool xodd = false;            // odd Px (periodic along x, odd Nx): full-length X passes
   long kcount = 0;
@@ -1445,6 +1528,7 @@ struct DemagPlan : DemagBase {
         nl_x = nl;
       }
     }
+    nl_xi = nl_x;
     nl_y = CtTable<T>::col(P[1]);
     if (!nl_y) nl_y = std::min(lines_for(P[1]), 64);
 
@@ -1543,61 +1627,76 @@ struct DemagPlan : DemagBase {
 
   // X-fwd through Y-inv (3D) or the 2D Y pass: leaves the convolved half
   // spectrum (rows y < Ny) in spec[] for the X inverse.
-  void spectral(const void *const mv[3], const void *vol, const void *Msa, double Ms_mul, hipStream_t s) {
+  void pass_xfwd(const void *const mv[3], const void *vol, const void *Msa, double Ms_mul, hipStream_t s) {
     const int Nx = N[0], Ny = N[1], Nz = N[2];
-    C2<T> *sp[3] = {spec[0].as<C2<T>>(), spec[1].as<C2<T>>(), spec[2].as<C2<T>>()};
-    // X forward
-    {
-      XArgs<T> a{};
-      a.f = tx.len; a.tw2 = (const C2<T> *)tx2.buf.p; a.mode = 0; a.nl = nl_x; a.fhalf.init(P[0] / 4 + 1);
-      a.Nx = Nx; a.rows = Ny * Nz; a.fRy.init(Ny); a.Py = is2d ? Ny : P[1]; a.Hp = Hp; a.in_pitch = Nx;
-      for (int c = 0; c < 3; c++) { a.src[c] = (const T *)mv[c]; a.out[c] = sp[c]; }
-      a.Ms = Par<T>{(const T *)Msa, (T)Ms_mul}; a.vol = (const T *)vol;
-      dim3 g((a.rows + nl_x - 1) / nl_x, 3);
-      if (xodd) {
-        a.f = tx2.len; a.fhalf.init(H);
-        hipLaunchKernelGGL(k_xfwd_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
-        UM_HIP(hipGetLastError());
-      } else {
-        launch_xfwd<T>(a, P[0] / 2, nl_x, g, s);
-      }
+    XArgs<T> a{};
+    a.f = tx.len; a.tw2 = (const C2<T> *)tx2.buf.p; a.mode = 0; a.nl = nl_x; a.fhalf.init(P[0] / 4 + 1);
+    a.Nx = Nx; a.rows = Ny * Nz; a.fRy.init(Ny); a.Py = is2d ? Ny : P[1]; a.Hp = Hp; a.in_pitch = Nx;
+    for (int c = 0; c < 3; c++) { a.src[c] = (const T *)mv[c]; a.out[c] = spec[c].as<C2<T>>(); }
+    a.Ms = Par<T>{(const T *)Msa, (T)Ms_mul}; a.vol = (const T *)vol;
+    dim3 g((a.rows + nl_x - 1) / nl_x, 3);
+    if (xodd) {
+      a.f = tx2.len; a.fhalf.init(H);
+      hipLaunchKernelGGL(k_xfwd_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
+      UM_HIP(hipGetLastError());
+    } else {
+      launch_xfwd<T>(a, P[0] / 2, nl_x, g, s);
     }
+  }
+  // Y forward (dir < 0) or inverse (dir > 0), 3D only
+  void pass_y(int dir, hipStream_t s) {
+    ColArgs<T> ya{};
+    ya.f = ty.len; ya.nl = nl_y; ya.fnl.init(nl_y); ya.H = H; ya.estride = Hp; ya.pstride = (long)P[1] * Hp;
+    for (int c = 0; c < 3; c++) ya.buf[c] = spec[c].as<C2<T>>();
+    dim3 gy((H + nl_y - 1) / nl_y, N[2], 3);
+    if (dir < 0) {
+      ya.nvalid = N[1]; ya.nout = P[1];
+      if (!launch_coly<T, -1>(ya, P[1], N[2], s)) launch_col<T, -1>(ya, P[1], nl_y, gy, s);
+    } else {
+      ya.nvalid = P[1]; ya.nout = N[1];
+      if (!launch_coly<T, +1>(ya, P[1], N[2], s)) launch_col<T, +1>(ya, P[1], nl_y, gy, s);
+    }
+  }
+  // the convolution pass: Z (3D) or Y (2D) forward, tensor multiply, inverse
+  void pass_conv(hipStream_t s) {
     const T *K[6];
     for (int c = 0; c < 6; c++) K[c] = kern.as<T>() + c * kcount;
-    if (!is2d) {
-      ColArgs<T> ya{};
-      ya.f = ty.len; ya.nl = nl_y; ya.fnl.init(nl_y); ya.H = H; ya.estride = Hp; ya.pstride = (long)P[1] * Hp;
-      for (int c = 0; c < 3; c++) ya.buf[c] = sp[c];
-      dim3 gy((H + nl_y - 1) / nl_y, Nz, 3);
-      ya.nvalid = Ny; ya.nout = P[1];
-      if (!launch_coly<T, -1>(ya, P[1], Nz, s)) launch_col<T, -1>(ya, P[1], nl_y, gy, s);
-      if (zreg) {
-        ConvZArgs<T> za{};
-        za.H = H; za.Py = P[1]; za.KHy = KHy; za.zstride = (long)P[1] * Hp; za.ystride = Hp;
-        for (int c = 0; c < 6; c++) za.K[c] = K[c];
-        for (int c = 0; c < 3; c++) za.buf[c] = sp[c];
-        launch_convz<T>(za, Nz, s);
-      } else {
-        ConvArgs<T> ca{};
-        ca.f = tz.len; ca.B = B_conv; ca.fB.init(B_conv); ca.f3B.init(3 * B_conv); ca.H = H; ca.nvalid = Nz;
-        ca.estride = (long)P[1] * Hp; ca.pstride = Hp;
-        ca.is2d = 0; ca.Py = P[1]; ca.Pz = P[2]; ca.KHy = KHy;
-        for (int c = 0; c < 6; c++) ca.K[c] = K[c];
-        for (int c = 0; c < 3; c++) ca.buf[c] = sp[c];
-        dim3 gc((H + B_conv - 1) / B_conv, P[1] / 2 + 1);   // ky and Py-ky per work-group
-        launch_conv<T>(ca, P[2], B_conv, gc, s);
-      }
-      ya.nvalid = P[1]; ya.nout = Ny;
-      if (!launch_coly<T, +1>(ya, P[1], Nz, s)) launch_col<T, +1>(ya, P[1], nl_y, gy, s);
+    if (!is2d && zreg) {
+      ConvZArgs<T> za{};
+      za.H = H; za.Py = P[1]; za.KHy = KHy; za.zstride = (long)P[1] * Hp; za.ystride = Hp;
+      for (int c = 0; c < 6; c++) za.K[c] = K[c];
+      for (int c = 0; c < 3; c++) za.buf[c] = spec[c].as<C2<T>>();
+      launch_convz<T>(za, N[2], s);
+      return;
+    }
+    ConvArgs<T> ca{};
+    ca.f = is2d ? ty.len : tz.len; ca.B = B_conv; ca.fB.init(B_conv); ca.f3B.init(3 * B_conv); ca.H = H;
+    ca.nvalid = is2d ? N[1] : N[2];
+    ca.estride = is2d ? Hp : (long)P[1] * Hp; ca.pstride = is2d ? 0 : Hp;
+    ca.is2d = is2d; ca.Py = P[1]; ca.Pz = is2d ? 1 : P[2]; ca.KHy = KHy;
+    for (int c = 0; c < 6; c++) ca.K[c] = K[c];
+    for (int c = 0; c < 3; c++) ca.buf[c] = spec[c].as<C2<T>>();
+    dim3 gc((H + B_conv - 1) / B_conv, is2d ? 1 : P[1] / 2 + 1);   // 3D: ky and Py-ky per work-group
+    launch_conv<T>(ca, is2d ? P[1] : P[2], B_conv, gc, s);
+  }
+  // X forward through Y inverse (3D) or the 2D Y pass: leaves the convolved
+  // half spectrum (rows y < Ny) in spec[] for the X inverse.
+  void spectral(const void *const mv[3], const void *vol, const void *Msa, double Ms_mul, hipStream_t s) {
+    pass_xfwd(mv, vol, Msa, Ms_mul, s);
+    if (!is2d) pass_y(-1, s);
+    pass_conv(s);
+    if (!is2d) pass_y(+1, s);
+  }
+  void pass_xinv(void *const Bv[3], hipStream_t s) {
+    XiArgs<T> a = xinv_args(nl_xi);
+    for (int c = 0; c < 3; c++) a.dst[c] = (T *)Bv[c];
+    dim3 g((a.rows + nl_xi - 1) / nl_xi, 3);
+    if (xodd) {
+      a.f = tx2.len; a.fhalf.init(H);
+      hipLaunchKernelGGL(k_xinv_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_xi * P[0], s, a);
+      UM_HIP(hipGetLastError());
     } else {
-      ConvArgs<T> ca{};
-      ca.f = ty.len; ca.B = B_conv; ca.fB.init(B_conv); ca.f3B.init(3 * B_conv); ca.H = H; ca.nvalid = Ny;
-      ca.estride = Hp; ca.pstride = 0;
-      ca.is2d = 1; ca.Py = P[1]; ca.Pz = 1; ca.KHy = KHy;
-      for (int c = 0; c < 6; c++) ca.K[c] = K[c];
-      for (int c = 0; c < 3; c++) ca.buf[c] = sp[c];
-      dim3 gc((H + B_conv - 1) / B_conv, 1);
-      launch_conv<T>(ca, P[1], B_conv, gc, s);
+      launch_xinv<T>(a, P[0] / 2, nl_xi, g, s, xi4);
     }
   }
 
@@ -1612,16 +1711,7 @@ struct DemagPlan : DemagBase {
   void exec(void *const Bv[3], const void *const mv[3], const void *vol, const void *Msa,
             double Ms_mul, hipStream_t s) override {
     spectral(mv, vol, Msa, Ms_mul, s);
-    XiArgs<T> a = xinv_args(nl_x);
-    for (int c = 0; c < 3; c++) a.dst[c] = (T *)Bv[c];
-    dim3 g((a.rows + nl_x - 1) / nl_x, 3);
-    if (xodd) {
-      a.f = tx2.len; a.fhalf.init(H);
-      hipLaunchKernelGGL(k_xinv_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
-      UM_HIP(hipGetLastError());
-    } else {
-      launch_xinv<T>(a, P[0] / 2, nl_x, g, s);
-    }
+    pass_xinv(Bv, s);
   }
 
   // Demag convolution fused with the effective field and torque: the X
@@ -1647,6 +1737,182 @@ struct DemagPlan : DemagBase {
       launch_xinv_field(a, f, n, R, s);
       return true;
     }
+  }
+
+  // ---- schedule knobs: nl_x nl_xi xi4 nl_y B_conv sched(x) sched(y) sched(z)
+  TwTable<T> &xtab() { return xodd ? tx2 : tx; }
+  void schedule(int out[DEMAG_NKNOB]) const override {
+    const int v[DEMAG_NKNOB] = {nl_x, nl_xi, xi4 ? 1 : 0, nl_y, B_conv, xodd ? tx2.sched : tx.sched, ty.sched, tz.sched};
+    for (int i = 0; i < DEMAG_NKNOB; i++) out[i] = v[i];
+  }
+  void apply(const std::vector<int> &v) {
+    const int nx = xodd ? P[0] : P[0] / 2;
+    bool ok = (int)v.size() == DEMAG_NKNOB && v[0] >= 1 && v[0] <= lines_for(nx) && v[1] >= 1 &&
+              v[1] <= lines_for(nx) && v[3] >= 1 && v[3] <= std::min(lines_for(P[1]), 64) && conv_ok(v[4]);
+    for (int i = 5; ok && i < DEMAG_NKNOB; i++) ok = v[i] >= 0 && v[i] < FFT_NSCHED;
+    UM_REQUIRE(ok, UM_EINVAL, "invalid demag launch schedule");
+    nl_x = v[0]; nl_xi = v[1]; xi4 = v[2] != 0; nl_y = v[3]; B_conv = v[4];
+    xtab().schedule(v[5]); ty.schedule(v[6]); tz.schedule(v[7]);
+  }
+  std::string tune_key() const {
+    char b[160];
+    std::snprintf(b, sizeof b, "%s N=%d,%d,%d P=%d,%d,%d", sizeof(T) == 4 ? "f32" : "f64", N[0], N[1], N[2], P[0],
+                  P[1], P[2]);
+    return b;
+  }
+  static std::vector<int> halvings(int L) {
+    std::vector<int> r;
+    for (int v = L; v >= 1; v /= 2) r.push_back(v);
+    if (r.empty() || r.back() != 1) r.push_back(1);
+    return r;
+  }
+  static void add(std::vector<int> &v, int x) {
+    if (x >= 1 && std::find(v.begin(), v.end(), x) == v.end()) v.push_back(x);
+  }
+  bool conv_ok(int B) const {
+    const int convlen = is2d ? P[1] : P[2];
+    return B >= 1 && B * (convlen / 2 + 1) <= FFT_NT * CONV_QK && 3 * B * convlen <= FFT_NT * FftEpt<T>::value;
+  }
+
+  void tune(hipStream_t s) override {
+    const std::string key = tune_key();
+    std::vector<int> v;
+    int mode = 0;
+    const bool known = tune_lookup(key, v, &mode);
+    if (mode == 0) return;   // static tables
+    if (known) {
+      std::vector<int> keep(DEMAG_NKNOB);
+      schedule(keep.data());
+      try { apply(v); return; } catch (const Error &) { apply(keep); }
+    }
+    if (mode != 1) return;   // mode 2: known results only, never timed
+    const size_t n = (size_t)N[0] * N[1] * N[2];
+    DevBuf mb(sizeof(T) * 3 * n), bb(sizeof(T) * 3 * n);
+    fill_value<T>(mb.as<T>(), T(0), 3 * n, s);
+    const void *mv[3] = {mb.as<T>(), mb.as<T>() + n, mb.as<T>() + 2 * n};
+    void *bv[3] = {bb.as<T>(), bb.as<T>() + n, bb.as<T>() + 2 * n};
+    hipEvent_t e0, e1;
+    UM_HIP(hipEventCreate(&e0));
+    UM_HIP(hipEventCreate(&e1));
+    // best of 3 timed launches after one warm-up; a candidate that fails to
+    // launch (LDS limits) is skipped
+    auto timed = [&](auto &&run) -> float {
+      try {
+        run();
+        float best = 1e30f;
+        for (int r = 0; r < 3; r++) {
+          UM_HIP(hipEventRecord(e0, s));
+          run();
+          UM_HIP(hipEventRecord(e1, s));
+          UM_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          UM_HIP(hipEventElapsedTime(&ms, e0, e1));
+          best = std::min(best, ms);
+        }
+        return best;
+      } catch (const Error &) {
+        (void)hipGetLastError();
+        return 1e30f;
+      }
+    };
+    try {
+      spectral(mv, nullptr, nullptr, 1.0, s);   // valid spectra in the buffers
+      // 1. X forward: rows per work-group x radix schedule
+      {
+        std::vector<int> nls;
+        const int nx = xodd ? P[0] : P[0] / 2;
+        add(nls, nl_x);
+        if (!xodd && std::is_same<T, float>::value && CtTable<T>::x(nx)) {
+          add(nls, CtTable<T>::x(nx));
+          add(nls, CtTable<T>::x(nx) / 4);
+          add(nls, 1);
+        }
+        for (int x : halvings(lines_for(nx))) add(nls, x);
+        float best = 1e30f;
+        int bn = nl_x, bs = xtab().sched;
+        for (int nl : nls)
+          for (int sc = 0; sc < FFT_NSCHED; sc++) {
+            nl_x = nl;
+            try { xtab().schedule(sc); } catch (const Error &) { continue; }   // too many passes
+            const float t = timed([&] { pass_xfwd(mv, nullptr, nullptr, 1.0, s); });
+            if (t < best) { best = t; bn = nl; bs = sc; }
+          }
+        nl_x = bn;
+        xtab().schedule(bs);
+      }
+      // 2. X inverse: four-step kernel or the Stockham one with rows per work-group
+      {
+        const int nx = xodd ? P[0] : P[0] / 2;
+        std::vector<int> nls;
+        add(nls, nl_xi);
+        if (!xodd && CtTable<T>::x(nx)) add(nls, CtTable<T>::x(nx));
+        for (int x : halvings(lines_for(nx))) add(nls, x);
+        float best = 1e30f;
+        int bn = nl_xi;
+        bool b4 = xi4;
+        for (int four : {1, 0})
+          for (int nl : nls) {
+            if (four && nl != nls[0]) continue;   // the four-step kernel has its own rows
+            xi4 = four != 0;
+            nl_xi = nl;
+            const float t = timed([&] { pass_xinv(bv, s); });
+            if (t < best) { best = t; bn = nl; b4 = xi4; }
+          }
+        nl_xi = bn;
+        xi4 = b4;
+      }
+      // 3. Y passes (3D): generic lines per work-group x radix schedule
+      if (!is2d) {
+        std::vector<int> nls;
+        add(nls, nl_y);
+        add(nls, CtTable<T>::col(P[1]));
+        for (int x : halvings(std::min(lines_for(P[1]), 64))) add(nls, x);
+        float best = 1e30f;
+        int bn = nl_y, bs = ty.sched;
+        for (int nl : nls)
+          for (int sc = 0; sc < FFT_NSCHED; sc++) {
+            nl_y = nl;
+            try { ty.schedule(sc); } catch (const Error &) { continue; }
+            const float t = timed([&] { pass_y(-1, s); pass_y(+1, s); });
+            if (t < best) { best = t; bn = nl; bs = sc; }
+          }
+        nl_y = bn;
+        ty.schedule(bs);
+      }
+      // 4. convolution pass: columns per work-group x radix schedule
+      if (is2d || !zreg) {
+        TwTable<T> &tc = is2d ? ty : tz;
+        const int convlen = is2d ? P[1] : P[2];
+        std::vector<int> bs_;
+        add(bs_, B_conv);
+        const int ct = is2d ? CtTable<T>::conv2(convlen) : CtTable<T>::conv3(convlen);
+        add(bs_, ct);
+        add(bs_, ct / 4);
+        for (int x : halvings(std::min(lines_for(convlen) / 3, 32))) add(bs_, x);
+        float best = 1e30f;
+        int bb_ = B_conv, bsc = tc.sched;
+        for (int B : bs_) {
+          if (!conv_ok(B)) continue;
+          for (int sc = 0; sc < FFT_NSCHED; sc++) {
+            B_conv = B;
+            try { tc.schedule(sc); } catch (const Error &) { continue; }
+            const float t = timed([&] { pass_conv(s); });
+            if (t < best) { best = t; bb_ = B; bsc = sc; }
+          }
+        }
+        B_conv = bb_;
+        tc.schedule(bsc);
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    UM_HIP(hipEventDestroy(e0));
+    UM_HIP(hipEventDestroy(e1));
+    std::vector<int> out(DEMAG_NKNOB);
+    schedule(out.data());
+    tune_store(key, out);
   }
 
   void kspec(int c, void *host) override {
@@ -1732,6 +1998,7 @@ bool demag_exec_field(DemagBase *d, const void *const m[3], const void *vol, con
 }
 void demag_destroy(DemagBase *d) { delete d; }
 void demag_kspec(DemagBase *d, int c, void *host) { d->kspec(c, host); }
+void demag_schedule(DemagBase *d, int out[DEMAG_NKNOB]) { d->schedule(out); }
 void demag_info(DemagBase *d, int P[3], int *prec) {
   for (int i = 0; i < 3; i++) P[i] = d->P[i];
   *prec = d->prec;
@@ -1741,11 +2008,11 @@ DemagBase *demag_create(int prec, const int N[3], const int P[3], double *const 
   DemagBase *d = nullptr;
   if (prec == UM_F32) {
     auto *p = new DemagPlan<float>();
-    try { p->setup(N, P, s); p->load_kernel(Kdev, s); } catch (...) { delete p; throw; }
+    try { p->setup(N, P, s); p->load_kernel(Kdev, s); p->tune(s); } catch (...) { delete p; throw; }
     d = p;
   } else {
     auto *p = new DemagPlan<double>();
-    try { p->setup(N, P, s); p->load_kernel(Kdev, s); } catch (...) { delete p; throw; }
+    try { p->setup(N, P, s); p->load_kernel(Kdev, s); p->tune(s); } catch (...) { delete p; throw; }
     d = p;
   }
   return d;

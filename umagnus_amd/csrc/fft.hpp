@@ -45,9 +45,9 @@ template <typename T> struct FftEpt;
 template <> struct FftEpt<float> { static constexpr int value = 32; };
 template <> struct FftEpt<double> { static constexpr int value = 16; };
 
-// Host side: radix plan + twiddle table for length n (throws UM_EUNSUP if n
-// has a prime factor above 7 or exceeds the LDS budget).
-std::vector<int> fft_radices(int n);
+// Host side: radix plan for length n under radix schedule `sched` (throws
+// UM_EUNSUP if n has a prime factor above 7).
+std::vector<int> fft_radices(int n, int sched = 0);
 bool fft_supported(int n);
 
 }  // namespace um

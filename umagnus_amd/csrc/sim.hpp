@@ -38,6 +38,13 @@ bool demag_exec_field(DemagBase *d, const void *const m[3], const void *vol, con
 void demag_destroy(DemagBase *d);
 void demag_kspec(DemagBase *d, int c, void *host);
 void demag_info(DemagBase *d, int P[3], int *prec);
+// launch schedule of a plan: nl_x nl_xi xi4 nl_y B_conv sched_x sched_y sched_z
+constexpr int DEMAG_NKNOB = 8;
+void demag_schedule(DemagBase *d, int out[DEMAG_NKNOB]);
+// autotuning of new plans: mode 0 off (static tables), 1 time unknown shapes,
+// 2 apply known results only; file: per-shape results read at this call and
+// appended as shapes are tuned (NULL: none)
+void set_autotune(int mode, const char *file);
 
 struct SimBase {
   virtual ~SimBase() {}

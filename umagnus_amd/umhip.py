@@ -40,6 +40,7 @@ EXPORTS = [
     "um_sim_tape_begin", "um_sim_tape_end", "um_sim_tape_steps", "um_sim_backward",
     "um_add_dmi_bulk", "um_add_zhangli", "um_add_slonczewski", "um_sim_set_dmi_bulk", "um_sim_set_stt_flags",
     "um_sim_minimize", "um_set_kernel_cache", "um_kernel_cache_basename", "um_load_kernel", "um_save_kernel",
+    "um_set_autotune", "um_demag_schedule",
     "um_sim_create", "um_sim_destroy", "um_sim_set_scalar", "um_sim_set_vector", "um_sim_set_regions",
     "um_sim_set_exchange_luts", "um_sim_set_bext_lut", "um_sim_set_frozen_lut", "um_sim_set_geometry",
     "um_sim_set_flags", "um_sim_set_demag_accuracy", "um_sim_set_thermal_seed", "um_sim_set_m",
@@ -345,6 +346,12 @@ class DemagConvolution:
         _call("um_demag_kspec", self.h, c, out.ctypes.data_as(C.c_void_p))
         return out
 
+    def schedule(self):
+        """Launch schedule: nl_x, nl_xi, four_step, nl_y, B_conv, sched_x, sched_y, sched_z."""
+        out = (C.c_int * 8)()
+        _call("um_demag_schedule", self.h, out)
+        return list(out)
+
     def Free(self):
         if self.h:
             _call("um_demag_destroy", self.h)
@@ -386,6 +393,12 @@ def demag_kernel_newell(N, cellsize, pbc=(0, 0, 0), asymptotic_radius=32, zero_s
     _call("um_demag_kernel_newell", N[0], N[1], N[2], (C.c_int * 3)(*pbc), (C.c_double * 3)(*cellsize),
           int(asymptotic_radius), int(zero_self_demag), arr)
     return K
+
+
+def set_autotune(mode, path=None):
+    """Launch-schedule autotuning of new demag plans: 0 static tables, 1 time
+    unknown shapes, 2 known results only; `path`: per-shape results file."""
+    _call("um_set_autotune", int(mode), path.encode() if path else None)
 
 
 def set_kernel_cache(directory):
