@@ -564,7 +564,8 @@ struct ConvArgs {
   int is2d;
   int Py, Pz;      // logical padded sizes for mirror indexing
   int KHy;         // Py/2+1
-  const T *K[6];   // xx yy zz yz xz xy, each H x (Py/2+1) x (Pz/2+1)
+  int Hk;          // tensor row pitch (H rounded up to a 128-byte line)
+  const T *K[6];   // xx yy zz yz xz xy, each Hk x (Py/2+1) x (Pz/2+1)
   C2<T> *buf[3];
 };
 
@@ -604,11 +605,11 @@ __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
       const int kx = kx0 + b;
       if (kx < a.H) {
         if (!a.is2d) {
-          const long I = ((long)e * a.KHy + kya) * a.H + kx;
+          const long I = ((long)e * a.KHy + kya) * a.Hk + kx;
           kv[q][0] = a.K[0][I]; kv[q][1] = a.K[1][I]; kv[q][2] = a.K[2][I];
           kv[q][3] = a.K[3][I]; kv[q][4] = a.K[4][I]; kv[q][5] = a.K[5][I];
         } else {
-          const long I = (long)e * a.H + kx;
+          const long I = (long)e * a.Hk + kx;
           kv[q][0] = a.K[0][I]; kv[q][1] = a.K[1][I]; kv[q][2] = a.K[2][I]; kv[q][5] = a.K[5][I];
         }
       }
@@ -707,9 +708,10 @@ template <typename T>
 struct ConvZArgs {
   int H;           // valid kx columns
   int Py, KHy;     // padded y size and Py/2+1
+  int Hk;          // tensor row pitch (H rounded up to a 128-byte line)
   long zstride;    // complex elements between z planes of a component
   long ystride;    // complex elements between ky rows
-  const T *K[6];   // xx yy zz yz xz xy, each H x KHy x (Nz+1)
+  const T *K[6];   // xx yy zz yz xz xy, each Hk x KHy x (Nz+1)
   C2<T> *buf[3];
 };
 
@@ -756,7 +758,7 @@ __global__ __launch_bounds__(CONVZ_NT) void k_convz(ConvZArgs<T> a) {
   // (fixed trip count, unrolled: every load of the slab is in flight at once)
   {
     constexpr int TS = KZ * 6 * CONVZ_COLS, TQ = (TS + CONVZ_NT - 1) / CONVZ_NT;
-    const long kzs = (long)a.KHy * a.H;
+    const long kzs = (long)a.KHy * a.Hk;
     const int x = min(kx0 + (int)(threadIdx.x & (CONVZ_COLS - 1)), a.H - 1);
     T tv[TQ];
 #pragma unroll
@@ -764,7 +766,7 @@ __global__ __launch_bounds__(CONVZ_NT) void k_convz(ConvZArgs<T> a) {
       const int g = threadIdx.x + q * CONVZ_NT;
       if (TS % CONVZ_NT == 0 || g < TS) {
         const int ez = g / CONVZ_COLS, e = ez % 6, kz = ez / 6;
-        tv[q] = a.K[e][kz * kzs + (long)kya * a.H + x];
+        tv[q] = a.K[e][kz * kzs + (long)kya * a.Hk + x];
       }
     }
 #pragma unroll
@@ -1024,15 +1026,19 @@ __global__ void k_demag_mirror(BfArgs a) {
 // Non-redundant real parts of the tensor spectrum, scaled by 1/(Px Py Pz)
 // (scaleRealParts, opencl/conv_common.go:27-50); also the max |imag| seen.
 template <typename T>
-__global__ void k_extract_kspec(const double2 *__restrict__ spec, T *__restrict__ out, int H, int Hp,
+__global__ void k_extract_kspec(const double2 *__restrict__ spec, T *__restrict__ out, int H, int Hk, int Hp,
                                 int Py, int KHy, int KHz, double scale,
                                 unsigned long long *__restrict__ maximg_bits) {
-  const long total = (long)H * KHy * KHz;
+  const long total = (long)Hk * KHy * KHz;
   double mi = 0;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-    const int kx = (int)(t % H);
-    const int ky = (int)((t / H) % KHy);
-    const int kz = (int)(t / ((long)H * KHy));
+    const int kx = (int)(t % Hk);
+    const int ky = (int)((t / Hk) % KHy);
+    const int kz = (int)(t / ((long)Hk * KHy));
+    if (kx >= H) {   // row padding
+      out[t] = T(0);
+      continue;
+    }
     const double2 v = spec[((long)kz * Py + ky) * Hp + kx];
     out[t] = (T)(v.x * scale);
     mi = fmax(mi, fabs(v.y));
@@ -1483,7 +1489,7 @@ static void tune_store(const std::string &key, const std::vector<int> &v) {
 
 template <typename T>
 struct DemagPlan : DemagBase {
-  int H, Hp, is2d, KHy, KHz;
+  int H, Hp, Hk, is2d, KHy, KHz;   // Hk: tensor row pitch (H rounded up to 128 bytes)
   TwTable<T> tx, ty, tz, tx2;   // tx: length Px/2, tx2: length Px (post-processing), ty, tz
   DevBuf spec[3];               // per component: Nz*Py*Hp complex
   DevBuf kern;                  // 6 x H*KHy*KHz of T
@@ -1509,7 +1515,7 @@ struct DemagPlan : DemagBase {
     is2d = (N[2] == 1);
     UM_REQUIRE(!is2d || P[2] == 1, UM_EUNSUP, "2D grid with Z padding");
     H = P[0] / 2 + 1;
-    const int al = 128 / (int)sizeof(C2<T>);
+    const int al = 128 / (int)sizeof(C2<T>), al2 = 128 / (int)sizeof(T);
     Hp = (H + al - 1) / al * al;
     KHy = P[1] / 2 + 1;
     KHz = P[2] / 2 + 1;
@@ -1551,7 +1557,8 @@ struct DemagPlan : DemagBase {
     UM_REQUIRE(3 * B_conv * convlen <= FFT_NT * FftEpt<T>::value, UM_EUNSUP, "convolution tile exceeds LDS budget");
     const size_t sc = is2d ? (size_t)N[1] * Hp : (size_t)N[2] * P[1] * Hp;
     for (int c = 0; c < 3; c++) spec[c].alloc(sizeof(C2<T>) * sc);
-    kcount = (long)H * KHy * KHz;
+    Hk = (H + al2 - 1) / al2 * al2;
+    kcount = (long)Hk * KHy * KHz;
     kern.alloc(sizeof(T) * 6 * kcount);
   }
 
@@ -1611,7 +1618,7 @@ struct DemagPlan : DemagBase {
         UM_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL(k_extract_kspec<T>, dim3(blocks_for(kcount, 256, 4096)), dim3(256), 0, s,
-                         (const double2 *)sp.p, out, H, Hpd, P[1], KHy, KHz, scale,
+                         (const double2 *)sp.p, out, H, Hk, Hpd, P[1], KHy, KHz, scale,
                          (unsigned long long *)mx.p);
       UM_HIP(hipGetLastError());
     }
@@ -1663,7 +1670,7 @@ struct DemagPlan : DemagBase {
     for (int c = 0; c < 6; c++) K[c] = kern.as<T>() + c * kcount;
     if (!is2d && zreg) {
       ConvZArgs<T> za{};
-      za.H = H; za.Py = P[1]; za.KHy = KHy; za.zstride = (long)P[1] * Hp; za.ystride = Hp;
+      za.H = H; za.Py = P[1]; za.KHy = KHy; za.Hk = Hk; za.zstride = (long)P[1] * Hp; za.ystride = Hp;
       for (int c = 0; c < 6; c++) za.K[c] = K[c];
       for (int c = 0; c < 3; c++) za.buf[c] = spec[c].as<C2<T>>();
       launch_convz<T>(za, N[2], s);
@@ -1673,7 +1680,7 @@ struct DemagPlan : DemagBase {
     ca.f = is2d ? ty.len : tz.len; ca.B = B_conv; ca.fB.init(B_conv); ca.f3B.init(3 * B_conv); ca.H = H;
     ca.nvalid = is2d ? N[1] : N[2];
     ca.estride = is2d ? Hp : (long)P[1] * Hp; ca.pstride = is2d ? 0 : Hp;
-    ca.is2d = is2d; ca.Py = P[1]; ca.Pz = is2d ? 1 : P[2]; ca.KHy = KHy;
+    ca.is2d = is2d; ca.Py = P[1]; ca.Pz = is2d ? 1 : P[2]; ca.KHy = KHy; ca.Hk = Hk;
     for (int c = 0; c < 6; c++) ca.K[c] = K[c];
     for (int c = 0; c < 3; c++) ca.buf[c] = spec[c].as<C2<T>>();
     dim3 gc((H + B_conv - 1) / B_conv, is2d ? 1 : P[1] / 2 + 1);   // 3D: ky and Py-ky per work-group
@@ -1918,7 +1925,9 @@ struct DemagPlan : DemagBase {
   void kspec(int c, void *host) override {
     UM_REQUIRE(c >= 0 && c < 6, UM_EINVAL, "component out of range");
     UM_HIP(hipDeviceSynchronize());
-    UM_HIP(hipMemcpy(host, kern.as<T>() + c * kcount, sizeof(T) * kcount, hipMemcpyDeviceToHost));
+    // rows of H values out of the padded pitch Hk
+    UM_HIP(hipMemcpy2D(host, sizeof(T) * H, kern.as<T>() + c * kcount, sizeof(T) * Hk, sizeof(T) * H,
+                       (size_t)KHy * KHz, hipMemcpyDeviceToHost));
   }
 };
 
