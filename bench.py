@@ -418,6 +418,7 @@ def main():
                         "Ms 1.1e6, Aex 1.6e-11, Dind 3e-3, Ku1 8e5 (z), alpha 0.1",
             "cells": n, "solver": "RK45DP", "evaluations_per_step": 6,
             "step_fraction_of_hbm_roofline": round(b_step * args.steps / t / 1e9 / HBM_PEAK_GBS, 4),
+            "heff_evals_per_s": round(1.0 / eval_s, 1),
         },
         "roofline": {
             "kernel": "one effective-field + torque evaluation: demag FFT passes (X-fwd R2C, Y-fwd, "
