@@ -365,7 +365,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
 

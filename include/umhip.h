@@ -328,7 +328,7 @@ int um_sim_set_profiling(void *sim, int on);
 int um_sim_kernel_times(void *sim, double out_ms[3], int *n_evals);
 
 /* ---- HIP graphs and ensembles (no counterpart in the reference, whose steps
- * are enqueued call by call; SURVEY-listed um_graph_* of the boundary) ------- */
+ * are enqueued call by call) ------------------------------------------------ */
 /* With FixDt set, um_sim_steps runs one step eagerly, then replays a captured
  * two-step HIP graph (no host synchronisation per step: the step statistics
  * and the thermal Philox step counter stay on the device) and reads the

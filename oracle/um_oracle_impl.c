@@ -8,7 +8,8 @@
  * with -ffp-contract=off so that every product and sum rounds as written.
  * Each function cites the reference file:line it restates.  The FP64 build
  * is the "all-FP64 oracle": it deliberately does not reproduce the float32
- * roundings that uMagNUS64 inherits from amul(float mul) and pow4.
+ * roundings that uMagNUS64 inherits from amul(float mul) and pow4 (float32
+ * multipliers of its FP64 kernels).
  */
 #include "um_oracle.h"
 #include <math.h>
@@ -525,7 +526,7 @@ double OR(maxvecnorm)(const real *x, const real *y, const real *z, int N) {
 }
 /* MaxVecDiff (reduce.go:214-251).  The kernel widens the scalar distance to a
  * vector (reducemaxvecdiff2.cl:24), so the reference returns sqrt(3)*|a-b|;
- * sqrt3_quirk selects that behaviour (SURVEY Q3). */
+ * sqrt3_quirk selects that behaviour. */
 double OR(maxvecdiff)(const real *x1, const real *y1, const real *z1, const real *x2,
                       const real *y2, const real *z2, int N, int sqrt3_quirk) {
   real mx = 0;
@@ -541,7 +542,7 @@ double OR(maxvecdiff)(const real *x1, const real *y1, const real *z1, const real
   return sqrt((double)mx);
 }
 /* MaxAbs (reduce.go:105-128): the mathematically intended max|x| (reduce_test.go:75-84);
- * the reference kernel omits fabs, which only matters for negative inputs. */
+ * the reference kernel (reducemaxabs.cl) omits fabs, which only matters for negative inputs. */
 double OR(maxabs)(const real *x, int N) {
   double mx = 0;
   for (int i = 0; i < N; i++) {
@@ -550,7 +551,10 @@ double OR(maxabs)(const real *x, int N) {
   }
   return mx;
 }
-/* Sum / Dot (reduce.go:26-102): the intended sums, accumulated in FP64. */
+/* Sum / Dot (reduce.go:26-102): the intended sums, accumulated in FP64.  The
+ * runtime-compiled reducesum.cl / reducedot.cl add the first element of each
+ * work-item twice (reducesum.cl:12 and :18); the tests' expectations encode
+ * the plain sum, which is what this computes. */
 double OR(sum)(const real *x, int N) {
   double s = 0;
   for (int i = 0; i < N; i++) s += (double)x[i];

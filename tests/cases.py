@@ -14,7 +14,7 @@ def random_unit(shape, seed):
     return m / np.linalg.norm(m, axis=0)
 
 
-def config1_m(seed=1):
+def strip_m(seed=1):
     """128x32x1: (1,1,0)/sqrt2 plus a seeded uniform +-0.1 perturbation, renormalised."""
     rng = np.random.default_rng(seed)
     m = np.zeros((3, 1, 32, 128))

@@ -1,6 +1,6 @@
 """The launch-schedule autotuner on the GPU: every radix schedule of the
 generic Stockham engine, forced through a tuning file, gives the demag field of
-the oracle; tuned plans (including the config-4 grid sizes with a factor 3,
+the oracle; tuned plans (including the ensemble grid sizes with a factor 3,
 odd periodic Nx and 2D) stay within the convolution tolerances; results are
 kept per shape, written to the tuning file and reused by mode 2.
 

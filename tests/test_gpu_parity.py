@@ -10,7 +10,7 @@ import sys
 import numpy as np
 import pytest
 
-from cases import GAMMA, MU0, config1_m, random_unit, relerr, setup
+from cases import GAMMA, MU0, strip_m, random_unit, relerr, setup
 
 pytestmark = pytest.mark.gpu
 
@@ -273,8 +273,8 @@ def _both(oracle, gpu, N, cs, prec, **params):
 
 
 @pytest.mark.parametrize("prec", [1, 0])
-def test_sim_beff_torque_config2_like(gpu, oracle, prec):
-    """Config-2 physics (DMI, Ku1 || z, alpha 0.1, random m) on a small 3D grid."""
+def test_sim_beff_torque_film_physics(gpu, oracle, prec):
+    """The 512x512x32 workload's physics (DMI, Ku1 || z, alpha 0.1, random m) on a small 3D grid."""
     N, cs = (32, 24, 6), (2e-9, 2e-9, 2e-9)
     so, sg = _both(oracle, gpu, N, cs, prec, Ms=1.1e6, Aex=1.6e-11, Dind=3e-3, Ku1=8e5, alpha=0.1)
     m = random_unit((N[2], N[1], N[0]), 21)
@@ -400,11 +400,11 @@ def test_rk4_golden(gpu):
 
 
 @pytest.mark.parametrize("prec", [1, 0])
-def test_config1_trajectory_vs_oracle(gpu, oracle, prec):
-    """Config 1 (128x32x1, RK45DP adaptive, MaxErr 1e-5): <m>(t) within 1e-4 of the FP64 oracle."""
+def test_strip_trajectory_vs_oracle(gpu, oracle, prec):
+    """The 128x32x1 strip (RK45DP adaptive, MaxErr 1e-5): <m>(t) within 1e-4 of the FP64 oracle."""
     N, cs = (128, 32, 1), (3.90625e-9, 3.90625e-9, 3e-9)
     so, sg = _both(oracle, gpu, N, cs, prec, Ms=8e5, Aex=1.3e-11, alpha=0.02, Bext=(-24.6e-3, 4.3e-3, 0.0))
-    m = config1_m(1)
+    m = strip_m(1)
     so.set_m(m)
     sg.set_m(m)
     for k in range(4):
@@ -416,8 +416,8 @@ def test_config1_trajectory_vs_oracle(gpu, oracle, prec):
 
 
 # ---- full-size properties (512 x 512 x 32) -----------------------------------------------------
-def test_config2_full_size_properties(gpu):
-    """Config 2 at full size: FP32 field within 1e-5 of the FP64 GPU path, demag
+def test_512x512x32_full_size_properties(gpu):
+    """The 512x512x32 workload at full size: FP32 field within 1e-5 of the FP64 GPU path, demag
     linearity and symmetry (<D u, v> = <u, D v>), unit |m| after RK45DP steps."""
     import torch
     N, cs = (512, 512, 32), (2e-9, 2e-9, 2e-9)
@@ -434,7 +434,7 @@ def test_config2_full_size_properties(gpu):
     s32.set_m(m)
     b32 = s32.beff()
     e = relerr(b32, b64)
-    _say("config2 FP32 vs FP64 Beff relerr", e)
+    _say("512x512x32 FP32 vs FP64 Beff relerr", e)
     assert e <= 1e-5
     # demag symmetry on the FP32 path: <D u, v> = <u, D v>
     conv = gpu.DemagConvolution(N, cs, precision=0)
@@ -583,12 +583,12 @@ def test_energy_terms_vs_oracle(gpu, oracle, prec):
 
 @pytest.mark.parametrize("prec", [1, 0])
 def test_rk23_trajectory_vs_oracle(gpu, oracle, prec):
-    """Adaptive RK23 (engine/rk23.go) on config-1 physics: <m>(t) vs the FP64 oracle."""
+    """Adaptive RK23 (engine/rk23.go) on the 128x32x1 strip's physics: <m>(t) vs the FP64 oracle."""
     N, cs = (64, 16, 1), (3.90625e-9, 3.90625e-9, 3e-9)
     so, sg = _both(oracle, gpu, N, cs, prec, Ms=8e5, Aex=1.3e-11, alpha=0.02, Bext=(-24.6e-3, 4.3e-3, 0.0))
     for s in (so, sg):
         s.set_solver(gpu.SOLVER_RK23)
-    m = config1_m(1)[:, :, :16, :64]
+    m = strip_m(1)[:, :, :16, :64]
     so.set_m(m)
     sg.set_m(m)
     for _ in range(3):
