@@ -89,7 +89,10 @@ CONV_SIZES = [(3, 4, 1), (16, 8, 1), (20, 12, 1), (5, 4, 2), (6, 5, 3), (8, 6, 4
               (128, 6, 3), (256, 5, 2), (512, 3, 2), (1024, 2, 1),
               # small grids: X rows and 2D conv columns split over more work-groups
               (128, 32, 1), (256, 256, 1), (512, 64, 1), (1024, 8, 1), (32, 256, 1), (256, 64, 8),
-              (256, 256, 8), (128, 128, 16), (64, 64, 64), (1536, 128, 1), (3072, 32, 1), (3072, 64, 1)]
+              (256, 256, 8), (128, 128, 16), (64, 64, 64), (1536, 128, 1), (3072, 32, 1), (3072, 64, 1),
+              # ensemble sizes: X lengths 32/48/96/192 (full and small-grid rows), Y 96/192/384, Pz 3/5/7
+              (32, 48, 2), (48, 96, 3), (96, 192, 4), (192, 32, 2), (48, 48, 1), (96, 96, 1), (192, 192, 1),
+              (32, 192, 1), (192, 256, 4), (256, 96, 3), (96, 32, 4), (48, 192, 2)]
 
 
 @pytest.mark.parametrize("N", CONV_SIZES)

@@ -1120,11 +1120,25 @@ struct TwTable {
 // Lines per work-group of each specialised length (LDS budget ~40-72 KiB so two
 // or more work-groups share a CU); other lengths use the generic kernels.
 template <typename T> struct CtTable;
+// The 32/48/96/192-point X, 96/192/384-point Y and Pz = 3/5/7 Z entries serve
+// small grids with Nx, Ny in {32, 48, 96, 192} and Nz <= 4 (ensembles).
 template <> struct CtTable<float> {
-  static int x(int n) { switch (n) { case 64: return 32; case 128: return 16; case 256: return 16; case 512: return 8; case 1024: return 4; } return 0; }
-  static int col(int n) { switch (n) { case 64: return 32; case 128: return 16; case 256: return 16; case 512: return 8; case 1024: return 8; case 2048: return 4; } return 0; }
-  static int conv3(int n) { switch (n) { case 16: return 16; case 32: return 16; case 64: return 16; } return 0; }
-  static int conv2(int n) { switch (n) { case 64: return 16; case 128: return 16; case 256: return 8; case 512: return 4; } return 0; }
+  static int x(int n) {
+    switch (n) { case 32: return 32; case 48: return 32; case 64: return 32; case 96: return 16; case 128: return 16;
+                 case 192: return 16; case 256: return 16; case 512: return 8; case 1024: return 4; }
+    return 0;
+  }
+  static int col(int n) {
+    switch (n) { case 64: return 32; case 96: return 16; case 128: return 16; case 192: return 8; case 256: return 16;
+                 case 384: return 8; case 512: return 8; case 1024: return 8; case 2048: return 4; }
+    return 0;
+  }
+  static int conv3(int n) { switch (n) { case 3: return 32; case 5: return 32; case 7: return 32; case 16: return 16; case 32: return 16; case 64: return 16; } return 0; }
+  static int conv2(int n) {
+    switch (n) { case 64: return 16; case 96: return 16; case 128: return 16; case 192: return 8; case 256: return 8;
+                 case 384: return 4; case 512: return 4; }
+    return 0;
+  }
 };
 template <> struct CtTable<double> {
   static int x(int n) { switch (n) { case 256: return 8; case 512: return 4; } return 0; }
@@ -1187,6 +1201,19 @@ static void launch_xfwd(const XArgs<T> &a, int n, int nl, dim3 grid, hipStream_t
     UM_CT_CASE((k_xfwd<float, 512, 2>), 512, 2, ct)
     UM_CT_CASE((k_xfwd<float, 512, 1>), 512, 1, ct)
     UM_CT_CASE((k_xfwd<float, 1024, 1>), 1024, 1, ct)
+    // small grids of the other X lengths
+    UM_CT_CASE((k_xfwd<float, 32, 32>), 32, 32, ct)
+    UM_CT_CASE((k_xfwd<float, 32, 8>), 32, 8, ct)
+    UM_CT_CASE((k_xfwd<float, 32, 1>), 32, 1, ct)
+    UM_CT_CASE((k_xfwd<float, 48, 32>), 48, 32, ct)
+    UM_CT_CASE((k_xfwd<float, 48, 8>), 48, 8, ct)
+    UM_CT_CASE((k_xfwd<float, 48, 1>), 48, 1, ct)
+    UM_CT_CASE((k_xfwd<float, 96, 16>), 96, 16, ct)
+    UM_CT_CASE((k_xfwd<float, 96, 4>), 96, 4, ct)
+    UM_CT_CASE((k_xfwd<float, 96, 1>), 96, 1, ct)
+    UM_CT_CASE((k_xfwd<float, 192, 16>), 192, 16, ct)
+    UM_CT_CASE((k_xfwd<float, 192, 4>), 192, 4, ct)
+    UM_CT_CASE((k_xfwd<float, 192, 1>), 192, 1, ct)
   } else {
     UM_CT_CASE((k_xfwd<double, 256, 8>), 256, 8, ct)
     UM_CT_CASE((k_xfwd<double, 512, 4>), 512, 4, ct)
@@ -1204,6 +1231,18 @@ static void launch_xinv(const XiArgs<T> &a, int n, int nl, dim3 grid, hipStream_
     UM_CT_CASE((k_xinv<float, 256, 16>), 256, 16, ct)
     UM_CT_CASE((k_xinv<float, 512, 8>), 512, 8, ct)
     UM_CT_CASE((k_xinv<float, 1024, 4>), 1024, 4, ct)
+    UM_CT_CASE((k_xinv<float, 32, 32>), 32, 32, ct)
+    UM_CT_CASE((k_xinv<float, 32, 8>), 32, 8, ct)
+    UM_CT_CASE((k_xinv<float, 32, 1>), 32, 1, ct)
+    UM_CT_CASE((k_xinv<float, 48, 32>), 48, 32, ct)
+    UM_CT_CASE((k_xinv<float, 48, 8>), 48, 8, ct)
+    UM_CT_CASE((k_xinv<float, 48, 1>), 48, 1, ct)
+    UM_CT_CASE((k_xinv<float, 96, 16>), 96, 16, ct)
+    UM_CT_CASE((k_xinv<float, 96, 4>), 96, 4, ct)
+    UM_CT_CASE((k_xinv<float, 96, 1>), 96, 1, ct)
+    UM_CT_CASE((k_xinv<float, 192, 16>), 192, 16, ct)
+    UM_CT_CASE((k_xinv<float, 192, 4>), 192, 4, ct)
+    UM_CT_CASE((k_xinv<float, 192, 1>), 192, 1, ct)
   } else {
     UM_CT_CASE((k_xinv<double, 256, 8>), 256, 8, ct)
     UM_CT_CASE((k_xinv<double, 512, 4>), 512, 4, ct)
@@ -1221,6 +1260,9 @@ static void launch_col(const ColArgs<T> &a, int n, int nl, dim3 grid, hipStream_
     UM_CT_CASE((k_colfft<float, DIR, 512, 8>), 512, 8, ct)
     UM_CT_CASE((k_colfft<float, DIR, 1024, 8>), 1024, 8, ct)
     UM_CT_CASE((k_colfft<float, DIR, 2048, 4>), 2048, 4, ct)
+    UM_CT_CASE((k_colfft<float, DIR, 96, 16>), 96, 16, ct)
+    UM_CT_CASE((k_colfft<float, DIR, 192, 8>), 192, 8, ct)
+    UM_CT_CASE((k_colfft<float, DIR, 384, 8>), 384, 8, ct)
   } else {
     UM_CT_CASE((k_colfft<double, DIR, 512, 4>), 512, 4, ct)
     UM_CT_CASE((k_colfft<double, DIR, 1024, 4>), 1024, 4, ct)
@@ -1276,6 +1318,9 @@ static void launch_conv(const ConvArgs<T> &a, int n, int B, dim3 grid, hipStream
       UM_CT_CASE((k_conv<float, 16, 16>), 16, 16, ct)
       UM_CT_CASE((k_conv<float, 32, 16>), 32, 16, ct)
       UM_CT_CASE((k_conv<float, 64, 16>), 64, 16, ct)
+      UM_CT_CASE((k_conv<float, 3, 32>), 3, 32, ct)     // Pz = 2 Nz - 1 for Nz = 2, 3, 4
+      UM_CT_CASE((k_conv<float, 5, 32>), 5, 32, ct)
+      UM_CT_CASE((k_conv<float, 7, 32>), 7, 32, ct)
     } else {
       UM_CT_CASE((k_conv<double, 16, 8>), 16, 8, ct)
       UM_CT_CASE((k_conv<double, 64, 8>), 64, 8, ct)
@@ -1294,6 +1339,14 @@ static void launch_conv(const ConvArgs<T> &a, int n, int B, dim3 grid, hipStream
       UM_CT_CASE((k_conv<float, 256, 2>), 256, 2, ct)
       UM_CT_CASE((k_conv<float, 256, 1>), 256, 1, ct)
       UM_CT_CASE((k_conv<float, 512, 1>), 512, 1, ct)
+      UM_CT_CASE((k_conv<float, 96, 16>), 96, 16, ct)
+      UM_CT_CASE((k_conv<float, 96, 4>), 96, 4, ct)
+      UM_CT_CASE((k_conv<float, 96, 1>), 96, 1, ct)
+      UM_CT_CASE((k_conv<float, 192, 8>), 192, 8, ct)
+      UM_CT_CASE((k_conv<float, 192, 2>), 192, 2, ct)
+      UM_CT_CASE((k_conv<float, 192, 1>), 192, 1, ct)
+      UM_CT_CASE((k_conv<float, 384, 4>), 384, 4, ct)
+      UM_CT_CASE((k_conv<float, 384, 1>), 384, 1, ct)
     } else {
       UM_CT_CASE((k_conv<double, 256, 4>), 256, 4, ct)
       UM_CT_CASE((k_conv<double, 512, 2>), 512, 2, ct)
