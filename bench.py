@@ -158,7 +158,7 @@ def batched(steps=60):
             "tuned": round(res["tuned"], 2), "tuned_speedup": round(res["tuned"] / res["plain"], 3)}
 
 
-def autotune(shapes=((512, 512, 32), (192, 96, 3), (96, 96, 2), (256, 256, 1), (48, 192, 4)), reps=20):
+def autotune(shapes=((512, 512, 32), (100, 100, 2), (120, 80, 1), (250, 250, 1), (48, 192, 4)), reps=20):
     """Demag convolution time per call with the static launch tables vs the
     autotuned schedule (FP32), per shape."""
     import torch

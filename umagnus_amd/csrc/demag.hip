@@ -592,31 +592,41 @@ __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
   const int kyb = a.Py - kya;
   const int ntiles = (!a.is2d && kya != 0 && kyb != kya) ? 2 : 1;
   const int nh = n / 2 + 1;   // unique points along the axis: k and n-k share |K|
-  constexpr int QK = CN > 0 ? (CB * (CN / 2 + 1) + FFT_NT - 1) / FFT_NT : CONV_QK;
-  // 1. tensor prefetch (independent of the data, so it overlaps the loads below)
+  // tensor values of frequency e, column kx (2D: yz and xz are not used)
+  auto tensor = [&](int e, int kx, T *k) {
+#pragma unroll
+    for (int j = 0; j < 6; j++) k[j] = T(0);
+    if (!a.is2d) {
+      const long I = ((long)e * a.KHy + kya) * a.Hk + kx;
+      k[0] = a.K[0][I]; k[1] = a.K[1][I]; k[2] = a.K[2][I];
+      k[3] = a.K[3][I]; k[4] = a.K[4][I]; k[5] = a.K[5][I];
+    } else {
+      const long I = (long)e * a.Hk + kx;
+      k[0] = a.K[0][I]; k[1] = a.K[1][I]; k[2] = a.K[2][I]; k[5] = a.K[5][I];
+    }
+  };
+  // 1. tensor prefetch (independent of the data, so it overlaps the loads
+  // below); the run-time-length kernel reads it at the multiply instead, to
+  // keep its register count down
+  constexpr int QK = CN > 0 ? (CB * (CN / 2 + 1) + FFT_NT - 1) / FFT_NT : 1;
   T kv[QK][6];
+  if constexpr (CN > 0) {
 #pragma unroll
-  for (int q = 0; q < QK; q++) {
-    const int g = threadIdx.x + q * FFT_NT;
+    for (int q = 0; q < QK; q++) {
+      const int g = threadIdx.x + q * FFT_NT;
 #pragma unroll
-    for (int j = 0; j < 6; j++) kv[q][j] = T(0);
-    if (g < B * nh) {
-      const int e = cdiv<CB>(g, a.fB), b = g - e * B;
-      const int kx = kx0 + b;
-      if (kx < a.H) {
-        if (!a.is2d) {
-          const long I = ((long)e * a.KHy + kya) * a.Hk + kx;
-          kv[q][0] = a.K[0][I]; kv[q][1] = a.K[1][I]; kv[q][2] = a.K[2][I];
-          kv[q][3] = a.K[3][I]; kv[q][4] = a.K[4][I]; kv[q][5] = a.K[5][I];
-        } else {
-          const long I = (long)e * a.Hk + kx;
-          kv[q][0] = a.K[0][I]; kv[q][1] = a.K[1][I]; kv[q][2] = a.K[2][I]; kv[q][5] = a.K[5][I];
-        }
+      for (int j = 0; j < 6; j++) kv[q][j] = T(0);
+      if (g < B * nh) {
+        const int e = cdiv<CB>(g, a.fB), b = g - e * B;
+        if (kx0 + b < a.H) tensor(e, kx0 + b, kv[q]);
       }
     }
   }
   // 2. spectrum loads of a tile
-  constexpr int SL = CN > 0 ? (3 * CB * CN + FFT_NT - 1) / FFT_NT : EPT;
+  // (fixed-length kernels: every load of the tile in flight at once; the
+  // run-time-length kernel streams them to keep its register count down)
+  constexpr int SL = CN > 0 ? (3 * CB * CN + FFT_NT - 1) / FFT_NT : 1;
+  (void)EPT;
   C2<T> v[SL];
   auto load = [&](int ky) {
     const long off = (long)ky * a.pstride;
@@ -646,46 +656,77 @@ __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
   };
   for (int t = 0; t < ntiles; t++) {
     const int ky = (t == 0) ? kya : kyb;
-    load(ky);
+    if constexpr (CN > 0) {
+      load(ky);
 #pragma unroll
-    for (int s = 0; s < SL; s++) {
-      const int g = threadIdx.x + s * FFT_NT;
-      if (g < total) lds[g] = v[s];
+      for (int s = 0; s < SL; s++) {
+        const int g = threadIdx.x + s * FFT_NT;
+        if (g < total) lds[g] = v[s];
+      }
+    } else {
+      const long off = (long)ky * a.pstride;
+#pragma unroll 4
+      for (int g = threadIdx.x; g < total; g += FFT_NT) {
+        const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
+        const int c = cdiv<CB>(l, a.fB), b = l - c * B;
+        const int kx = kx0 + b;
+        C2<T> w{T(0), T(0)};
+        if (kx < a.H && e < a.nvalid) w = a.buf[c][off + (long)e * a.estride + kx];
+        lds[g] = w;
+      }
     }
     __syncthreads();
     run_fft<T, CN, 3 * CB, -1, 1, 1>(lds, twl, nl, a.f3B, a.f);
     // 3. tensor multiply at k and at its mirror n-k (signs of the odd components flip)
     const T sy = (t == 0) ? T(1) : T(-1);   // ky mirror: yz and xy change sign
+    auto multiply = [&](int e, int b, const T *k) {
+      const T yz = sy * k[3], xy = sy * k[5];
+      mul(lds + e * nl + b, k[0], k[1], k[2], yz, k[4], xy);
+      const int e2 = n - e;
+      if (e != 0 && e2 != e) {
+        // 3D mirror in z flips yz and xz (kernmulrsymm3d.cl:67-71); 2D mirror in y flips xy
+        if (!a.is2d) mul(lds + e2 * nl + b, k[0], k[1], k[2], -yz, -k[4], xy);
+        else mul(lds + e2 * nl + b, k[0], k[1], k[2], k[3], k[4], -k[5]);
+      }
+    };
+    if constexpr (CN > 0) {
 #pragma unroll
-    for (int q = 0; q < QK; q++) {
-      const int g = threadIdx.x + q * FFT_NT;
-      if (g < B * nh) {
+      for (int q = 0; q < QK; q++) {
+        const int g = threadIdx.x + q * FFT_NT;
+        if (g < B * nh) {
+          const int e = cdiv<CB>(g, a.fB), b = g - e * B;
+          if (kx0 + b < a.H) multiply(e, b, kv[q]);
+        }
+      }
+    } else {
+#pragma unroll 1
+      for (int g = threadIdx.x; g < B * nh; g += FFT_NT) {
         const int e = cdiv<CB>(g, a.fB), b = g - e * B;
         if (kx0 + b < a.H) {
-          const T *k = kv[q];
-          const T yz = sy * k[3], xy = sy * k[5];
-          mul(lds + e * nl + b, k[0], k[1], k[2], yz, k[4], xy);
-          const int e2 = n - e;
-          if (e != 0 && e2 != e) {
-            // 3D mirror in z flips yz and xz (kernmulrsymm3d.cl:67-71); 2D mirror in y flips xy
-            if (!a.is2d) mul(lds + e2 * nl + b, k[0], k[1], k[2], -yz, -k[4], xy);
-            else mul(lds + e2 * nl + b, k[0], k[1], k[2], k[3], k[4], -k[5]);
-          }
+          T k[6];
+          tensor(e, kx0 + b, k);
+          multiply(e, b, k);
         }
       }
     }
     __syncthreads();
     run_fft<T, CN, 3 * CB, +1, 1, 1>(lds, twl, nl, a.f3B, a.f);
     const long off = (long)ky * a.pstride;
+    auto store = [&](int g) {
+      const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
+      const int c = cdiv<CB>(l, a.fB), b = l - c * B;
+      const int kx = kx0 + b;
+      if (kx < a.H && e < a.nvalid) a.buf[c][off + (long)e * a.estride + kx] = lds[g];
+    };
+    if constexpr (CN > 0) {
 #pragma unroll
-    for (int s = 0; s < SL; s++) {
-      const int g = threadIdx.x + s * FFT_NT;
-      if (g < total) {
-        const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
-        const int c = cdiv<CB>(l, a.fB), b = l - c * B;
-        const int kx = kx0 + b;
-        if (kx < a.H && e < a.nvalid) a.buf[c][off + (long)e * a.estride + kx] = lds[g];
+      for (int s = 0; s < SL; s++) {
+        const int g = threadIdx.x + s * FFT_NT;
+        if (g < total) store(g);
       }
+    } else {
+#pragma unroll 4
+      for (int g = threadIdx.x; g < total; g += FFT_NT) store(g);
     }
     __syncthreads();
   }
