@@ -197,3 +197,36 @@ def test_interfacial_and_bulk_dmi_refused(gpu):
     s.set_m(uniform_m((8, 8, 1), (1, 0, 0)))
     with pytest.raises(gpu.UmError):
         s.torque()
+
+
+@pytest.mark.parametrize("dmi", ["bulk", "interfacial"])
+def test_four_cell_kernel_stt_vs_oracle(gpu, oracle, dmi):
+    """512 x 256 x 4 cells (the four-cell field kernel's size range, FP32):
+    B_eff and torque with bulk or interfacial DMI, Zhang-Li and Slonczewski
+    torques and per-cell regions vs the FP64 oracle (1e-5 normwise), then 5
+    RK45DP steps (<m> within 1e-6)."""
+    from umagnus_amd.params import exchange_lut
+    _say("four-cell stt", dmi)
+    N, cs = (512, 256, 4), (2e-9, 2e-9, 1.5e-9)
+    shape = (N[2], N[1], N[0])
+    regions = (np.arange(N[0])[None, None, :] // 128 + np.zeros(shape, int)).astype(np.uint8)
+
+    def make(mod, dt):
+        bulk = dmi == "bulk"
+        s = _stt_sim(mod, N, cs, dt, bulk=bulk)
+        if not bulk:
+            s.set_exchange_luts(exchange_lut({0: 1.3e-11, 1: 1.1e-11, 2: 1.5e-11, 3: 1.2e-11}, dtype=dt),
+                                exchange_lut({0: 1e-3, 1: -2e-3, 2: 1.5e-3, 3: 5e-4}, dtype=dt))
+        s.set_regions(regions)
+        s.set_stt_flags(0, 0, fixed_layer_top=False)
+        return s
+    sg, so = make(gpu, np.float32), make(oracle, np.float64)
+    m = random_unit(shape, 12)
+    sg.set_m(m)
+    so.set_m(m)
+    assert _err(sg.beff(), so.beff()) <= 1e-5
+    assert _err(sg.torque(), so.torque()) <= 1e-5
+    for s in (sg, so):
+        s.set_solver_params(dt=1e-14, fixdt=1e-14)
+        s.steps(5)
+    assert np.abs(sg.average_m() - so.average_m()).max() <= 1e-6

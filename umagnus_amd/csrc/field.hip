@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
 }
 
 // four x cells per thread; consecutive threads take consecutive x groups
-template <typename T, bool REG, int EX>
+template <typename T, bool REG, int EX, bool STT>
 __global__ __launch_bounds__(256) void k_field_torque4(FieldArgs<T> a) {
   const int ngx = a.g.Nx >> 2;
   const long gid = blockIdx.x * 256L + threadIdx.x;
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void k_field_torque4(FieldArgs<T> a) {
     if (a.bd[0]) ld4(a.bd[c] + I0, bdv[c]);
     else bdv[c][0] = bdv[c][1] = bdv[c][2] = bdv[c][3] = T(0);
   }
-  field_group4<T, REG, EX>(a, gx * 4, iy, iz, bdv);
+  field_group4<T, REG, EX, STT>(a, gx * 4, iy, iz, bdv);
 }
 
 // whether the four-cell kernel applies: FP32 (in FP64 the four cells need
@@ -268,7 +268,6 @@ __global__ __launch_bounds__(256) void k_field_torque4(FieldArgs<T> a) {
 template <typename T>
 static bool field4_ok(const FieldArgs<T> &a) {
   if (sizeof(T) != 4 || a.g.Nx % 4) return false;
-  if (a.exch_mode == 3 || a.has_zl || a.has_sl) return false;   // bulk DMI and STT: one-cell kernel
   auto al = [](const void *p) { return ((uintptr_t)p & (4 * sizeof(T) - 1)) == 0; };
   for (int c = 0; c < 3; c++) {
     if (!al(a.m[c])) return false;
@@ -288,11 +287,17 @@ void launch_field_torque(const FieldArgs<T> &a, hipStream_t s) {
     const long n4 = (long)a.g.Nx / 4 * a.g.Ny * a.g.Nz;
     const dim3 grid((unsigned)((n4 + 255) / 256));
     const bool reg = a.regions != nullptr;
-#define UM_F4(R, E) hipLaunchKernelGGL((k_field_torque4<T, R, E>), grid, dim3(256), 0, s, a)
+    const bool stt = a.has_zl || a.has_sl;
+#define UM_F4(R, E)                                                                    \
+  do {                                                                                 \
+    if (stt) hipLaunchKernelGGL((k_field_torque4<T, R, E, true>), grid, dim3(256), 0, s, a);  \
+    else hipLaunchKernelGGL((k_field_torque4<T, R, E, false>), grid, dim3(256), 0, s, a);     \
+  } while (0)
     if constexpr (sizeof(T) == 4) {
       if (a.exch_mode == 0) { if (reg) UM_F4(true, 0); else UM_F4(false, 0); }
       else if (a.exch_mode == 1) { if (reg) UM_F4(true, 1); else UM_F4(false, 1); }
-      else { if (reg) UM_F4(true, 2); else UM_F4(false, 2); }
+      else if (a.exch_mode == 2) { if (reg) UM_F4(true, 2); else UM_F4(false, 2); }
+      else { if (reg) UM_F4(true, 3); else UM_F4(false, 3); }
     }
 #undef UM_F4
     UM_HIP(hipGetLastError());

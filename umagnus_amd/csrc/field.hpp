@@ -465,6 +465,45 @@ __device__ __forceinline__ V3<T> dmi_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb,
   return h;
 }
 
+// exchange + bulk DMI from gathered neighbours (dmibulk.cl:31-235): the same
+// sums as dmibulk_cell; a neighbour outside the mesh (inside false) is passed
+// as zero and extrapolated from the boundary condition unless OpenBC
+template <typename T, bool REG>
+__device__ __forceinline__ V3<T> dmibulk_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb, int r0, const int (&rn)[6],
+                                            const T *aLUT, const T *dLUT, T A0, T D0, T cx, T cy, T cz,
+                                            int openbc) {
+  V3<T> h = v3(T(0), T(0), T(0));
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    if (j >= nnb) break;
+    const int axis = j >> 1;
+    const T sgn = (j & 1) ? T(1) : T(-1);
+    const T c = axis == 0 ? cx : axis == 1 ? cy : cz;
+    V3<T> n = mn[j];
+    const bool present = !is0(n);
+    const int rj = present ? rn[j] : r0;
+    const T A = REG ? aLUT[symidx(r0, rj)] : A0;
+    const T D = REG ? dLUT[symidx(r0, rj)] : D0;
+    const T D_2A = D / (T(2.0) * A);
+    if (!present && openbc) continue;
+    if (!present) {
+      const T f = sgn * c * D_2A;
+      if (axis == 0) { n.x = m0.x; n.y = m0.y - (f * m0.z); n.z = m0.z + (f * m0.y); }
+      else if (axis == 1) { n.x = m0.x + (f * m0.z); n.y = m0.y; n.z = m0.z - (f * m0.x); }
+      else { n.x = m0.x - (f * m0.y); n.y = m0.y + (f * m0.x); n.z = m0.z; }
+    }
+    const T e = (T(2.0) * A / (c * c));
+    h.x += e * (n.x - m0.x);
+    h.y += e * (n.y - m0.y);
+    h.z += e * (n.z - m0.z);
+    const T dc = D / c;
+    if (axis == 0) { h.y += dc * (sgn * n.z); h.z -= dc * (sgn * n.y); }
+    else if (axis == 1) { h.x -= dc * (sgn * n.z); h.z += dc * (sgn * n.x); }
+    else { h.x += dc * (sgn * n.y); h.y -= dc * (sgn * n.x); }
+  }
+  return h;
+}
+
 // Inputs of a four-cell group (ix0..ix0+3, iy, iz): its m and region bytes and
 // those of the stencil neighbours.  Loading them is split from the arithmetic
 // so a caller can issue the loads early and overlap them with other work.
@@ -516,13 +555,15 @@ __device__ __forceinline__ void g4_load(const FieldArgs<T> &a, int ix0, int iy, 
   }
   if (!REG) {
     q.A0 = a.aLUT[0];
-    if (EX == 2) q.D0 = a.dLUT[0];
+    if (EX >= 2) q.D0 = a.dLUT[0];
   }
 }
 
 // Effective field and torque of the group from its loaded inputs; bdv: demag
 // field of the four cells (zero when absent).  Same term order as k_field_torque.
-template <typename T, bool REG, int EX>
+// EX: 0 none, 1 exchange, 2 + interfacial DMI, 3 + bulk DMI; STT: Zhang-Li and
+// Slonczewski torques (per the a.has_zl / a.has_sl flags).
+template <typename T, bool REG, int EX, bool STT = false>
 __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int iy, int iz, const G4In<T> &q,
                                            const T (&bdv)[3][4]) {
   const Mesh &g = a.g;
@@ -530,6 +571,7 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
   const int nnb = g.Nz != 1 ? 6 : 4;
   const bool inxl = (ix0 - 1 >= 0) || (g.pbc & 1), inxr = (ix0 + 4 < g.Nx) || (g.pbc & 1);
   const bool inyl = (iy - 1 >= 0) || (g.pbc & 2), inyh = (iy + 1 < g.Ny) || (g.pbc & 2);
+  const bool inzl = (iz - 1 >= 0) || (g.pbc & 4), inzh = (iz + 1 < g.Nz) || (g.pbc & 4);
   T ob[3][4], ot[3][4];
 #pragma unroll
   for (int w = 0; w < 4; w++) {
@@ -559,6 +601,15 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
       V3<T> h;
       if (EX == 1) {
         h = exchange_nb<T, REG>(m0, mn, nnb, q.rc[w], rn, a.aLUT, q.A0, a.wx, a.wy, a.wz);
+      } else if (EX == 3) {
+        const V3<T> zero = v3(T(0), T(0), T(0));
+        if (w == 0 && !inxl) mn[0] = zero;
+        if (w == 3 && !inxr) mn[1] = zero;
+        if (!inyl) mn[2] = zero;
+        if (!inyh) mn[3] = zero;
+        if (!inzl) mn[4] = zero;
+        if (!inzh) mn[5] = zero;
+        h = dmibulk_nb<T, REG>(m0, mn, nnb, q.rc[w], rn, a.aLUT, a.dLUT, q.A0, q.D0, a.cx, a.cy, a.cz, a.openbc);
       } else {
         const V3<T> zero = v3(T(0), T(0), T(0));
         if (w == 0 && !inxl) mn[0] = zero;
@@ -604,6 +655,40 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
       const V3<T> c = vcross(m0, vcross(m0, B));
       t = v3(-c.x, -c.y, -c.z);
     }
+    if constexpr (STT) {
+      // AddSTTorque (engine/torque.go:147-196), before FreezeSpins; the
+      // Zhang-Li differences use the clamped / wrapped neighbours as zhangli_cell
+      if (a.has_zl) {
+        const SttPar<T> &p = a.stt;
+        const T alpha = a.alpha.at(I), xi = p.xi.at(I), pol = p.pol.at(I);
+        const T invMs = inv_msat(a.Ms, I);
+        const T b = invMs * T(UM_MUB / (2 * UM_QE * UM_GAMMA0)) / (T(1.0) + xi * xi);
+        const V3<T> J = v3(pol * p.J[0].at(I), pol * p.J[1].at(I), pol * p.J[2].at(I));
+        V3<T> hs = v3(T(0), T(0), T(0));
+        auto add = [&](V3<T> ma, V3<T> mc, T f) {
+          hs.x += f * (ma.x - mc.x);
+          hs.y += f * (ma.y - mc.y);
+          hs.z += f * (ma.z - mc.z);
+        };
+        const V3<T> xl = w > 0 ? v3(q.mc[0][w - 1], q.mc[1][w - 1], q.mc[2][w - 1]) : v3(q.mxl[0], q.mxl[1], q.mxl[2]);
+        const V3<T> xr = w < 3 ? v3(q.mc[0][w + 1], q.mc[1][w + 1], q.mc[2][w + 1]) : v3(q.mxr[0], q.mxr[1], q.mxr[2]);
+        if (J.x != T(0)) add(xr, xl, (b / a.cx) * J.x);
+        if (J.y != T(0))
+          add(v3(q.myh[0][w], q.myh[1][w], q.myh[2][w]), v3(q.myl[0][w], q.myl[1][w], q.myl[2][w]), (b / a.cy) * J.y);
+        if (J.z != T(0) && nnb == 6)
+          add(v3(q.mzh[0][w], q.mzh[1][w], q.mzh[2][w]), v3(q.mzl[0][w], q.mzl[1][w], q.mzl[2][w]), (b / a.cz) * J.z);
+        const V3<T> mh = vcross(m0, hs), mmh = vcross(m0, mh);
+        const T gg = T(-1.0) / (T(1.0) + alpha * alpha);
+        const T c1 = T(1.0) + xi * alpha, c2 = xi - alpha;
+        t.x += gg * (c1 * mmh.x + c2 * mh.x);
+        t.y += gg * (c1 * mmh.y + c2 * mh.y);
+        t.z += gg * (c1 * mmh.z + c2 * mh.z);
+      }
+      if (a.has_sl) {
+        const V3<T> z = slonczewski_cell(I, m0, a.Ms, a.alpha, a.stt);
+        t.x += z.x; t.y += z.y; t.z += z.z;
+      }
+    }
     if (a.frozenLUT && a.frozenLUT[q.rc[w]] != T(0)) t = v3(T(0), T(0), T(0));
     ot[0][w] = t.x; ot[1][w] = t.y; ot[2][w] = t.z;
   }
@@ -617,11 +702,11 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
   }
 }
 
-template <typename T, bool REG, int EX>
+template <typename T, bool REG, int EX, bool STT = false>
 __device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int iy, int iz, const T (&bdv)[3][4]) {
   G4In<T> q;
-  g4_load<T, REG, EX>(a, ix0, iy, iz, q);
-  g4_compute<T, REG, EX>(a, ix0, iy, iz, q, bdv);
+  g4_load<T, REG, (STT && EX == 0) ? 1 : EX>(a, ix0, iy, iz, q);   // STT needs the neighbours
+  g4_compute<T, REG, EX, STT>(a, ix0, iy, iz, q, bdv);
 }
 
 template <typename T> void launch_field_torque(const FieldArgs<T> &a, hipStream_t s);
