@@ -1,0 +1,327 @@
+"""More of the reference's own test scripts (test/*.mx3) run through the
+simulation handle on the GPU, with the reference's expectations and
+tolerances: demag factors of films and rods, exchange field region averages,
+3D and rotated DMI walls, energy conservation and dissipation, Zeeman energy,
+frozen spins, standard problem 4b with the brute-force tensor, and the
+Slonczewski / Zhang-Li standard problems 5b, 5c (2D and 3D) and their Newell
+variants.  Scripts that need extension functions (ext_*) or the script RNG are
+not included.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from cases import MU0, coords, setup, two_domain, uniform_m, vortex
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    from umagnus_amd import umhip
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    umhip.init(0)
+    return umhip
+
+
+def _demag_factors(gpu, N, cs, prec=0):
+    dt = np.float32 if prec == 0 else np.float64
+    s = gpu.Sim(N, cs, dtype=dt)
+    setup(s, gpu, Ms=1 / MU0, Aex=1e-11, alpha=0.1, dtype=dt)
+    out = []
+    for v in np.eye(3):
+        s.set_m(uniform_m(N, v))
+        out.append(s.demag_field().reshape(3, -1).astype(np.float64).mean(axis=1))
+    return np.array(out)       # row: m direction, column: B component
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_demag2d_film(gpu, prec):
+    """test/demag2D.mx3: 128 x 128 x 1 flat cells, kxx, kyy ~ 0 (1e-2), kzz ~ -1 (2e-2), off-diagonal 1e-9."""
+    k = _demag_factors(gpu, (128, 128, 1), (1e-9, 1e-9, 0.5e-9), prec)
+    assert np.abs(k[~np.eye(3, dtype=bool)]).max() < 1e-9, k
+    assert abs(k[0, 0]) < 1e-2 and abs(k[1, 1]) < 1e-2 and abs(k[2, 2] + 1) < 2e-2, k
+
+
+def test_demag3d_film(gpu):
+    """test/demag3Dfilm.mx3: 64 x 48 x 2; kxx, kyy ~ 0 (3e-2), kzz ~ -1 (6e-2)."""
+    k = _demag_factors(gpu, (64, 48, 2), (1e-9, 1.2e-9, 0.5e-9))
+    assert np.abs(k[~np.eye(3, dtype=bool)]).max() < 1e-9, k
+    assert abs(k[0, 0]) < 3e-2 and abs(k[1, 1]) < 3e-2 and abs(k[2, 2] + 1) < 6e-2, k
+
+
+def test_demag_rod(gpu):
+    """test/demagrod.mx3: 2 x 2 x 64 cells of 1 x 1 x 2 nm: diag (-0.496, -0.496, 0), 1e-2."""
+    k = _demag_factors(gpu, (2, 2, 64), (1e-9, 1e-9, 2e-9))
+    assert np.abs(np.diag(k) - np.array([-0.496, -0.496, 0])).max() < 1e-2, k
+    assert np.abs(k[~np.eye(3, dtype=bool)]).max() < 1e-2, k
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_exchange_region_averages(gpu, prec):
+    """test/exchange.mx3: 300 x 200 x 4 cells of 2 x 3 x 4 nm, twodomain(1,2,3,
+    -1,-3,5, 3,-1,2), demag off; <B_exch> = 0 and its averages over x < 0 and
+    x >= 0 (tolerance 1.2e-8 T)."""
+    _say("exchange.mx3", prec)
+    N, cs = (300, 200, 4), (2e-9, 3e-9, 4e-9)
+    dt = np.float32 if prec == 0 else np.float64
+    s = gpu.Sim(N, cs, dtype=dt)
+    setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=0.1, demag=False, dtype=dt)
+    s.set_m(two_domain(N, cs, (1, 2, 3), (-1, -3, 5), (3, -1, 2)))
+    B = s.beff().astype(np.float64)
+    X, _, _ = coords(N, cs)
+    tol = 1.2e-8
+    assert np.abs(B.reshape(3, -1).mean(axis=1)).max() < tol
+    b0 = B[:, X < 0].mean(axis=1)
+    b1 = B[:, X >= 0].mean(axis=1)
+    assert np.abs(b0 - np.array([0.00118888104756673, -0.00164631487528483, -0.00074591267903645])).max() < tol, b0
+    assert np.abs(b1 - np.array([-0.00118888041178385, 0.00164631512959798, 0.00074591242472330])).max() < tol, b1
+
+
+def test_exchange3d_vortex_relax(gpu):
+    """test/exchange3d.mx3: 32 x 32 x 8 cubes of 100/32 nm, vortex(1, 1), relax():
+    <mz> = 0.02785073 (1e-5)."""
+    N = (32, 32, 8)
+    c = 100e-9 / 32
+    s = gpu.Sim(N, (c, c, c), dtype=np.float32)
+    setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=3.0, dtype=np.float32)
+    s.set_m(vortex(N, (c, c, c), 1, 1))
+    s.relax()
+    mz = s.average_m()[2]
+    _say("mz", mz)
+    assert abs(mz - 0.02785073) < 1e-5, mz
+
+
+def test_dmi3d_wall(gpu):
+    """test/dmi3d.mx3: 64 x 64 x 2 cells, interfacial DMI turns a Bloch wall
+    towards Neel; m at cell (32, 32, 0) after 0.5 ns = (0.8979, 0.3989, 0.1859), 1e-3."""
+    N, c = (64, 64, 2), 0.6e-9
+    cs = (c, c, c / 2)
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    setup(s, gpu, Ms=1100e3, Aex=16e-12, alpha=3.0, Ku1=1.27e6, anisU=(0, 0, 1), Dind=0.1e-3, dtype=np.float32)
+    s.set_m(two_domain(N, cs, (0, 0, -1), (1, 1, 0), (0, 0, 1)))
+    s.run(0.5e-9)
+    got = s.get_m()[:, 0, 32, 32]
+    _say("m(32,32,0)", got)
+    assert np.abs(got - np.array([0.8979, 0.3989, 0.1859])).max() < 1e-3, got
+
+
+def _rotz(N, cs, cfg, theta):
+    """Config.RotZ (engine/config.go:207-218) of a two-domain configuration."""
+    X, Y, Z = coords(N, cs)
+    co, si = math.cos(theta), math.sin(theta)
+    Xr, Yr = X * co + Y * si, -X * si + Y * co
+    m1, mw, m2 = cfg
+    ww = 2 * cs[0]
+    gauss = np.exp(-(Xr / ww) ** 2)
+    m = np.empty((3,) + X.shape)
+    for c in range(3):
+        base = np.where(Xr < 0, m1[c], m2[c])
+        m[c] = (1 - gauss) * base + gauss * mw[c]
+    mx, my = m[0] * co - m[1] * si, m[0] * si + m[1] * co
+    return np.stack([mx, my, m[2]])
+
+
+def test_dmi_y_wall(gpu):
+    """test/dmiY.mx3: test/dmi.mx3 rotated by 90 degrees (y derivatives): 128 x 128 x 1,
+    MaxDt 5e-13, 5 ns; <m> = (-0.035, 0.059) within 0.01."""
+    N = (128, 128, 1)
+    cs = (250e-9 / 128, 250e-9 / 128, 0.6e-9)
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    setup(s, gpu, Ms=1100e3, Aex=16e-12, alpha=3.0, Ku1=1.27e6, anisU=(0, 0, 1), Dind=0.1e-3, dtype=np.float32)
+    s.set_solver_params(maxdt=5e-13)
+    s.set_m(_rotz(N, cs, ((0, 0, -1), (1, 1, 0), (0, 0, 1)), math.pi / 2))
+    s.run(5e-9)
+    got = s.average_m()
+    _say("m", got)
+    assert abs(got[0] + 0.035) < 0.01 and abs(got[1] - 0.059) < 0.01, got
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_anisotropy_energy_conservation(gpu, prec):
+    """test/anisenergy.mx3: 32 x 10 x 2, no demag, Ku1 along z: the energy change
+    without damping over 1 ns is below 1e-6 of the damped one."""
+    _say("anisenergy", prec)
+    N, c = (32, 10, 2), 1e-9
+    dt = np.float32 if prec == 0 else np.float64
+    s = gpu.Sim(N, (c, 2 * c, 3 * c), dtype=dt)
+    setup(s, gpu, Ms=1000e3, Aex=10e-12, alpha=1.0, Ku1=1e6, anisU=(0, 0, 1), demag=False, dtype=dt)
+    s.set_m(uniform_m(N, (1, 0, 0.1)))
+    e0 = s.energy(9)
+    s.run(1e-9)
+    d1 = s.energy(9) - e0
+    s.set_m(uniform_m(N, (1, 0, 0.1)))
+    e0 = s.energy(9)
+    s.set_scalar(gpu.P_ALPHA, 0.0)
+    s.run(1e-9)
+    d2 = s.energy(9) - e0
+    _say("dE", d1, d2)
+    assert abs(d2 / d1) < 1e-6, (d1, d2)
+
+
+def test_zeeman_energy(gpu):
+    """test/zeemanenergy.mx3: E_zeeman = -B Ms V for B along m, 0 across, +B Ms V against (tol B Ms V / 1e5)."""
+    N, cs = (128, 32, 2), (5e-9, 4e-9, 3e-9)
+    V = np.prod(N) * np.prod(cs)
+    Ms, B = 100e3, 1e-3
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    setup(s, gpu, Ms=Ms, Aex=1e-11, alpha=0.1, dtype=np.float32)
+    s.set_m(uniform_m(N, (1, 0, 0)))
+    tol = B * Ms * V / 1e5
+    for b, want in (((B, 0, 0), -B * Ms * V), ((0, B, 0), 0.0), ((-B, 0, 0), B * Ms * V)):
+        s.set_vector(gpu.P_BEXT, b)
+        assert abs(s.energy(2) - want) < tol, (b, s.energy(2), want)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_dmi_energy_dissipation(gpu, prec):
+    """test/dmienergy.mx3: 64 x 64 x 1 film with DMI 3e-3, anisotropy and demag,
+    alpha 0.01 from uniform(1,1,1): E_total after 10 and 20 ps minus the start,
+    -2.62692723e-20 and -4.93337113e-20 J (tolerance 1e-23 J)."""
+    _say("dmienergy", prec)
+    N = (64, 64, 1)
+    cs = (1.15e-9 / 2, 1.15e-9 / 2, 1.15e-9)
+    dt = np.float32 if prec == 0 else np.float64
+    s = gpu.Sim(N, cs, dtype=dt)
+    setup(s, gpu, Ms=6e5, Aex=20e-12, alpha=0.01, Ku1=5.9e5, anisU=(0, 0, 1), Dind=3e-3, dtype=dt)
+    s.set_m(uniform_m(N, (1, 1, 1)))
+    e0 = s.energy(9)
+    s.run(10e-12)
+    e1 = s.energy(9)
+    s.run(10e-12)
+    e2 = s.energy(9)
+    _say("dE", e1 - e0, e2 - e0)
+    assert abs((e1 - e0) - -2.62692723e-20) < 0.001e-20, e1 - e0
+    assert abs((e2 - e0) - -4.93337113e-20) < 0.001e-20, e2 - e0
+
+
+def test_frozen_spins(gpu):
+    """test/frozenspins.mx3: standard problem 4 (3D) with every spin frozen:
+    relax() and 1 ns in the reversal field leave <m> at its start (1e-4)."""
+    N = (128, 32, 2)
+    cs = (500e-9 / 128, 125e-9 / 32, 3e-9 / 2)
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=0.02, frozen=np.ones(256, np.float32), dtype=np.float32)
+    s.set_m(uniform_m(N, (1, 0.1, 0)))
+    want = np.array([0.9950371980667114, 0.09950374066829681, 0])
+    s.relax()
+    assert np.abs(s.average_m() - want).max() < 1e-4, s.average_m()
+    s.set_vector(gpu.P_BEXT, (-24.6e-3, 4.3e-3, 0))
+    s.run(1e-9)
+    assert np.abs(s.average_m() - want).max() < 1e-4, s.average_m()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_standard_problem4b(gpu, prec):
+    """test/standardproblem4b.mx3: 128 x 32 x 1 (brute-force tensor), relax() then
+    B_ext = (-35.5, -6.3, 0) mT for 1 ns; TOL 1e-3."""
+    _say("sp4b", prec)
+    N = (128, 32, 1)
+    dt = np.float32 if prec == 0 else np.float64
+    s = gpu.Sim(N, (500e-9 / 128, 125e-9 / 32, 3e-9), dtype=dt)
+    setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=0.02, dtype=dt)
+    s.set_m(uniform_m(N, (1, 0.1, 0)))
+    s.relax()
+    got = s.average_m()
+    assert np.abs(got - np.array([0.9669684171676636, 0.1252732276916504, 0])).max() < 1e-3, got
+    s.set_vector(gpu.P_BEXT, (-35.5e-3, -6.3e-3, 0))
+    s.run(1e-9)
+    got = s.average_m()
+    _say("m", got)
+    assert np.abs(got - np.array([-0.9692331552505493, -0.11889542198181152, -0.0053076110780239105])).max() < 1e-3
+
+
+def _sp5bc(gpu, N, dtype, epsp, lam=2.0):
+    """The Slonczewski film of test/std5b.mx3 / std5c.mx3: 160 x 80 x 5 nm,
+    I = -6 mA through the film area, fixed layer at 20 degrees in-plane."""
+    length, width, thick = 160e-9, 80e-9, 5e-9
+    s = gpu.Sim(N, (length / N[0], width / N[1], thick / N[2]), dtype=dtype)
+    setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=0.01, dtype=dtype)
+    s.set_scalar(gpu.P_POL, 0.5669)
+    s.set_scalar(gpu.P_LAMBDA, lam)
+    s.set_scalar(gpu.P_EPSPRIME, epsp)
+    th = math.pi * 20 / 180
+    s.set_vector(gpu.P_FIXEDLAYER, (math.cos(th), math.sin(th), 0))
+    s.set_vector(gpu.P_J, (0, 0, -0.006 / (length * width)))
+    return s
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_std5b_epsilon_prime(gpu, prec):
+    """test/std5b.mx3: RK45DP with MaxErr 1e-6, MinDt 1e-18, MaxDt 1e-12,
+    gammaLL 2.211e5/mu0, DemagAccuracy 29, epsilon' = 1: <m> after 1 ns vs OOMMF
+    (-0.953323544827031, -0.279948071263437, 0.00528093273512820), TOL 1e-5."""
+    _say("std5b", prec)
+    N = (64, 32, 1)
+    dt = np.float32 if prec == 0 else np.float64
+    s = _sp5bc(gpu, N, dt, 1.0)
+    s.set_demag_accuracy(29)
+    s.set_solver_params(maxerr=1e-6, mindt=1e-18, maxdt=1e-12, gammaLL=2.211e5 / MU0)
+    s.set_m(uniform_m(N, (1, 0.1, 0)))
+    s.run(1e-9)
+    got = s.average_m()
+    _say("m", got)
+    assert np.abs(got - np.array([-0.953323544827031, -0.279948071263437, 0.00528093273512820])).max() < 1e-5, got
+
+
+@pytest.mark.parametrize("N,tol", [((64, 32, 1), 1e-5), ((64, 32, 2), 1e-2)])
+def test_std5c(gpu, N, tol):
+    """test/std5c.mx3 (2D, TOL 1e-5) and std5c3d.mx3 (3D, TOL 1e-2): epsilon' = 0,
+    0.5 ns from uniform(1, 0, 0) vs OOMMF (0.9056, -0.2861, -0.0110)."""
+    _say("std5c", N)
+    s = _sp5bc(gpu, N, np.float32, 0.0)
+    s.set_m(uniform_m(N, (1, 0, 0)))
+    s.run(0.5e-9)
+    got = s.average_m()
+    _say("m", got)
+    assert np.abs(got - np.array([0.905612587928772, -0.2860856354236603, -0.011005667969584465])).max() < tol, got
+
+
+def test_standard_problem5_newell(gpu):
+    """test/standardproblem5_newell.mx3: standard problem 5 with the Newell tensor; TOL 1e-4."""
+    N, cs = (32, 32, 4), (100e-9 / 32, 100e-9 / 32, 10e-9 / 4)
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=0.1, dtype=np.float32)
+    s.set_demag_newell(1)
+    s.set_m(vortex(N, cs, 1, 1))
+    s.relax()
+    s.set_scalar(gpu.P_XI, 0.05)
+    s.set_vector(gpu.P_J, (1e12, 0, 0))
+    s.set_scalar(gpu.P_POL, 1.0)
+    s.run(1e-9)
+    got = s.average_m()
+    _say("m", got)
+    assert np.abs(got - np.array([-0.234637409, -0.09450896, 0.02294345])).max() < 1e-4, got
+
+
+def test_standard_problem5a_newell(gpu):
+    """test/standardproblem5a_newell.mx3: minimize() with the Newell tensor, then
+    Slonczewski reversal for 1 ns; TOL 1e-3."""
+    N = (64, 32, 1)
+    sX, sY, sZ = 160e-9, 80e-9, 5e-9
+    s = gpu.Sim(N, (sX / N[0], sY / N[1], sZ / N[2]), dtype=np.float32)
+    setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=3.0, dtype=np.float32)
+    s.set_demag_newell(1)
+    s.set_m(uniform_m(N, (1, 1, 0.001)))
+    s.minimize()
+    got = s.average_m()
+    assert np.abs(got - np.array([0.9585264325141907, 0.20422574877738953, 0])).max() < 1e-3, got
+    s.set_scalar(gpu.P_ALPHA, 0.01)
+    s.set_scalar(gpu.P_LAMBDA, 1.0)
+    s.set_scalar(gpu.P_POL, 0.5669)
+    s.set_scalar(gpu.P_EPSPRIME, 0.0)
+    prad = math.pi / 180
+    s.set_vector(gpu.P_FIXEDLAYER, (math.cos(prad), math.sin(prad), 0))
+    s.set_vector(gpu.P_J, (0, 0, -0.008 / (sX * sY)))
+    s.run(1e-9)
+    got = s.average_m()
+    _say("m", got)
+    assert np.abs(got - np.array([0.6427677195465802, 0.5114272236824036, -0.1541234254837036])).max() < 1e-3, got
