@@ -2,10 +2,12 @@
 simulation handle on the GPU, with the reference's expectations and
 tolerances: demag factors of films and rods, exchange field region averages,
 3D and rotated DMI walls, energy conservation and dissipation, Zeeman energy,
-frozen spins, standard problem 4b with the brute-force tensor, and the
+frozen spins, standard problem 4b with the brute-force tensor, the
 Slonczewski / Zhang-Li standard problems 5b, 5c (2D and 3D) and their Newell
-variants.  Scripts that need extension functions (ext_*) or the script RNG are
-not included.
+variants, inter-region exchange scaling (ext_ScaleExchange, a LUT entry here),
+uniaxial and cubic anisotropy energies of one cell, and the OpenBC edge
+canting of a DMI chain.  Scripts that need other extension functions (ext_*)
+or the script RNG are not included.
 """
 import math
 import sys
@@ -325,3 +327,131 @@ def test_standard_problem5a_newell(gpu):
     got = s.average_m()
     _say("m", got)
     assert np.abs(got - np.array([0.6427677195465802, 0.5114272236824036, -0.1541234254837036])).max() < 1e-3, got
+
+
+def _sp4_regions(gpu, scale, alpha):
+    """128 x 32 x 1 standard-problem-4 film split into regions 1 (y >= 0) and 2
+    (y < 0) with the inter-region exchange scaled (ext_ScaleExchange)."""
+    from umagnus_amd.params import exchange_lut
+    N, cs = (128, 32, 1), (500e-9 / 128, 125e-9 / 32, 3e-9)
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    _, Y, _ = coords(N, cs)
+    reg = np.where(Y >= 0, 1, 2).astype(np.uint8)
+    setup(s, gpu, Ms=800e3, Aex=13e-12, alpha=alpha, regions=reg, dtype=np.float32)
+    s.set_exchange_luts(exchange_lut(13e-12, scale={(1, 2): scale}, dtype=np.float32), None)
+    s.set_m(uniform_m(N, (1, 0.1, 0)))
+    return s
+
+
+def test_interexchange_afm(gpu):
+    """test/interexchange.mx3: the two halves coupled antiferromagnetically
+    (scale -1), alpha 3, 10 ns: <m> = (0, 0.114170685410499, 0), 1e-5."""
+    s = _sp4_regions(gpu, -1.0, 3.0)
+    s.run(10e-9)
+    got = s.average_m()
+    _say("m", got)
+    assert np.abs(got - np.array([0, 0.114170685410499, 0])).max() < 1e-5, got
+
+
+def test_interexchange_scale_one(gpu):
+    """test/interexchange2.mx3: scale 1 reproduces standard problem 4; TOL 1e-3."""
+    s = _sp4_regions(gpu, 1.0, 0.02)
+    s.relax()
+    assert np.abs(s.average_m() - np.array([0.96696, 0.12528, 0])).max() < 1e-3, s.average_m()
+    s.set_vector(gpu.P_BEXT, (-24.6e-3, 4.3e-3, 0))
+    s.run(1e-9)
+    got = s.average_m()
+    _say("m", got)
+    assert np.abs(got - np.array([-0.9845, 0.1268, 0.0432])).max() < 1e-3, got
+
+
+def test_uniaxial_anisotropy_relax(gpu):
+    """test/uniaxialanisotropy.mx3: easy axis along x against a field along y,
+    relax() at each field, <my> vs OOMMF (TOL 1e-3); regions 2 and 3 are
+    defined but carry the same parameters."""
+    N, cs = (64, 64, 1), (4e-9, 4e-9, 2e-9)
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    X, Y, _ = coords(N, cs)
+    reg = np.zeros(X.shape, np.uint8)
+    reg[(X / 100e-9) ** 2 + (Y / 100e-9) ** 2 <= 0.25] = 2
+    reg[(np.abs(X) < 50e-9) & (np.abs(Y) < 10e-9)] = 3
+    setup(s, gpu, Ms=1100e3, Aex=13e-12, alpha=1.0, Ku1=0.5e6, anisU=(1, 0, 0), regions=reg, dtype=np.float32)
+    s.set_m(uniform_m(N, (1, 1, 0)))
+    for B, my in ((0.0, 0.0), (0.01, 0.011), (0.03, 0.033), (0.10, 0.110), (0.30, 0.331)):
+        s.set_vector(gpu.P_BEXT, (0, B, 0))
+        s.relax()
+        assert abs(s.average_m()[1] - my) < 1e-3, (B, s.average_m())
+
+
+def _one_cell(gpu, m):
+    s = gpu.Sim((1, 1, 1), (1e-9,) * 3, dtype=np.float32)
+    setup(s, gpu, Ms=1000e3, Aex=0.0, alpha=0.1, dtype=np.float32)
+    s.set_m(uniform_m((1, 1, 1), m))
+    return s
+
+
+def test_uniaxial_energy_one_cell(gpu):
+    """test/uniaxial_full.mx3: one cell at 17 degrees from the x axis; E_anis / V
+    for Ku2 = +-1e6 and Ku1 = +-1e6 (TOL 1 J/m^3)."""
+    V = 1e-27
+    th = 17 * math.pi / 180
+    s = _one_cell(gpu, (math.cos(th), math.sin(th), 0))
+    s.set_vector(gpu.P_ANISU, (1, 0, 0))
+    ean = lambda: sum(s.energy(t) for t in (3, 4, 5, 6, 7)) / V
+    for k1, k2, want in ((0, 1e6, -836344.9375), (0, -1e6, 836344.9375), (1e6, 0, -914519), (-1e6, 0, 914519)):
+        s.set_scalar(gpu.P_KU1, k1)
+        s.set_scalar(gpu.P_KU2, k2)
+        assert abs(ean() - want) < 1, (k1, k2, ean())
+
+
+def test_cubic_energy_one_cell(gpu):
+    """test/cubic_full.mx3: one cell at (theta, phi) = (23, 11) degrees; E_anis / V
+    for each of Kc1, Kc2, Kc3 = +-1e6 (TOL 1e-2 J/m^3)."""
+    V = 1e-27
+    th, ph = 23 * math.pi / 180, 11 * math.pi / 180
+    s = _one_cell(gpu, (math.sin(ph) * math.cos(th), math.sin(ph) * math.sin(th), math.cos(ph)))
+    s.set_vector(gpu.P_ANISC1, (1, 0, 0))
+    s.set_vector(gpu.P_ANISC2, (0, 1, 0))
+    ean = lambda: sum(s.energy(t) for t in (3, 4, 5, 6, 7)) / V
+    cases = (((1e6, 0, 0), 35254.007), ((-1e6, 0, 0), -35254.007), ((0, 1e6, 0), 165.23309326171875),
+             ((0, -1e6, 0), -165.23309326171875), ((0, 0, 1e6), 912.3794555664062))
+    for (k1, k2, k3), want in cases:
+        s.set_scalar(gpu.P_KC1, k1)
+        s.set_scalar(gpu.P_KC2, k2)
+        s.set_scalar(gpu.P_KC3, k3)
+        assert abs(ean() - want) < 1e-2, ((k1, k2, k3), ean())
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_open_bc_edge_canting(gpu, axis):
+    """test/openbc.mx3: a chain of 1024 cells (along x, then along y) with DMI at
+    90% of the critical strength, minimized (MinimizerStop 1e-7): with OpenBC the
+    edge canting matches the 1D analytic value (1e-3) and the periodic chain's
+    (1e-5), in the reduced units of the script."""
+    from umagnus_amd.params import exchange_lut
+    ncell, c = 1024, 0.05
+    N = (ncell, 1, 1) if axis == 0 else (1, ncell, 1)
+    D = 0.9 * 4 / math.pi
+
+    def make(pbc, openbc):
+        s = gpu.Sim(N, (c, c, c), pbc=pbc, dtype=np.float64)
+        setup(s, gpu, Ms=1.0, Aex=1.0, alpha=0.1, Ku1=1.0, anisU=(0, 0, 1), Dind=D, demag=False, openbc=openbc,
+              dtype=np.float64)
+        return s
+
+    def cant(s):
+        m = s.get_m()[:, 0, 0, 0]
+        return math.atan2(m[axis], m[2])
+    theta0 = math.asin(D / 2)
+    analytic = 2 * math.atan(math.exp(-c / 2) * math.tan(theta0 / 2))
+    s = make((0, 0, 0), True)
+    s.set_m(uniform_m(N, (0, 0, 1)))
+    s.minimize(stop=1e-7)
+    c_open = cant(s)
+    sp = make((0, 1, 0) if axis == 0 else (1, 0, 0), False)
+    sp.set_m(s.get_m())
+    sp.minimize(stop=1e-7)
+    c_per = cant(sp)
+    _say("cant", c_open, c_per, analytic)
+    assert abs(c_open - analytic) < 1e-3
+    assert abs(c_open - c_per) < 1e-5
