@@ -5,8 +5,9 @@ tolerances: demag factors of films and rods, exchange field region averages,
 frozen spins, standard problem 4b with the brute-force tensor, the
 Slonczewski / Zhang-Li standard problems 5b, 5c (2D and 3D) and their Newell
 variants, inter-region exchange scaling (ext_ScaleExchange, a LUT entry here),
-uniaxial and cubic anisotropy energies of one cell, and the OpenBC edge
-canting of a DMI chain.  Scripts that need other extension functions (ext_*)
+uniaxial and cubic anisotropy energies of one cell, the OpenBC edge canting
+of a DMI chain, the minimizer, the isotropy of a sphere's energy, a thermal
+equilibrium under RK4 and demag with regions.  Scripts that need other extension functions (ext_*)
 or the script RNG are not included.
 """
 import math
@@ -455,3 +456,82 @@ def test_open_bc_edge_canting(gpu, axis):
     _say("cant", c_open, c_per, analytic)
     assert abs(c_open - analytic) < 1e-3
     assert abs(c_open - c_per) < 1e-5
+
+
+def test_minimizer_sp4(gpu):
+    """test/minimizer.mx3: standard problem 4 (2D) minimized from uniform(1, .1, 0)
+    after a Msat change (the exchange LUT must not keep the old Msat):
+    <m> = (0.9669684171676636, 0.1252817150004892, 0), TOL 1e-5."""
+    N = (128, 32, 1)
+    s = gpu.Sim(N, (500e-9 / 128, 125e-9 / 32, 3e-9), dtype=np.float32)
+    setup(s, gpu, Ms=1600e3, Aex=13e-12, alpha=0.02, dtype=np.float32)
+    s.set_m(uniform_m(N, (1, 0.1, 0)))
+    s.energy(9)
+    s.set_scalar(gpu.P_MSAT, 800e3)
+    s.minimize()
+    got = s.average_m()
+    _say("m", got, s.state()["nsteps"])
+    assert np.abs(got - np.array([0.9669684171676636, 0.1252817150004892, 0])).max() < 1e-5, got
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_isotropic_energy_sphere(gpu, prec):
+    """test/isotropicEnergy.mx3: a sphere of 32^3 cubes (Ellipsoid geometry): the
+    total energy of uniform states does not depend on their direction (1e-6)."""
+    _say("isotropic", prec)
+    N, c = (32, 32, 32), 2e-9
+    cs = (c, c, c)
+    dt = np.float32 if prec == 0 else np.float64
+    X, Y, Z = coords(N, cs)
+    vol = ((X / (N[0] * c)) ** 2 + (Y / (N[1] * c)) ** 2 + (Z / (N[2] * c)) ** 2 <= 0.25).astype(np.float64)
+    s = gpu.Sim(N, cs, dtype=dt)
+    setup(s, gpu, Ms=860e3, Aex=13e-12, alpha=1e-5, vol=vol, dtype=dt)
+    s.set_m(uniform_m(N, (1, 0, 0)))
+    e0 = s.energy(9)
+    phi = 0.0
+    worst = 0.0
+    while phi < 4 * math.pi:
+        th = phi * 1.234
+        s.set_m(uniform_m(N, (math.sin(phi) * math.cos(th), math.cos(phi) * math.cos(th), math.sin(th))))
+        worst = max(worst, abs((s.energy(9) - e0) / e0))
+        phi += 0.1
+    _say("worst", worst)
+    assert worst < 1e-6, worst
+
+
+def test_rk4_temperature(gpu):
+    """test/rk4temperature.mx3: 768 x 768 x 1 cells of 10 nm, no exchange or
+    demag, Ku1 = 2500 along z, T = 100 K, RK4 with FixDt 2e-12 for 5 ns:
+    <m> = (0, 0, 0.8366) within 1e-3 (a statistical equilibrium: the Philox
+    stream differs from the reference's generator)."""
+    N, c = (768, 768, 1), 10e-9
+    s = gpu.Sim(N, (c, c, c), dtype=np.float32)
+    setup(s, gpu, Ms=1e6, Aex=0.0, alpha=0.1, Ku1=1e4 / 4, anisU=(0, 0, 1), temp=100.0, seed=7, demag=False,
+          dtype=np.float32)
+    s.set_solver(gpu.SOLVER_RK4)
+    s.set_solver_params(dt=2e-12, fixdt=2e-12)
+    s.set_m(uniform_m(N, (0, 0, 1)))
+    s.run(5e-9)
+    got = s.average_m()
+    _say("m", got)
+    assert np.abs(got - np.array([0, 0, 0.8366])).max() < 1e-3, got
+
+
+def test_demag_regions(gpu):
+    """test/demagRegion.mx3: 64 x 64 x 2 with three regions of Msat 2/mu0:
+    kxx, kyy ~ 0 and kzz ~ -2 (0.1), off-diagonal 1e-8; Msat 1/mu0: kzz ~ -1."""
+    N, cs = (64, 64, 2), (1e-9, 1e-9, 0.5e-9)
+    X, Y, _ = coords(N, cs)
+    reg = np.where(X >= 0, 1, 2).astype(np.uint8)
+    reg[(X / 10e-9) ** 2 + (Y / 20e-9) ** 2 <= 0.25] = 3
+    s = gpu.Sim(N, cs, dtype=np.float32)
+    setup(s, gpu, Ms=2 / MU0, Aex=1e-11, alpha=0.1, regions=reg, dtype=np.float32)
+    k = []
+    for v in np.eye(3):
+        s.set_m(uniform_m(N, v))
+        k.append(s.demag_field().reshape(3, -1).astype(np.float64).mean(axis=1))
+    k = np.array(k)
+    assert np.abs(k[~np.eye(3, dtype=bool)]).max() < 1e-8, k
+    assert abs(k[0, 0]) < 0.1 and abs(k[1, 1]) < 0.1 and abs(k[2, 2] + 2) < 0.1, k
+    s.set_scalar(gpu.P_MSAT, 1 / MU0)
+    assert abs(s.demag_field()[2].astype(np.float64).mean() + 1) < 0.1
