@@ -242,6 +242,10 @@ int um_sim_set_exchange_luts(void *sim, const void *aLUT, const void *dLUT);
 int um_sim_set_bext_lut(void *sim, const void *lx, const void *ly, const void *lz);
 /* FrozenSpins per region (engine/torque.go:348-352), 256 values, NULL clears */
 int um_sim_set_frozen_lut(void *sim, const void *lut);
+/* NoDemagSpins per region (engine/demag.go:22-23, setMaskedDemagField
+ * :67-98): flagged cells neither generate nor receive the demag field;
+ * 256 values, NULL clears */
+int um_sim_set_nodemag_lut(void *sim, const void *lut);
 /* geometry (engine/geom.go): per-cell volume fraction, NULL = full */
 int um_sim_set_geometry(void *sim, const void *vol);
 /* EnableDemag, OpenBC, DoPrecess, and the MaxVecDiff sqrt(3) widening in Heun */

@@ -276,6 +276,11 @@ class Sim:
         ls = [np.ascontiguousarray(l, self.dtype) for l in lut3]
         self._f("sim_set_bext_lut")(self._h(), ptr(ls[0]), ptr(ls[1]), ptr(ls[2]))
 
+    def set_nodemag_lut(self, lut):
+        """NoDemagSpins per region (engine/demag.go:22-23, 67-98); None clears."""
+        self._f("sim_set_nodemag_lut")(self._h(), ptr(np.ascontiguousarray(lut, self.dtype)) if lut is not None
+                                       else None)
+
     def set_frozen_lut(self, lut):
         self._f("sim_set_frozen_lut")(self._h(), ptr(np.ascontiguousarray(lut, self.dtype)))
 

@@ -176,6 +176,7 @@ enum { UM_SOLVER_HEUN = 2, UM_SOLVER_RK23 = 3, UM_SOLVER_RK4 = 4, UM_SOLVER_RK45
   void um_or_sim_set_bext_lut##SFX(void *h, const real *lx, const real *ly,                \
         const real *lz);                                                                   \
   void um_or_sim_set_frozen_lut##SFX(void *h, const real *lut);                            \
+  void um_or_sim_set_nodemag_lut##SFX(void *h, const real *lut);                           \
   void um_or_sim_set_geometry##SFX(void *h, const real *vol);                              \
   void um_or_sim_set_flags##SFX(void *h, int enable_demag, int openbc, int precess,        \
         int heun_sqrt3);                                                                   \

@@ -996,6 +996,7 @@ typedef struct {
   real *aLUT, *dLUT;                 /* symmetric LUTs, UM_OR_SYMLUT entries */
   real *bextLUT[3];                  /* B_ext per region (NULL = none) */
   real *frozenLUT;
+  real *nodemagLUT;   /* NoDemagSpins per region (NULL: none set) */
   real *vol;
   int enable_demag, openbc, precess, heun_sqrt3, has_dmi;
   real *dbLUT; int has_dmibulk;      /* bulk DMI (Dbulk) */
@@ -1049,7 +1050,7 @@ void OR(sim_free)(void *h) {
   sim_t *s = (sim_t *)h;
   if (!s) return;
   for (int i = 0; i < UM_P_COUNT; i++) for (int c = 0; c < 3; c++) free(s->p[i][c].arr);
-  free(s->regions); free(s->aLUT); free(s->dLUT); free(s->dbLUT); free(s->frozenLUT); free(s->vol);
+  free(s->regions); free(s->aLUT); free(s->dLUT); free(s->dbLUT); free(s->frozenLUT); free(s->nodemagLUT); free(s->vol);
   for (int c = 0; c < 3; c++) free(s->bextLUT[c]);
   if (s->demag) OR(demag_free)(s->demag);
   free(s->m); free(s->k1);
@@ -1122,6 +1123,15 @@ void OR(sim_set_bext_lut)(void *h, const real *lx, const real *ly, const real *l
     s->bextLUT[c] = dupn(l[c], UM_OR_NREGION);
   }
 }
+/* NoDemagSpins (engine/demag.go:22-23): per-region flag, NULL clears */
+void OR(sim_set_nodemag_lut)(void *h, const real *lut) {
+  sim_t *s = (sim_t *)h;
+  free(s->nodemagLUT);
+  s->nodemagLUT = NULL;
+  if (!lut) return;
+  for (int r = 0; r < UM_OR_NREGION; r++)
+    if (lut[r] != 0) { s->nodemagLUT = dupn(lut, UM_OR_NREGION); break; }
+}
 void OR(sim_set_frozen_lut)(void *h, const real *lut) {
   sim_t *s = (sim_t *)h;
   free(s->frozenLUT);
@@ -1189,13 +1199,24 @@ static int param_nonzero(sim_t *s, int id, int nc) {
 #define PA_ARR(id) ((const real *)s->p[id][0].arr)
 #define PA_MUL(id) ((real)s->p[id][0].mul)
 
-/* SetDemagField, engine/demag.go:42-65 (NoDemagSpins not modelled) */
+/* SetDemagField, engine/demag.go:42-65; with NoDemagSpins set, the masked
+ * field of setMaskedDemagField (:67-98): the geometry (or 1) zeroed in the
+ * flagged regions is the convolution's vol, and the field is zeroed there. */
 static void set_demag(sim_t *s, real *B) {
   int n = s->n;
   if (!s->enable_demag) { memset(B, 0, sizeof(real) * 3 * n); return; }
   if (!s->demag)
     s->demag = s->newell ? sim_newell_plan(s) : OR(demag_new)(s->N, s->pbc, s->cs, s->demag_acc, 0);
-  OR(demag_exec)(s->demag, B, B + n, B + 2 * n, s->m, s->m + n, s->m + 2 * n, s->vol, PA(UM_P_MSAT, 0));
+  if (!s->nodemagLUT) {
+    OR(demag_exec)(s->demag, B, B + n, B + 2 * n, s->m, s->m + n, s->m + 2 * n, s->vol, PA(UM_P_MSAT, 0));
+    return;
+  }
+  real *buf = (real *)malloc(sizeof(real) * n);
+  for (int i = 0; i < n; i++) buf[i] = s->vol ? s->vol[i] : (real)1;
+  OR(zeromask)(buf, s->nodemagLUT, s->regions, n);
+  OR(demag_exec)(s->demag, B, B + n, B + 2 * n, s->m, s->m + n, s->m + 2 * n, buf, PA(UM_P_MSAT, 0));
+  free(buf);
+  for (int c = 0; c < 3; c++) OR(zeromask)(B + c * n, s->nodemagLUT, s->regions, n);
 }
 
 /* SetEffectiveField, engine/effectivefield.go:14-24 */

@@ -115,3 +115,40 @@ def cylinder_geom(N, cs, diam, height):
     """SetGeom(Cylinder(diam, height)), engine/shape.go:61-66."""
     X, Y, Z = coords(N, cs)
     return ((np.abs(Z) <= height / 2) & ((X / diam) ** 2 + (Y / diam) ** 2 <= 0.25)).astype(np.float64)
+
+
+def nodemag_script(mod, dtype):
+    """test/nodemagspins.mx3 on a simulation module (the oracle or the GPU one):
+    128 x 32 x 1 standard-problem-4 film, m = uniform(1, 1, 1), regions LEFT
+    (x < 0) and RIGHT (x >= 0).  Returns the checks as (label, got, want, tol)."""
+    N, cs = (128, 32, 1), (500e-9 / 128, 125e-9 / 32, 3e-9)
+    s = mod.Sim(N, cs, dtype=dtype)
+    X, Y, _ = coords(N, cs)
+    reg = np.where(X < 0, 0, 1).astype(np.uint8)
+    setup(s, mod, Ms=800e3, Aex=13e-12, alpha=0.02, regions=reg, dtype=dtype)
+    s.set_m(uniform_m(N, (1, 1, 1)))
+    out = []
+
+    def avg(r):
+        B = s.demag_field().astype(np.float64)
+        return B[:, reg == r].mean(axis=1)
+
+    def lut(left, right):
+        v = np.zeros(256)
+        v[0], v[1] = left, right
+        return v
+    full = np.array([-0.005299, -0.022127, -0.552248])
+    out += [("L", avg(0), full, 1e-4), ("R", avg(1), full, 1e-4)]
+    s.set_nodemag_lut(lut(0, 1))
+    out += [("L", avg(0), np.array([-0.0103589, -0.02122757, -0.5480939]), 1e-4), ("R", avg(1), np.zeros(3), 0.0)]
+    s.set_nodemag_lut(lut(1, 1))
+    out += [("L", avg(0), np.zeros(3), 0.0), ("R", avg(1), np.zeros(3), 0.0)]
+    # the air gap: geometry y < 0, region 2 for y >= 0
+    reg = np.where(Y >= 0, 2, reg).astype(np.uint8)
+    s.set_regions(reg)
+    s.set_geometry((Y < 0).astype(np.float64))
+    out += [("L", avg(0), np.zeros(3), 0.0), ("R", avg(1), np.zeros(3), 0.0), ("A", avg(2), np.zeros(3), 0.0)]
+    s.set_nodemag_lut(lut(0, 0))
+    half = np.array([-0.0046074, -0.0391400, -0.53594726])
+    out += [("L", avg(0), half, 1e-4), ("R", avg(1), half, 1e-4)]
+    return out

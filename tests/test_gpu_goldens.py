@@ -535,3 +535,33 @@ def test_demag_regions(gpu):
     assert abs(k[0, 0]) < 0.1 and abs(k[1, 1]) < 0.1 and abs(k[2, 2] + 2) < 0.1, k
     s.set_scalar(gpu.P_MSAT, 1 / MU0)
     assert abs(s.demag_field()[2].astype(np.float64).mean() + 1) < 0.1
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_nodemagspins(gpu, oracle, prec):
+    """test/nodemagspins.mx3: NoDemagSpins per region (flagged cells neither
+    generate nor receive the demag field), with and without an air gap; then
+    B_eff and torque of a random state with regions, geometry and flags vs the
+    oracle (FP64 1e-11, FP32 1e-5)."""
+    from cases import nodemag_script, random_unit
+    dt = np.float32 if prec == 0 else np.float64
+    for label, got, want, tol in nodemag_script(gpu, dt):
+        assert np.abs(got - want).max() <= tol, (label, got, want)
+    N, cs = (24, 16, 3), (2e-9, 3e-9, 2e-9)
+    rng = np.random.default_rng(3)
+    reg = rng.integers(0, 4, (N[2], N[1], N[0])).astype(np.uint8)
+    vol = (rng.uniform(size=reg.shape) > 0.1).astype(np.float64)
+    flags = np.zeros(256)
+    flags[[1, 3]] = 1
+    m = random_unit((N[2], N[1], N[0]), 8)
+    sims = []
+    for mod, d in ((gpu, dt), (oracle, np.float64)):
+        s = mod.Sim(N, cs, dtype=d)
+        setup(s, mod, Ms=7e5, Aex=1.1e-11, alpha=0.1, Ku1=2e4, anisU=(0, 1, 1), regions=reg, vol=vol, dtype=d)
+        s.set_nodemag_lut(flags)
+        s.set_m(m)
+        sims.append(s)
+    tol = 1e-11 if prec == 1 else 1e-5
+    for f in ("demag_field", "beff", "torque"):
+        a, b = getattr(sims[0], f)(), getattr(sims[1], f)()
+        assert np.abs(a - b).max() <= tol * np.abs(b).max(), f

@@ -532,3 +532,13 @@ def test_minimize_stoner_wohlfarth(oracle):
         assert abs(m[1] - B * Ms / (2 * K1)) < 1e-5, (B, m)
         assert abs(m[2]) < 1e-5
         assert s.state()["t"] == 0.0
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_nodemagspins_golden(oracle, dt):
+    """test/nodemagspins.mx3 on the oracle: NoDemagSpins per region removes the
+    flagged cells from the demag sources and zeroes their field
+    (engine/demag.go:67-98); region averages vs the reference's values."""
+    from cases import nodemag_script
+    for label, got, want, tol in nodemag_script(oracle, dt):
+        assert np.abs(got - want).max() <= tol, (label, got, want)

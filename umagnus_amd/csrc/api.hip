@@ -683,6 +683,9 @@ int um_sim_set_bext_lut(void *sim, const void *lx, const void *ly, const void *l
   });
 }
 int um_sim_set_frozen_lut(void *sim, const void *lut) { return guard([&] { SIMCHK(sim); SIM(sim)->set_frozen_lut(lut); }); }
+int um_sim_set_nodemag_lut(void *sim, const void *lut) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->set_nodemag_lut(lut); });
+}
 int um_sim_set_geometry(void *sim, const void *vol) { return guard([&] { SIMCHK(sim); SIM(sim)->set_geometry(vol); }); }
 int um_sim_set_flags(void *sim, int d, int ob, int pr, int hs) {
   return guard([&] { SIMCHK(sim); SIM(sim)->set_flags(d, ob, pr, hs); });

@@ -710,5 +710,7 @@ __device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int
 }
 
 template <typename T> void launch_field_torque(const FieldArgs<T> &a, hipStream_t s);
+// ZeroMask (opencl/zeromask.go): dst = 0 where lut[region] != 0
+template <typename T> void zeromask(void *dst, const void *lut, const uint8_t *regions, int N, hipStream_t s);
 
 }  // namespace um

@@ -19,6 +19,16 @@ static constexpr double MU0_S = 4 * M_PI * 1e-7;   // mag/constants.go:8
 static constexpr int NREGION = 256;
 static constexpr int SYMLUT = NREGION * (NREGION + 1) / 2;
 
+// demag input mask of NoDemagSpins (engine/demag.go:67-83): the geometry (or 1)
+// with the flagged regions zeroed
+template <typename T>
+__global__ void k_nodemag_vol(T *out, const T *vol, const T *lut, const uint8_t *regions, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const T v = vol ? vol[i] : T(1);
+    out[i] = lut[regions ? regions[i] : 0] != T(0) ? T(0) : v;
+  }
+}
+
 template <typename T>
 struct Sim : SimBase {
   int N[3];
@@ -34,6 +44,9 @@ struct Sim : SimBase {
   };
   PSlot p[UM_P_COUNT][3];
   DevBuf regions, aLUT, dLUT, dbLUT, bextLUT, frozenLUT, vol;
+  DevBuf nodemagLUT, ndvol;   // NoDemagSpins per region, and the masked demag input
+  bool has_nodemag = false;
+  long nd_gen = -1;
   bool has_regions = false, has_dmi = false, has_dmibulk = false, has_bextlut = false, has_frozen = false,
        has_vol = false;
   int disable_zl = 0, disable_sl = 0;   // DisableZhangLiTorque / DisableSlonczewskiTorque (engine/torque.go:26-27)
@@ -207,6 +220,13 @@ struct Sim : SimBase {
     upload(frozenLUT, l, sizeof(T) * NREGION);
     has_frozen = any_nonzero((const T *)l, NREGION);
   }
+  // NoDemagSpins (engine/demag.go:22-23)
+  void set_nodemag_lut(const void *l) override {
+    gen++;
+    if (!l) { nodemagLUT.release(); ndvol.release(); has_nodemag = false; return; }
+    upload(nodemagLUT, l, sizeof(T) * NREGION);
+    has_nodemag = any_nonzero((const T *)l, NREGION);
+  }
   void set_geometry(const void *v) override {
     gen++;
     if (v) { upload(vol, v, sizeof(T) * n); has_vol = true; }
@@ -334,13 +354,29 @@ struct Sim : SimBase {
     a.stt.flp = (T)flp;
     return a;
   }
-  // B_demag into bd (SetDemagField, engine/demag.go:42-65)
+  // B_demag into bd (SetDemagField, engine/demag.go:42-65); with NoDemagSpins
+  // the flagged cells neither generate nor receive the field (:67-98)
   void compute_demag(const T *mm) {
     if (!enable_demag) return;
     ensure_demag();
     void *B[3] = {bd, bd + n, bd + 2 * n};
     const void *mv[3] = {mm, mm + n, mm + 2 * n};
-    demag_exec(demag, B, mv, vol.p, p[UM_P_MSAT][0].arr.p, p[UM_P_MSAT][0].mul, st);
+    const void *v = vol.p;
+    if (has_nodemag) {
+      if (nd_gen != gen || !ndvol.p) {
+        if (!ndvol.p) ndvol.alloc(sizeof(T) * n);
+        hipLaunchKernelGGL(k_nodemag_vol<T>, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, st, ndvol.as<T>(),
+                           (const T *)vol.p, nodemagLUT.as<T>(), has_regions ? (const uint8_t *)regions.p : nullptr,
+                           (long)n);
+        UM_HIP(hipGetLastError());
+        nd_gen = gen;
+      }
+      v = ndvol.p;
+    }
+    demag_exec(demag, B, mv, v, p[UM_P_MSAT][0].arr.p, p[UM_P_MSAT][0].mul, st);
+    if (has_nodemag)
+      for (int c = 0; c < 3; c++)
+        zeromask<T>(bd + c * n, nodemagLUT.p, has_regions ? (const uint8_t *)regions.p : nullptr, n, st);
   }
   // torqueFn (engine/run.go:127-130): SetTorque into dst
   hipEvent_t next_event() {
@@ -383,7 +419,8 @@ struct Sim : SimBase {
       // one pipeline: B_demag stays on chip (the field share is timed with the demag)
       ensure_demag();
       const void *mv[3] = {m, m + n, m + 2 * n};
-      fused = demag_exec_field(demag, mv, vol.p, p[UM_P_MSAT][0].arr.p, p[UM_P_MSAT][0].mul, &a, st);
+      if (!has_nodemag)
+        fused = demag_exec_field(demag, mv, vol.p, p[UM_P_MSAT][0].arr.p, p[UM_P_MSAT][0].mul, &a, st);
       if (!fused) compute_demag(m);
     }
     if (profiling) UM_HIP(hipEventRecord(e1, st));
@@ -830,6 +867,7 @@ struct Sim : SimBase {
     UM_REQUIRE(precess && !has_frozen && !has_vol && !relaxing, UM_EUNSUP,
                "the adjoint needs precession on, no frozen spins and no geometry");
     UM_REQUIRE(!has_dmibulk && !nz(UM_P_J, 3), UM_EUNSUP, "no adjoint of bulk DMI or spin-transfer torques");
+    UM_REQUIRE(!has_nodemag, UM_EUNSUP, "no adjoint with NoDemagSpins");
     tape_end();
     tape_K = every > 0 ? std::min<long>(every, cap) : cap;
     const long nck = (cap + tape_K - 1) / tape_K;

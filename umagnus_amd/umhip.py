@@ -40,7 +40,7 @@ EXPORTS = [
     "um_sim_tape_begin", "um_sim_tape_end", "um_sim_tape_steps", "um_sim_backward",
     "um_add_dmi_bulk", "um_add_zhangli", "um_add_slonczewski", "um_sim_set_dmi_bulk", "um_sim_set_stt_flags",
     "um_sim_minimize", "um_set_kernel_cache", "um_kernel_cache_basename", "um_load_kernel", "um_save_kernel",
-    "um_set_autotune", "um_demag_schedule",
+    "um_set_autotune", "um_demag_schedule", "um_sim_set_nodemag_lut",
     "um_sim_create", "um_sim_destroy", "um_sim_set_scalar", "um_sim_set_vector", "um_sim_set_regions",
     "um_sim_set_exchange_luts", "um_sim_set_bext_lut", "um_sim_set_frozen_lut", "um_sim_set_geometry",
     "um_sim_set_flags", "um_sim_set_demag_accuracy", "um_sim_set_thermal_seed", "um_sim_set_m",
@@ -495,6 +495,10 @@ class Sim:
 
     def set_frozen_lut(self, lut):
         _call("um_sim_set_frozen_lut", self.h, self._p(self._a(lut)))
+
+    def set_nodemag_lut(self, lut):
+        """NoDemagSpins per region (256 values); None clears."""
+        _call("um_sim_set_nodemag_lut", self.h, self._p(self._a(lut)) if lut is not None else None)
 
     def set_geometry(self, vol):
         _call("um_sim_set_geometry", self.h, self._p(self._a(vol)))
