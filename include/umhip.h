@@ -246,6 +246,14 @@ int um_sim_set_frozen_lut(void *sim, const void *lut);
  * :67-98): flagged cells neither generate nor receive the demag field;
  * 256 values, NULL clears */
 int um_sim_set_nodemag_lut(void *sim, const void *lut);
+/* Time-dependent parameters (a script's `Ku1 = K * sin(2*pi*f*t)`, re-evaluated
+ * per stage by regionwise.update, engine/parameter.go:70-86): fn(t, values,
+ * ctx) writes the parameter's ncomp (1 or 3) uniform multipliers at stage time
+ * t before every effective-field evaluation; a per-cell array set with
+ * um_sim_set_scalar/_vector is scaled by them.  fn = NULL removes it.  Steps
+ * with a time function are not graph-replayed. */
+typedef void (*um_time_fn)(double t, double *values, void *ctx);
+int um_sim_set_time_function(void *sim, int param, int ncomp, um_time_fn fn, void *ctx);
 /* geometry (engine/geom.go): per-cell volume fraction, NULL = full */
 int um_sim_set_geometry(void *sim, const void *vol);
 /* EnableDemag, OpenBC, DoPrecess, and the MaxVecDiff sqrt(3) widening in Heun */

@@ -565,3 +565,45 @@ def test_nodemagspins(gpu, oracle, prec):
     for f in ("demag_field", "beff", "torque"):
         a, b = getattr(sims[0], f)(), getattr(sims[1], f)()
         assert np.abs(a - b).max() <= tol * np.abs(b).max(), f
+
+
+def _timedep_film(gpu, vol=None):
+    N, c = (32, 32, 1), 4e-9
+    s = gpu.Sim(N, (c, c, c), dtype=np.float32)
+    setup(s, gpu, Ms=860e3, Aex=13e-12, alpha=0.2, anisU=(0, 1, 0), vol=vol, dtype=np.float32)
+    s.set_m(uniform_m(N, (1, 1, 0)))
+    return s, N, (c, c, c)
+
+
+@pytest.mark.parametrize("script", ["timedep", "timedepRegion"])
+def test_time_dependent_anisotropy(gpu, script):
+    """test/timedep.mx3 (Ku1 = 1e5 sin(2 pi f t), f = 1 GHz, 0.5 ns: <m> =
+    (0, 0.9909376502037048, 0), TOL 1e-5) and timedepRegion.mx3 (the same
+    function set on region 1, which covers the grid; TOL 1e-4)."""
+    s, N, _ = _timedep_film(gpu)
+    if script == "timedepRegion":
+        s.set_regions(np.ones((N[2], N[1], N[0]), np.uint8))
+    f = 1e9
+    s.set_time_function(gpu.P_KU1, lambda t: 1e5 * math.sin(2 * math.pi * f * t))
+    s.run(0.5e-9)
+    got = s.average_m()
+    _say(script, got)
+    want, tol = (np.array([0, 0.9909376502037048, 0]), 1e-5) if script == "timedep" else \
+        (np.array([0, 0.99090, 0]), 1e-4)
+    assert np.abs(got - want).max() < tol, got
+
+
+def test_time_dependent_field(gpu):
+    """test/timedep3.mx3: rotating B_ext = A (sin, cos, 0)(2 pi f t) on a disc,
+    0.2 ns: <m> = (0.6773565132629695, 0.7201919931496306, 0.02121575360227688), TOL 1e-5."""
+    from cases import ellipse_geom
+    N, c = (32, 32, 1), 4e-9
+    s, N, cs = _timedep_film(gpu, vol=ellipse_geom(N, (c, c, c), 32 * c, 32 * c))
+    s.set_scalar(gpu.P_KU1, 0.0)
+    f, A = 1e9, 0.01
+    s.set_time_function(gpu.P_BEXT, lambda t: (A * math.sin(2 * math.pi * f * t), A * math.cos(2 * math.pi * f * t),
+                                               0.0), ncomp=3)
+    s.run(0.2e-9)
+    got = s.average_m()
+    _say("timedep3", got)
+    assert np.abs(got - np.array([0.6773565132629695, 0.7201919931496306, 0.02121575360227688])).max() < 1e-5, got

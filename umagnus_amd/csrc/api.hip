@@ -683,6 +683,9 @@ int um_sim_set_bext_lut(void *sim, const void *lx, const void *ly, const void *l
   });
 }
 int um_sim_set_frozen_lut(void *sim, const void *lut) { return guard([&] { SIMCHK(sim); SIM(sim)->set_frozen_lut(lut); }); }
+int um_sim_set_time_function(void *sim, int param, int ncomp, um_time_fn fn, void *ctx) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->set_time_function(param, ncomp, fn, ctx); });
+}
 int um_sim_set_nodemag_lut(void *sim, const void *lut) {
   return guard([&] { SIMCHK(sim); SIM(sim)->set_nodemag_lut(lut); });
 }

@@ -41,8 +41,19 @@ struct Sim : SimBase {
     DevBuf arr;
     double mul = 0;
     bool nonzero = false;   // host-side knowledge: any non-zero value
+    bool arr_nz = false;    // the per-cell array holds a non-zero value
   };
   PSlot p[UM_P_COUNT][3];
+  // time-dependent parameters: a host function of the stage time gives the
+  // uniform multiplier(s), re-evaluated for every evaluation as
+  // regionwise.update does per stage (engine/parameter.go:70-86)
+  struct TimeFn {
+    um_time_fn fn = nullptr;
+    void *ctx = nullptr;
+    int nc = 1;
+  };
+  TimeFn tfn[UM_P_COUNT];
+  bool any_tfn = false;
   DevBuf regions, aLUT, dLUT, dbLUT, bextLUT, frozenLUT, vol;
   DevBuf nodemagLUT, ndvol;   // NoDemagSpins per region, and the masked demag input
   bool has_nodemag = false;
@@ -158,10 +169,34 @@ struct Sim : SimBase {
     s.mul = mul;
     if (arr) {
       upload(s.arr, arr, sizeof(T) * n);
-      s.nonzero = (mul != 0) && any_nonzero((const T *)arr, n);
+      s.arr_nz = any_nonzero((const T *)arr, n);
+      s.nonzero = (mul != 0) && s.arr_nz;
     } else {
       s.arr.release();
+      s.arr_nz = false;
       s.nonzero = (mul != 0);
+    }
+  }
+  void set_time_function(int id, int nc, um_time_fn fn, void *ctx) override {
+    UM_REQUIRE(id >= 0 && id < UM_P_COUNT && (nc == 1 || nc == 3), UM_EINVAL, "time function: parameter id / size");
+    gen++;
+    tfn[id] = TimeFn{fn, ctx, nc};
+    any_tfn = false;
+    for (auto &f : tfn) any_tfn = any_tfn || f.fn != nullptr;
+    if (id == UM_P_MSAT) invalidate_fsal();
+  }
+  void apply_time_functions() {
+    if (!any_tfn) return;
+    for (int id = 0; id < UM_P_COUNT; id++) {
+      const TimeFn &f = tfn[id];
+      if (!f.fn) continue;
+      double v[3] = {0, 0, 0};
+      f.fn(Time, v, f.ctx);
+      for (int c = 0; c < f.nc; c++) {
+        PSlot &s = p[id][c];
+        s.mul = v[c];
+        s.nonzero = v[c] != 0 && (!s.arr.p || s.arr_nz);
+      }
     }
   }
   void set_scalar(int id, const void *arr, double mul) override {
@@ -299,6 +334,7 @@ struct Sim : SimBase {
     UM_HIP(hipStreamSynchronize(st));
   }
   FieldArgs<T> field_args(const T *mm) {
+    apply_time_functions();
     FieldArgs<T> a{};
     a.g = Mesh{N[0], N[1], N[2], pbccode};
     a.wx = (T)(2 / (cs[0] * cs[0]));
@@ -1028,7 +1064,7 @@ struct Sim : SimBase {
   // ---- graph replay (FixDt only) ---------------------------------------------------------
   void set_graph(int on) override { graph_mode = on; }
   long step_count() const override { return nsteps; }
-  bool graph_usable() const { return graph_mode && fixdt != 0 && !profiling && !tape_on; }
+  bool graph_usable() const { return graph_mode && fixdt != 0 && !profiling && !tape_on && !any_tfn; }
   // one eager step (FSAL setup), then capture two steps if the cached graph is
   // stale; returns how many two-step replays fit before `stop`
   long graph_prepare(long stop) override {

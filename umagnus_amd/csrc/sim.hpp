@@ -55,6 +55,7 @@ struct SimBase {
   virtual void set_bext_lut(const void *const l[3]) = 0;
   virtual void set_frozen_lut(const void *l) = 0;
   virtual void set_nodemag_lut(const void *l) = 0;
+  virtual void set_time_function(int id, int nc, um_time_fn fn, void *ctx) = 0;
   virtual void set_geometry(const void *vol) = 0;
   virtual void set_flags(int demag, int openbc, int precess, int heun_sqrt3) = 0;
   virtual void set_demag_accuracy(double a) = 0;

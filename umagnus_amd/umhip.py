@@ -40,7 +40,7 @@ EXPORTS = [
     "um_sim_tape_begin", "um_sim_tape_end", "um_sim_tape_steps", "um_sim_backward",
     "um_add_dmi_bulk", "um_add_zhangli", "um_add_slonczewski", "um_sim_set_dmi_bulk", "um_sim_set_stt_flags",
     "um_sim_minimize", "um_set_kernel_cache", "um_kernel_cache_basename", "um_load_kernel", "um_save_kernel",
-    "um_set_autotune", "um_demag_schedule", "um_sim_set_nodemag_lut",
+    "um_set_autotune", "um_demag_schedule", "um_sim_set_nodemag_lut", "um_sim_set_time_function",
     "um_sim_create", "um_sim_destroy", "um_sim_set_scalar", "um_sim_set_vector", "um_sim_set_regions",
     "um_sim_set_exchange_luts", "um_sim_set_bext_lut", "um_sim_set_frozen_lut", "um_sim_set_geometry",
     "um_sim_set_flags", "um_sim_set_demag_accuracy", "um_sim_set_thermal_seed", "um_sim_set_m",
@@ -495,6 +495,28 @@ class Sim:
 
     def set_frozen_lut(self, lut):
         _call("um_sim_set_frozen_lut", self.h, self._p(self._a(lut)))
+
+    _TFN = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+    def set_time_function(self, pid, fn, ncomp=1):
+        """Parameter pid as a function of time: fn(t) -> value (ncomp 1) or 3 values,
+        evaluated at every stage; None removes it."""
+        if not hasattr(self, "_tfn"):
+            self._tfn = {}
+        if fn is None:
+            self._tfn.pop(pid, None)
+            _call("um_sim_set_time_function", self.h, int(pid), int(ncomp), None, None)
+            return
+
+        def cb(t, out, ctx):
+            v = fn(t)
+            if ncomp == 1:
+                out[0] = float(v)
+            else:
+                for c in range(3):
+                    out[c] = float(v[c])
+        self._tfn[pid] = Sim._TFN(cb)
+        _call("um_sim_set_time_function", self.h, int(pid), int(ncomp), self._tfn[pid], None)
 
     def set_nodemag_lut(self, lut):
         """NoDemagSpins per region (256 values); None clears."""
