@@ -32,8 +32,10 @@ def gpu():
 def test_minimize_vs_oracle(gpu, oracle, prec):
     """Same descent on a 24 x 16 x 2 film with demag, exchange, interfacial DMI,
     uniaxial anisotropy and a field.  In FP64 the first 25 Barzilai-Borwein
-    steps from a random state agree to 1e-9 (the method is non-monotone, so
-    later steps amplify rounding); from a smooth state the converged minima
+    steps from a random state agree to 2e-8: the method is non-monotone and
+    amplifies rounding, so that the oracle itself, started from m perturbed
+    by 1e-15 relative, ends 2e-9 to 5e-9 away after these 25 steps, while
+    a 1% change of Dind ends 1.2e-2 away.  From a smooth state the converged minima
     agree in <m> (1e-4, the stopping tolerance) and energy.  A random start
     is not compared in FP32: its first steps flip cells, and FP32 step
     lengths send them elsewhere."""
@@ -54,7 +56,7 @@ def test_minimize_vs_oracle(gpu, oracle, prec):
             s.minimize(stop=1e30, samples=25)      # exactly 25 steps
         assert sg.state()["nsteps"] == so.state()["nsteps"] == 25
         assert sg.state()["t"] == 0.0
-        assert np.abs(sg.get_m() - so.get_m()).max() < 1e-9
+        assert np.abs(sg.get_m() - so.get_m()).max() < 2e-8
     # converged minima (stop: max|dm| <= 1e-6 over 10 steps)
     m = uniform_m(N, (1, 0.3, 0.1))
     sg, so = make(gpu, dt, m), make(oracle, np.float64, m)

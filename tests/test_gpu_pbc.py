@@ -4,8 +4,9 @@ convolution built on them, including odd Nx (a full-length X transform), vs
 the oracle; the reference's PBC goldens test/demag2Dpbc.mx3, demag2Dpbc2.mx3,
 demagrodpbc.mx3, zhangliPBC.mx3, pbc1.mx3 and dmipbc.mx3.
 
-Tolerances: the brute-force tensor 1e-14 of max|K| (same FP64 operations in
-the same order); Newell 1e-6 relative + 1e-12 of max|K| (libm rounding
+Tolerances: the brute-force tensor 2e-12 of max|K| (same FP64 operations;
+the offsets next to the source are summed as wavefront trees, the oracle's
+sequential sums carry ~n eps of their thousands of cancelling terms); Newell 1e-6 relative + 1e-12 of max|K| (libm rounding
 amplified by the 27-term cancellation); fields FP64 1e-10, FP32 1e-5 normwise.
 """
 import sys
@@ -56,10 +57,13 @@ def test_bf_tensor_pbc_vs_oracle(gpu, oracle, N, cs, pbc):
     _say("bf pbc tensor", N, pbc)
     Kg = gpu.demag_kernel_bf(N, cs, pbc=pbc)
     Ko = oracle.demag_kernel_bf(N, cs, pbc=pbc)
+    # near offsets: tree vs sequential sums; images can cancel an off-diagonal
+    # component to rounding noise (yz at 1e-15 with Pz = 3), so the bound is
+    # scaled by the whole tensor
+    scale = max(np.abs(k).max() for k in Ko)
     for c in range(6):
         assert Kg[c].shape == Ko[c].shape
-        scale = max(np.abs(Ko[c]).max(), 1e-300)
-        assert np.abs(Kg[c] - Ko[c]).max() <= 1e-14 * scale, c
+        assert np.abs(Kg[c] - Ko[c]).max() <= 2e-12 * scale, c
 
 
 @pytest.mark.parametrize("N,cs,pbc", TENSOR_CASES)

@@ -941,13 +941,43 @@ static void launch_convz(const ConvZArgs<T> &a, int Nz, hipStream_t s) {
 // z outer, y, x inner, each ascending, and sums their FP64 fields in that
 // order.  Without PBC an element has at most one image.  FP contraction is
 // disabled so the FP64 values match a sequential evaluation.
+//
+// The integration grid shrinks as (accuracy / distance)^5, so the few offsets
+// next to the source hold nearly all the points (31104 each at accuracy 6,
+// 3.3e8 at accuracy 29 for 2.5 x 2.5 x 5 nm cells, against 4 far away).  The
+// offsets inside a small "near box" (|x| <= R[0], |y| <= R[1], |z| <= R[2],
+// chosen on the host so that every offset outside it has at most
+// BF_NEAR_POINTS points) are therefore taken out of the per-element threads
+// and integrated by whole wavefronts, BF_CHUNK points per wave, each wave's
+// partial sum reduced across its lanes; a last kernel adds the near images of
+// every element, in the reference image order, to the far sum.
 // ---------------------------------------------------------------------------
 struct BfArgs {
   int P[3];
   int r1[3], r2[3];      // image offsets per axis: r1 <= x <= r2
   int H[3];              // wrapped destinations per axis: 0..P/2
+  int R[3];              // near box, integrated by k_demag_bf_near (-1: none)
   double cs[3], L, accuracy;
   double *K[6];          // xx yy zz yz xz xy
+};
+
+constexpr long long BF_NEAR_POINTS = 4096;  // offsets above this go to the near kernel
+constexpr int BF_CHUNK_ITERS = 256;         // points per lane per chunk
+constexpr long long BF_CHUNK = 64LL * BF_CHUNK_ITERS;
+
+// One near-box offset and source direction, with its integration grid
+// (nv, nw already doubled as in the reference).
+struct BfTask {
+  int x, y, z, s;
+  int n[5];              // nv nw nx ny nz
+  long long npts;
+  int chunk0, nchunk;    // its chunks in the partial-sum array
+};
+// One destination element and direction, with its near tasks [t0, t1) in
+// image order.
+struct BfGroup {
+  long long I;
+  int s, t0, t1;
 };
 
 __device__ __forceinline__ double bf_delta(int d) {
@@ -1027,6 +1057,7 @@ __global__ void k_demag_bf(BfArgs a) {
     for (int z = image_first(zw, a.P[2], a.r1[2]); z <= a.r2[2]; z += a.P[2])
       for (int y = image_first(yw, a.P[1], a.r1[1]); y <= a.r2[1]; y += a.P[1])
         for (int x = image_first(xw, a.P[0], a.r1[0]); x <= a.r2[0]; x += a.P[0]) {
+          if (abs(x) <= a.R[0] && abs(y) <= a.R[1] && abs(z) <= a.R[2]) continue;   // near kernel
           double B[3];
           bf_image(a, s, x, y, z, B);
           S0 += B[0];
@@ -1039,6 +1070,104 @@ __global__ void k_demag_bf(BfArgs a) {
     else if (s == 1) { a.K[1][I] = S1; a.K[3][I] = S2; }
     else { a.K[2][I] = S2; }
   }
+}
+
+// Near-box integration: one wavefront per chunk of BF_CHUNK consecutive points
+// of one task, in the order of the reference's loops (nv, nw, nx, ny, nz with
+// nz fastest); lane l takes points l, l + 64, ...  The positions are formed
+// with the same expressions as bf_image.
+__global__ void __launch_bounds__(256) k_demag_bf_near(BfArgs a, const BfTask *tasks, const int *chunk_task,
+                                                       int nchunk, double *part) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= nchunk) return;   // wave-uniform
+  const BfTask t = tasks[chunk_task[c]];
+  const int s = t.s, u = s, v = (s + 1) % 3, w = (s + 2) % 3;
+  const double *cs = a.cs;
+  const int nv = t.n[0], nw = t.n[1], nx = t.n[2], ny = t.n[3], nz = t.n[4];
+  const double scale = 1 / (double)((long long)nv * nw * nx * ny * nz);
+  const double charge = (cs[v] * cs[w]) * scale;
+  const double pu1 = cs[u] / 2., pu2 = -pu1;
+  const double v0 = -(cs[v] / 2.) + cs[v] / (double)(2 * nv), vs = cs[v] / (double)nv;
+  const double w0 = -(cs[w] / 2.) + cs[w] / (double)(2 * nw), ws = cs[w] / (double)nw;
+  const double x0 = (double)t.x * cs[0] - cs[0] / 2 + cs[0] / (double)(2 * nx), xs = cs[0] / (double)nx;
+  const double y0 = (double)t.y * cs[1] - cs[1] / 2 + cs[1] / (double)(2 * ny), ys = cs[1] / (double)ny;
+  const double z0 = (double)t.z * cs[2] - cs[2] / 2 + cs[2] / (double)(2 * nz), zs = cs[2] / (double)nz;
+  const long long first = (long long)(c - t.chunk0) * BF_CHUNK;
+  const long long end = first + BF_CHUNK < t.npts ? first + BF_CHUNK : t.npts;
+  long long idx = first + lane;
+  long long q = idx;
+  int ga = (int)(q % nz); q /= nz;
+  int be = (int)(q % ny); q /= ny;
+  int al = (int)(q % nx); q /= nx;
+  int j = (int)(q % nw);
+  int i = (int)(q / nw);
+  double B0 = 0, B1 = 0, B2 = 0;
+  for (; idx < end; idx += 64) {
+    const double pv = v0 + (double)i * vs, pw = w0 + (double)j * ws;
+    const double rx = x0 + xs * (double)al, ry = y0 + ys * (double)be, rz = z0 + zs * (double)ga;
+    // pole[u] = pu, pole[v] = pv, pole[w] = pw
+    double p0 = u == 0 ? pu1 : (v == 0 ? pv : pw);
+    double p1 = u == 1 ? pu1 : (v == 1 ? pv : pw);
+    double p2 = u == 2 ? pu1 : (v == 2 ? pv : pw);
+    double X = rx - p0, Y = ry - p1, Z = rz - p2;
+    double r = sqrt(X * X + Y * Y + Z * Z);
+    double qr = charge / (4 * M_PI * r * r * r);
+    const double bx = X * qr, by = Y * qr, bz = Z * qr;
+    p0 = u == 0 ? pu2 : p0;
+    p1 = u == 1 ? pu2 : p1;
+    p2 = u == 2 ? pu2 : p2;
+    X = rx - p0; Y = ry - p1; Z = rz - p2;
+    r = sqrt(X * X + Y * Y + Z * Z);
+    qr = -charge / (4 * M_PI * r * r * r);
+    B0 += (bx + X * qr);
+    B1 += (by + Y * qr);
+    B2 += (bz + Z * qr);
+    // advance 64 points in the mixed radix (nv, nw, nx, ny, nz)
+    ga += 64;
+    int k = ga / nz; ga -= k * nz; be += k;
+    k = be / ny; be -= k * ny; al += k;
+    k = al / nx; al -= k * nx; j += k;
+    k = j / nw; j -= k * nw; i += k;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    B0 += __shfl_xor(B0, o, 64);
+    B1 += __shfl_xor(B1, o, 64);
+    B2 += __shfl_xor(B2, o, 64);
+  }
+  if (lane == 0) {
+    part[3 * (long)c] = B0;
+    part[3 * (long)c + 1] = B1;
+    part[3 * (long)c + 2] = B2;
+  }
+}
+
+// Adds each element's near images, in image order (each image the ordered sum
+// of its chunks), to the far sum written by k_demag_bf.
+__global__ void k_demag_bf_near_add(BfArgs a, const BfGroup *groups, int ngroup, const BfTask *tasks,
+                                    const double *part) {
+#pragma clang fp contract(off)
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroup) return;
+  const BfGroup G = groups[g];
+  double S0 = 0, S1 = 0, S2 = 0;
+  for (int ti = G.t0; ti < G.t1; ti++) {
+    const BfTask t = tasks[ti];
+    double T0 = 0, T1 = 0, T2 = 0;
+    for (int c = t.chunk0; c < t.chunk0 + t.nchunk; c++) {
+      T0 += part[3 * (long)c];
+      T1 += part[3 * (long)c + 1];
+      T2 += part[3 * (long)c + 2];
+    }
+    S0 += T0;
+    S1 += T1;
+    S2 += T2;
+  }
+  const long long I = G.I;
+  if (G.s == 0) { a.K[0][I] += S0; a.K[5][I] += S1; a.K[4][I] += S2; }
+  else if (G.s == 1) { a.K[1][I] += S1; a.K[3][I] += S2; }
+  else { a.K[2][I] += S2; }
 }
 
 // Mirror reconstruction, mag/demagkernel.go:280-324: every element with a
@@ -2050,6 +2179,22 @@ void kernel_ranges(const int P[3], const int pbc[3], int r1[3], int r2[3]) {
 }
 
 // Brute-force tensor on the GPU into device arrays Kdev[6] of size P (FP64).
+// The integration grid of bf_image for offset (x, y, z) and direction s_, on
+// the host: nv, nw (doubled), nx, ny, nz.
+static void bf_grid(const double cs[3], double L, double accuracy, int s_, int x, int y, int z, int n[5]) {
+  auto delta = [](int d) { d = std::abs(d); return (double)(d > 0 ? d - 1 : 0); };
+  const double dx = delta(x) * cs[0], dy = delta(y) * cs[1], dz = delta(z) * cs[2];
+  double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+  if (d == 0) d = L;
+  const double maxSize = d / accuracy;
+  auto cnt = [&](double c) { return (int)(std::fmax(c / maxSize, 1.0) + 0.5); };
+  n[0] = 2 * cnt(cs[(s_ + 1) % 3]);
+  n[1] = 2 * cnt(cs[(s_ + 2) % 3]);
+  n[2] = cnt(cs[0]);
+  n[3] = cnt(cs[1]);
+  n[4] = cnt(cs[2]);
+}
+
 void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3], double accuracy,
                             double *const Kdev[6], int P[3], hipStream_t s) {
   for (int i = 0; i < 3; i++)
@@ -2071,9 +2216,89 @@ void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3]
     a.K[c] = Kdev[c];
     fill_value<double>(Kdev[c], 0.0, np, s);
   }
+  // near box: the point count only falls with the distance, which only grows
+  // with |offset| per axis, so R[c] = the last heavy offset along axis c
+  for (int c = 0; c < 3; c++) a.R[c] = -1;
+  auto points = [&](int s_, int x, int y, int z, int n[5]) {
+    bf_grid(a.cs, a.L, accuracy, s_, x, y, z, n);
+    return (long long)n[0] * n[1] * n[2] * n[3] * n[4];
+  };
+  auto heaviest = [&](int x, int y, int z) {
+    long long m = 0;
+    int n[5];
+    for (int s_ = 0; s_ < 3; s_++) m = std::max(m, points(s_, x, y, z, n));
+    return m;
+  };
+  if (heaviest(0, 0, 0) > BF_NEAR_POINTS)
+    for (int c = 0; c < 3; c++) {
+      const int lim = std::max(-a.r1[c], a.r2[c]);
+      int r = 0;
+      for (; r < lim; r++) {
+        int o[3] = {0, 0, 0};
+        o[c] = r + 1;
+        if (heaviest(o[0], o[1], o[2]) <= BF_NEAR_POINTS) break;
+      }
+      a.R[c] = r;
+    }
+  std::vector<BfTask> tasks;
+  std::vector<long long> elem;
+  if (a.R[0] >= 0)
+    for (int z = std::max(a.r1[2], -a.R[2]); z <= std::min(a.r2[2], a.R[2]); z++)
+      for (int y = std::max(a.r1[1], -a.R[1]); y <= std::min(a.r2[1], a.R[1]); y++)
+        for (int x = std::max(a.r1[0], -a.R[0]); x <= std::min(a.r2[0], a.R[0]); x++) {
+          const int o[3] = {x, y, z};
+          int wr[3];
+          bool in = true;
+          for (int c = 0; c < 3; c++) {
+            wr[c] = ((o[c] % P[c]) + P[c]) % P[c];
+            in = in && wr[c] < a.H[c];
+          }
+          if (!in) continue;   // its element is filled by the mirror
+          for (int s_ = 0; s_ < 3; s_++) {
+            BfTask t{};
+            t.x = x; t.y = y; t.z = z; t.s = s_;
+            t.npts = points(s_, x, y, z, t.n);
+            t.nchunk = (int)((t.npts + BF_CHUNK - 1) / BF_CHUNK);
+            tasks.push_back(t);
+            elem.push_back((((long long)wr[2] * P[1] + wr[1]) * P[0] + wr[0]) * 3 + s_);
+          }
+        }
+  // group by (element, direction), keeping the image order within a group
+  std::vector<int> order(tasks.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return elem[p] < elem[q]; });
+  std::vector<BfTask> sorted;
+  std::vector<BfGroup> groups;
+  std::vector<int> chunk_task;
+  for (int i : order) {
+    BfTask t = tasks[i];
+    t.chunk0 = (int)chunk_task.size();
+    UM_REQUIRE((long long)t.chunk0 + t.nchunk < (1LL << 28), UM_ENOMEM, "DemagAccuracy too high for the near-field split");
+    const int ti = (int)sorted.size();
+    for (int k = 0; k < t.nchunk; k++) chunk_task.push_back(ti);
+    sorted.push_back(t);
+    if (groups.empty() || groups.back().I * 3 + groups.back().s != elem[i])
+      groups.push_back(BfGroup{elem[i] / 3, (int)(elem[i] % 3), ti, ti});
+    groups.back().t1 = ti + 1;
+  }
   const long total = (long)a.H[0] * a.H[1] * a.H[2] * 3;
   hipLaunchKernelGGL(k_demag_bf, dim3(blocks_for(total, 64, 1 << 16)), dim3(64), 0, s, a);
   UM_HIP(hipGetLastError());
+  if (!groups.empty()) {
+    const int nchunk = (int)chunk_task.size();
+    DevBuf dt(sizeof(BfTask) * sorted.size()), dg(sizeof(BfGroup) * groups.size()),
+        dc(sizeof(int) * chunk_task.size()), part(sizeof(double) * 3 * (size_t)nchunk);
+    UM_HIP(hipMemcpyAsync(dt.p, sorted.data(), dt.bytes, hipMemcpyHostToDevice, s));
+    UM_HIP(hipMemcpyAsync(dg.p, groups.data(), dg.bytes, hipMemcpyHostToDevice, s));
+    UM_HIP(hipMemcpyAsync(dc.p, chunk_task.data(), dc.bytes, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_demag_bf_near, dim3((nchunk + 3) / 4), dim3(256), 0, s, a, (const BfTask *)dt.p,
+                       (const int *)dc.p, nchunk, (double *)part.p);
+    UM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_demag_bf_near_add, dim3(((int)groups.size() + 63) / 64), dim3(64), 0, s, a,
+                       (const BfGroup *)dg.p, (int)groups.size(), (const BfTask *)dt.p, (const double *)part.p);
+    UM_HIP(hipGetLastError());
+    UM_HIP(hipStreamSynchronize(s));   // before the host tables and device buffers go
+  }
   demag_kernel_mirror(P, Kdev, s);
   if (P[2] == 1) {
     fill_value<double>(Kdev[3], 0.0, np, s);
