@@ -1725,6 +1725,15 @@ struct DemagPlan : DemagBase {
   bool xodd = false;            // odd Px (periodic along x, odd Nx): full-length X passes
   long kcount = 0;
 
+  // The generic engine takes any lines per work-group: halve a choice while
+  // its pass (units lines, times mult in the other grid dimensions) would
+  // launch fewer than target work-groups, e.g. 100 x 100 x 2 (200 X rows, 81
+  // per group at most) would otherwise run on 3 x 3 work-groups.
+  static int spread(int nl, long units, long mult, long target = SMALL_GRID_WG) {
+    while (nl > 1 && mult * ((units + nl - 1) / nl) < target) nl = (nl + 1) / 2;
+    return nl;
+  }
+
   static int lines_for(int n) {
     const int cap = FFT_NT * FftEpt<T>::value;
     UM_REQUIRE(n <= cap, UM_EUNSUP, "FFT length " + std::to_string(n) + " exceeds the LDS budget");
@@ -1757,9 +1766,11 @@ struct DemagPlan : DemagBase {
         nl_x = nl;
       }
     }
+    if (xodd || !CtTable<T>::x(P[0] / 2)) nl_x = spread(nl_x, (long)N[1] * N[2], 3);
     nl_xi = nl_x;
     nl_y = CtTable<T>::col(P[1]);
-    if (!nl_y) nl_y = std::min(lines_for(P[1]), 64);
+    // the Y columns of a 3D grid run Nz planes deep: fuller groups pay there
+    if (!nl_y) nl_y = spread(std::min(lines_for(P[1]), 64), H, 3L * N[2], SMALL_GRID_WG / 6);
 
     zreg = !is2d && P[2] == 2 * N[2] && ConvZTable<T>::has(N[2]);
     const int convlen = is2d ? P[1] : P[2];
@@ -1775,6 +1786,7 @@ struct DemagPlan : DemagBase {
       UM_REQUIRE(bmax >= 1, UM_EUNSUP, "convolution axis too long");
       B_conv = 1;
       while (B_conv * 2 <= bmax) B_conv *= 2;
+      B_conv = spread(B_conv, H, is2d ? 1 : P[1] / 2 + 1);
     }
     UM_REQUIRE(B_conv * (convlen / 2 + 1) <= FFT_NT * CONV_QK, UM_EUNSUP, "convolution prefetch slots exceeded");
     UM_REQUIRE(3 * B_conv * convlen <= FFT_NT * FftEpt<T>::value, UM_EUNSUP, "convolution tile exceeds LDS budget");
