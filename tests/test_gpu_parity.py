@@ -511,7 +511,7 @@ def _same_state(a, b):
     assert np.array_equal(a.get_m(), b.get_m())
 
 
-@pytest.mark.parametrize("solver", [5, 2, 4])
+@pytest.mark.parametrize("solver", [5, 2, 4, 3])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_graph_steps_match_eager(gpu, solver, prec):
     """Fixed-dt steps replayed from a captured two-step graph are bit-identical to

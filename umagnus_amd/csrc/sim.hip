@@ -90,6 +90,7 @@ struct Sim : SimBase {
   // statistics and the Philox step counter live on the device meanwhile
   int graph_mode = 0;
   bool capturing = false;
+  bool counted = false;     // the captured step's counter increment is queued
   long gen = 0;                  // bumped by every setter: a captured graph is stale
   DevBuf gstat;                  // {lastErr, peakErr, lastTorque, step counter}
   double *hstat = nullptr;       // pinned mirror
@@ -552,9 +553,16 @@ struct Sim : SimBase {
     a.N = n;
     launch_stage(a, st);
   }
-  // fused norms -> host (one readback)
+  // device statistics of a captured step (count: its last statistics launch)
+  void step_stats(double h, int mode, bool count) {
+    launch_step_stats((unsigned long long *)red.p, gstat.as<double>(), h, mode, count ? 1 : 0, st);
+    if (count) counted = true;
+  }
+  // fused norms -> host (one readback).  A captured step finds the slots
+  // cleared: graph_prepare clears them before the replays and every
+  // k_step_stats after reading them.
   void err_reduce(int mode, const T *const *src, int nsrc, const T *e, double out[3]) {
-    fill_bytes(red.p, red.bytes, st);
+    if (!capturing) fill_bytes(red.p, red.bytes, st);
     ErrArgs<T> a{};
     a.mode = mode;
     for (int j = 0; j < nsrc; j++)
@@ -642,7 +650,7 @@ struct Sim : SimBase {
       if (maxdm < minslope) fail = 0;
       else { rlerr = r[2]; fail = 1; }
       if (fail == 0 || relerr <= 0.0 || rlerr < relerr || dt <= mindt || fixdt != 0) {
-        if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), (double)h, 0, st);
+        if (capturing) step_stats((double)h, 0, true);
         else {
           set_last_err(err);
           lastTorque = maxdm;
@@ -713,7 +721,7 @@ struct Sim : SimBase {
       if (maxdm < minslope) fail = 0;
       else { rlerr = r[2]; fail = 1; }
       if (fail == 0 || relerr <= 0.0 || rlerr < relerr || dt <= mindt || fixdt != 0) {
-        if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), (double)h, 0, st);
+        if (capturing) step_stats((double)h, 0, true);
         else {
           set_last_err(err);
           lastTorque = maxdm;
@@ -757,7 +765,7 @@ struct Sim : SimBase {
     const T *src[2] = {dy0, dy};
     err_reduce(1, src, 2, nullptr, r);
     double err = std::sqrt(heun_sqrt3 ? 3.0 * r[0] : r[0]) * (double)h;
-    if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), (double)h, heun_sqrt3 ? 3 : 1, st);
+    if (capturing) step_stats((double)h, heun_sqrt3 ? 3 : 1, false);
     if (err < maxerr || dt <= mindt || fixdt != 0) {
       const T *s[] = {m, dy, dy0};
       const T f[] = {T(1), T(0.5) * h, T(-0.5) * h};
@@ -768,7 +776,7 @@ struct Sim : SimBase {
       const T *sn[1] = {dy};
       double r2[3];
       err_reduce(2, sn, 1, nullptr, r2);
-      if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), 0.0, 2, st);
+      if (capturing) step_stats(0.0, 2, true);
       else lastTorque = std::sqrt(r2[0]);
     } else {
       Time -= dt;
@@ -798,7 +806,7 @@ struct Sim : SimBase {
     const T *src[2] = {k1, k4};
     err_reduce(1, src, 2, nullptr, r);
     const double err = std::sqrt(r[0]) * (double)h;
-    if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), (double)h, 1, st);
+    if (capturing) step_stats((double)h, 1, false);
     bool ok;
     if (err < maxerr || dt <= mindt || fixdt != 0) {
       nsteps++;
@@ -817,7 +825,7 @@ struct Sim : SimBase {
       const T *sn[1] = {k4};
       double r2[3];
       err_reduce(2, sn, 1, nullptr, r2);
-      if (capturing) launch_step_stats((const unsigned long long *)red.p, gstat.as<double>(), 0.0, 2, st);
+      if (capturing) step_stats(0.0, 2, true);
       else lastTorque = std::sqrt(r2[0]);
     } else {
       Time = t0;
@@ -1036,13 +1044,15 @@ struct Sim : SimBase {
       return;
     }
     if (profiling) UM_HIP(hipEventRecord(step_ev[0], st));
+    counted = false;
     switch (solver) {
       case UM_SOLVER_HEUN: step_heun(); break;
       case UM_SOLVER_RK23: step_rk23(); break;
       case UM_SOLVER_RK4: step_rk4(); break;
       default: step_rk45dp(); break;
     }
-    if (capturing) launch_step_count((unsigned long long *)(gstat.as<double>() + 3), st);
+    // a captured step counts itself in its last k_step_stats
+    if (capturing && !counted) launch_step_count((unsigned long long *)(gstat.as<double>() + 3), st);
     if (profiling) {
       UM_HIP(hipEventRecord(step_ev[1], st));
       UM_HIP(hipEventSynchronize(step_ev[1]));
@@ -1075,6 +1085,7 @@ struct Sim : SimBase {
     bool fresh = gexec && g_gen == gen && g_solver == solver && g_dt == fixdt;
     for (int i = 0; i < 8 && fresh; i++) fresh = (g_ptrs[i] == ptrs[i]);
     if (!fresh) capture2();
+    fill_bytes(red.p, red.bytes, st);   // the replays' reductions start from cleared slots
     hstat[0] = lastErr; hstat[1] = peakErr; hstat[2] = lastTorque;
     const unsigned long long ns = (unsigned long long)nsteps;
     std::memcpy(&hstat[3], &ns, 8);

@@ -360,9 +360,16 @@ template void launch_err<float>(const ErrArgs<float> &, hipStream_t);
 template void launch_err<double>(const ErrArgs<double> &, hipStream_t);
 
 // ---- step statistics on the device ----------------------------------------------------
-__global__ void k_step_stats(const unsigned long long *res, double *gs, double h, int mode) {
+// Folds one reduction of a captured step into gs (last error, peak error,
+// last torque, step counter as u64 bits in gs[3]) and clears the 4 reduction
+// slots for the next reduction, so a captured step needs no fill launches;
+// count: this is the step's last statistics launch.
+__global__ void k_step_stats(unsigned long long *res, double *gs, double h, int mode, int count) {
   if (threadIdx.x != 0) return;
   const double r0 = __longlong_as_double((long long)res[0]);
+  const double r1 = __longlong_as_double((long long)res[1]);
+  for (int i = 0; i < 4; i++) res[i] = 0ull;
+  if (count) *(unsigned long long *)(gs + 3) += 1ull;
   if (mode == 2) {
     gs[2] = sqrt(r0);
     return;
@@ -370,13 +377,13 @@ __global__ void k_step_stats(const unsigned long long *res, double *gs, double h
   const double err = sqrt(mode == 3 ? 3.0 * r0 : r0) * h;
   gs[0] = err;
   if (err > gs[1]) gs[1] = err;
-  if (mode == 0) gs[2] = sqrt(__longlong_as_double((long long)res[1]));
+  if (mode == 0) gs[2] = sqrt(r1);
 }
 __global__ void k_step_count(unsigned long long *counter) {
   if (threadIdx.x == 0) *counter += 1ull;
 }
-void launch_step_stats(const unsigned long long *res, double *gs, double h, int mode, hipStream_t s) {
-  hipLaunchKernelGGL(k_step_stats, dim3(1), dim3(64), 0, s, res, gs, h, mode);
+void launch_step_stats(unsigned long long *res, double *gs, double h, int mode, int count, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_stats, dim3(1), dim3(64), 0, s, res, gs, h, mode, count);
   UM_HIP(hipGetLastError());
 }
 void launch_step_count(unsigned long long *counter, hipStream_t s) {

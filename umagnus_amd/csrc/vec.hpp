@@ -55,7 +55,7 @@ template <typename T> void launch_err(const ErrArgs<T> &a, hipStream_t s);
 // lastTorque, step counter}.  mode 0: RK45DP (err = sqrt(res0) h, torque =
 // sqrt(res1)); 1 / 3: err = sqrt(mode * res0) h (Heun's sqrt3 widening = 3);
 // 2: torque = sqrt(res0).
-void launch_step_stats(const unsigned long long *res, double *gs, double h, int mode, hipStream_t s);
+void launch_step_stats(unsigned long long *res, double *gs, double h, int mode, int count, hipStream_t s);
 void launch_step_count(unsigned long long *counter, hipStream_t s);
 
 // Deterministic FP64 sums: out[0] = sum_i w_i * (a_i . b_i) over ncomp
