@@ -16,7 +16,9 @@ them independently of their code:
 - settemperature2 (kernels_src/cl/temperature2.cl:1-19, engine/temperature.go:
   109): B = eta sqrt(2 kB alpha T / (gamma V dt Ms)) against the closed form in
   FP64, the sample variance of normal noise against that formula, and 0 for
-  T = 0 or Ms = 0.
+  T = 0 or Ms = 0;
+- llnoprecess (kernels_src/cl/llnoprecess.cl:3-23): the part of B
+  perpendicular to m.
 """
 import ctypes as C
 
@@ -156,3 +158,24 @@ def test_settemperature2(dt):
         Bz = np.ones(n, dt)
         f(_p(Bz), _p(noise), _r(dt, k2), None, _r(dt, Ms_mul), None, _r(dt, T_mul), None, _r(dt, 0.1), n)
         assert np.all(Bz == 0)
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+def test_llnoprecess(dt):
+    """llnoprecess.cl:3-23: tau = -m x (m x B), which for unit m is the part of
+    B perpendicular to m (BAC-CAB), B - (m.B) m; a sign error or a missing
+    cross product fails."""
+    rng = np.random.default_rng(5)
+    n = 3000
+    m = rng.standard_normal((3, n))
+    m = (m / np.linalg.norm(m, axis=0)).astype(dt)
+    B = rng.standard_normal((3, n)).astype(dt)
+    t = np.zeros((3, n), dt)
+    orc.fn("llnoprecess", dt)(_p(t[0]), _p(t[1]), _p(t[2]), _p(m[0]), _p(m[1]), _p(m[2]),
+                              _p(B[0]), _p(B[1]), _p(B[2]), n)
+    m64, B64 = m.astype(np.float64), B.astype(np.float64)
+    mm = np.einsum("ij,ij->j", m64, m64)
+    want = B64 * mm - m64 * np.einsum("ij,ij->j", m64, B64)
+    eps = np.finfo(dt).eps
+    assert np.abs(t - want).max() < 16 * eps * np.abs(B64).max()
+    assert np.abs(np.einsum("ij,ij->j", t.astype(np.float64), m64)).max() < 16 * eps * np.abs(B64).max()
