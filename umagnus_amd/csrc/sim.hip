@@ -112,9 +112,13 @@ struct Sim : SimBase {
   int prof_evals = 0;
 
   Sim(const int Nin[3], const double csin[3], const int pbcin[3]) {
-    ensure_init();
     for (int i = 0; i < 3; i++) { N[i] = Nin[i]; cs[i] = csin[i]; pbc[i] = pbcin[i]; }
     UM_REQUIRE(N[0] > 0 && N[1] > 0 && N[2] > 0, UM_EINVAL, "grid size must be positive");
+    for (int i = 0; i < 3; i++) {
+      UM_REQUIRE(std::isfinite(cs[i]) && cs[i] > 0, UM_EINVAL, "cell size must be positive and finite");
+      UM_REQUIRE(pbc[i] >= 0 && pbc[i] <= (1 << 16), UM_EINVAL, "PBC repetitions out of range");
+    }
+    ensure_init();
     n = (long)N[0] * N[1] * N[2];
     UM_REQUIRE(n < (1L << 31), UM_EUNSUP, "grid too large for 32-bit cell indices");
     pbccode = (pbc[0] ? 1 : 0) | (pbc[1] ? 2 : 0) | (pbc[2] ? 4 : 0);
@@ -501,7 +505,7 @@ struct Sim : SimBase {
   void set_solver(int s) override {
     gen++;
     UM_REQUIRE(s == UM_SOLVER_HEUN || s == UM_SOLVER_RK23 || s == UM_SOLVER_RK4 || s == UM_SOLVER_RK45DP, UM_EUNSUP,
-               "solver " + std::to_string(s) + " not available (Heun=2, RK4=4, RK45DP=5)");
+               "solver " + std::to_string(s) + " not available (Heun=2, RK23=3, RK4=4, RK45DP=5)");
     solver = s;
     k1_valid = false;
   }
