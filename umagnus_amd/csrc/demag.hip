@@ -2252,6 +2252,10 @@ void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3]
       }
       a.R[c] = r;
     }
+  // an extreme DemagAccuracy (a near box of millions of offsets) keeps every
+  // offset in the per-element kernel rather than building huge task tables
+  if ((2L * a.R[0] + 1) * (2L * a.R[1] + 1) * (2L * a.R[2] + 1) > (1L << 22))
+    for (int c = 0; c < 3; c++) a.R[c] = -1;
   std::vector<BfTask> tasks;
   std::vector<long long> elem;
   if (a.R[0] >= 0)
