@@ -483,12 +483,21 @@ __device__ __forceinline__ int xcd_tile(int w, int total) {
 }
 
 template <typename T, int A, int DIR, int C, bool HALF, int ZB>
-__global__ __launch_bounds__(A * C) void k_coly(ColArgs<T> a) {
+__global__ __launch_bounds__(A * C, (HALF && sizeof(C2<T>) * A * (A * C + C) > 65536) ? 4 : 1) void k_coly(
+    ColArgs<T> a) {
   // transpose row stride A*C + C: a wave's 64 b64 accesses then share each
   // LDS bank pair exactly twice (the minimum for 8-byte lanes)
   constexpr int P = A * A, S = A * C + C;
+  // SPLIT: a transpose of A x (A*C) complex values would hold 135 KB of LDS
+  // at A = 32 (one work-group per CU); exchanging the real and then the
+  // imaginary parts halves it, so two work-groups (4 waves per SIMD, hence
+  // the launch bound of the half-column passes; the full-column inverse
+  // would spill under it) overlap their load and arithmetic phases, without
+  // extra registers (each half is read back into the slots just written out)
+  constexpr bool SPLIT = sizeof(C2<T>) * A * S > 65536;
   extern __shared__ __align__(16) unsigned char smem_[];
   C2<T> *lds = (C2<T> *)smem_;
+  T *ldr = (T *)smem_;
   const int ngx = gridDim.x, ngy = gridDim.y;
   const int w = xcd_tile(blockIdx.x + ngx * (blockIdx.y + ngy * blockIdx.z), ngx * ngy * gridDim.z);
   const int gx = w % ngx, gyz = w / ngx, gy = gyz % ngy, gz = gyz / ngy;
@@ -529,11 +538,25 @@ __global__ __launch_bounds__(A * C) void k_coly(ColArgs<T> a) {
       v[k2] = cmul(v[k2], t);
     }
     if (p > 0) __syncthreads();   // the previous plane's transpose reads are done
+    if constexpr (SPLIT) {
 #pragma unroll
-    for (int k2 = 0; k2 < A; k2++) lds[k2 * S + threadIdx.x] = v[k2];
-    __syncthreads();
+      for (int k2 = 0; k2 < A; k2++) ldr[k2 * S + threadIdx.x] = v[k2].x;
+      __syncthreads();
 #pragma unroll
-    for (int n1 = 0; n1 < A; n1++) v[n1] = lds[i * S + n1 * C + col];
+      for (int n1 = 0; n1 < A; n1++) v[n1].x = ldr[i * S + n1 * C + col];
+      __syncthreads();
+#pragma unroll
+      for (int k2 = 0; k2 < A; k2++) ldr[k2 * S + threadIdx.x] = v[k2].y;
+      __syncthreads();
+#pragma unroll
+      for (int n1 = 0; n1 < A; n1++) v[n1].y = ldr[i * S + n1 * C + col];
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < A; k2++) lds[k2 * S + threadIdx.x] = v[k2];
+      __syncthreads();
+#pragma unroll
+      for (int n1 = 0; n1 < A; n1++) v[n1] = lds[i * S + n1 * C + col];
+    }
     dft<A, DIR>(v);
     if (valid) {
       C2<T> *ub = a.buf[gz] + (long)(gy * ZB + p) * a.pstride;
@@ -1454,7 +1477,9 @@ static bool launch_coly(const ColArgs<T> &a, int P, int nz, hipStream_t s) {
   // inverse loads twice as many rows and would spill)
   const int zb = (sizeof(T) == 4 && DIR < 0 && nz % UM_COLY_ZB == 0) ? UM_COLY_ZB : 1;
   auto go = [&](auto kern, int A, int C, int ZB) {
-    const size_t lds = sizeof(C2<T>) * (size_t)A * (A * C + C);
+    // k_coly's SPLIT: real and imaginary parts exchanged one after the other
+    size_t lds = sizeof(C2<T>) * (size_t)A * (A * C + C);
+    if (lds > 65536) lds /= 2;
     allow_lds(kern, lds);
     const dim3 grid((a.H + C - 1) / C, nz / ZB, 3);
     hipLaunchKernelGGL(kern, grid, dim3(A * C), lds, s, a);
