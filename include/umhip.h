@@ -131,6 +131,80 @@ int um_div(int prec, void *dst, const void *a, const void *b, int N);
 /* Mul, opencl/madd.go:13 */
 int um_mul(int prec, void *dst, const void *a, const void *b, int N);
 
+/* ---- fused evaluation and RK stage entry points ----------------------------
+ * What an unchanged engine needs to run the fused kernels through its own
+ * call sequence: one pass per torque evaluation (SetTorque without B_demag),
+ * one pass per RK stage (MaddN + Normalize) and the step controller's norms
+ * with one readback (integralController). */
+
+/* A parameter in the MSlice convention (opencl/mslice.go:11-45): arr == NULL
+ * means uniform with value mul, else the per-cell value is mul*arr[i]. */
+typedef struct {
+  const void *arr;
+  double mul;
+} um_param;
+
+/* Inputs of um_field_torque.  Arrays are device pointers of the call's
+ * precision, vectors are three component arrays (SoA).  Zero-initialise the
+ * struct: every NULL pointer / zero parameter switches its term off. */
+typedef struct {
+  int Nx, Ny, Nz;
+  int pbc;                       /* PBC_code (data/mesh.go): bit 0 x, bit 1 y, bit 2 z */
+  double cellsize[3];
+  const void *m[3];              /* magnetization (M.Buffer()) */
+  const void *bdemag[3];         /* B_demag (um_demag_exec output), all NULL: no demag term */
+  void *torque[3];               /* out, SetTorque (engine/torque.go:124), all NULL: not written; may alias bdemag */
+  void *beff[3];                 /* out, SetEffectiveField (engine/effectivefield.go:14), all NULL: not written */
+  const uint8_t *regions;        /* region byte per cell, NULL: every cell in region 0 */
+  um_param Msat;
+  const void *aex_lut;           /* Aex_red SymmLUT, 256*257/2 reals (engine/exchange.go:153-177); NULL: no exchange */
+  const void *dmi_lut;           /* Dind_red SymmLUT, or Dbulk's when dmi_bulk = 1; NULL: no DMI (needs aex_lut) */
+  int dmi_bulk;
+  int openbc;                    /* OpenBC (engine/exchange.go:28) */
+  um_param Ku1, Ku2, anisU[3];   /* AddUniaxialAnisotropy2 (opencl/anisotropy.go:153) */
+  um_param Kc1, Kc2, Kc3, anisC1[3], anisC2[3];   /* AddCubicAnisotropy2 (opencl/anisotropy.go:12) */
+  const void *bext_lut[3];       /* B_ext region LUT, 256 reals per component (RegionAddV), or NULL */
+  um_param Bext[3];              /* B_ext's uniform / per-cell part */
+  int thermal;                   /* 1: B_therm from the Philox stream (seed, step, stream), as um_thermal_field */
+  uint64_t seed, step;
+  uint32_t stream;
+  um_param Temp;
+  double thermal_k;              /* 2 kB / (gammaLL V dt) (engine/temperature.go:109) */
+  const void *badd[3];           /* field added to B_eff last (AddCustomField, engine/customfield.go:68), or NULL */
+  um_param alpha;
+  int precess;                   /* 1: LLTorque, 0: LLNoPrecess (opencl/lltorque.go:16,97) */
+  int zhangli, slonczewski;      /* AddSTTorque terms (engine/torque.go:147-196) */
+  um_param J[3], xi, pol, lambda, eps_prime, free_layer_thickness, fixed_layer[3];
+  double fixed_layer_top;        /* CurrentSignFromFixedLayerPosition: +1 fixed layer on top, -1 below */
+  const void *tadd[3];           /* torque added before FreezeSpins (AddCustomTorques, engine/torque.go:115), or NULL */
+  const void *frozen_lut;        /* FrozenSpins per region (256 reals, non-zero: frozen), or NULL */
+} um_field_args;
+
+/* SetTorque (engine/torque.go:124-133) after SetDemagField: B_eff = bdemag +
+ * exchange/DMI + anisotropy + B_ext + B_therm + badd in the order of
+ * SetEffectiveField (engine/effectivefield.go:14-24), then LLTorque or
+ * LLNoPrecess, the spin-transfer torques, tadd and FreezeSpins, in one pass
+ * per cell on the library stream. */
+int um_field_torque(int prec, const um_field_args *a);
+
+/* One RK stage: Madd<nsrc>(dst, src...) then Normalize(dst, vol) in one pass
+ * per cell (engine/rk45dp.go:50-87, opencl/madd.go:111-489).  src holds nsrc
+ * vectors as 3*nsrc component pointers (src[3*j + c]), nsrc in 1..8; a
+ * three-source sum is grouped x0 + (x1 + x2) like madd3.cl.  normalize = 0
+ * leaves the sum as it is (Heun's first stage, engine/heun.go:32).  dst may
+ * alias a source. */
+int um_rk45dp_stage(int prec, void *const dst[3], int nsrc, const void *const *src, const double *fac,
+                    const void *vol, int normalize, int N);
+
+/* The norms integralController needs (engine/stepcontroller.go:35-62) from
+ * the stage torques in one pass and one readback, without an Err buffer:
+ * Err = sum_j e[j]*k_j (rk45dp.go:95; k = k1 k3 k4 k5 k6 k7 for RK45DP,
+ * k1..k4 for RK23), delM = k[nk-1].  nk in 1..6; k holds 3*nk component
+ * pointers (k[3*j + c]).  out[0] = MaxVecNorm(Err)*h (err), out[1] =
+ * MaxAbs(VecNorm(Err)/VecNorm(delM)) with 0 where |delM| = 0 (rlerr),
+ * out[2] = MaxVecNorm(delM) (maxdm).  A NaN anywhere in Err gives err = NaN. */
+int um_rk45dp_error(int prec, int nk, const void *const *k, const double *e, double h, int N, double out[3]);
+
 /* ---- reductions: opencl/reduce.go (results in FP64, one readback) --------- */
 int um_reduce_sum(int prec, const void *x, int N, double *out);
 int um_reduce_dot(int prec, const void *a, const void *b, int N, double *out);
@@ -192,6 +266,16 @@ int um_demag_kernel_newell(int Nx, int Ny, int Nz, const int pbc[3], const doubl
 int um_set_autotune(int mode, const char *file);
 /* the schedule of a plan: nl_x nl_xi four_step nl_y B_conv sched_x sched_y sched_z */
 int um_demag_schedule(void *plan, int out[8]);
+
+/* ---- launch trace (diagnostics; no counterpart in the reference) ----
+ * um_trace_kernels(1) clears the trace and records the demangled name of
+ * every kernel the library launches from then on (e.g.
+ * "void um::k_xfwd<float, 512, 8>(um::XArgs<float>)"), one per launch, until
+ * um_trace_kernels(0).  um_traced_kernels copies the names, one per line,
+ * NUL-terminated and truncated to out_len; *needed (nullable) receives the
+ * full length including the NUL. */
+int um_trace_kernels(int on);
+int um_traced_kernels(char *out, size_t out_len, size_t *needed);
 
 /* ---- demag tensor cache: mag/demagkernel.go:17-109, newellDemagKernel.go:1006-1106 ----
  * The -cache flag (engine/gofiles.go:17): with a directory set, um_demag_create,

@@ -63,6 +63,28 @@ def relerr(got, want):
     return float(np.abs(np.asarray(got, np.float64) - want).max() / max(np.abs(want).max(), 1e-300))
 
 
+def percell_relerr(got, want):
+    """max over cells of |got_i - want_i| / |want_i| (vector norms per cell; cells
+    where want_i = 0 must match exactly)."""
+    d = np.linalg.norm(np.asarray(got, np.float64) - want, axis=0)
+    w = np.linalg.norm(np.asarray(want, np.float64), axis=0)
+    if np.any(d[w == 0] != 0):
+        return float("inf")
+    return float((d[w > 0] / w[w > 0]).max()) if np.any(w > 0) else 0.0
+
+
+def percell_torque_relerr(got, want, beff):
+    """max over cells of |tau_got_i - tau_want_i| / |B_eff_i|: the torque error of
+    a cell against its field, which bounds |tau_i| (|m x B| <= |B| for unit m).
+    Relative to |tau_i| itself the error is unbounded where m is nearly parallel
+    to B and the cross product cancels."""
+    d = np.linalg.norm(np.asarray(got, np.float64) - want, axis=0)
+    b = np.linalg.norm(np.asarray(beff, np.float64), axis=0)
+    if np.any(d[b == 0] != 0):
+        return float("inf")
+    return float((d[b > 0] / b[b > 0]).max()) if np.any(b > 0) else 0.0
+
+
 # ---- initial states and shapes of the reference's scripts (engine/config.go, shape.go) ----------
 def coords(N, cs):
     """Index2Coord (engine/util.go:132-140): cell centres, origin at the grid centre."""

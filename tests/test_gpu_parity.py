@@ -443,6 +443,75 @@ def test_strip_trajectory_vs_oracle(gpu, oracle, prec):
         assert d < (1e-9 if prec == 1 else 1e-4)
 
 
+# ---- the timed kernels at the timed size --------------------------------------------------------
+TIMED_512x512x32 = ["void um::k_xfwd<float, 512, 8>(", "void um::k_coly<float, 32, -1, 16, true, 1>(",
+                    "void um::k_convz<float, 32, 8, 4>(", "void um::k_coly<float, 32, 1, 16, true, 1>(",
+                    "void um::k_xi4<float, 512>(", "void um::k_field_torque4<float, false, 2, false>("]
+
+
+def test_timed_kernels_512x512x32_vs_oracle(gpu, oracle):
+    """The 512x512x32 FP32 workload (DMI, Ku1 || z, alpha 0.1, random m) through
+    exactly the kernels the benchmark times -- the launch trace names the six
+    evaluation kernels -- against the FP64 oracle at full size, per cell:
+    |dB_i| / |B_i| <= 1e-5 for the effective field and |dtau_i| / |B_i| <= 1e-5
+    for the torque (the field bounds the torque of a unit m), and normwise."""
+    from cases import percell_relerr, percell_torque_relerr
+    N, cs = (512, 512, 32), (2e-9, 2e-9, 2e-9)
+    m = random_unit((N[2], N[1], N[0]), 2)
+    params = dict(Ms=1.1e6, Aex=1.6e-11, Dind=3e-3, Ku1=8e5, alpha=0.1)
+    so = oracle.Sim(N, cs, dtype=np.float64)
+    setup(so, oracle, dtype=np.float64, **params)
+    so.set_m(m)
+    b_o = so.beff()
+    t_o = so.torque()
+    del so
+    _say("oracle done")
+    sg = gpu.Sim(N, cs, dtype=np.float32)
+    setup(sg, gpu, dtype=np.float32, **params)
+    sg.set_m(m)
+    sg.torque()                       # plan and tensor setup outside the trace
+    with gpu.KernelTrace() as tr:
+        t_g = sg.torque()
+    names = tr.names
+    _say("traced", names)
+    for k in TIMED_512x512x32:
+        assert sum(n.startswith(k) for n in names) == 1, (k, names)
+    b_g = sg.beff()
+    eb, et = percell_relerr(b_g, b_o), percell_torque_relerr(t_g, t_o, b_o)
+    _say("per cell: B", eb, "tau", et, "normwise:", relerr(b_g, b_o), relerr(t_g, t_o))
+    assert eb <= 1e-5 and et <= 1e-5
+    assert relerr(b_g, b_o) <= 1e-5 and relerr(t_g, t_o) <= 1e-5
+
+
+@pytest.mark.parametrize("N,expect", [((512, 64, 32), ["void um::k_xfwd<float, 512, 8>(", "void um::k_xi4<float, 512>(",
+                                                       "void um::k_convz<float, 32, 8, 4>(",
+                                                       "void um::k_field_torque4<float, false, 2, false>("]),
+                                      ((32, 512, 32), ["void um::k_coly<float, 32, -1, 16, true, 1>(",
+                                                       "void um::k_coly<float, 32, 1, 16, true, 1>(",
+                                                       "void um::k_convz<float, 32, 8, 4>(",
+                                                       "void um::k_field_torque4<float, false, 2, false>("])])
+def test_timed_kernels_slabs_vs_oracle(gpu, oracle, N, expect):
+    """Smaller grids that select the timed X passes (Nx = 512, >= 2048 rows) or
+    the timed Y passes (Ny = 512) with the timed Z pass and four-cell field
+    kernel, config-2 physics, FP32 vs the FP64 oracle per cell."""
+    from cases import percell_relerr, percell_torque_relerr
+    cs = (2e-9, 2e-9, 2e-9)
+    m = random_unit((N[2], N[1], N[0]), 3)
+    params = dict(Ms=1.1e6, Aex=1.6e-11, Dind=3e-3, Ku1=8e5, alpha=0.1)
+    so, sg = _both(oracle, gpu, N, cs, 0, **params)
+    so.set_m(m)
+    sg.set_m(m)
+    sg.torque()
+    with gpu.KernelTrace() as tr:
+        t_g = sg.torque()
+    for k in expect:
+        assert any(n.startswith(k) for n in tr.names), (k, tr.names)
+    b_o, t_o, b_g = so.beff(), so.torque(), sg.beff()
+    eb, et = percell_relerr(b_g, b_o), percell_torque_relerr(t_g, t_o, b_o)
+    _say(N, "per cell: B", eb, "tau", et)
+    assert eb <= 1e-5 and et <= 1e-5
+
+
 # ---- full-size properties (512 x 512 x 32) -----------------------------------------------------
 def test_512x512x32_full_size_properties(gpu):
     """The 512x512x32 workload at full size: FP32 field within 1e-5 of the FP64 GPU path, demag

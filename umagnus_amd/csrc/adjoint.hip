@@ -234,17 +234,17 @@ __global__ void k_adj_cast(T *dst, const double *src, long N) {
 
 template <typename T>
 void launch_adj_stage(const AdjArgs<T> &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_stage<T>, dim3((unsigned)adj_blocks(a.N)), dim3(ADJ_NT), 0, s, a);
+  UM_LAUNCH(k_adj_stage<T>, dim3((unsigned)adj_blocks(a.N)), dim3(ADJ_NT), 0, s, a);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_adj_final(const AdjFinalArgs<T> &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_final<T>, dim3((unsigned)adj_blocks(a.N)), dim3(ADJ_NT), 0, s, a);
+  UM_LAUNCH(k_adj_final<T>, dim3((unsigned)adj_blocks(a.N)), dim3(ADJ_NT), 0, s, a);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_adj_cast(T *dst, const double *src, long N, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_cast<T>, dim3((unsigned)adj_blocks(N)), dim3(ADJ_NT), 0, s, dst, src, N);
+  UM_LAUNCH(k_adj_cast<T>, dim3((unsigned)adj_blocks(N)), dim3(ADJ_NT), 0, s, dst, src, N);
   UM_HIP(hipGetLastError());
 }
 

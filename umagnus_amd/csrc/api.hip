@@ -1,5 +1,10 @@
 // C ABI of libumhip.so (declared in include/umhip.h).
+#include <cxxabi.h>
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -11,6 +16,24 @@ namespace um {
 
 static thread_local std::string g_err;
 void set_last_error(const std::string &msg) { g_err = msg; }
+
+// launch trace: the host stub of a __global__ function carries the kernel's
+// mangled name, so dladdr + the C++ demangler name the launched variant
+bool g_trace_on = false;
+static std::mutex g_trace_mu;
+static std::vector<std::string> g_trace;
+void trace_kernel(const void *stub) {
+  Dl_info info{};
+  std::string name = "?";
+  if (dladdr(stub, &info) && info.dli_sname) {
+    int st = 0;
+    char *d = abi::__cxa_demangle(info.dli_sname, nullptr, nullptr, &st);
+    name = (st == 0 && d) ? d : info.dli_sname;
+    std::free(d);
+  }
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  g_trace.push_back(name);
+}
 
 static std::mutex g_mu;
 static bool g_inited = false;
@@ -429,6 +452,164 @@ int um_mul(int prec, void *dst, const void *a, const void *b, int N) {
   });
 }
 
+// ---- fused evaluation, RK stage and step-controller norms -------------------------------
+extern "C++" {
+template <typename T>
+static Par<T> upar(const um_param &p) { return Par<T>{(const T *)p.arr, (T)p.mul}; }
+static bool unz(const um_param &p) { return p.arr != nullptr || p.mul != 0; }
+static bool all_or_none(const void *const p[3]) { return (p[0] && p[1] && p[2]) || (!p[0] && !p[1] && !p[2]); }
+
+template <typename T>
+static FieldArgs<T> field_args_from(const um_field_args &u) {
+  FieldArgs<T> a{};
+  a.g = Mesh{u.Nx, u.Ny, u.Nz, u.pbc & 7};
+  const double *cs = u.cellsize;
+  // float32(2/c^2) in the FP32 build (opencl/exchange.go:17-20), cell sizes as T (dmi.go)
+  a.wx = (T)(2 / (cs[0] * cs[0]));
+  a.wy = (T)(2 / (cs[1] * cs[1]));
+  a.wz = (T)(2 / (cs[2] * cs[2]));
+  a.cx = (T)cs[0]; a.cy = (T)cs[1]; a.cz = (T)cs[2];
+  for (int c = 0; c < 3; c++) {
+    a.m[c] = (const T *)u.m[c];
+    a.bd[c] = (const T *)u.bdemag[c];
+    a.tau[c] = (T *)u.torque[c];
+    a.beff[c] = (T *)u.beff[c];
+    a.u[c] = upar<T>(u.anisU[c]);
+    a.c1[c] = upar<T>(u.anisC1[c]);
+    a.c2[c] = upar<T>(u.anisC2[c]);
+    a.bext[c] = upar<T>(u.Bext[c]);
+    a.bextLUT[c] = (const T *)u.bext_lut[c];
+    a.badd[c] = (const T *)u.badd[c];
+    a.tadd[c] = (const T *)u.tadd[c];
+    a.stt.J[c] = upar<T>(u.J[c]);
+    a.stt.fixp[c] = upar<T>(u.fixed_layer[c]);
+  }
+  a.regions = u.regions;
+  a.Ms = upar<T>(u.Msat);
+  a.aLUT = (const T *)u.aex_lut;
+  a.dLUT = (const T *)u.dmi_lut;
+  a.exch_mode = !u.aex_lut ? 0 : !u.dmi_lut ? 1 : u.dmi_bulk ? 3 : 2;
+  a.openbc = u.openbc;
+  a.has_uniax = unz(u.Ku1) || unz(u.Ku2);
+  a.Ku1 = upar<T>(u.Ku1); a.Ku2 = upar<T>(u.Ku2);
+  a.has_cubic = unz(u.Kc1) || unz(u.Kc2) || unz(u.Kc3);
+  a.Kc1 = upar<T>(u.Kc1); a.Kc2 = upar<T>(u.Kc2); a.Kc3 = upar<T>(u.Kc3);
+  a.has_bext = unz(u.Bext[0]) || unz(u.Bext[1]) || unz(u.Bext[2]);
+  a.has_therm = u.thermal != 0;
+  a.temp = upar<T>(u.Temp);
+  a.kth = (T)u.thermal_k;
+  a.seed = u.seed;
+  a.step = u.step;
+  a.step_dev = nullptr;
+  a.stream = u.stream;
+  a.alpha = upar<T>(u.alpha);
+  a.precess = u.precess;
+  a.frozenLUT = (const T *)u.frozen_lut;
+  const bool has_j = unz(u.J[0]) || unz(u.J[1]) || unz(u.J[2]);
+  a.has_zl = has_j && u.zhangli;
+  a.has_sl = has_j && u.slonczewski && (unz(u.fixed_layer[0]) || unz(u.fixed_layer[1]) || unz(u.fixed_layer[2]));
+  a.stt.xi = upar<T>(u.xi);
+  a.stt.pol = upar<T>(u.pol);
+  a.stt.lambda = upar<T>(u.lambda);
+  a.stt.epsp = upar<T>(u.eps_prime);
+  a.stt.thick = upar<T>(u.free_layer_thickness);
+  a.stt.mesh_thick = (T)(u.Nz * cs[2]);
+  a.stt.flp = (T)u.fixed_layer_top;
+  return a;
+}
+}  // extern "C++"
+
+int um_field_torque(int prec, const um_field_args *u) {
+  return guard([&] {
+    UM_REQUIRE(u, UM_EINVAL, "um_field_torque: NULL arguments");
+    UM_REQUIRE(u->Nx > 0 && u->Ny > 0 && u->Nz > 0 && (long)u->Nx * u->Ny * u->Nz < (1L << 31), UM_EINVAL,
+               "um_field_torque: grid size");
+    for (int c = 0; c < 3; c++)
+      UM_REQUIRE(std::isfinite(u->cellsize[c]) && u->cellsize[c] > 0, UM_EINVAL, "um_field_torque: cell size");
+    check3(u->m, "um_field_torque: m");
+    UM_REQUIRE(all_or_none(u->bdemag) && all_or_none(u->torque) && all_or_none(u->beff) && all_or_none(u->badd) &&
+                   all_or_none(u->tadd) && all_or_none(u->bext_lut),
+               UM_EINVAL, "um_field_torque: a vector must have all three components or none");
+    UM_REQUIRE(u->torque[0] || u->beff[0], UM_EINVAL, "um_field_torque: neither torque nor beff requested");
+    UM_REQUIRE(!u->dmi_lut || u->aex_lut, UM_EINVAL, "um_field_torque: DMI needs the exchange LUT");
+    const bool has_j = unz(u->J[0]) || unz(u->J[1]) || unz(u->J[2]);
+    UM_REQUIRE(!has_j || !(u->zhangli || u->slonczewski) || unz(u->pol), UM_EINVAL,
+               "spin polarization should not be 0");
+    DISPATCH(prec, launch_field_torque(field_args_from<float>(*u), lib_stream()),
+             launch_field_torque(field_args_from<double>(*u), lib_stream()));
+  });
+}
+
+int um_rk45dp_stage(int prec, void *const dst[3], int nsrc, const void *const *src, const double *fac,
+                    const void *vol, int normalize, int N) {
+  return guard([&] {
+    check3((const void *const *)dst, "um_rk45dp_stage: dst");
+    UM_REQUIRE(src && fac && nsrc >= 1 && nsrc <= 8 && N >= 0, UM_EINVAL, "um_rk45dp_stage: bad arguments");
+    for (int j = 0; j < 3 * nsrc; j++) UM_REQUIRE(src[j], UM_EINVAL, "um_rk45dp_stage: NULL source component");
+    auto go = [&](auto zero) {
+      using T = decltype(zero);
+      StageArgs<T> a{};
+      a.nsrc = nsrc;
+      a.ncomp = 3;
+      a.group3 = 1;
+      a.normalize = normalize != 0;
+      for (int j = 0; j < nsrc; j++) {
+        for (int c = 0; c < 3; c++) a.src[j][c] = (const T *)src[3 * j + c];
+        a.fac[j] = (T)fac[j];
+      }
+      for (int c = 0; c < 3; c++) a.dst[c] = (T *)dst[c];
+      a.vol = (const T *)vol;
+      a.N = N;
+      launch_stage(a, lib_stream());
+    };
+    DISPATCH(prec, go(0.0f), go(0.0));
+  });
+}
+
+// Err = sum e_j k_j with delM = k[nk-1], as the fused six-source error kernel:
+// shorter lists repeat delM with zero coefficients
+extern "C++" {
+template <typename T>
+static void rk_err_args(ErrArgs<T> &a, int nk, const void *const *k, const double *e, long N, unsigned long long *res) {
+  a.mode = 0;
+  for (int j = 0; j < 6; j++) {
+    const int s = j < nk - 1 ? j : (j == 5 ? nk - 1 : -1);
+    for (int c = 0; c < 3; c++) a.k[j][c] = (const T *)k[3 * (s < 0 ? nk - 1 : s) + c];
+    a.e[j] = s < 0 ? T(0) : (T)e[s];
+  }
+  a.N = N;
+  a.res = res;
+}
+static void check_rk_err(int nk, const void *const *k, const double *e, double h, int N) {
+  UM_REQUIRE(k && e && nk >= 1 && nk <= 6 && N >= 0 && std::isfinite(h), UM_EINVAL, "um_rk45dp_error: bad arguments");
+  for (int j = 0; j < 3 * nk; j++) UM_REQUIRE(k[j], UM_EINVAL, "um_rk45dp_error: NULL stage component");
+}
+}  // extern "C++"
+
+int um_rk45dp_error(int prec, int nk, const void *const *k, const double *e, double h, int N, double out[3]) {
+  return guard([&] {
+    check_rk_err(nk, k, e, h, N);
+    UM_REQUIRE(out, UM_EINVAL, "um_rk45dp_error: NULL out");
+    hipStream_t s = lib_stream();
+    RedScratch &r = scratch();
+    fill_bytes(r.red.p, r.red.bytes, s);
+    auto go = [&](auto zero) {
+      using T = decltype(zero);
+      ErrArgs<T> a{};
+      rk_err_args<T>(a, nk, k, e, N, (unsigned long long *)r.red.p);
+      launch_err(a, s);
+    };
+    DISPATCH(prec, go(0.0f), go(0.0));
+    UM_HIP(hipMemcpyAsync(r.h, r.red.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    UM_HIP(hipStreamSynchronize(s));
+    double v[3];
+    for (int i = 0; i < 3; i++) memcpy(&v[i], &r.h[i], 8);
+    out[0] = std::sqrt(v[0]) * h;
+    out[1] = v[2];
+    out[2] = std::sqrt(v[1]);
+  });
+}
+
 // ---- reductions ---------------------------------------------------------------------
 int um_reduce_sum(int prec, const void *x, int N, double *out) {
   return guard([&] {
@@ -521,6 +702,28 @@ int um_demag_schedule(void *plan, int out[8]) {
   return guard([&] {
     UM_REQUIRE(plan && out, UM_EINVAL, "um_demag_schedule: bad arguments");
     demag_schedule((DemagBase *)plan, out);
+  });
+}
+int um_trace_kernels(int on) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    g_trace.clear();
+    g_trace_on = on != 0;
+  });
+}
+int um_traced_kernels(char *out, size_t out_len, size_t *needed) {
+  return guard([&] {
+    std::string all;
+    {
+      std::lock_guard<std::mutex> lk(g_trace_mu);
+      for (const auto &n : g_trace) { all += n; all += '\n'; }
+    }
+    if (needed) *needed = all.size() + 1;
+    if (out && out_len > 0) {
+      const size_t k = std::min(out_len - 1, all.size());
+      memcpy(out, all.data(), k);
+      out[k] = 0;
+    }
   });
 }
 int um_set_kernel_cache(const char *dir) { return guard([&] { set_kernel_cache(dir); }); }

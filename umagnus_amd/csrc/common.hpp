@@ -65,6 +65,18 @@ struct Par {
 template <typename T>
 inline Par<T> par(const void *arr, double mul) { return Par<T>{(const T *)arr, (T)mul}; }
 
+// ---- launch trace ----------------------------------------------------------
+// While tracing is on (um_trace_kernels), every launch through UM_LAUNCH
+// records the demangled name of the kernel it starts, so a caller can check
+// which compiled variant a plan or a simulation selects.
+extern bool g_trace_on;
+void trace_kernel(const void *host_stub);
+#define UM_LAUNCH(kern, ...)                                                \
+  do {                                                                      \
+    if (::um::g_trace_on) ::um::trace_kernel((const void *)(kern));         \
+    hipLaunchKernelGGL(kern, __VA_ARGS__);                                  \
+  } while (0)
+
 // ---- launch helpers --------------------------------------------------------
 inline unsigned blocks_for(long n, int threads, long cap = 1L << 20) {
   long b = (n + threads - 1) / threads;

@@ -942,11 +942,11 @@ static void launch_convz(const ConvZArgs<T> &a, int Nz, hipStream_t s) {
   constexpr int FR = ConvZTable<T>::FR, G = ConvZTable<T>::G;
   const dim3 grid((a.H + CONVZ_COLS - 1) / CONVZ_COLS, a.Py / 2 + 1);
   switch (Nz) {
-    case 8: hipLaunchKernelGGL((k_convz<T, 8, FR, G>), grid, dim3(CONVZ_NT), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((k_convz<T, 16, FR, G>), grid, dim3(CONVZ_NT), 0, s, a); break;
+    case 8: UM_LAUNCH((k_convz<T, 8, FR, G>), grid, dim3(CONVZ_NT), 0, s, a); break;
+    case 16: UM_LAUNCH((k_convz<T, 16, FR, G>), grid, dim3(CONVZ_NT), 0, s, a); break;
     case 32:
       if constexpr (std::is_same<T, float>::value) {
-        hipLaunchKernelGGL((k_convz<T, 32, FR, G>), grid, dim3(CONVZ_NT), 0, s, a);
+        UM_LAUNCH((k_convz<T, 32, FR, G>), grid, dim3(CONVZ_NT), 0, s, a);
         break;
       }
       [[fallthrough]];
@@ -1348,7 +1348,7 @@ static void allow_lds(K kern, size_t bytes) {
 #define UM_CT_CASE(KERN, N, NL, LDSB)                                        \
   if (n == N && nl == NL) {                                                  \
     allow_lds(KERN, LDSB);                                                   \
-    hipLaunchKernelGGL(KERN, grid, dim3(FFT_NT), LDSB, s, a);                \
+    UM_LAUNCH(KERN, grid, dim3(FFT_NT), LDSB, s, a);                \
     UM_HIP(hipGetLastError());                                               \
     return;                                                                  \
   }
@@ -1363,7 +1363,7 @@ static bool launch_xi4(const XiArgs<T> &a, int n, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (!x4_ok<T>(n, a.Nx)) return false;
     auto go = [&](auto kern, int rows_per_wg) {
-      hipLaunchKernelGGL(kern, dim3((a.rows + rows_per_wg - 1) / rows_per_wg, 3), dim3(XF4_NT), 0, s, a);
+      UM_LAUNCH(kern, dim3((a.rows + rows_per_wg - 1) / rows_per_wg, 3), dim3(XF4_NT), 0, s, a);
       UM_HIP(hipGetLastError());
       return true;
     };
@@ -1411,7 +1411,7 @@ static void launch_xfwd(const XArgs<T> &a, int n, int nl, dim3 grid, hipStream_t
     UM_CT_CASE((k_xfwd<double, 256, 8>), 256, 8, ct)
     UM_CT_CASE((k_xfwd<double, 512, 4>), 512, 4, ct)
   }
-  hipLaunchKernelGGL((k_xfwd<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
+  UM_LAUNCH((k_xfwd<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
@@ -1440,7 +1440,7 @@ static void launch_xinv(const XiArgs<T> &a, int n, int nl, dim3 grid, hipStream_
     UM_CT_CASE((k_xinv<double, 256, 8>), 256, 8, ct)
     UM_CT_CASE((k_xinv<double, 512, 4>), 512, 4, ct)
   }
-  hipLaunchKernelGGL((k_xinv<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
+  UM_LAUNCH((k_xinv<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
   UM_HIP(hipGetLastError());
 }
 template <typename T, int DIR>
@@ -1460,7 +1460,7 @@ static void launch_col(const ColArgs<T> &a, int n, int nl, dim3 grid, hipStream_
     UM_CT_CASE((k_colfft<double, DIR, 512, 4>), 512, 4, ct)
     UM_CT_CASE((k_colfft<double, DIR, 1024, 4>), 1024, 4, ct)
   }
-  hipLaunchKernelGGL((k_colfft<T, DIR, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
+  UM_LAUNCH((k_colfft<T, DIR, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
   UM_HIP(hipGetLastError());
 }
 // Register four-step Y pass when P = A*A (A = 8, 16, 32 in FP32; 8, 16 in FP64)
@@ -1482,7 +1482,7 @@ static bool launch_coly(const ColArgs<T> &a, int P, int nz, hipStream_t s) {
     if (lds > 65536) lds /= 2;
     allow_lds(kern, lds);
     const dim3 grid((a.H + C - 1) / C, nz / ZB, 3);
-    hipLaunchKernelGGL(kern, grid, dim3(A * C), lds, s, a);
+    UM_LAUNCH(kern, grid, dim3(A * C), lds, s, a);
     UM_HIP(hipGetLastError());
     return true;
   };
@@ -1547,7 +1547,7 @@ static void launch_conv(const ConvArgs<T> &a, int n, int B, dim3 grid, hipStream
       UM_CT_CASE((k_conv<double, 512, 2>), 512, 2, ct)
     }
   }
-  hipLaunchKernelGGL((k_conv<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * 3 * B * n, s, a);
+  UM_LAUNCH((k_conv<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * 3 * B * n, s, a);
   UM_HIP(hipGetLastError());
 }
 #undef UM_CT_CASE
@@ -1649,7 +1649,7 @@ static void launch_xinv_field(const XiArgs<T> &a, const FieldArgs<T> &f, int n, 
   const bool reg = f.regions != nullptr;
   auto go = [&](auto kern) {
     allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, grid, dim3(FFT_NT), lds, s, a, f);
+    UM_LAUNCH(kern, grid, dim3(FFT_NT), lds, s, a, f);
     UM_HIP(hipGetLastError());
   };
 #define UM_XF(CN, RR)                                                                   \
@@ -1855,10 +1855,10 @@ struct DemagPlan : DemagBase {
       if (xodd) {
         xa.f = dx2.len; xa.nl = capd / P[0]; xa.fhalf.init(H);
         dim3 gx((xa.rows + xa.nl - 1) / xa.nl, 1);
-        hipLaunchKernelGGL(k_xfwd_odd<double>, gx, dim3(FFT_NT), sizeof(double2) * xa.nl * P[0], s, xa);
+        UM_LAUNCH(k_xfwd_odd<double>, gx, dim3(FFT_NT), sizeof(double2) * xa.nl * P[0], s, xa);
       } else {
         dim3 gx((xa.rows + xa.nl - 1) / xa.nl, 1);
-        hipLaunchKernelGGL((k_xfwd<double, 0, 0>), gx, dim3(FFT_NT), sizeof(double2) * xa.nl * (P[0] / 2), s, xa);
+        UM_LAUNCH((k_xfwd<double, 0, 0>), gx, dim3(FFT_NT), sizeof(double2) * xa.nl * (P[0] / 2), s, xa);
       }
       UM_HIP(hipGetLastError());
       if (P[1] > 1) {
@@ -1866,7 +1866,7 @@ struct DemagPlan : DemagBase {
         ca.f = dy.len; ca.nl = std::min(capd / P[1], 64); ca.fnl.init(ca.nl); ca.H = H; ca.nvalid = P[1]; ca.nout = P[1];
         ca.estride = Hpd; ca.pstride = (long)P[1] * Hpd; ca.buf[0] = (double2 *)sp.p;
         dim3 g((H + ca.nl - 1) / ca.nl, P[2], 1);
-        hipLaunchKernelGGL((k_colfft<double, -1, 0, 0>), g, dim3(FFT_NT), sizeof(double2) * ca.nl * P[1], s, ca);
+        UM_LAUNCH((k_colfft<double, -1, 0, 0>), g, dim3(FFT_NT), sizeof(double2) * ca.nl * P[1], s, ca);
         UM_HIP(hipGetLastError());
       }
       if (P[2] > 1) {
@@ -1874,10 +1874,10 @@ struct DemagPlan : DemagBase {
         ca.f = dz.len; ca.nl = std::min(capd / P[2], 64); ca.fnl.init(ca.nl); ca.H = H; ca.nvalid = P[2]; ca.nout = P[2];
         ca.estride = (long)P[1] * Hpd; ca.pstride = Hpd; ca.buf[0] = (double2 *)sp.p;
         dim3 g((H + ca.nl - 1) / ca.nl, P[1], 1);
-        hipLaunchKernelGGL((k_colfft<double, -1, 0, 0>), g, dim3(FFT_NT), sizeof(double2) * ca.nl * P[2], s, ca);
+        UM_LAUNCH((k_colfft<double, -1, 0, 0>), g, dim3(FFT_NT), sizeof(double2) * ca.nl * P[2], s, ca);
         UM_HIP(hipGetLastError());
       }
-      hipLaunchKernelGGL(k_extract_kspec<T>, dim3(blocks_for(kcount, 256, 4096)), dim3(256), 0, s,
+      UM_LAUNCH(k_extract_kspec<T>, dim3(blocks_for(kcount, 256, 4096)), dim3(256), 0, s,
                          (const double2 *)sp.p, out, H, Hk, Hpd, P[1], KHy, KHz, scale,
                          (unsigned long long *)mx.p);
       UM_HIP(hipGetLastError());
@@ -1904,7 +1904,7 @@ struct DemagPlan : DemagBase {
     dim3 g((a.rows + nl_x - 1) / nl_x, 3);
     if (xodd) {
       a.f = tx2.len; a.fhalf.init(H);
-      hipLaunchKernelGGL(k_xfwd_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
+      UM_LAUNCH(k_xfwd_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
       UM_HIP(hipGetLastError());
     } else {
       launch_xfwd<T>(a, P[0] / 2, nl_x, g, s);
@@ -1960,7 +1960,7 @@ struct DemagPlan : DemagBase {
     dim3 g((a.rows + nl_xi - 1) / nl_xi, 3);
     if (xodd) {
       a.f = tx2.len; a.fhalf.init(H);
-      hipLaunchKernelGGL(k_xinv_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_xi * P[0], s, a);
+      UM_LAUNCH(k_xinv_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_xi * P[0], s, a);
       UM_HIP(hipGetLastError());
     } else {
       launch_xinv<T>(a, P[0] / 2, nl_xi, g, s, xi4);
@@ -2323,7 +2323,7 @@ void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3]
     groups.back().t1 = ti + 1;
   }
   const long total = (long)a.H[0] * a.H[1] * a.H[2] * 3;
-  hipLaunchKernelGGL(k_demag_bf, dim3(blocks_for(total, 64, 1 << 16)), dim3(64), 0, s, a);
+  UM_LAUNCH(k_demag_bf, dim3(blocks_for(total, 64, 1 << 16)), dim3(64), 0, s, a);
   UM_HIP(hipGetLastError());
   if (!groups.empty()) {
     const int nchunk = (int)chunk_task.size();
@@ -2332,10 +2332,10 @@ void demag_kernel_bf_device(const int N[3], const int pbc[3], const double cs[3]
     UM_HIP(hipMemcpyAsync(dt.p, sorted.data(), dt.bytes, hipMemcpyHostToDevice, s));
     UM_HIP(hipMemcpyAsync(dg.p, groups.data(), dg.bytes, hipMemcpyHostToDevice, s));
     UM_HIP(hipMemcpyAsync(dc.p, chunk_task.data(), dc.bytes, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_demag_bf_near, dim3((nchunk + 3) / 4), dim3(256), 0, s, a, (const BfTask *)dt.p,
+    UM_LAUNCH(k_demag_bf_near, dim3((nchunk + 3) / 4), dim3(256), 0, s, a, (const BfTask *)dt.p,
                        (const int *)dc.p, nchunk, (double *)part.p);
     UM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_demag_bf_near_add, dim3(((int)groups.size() + 63) / 64), dim3(64), 0, s, a,
+    UM_LAUNCH(k_demag_bf_near_add, dim3(((int)groups.size() + 63) / 64), dim3(64), 0, s, a,
                        (const BfGroup *)dg.p, (int)groups.size(), (const BfTask *)dt.p, (const double *)part.p);
     UM_HIP(hipGetLastError());
     UM_HIP(hipStreamSynchronize(s));   // before the host tables and device buffers go
@@ -2353,7 +2353,7 @@ void demag_kernel_mirror(const int P[3], double *const K[6], hipStream_t s) {
   for (int i = 0; i < 3; i++) a.P[i] = P[i];
   for (int c = 0; c < 6; c++) a.K[c] = K[c];
   const size_t np = (size_t)P[0] * P[1] * P[2];
-  hipLaunchKernelGGL(k_demag_mirror, dim3(blocks_for(np, 256, 1 << 16)), dim3(256), 0, s, a);
+  UM_LAUNCH(k_demag_mirror, dim3(blocks_for(np, 256, 1 << 16)), dim3(256), 0, s, a);
   UM_HIP(hipGetLastError());
 }
 

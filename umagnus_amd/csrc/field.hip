@@ -213,6 +213,10 @@ __global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
     const V3<T> h = thermal_cell(I, a.seed, stp, a.stream, a.kth, a.Ms, a.temp, a.alpha);
     B.x += h.x; B.y += h.y; B.z += h.z;
   }
+  if (a.badd[0]) {   // AddCustomField (engine/effectivefield.go:23)
+    const V3<T> h = load3(a.badd, I);
+    B.x += h.x; B.y += h.y; B.z += h.z;
+  }
   if (a.beff[0]) {
     a.beff[0][I] = B.x;
     a.beff[1][I] = B.y;
@@ -233,6 +237,10 @@ __global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
     }
     if (a.has_sl) {
       const V3<T> z = slonczewski_cell(I, m0, a.Ms, a.alpha, a.stt);
+      t.x += z.x; t.y += z.y; t.z += z.z;
+    }
+    if (a.tadd[0]) {   // AddCustomTorques (engine/torque.go:129)
+      const V3<T> z = load3(a.tadd, I);
       t.x += z.x; t.y += z.y; t.z += z.z;
     }
     if (a.frozenLUT && a.frozenLUT[a.regions ? a.regions[I] : 0] != T(0)) t = v3(T(0), T(0), T(0));
@@ -290,8 +298,8 @@ void launch_field_torque(const FieldArgs<T> &a, hipStream_t s) {
     const bool stt = a.has_zl || a.has_sl;
 #define UM_F4(R, E)                                                                    \
   do {                                                                                 \
-    if (stt) hipLaunchKernelGGL((k_field_torque4<T, R, E, true>), grid, dim3(256), 0, s, a);  \
-    else hipLaunchKernelGGL((k_field_torque4<T, R, E, false>), grid, dim3(256), 0, s, a);     \
+    if (stt) UM_LAUNCH((k_field_torque4<T, R, E, true>), grid, dim3(256), 0, s, a);  \
+    else UM_LAUNCH((k_field_torque4<T, R, E, false>), grid, dim3(256), 0, s, a);     \
   } while (0)
     if constexpr (sizeof(T) == 4) {
       if (a.exch_mode == 0) { if (reg) UM_F4(true, 0); else UM_F4(false, 0); }
@@ -305,7 +313,7 @@ void launch_field_torque(const FieldArgs<T> &a, hipStream_t s) {
   }
   dim3 blk(64, 4, 1);
   dim3 grd((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.g.Nz);
-  hipLaunchKernelGGL(k_field_torque<T>, grd, blk, 0, s, a);
+  UM_LAUNCH(k_field_torque<T>, grd, blk, 0, s, a);
   UM_HIP(hipGetLastError());
 }
 template void launch_field_torque<float>(const FieldArgs<float> &, hipStream_t);
@@ -317,21 +325,21 @@ static inline dim3 grid3(const Mesh &g) { return dim3((g.Nx + 63) / 64, (g.Ny + 
 template <typename T>
 void add_exchange(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const uint8_t *regions, T wx,
                   T wy, T wz, Mesh g, hipStream_t s) {
-  hipLaunchKernelGGL(k_addexchange<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2],
+  UM_LAUNCH(k_addexchange<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2],
                      (const T *)m[0], (const T *)m[1], (const T *)m[2], Ms, aLUT, regions, wx, wy, wz, g);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void add_dmi(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const T *dLUT,
              const uint8_t *regions, T cx, T cy, T cz, Mesh g, int openbc, hipStream_t s) {
-  hipLaunchKernelGGL(k_adddmi<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2], (const T *)m[0],
+  UM_LAUNCH(k_adddmi<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2], (const T *)m[0],
                      (const T *)m[1], (const T *)m[2], Ms, aLUT, dLUT, regions, cx, cy, cz, g, openbc);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void add_dmi_bulk(void *const B[3], const void *const m[3], Par<T> Ms, const T *aLUT, const T *dLUT,
                   const uint8_t *regions, T cx, T cy, T cz, Mesh g, int openbc, hipStream_t s) {
-  hipLaunchKernelGGL(k_adddmibulk<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2],
+  UM_LAUNCH(k_adddmibulk<T>, grid3(g), dim3(64, 4), 0, s, (T *)B[0], (T *)B[1], (T *)B[2],
                      (const T *)m[0], (const T *)m[1], (const T *)m[2], Ms, aLUT, dLUT, regions, cx, cy, cz, g,
                      openbc);
   UM_HIP(hipGetLastError());
@@ -339,14 +347,14 @@ void add_dmi_bulk(void *const B[3], const void *const m[3], Par<T> Ms, const T *
 template <typename T>
 void add_stt(void *const t[3], const void *const m[3], Par<T> Ms, Par<T> alpha, const SttPar<T> &p, T cx, T cy,
              T cz, Mesh g, int zl, int sl, hipStream_t s) {
-  hipLaunchKernelGGL(k_addstt<T>, grid3(g), dim3(64, 4), 0, s, (T *)t[0], (T *)t[1], (T *)t[2], (const T *)m[0],
+  UM_LAUNCH(k_addstt<T>, grid3(g), dim3(64, 4), 0, s, (T *)t[0], (T *)t[1], (T *)t[2], (const T *)m[0],
                      (const T *)m[1], (const T *)m[2], Ms, alpha, p, cx, cy, cz, g, zl, sl);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void add_uniaxial2(void *const B[3], const void *const m[3], Par<T> Ms, Par<T> K1, Par<T> K2, const Par<T> u[3],
                    int N, hipStream_t s) {
-  hipLaunchKernelGGL(k_adduniaxial2<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)B[0], (T *)B[1],
+  UM_LAUNCH(k_adduniaxial2<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)B[0], (T *)B[1],
                      (T *)B[2], (const T *)m[0], (const T *)m[1], (const T *)m[2], Ms, K1, K2, u[0], u[1], u[2], N);
   UM_HIP(hipGetLastError());
 }
@@ -361,32 +369,32 @@ void add_cubic2(void *const B[3], const void *const m[3], Par<T> Ms, Par<T> K1, 
     a.c2[c] = c2[c];
   }
   a.Ms = Ms; a.K1 = K1; a.K2 = K2; a.K3 = K3; a.N = N;
-  hipLaunchKernelGGL(k_addcubic2<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, a);
+  UM_LAUNCH(k_addcubic2<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, a);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void region_addv(void *const d[3], const void *const l[3], const uint8_t *regions, int N, hipStream_t s) {
-  hipLaunchKernelGGL(k_regionaddv<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)d[0], (T *)d[1],
+  UM_LAUNCH(k_regionaddv<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)d[0], (T *)d[1],
                      (T *)d[2], (const T *)l[0], (const T *)l[1], (const T *)l[2], regions, N);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void zeromask(void *dst, const void *lut, const uint8_t *regions, int N, hipStream_t s) {
-  hipLaunchKernelGGL(k_zeromask<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)dst, (const T *)lut,
+  UM_LAUNCH(k_zeromask<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)dst, (const T *)lut,
                      regions, N);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void lltorque(void *const t[3], const void *const m[3], const void *const B[3], Par<T> alpha, int N,
               hipStream_t s) {
-  hipLaunchKernelGGL(k_lltorque<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)t[0], (T *)t[1],
+  UM_LAUNCH(k_lltorque<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)t[0], (T *)t[1],
                      (T *)t[2], (const T *)m[0], (const T *)m[1], (const T *)m[2], (const T *)B[0],
                      (const T *)B[1], (const T *)B[2], alpha, N);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void llnoprecess(void *const t[3], const void *const m[3], const void *const B[3], int N, hipStream_t s) {
-  hipLaunchKernelGGL(k_llnoprecess<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)t[0], (T *)t[1],
+  UM_LAUNCH(k_llnoprecess<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)t[0], (T *)t[1],
                      (T *)t[2], (const T *)m[0], (const T *)m[1], (const T *)m[2], (const T *)B[0],
                      (const T *)B[1], (const T *)B[2], N);
   UM_HIP(hipGetLastError());
@@ -394,7 +402,7 @@ void llnoprecess(void *const t[3], const void *const m[3], const void *const B[3
 template <typename T>
 void thermal_field(void *const B[3], uint64_t seed, uint64_t step, uint32_t stream, T k, Par<T> Ms, Par<T> temp,
                    Par<T> alpha, int N, hipStream_t s) {
-  hipLaunchKernelGGL(k_thermal<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)B[0], (T *)B[1],
+  UM_LAUNCH(k_thermal<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, (T *)B[0], (T *)B[1],
                      (T *)B[2], seed, step, stream, k, Ms, temp, alpha, N);
   UM_HIP(hipGetLastError());
 }

@@ -357,6 +357,8 @@ struct FieldArgs {
   const T *frozenLUT;    // FrozenSpins per region or nullptr
   int has_zl, has_sl;    // Zhang-Li / Slonczewski torque on (one-cell kernel only)
   SttPar<T> stt;
+  const T *badd[3];      // extra field added last (AddCustomField, engine/customfield.go:68) or nullptr
+  const T *tadd[3];      // extra torque added before FreezeSpins (AddCustomTorques, engine/torque.go:115) or nullptr
 };
 
 // ---- four consecutive x cells per thread ----------------------------------------
@@ -647,6 +649,11 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
       const V3<T> h = thermal_cell(I, a.seed, stp, a.stream, a.kth, a.Ms, a.temp, a.alpha);
       B.x += h.x; B.y += h.y; B.z += h.z;
     }
+    if (a.badd[0]) {
+      B.x += a.badd[0][I];
+      B.y += a.badd[1][I];
+      B.z += a.badd[2][I];
+    }
     ob[0][w] = B.x; ob[1][w] = B.y; ob[2][w] = B.z;
     V3<T> t;
     if (a.precess) {
@@ -688,6 +695,11 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
         const V3<T> z = slonczewski_cell(I, m0, a.Ms, a.alpha, a.stt);
         t.x += z.x; t.y += z.y; t.z += z.z;
       }
+    }
+    if (a.tadd[0]) {
+      t.x += a.tadd[0][I];
+      t.y += a.tadd[1][I];
+      t.z += a.tadd[2][I];
     }
     if (a.frozenLUT && a.frozenLUT[q.rc[w]] != T(0)) t = v3(T(0), T(0), T(0));
     ot[0][w] = t.x; ot[1][w] = t.y; ot[2][w] = t.z;

@@ -297,7 +297,7 @@ void demag_tensor(const int N[3], const int pbc[3], const double cs[3], int newe
   else demag_kernel_bf_device(N, pbc, cs, accuracy, Kdev, P2, s);
   if (dir.empty()) return;
   for (int c = 0; c < 6; c++) {
-    hipLaunchKernelGGL(k_round_f32, dim3(blocks_for((long)np, 256, 4096)), dim3(256), 0, s, Kdev[c], np);
+    UM_LAUNCH(k_round_f32, dim3(blocks_for((long)np, 256, 4096)), dim3(256), 0, s, Kdev[c], np);
     UM_HIP(hipGetLastError());
   }
   const std::string base = kernel_cache_basename(dir, N, pbc, cs, newell, accuracy, zero_self_demag);

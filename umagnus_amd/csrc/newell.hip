@@ -508,7 +508,7 @@ void demag_kernel_newell_device(const int N[3], const int pbc[3], const double c
   for (int c = 0; c < 6; c++) fill_value<double>(Kdev[c], 0.0, np, s);
   const long total = (long)a.H[0] * a.H[1] * a.H[2];
   UM_REQUIRE(total < (1L << 31), UM_EINVAL, "Newell tensor too large");
-  hipLaunchKernelGGL(k_demag_newell, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, s, a);
+  UM_LAUNCH(k_demag_newell, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, s, a);
   UM_HIP(hipGetLastError());
   demag_kernel_mirror(P, Kdev, s);
   if (P[2] == 1) {

@@ -93,6 +93,7 @@ struct Sim : SimBase {
   bool counted = false;     // the captured step's counter increment is queued
   long gen = 0;                  // bumped by every setter: a captured graph is stale
   DevBuf gstat;                  // {lastErr, peakErr, lastTorque, step counter}
+  bool gstat_live = false;       // gstat holds this session's statistics (uploaded by graph_prepare)
   double *hstat = nullptr;       // pinned mirror
   hipGraphExec_t gexec = nullptr;
   long g_gen = -1;
@@ -406,7 +407,7 @@ struct Sim : SimBase {
     if (has_nodemag) {
       if (nd_gen != gen || !ndvol.p) {
         if (!ndvol.p) ndvol.alloc(sizeof(T) * n);
-        hipLaunchKernelGGL(k_nodemag_vol<T>, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, st, ndvol.as<T>(),
+        UM_LAUNCH(k_nodemag_vol<T>, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, st, ndvol.as<T>(),
                            (const T *)vol.p, nodemagLUT.as<T>(), has_regions ? (const uint8_t *)regions.p : nullptr,
                            (long)n);
         UM_HIP(hipGetLastError());
@@ -1068,7 +1069,6 @@ struct Sim : SimBase {
   }
   // Steps(n), engine/run.go:197-200; RunWhile drops the FSAL torque (:207)
   void steps(int cnt) override {
-    k1_valid = false;
     const long stop = nsteps + cnt;
     long reps = graph_prepare(stop);
     while (reps-- > 0) graph_replay();
@@ -1081,7 +1081,10 @@ struct Sim : SimBase {
   bool graph_usable() const { return graph_mode && fixdt != 0 && !profiling && !tape_on && !any_tfn; }
   // one eager step (FSAL setup), then capture two steps if the cached graph is
   // stale; returns how many two-step replays fit before `stop`
+  // Every Steps/RunWhile starts without the FSAL torque (engine/run.go:207,
+  // stepper.Free()), so the ensemble entry point and Steps agree
   long graph_prepare(long stop) override {
+    k1_valid = false;
     if (!graph_usable() || nsteps >= stop) return 0;
     step();
     if (nsteps + 2 > stop) return 0;
@@ -1094,6 +1097,7 @@ struct Sim : SimBase {
     const unsigned long long ns = (unsigned long long)nsteps;
     std::memcpy(&hstat[3], &ns, 8);
     UM_HIP(hipMemcpyAsync(gstat.p, hstat, 4 * sizeof(double), hipMemcpyHostToDevice, st));
+    gstat_live = true;
     return (stop - nsteps) / 2;
   }
   void capture2() {
@@ -1134,10 +1138,11 @@ struct Sim : SimBase {
   }
   // device statistics back to the host, then the remaining steps eagerly
   void graph_finish(long stop) override {
-    if (graph_usable() && gexec) {
+    if (gstat_live) {   // only statistics this session's replays folded on the device
       UM_HIP(hipMemcpyAsync(hstat, gstat.p, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
       UM_HIP(hipStreamSynchronize(st));
       lastErr = hstat[0]; peakErr = hstat[1]; lastTorque = hstat[2];
+      gstat_live = false;
     }
     while (nsteps < stop) step();
     UM_HIP(hipStreamSynchronize(st));

@@ -16,11 +16,11 @@ __global__ void k_fill_tail(unsigned char *dst, size_t n) {
 void fill_bytes(void *dst, size_t bytes, hipStream_t s) {
   const size_t nw = bytes / 4;
   if (nw) {
-    hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for((long)nw, 256, 8192)), dim3(256), 0, s, (uint32_t *)dst, nw);
+    UM_LAUNCH(k_fill_bytes, dim3(blocks_for((long)nw, 256, 8192)), dim3(256), 0, s, (uint32_t *)dst, nw);
     UM_HIP(hipGetLastError());
   }
   if (bytes % 4) {
-    hipLaunchKernelGGL(k_fill_tail, dim3(1), dim3(64), 0, s, (unsigned char *)dst + nw * 4, bytes % 4);
+    UM_LAUNCH(k_fill_tail, dim3(1), dim3(64), 0, s, (unsigned char *)dst + nw * 4, bytes % 4);
     UM_HIP(hipGetLastError());
   }
 }
@@ -32,7 +32,7 @@ __global__ void k_fill_value(T *dst, T v, size_t n) {
 template <typename T>
 void fill_value(T *dst, T v, size_t n, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_fill_value<T>, dim3(blocks_for((long)n, 256, 8192)), dim3(256), 0, s, dst, v, n);
+  UM_LAUNCH(k_fill_value<T>, dim3(blocks_for((long)n, 256, 8192)), dim3(256), 0, s, dst, v, n);
   UM_HIP(hipGetLastError());
 }
 template void fill_value<float>(float *, float, size_t, hipStream_t);
@@ -180,14 +180,14 @@ void launch_stage(const StageArgs<T> &a, hipStream_t s) {
     const bool g3 = a.group3 && a.nsrc == 3, nrm = a.normalize && a.ncomp == 3;
 #define UM_STAGE(NS)                                                                          \
   case NS:                                                                                    \
-    if (a.ncomp == 1) hipLaunchKernelGGL((k_stage_v<T, NS, 1, false, false>), grid, dim3(256), 0, s, a); \
-    else if (nrm) hipLaunchKernelGGL((k_stage_v<T, NS, 3, false, true>), grid, dim3(256), 0, s, a);      \
-    else hipLaunchKernelGGL((k_stage_v<T, NS, 3, false, false>), grid, dim3(256), 0, s, a);             \
+    if (a.ncomp == 1) UM_LAUNCH((k_stage_v<T, NS, 1, false, false>), grid, dim3(256), 0, s, a); \
+    else if (nrm) UM_LAUNCH((k_stage_v<T, NS, 3, false, true>), grid, dim3(256), 0, s, a);      \
+    else UM_LAUNCH((k_stage_v<T, NS, 3, false, false>), grid, dim3(256), 0, s, a);             \
     break;
     if (g3) {
-      if (a.ncomp == 1) hipLaunchKernelGGL((k_stage_v<T, 3, 1, true, false>), grid, dim3(256), 0, s, a);
-      else if (nrm) hipLaunchKernelGGL((k_stage_v<T, 3, 3, true, true>), grid, dim3(256), 0, s, a);
-      else hipLaunchKernelGGL((k_stage_v<T, 3, 3, true, false>), grid, dim3(256), 0, s, a);
+      if (a.ncomp == 1) UM_LAUNCH((k_stage_v<T, 3, 1, true, false>), grid, dim3(256), 0, s, a);
+      else if (nrm) UM_LAUNCH((k_stage_v<T, 3, 3, true, true>), grid, dim3(256), 0, s, a);
+      else UM_LAUNCH((k_stage_v<T, 3, 3, true, false>), grid, dim3(256), 0, s, a);
     } else {
       switch (a.nsrc) {
         UM_STAGE(1) UM_STAGE(2) UM_STAGE(3) UM_STAGE(4) UM_STAGE(5) UM_STAGE(6) UM_STAGE(7) UM_STAGE(8)
@@ -197,7 +197,7 @@ void launch_stage(const StageArgs<T> &a, hipStream_t s) {
     UM_HIP(hipGetLastError());
     return;
   }
-  hipLaunchKernelGGL(k_stage<T>, dim3(blocks_for(a.N, 256, 16384)), dim3(256), 0, s, a);
+  UM_LAUNCH(k_stage<T>, dim3(blocks_for(a.N, 256, 16384)), dim3(256), 0, s, a);
   UM_HIP(hipGetLastError());
 }
 template void launch_stage<float>(const StageArgs<float> &, hipStream_t);
@@ -347,13 +347,13 @@ void launch_err(const ErrArgs<T> &a, hipStream_t s) {
     for (int c = 0; c < 3; c++) vec = vec && ((uintptr_t)a.k[j][c] & 15) == 0;
   if (vec) {
     const dim3 grid(blocks_for(a.N / W, 256, 2048));
-    if (a.mode == 0) hipLaunchKernelGGL((k_err_v<T, 0>), grid, dim3(256), 0, s, a);
-    else if (a.mode == 1) hipLaunchKernelGGL((k_err_v<T, 1>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_err_v<T, 2>), grid, dim3(256), 0, s, a);
+    if (a.mode == 0) UM_LAUNCH((k_err_v<T, 0>), grid, dim3(256), 0, s, a);
+    else if (a.mode == 1) UM_LAUNCH((k_err_v<T, 1>), grid, dim3(256), 0, s, a);
+    else UM_LAUNCH((k_err_v<T, 2>), grid, dim3(256), 0, s, a);
     UM_HIP(hipGetLastError());
     return;
   }
-  hipLaunchKernelGGL(k_err<T>, dim3(blocks_for(a.N, 256, 2048)), dim3(256), 0, s, a);
+  UM_LAUNCH(k_err<T>, dim3(blocks_for(a.N, 256, 2048)), dim3(256), 0, s, a);
   UM_HIP(hipGetLastError());
 }
 template void launch_err<float>(const ErrArgs<float> &, hipStream_t);
@@ -383,11 +383,11 @@ __global__ void k_step_count(unsigned long long *counter) {
   if (threadIdx.x == 0) *counter += 1ull;
 }
 void launch_step_stats(unsigned long long *res, double *gs, double h, int mode, int count, hipStream_t s) {
-  hipLaunchKernelGGL(k_step_stats, dim3(1), dim3(64), 0, s, res, gs, h, mode, count);
+  UM_LAUNCH(k_step_stats, dim3(1), dim3(64), 0, s, res, gs, h, mode, count);
   UM_HIP(hipGetLastError());
 }
 void launch_step_count(unsigned long long *counter, hipStream_t s) {
-  hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, s, counter);
+  UM_LAUNCH(k_step_count, dim3(1), dim3(64), 0, s, counter);
   UM_HIP(hipGetLastError());
 }
 
@@ -496,15 +496,15 @@ __global__ void k_min_final(const double *partial, int n, double *out) {
 }
 template <typename T>
 void launch_min_step(T *m, T *m0, const T *k, T h, long N, hipStream_t s) {
-  hipLaunchKernelGGL(k_min_step<T>, dim3(blocks_for(N, 256, 16384)), dim3(256), 0, s, m, m0, k, h, N);
+  UM_LAUNCH(k_min_step<T>, dim3(blocks_for(N, 256, 16384)), dim3(256), 0, s, m, m0, k, h, N);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_min_stats(const T *m, const T *m0, const T *k, const T *k0, long N, double *partial, double *out,
                       hipStream_t s) {
-  hipLaunchKernelGGL(k_min_stats<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, m, m0, k, k0, N, partial);
+  UM_LAUNCH(k_min_stats<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, m, m0, k, k0, N, partial);
   UM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(64), 0, s, (const double *)partial, RED_BLOCKS, out);
+  UM_LAUNCH(k_min_final, dim3(1), dim3(64), 0, s, (const double *)partial, RED_BLOCKS, out);
   UM_HIP(hipGetLastError());
 }
 template void launch_min_step<float>(float *, float *, const float *, float, long, hipStream_t);
@@ -539,9 +539,9 @@ void launch_wdot(const T *const *a, const T *const *b, int ncomp, const T *w, T 
   args.wmul = wmul;
   args.N = N;
   args.partial = partial;
-  hipLaunchKernelGGL(k_wdot<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, args);
+  UM_LAUNCH(k_wdot<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, args);
   UM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(64), 0, s, (const double *)partial, RED_BLOCKS, out);
+  UM_LAUNCH(k_final_sum, dim3(1), dim3(64), 0, s, (const double *)partial, RED_BLOCKS, out);
   UM_HIP(hipGetLastError());
 }
 template void launch_wdot<float>(const float *const *, const float *const *, int, const float *, float, long,
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(256) void k_maxabs(const T *x, long N, unsigned lon
 }
 template <typename T>
 void launch_maxabs(const T *x, long N, unsigned long long *res, hipStream_t s) {
-  hipLaunchKernelGGL(k_maxabs<T>, dim3(blocks_for(N, 256, 2048)), dim3(256), 0, s, x, N, res);
+  UM_LAUNCH(k_maxabs<T>, dim3(blocks_for(N, 256, 2048)), dim3(256), 0, s, x, N, res);
   UM_HIP(hipGetLastError());
 }
 template void launch_maxabs<float>(const float *, long, unsigned long long *, hipStream_t);
@@ -587,17 +587,17 @@ __global__ void k_mul(T *dst, const T *a, const T *b, long N) {
 }
 template <typename T>
 void launch_vecnorm(T *dst, const T *const a[3], long N, hipStream_t s) {
-  hipLaunchKernelGGL(k_vecnorm<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, dst, a[0], a[1], a[2], N);
+  UM_LAUNCH(k_vecnorm<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, dst, a[0], a[1], a[2], N);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_div(T *dst, const T *a, const T *b, long N, hipStream_t s) {
-  hipLaunchKernelGGL(k_div<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, dst, a, b, N);
+  UM_LAUNCH(k_div<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, dst, a, b, N);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_mul(T *dst, const T *a, const T *b, long N, hipStream_t s) {
-  hipLaunchKernelGGL(k_mul<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, dst, a, b, N);
+  UM_LAUNCH(k_mul<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, dst, a, b, N);
   UM_HIP(hipGetLastError());
 }
 template void launch_vecnorm<float>(float *, const float *const[3], long, hipStream_t);

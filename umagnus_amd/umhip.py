@@ -48,7 +48,8 @@ EXPORTS = [
     "um_sim_set_solver", "um_sim_set_solver_params", "um_sim_get_state", "um_sim_step", "um_sim_steps",
     "um_sim_run", "um_sim_average_m", "um_sim_energy", "um_sim_stream", "um_sim_set_profiling",
     "um_sim_kernel_times", "um_sim_set_graph", "um_sims_steps", "um_graph_begin", "um_graph_end",
-    "um_graph_launch", "um_graph_destroy", "um_sim_relax",
+    "um_graph_launch", "um_graph_destroy", "um_sim_relax", "um_trace_kernels", "um_traced_kernels",
+    "um_field_torque", "um_rk45dp_stage", "um_rk45dp_error",
 ]
 
 
@@ -308,6 +309,99 @@ def MaxVecDiff(a, b, sqrt3_quirk=False):
     return _red("um_reduce_maxvecdiff", _prec(a[0]), _vec(a), _vec(b), a[0].numel(), int(sqrt3_quirk))
 
 
+# ---- fused evaluation / RK stage / controller norms (um_field_torque, um_rk45dp_*) ----------
+class um_param(C.Structure):
+    _fields_ = [("arr", C.c_void_p), ("mul", C.c_double)]
+
+
+class um_field_args(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("pbc", C.c_int),
+                ("cellsize", C.c_double * 3), ("m", C.c_void_p * 3), ("bdemag", C.c_void_p * 3),
+                ("torque", C.c_void_p * 3), ("beff", C.c_void_p * 3), ("regions", C.c_void_p),
+                ("Msat", um_param), ("aex_lut", C.c_void_p), ("dmi_lut", C.c_void_p), ("dmi_bulk", C.c_int),
+                ("openbc", C.c_int), ("Ku1", um_param), ("Ku2", um_param), ("anisU", um_param * 3),
+                ("Kc1", um_param), ("Kc2", um_param), ("Kc3", um_param), ("anisC1", um_param * 3),
+                ("anisC2", um_param * 3), ("bext_lut", C.c_void_p * 3), ("Bext", um_param * 3),
+                ("thermal", C.c_int), ("seed", C.c_uint64), ("step", C.c_uint64), ("stream", C.c_uint32),
+                ("Temp", um_param), ("thermal_k", C.c_double), ("badd", C.c_void_p * 3), ("alpha", um_param),
+                ("precess", C.c_int), ("zhangli", C.c_int), ("slonczewski", C.c_int), ("J", um_param * 3),
+                ("xi", um_param), ("pol", um_param), ("lambda_", um_param), ("eps_prime", um_param),
+                ("free_layer_thickness", um_param), ("fixed_layer", um_param * 3),
+                ("fixed_layer_top", C.c_double), ("tadd", C.c_void_p * 3), ("frozen_lut", C.c_void_p)]
+
+
+def _up(v):
+    """um_param from a float, None (zero) or a device tensor (per cell)."""
+    arr, mul = _par(v)
+    return um_param(arr.value if arr is not None else None, mul.value)
+
+
+def _v3(ts):
+    if ts is None:
+        return (C.c_void_p * 3)()
+    return (C.c_void_p * 3)(*[t.data_ptr() for t in ts])
+
+
+def FieldTorque(m, cellsize, *, Msat, alpha, torque=None, beff=None, bdemag=None, regions=None, pbc=0,
+                aex_lut=None, dmi_lut=None, dmi_bulk=False, openbc=False, Ku1=0.0, Ku2=0.0, anisU=(0, 0, 0),
+                Kc1=0.0, Kc2=0.0, Kc3=0.0, anisC1=(0, 0, 0), anisC2=(0, 0, 0), bext_lut=None, Bext=(0, 0, 0),
+                thermal=None, badd=None, precess=True, stt=None, tadd=None, frozen_lut=None):
+    """SetTorque / SetEffectiveField in one pass (um_field_torque).  Device
+    arrays are torch tensors; parameters are floats or per-cell tensors.
+    thermal: (seed, step, stream, Temp, k) or None; stt: dict with J (3),
+    xi, pol, lambda_, eps_prime, free_layer_thickness, fixed_layer (3),
+    fixed_layer_top, zhangli, slonczewski."""
+    init()
+    Nz, Ny, Nx = m[0].shape
+    a = um_field_args()
+    a.Nx, a.Ny, a.Nz, a.pbc = Nx, Ny, Nz, int(pbc)
+    a.cellsize = (C.c_double * 3)(*cellsize)
+    a.m, a.bdemag, a.torque, a.beff = _v3(m), _v3(bdemag), _v3(torque), _v3(beff)
+    a.regions = regions.data_ptr() if regions is not None else None
+    a.Msat, a.alpha = _up(Msat), _up(alpha)
+    a.aex_lut = aex_lut.data_ptr() if aex_lut is not None else None
+    a.dmi_lut = dmi_lut.data_ptr() if dmi_lut is not None else None
+    a.dmi_bulk, a.openbc = int(dmi_bulk), int(openbc)
+    a.Ku1, a.Ku2, a.Kc1, a.Kc2, a.Kc3 = _up(Ku1), _up(Ku2), _up(Kc1), _up(Kc2), _up(Kc3)
+    a.anisU = (um_param * 3)(*[_up(x) for x in anisU])
+    a.anisC1 = (um_param * 3)(*[_up(x) for x in anisC1])
+    a.anisC2 = (um_param * 3)(*[_up(x) for x in anisC2])
+    a.bext_lut = _v3(bext_lut)
+    a.Bext = (um_param * 3)(*[_up(x) for x in Bext])
+    if thermal is not None:
+        seed, step, stream, temp, k = thermal
+        a.thermal, a.seed, a.step, a.stream, a.Temp, a.thermal_k = 1, seed, step, stream, _up(temp), float(k)
+    a.badd, a.tadd = _v3(badd), _v3(tadd)
+    a.precess = int(precess)
+    if stt is not None:
+        a.zhangli, a.slonczewski = int(stt.get("zhangli", 1)), int(stt.get("slonczewski", 1))
+        a.J = (um_param * 3)(*[_up(x) for x in stt["J"]])
+        a.fixed_layer = (um_param * 3)(*[_up(x) for x in stt.get("fixed_layer", (0, 0, 0))])
+        for key in ("xi", "pol", "lambda_", "eps_prime", "free_layer_thickness"):
+            setattr(a, key, _up(stt.get(key, 0.0)))
+        a.fixed_layer_top = float(stt.get("fixed_layer_top", 1.0))
+    a.frozen_lut = frozen_lut.data_ptr() if frozen_lut is not None else None
+    _call("um_field_torque", _prec(m[0]), C.byref(a))
+
+
+def RK45DPStage(dst, srcs, facs, vol=None, normalize=True):
+    """MaddN(dst, srcs...) then Normalize(dst, vol) in one pass (um_rk45dp_stage)."""
+    init()
+    flat = (C.c_void_p * (3 * len(srcs)))(*[t.data_ptr() for v in srcs for t in v])
+    _call("um_rk45dp_stage", _prec(dst[0]), _vec(dst), len(srcs), flat, (C.c_double * len(facs))(*facs),
+          _dp(vol), int(normalize), dst[0].numel())
+
+
+def RK45DPError(ks, e, h):
+    """integralController's (err, rlerr, maxdm) for Err = sum e_j k_j, delM = ks[-1] (um_rk45dp_error)."""
+    init()
+    flat = (C.c_void_p * (3 * len(ks)))(*[t.data_ptr() for v in ks for t in v])
+    out = (C.c_double * 3)()
+    _call("um_rk45dp_error", _prec(ks[0][0]), len(ks), flat, (C.c_double * len(e))(*e), C.c_double(h),
+          ks[0][0].numel(), out)
+    return tuple(out)
+
+
 class DemagConvolution:
     """opencl.DemagConvolution (opencl/conv_demag.go:14-23): NewDemag + Exec."""
 
@@ -393,6 +487,25 @@ def demag_kernel_newell(N, cellsize, pbc=(0, 0, 0), asymptotic_radius=32, zero_s
     _call("um_demag_kernel_newell", N[0], N[1], N[2], (C.c_int * 3)(*pbc), (C.c_double * 3)(*cellsize),
           int(asymptotic_radius), int(zero_self_demag), arr)
     return K
+
+
+class KernelTrace:
+    """`with KernelTrace() as t: ...` records the kernels the library launches
+    inside the block (um_trace_kernels); t.names lists their demangled names."""
+
+    def __enter__(self):
+        _call("um_trace_kernels", 1)
+        self.names = []
+        return self
+
+    def __exit__(self, et, ev, tb):
+        need = C.c_size_t()
+        _call("um_traced_kernels", None, C.c_size_t(0), C.byref(need))
+        buf = C.create_string_buffer(need.value)
+        _call("um_traced_kernels", buf, C.c_size_t(need.value), None)
+        _call("um_trace_kernels", 0)
+        self.names = [l for l in buf.value.decode().split("\n") if l]
+        return False
 
 
 def set_autotune(mode, path=None):
