@@ -133,7 +133,7 @@ def batched(steps=60):
         return sims
 
     res = {}
-    for mode in ("plain", "batched", "tuned"):
+    for mode in ("plain", "graph_one_at_a_time", "batched", "tuned"):
         if mode == "tuned":
             umhip.set_autotune(1)
         try:
@@ -142,11 +142,12 @@ def batched(steps=60):
             umhip.set_autotune(0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        if mode != "plain":
+        if mode in ("batched", "tuned"):
             umhip.sims_steps(sims, steps)
         else:
-            for s in sims:
+            for s in sims:     # one member after another, eager or from its captured graph
                 s.steps(steps)
+                torch.cuda.synchronize()
         torch.cuda.synchronize()
         res[mode] = cells * steps / (time.perf_counter() - t0) / 1e6
         del sims
@@ -154,7 +155,9 @@ def batched(steps=60):
     return {"workload": f"{len(sizes)} instances, {cells} cells in total, Nx, Ny in {set(ENSEMBLE_SIZES)}, "
                         "Nz 1..4, 2 nm, Heun FixDt 5e-15 s, T = 300 K, FP32",
             "steps": steps, "unit": "Mcell·steps/s", "plain": round(res["plain"], 2),
+            "graph_one_at_a_time": round(res["graph_one_at_a_time"], 2),
             "value": round(res["batched"], 2), "speedup": round(res["batched"] / res["plain"], 3),
+            "speedup_vs_graph_one_at_a_time": round(res["batched"] / res["graph_one_at_a_time"], 3),
             "tuned": round(res["tuned"], 2), "tuned_speedup": round(res["tuned"] / res["plain"], 3)}
 
 
@@ -226,47 +229,55 @@ def adjoint_workload(seed=6):
 
 def adjoint(steps=2000):
     """Cost of the gradient (taped forward + reverse sweep) against a plain
-    forward run of the same steps (graph replays), FP32, with sqrt(steps)
-    checkpointing and with the whole trajectory kept in HBM."""
+    forward run of the same steps (graph replays), with sqrt(steps)
+    checkpointing and with the whole trajectory kept in HBM.  FP64 first: the
+    FP64 gradient is the one that matches central differences (to ~5e-6 per
+    cell against the Richardson-extrapolated difference quotient); the FP32
+    run is reported beside it (its trajectory, and so its gradient, follows the
+    FP32 rounding of 2000 steps)."""
     import torch
     from umagnus_amd import umhip
     N, m0, tgt, Ms, make = adjoint_workload()
-    sim = make(umhip, np.float32)
-    sim.set_graph(1)
-    sim.steps(2)                          # plans, graph capture
-    sim.set_m(m0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sim.steps(steps)
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    sim.set_graph(0)
-    res, fwd = {}, {}
-    for name, every in (("sqrt_checkpoint", int(math.ceil(math.sqrt(steps)))), ("full_tape", 0)):
+    out = {"workload": "256x256x1 cells of 2x2x1 nm, Ms 8e5 (per-cell map), Aex 1.3e-11, Dind 2e-3, "
+                       "alpha 0.3, RK4 FixDt 1e-13 s; loss sum |m - target|^2; gradients wrt "
+                       "per-cell Ms, uniform B_ext and m0",
+           "steps": steps, "checkpoint_every": int(math.ceil(math.sqrt(steps)))}
+    for name, dt in (("f64", np.float64), ("f32", np.float32)):
+        sim = make(umhip, dt)
+        sim.set_graph(1)
+        sim.steps(2)                          # plans, graph capture
         sim.set_m(m0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        sim.tape_begin(steps, every)
         sim.steps(steps)
-        mN = sim.get_m()
-        fwd[name] = time.perf_counter() - t0
-        _, gms, gb = sim.backward(2 * (mN - tgt))
-        sim.tape_end()
         torch.cuda.synchronize()
-        res[name] = time.perf_counter() - t0
-        ok = bool(np.isfinite(gms).all() and np.isfinite(gb).all())
-    return {"workload": "256x256x1 cells of 2x2x1 nm, Ms 8e5 (per-cell map), Aex 1.3e-11, Dind 2e-3, "
-                        "alpha 0.3, RK4 FixDt 1e-13 s, FP32; loss sum |m - target|^2; gradients wrt "
-                        "per-cell Ms, uniform B_ext and m0",
-            "steps": steps, "forward_s": round(t_fwd, 4),
-            "gradient_s": round(res["sqrt_checkpoint"], 4),
-            "taped_forward_s": round(fwd["sqrt_checkpoint"], 4),
-            "reverse_sweep_s": round(res["sqrt_checkpoint"] - fwd["sqrt_checkpoint"], 4),
-            "cost_vs_forward": round(res["sqrt_checkpoint"] / t_fwd, 3),
-            "checkpoint_every": int(math.ceil(math.sqrt(steps))),
-            "full_tape_gradient_s": round(res["full_tape"], 4),
-            "full_tape_cost_vs_forward": round(res["full_tape"] / t_fwd, 3),
-            "finite": ok}
+        t_fwd = time.perf_counter() - t0
+        sim.set_graph(0)
+        res, fwd = {}, {}
+        ok = True
+        for key, every in (("sqrt_checkpoint", int(math.ceil(math.sqrt(steps)))), ("full_tape", 0)):
+            sim.set_m(m0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sim.tape_begin(steps, every)
+            sim.steps(steps)
+            mN = sim.get_m()
+            fwd[key] = time.perf_counter() - t0
+            _, gms, gb = sim.backward(2 * (mN - tgt))
+            sim.tape_end()
+            torch.cuda.synchronize()
+            res[key] = time.perf_counter() - t0
+            ok = ok and bool(np.isfinite(gms).all() and np.isfinite(gb).all())
+        out[name] = {"forward_s": round(t_fwd, 4), "gradient_s": round(res["sqrt_checkpoint"], 4),
+                     "taped_forward_s": round(fwd["sqrt_checkpoint"], 4),
+                     "reverse_sweep_s": round(res["sqrt_checkpoint"] - fwd["sqrt_checkpoint"], 4),
+                     "cost_vs_forward": round(res["sqrt_checkpoint"] / t_fwd, 3),
+                     "full_tape_gradient_s": round(res["full_tape"], 4),
+                     "full_tape_cost_vs_forward": round(res["full_tape"] / t_fwd, 3), "finite": ok}
+        del sim
+        torch.cuda.empty_cache()
+    out["cost_vs_forward"] = out["f64"]["cost_vs_forward"]
+    return out
 
 
 def precision_workload(N=(1024, 1024, 8), seed=9, alpha=0.1):
@@ -320,8 +331,11 @@ def precision(times=(1e-12, 1e-11, 5e-11)):
         m32, m64 = s32.get_m().astype(np.float64), s64.get_m()
         d = np.linalg.norm(m32 - m64, axis=0)
         E32, E64 = s32.energy(9), s64.energy(9)
+        a32, a64 = s32.average_m(), s64.average_m()
         rows.append({"t": t, "steps_f32": int(s32.state()["nsteps"]), "steps_f64": int(s64.state()["nsteps"]),
                      "max_m_error": float(d.max()), "rms_m_error": float(np.sqrt((d ** 2).mean())),
+                     "avg_m_f64": [round(float(x), 7) for x in a64],
+                     "avg_m_error": float(np.abs(a32 - a64).max()),
                      "rel_energy_error": float(abs(E32 - E64) / abs(E64))})
     del s32, s64
     torch.cuda.empty_cache()
@@ -330,24 +344,81 @@ def precision(times=(1e-12, 1e-11, 5e-11)):
             "checkpoints": rows, "run_s_f32": round(t_run[32], 3), "run_s_f64": round(t_run[64], 3)}
 
 
-def cpu_baseline(N, cs):
-    """The FP32 oracle (C, OpenMP) on the host: one RK45DP step of the same
-    workload after one warm-up step (setup and tensor excluded)."""
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(N, cs, reps=5):
+    """The FP32 oracle (C, OpenMP) on the host cores: after one warm-up step
+    (which includes the tensor setup), `reps` single RK45DP steps of the same
+    workload are timed and the median is reported.  The thread count is the
+    OpenMP setting of the environment (the host's share of cores), else nproc."""
+    nproc = os.cpu_count() or 1
+    os.environ.setdefault("OMP_NUM_THREADS", str(nproc))
     from oracle import pyoracle
     s = pyoracle.Sim(N, cs, dtype=np.float32)
     configure(s, pyoracle, np.float32)
     s.set_m(seeded_m(N))
     s.step()  # FSAL torque + first step + tensor setup
-    t0 = time.perf_counter()
-    s.step()
-    t = time.perf_counter() - t0
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        s.step()
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
     n = N[0] * N[1] * N[2]
     return {"value": round(n / t / 1e6, 4), "unit": "Mcell·steps/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
             "kind": "port",
-            "sample": f"1 RK45DP step (6 torque evaluations) of the {N[0]}x{N[1]}x{N[2]} FP32 workload, "
-                      f"{t:.2f} s, C oracle with OpenMP"}
+            "sample": f"median of {reps} single RK45DP steps (6 torque evaluations each) of the "
+                      f"{N[0]}x{N[1]}x{N[2]} FP32 workload after a warm-up step: {t:.2f} s per step "
+                      f"(all: {', '.join(f'{x:.2f}' for x in ts)}); C restatement of the reference's unfused "
+                      "sequence with OpenMP, not the reference binary",
+            "nproc": nproc, "cpu_model": cpu_model()}
+
+
+def strip_run():
+    """The 128x32x1 strip of 3.90625 x 3.90625 x 3 nm cells: Ms 8e5, Aex 1.3e-11,
+    alpha 0.02, m = (1,1,0)/sqrt2 plus a seeded +-0.1 perturbation, B_ext =
+    (-24.6, 4.3, 0) mT, adaptive RK45DP (MaxErr 1e-5) for 1 ns, FP32 and FP64:
+    wall time of the run (latency-bound: 4096 cells), steps and <m>(1 ns)."""
+    import torch
+    from umagnus_amd import umhip
+    from umagnus_amd.params import exchange_lut
+    N, cs = (128, 32, 1), (3.90625e-9, 3.90625e-9, 3e-9)
+    rng = np.random.default_rng(1)
+    m = np.zeros((3, 1, 32, 128))
+    m[0] = m[1] = 1 / math.sqrt(2)
+    m += rng.uniform(-0.1, 0.1, m.shape)
+    m /= np.linalg.norm(m, axis=0)
+    out = {"workload": "128x32x1 cells of 3.90625x3.90625x3 nm, Ms 8e5, Aex 1.3e-11, alpha 0.02, "
+                       "B_ext (-24.6, 4.3, 0) mT, RK45DP MaxErr 1e-5, 1 ns"}
+    for name, dt in (("f32", np.float32), ("f64", np.float64)):
+        s = umhip.Sim(N, cs, dtype=dt)
+        s.set_scalar(umhip.P_MSAT, 8e5)
+        s.set_scalar(umhip.P_ALPHA, 0.02)
+        s.set_vector(umhip.P_BEXT, (-24.6e-3, 4.3e-3, 0.0))
+        s.set_exchange_luts(exchange_lut(1.3e-11, dtype=dt), None)
+        s.set_solver(umhip.SOLVER_RK45DP)
+        s.set_solver_params(dt=1e-15, maxerr=1e-5)
+        s.set_m(m)
+        s.torque()                      # tensor and plan outside the timed run
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(1e-9)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = s.state()
+        out[name] = {"wall_s": round(wall, 4), "steps": int(st["nsteps"]), "rejected": int(st["nundone"]),
+                     "us_per_step": round(wall / max(st["nsteps"], 1) * 1e6, 2),
+                     "m_avg": [round(float(x), 7) for x in s.average_m()]}
+        del s
+    return out
 
 
 def main():
@@ -363,6 +434,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-strip", action="store_true")
+    ap.add_argument("--precision-times", type=str, default="1e-12,1e-11,5e-11",
+                    help="comma-separated checkpoints (s) of the precision workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
@@ -442,9 +516,11 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint()
     if not args.no_precision:
-        out["precision"] = precision()
+        out["precision"] = precision(tuple(float(x) for x in args.precision_times.split(",")))
     if not args.no_autotune:
         out["autotune"] = autotune()
+    if not args.no_strip:
+        out["strip"] = strip_run()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, cs)
     print(json.dumps(out), flush=True)

@@ -338,6 +338,21 @@ int um_sim_set_nodemag_lut(void *sim, const void *lut);
  * with a time function are not graph-replayed. */
 typedef void (*um_time_fn)(double t, double *values, void *ctx);
 int um_sim_set_time_function(void *sim, int param, int ncomp, um_time_fn fn, void *ctx);
+/* Extra per-cell field and torque terms, each a host array per component
+ * (n values of the handle's precision) times a multiplier:
+ *  kind 0: a B_ext mask term, Excitation.Add(mask, f(t)) (engine/excitation.go:
+ *          41-53, one Madd2 per term), counted in the Zeeman energy;
+ *  kind 1: a custom field added to B_eff last (AddFieldTerm,
+ *          engine/customfield.go:68), not counted in any energy term;
+ *  kind 2: a custom torque added before FreezeSpins (AddTorqueTerm,
+ *          engine/torque.go:108-121).
+ * fn (nullable) gives the multiplier fn(t)[0] at every stage time instead of
+ * mul; steps with such a term are not graph-replayed.  Terms enter the fused
+ * kernel as one summed field / torque array.  The adjoint tape rejects them.
+ * um_sim_clear_field_terms removes the terms of one kind (-1: all). */
+int um_sim_add_field_term(void *sim, int kind, const void *ax, const void *ay, const void *az, double mul,
+                          um_time_fn fn, void *ctx);
+int um_sim_clear_field_terms(void *sim, int kind);
 /* geometry (engine/geom.go): per-cell volume fraction, NULL = full */
 int um_sim_set_geometry(void *sim, const void *vol);
 /* EnableDemag, OpenBC, DoPrecess, and the MaxVecDiff sqrt(3) widening in Heun */

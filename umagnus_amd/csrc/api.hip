@@ -566,6 +566,7 @@ int um_rk45dp_stage(int prec, void *const dst[3], int nsrc, const void *const *s
   });
 }
 
+
 // Err = sum e_j k_j with delM = k[nk-1], as the fused six-source error kernel:
 // shorter lists repeat delM with zero coefficients
 extern "C++" {
@@ -888,6 +889,17 @@ int um_sim_set_bext_lut(void *sim, const void *lx, const void *ly, const void *l
 int um_sim_set_frozen_lut(void *sim, const void *lut) { return guard([&] { SIMCHK(sim); SIM(sim)->set_frozen_lut(lut); }); }
 int um_sim_set_time_function(void *sim, int param, int ncomp, um_time_fn fn, void *ctx) {
   return guard([&] { SIMCHK(sim); SIM(sim)->set_time_function(param, ncomp, fn, ctx); });
+}
+int um_sim_add_field_term(void *sim, int kind, const void *ax, const void *ay, const void *az, double mul,
+                          um_time_fn fn, void *ctx) {
+  return guard([&] {
+    SIMCHK(sim);
+    const void *a[3] = {ax, ay, az};
+    SIM(sim)->add_field_term(kind, a, mul, fn, ctx);
+  });
+}
+int um_sim_clear_field_terms(void *sim, int kind) {
+  return guard([&] { SIMCHK(sim); SIM(sim)->clear_field_terms(kind); });
 }
 int um_sim_set_nodemag_lut(void *sim, const void *lut) {
   return guard([&] { SIMCHK(sim); SIM(sim)->set_nodemag_lut(lut); });

@@ -54,6 +54,18 @@ struct Sim : SimBase {
   };
   TimeFn tfn[UM_P_COUNT];
   bool any_tfn = false;
+  // extra field / torque terms (B_ext mask terms, custom fields and torques),
+  // summed into xfield / xtorque before every evaluation
+  struct ExtraTerm {
+    DevBuf arr;   // 3n values
+    double mul = 1;
+    um_time_fn fn = nullptr;
+    void *ctx = nullptr;
+    int kind = 0;
+  };
+  std::vector<ExtraTerm> xterms;
+  DevBuf xfield, xtorque;
+  bool xterm_tfn = false;
   DevBuf regions, aLUT, dLUT, dbLUT, bextLUT, frozenLUT, vol;
   DevBuf nodemagLUT, ndvol;   // NoDemagSpins per region, and the masked demag input
   bool has_nodemag = false;
@@ -204,6 +216,63 @@ struct Sim : SimBase {
         s.nonzero = v[c] != 0 && (!s.arr.p || s.arr_nz);
       }
     }
+  }
+  void add_field_term(int kind, const void *const a[3], double mul, um_time_fn fn, void *ctx) override {
+    UM_REQUIRE(kind >= 0 && kind <= 2, UM_EINVAL, "field term kind: 0 B_ext mask, 1 custom field, 2 custom torque");
+    UM_REQUIRE(a && a[0] && a[1] && a[2], UM_EINVAL, "field term: NULL component");
+    UM_REQUIRE(xterms.size() < 64, UM_EUNSUP, "at most 64 extra field / torque terms");
+    gen++;
+    ExtraTerm t;
+    t.arr.alloc(sizeof(T) * 3 * n);
+    for (int c = 0; c < 3; c++)
+      UM_HIP(hipMemcpyAsync(t.arr.template as<T>() + c * n, a[c], sizeof(T) * n, hipMemcpyHostToDevice, st));
+    UM_HIP(hipStreamSynchronize(st));
+    t.mul = mul; t.fn = fn; t.ctx = ctx; t.kind = kind;
+    xterms.push_back(std::move(t));
+    xterm_tfn = false;
+    for (auto &x : xterms) xterm_tfn = xterm_tfn || x.fn != nullptr;
+  }
+  void clear_field_terms(int kind) override {
+    UM_REQUIRE(kind >= -1 && kind <= 2, UM_EINVAL, "field term kind: -1 all, 0, 1 or 2");
+    gen++;
+    UM_HIP(hipStreamSynchronize(st));
+    std::vector<ExtraTerm> keep;
+    for (auto &x : xterms) if (kind >= 0 && x.kind != kind) keep.push_back(std::move(x));
+    xterms = std::move(keep);
+    xterm_tfn = false;
+    for (auto &x : xterms) xterm_tfn = xterm_tfn || x.fn != nullptr;
+  }
+  // dst = sum of the terms whose kind bit is set in kinds, each times its
+  // multiplier at the current stage time; false (dst untouched) when none
+  bool combine_terms(DevBuf &dst, unsigned kinds) {
+    std::vector<const ExtraTerm *> sel;
+    for (auto &x : xterms) if (kinds & (1u << x.kind)) sel.push_back(&x);
+    if (sel.empty()) return false;
+    if (!dst.p) dst.alloc(sizeof(T) * 3 * n);
+    for (size_t j0 = 0; j0 < sel.size(); j0 += 7) {
+      StageArgs<T> a{};
+      a.ncomp = 3;
+      int ns = 0;
+      if (j0 > 0) {   // chain: dst + the next seven terms
+        for (int c = 0; c < 3; c++) a.src[ns][c] = dst.as<T>() + c * n;
+        a.fac[ns++] = T(1);
+      }
+      for (size_t j = j0; j < sel.size() && j < j0 + 7; j++) {
+        double mul = sel[j]->mul;
+        if (sel[j]->fn) {
+          double v[3] = {0, 0, 0};
+          sel[j]->fn(Time, v, sel[j]->ctx);
+          mul = v[0];
+        }
+        for (int c = 0; c < 3; c++) a.src[ns][c] = sel[j]->arr.template as<T>() + c * n;
+        a.fac[ns++] = (T)mul;
+      }
+      a.nsrc = ns;
+      for (int c = 0; c < 3; c++) a.dst[c] = dst.as<T>() + c * n;
+      a.N = n;
+      launch_stage(a, st);
+    }
+    return true;
   }
   void set_scalar(int id, const void *arr, double mul) override {
     gen++;
@@ -394,6 +463,12 @@ struct Sim : SimBase {
     a.stt.thick = P(UM_P_FREELAYER);
     a.stt.mesh_thick = (T)(N[2] * cs[2]);
     a.stt.flp = (T)flp;
+    if (!xterms.empty()) {
+      if (combine_terms(xfield, 1u | 2u))
+        for (int c = 0; c < 3; c++) a.badd[c] = xfield.as<T>() + c * n;
+      if (combine_terms(xtorque, 4u))
+        for (int c = 0; c < 3; c++) a.tadd[c] = xtorque.as<T>() + c * n;
+    }
     return a;
   }
   // B_demag into bd (SetDemagField, engine/demag.go:42-65); with NoDemagSpins
@@ -917,6 +992,7 @@ struct Sim : SimBase {
                "the adjoint needs precession on, no frozen spins and no geometry");
     UM_REQUIRE(!has_dmibulk && !nz(UM_P_J, 3), UM_EUNSUP, "no adjoint of bulk DMI or spin-transfer torques");
     UM_REQUIRE(!has_nodemag, UM_EUNSUP, "no adjoint with NoDemagSpins");
+    UM_REQUIRE(xterms.empty(), UM_EUNSUP, "no adjoint with extra field or torque terms");
     tape_end();
     tape_K = every > 0 ? std::min<long>(every, cap) : cap;
     const long nck = (cap + tape_K - 1) / tape_K;
@@ -1078,7 +1154,9 @@ struct Sim : SimBase {
   // ---- graph replay (FixDt only) ---------------------------------------------------------
   void set_graph(int on) override { graph_mode = on; }
   long step_count() const override { return nsteps; }
-  bool graph_usable() const { return graph_mode && fixdt != 0 && !profiling && !tape_on && !any_tfn; }
+  bool graph_usable() const {
+    return graph_mode && fixdt != 0 && !profiling && !tape_on && !any_tfn && !xterm_tfn;
+  }
   // one eager step (FSAL setup), then capture two steps if the cached graph is
   // stale; returns how many two-step replays fit before `stop`
   // Every Steps/RunWhile starts without the FSAL torque (engine/run.go:207,
@@ -1200,6 +1278,10 @@ struct Sim : SimBase {
         a.has_bext = 0;
         for (int c = 0; c < 3; c++) a.bextLUT[c] = nullptr;
       }
+      // the Zeeman energy holds B_ext's mask terms; custom fields and torques have none
+      for (int c = 0; c < 3; c++) a.badd[c] = a.tadd[c] = nullptr;
+      if (term == 2 && combine_terms(xfield, 1u))
+        for (int c = 0; c < 3; c++) a.badd[c] = xfield.as<T>() + c * n;
       a.has_uniax = (term == 3 || term == 4) ? a.has_uniax : 0;
       if (term == 3) a.Ku2 = Par<T>{nullptr, T(0)};
       if (term == 4) a.Ku1 = Par<T>{nullptr, T(0)};

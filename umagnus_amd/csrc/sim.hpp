@@ -56,6 +56,8 @@ struct SimBase {
   virtual void set_frozen_lut(const void *l) = 0;
   virtual void set_nodemag_lut(const void *l) = 0;
   virtual void set_time_function(int id, int nc, um_time_fn fn, void *ctx) = 0;
+  virtual void add_field_term(int kind, const void *const a[3], double mul, um_time_fn fn, void *ctx) = 0;
+  virtual void clear_field_terms(int kind) = 0;
   virtual void set_geometry(const void *vol) = 0;
   virtual void set_flags(int demag, int openbc, int precess, int heun_sqrt3) = 0;
   virtual void set_demag_accuracy(double a) = 0;

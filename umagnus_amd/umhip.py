@@ -49,7 +49,8 @@ EXPORTS = [
     "um_sim_run", "um_sim_average_m", "um_sim_energy", "um_sim_stream", "um_sim_set_profiling",
     "um_sim_kernel_times", "um_sim_set_graph", "um_sims_steps", "um_graph_begin", "um_graph_end",
     "um_graph_launch", "um_graph_destroy", "um_sim_relax", "um_trace_kernels", "um_traced_kernels",
-    "um_field_torque", "um_rk45dp_stage", "um_rk45dp_error",
+    "um_field_torque", "um_rk45dp_stage", "um_rk45dp_error", "um_rk45dp_error_noalloc",
+    "um_sim_add_field_term", "um_sim_clear_field_terms",
 ]
 
 
@@ -630,6 +631,27 @@ class Sim:
                     out[c] = float(v[c])
         self._tfn[pid] = Sim._TFN(cb)
         _call("um_sim_set_time_function", self.h, int(pid), int(ncomp), self._tfn[pid], None)
+
+    FIELD_TERM_BEXT, FIELD_TERM_CUSTOM, TORQUE_TERM_CUSTOM = 0, 1, 2
+
+    def add_field_term(self, kind, arr3, mul=1.0, fn=None):
+        """Extra per-cell term (um_sim_add_field_term): kind 0 B_ext mask term
+        (Excitation.Add), 1 custom field (AddFieldTerm), 2 custom torque
+        (AddTorqueTerm); arr3 = three (Nz, Ny, Nx) arrays; fn(t) -> multiplier."""
+        a = [self._a(x) for x in arr3]
+        if not hasattr(self, "_xfn"):
+            self._xfn = []
+        cb = None
+        if fn is not None:
+            def _cb(t, out, ctx):
+                out[0] = float(fn(t))
+            cb = Sim._TFN(_cb)
+            self._xfn.append(cb)
+        _call("um_sim_add_field_term", self.h, int(kind), self._p(a[0]), self._p(a[1]), self._p(a[2]),
+              C.c_double(mul), cb, None)
+
+    def clear_field_terms(self, kind=-1):
+        _call("um_sim_clear_field_terms", self.h, int(kind))
 
     def set_nodemag_lut(self, lut):
         """NoDemagSpins per region (256 values); None clears."""
