@@ -380,6 +380,25 @@ class EngineRK45DP:
         return _h3(self.m)
 
 
+def _two_kernel_sim(gpu, N, cs, dtype, p, m):
+    """A simulation handle whose torque runs the X inverse and the field kernel
+    apart, the kernels um_demag_exec + um_field_torque launch: the engine
+    sequence over the boundary is then the same arithmetic, bit for bit (the
+    fused X-inverse + field kernel computes the same terms, with the compiler
+    free to contract them into FMAs differently).  The demag plan reads the
+    switch when it is built, at the first evaluation."""
+    import os
+    os.environ["UMHIP_FUSED_FIELD"] = "0"
+    try:
+        sim = gpu.Sim(N, cs, dtype=dtype)
+        setup(sim, gpu, dtype=dtype, **p)
+        sim.set_m(m)
+        sim.demag_field()   # builds the plan without counting an evaluation
+        return sim
+    finally:
+        os.environ.pop("UMHIP_FUSED_FIELD", None)
+
+
 @pytest.mark.parametrize("prec", [1, 0])
 def test_engine_rk45dp_adaptive_matches_sim(gpu, prec):
     """The 128x32x1 strip, adaptive RK45DP (MaxErr 1e-5), 40 steps: the engine's
@@ -389,9 +408,7 @@ def test_engine_rk45dp_adaptive_matches_sim(gpu, prec):
     p = dict(Ms=8e5, Aex=1.3e-11, alpha=0.02, Bext=(-24.6e-3, 4.3e-3, 0.0))
     m = strip_m(1)
     eng = EngineRK45DP(gpu, N, cs, dtype, p, m)
-    sim = gpu.Sim(N, cs, dtype=dtype)
-    setup(sim, gpu, dtype=dtype, **p)
-    sim.set_m(m)
+    sim = _two_kernel_sim(gpu, N, cs, dtype, p, m)
     sim.set_solver_params(dt=1e-15)
     while eng.nsteps < 40:
         eng.step()
@@ -416,9 +433,7 @@ def test_engine_rk45dp_fixed_vs_oracle(gpu, oracle, prec):
     eng = EngineRK45DP(gpu, N, cs, dtype, p, m, dt=1e-14, fixdt=1e-14)
     for _ in range(5):
         eng.step()
-    sim = gpu.Sim(N, cs, dtype=dtype)
-    setup(sim, gpu, dtype=dtype, **p)
-    sim.set_m(m)
+    sim = _two_kernel_sim(gpu, N, cs, dtype, p, m)
     sim.set_solver_params(dt=1e-14, fixdt=1e-14)
     sim.steps(5)
     assert np.array_equal(sim.get_m(), eng.get_m())

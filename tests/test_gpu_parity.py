@@ -363,6 +363,67 @@ def test_sim_beff_dmi_regions_vacuum(gpu, oracle, prec, N, openbc):
     assert relerr(sg.torque(), so.torque()) <= tol
 
 
+FUSED_CASES = [((64, 12, 3), "dmi_regions"), ((128, 10, 1), "exchange"), ((256, 9, 2), "dmi"),
+               ((512, 6, 4), "dmi"), ((512, 7, 3), "none_regions"), ((1024, 5, 2), "exchange_regions"),
+               ((256, 8, 5), "dmi_pbc")]
+
+
+@pytest.mark.parametrize("N,terms", FUSED_CASES)
+def test_fused_xinv_field_vs_oracle(gpu, oracle, N, terms):
+    """The fused X-inverse + field + torque kernel (k_xi4_field) at every row
+    length it covers (Nx = 64 ... 1024), with and without a region map, with
+    no exchange, exchange only and exchange + interfacial DMI, partly filled
+    row blocks (Ny not a multiple of the rows per work-group), 2D and periodic
+    grids: the launch trace shows the fused kernel and no separate field
+    kernel; the torque is within 1e-6 (normwise) of the two-kernel path
+    (k_xi4 + k_field_torque4, UMHIP_FUSED_FIELD=0: the same arithmetic, with
+    the compiler free to contract products into FMAs differently) and within
+    1e-5 per cell of the FP64 oracle."""
+    import os
+    from cases import percell_relerr, percell_torque_relerr
+    _say("fused", N, terms)
+    cs = (2e-9, 2.5e-9, 1.5e-9)
+    shape = (N[2], N[1], N[0])
+    rng = np.random.default_rng(N[0] + N[1])
+    params = dict(Ms=9e5, Aex=1.3e-11, Ku1=5e5, alpha=0.1)
+    if "dmi" in terms:
+        params["Dind"] = 2e-3
+    if "none" in terms:
+        params["Aex"] = 0.0
+    if "regions" in terms:
+        params["regions"] = rng.integers(0, 3, shape).astype(np.uint8)
+        params["Aex"] = {0: 1e-11, 1: 2e-11, 2: 1.5e-11} if params["Aex"] else 0.0
+        if "Dind" in params:
+            params["Dind"] = {0: 2e-3, 1: -1e-3, 2: 3e-3}
+    so = oracle.Sim(N, cs, dtype=np.float64, pbc=(0, 2, 1) if "pbc" in terms else (0, 0, 0))
+    setup(so, oracle, dtype=np.float64, **params)
+    m = random_unit(shape, 4)
+    so.set_m(m)
+    got = {}
+    for fused in ("0", "1"):
+        os.environ["UMHIP_FUSED_FIELD"] = fused
+        try:
+            sg = gpu.Sim(N, cs, dtype=np.float32, pbc=(0, 2, 1) if "pbc" in terms else (0, 0, 0))
+            setup(sg, gpu, dtype=np.float32, **params)
+            sg.set_m(m)
+            sg.torque()
+            with gpu.KernelTrace() as tr:
+                got[fused] = sg.torque()
+        finally:
+            os.environ.pop("UMHIP_FUSED_FIELD", None)
+        fk = [n for n in tr.names if n.startswith("void um::k_xi4_field<float, %d," % N[0])]
+        sk = [n for n in tr.names if "k_field_torque" in n]
+        assert (len(fk), len(sk)) == ((1, 0) if fused == "1" else (0, 1)), tr.names
+    d01 = relerr(got["1"], got["0"])
+    _say("fused vs two kernels", d01)
+    assert d01 <= 1e-6
+    b_o, t_o = so.beff(), so.torque()
+    eb = percell_relerr(sg.beff(), b_o)
+    et = percell_torque_relerr(got["1"], t_o, b_o)
+    _say("per cell: B", eb, "tau", et)
+    assert et <= 1e-5 and relerr(got["1"], t_o) <= 1e-5
+
+
 # ---- steppers ---------------------------------------------------------------------------------
 def _macrospin(mod, solver, fixdt, t_end, dtype, alpha=0.1, Bz=0.1):
     s = mod.Sim((1, 1, 1), (1e-9, 1e-9, 1e-9), dtype=dtype)
@@ -446,13 +507,14 @@ def test_strip_trajectory_vs_oracle(gpu, oracle, prec):
 # ---- the timed kernels at the timed size --------------------------------------------------------
 TIMED_512x512x32 = ["void um::k_xfwd<float, 512, 8>(", "void um::k_coly<float, 32, -1, 16, true, 1>(",
                     "void um::k_convz<float, 32, 8, 4>(", "void um::k_coly<float, 32, 1, 16, true, 1>(",
-                    "void um::k_xi4<float, 512>(", "void um::k_field_torque4<float, false, 2, false>("]
+                    "void um::k_xi4_field<float, 512, false, 2, true>("]
 
 
 def test_timed_kernels_512x512x32_vs_oracle(gpu, oracle):
     """The 512x512x32 FP32 workload (DMI, Ku1 || z, alpha 0.1, random m) through
-    exactly the kernels the benchmark times -- the launch trace names the six
-    evaluation kernels -- against the FP64 oracle at full size, per cell:
+    exactly the kernels the benchmark times -- the launch trace names the five
+    evaluation kernels, the last the X inverse fused with the field and torque
+    (uniform parameters, no region map) -- against the FP64 oracle at full size, per cell:
     |dB_i| / |B_i| <= 1e-5 for the effective field and |dtau_i| / |B_i| <= 1e-5
     for the torque (the field bounds the torque of a unit m), and normwise."""
     from cases import percell_relerr, percell_torque_relerr
@@ -483,17 +545,18 @@ def test_timed_kernels_512x512x32_vs_oracle(gpu, oracle):
     assert relerr(b_g, b_o) <= 1e-5 and relerr(t_g, t_o) <= 1e-5
 
 
-@pytest.mark.parametrize("N,expect", [((512, 64, 32), ["void um::k_xfwd<float, 512, 8>(", "void um::k_xi4<float, 512>(",
-                                                       "void um::k_convz<float, 32, 8, 4>(",
-                                                       "void um::k_field_torque4<float, false, 2, false>("]),
+@pytest.mark.parametrize("N,expect", [((512, 64, 32), ["void um::k_xfwd<float, 512, 8>(",
+                                                       "void um::k_xi4_field<float, 512, false, 2, true>(",
+                                                       "void um::k_convz<float, 32, 8, 4>("]),
                                       ((32, 512, 32), ["void um::k_coly<float, 32, -1, 16, true, 1>(",
                                                        "void um::k_coly<float, 32, 1, 16, true, 1>(",
                                                        "void um::k_convz<float, 32, 8, 4>(",
-                                                       "void um::k_field_torque4<float, false, 2, false>("])])
+                                                       "void um::k_field_torque4<float, false, 2, false, true>("])])
 def test_timed_kernels_slabs_vs_oracle(gpu, oracle, N, expect):
-    """Smaller grids that select the timed X passes (Nx = 512, >= 2048 rows) or
-    the timed Y passes (Ny = 512) with the timed Z pass and four-cell field
-    kernel, config-2 physics, FP32 vs the FP64 oracle per cell."""
+    """Smaller grids that select the timed X passes (Nx = 512, >= 2048 rows:
+    X forward and the fused X inverse + field kernel) or the timed Y passes
+    (Ny = 512) with the timed Z pass and the four-cell field kernel of uniform
+    parameters, config-2 physics, FP32 vs the FP64 oracle per cell."""
     from cases import percell_relerr, percell_torque_relerr
     cs = (2e-9, 2e-9, 2e-9)
     m = random_unit((N[2], N[1], N[0]), 3)

@@ -1553,86 +1553,122 @@ static void launch_conv(const ConvArgs<T> &a, int n, int B, dim3 grid, hipStream
 #undef UM_CT_CASE
 
 // ---------------------------------------------------------------------------
-// X inverse fused with the four-cell effective field + torque (FP32): a
-// work-group inverts R rows of all three components (3R lines in LDS), then
-// evaluates the field of those R x Nx cells with B_demag read from LDS.
+// X inverse fused with the four-cell effective field + torque (FP32).  A
+// work-group of XF_NT threads inverts R rows (y0 .. y0+R-1 of one z plane) of
+// all three components with the register four-step of k_xi4 (3R lines, G per
+// 32-lane group), leaves the real B_demag lines in LDS, then evaluates the
+// effective field and torque of those R x Nx cells: B_demag never touches HBM.
+// Work-groups run z fastest (tile w -> y block w / Nz, plane w % Nz), so the
+// z +- 1 rows of m that the stencil reads were just read by the neighbouring
+// work-groups of the same XCD and are still in its L2.
 // ---------------------------------------------------------------------------
-// rows per work-group for X length n (complex): 3 R n = 3072 lines' elements
-static int xf_rows(int n) {
-  switch (n) { case 64: return 16; case 128: return 8; case 256: return 4; case 512: return 2; case 1024: return 1; }
-  return 0;
-}
+constexpr int XF_NT = 192;
 
-template <typename T, int CN, int R, bool REG, int EX>
-__global__ __launch_bounds__(FFT_NT) void k_xinv_field(XiArgs<T> a, FieldArgs<T> f) {
-  constexpr int NL = 3 * R, HALF = CN / 2 + 1, TOT = NL * HALF, S = (TOT + FFT_NT - 1) / FFT_NT;
-  extern __shared__ __align__(16) unsigned char smem_[];
-  C2<T> *lds = (C2<T> *)smem_;
-  C2<T> *twl = lds + NL * CN;
-  for (int g = threadIdx.x; g < 2 * CN; g += FFT_NT) twl[g] = a.tw2[g];
-  const int row0 = xcd_tile(blockIdx.x, gridDim.x) * R;   // neighbouring row blocks share an XCD
-  // this thread's four-cell group (R*Nx/4 <= FFT_NT): its m and stencil loads
-  // go out first and overlap the spectrum loads and the FFT
-  const int ngx = f.g.Nx >> 2;
-  const int gr = threadIdx.x / ngx, ggx = threadIdx.x - gr * ngx, grow = row0 + gr;
-  const bool gvalid = gr < R && grow < a.rows;
-  const int giz = gvalid ? (int)a.fRy.div(grow) : 0, giy = gvalid ? grow - giz * (int)a.fRy.d : 0;
-  G4In<T> q;
-  if (gvalid) g4_load<T, REG, EX>(f, 4 * ggx, giy, giz, q);
-  // Z'[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k]); line l = c*R + r
-  C2<T> xk[S], xn[S];
+template <int N> struct XFShape {
+  static constexpr int A = X4Shape<N>::A, B = X4Shape<N>::B, G = B / A;
+  static constexpr int R = 64 * G / B;    // rows per work-group
+  static constexpr int L = 3 * R;         // lines (component-major: line = c R + j)
+  static_assert(L / G * B == XF_NT, "the lines of R rows fill the work-group");
+};
+
+template <typename T, int N, bool REG, int EX, bool UNI>
+__global__ __launch_bounds__(XF_NT) void k_xi4_field(XiArgs<T> a, FieldArgs<T> f) {
+  using S = XFShape<N>;
+  constexpr int A = S::A, B = S::B, G = S::G, R = S::R, TS = B * (A + 1), HB = B / 2;
+  static_assert(S::L * N <= S::L * TS, "B_demag lines fit the transpose space");
+  // the transpose moves the real, then the imaginary parts (half the LDS of
+  // complex entries: more work-groups per CU); afterwards line (c, j) holds
+  // B_demag_c of row j, x < N
+  __shared__ __align__(16) T ldr[S::L * TS];
+  const int Ny = (int)a.fRy.d, Nz = a.rows / Ny;
+  const int w = xcd_tile(blockIdx.x, gridDim.x);
+  const int yb = w / Nz, z = w - yb * Nz, y0 = yb * R;
+  const int grp = threadIdx.x / B, l = threadIdx.x % B;
+  C2<T> v[B];
+  const int h1 = l / A, n1 = l % A, line1 = grp * G + h1;
+  {
+    const int c = line1 / R, y = y0 + (line1 - c * R);
 #pragma unroll
-  for (int s = 0; s < S; s++) {
-    const int g = threadIdx.x + s * FFT_NT;
-    xk[s] = C2<T>{T(0), T(0)};
-    xn[s] = xk[s];
-    if (TOT % FFT_NT == 0 || g < TOT) {
-      const int l = g / HALF, k = g - l * HALF;
-      const int c = l / R, r = row0 + (l - c * R);
-      if (r < a.rows) {
-        const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
-        const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
-        xk[s] = irow[k];
-        xn[s] = irow[CN - k];
+    for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
+    if (y < Ny) {
+      const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
+      // two halves of loads: half the staging registers live at a time
+#pragma unroll
+      for (int hb = 0; hb < 2; hb++) {
+        C2<T> xk[HB], xn[HB];
+#pragma unroll
+        for (int q = 0; q < HB; q++) {
+          const int k = n1 + A * (hb * HB + q);
+          xk[q] = irow[k];
+          xn[q] = irow[N - k];
+        }
+#pragma unroll
+        for (int q = 0; q < HB; q++) {
+          const int n2 = hb * HB + q, k = n1 + A * n2;
+          const C2<T> sm = cadd(xk[q], cconj(xn[q])), d = csub(xk[q], cconj(xn[q]));
+          const C2<T> t = cmul(cconj(a.tw2[k]), d);
+          v[n2] = C2<T>{sm.x - t.y, sm.y + t.x};
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
+    dft<B, +1>(v);
+#pragma unroll
+    for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(a.tw2[2 * n1 * k2]));   // W_n^-(n1 k2)
   }
+  C2<T> u[G][A];
+  T *tw = ldr + line1 * TS + n1;
 #pragma unroll
-  for (int s = 0; s < S; s++) {
-    const int g = threadIdx.x + s * FFT_NT;
-    if (TOT % FFT_NT == 0 || g < TOT) {
-      const int l = g / HALF, k = g - l * HALF;
-      const C2<T> Xk = xk[s], Xnk = xn[s];
-      C2<T> sm = cadd(Xk, cconj(Xnk)), d = csub(Xk, cconj(Xnk));
-      C2<T> t = cmul(cconj(a.tw2[k]), d);
-      lds[l * CN + k] = C2<T>{sm.x - t.y, sm.y + t.x};
-      if (k != 0 && CN - k != k) {
-        sm = cadd(Xnk, cconj(Xk));
-        d = csub(Xnk, cconj(Xk));
-        t = cmul(cconj(a.tw2[CN - k]), d);
-        lds[l * CN + CN - k] = C2<T>{sm.x - t.y, sm.y + t.x};
-      }
-    }
+  for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].x;
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < G; h++) {
+    const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
+#pragma unroll
+    for (int q = 0; q < A; q++) u[h][q].x = t[q];
   }
   __syncthreads();
-  ct_fft<T, CN, NL, +1, 0, 2>(lds, twl);
-  // field + torque of the group; the real output x of line l is lds[l*CN + x/2]
-  if (gvalid) {
+#pragma unroll
+  for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].y;
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < G; h++) {
+    const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
+#pragma unroll
+    for (int q = 0; q < A; q++) u[h][q].y = t[q];
+  }
+  T *bl = ldr;
+  __syncthreads();   // every transpose read is done: the space now takes the real lines
+#pragma unroll
+  for (int h = 0; h < G; h++) {
+    dft<A, +1>(u[h]);
+    T *o = bl + (grp * G + h) * N;
+#pragma unroll
+    for (int k1 = 0; k1 < A / 2; k1++) *(float2 *)(o + 2 * (l + B * k1)) = float2{u[h][k1].x, u[h][k1].y};
+  }
+  __syncthreads();
+  constexpr int NGX = N / 4, NG = R * NGX;
+  for (int g = threadIdx.x; g < NG; g += XF_NT) {
+    const int j = g / NGX, gx = g - j * NGX, y = y0 + j;
+    if (y >= Ny) continue;
     T bdv[3][4];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const C2<T> *b = lds + (c * R + gr) * CN + 2 * ggx;
-      const C2<T> u0 = b[0], u1 = b[1];
-      bdv[c][0] = u0.x; bdv[c][1] = u0.y; bdv[c][2] = u1.x; bdv[c][3] = u1.y;
+      const float4 b = *(const float4 *)(bl + (c * R + j) * N + 4 * gx);
+      bdv[c][0] = b.x; bdv[c][1] = b.y; bdv[c][2] = b.z; bdv[c][3] = b.w;
     }
-    g4_compute<T, REG, EX>(f, 4 * ggx, giy, giz, q, bdv);
+    field_group4<T, REG, EX, false, UNI, false>(f, 4 * gx, y, z, bdv);   // registers over instructions
   }
 }
 
+// whether the fused kernel covers these field terms and buffers: the
+// four-cell kernel's terms without spin-transfer torques or bulk DMI, every
+// per-cell array aligned to four cells
 template <typename T>
-static bool field4_aligned(const FieldArgs<T> &a) {
-  if (a.g.Nx % 4) return false;
-  if (a.exch_mode == 3 || a.has_zl || a.has_sl) return false;   // terms of the one-cell kernel only
+static bool xi4_field_ok(const FieldArgs<T> &a, int n) {
+  if (!std::is_same<T, float>::value || a.g.Nx != n || a.g.Nx % 4) return false;
+  if (n != 64 && n != 128 && n != 256 && n != 512 && n != 1024) return false;
+  if (a.exch_mode == 3 || a.has_zl || a.has_sl) return false;
   auto al = [](const void *p) { return ((uintptr_t)p & (4 * sizeof(T) - 1)) == 0; };
   for (int c = 0; c < 3; c++) {
     if (!al(a.m[c])) return false;
@@ -1643,25 +1679,34 @@ static bool field4_aligned(const FieldArgs<T> &a) {
 }
 
 template <typename T>
-static void launch_xinv_field(const XiArgs<T> &a, const FieldArgs<T> &f, int n, int R, hipStream_t s) {
-  const dim3 grid((a.rows + R - 1) / R);
-  const size_t lds = sizeof(C2<T>) * ((size_t)3 * R * n + 2 * n);
-  const bool reg = f.regions != nullptr;
-  auto go = [&](auto kern) {
-    allow_lds(kern, lds);
-    UM_LAUNCH(kern, grid, dim3(FFT_NT), lds, s, a, f);
-    UM_HIP(hipGetLastError());
-  };
-#define UM_XF(CN, RR)                                                                   \
-  if (n == CN) {                                                                        \
-    if (f.exch_mode == 0) { if (reg) go(k_xinv_field<T, CN, RR, true, 0>); else go(k_xinv_field<T, CN, RR, false, 0>); } \
-    else if (f.exch_mode == 1) { if (reg) go(k_xinv_field<T, CN, RR, true, 1>); else go(k_xinv_field<T, CN, RR, false, 1>); } \
-    else { if (reg) go(k_xinv_field<T, CN, RR, true, 2>); else go(k_xinv_field<T, CN, RR, false, 2>); } \
-    return;                                                                             \
+static void launch_xi4_field(const XiArgs<T> &a, const FieldArgs<T> &f, int n, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    const int Ny = a.fRy.d, Nz = a.rows / Ny;
+    const bool reg = f.regions != nullptr, uni = field_uniform(f);
+    auto go = [&](auto kern, int R) {
+      const dim3 grid((unsigned)(((Ny + R - 1) / R) * Nz));
+      UM_LAUNCH(kern, grid, dim3(XF_NT), 0, s, a, f);
+      UM_HIP(hipGetLastError());
+    };
+#define UM_XF2(CN, RG, E)                                                                            \
+  { if (uni) go(k_xi4_field<T, CN, RG, E, true>, XFShape<CN>::R); else go(k_xi4_field<T, CN, RG, E, false>, XFShape<CN>::R); }
+#define UM_XF(CN)                                                                                    \
+  if (n == CN) {                                                                                     \
+    if (f.exch_mode == 0) { if (reg) UM_XF2(CN, true, 0) else UM_XF2(CN, false, 0) }                 \
+    else if (f.exch_mode == 1) { if (reg) UM_XF2(CN, true, 1) else UM_XF2(CN, false, 1) }            \
+    else { if (reg) UM_XF2(CN, true, 2) else UM_XF2(CN, false, 2) }                                  \
+    return;                                                                                          \
   }
-  UM_XF(64, 16) UM_XF(128, 8) UM_XF(256, 4) UM_XF(512, 2) UM_XF(1024, 1)
+    UM_XF(64) UM_XF(128) UM_XF(256) UM_XF(512) UM_XF(1024)
 #undef UM_XF
+#undef UM_XF2
+  }
   throw Error(UM_EUNSUP, "fused X inverse: unsupported length");
+}
+
+static bool fused_field_default() {
+  const char *e = std::getenv("UMHIP_FUSED_FIELD");
+  return !(e && e[0] == '0');
 }
 
 struct DemagBase {
@@ -1746,6 +1791,9 @@ struct DemagPlan : DemagBase {
   // lines, convolution columns, and the generic engine's radix schedules
   int nl_x = 1, nl_xi = 1, nl_y = 1, B_conv = 1;
   bool xi4 = true;
+  // X inverse fused with the field + torque kernel where it applies
+  // (environment UMHIP_FUSED_FIELD=0 keeps the two kernels apart)
+  bool fused_field = fused_field_default();
   bool zreg = false;            // register Z pass (Pz = 2 Nz, Nz in ConvZTable)
   bool xodd = false;            // odd Px (periodic along x, odd Nx): full-length X passes
   long kcount = 0;
@@ -1984,24 +2032,18 @@ struct DemagPlan : DemagBase {
   // Demag convolution fused with the effective field and torque: the X
   // inverse of all three components of a few rows stays in LDS and feeds the
   // four-cell field kernel, so B_demag never goes to HBM.  false: not
-  // available for this size / precision (the caller runs exec + field).
+  // available for this size / precision / set of terms (the caller runs exec
+  // and the field kernel).
   bool exec_field(const void *const mv[3], const void *vol, const void *Msa, double Ms_mul, const void *fav,
                   hipStream_t s) override {
-#ifndef UM_FUSED_FIELD
-    // measured slower than the separate X inverse + four-cell field kernels at
-    // 512x512x32 (the fused kernel is VALU- and latency-bound), so opt-in only
-    return false;
-#endif
     if constexpr (!std::is_same<T, float>::value) {
       return false;
     } else {
       const FieldArgs<T> &f = *(const FieldArgs<T> *)fav;
-      const int n = P[0] / 2, R = xf_rows(n);
-      // one four-cell group per thread
-      if (xodd || !R || !field4_aligned(f) || R * (N[0] / 4) > FFT_NT) return false;
+      const int n = P[0] / 2;
+      if (!fused_field || xodd || !xi4 || !x4_ok<T>(n, N[0]) || !xi4_field_ok(f, n)) return false;
       spectral(mv, vol, Msa, Ms_mul, s);
-      const XiArgs<T> a = xinv_args(R);
-      launch_xinv_field(a, f, n, R, s);
+      launch_xi4_field(xinv_args(1), f, n, s);
       return true;
     }
   }

@@ -251,15 +251,16 @@ __global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
 }
 
 // four x cells per thread; consecutive threads take consecutive x groups
-template <typename T, bool REG, int EX, bool STT>
+template <typename T, bool REG, int EX, bool STT, bool UNI>
 __global__ __launch_bounds__(256) void k_field_torque4(FieldArgs<T> a) {
-  const int ngx = a.g.Nx >> 2;
-  const long gid = blockIdx.x * 256L + threadIdx.x;
-  const long rows = (long)a.g.Ny * a.g.Nz;
+  // 32-bit indices: the launcher takes this kernel only below 2^31 groups
+  const uint32_t ngx = (uint32_t)a.g.Nx >> 2;
+  const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t rows = (uint32_t)a.g.Ny * (uint32_t)a.g.Nz;
   if (gid >= rows * ngx) return;
-  const long row = gid / ngx;
+  const uint32_t row = gid / ngx;
   const int gx = (int)(gid - row * ngx);
-  const int iy = (int)(row % a.g.Ny), iz = (int)(row / a.g.Ny);
+  const int iz = (int)(row / (uint32_t)a.g.Ny), iy = (int)(row - (uint32_t)iz * (uint32_t)a.g.Ny);
   const long I0 = a.g.idx(gx * 4, iy, iz);
   T bdv[3][4];
 #pragma unroll
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(256) void k_field_torque4(FieldArgs<T> a) {
     if (a.bd[0]) ld4(a.bd[c] + I0, bdv[c]);
     else bdv[c][0] = bdv[c][1] = bdv[c][2] = bdv[c][3] = T(0);
   }
-  field_group4<T, REG, EX, STT>(a, gx * 4, iy, iz, bdv);
+  field_group4<T, REG, EX, STT, UNI>(a, gx * 4, iy, iz, bdv);
 }
 
 // whether the four-cell kernel applies: FP32 (in FP64 the four cells need
@@ -291,15 +292,17 @@ void launch_field_torque(const FieldArgs<T> &a, hipStream_t s) {
   // four cells per thread pays on large grids; below ~512 work-groups the
   // one-cell kernel's four times more work-groups hide the latency better
   const long ncell = (long)a.g.Nx * a.g.Ny * a.g.Nz;
-  if (ncell >= 512L * 256 * 4 && field4_ok(a)) {
+  if (ncell >= 512L * 256 * 4 && ncell < (1L << 31) && field4_ok(a)) {
     const long n4 = (long)a.g.Nx / 4 * a.g.Ny * a.g.Nz;
     const dim3 grid((unsigned)((n4 + 255) / 256));
     const bool reg = a.regions != nullptr;
     const bool stt = a.has_zl || a.has_sl;
-#define UM_F4(R, E)                                                                    \
-  do {                                                                                 \
-    if (stt) UM_LAUNCH((k_field_torque4<T, R, E, true>), grid, dim3(256), 0, s, a);  \
-    else UM_LAUNCH((k_field_torque4<T, R, E, false>), grid, dim3(256), 0, s, a);     \
+    const bool uni = field_uniform(a);
+#define UM_F4(R, E)                                                                          \
+  do {                                                                                       \
+    if (stt) UM_LAUNCH((k_field_torque4<T, R, E, true, false>), grid, dim3(256), 0, s, a);  \
+    else if (uni) UM_LAUNCH((k_field_torque4<T, R, E, false, true>), grid, dim3(256), 0, s, a); \
+    else UM_LAUNCH((k_field_torque4<T, R, E, false, false>), grid, dim3(256), 0, s, a);     \
   } while (0)
     if constexpr (sizeof(T) == 4) {
       if (a.exch_mode == 0) { if (reg) UM_F4(true, 0); else UM_F4(false, 0); }

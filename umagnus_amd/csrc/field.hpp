@@ -238,6 +238,14 @@ __device__ __forceinline__ V3<T> slonczewski_cell(long i, V3<T> m, const Par<T> 
 }
 
 // ---- uniaxial anisotropy, kernels_src/cl/uniaxialanisotropy2.cl:17-35 -------
+// the field from the unit axis u and K1/Ms, K2/Ms
+template <typename T>
+__device__ __forceinline__ V3<T> uniaxial_apply(V3<T> m, V3<T> u, T K1, T K2) {
+  const T mu = vdot(m, u);
+  const T f = T(2.0) * K1 * mu;
+  const T q = T(4.0) * K2 * (mu * mu * mu);
+  return v3(f * u.x + q * u.x, f * u.y + q * u.y, f * u.z + q * u.z);
+}
 template <typename T>
 __device__ __forceinline__ V3<T> uniaxial_cell(long i, V3<T> m, const Par<T> &Ms, const Par<T> &K1_,
                                                const Par<T> &K2_, const Par<T> *u_) {
@@ -245,13 +253,12 @@ __device__ __forceinline__ V3<T> uniaxial_cell(long i, V3<T> m, const Par<T> &Ms
   const T invMs = inv_msat(Ms, i);
   const T K1 = K1_.at(i) * invMs;
   const T K2 = K2_.at(i) * invMs;
-  const T mu = vdot(m, u);
-  const T f = T(2.0) * K1 * mu;
-  const T q = T(4.0) * K2 * (mu * mu * mu);
-  return v3(f * u.x + q * u.x, f * u.y + q * u.y, f * u.z + q * u.z);
+  return uniaxial_apply(m, u, K1, K2);
 }
 
 // ---- cubic anisotropy, kernels_src/cl/cubicanisotropy2.cl:28-62 -------------
+template <typename T>
+__device__ __forceinline__ V3<T> cubic_apply(V3<T> m, V3<T> u1, V3<T> u2, V3<T> u3, T k1, T k2, T k3);
 template <typename T>
 __device__ __forceinline__ V3<T> cubic_cell(long i, V3<T> m, const Par<T> &Ms, const Par<T> &k1_,
                                             const Par<T> &k2_, const Par<T> &k3_, const Par<T> *c1_,
@@ -261,6 +268,11 @@ __device__ __forceinline__ V3<T> cubic_cell(long i, V3<T> m, const Par<T> &Ms, c
   const V3<T> u1 = vnormalized(v3(c1_[0].at(i), c1_[1].at(i), c1_[2].at(i)));
   const V3<T> u2 = vnormalized(v3(c2_[0].at(i), c2_[1].at(i), c2_[2].at(i)));
   const V3<T> u3 = vcross(u1, u2);
+  return cubic_apply(m, u1, u2, u3, k1, k2, k3);
+}
+// the cubic field from the unit axes and K1/Ms, K2/Ms, K3/Ms
+template <typename T>
+__device__ __forceinline__ V3<T> cubic_apply(V3<T> m, V3<T> u1, V3<T> u2, V3<T> u3, T k1, T k2, T k3) {
   const T a = vdot(u1, m), b = vdot(u2, m), c = vdot(u3, m);
   const T a2 = a * a, b2 = b * b, c2 = c * c;
   const T a4 = a2 * a2, b4 = b2 * b2, c4 = c2 * c2;
@@ -317,11 +329,15 @@ __device__ __forceinline__ V3<T> thermal_cell(long i, uint64_t seed, uint64_t st
 
 // ---- LL torque, kernels_src/cl/lltorque2.cl:11-24 ---------------------------------
 template <typename T>
-__device__ __forceinline__ V3<T> lltorque_cell(V3<T> m, V3<T> H, T alpha) {
+__device__ __forceinline__ V3<T> lltorque_g(V3<T> m, V3<T> H, T alpha, T gilb) {
   const V3<T> mxH = vcross(m, H);
-  const T gilb = T(-1.0) / (T(1.0) + alpha * alpha);
   const V3<T> mm = vcross(m, mxH);
   return v3(gilb * (mxH.x + alpha * mm.x), gilb * (mxH.y + alpha * mm.y), gilb * (mxH.z + alpha * mm.z));
+}
+template <typename T> __device__ __forceinline__ T gilbert(T alpha) { return T(-1.0) / (T(1.0) + alpha * alpha); }
+template <typename T>
+__device__ __forceinline__ V3<T> lltorque_cell(V3<T> m, V3<T> H, T alpha) {
+  return lltorque_g(m, H, alpha, gilbert(alpha));
 }
 
 // ---- the fused field + torque pass ----------------------------------------------
@@ -390,10 +406,29 @@ __device__ __forceinline__ void ld4u8(const uint8_t *p, int (&r)[4]) {
   r[0] = u & 255; r[1] = (u >> 8) & 255; r[2] = (u >> 16) & 255; r[3] = u >> 24;
 }
 
+// Stencil weights of a region-free group, formed once per thread from the one
+// uniform A and D with the operations exchange_nb / dmi_nb apply per neighbour.
+template <typename T>
+struct NbPre {
+  T s[3];    // exchange: w_axis * A
+  T e[3];    // DMI path: 2 A / c_axis^2
+  T dc[2];   // D / c_x, D / c_y
+  T fh;      // 0.5 D / A (boundary extrapolation)
+};
+template <typename T>
+__device__ __forceinline__ NbPre<T> nb_pre(T A, T D, T wx, T wy, T wz, T cx, T cy, T cz) {
+  NbPre<T> p;
+  p.s[0] = wx * A; p.s[1] = wy * A; p.s[2] = wz * A;
+  p.e[0] = (T(2.0) * A / (cx * cx)); p.e[1] = (T(2.0) * A / (cy * cy)); p.e[2] = (T(2.0) * A / (cz * cz));
+  p.dc[0] = D / cx; p.dc[1] = D / cy;
+  p.fh = (T(0.5) * D / A);
+  return p;
+}
+
 // exchange from gathered neighbours (order x-1, x+1, y-1, y+1, z-1, z+1), exchange.cl:5-87
-template <typename T, bool REG>
+template <typename T, bool REG, bool PRE = false>
 __device__ __forceinline__ V3<T> exchange_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb, int r0, const int (&rn)[6],
-                                             const T *aLUT, T A0, T wx, T wy, T wz) {
+                                             const T *aLUT, T A0, T wx, T wy, T wz, const NbPre<T> &p) {
   V3<T> B = v3(T(0), T(0), T(0));
 #pragma unroll
   for (int j = 0; j < 6; j++) {
@@ -402,7 +437,7 @@ __device__ __forceinline__ V3<T> exchange_nb(V3<T> m0, const V3<T> (&mn)[6], int
     if (is0(n)) n = m0;
     const T aj = REG ? aLUT[symidx(r0, rn[j])] : A0;
     const T w = j < 2 ? wx : j < 4 ? wy : wz;
-    const T s = w * aj;
+    const T s = PRE ? p.s[j >> 1] : w * aj;
     B.x += s * (n.x - m0.x);
     B.y += s * (n.y - m0.y);
     B.z += s * (n.z - m0.z);
@@ -412,9 +447,10 @@ __device__ __forceinline__ V3<T> exchange_nb(V3<T> m0, const V3<T> (&mn)[6], int
 
 // exchange + interfacial DMI from gathered neighbours, dmi.cl:8-148; an in-plane
 // neighbour outside the mesh (inside[j] false) is passed as zero
-template <typename T, bool REG>
+template <typename T, bool REG, bool PRE = false>
 __device__ __forceinline__ V3<T> dmi_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb, int r0, const int (&rn)[6],
-                                        const T *aLUT, const T *dLUT, T A0, T D0, T cx, T cy, T cz, int openbc) {
+                                        const T *aLUT, const T *dLUT, T A0, T D0, T cx, T cy, T cz, int openbc,
+                                        const NbPre<T> &p) {
   V3<T> h = v3(T(0), T(0), T(0));
 #pragma unroll
   for (int j = 0; j < 4; j++) {
@@ -427,7 +463,7 @@ __device__ __forceinline__ V3<T> dmi_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb,
     const T D = REG ? dLUT[symidx(r0, rj)] : D0;
     if (!is0(n) || !openbc) {
       if (is0(n)) {
-        const T f = sgn * c * (T(0.5) * D / A);
+        const T f = sgn * c * (PRE ? p.fh : (T(0.5) * D / A));
         if (axis == 0) {
           n.x = m0.x - (f * m0.z);
           n.y = m0.y;
@@ -438,11 +474,11 @@ __device__ __forceinline__ V3<T> dmi_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb,
           n.z = m0.z + (f * m0.y);
         }
       }
-      const T e = (T(2.0) * A / (c * c));
+      const T e = PRE ? p.e[axis] : (T(2.0) * A / (c * c));
       h.x += e * (n.x - m0.x);
       h.y += e * (n.y - m0.y);
       h.z += e * (n.z - m0.z);
-      const T dc = D / c;
+      const T dc = PRE ? p.dc[axis] : D / c;
       if (axis == 0) {
         h.x += dc * (sgn * n.z);
         h.z -= dc * (sgn * n.x);
@@ -458,7 +494,7 @@ __device__ __forceinline__ V3<T> dmi_nb(V3<T> m0, const V3<T> (&mn)[6], int nnb,
       V3<T> n = mn[j];
       if (is0(n)) n = m0;
       const T A = REG ? aLUT[symidx(r0, rn[j])] : A0;
-      const T e = (T(2.0) * A / (cz * cz));
+      const T e = PRE ? p.e[2] : (T(2.0) * A / (cz * cz));
       h.x += e * (n.x - m0.x);
       h.y += e * (n.y - m0.y);
       h.z += e * (n.z - m0.z);
@@ -565,10 +601,67 @@ __device__ __forceinline__ void g4_load(const FieldArgs<T> &a, int ix0, int iy, 
 // field of the four cells (zero when absent).  Same term order as k_field_torque.
 // EX: 0 none, 1 exchange, 2 + interfacial DMI, 3 + bulk DMI; STT: Zhang-Li and
 // Slonczewski torques (per the a.has_zl / a.has_sl flags).
-template <typename T, bool REG, int EX, bool STT = false>
+// UNI: Ms, alpha, the anisotropy, B_ext and temperature parameters are all
+// uniform (no per-cell arrays, checked by the launcher); their per-cell
+// quotients (1/Ms, K/Ms, the normalised axes, the Gilbert factor) are then the
+// same for the four cells and are formed once.
+template <bool UNI, typename T>
+__device__ __forceinline__ Par<T> upar(const Par<T> &p) {
+  if constexpr (UNI) return Par<T>{nullptr, p.mul};
+  else return p;
+}
+template <typename T>
+__host__ bool field_uniform(const FieldArgs<T> &a) {
+  const Par<T> *ps[] = {&a.Ms, &a.alpha, &a.Ku1, &a.Ku2, &a.u[0], &a.u[1], &a.u[2], &a.Kc1, &a.Kc2, &a.Kc3,
+                        &a.c1[0], &a.c1[1], &a.c1[2], &a.c2[0], &a.c2[1], &a.c2[2], &a.bext[0], &a.bext[1],
+                        &a.bext[2], &a.temp};
+  for (const Par<T> *p : ps)
+    if (p->arr) return false;
+  return true;
+}
+
+// HOIST: with UNI, form the stencil weights and the anisotropy / torque
+// constants once per thread instead of once per cell (fewer instructions,
+// more live registers: the choice of the calling kernel).
+template <typename T, bool REG, int EX, bool STT = false, bool UNI = false, bool HOIST = UNI>
 __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int iy, int iz, const G4In<T> &q,
                                            const T (&bdv)[3][4]) {
   const Mesh &g = a.g;
+  const Par<T> Ms = upar<UNI>(a.Ms), alpha = upar<UNI>(a.alpha), Ku1 = upar<UNI>(a.Ku1), Ku2 = upar<UNI>(a.Ku2);
+  const Par<T> Kc1 = upar<UNI>(a.Kc1), Kc2 = upar<UNI>(a.Kc2), Kc3 = upar<UNI>(a.Kc3), temp = upar<UNI>(a.temp);
+  Par<T> u[3], c1[3], c2[3], bext[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    u[c] = upar<UNI>(a.u[c]);
+    c1[c] = upar<UNI>(a.c1[c]);
+    c2[c] = upar<UNI>(a.c2[c]);
+    bext[c] = upar<UNI>(a.bext[c]);
+  }
+  // quantities that are the same for the four cells, formed once
+  constexpr bool H = UNI && HOIST;
+  constexpr bool PRE = !REG && H;   // region-free: one A, one D
+  NbPre<T> nbp{};
+  if constexpr (PRE) {
+    if (EX == 1 || EX == 2) nbp = nb_pre(q.A0, q.D0, a.wx, a.wy, a.wz, a.cx, a.cy, a.cz);
+  }
+  T invMs_u = T(0), K1_u = T(0), K2_u = T(0), k1_u = T(0), k2_u = T(0), k3_u = T(0), alpha_u = T(0), gilb_u = T(0);
+  V3<T> u_u{}, c1_u{}, c2_u{}, c3_u{};
+  if constexpr (H) {
+    invMs_u = inv_msat(Ms, 0);
+    if (a.has_uniax) {
+      u_u = vnormalized(v3(u[0].mul, u[1].mul, u[2].mul));
+      K1_u = Ku1.mul * invMs_u;
+      K2_u = Ku2.mul * invMs_u;
+    }
+    if (a.has_cubic) {
+      k1_u = Kc1.mul * invMs_u; k2_u = Kc2.mul * invMs_u; k3_u = Kc3.mul * invMs_u;
+      c1_u = vnormalized(v3(c1[0].mul, c1[1].mul, c1[2].mul));
+      c2_u = vnormalized(v3(c2[0].mul, c2[1].mul, c2[2].mul));
+      c3_u = vcross(c1_u, c2_u);
+    }
+    alpha_u = alpha.mul;
+    gilb_u = gilbert(alpha_u);
+  }
   const long I0 = g.idx(ix0, iy, iz);
   const int nnb = g.Nz != 1 ? 6 : 4;
   const bool inxl = (ix0 - 1 >= 0) || (g.pbc & 1), inxr = (ix0 + 4 < g.Nx) || (g.pbc & 1);
@@ -602,7 +695,7 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
       }
       V3<T> h;
       if (EX == 1) {
-        h = exchange_nb<T, REG>(m0, mn, nnb, q.rc[w], rn, a.aLUT, q.A0, a.wx, a.wy, a.wz);
+        h = exchange_nb<T, REG, PRE>(m0, mn, nnb, q.rc[w], rn, a.aLUT, q.A0, a.wx, a.wy, a.wz, nbp);
       } else if (EX == 3) {
         const V3<T> zero = v3(T(0), T(0), T(0));
         if (w == 0 && !inxl) mn[0] = zero;
@@ -618,19 +711,20 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
         if (w == 3 && !inxr) mn[1] = zero;
         if (!inyl) mn[2] = zero;
         if (!inyh) mn[3] = zero;
-        h = dmi_nb<T, REG>(m0, mn, nnb, q.rc[w], rn, a.aLUT, a.dLUT, q.A0, q.D0, a.cx, a.cy, a.cz, a.openbc);
+        h = dmi_nb<T, REG, PRE>(m0, mn, nnb, q.rc[w], rn, a.aLUT, a.dLUT, q.A0, q.D0, a.cx, a.cy, a.cz, a.openbc, nbp);
       }
-      const T invMs = inv_msat(a.Ms, I);
+      const T invMs = H ? invMs_u : inv_msat(Ms, I);
       B.x += h.x * invMs;
       B.y += h.y * invMs;
       B.z += h.z * invMs;
     }
     if (a.has_uniax) {
-      const V3<T> h = uniaxial_cell(I, m0, a.Ms, a.Ku1, a.Ku2, a.u);
+      const V3<T> h = H ? uniaxial_apply(m0, u_u, K1_u, K2_u) : uniaxial_cell(I, m0, Ms, Ku1, Ku2, u);
       B.x += h.x; B.y += h.y; B.z += h.z;
     }
     if (a.has_cubic) {
-      const V3<T> h = cubic_cell(I, m0, a.Ms, a.Kc1, a.Kc2, a.Kc3, a.c1, a.c2);
+      const V3<T> h = H ? cubic_apply(m0, c1_u, c2_u, c3_u, k1_u, k2_u, k3_u)
+                          : cubic_cell(I, m0, Ms, Kc1, Kc2, Kc3, c1, c2);
       B.x += h.x; B.y += h.y; B.z += h.z;
     }
     if (a.bextLUT[0]) {
@@ -640,13 +734,13 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
       B.z += a.bextLUT[2][r];
     }
     if (a.has_bext) {
-      B.x += a.bext[0].at(I);
-      B.y += a.bext[1].at(I);
-      B.z += a.bext[2].at(I);
+      B.x += bext[0].at(I);
+      B.y += bext[1].at(I);
+      B.z += bext[2].at(I);
     }
     if (a.has_therm) {
       const uint64_t stp = a.step_dev ? (uint64_t)*a.step_dev : a.step;
-      const V3<T> h = thermal_cell(I, a.seed, stp, a.stream, a.kth, a.Ms, a.temp, a.alpha);
+      const V3<T> h = thermal_cell(I, a.seed, stp, a.stream, a.kth, Ms, temp, alpha);
       B.x += h.x; B.y += h.y; B.z += h.z;
     }
     if (a.badd[0]) {
@@ -657,7 +751,7 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
     ob[0][w] = B.x; ob[1][w] = B.y; ob[2][w] = B.z;
     V3<T> t;
     if (a.precess) {
-      t = lltorque_cell(m0, B, a.alpha.at(I));
+      t = H ? lltorque_g(m0, B, alpha_u, gilb_u) : lltorque_cell(m0, B, alpha.at(I));
     } else {
       const V3<T> c = vcross(m0, vcross(m0, B));
       t = v3(-c.x, -c.y, -c.z);
@@ -714,11 +808,11 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
   }
 }
 
-template <typename T, bool REG, int EX, bool STT = false>
+template <typename T, bool REG, int EX, bool STT = false, bool UNI = false, bool HOIST = UNI>
 __device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int iy, int iz, const T (&bdv)[3][4]) {
   G4In<T> q;
   g4_load<T, REG, (STT && EX == 0) ? 1 : EX>(a, ix0, iy, iz, q);   // STT needs the neighbours
-  g4_compute<T, REG, EX, STT>(a, ix0, iy, iz, q, bdv);
+  g4_compute<T, REG, EX, STT, UNI, HOIST>(a, ix0, iy, iz, q, bdv);
 }
 
 template <typename T> void launch_field_torque(const FieldArgs<T> &a, hipStream_t s);
