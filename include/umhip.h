@@ -205,6 +205,18 @@ int um_rk45dp_stage(int prec, void *const dst[3], int nsrc, const void *const *s
  * out[2] = MaxVecNorm(delM) (maxdm).  A NaN anywhere in Err gives err = NaN. */
 int um_rk45dp_error(int prec, int nk, const void *const *k, const double *e, double h, int N, double out[3]);
 
+/* um_rk45dp_error without any allocation and without a readback; no
+ * counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops): the three results (err,
+ * rlerr, maxdm, same values as um_rk45dp_error) are written to out_dev, a
+ * device array of 3 doubles, on the library stream, and nothing waits for
+ * them.  The error kernel's max slots live in a 32-byte device array held by
+ * each GPU until the process ends (its size does not depend on N); um_init
+ * must have been called.  Calls on one GPU share that array, so calls that
+ * overlap on one GPU (other streams, other threads) race. */
+int um_rk45dp_error_noalloc(int prec, int nk, const void *const *k, const double *e, double h, int N,
+                            double *out_dev);
+
 /* ---- reductions: opencl/reduce.go (results in FP64, one readback) --------- */
 int um_reduce_sum(int prec, const void *x, int N, double *out);
 int um_reduce_dot(int prec, const void *a, const void *b, int N, double *out);

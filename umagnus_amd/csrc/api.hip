@@ -41,14 +41,30 @@ static int g_device = 0;
 static hipStream_t g_own_stream = nullptr;
 static hipStream_t g_stream = nullptr;
 
+// Device scratch of the allocation-free controller norms
+// (um_rk45dp_error_noalloc): the error kernel's max slots.  Every GPU the
+// library runs on holds its own copy until the process ends, at its own
+// address; host code uses that address (g_scratch_dev), never the array's
+// name, which on the host is only a stand-in for the symbol.
+constexpr int NOALLOC_SLOTS = 4;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_SLOTS];
+static unsigned long long *g_scratch_dev[MAX_DEVICES];
+
 void ensure_init() {
   std::lock_guard<std::mutex> lk(g_mu);
   if (g_inited) return;
   int n = 0;
   UM_HIP(hipGetDeviceCount(&n));
   UM_REQUIRE(n > 0, UM_EHIP, "no HIP device");
+  UM_REQUIRE(g_device >= 0 && g_device < n && g_device < MAX_DEVICES, UM_EINVAL, "um_init: no such device");
   UM_HIP(hipSetDevice(g_device));
   UM_HIP(hipStreamCreateWithFlags(&g_own_stream, hipStreamNonBlocking));
+  // with the device current: this GPU's copy of the scratch array
+  void *sp = nullptr;
+  UM_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  UM_REQUIRE(sp != nullptr, UM_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+  g_scratch_dev[g_device] = (unsigned long long *)sp;
   g_stream = g_own_stream;
   g_inited = true;
 }
@@ -111,13 +127,13 @@ struct RedScratch {
     if (hs) (void)hipHostFree(hs);
   }
 };
-static RedScratch &scratch() {
+static RedScratch &red_scratch() {
   static RedScratch *r = new RedScratch();
   return *r;
 }
 
 static double read_max_slot(hipStream_t s) {
-  RedScratch &r = scratch();
+  RedScratch &r = red_scratch();
   UM_HIP(hipMemcpyAsync(r.h, r.red.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   UM_HIP(hipStreamSynchronize(s));
   double v;
@@ -127,7 +143,7 @@ static double read_max_slot(hipStream_t s) {
 template <typename T>
 static double err_max(int mode, const void *const a[3], const void *const b[3], long N) {
   hipStream_t s = lib_stream();
-  RedScratch &r = scratch();
+  RedScratch &r = red_scratch();
   fill_bytes(r.red.p, r.red.bytes, s);
   ErrArgs<T> e{};
   e.mode = mode;
@@ -143,7 +159,7 @@ static double err_max(int mode, const void *const a[3], const void *const b[3], 
 template <typename T>
 static double sum_wdot(const void *a, const void *b, long N) {
   hipStream_t s = lib_stream();
-  RedScratch &r = scratch();
+  RedScratch &r = red_scratch();
   const T *aa[1] = {(const T *)a};
   const T *bb[1] = {(const T *)b};
   launch_wdot<T>(aa, b ? bb : nullptr, 1, nullptr, T(1), N, r.partial.as<double>(), r.sum.as<double>(), s);
@@ -592,7 +608,7 @@ int um_rk45dp_error(int prec, int nk, const void *const *k, const double *e, dou
     check_rk_err(nk, k, e, h, N);
     UM_REQUIRE(out, UM_EINVAL, "um_rk45dp_error: NULL out");
     hipStream_t s = lib_stream();
-    RedScratch &r = scratch();
+    RedScratch &r = red_scratch();
     fill_bytes(r.red.p, r.red.bytes, s);
     auto go = [&](auto zero) {
       using T = decltype(zero);
@@ -608,6 +624,45 @@ int um_rk45dp_error(int prec, int nk, const void *const *k, const double *e, dou
     out[0] = std::sqrt(v[0]) * h;
     out[1] = v[2];
     out[2] = std::sqrt(v[1]);
+  });
+}
+
+extern "C++" {
+// (err, rlerr, maxdm) from the max slots, as um_rk45dp_error does on the host
+__global__ void k_rk_err_final(const unsigned long long *res, double h, double *out) {
+  if (threadIdx.x == 0) {
+    out[0] = __builtin_sqrt(__longlong_as_double((long long)res[0])) * h;
+    out[1] = __longlong_as_double((long long)res[2]);
+    out[2] = __builtin_sqrt(__longlong_as_double((long long)res[1]));
+  }
+}
+}  // extern "C++"
+
+int um_rk45dp_error_noalloc(int prec, int nk, const void *const *k, const double *e, double h, int N,
+                            double *out_dev) {
+  return guard([&] {
+    check_rk_err(nk, k, e, h, N);
+    UM_REQUIRE(out_dev, UM_EINVAL, "um_rk45dp_error_noalloc: NULL out_dev");
+    bool inited;
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      inited = g_inited;
+    }
+    // the scratch address is looked up by um_init; nothing is created here
+    UM_REQUIRE(inited, UM_ESTATE, "um_rk45dp_error_noalloc: call um_init first");
+    unsigned long long *slots = g_scratch_dev[g_device];
+    UM_REQUIRE(slots, UM_ESTATE, "um_rk45dp_error_noalloc: no scratch address for this device");
+    hipStream_t s = g_stream;
+    fill_bytes(slots, NOALLOC_SLOTS * sizeof(unsigned long long), s);   // a kernel, not a memset node
+    auto go = [&](auto zero) {
+      using T = decltype(zero);
+      ErrArgs<T> a{};
+      rk_err_args<T>(a, nk, k, e, N, slots);
+      launch_err(a, s);
+    };
+    DISPATCH(prec, go(0.0f), go(0.0));
+    UM_LAUNCH(k_rk_err_final, dim3(1), dim3(64), 0, s, (const unsigned long long *)slots, h, out_dev);
+    UM_HIP(hipGetLastError());
   });
 }
 
@@ -628,7 +683,7 @@ int um_reduce_maxabs(int prec, const void *x, int N, double *out) {
   return guard([&] {
     UM_REQUIRE(x && out, UM_EINVAL, "um_reduce_maxabs: NULL argument");
     hipStream_t s = lib_stream();
-    RedScratch &r = scratch();
+    RedScratch &r = red_scratch();
     fill_bytes(r.red.p, r.red.bytes, s);
     DISPATCH(prec, launch_maxabs<float>((const float *)x, N, (unsigned long long *)r.red.p, s),
              launch_maxabs<double>((const double *)x, N, (unsigned long long *)r.red.p, s));

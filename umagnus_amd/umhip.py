@@ -49,7 +49,7 @@ EXPORTS = [
     "um_sim_run", "um_sim_average_m", "um_sim_energy", "um_sim_stream", "um_sim_set_profiling",
     "um_sim_kernel_times", "um_sim_set_graph", "um_sims_steps", "um_graph_begin", "um_graph_end",
     "um_graph_launch", "um_graph_destroy", "um_sim_relax", "um_trace_kernels", "um_traced_kernels",
-    "um_field_torque", "um_rk45dp_stage", "um_rk45dp_error",
+    "um_field_torque", "um_rk45dp_stage", "um_rk45dp_error", "um_rk45dp_error_noalloc",
     "um_sim_add_field_term", "um_sim_clear_field_terms",
 ]
 
