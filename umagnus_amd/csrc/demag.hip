@@ -394,6 +394,103 @@ __global__ __launch_bounds__(XF4_NT) void k_xi4(XiArgs<T> a) {
   }
 }
 
+// Register four-step X forward (FP32, Px = 2 Nx, n = Px/2 = Nx = A*B): the
+// two-for-one packed row z[j] = x[2j] + i x[2j+1] is zero for j >= n/2, i.e.
+// for n2 >= B/2, so step 1 loads half its points.  Step 1: lane (h, n1)
+// transforms z[n1 + A n2] over n2 in registers and twiddles by W_n^(n1 k2);
+// one LDS transpose (real, then imaginary parts) hands lane k2 the A values
+// over n1; step 2 gives Z[k2 + B k1].  The unpacking
+//   X[k] = E[k] + W_2n^k O[k],  E = (Z[k] + conj Z[n-k]) / 2,  O = (Z[k] - conj Z[n-k]) / 2i
+// takes Z[n-k] from the group's lane B - k2 (lane 0 holds its own partner)
+// through a cross-lane permute, and each lane stores the X[k] it holds
+// (lane 0 also X[n]).
+template <typename T, int N>
+__global__ __launch_bounds__(XF4_NT) void k_xf4(XArgs<T> a) {
+  constexpr int A = X4Shape<N>::A, B = X4Shape<N>::B, G = B / A;
+  constexpr int GROUPS = XF4_NT / B, ROWS = GROUPS * G, TS = B * (A + 1), HB = B / 2;
+  __shared__ T ldr[ROWS * TS];
+  const int c = blockIdx.y;
+  const int grp = threadIdx.x / B, l = threadIdx.x % B;
+  const int row0 = blockIdx.x * ROWS + grp * G;
+  const bool uniform = a.Ms.arr == nullptr && a.vol == nullptr;
+  const T fac = T(MU0) * a.Ms.mul;
+  const int h1 = l / A, n1 = l % A;
+  {
+    const int r = row0 + h1;
+    C2<T> v[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
+    if (r < a.rows) {
+      const long i0 = (long)r * a.in_pitch;
+      const T *src = a.src[c] + i0;
+#pragma unroll
+      for (int n2 = 0; n2 < HB; n2++) v[n2] = *(const C2<T> *)(src + 2 * (n1 + A * n2));
+      if (uniform) {
+#pragma unroll
+        for (int n2 = 0; n2 < HB; n2++) v[n2] = C2<T>{fac * v[n2].x, fac * v[n2].y};
+      } else {
+#pragma unroll
+        for (int n2 = 0; n2 < HB; n2++) {
+          const long i = i0 + 2 * (n1 + A * n2);
+          v[n2].x = (T(MU0) * a.Ms.at(i)) * (a.vol ? a.vol[i] : T(1)) * v[n2].x;
+          v[n2].y = (T(MU0) * a.Ms.at(i + 1)) * (a.vol ? a.vol[i + 1] : T(1)) * v[n2].y;
+        }
+      }
+    }
+    dft<B, -1>(v);
+#pragma unroll
+    for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], a.tw2[2 * n1 * k2]);   // W_n^(n1 k2)
+    T *tw = ldr + (grp * G + h1) * TS + n1;
+#pragma unroll
+    for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].x;
+    __syncthreads();
+    C2<T> u[G][A];
+#pragma unroll
+    for (int h = 0; h < G; h++) {
+      const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
+#pragma unroll
+      for (int q = 0; q < A; q++) u[h][q].x = t[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].y;
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < G; h++) {
+      const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
+#pragma unroll
+      for (int q = 0; q < A; q++) u[h][q].y = t[q];
+    }
+    // unpack and store: lane l = k2 holds Z[k2 + B k1]
+    const int base = threadIdx.x - l;                       // first lane of the group
+    const int src_lane = base + ((B - l) & (B - 1));        // holder of Z[n - k]
+#pragma unroll
+    for (int h = 0; h < G; h++) {
+      dft<A, -1>(u[h]);
+      const int rr = row0 + h;
+      const int z = (int)a.fRy.div(rr < a.rows ? rr : 0), y = (rr < a.rows ? rr : 0) - z * (int)a.fRy.d;
+      C2<T> *orow = a.out[c] + ((long)z * a.Py + y) * a.Hp;
+#pragma unroll
+      for (int k1 = 0; k1 < A; k1++) {
+        // partner element: A-1-k1 of lane B-l, or (A-k1) mod A of lane 0 itself
+        const C2<T> q = u[h][A - 1 - k1];
+        C2<T> Zp{__shfl(q.x, src_lane, 64), __shfl(q.y, src_lane, 64)};
+        if (l == 0) Zp = u[h][(A - k1) & (A - 1)];
+        const C2<T> Zk = u[h][k1];
+        const C2<T> Bc = cconj(Zp);
+        const C2<T> E = C2<T>{T(0.5) * (Zk.x + Bc.x), T(0.5) * (Zk.y + Bc.y)};
+        const C2<T> O = C2<T>{T(0.5) * (Zk.y - Bc.y), T(-0.5) * (Zk.x - Bc.x)};
+        const int k = l + B * k1;
+        const C2<T> t = cmul(a.tw2[k], O);
+        if (rr < a.rows) {
+          orow[k] = cadd(E, t);
+          if (k == 0) orow[N] = cconj(csub(E, t));
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Column pass along y (or z): nl adjacent kx columns per work-group.
 // ---------------------------------------------------------------------------
@@ -1376,6 +1473,23 @@ static bool launch_xi4(const XiArgs<T> &a, int n, hipStream_t s) {
 }
 
 template <typename T>
+static bool launch_xf4(const XArgs<T> &a, int n, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (a.mode != 0 || !x4_ok<T>(n, a.Nx) || a.in_pitch != a.Nx) return false;
+    auto go = [&](auto kern, int rows_per_wg) {
+      UM_LAUNCH(kern, dim3((a.rows + rows_per_wg - 1) / rows_per_wg, 3), dim3(XF4_NT), 0, s, a);
+      UM_HIP(hipGetLastError());
+      return true;
+    };
+#define UM_XF4(NN) \
+    if (n == NN) return go(k_xf4<T, NN>, XF4_NT / X4Shape<NN>::B * (X4Shape<NN>::B / X4Shape<NN>::A));
+    UM_XF4(64) UM_XF4(128) UM_XF4(256) UM_XF4(512) UM_XF4(1024)
+#undef UM_XF4
+  }
+  return false;
+}
+
+template <typename T>
 static void launch_xfwd(const XArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s) {
   const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + 2 * n);
   if constexpr (std::is_same<T, float>::value) {
@@ -1954,7 +2068,7 @@ struct DemagPlan : DemagBase {
       a.f = tx2.len; a.fhalf.init(H);
       UM_LAUNCH(k_xfwd_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
       UM_HIP(hipGetLastError());
-    } else {
+    } else if (!(xi4 && launch_xf4<T>(a, P[0] / 2, s))) {
       launch_xfwd<T>(a, P[0] / 2, nl_x, g, s);
     }
   }
