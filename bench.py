@@ -13,9 +13,10 @@ Prints one JSON line.  `value` is Mcell*steps/s; `roofline` prices one
 effective-field + torque evaluation (the demag FFT passes plus the fused
 field/torque kernel, timed with HIP events on the simulation stream over the
 timed region) at its algorithmic HBM bytes against the 8 TB/s HBM3E peak.
-`batched` compares a 64-member thermal Heun ensemble stepped together
-(graph replays on per-member streams) with the same members stepped one
-after another; `adjoint` times the gradient of a 2000-step RK4 film run
+`batched` compares the 256-member thermal Heun ensemble stepped in batched
+launches (one launch per kernel variant and phase for all members) with the
+same members stepped one after another (eager, and from captured graphs) and
+with graph replays interleaved over the members' streams; `adjoint` times the gradient of a 2000-step RK4 film run
 (taped forward + reverse sweep) against the plain forward run; `precision`
 compares the FP32 path with the all-FP64 one on a 1024x1024x8 Voronoi-grain
 film with the Newell tensor.
@@ -106,16 +107,19 @@ def ensemble_members(count=256, seed=4):
 def batched(steps=60):
     """Ensemble throughput: 256 members, Heun at FixDt 5e-15 s with T = 300 K
     (one Philox stream per member), stepped one after another (eager, the
-    reference's queue order) vs together through graph replays (um_sims_steps),
-    the latter also with autotuned launch schedules.  A bounded number of
-    steps per member (the ensemble's 20000 steps repeat the same step)."""
+    reference's queue order; and from each member's captured graph) vs
+    together: graph replays interleaved over the members' streams
+    (um_sims_steps) and batched launches, one per kernel variant and phase for
+    the whole ensemble (um_sims_steps_batched), the latter also with autotuned
+    launch schedules.  A bounded number of steps per member (the ensemble's
+    20000 steps repeat the same step)."""
     import torch
     from umagnus_amd import umhip
     from umagnus_amd.params import exchange_lut
     sizes = ensemble_members()
     cells = sum(a * b * c for a, b, c in sizes)
 
-    def make(graph):
+    def make(graph, warm="steps"):
         sims = []
         for i, N in enumerate(sizes):
             s = umhip.Sim(N, (2e-9,) * 3, dtype=np.float32)
@@ -128,21 +132,26 @@ def batched(steps=60):
             s.set_solver_params(dt=5e-15, fixdt=5e-15)
             s.set_m(seeded_m(N, 3 + i))
             s.set_graph(graph)
-            s.steps(4)    # setup, and the graph capture, outside the timed region
+            if warm == "steps":
+                s.steps(4)    # setup, and the graph capture, outside the timed region
             sims.append(s)
+        if warm == "batched":
+            umhip.sims_steps_batched(sims, 4)   # setup and the merged launch plan, untimed
         return sims
 
-    res = {}
-    for mode in ("plain", "graph_one_at_a_time", "batched", "tuned"):
+    res, launches = {}, 0
+    for mode in ("plain", "graph_one_at_a_time", "interleaved_graphs", "batched", "tuned"):
         if mode == "tuned":
             umhip.set_autotune(1)
         try:
-            sims = make(mode != "plain")
+            sims = make(mode != "plain", "batched" if mode in ("batched", "tuned") else "steps")
         finally:
             umhip.set_autotune(0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         if mode in ("batched", "tuned"):
+            launches = umhip.sims_steps_batched(sims, steps)
+        elif mode == "interleaved_graphs":
             umhip.sims_steps(sims, steps)
         else:
             for s in sims:     # one member after another, eager or from its captured graph
@@ -156,6 +165,8 @@ def batched(steps=60):
                         "Nz 1..4, 2 nm, Heun FixDt 5e-15 s, T = 300 K, FP32",
             "steps": steps, "unit": "Mcell·steps/s", "plain": round(res["plain"], 2),
             "graph_one_at_a_time": round(res["graph_one_at_a_time"], 2),
+            "interleaved_graphs": round(res["interleaved_graphs"], 2),
+            "launches_per_two_steps": launches,
             "value": round(res["batched"], 2), "speedup": round(res["batched"] / res["plain"], 3),
             "speedup_vs_graph_one_at_a_time": round(res["batched"] / res["graph_one_at_a_time"], 3),
             "tuned": round(res["tuned"], 2), "tuned_speedup": round(res["tuned"] / res["plain"], 3)}

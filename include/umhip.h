@@ -465,6 +465,20 @@ int um_sim_set_graph(void *sim, int on);
  * grids run concurrently.  Members in graph mode with FixDt replay graphs;
  * others step eagerly in turn. */
 int um_sims_steps(int nsims, void *const *sims, int n);
+/* Ensemble stepping in batched launches, the one-launch form of the same job
+ * queue (cmd/uMagNUS/queue.go:86-151): n steps of every simulation in sims[]
+ * (each handle once).  After one eager step per member, the launches of two
+ * fixed-dt steps of every member in graph mode are recorded and merged: each
+ * kernel variant runs once per phase for all members that reach it, every
+ * work-group working on its own member's grid and buffers, so each member
+ * computes what it computes alone (um_sims_steps, um_sim_steps), bit for bit.
+ * The merged sequence is replayed as one graph per two steps; it is rebuilt
+ * when a member's parameters, solver, FixDt or buffers change.  Members not in
+ * graph mode with FixDt step eagerly in turn.  *launches_per_two_steps (when
+ * not NULL) receives the number of launches that replay two steps of the
+ * whole ensemble, or 0 when a member needs a kernel without a batched form
+ * and the members were replayed one graph each. */
+int um_sims_steps_batched(int nsims, void *const *sims, int n, int *launches_per_two_steps);
 /* Generic capture of the library stream (um_set_stream; a stream other than
  * the NULL stream, which HIP cannot capture): every um_* call between begin
  * and end is recorded instead of run; launch replays it on the library

@@ -6,8 +6,11 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 
+#include "batch.hpp"
 #include "field.hpp"
 #include "sim.hpp"
 #include "vec.hpp"
@@ -33,6 +36,19 @@ void trace_kernel(const void *stub) {
   }
   std::lock_guard<std::mutex> lk(g_trace_mu);
   g_trace.push_back(name);
+}
+
+thread_local BatchRec *g_rec = nullptr;
+void batch_refuse(const void *stub) {
+  Dl_info info{};
+  std::string name = "?";
+  if (dladdr(stub, &info) && info.dli_sname) {
+    int st = 0;
+    char *d = abi::__cxa_demangle(info.dli_sname, nullptr, nullptr, &st);
+    name = (st == 0 && d) ? d : info.dli_sname;
+    std::free(d);
+  }
+  throw Error(UM_EUNSUP, "kernel cannot run in a batched launch: " + name);
 }
 
 static std::mutex g_mu;
@@ -1070,6 +1086,225 @@ int um_sims_steps(int nsims, void *const *sims, int n) {
     for (long r = 0; r < most; r++)
       for (int i = 0; i < nsims; i++)
         if (r < reps[i]) SIM(sims[i])->graph_replay();
+    for (int i = 0; i < nsims; i++) SIM(sims[i])->graph_finish(stop[i]);
+  });
+}
+
+}  // extern "C"
+
+// ---- batched ensembles -----------------------------------------------------------------
+// The recorded two-step launch lists of the members, merged into batched
+// launches: phase by phase, the kernel family that most members have next runs
+// for all of them (one k_batch launch per distinct kernel variant), each
+// member's launches keeping their order.  The tables (arguments, work-group
+// prefix, member grids) live in one device buffer; the launch sequence is
+// captured once into a graph on the plan's stream.
+namespace um {
+struct BatchGroupLaunch {
+  void (*launch)(const void *, const uint32_t *, const uint3 *, int, unsigned, dim3, size_t, hipStream_t);
+  size_t a_off, s_off, g_off;
+  int nm;
+  unsigned total;
+  dim3 block;
+  size_t lds;
+};
+struct BatchPlan {
+  std::vector<const void *> sims;
+  std::vector<std::vector<uint64_t>> keys;
+  std::vector<BatchGroupLaunch> launches;
+  DevBuf tab;
+  hipStream_t st = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<hipEvent_t> ready;
+  int launches_per_replay() const { return (int)launches.size(); }
+  ~BatchPlan() {
+    if (st) (void)hipStreamSynchronize(st);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (hipEvent_t e : ready) (void)hipEventDestroy(e);
+    if (done) (void)hipEventDestroy(done);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+static std::mutex g_bmu;
+static BatchPlan *g_bplan = nullptr;
+static long g_batch_launches = 0;   // batched launches per replay of the last plan
+
+static std::string body_family(const char *mangled) {
+  // the body's class name without its template arguments
+  std::string f(mangled);
+  const size_t i = f.find('I');
+  return i == std::string::npos ? f : f.substr(0, i);
+}
+
+static BatchPlan *batch_plan_build(const std::vector<SimBase *> &m) {
+  const int nm = (int)m.size();
+  std::vector<std::vector<BatchEntry>> rec(nm);
+  for (int i = 0; i < nm; i++) m[i]->batch_record(rec[i]);
+  std::unique_ptr<BatchPlan> P(new BatchPlan());
+  for (SimBase *x : m) {
+    P->sims.push_back((const void *)x);
+    P->keys.push_back(x->batch_key());
+  }
+  // merge the lists
+  std::map<std::string, int> famid;
+  std::vector<std::vector<int>> fam(nm);
+  for (int i = 0; i < nm; i++)
+    for (const BatchEntry &e : rec[i]) {
+      auto it = famid.emplace(body_family(e.fam), (int)famid.size()).first;
+      fam[i].push_back(it->second);
+    }
+  std::vector<size_t> head(nm, 0);
+  std::vector<unsigned char> host;
+  auto put = [&](const void *p, size_t bytes) {
+    const size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + bytes);
+    std::memcpy(host.data() + off, p, bytes);
+    return off;
+  };
+  for (;;) {
+    std::vector<int> count(famid.size(), 0);
+    bool any = false;
+    for (int i = 0; i < nm; i++)
+      if (head[i] < rec[i].size()) { count[fam[i][head[i]]]++; any = true; }
+    if (!any) break;
+    const int f = (int)(std::max_element(count.begin(), count.end()) - count.begin());
+    // members whose next launch is of family f, grouped by kernel variant
+    std::vector<std::pair<const BatchEntry *, std::vector<int>>> groups;
+    for (int i = 0; i < nm; i++) {
+      if (head[i] >= rec[i].size() || fam[i][head[i]] != f) continue;
+      const BatchEntry &e = rec[i][head[i]++];
+      bool placed = false;
+      for (auto &g : groups) {
+        const BatchEntry &r = *g.first;
+        if (r.key == e.key && r.lds == e.lds && r.block.x == e.block.x && r.block.y == e.block.y &&
+            r.block.z == e.block.z) {
+          g.second.push_back(i);
+          placed = true;
+          break;
+        }
+      }
+      if (!placed) groups.push_back({&e, std::vector<int>{i}});
+    }
+    for (auto &g : groups) {
+      const BatchEntry &r = *g.first;
+      const int gn = (int)g.second.size();
+      std::vector<unsigned char> args((size_t)gn * r.asize);
+      std::vector<uint32_t> start(gn);
+      std::vector<uint3> grid(gn);
+      unsigned long long total = 0;
+      for (int j = 0; j < gn; j++) {
+        const int i = g.second[j];
+        const BatchEntry &e = rec[i][head[i] - 1];
+        std::memcpy(args.data() + (size_t)j * r.asize, e.args.data(), r.asize);
+        start[j] = (uint32_t)total;
+        grid[j] = make_uint3(e.grid.x, e.grid.y, e.grid.z);
+        total += (unsigned long long)e.grid.x * e.grid.y * e.grid.z;
+      }
+      UM_REQUIRE(total > 0 && total < (1ull << 31), UM_EUNSUP, "batched launch too large");
+      BatchGroupLaunch L;
+      L.launch = r.launch;
+      L.a_off = put(args.data(), args.size());
+      L.s_off = put(start.data(), start.size() * sizeof(uint32_t));
+      L.g_off = put(grid.data(), grid.size() * sizeof(uint3));
+      L.nm = gn;
+      L.total = (unsigned)total;
+      L.block = r.block;
+      L.lds = r.lds;
+      P->launches.push_back(L);
+    }
+  }
+  P->tab.alloc(std::max<size_t>(host.size(), 256));
+  UM_HIP(hipMemcpy(P->tab.p, host.data(), host.size(), hipMemcpyHostToDevice));
+  UM_HIP(hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking));
+  UM_HIP(hipEventCreateWithFlags(&P->done, hipEventDisableTiming));
+  P->ready.resize(nm);
+  for (int i = 0; i < nm; i++) UM_HIP(hipEventCreateWithFlags(&P->ready[i], hipEventDisableTiming));
+  // the replay: every batched launch in order, captured once
+  const unsigned char *base = (const unsigned char *)P->tab.p;
+  hipGraph_t g = nullptr;
+  UM_HIP(hipStreamBeginCapture(P->st, hipStreamCaptureModeRelaxed));
+  try {
+    for (const BatchGroupLaunch &L : P->launches)
+      L.launch(base + L.a_off, (const uint32_t *)(base + L.s_off), (const uint3 *)(base + L.g_off), L.nm, L.total,
+               L.block, L.lds, P->st);
+  } catch (...) {
+    hipGraph_t junk = nullptr;
+    (void)hipStreamEndCapture(P->st, &junk);
+    if (junk) (void)hipGraphDestroy(junk);
+    throw;
+  }
+  UM_HIP(hipStreamEndCapture(P->st, &g));
+  hipError_t e = hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  UM_HIP(e);
+  return P.release();
+}
+
+static bool batch_plan_fresh(const BatchPlan *P, const std::vector<SimBase *> &m) {
+  if (!P || P->sims.size() != m.size()) return false;
+  for (size_t i = 0; i < m.size(); i++)
+    if (P->sims[i] != (const void *)m[i] || P->keys[i] != m[i]->batch_key()) return false;
+  return true;
+}
+}  // namespace um
+
+extern "C" {
+
+int um_sims_steps_batched(int nsims, void *const *sims, int n, int *launches_per_two_steps) {
+  return guard([&] {
+    UM_REQUIRE(nsims >= 0 && n >= 0 && (nsims == 0 || sims), UM_EINVAL, "um_sims_steps_batched: bad arguments");
+    for (int i = 0; i < nsims; i++) SIMCHK(sims[i]);
+    for (int i = 0; i < nsims; i++)
+      for (int j = 0; j < i; j++) UM_REQUIRE(sims[i] != sims[j], UM_EINVAL, "um_sims_steps_batched: a handle twice");
+    std::vector<long> stop(nsims), reps(nsims);
+    for (int i = 0; i < nsims; i++) {
+      stop[i] = SIM(sims[i])->step_count() + n;
+      reps[i] = SIM(sims[i])->batch_prepare(stop[i]);
+    }
+    // members whose every remaining pair of steps can be replayed together
+    long R = 0;
+    for (int i = 0; i < nsims; i++) R = std::max(R, reps[i]);
+    std::vector<SimBase *> mem;
+    for (int i = 0; i < nsims; i++)
+      if (reps[i] > 0) {
+        UM_REQUIRE(reps[i] == R, UM_ESTATE, "batched members out of step");
+        mem.push_back(SIM(sims[i]));
+      }
+    std::lock_guard<std::mutex> lk(g_bmu);
+    if (!mem.empty()) {
+      bool batched = true;
+      if (!batch_plan_fresh(g_bplan, mem)) {
+        delete g_bplan;
+        g_bplan = nullptr;
+        try {
+          g_bplan = batch_plan_build(mem);
+        } catch (const Error &e) {
+          if (e.code != UM_EUNSUP) throw;
+          batched = false;   // a kernel without a batched form: graph replays per member
+        }
+      }
+      if (batched) {
+        BatchPlan *P = g_bplan;
+        g_batch_launches = P->launches_per_replay();
+        for (size_t i = 0; i < mem.size(); i++) {
+          UM_HIP(hipEventRecord(P->ready[i], mem[i]->stream()));
+          UM_HIP(hipStreamWaitEvent(P->st, P->ready[i], 0));
+        }
+        for (long r = 0; r < R; r++) {
+          UM_HIP(hipGraphLaunch(P->gexec, P->st));
+          for (SimBase *x : mem) x->batch_advance();
+        }
+        UM_HIP(hipEventRecord(P->done, P->st));
+        for (SimBase *x : mem) UM_HIP(hipStreamWaitEvent(x->stream(), P->done, 0));
+      } else {
+        g_batch_launches = 0;
+        for (SimBase *x : mem) x->graph_ensure();
+        for (long r = 0; r < R; r++)
+          for (SimBase *x : mem) x->graph_replay();
+      }
+    }
+    if (launches_per_two_steps) *launches_per_two_steps = (int)g_batch_launches;
     for (int i = 0; i < nsims; i++) SIM(sims[i])->graph_finish(stop[i]);
   });
 }

@@ -71,8 +71,14 @@ inline Par<T> par(const void *arr, double mul) { return Par<T>{(const T *)arr, (
 // which compiled variant a plan or a simulation selects.
 extern bool g_trace_on;
 void trace_kernel(const void *host_stub);
+// While a batch recording is installed on the calling thread (batch.hpp), a
+// launch of a kernel that cannot be batched refuses the recording.
+struct BatchRec;
+extern thread_local BatchRec *g_rec;
+[[noreturn]] void batch_refuse(const void *host_stub);
 #define UM_LAUNCH(kern, ...)                                                \
   do {                                                                      \
+    if (::um::g_rec) ::um::batch_refuse((const void *)(kern));             \
     if (::um::g_trace_on) ::um::trace_kernel((const void *)(kern));         \
     hipLaunchKernelGGL(kern, __VA_ARGS__);                                  \
   } while (0)

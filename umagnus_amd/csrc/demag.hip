@@ -22,6 +22,7 @@
 #include <mutex>
 #include <type_traits>
 
+#include "batch.hpp"
 #include "fft_ct.hpp"
 #include "field.hpp"
 #include "sim.hpp"
@@ -78,87 +79,93 @@ struct XArgs {
 };
 
 template <typename T, int CN, int CNL>
-__global__ __launch_bounds__(FFT_NT) void k_xfwd(XArgs<T> a) {
-  constexpr int EPT = FftEpt<T>::value;
-  extern __shared__ __align__(16) unsigned char smem_[];
-  C2<T> *lds = (C2<T> *)smem_;
-  const int c = blockIdx.y;
-  const int n = CN > 0 ? CN : a.f.n, nl = CN > 0 ? CNL : a.nl, total = nl * n;
-  C2<T> *twl = lds + nl * n;   // W_{2n}^k, k < 2n (compile-time path only)
-  if constexpr (CN > 0) {
-    for (int g = threadIdx.x; g < 2 * CN; g += FFT_NT) twl[g] = a.tw2[g];
-  }
-  const int row0 = blockIdx.x * nl;
-  const T *src = a.src[c];
-  const int Nx = a.Nx;
-  // uniform Ms and no geometry: one factor mu0*Ms (copypadmul2.cl:15-19 with vol = 1)
-  const bool uniform = (a.mode != 0) || (a.Ms.arr == nullptr && a.vol == nullptr);
-  const T fac = (a.mode != 0) ? T(1) : T(MU0) * a.Ms.mul;
-  const bool pairs = ((Nx & 1) == 0) || a.mode != 0;   // rows start 8-byte aligned
-  constexpr int SL = CN > 0 ? (CN * CNL + FFT_NT - 1) / FFT_NT : EPT;
-  constexpr int BATCH = SL > EPT / 2 ? EPT / 2 : SL;
-#pragma unroll
-  for (int s0 = 0; s0 < SL; s0 += BATCH) {
-    C2<T> v[BATCH];
-#pragma unroll
-    for (int s = 0; s < BATCH; s++) {
-      const int g = threadIdx.x + (s0 + s) * FFT_NT;
-      v[s] = C2<T>{T(0), T(0)};
-      if (s0 + s < SL && g < total) {
-        const int l = cdiv<CN>(g, a.f.fn), e = g - l * n;
-        const int r = row0 + l;
-        if (r < a.rows) {
-          const long i0 = (long)r * a.in_pitch + 2 * e;
-          if (pairs && 2 * e + 1 < (a.mode != 0 ? 2 * n : Nx)) {
-            v[s] = *(const C2<T> *)(src + i0);
-          } else {
-            if (a.mode != 0 || 2 * e < Nx) v[s].x = src[i0];
-            if (a.mode != 0 || 2 * e + 1 < Nx) v[s].y = src[i0 + 1];
-          }
-        }
-      }
+struct BXfwd {
+  using Args = XArgs<T>;
+  static constexpr int NT = FFT_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const XArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    constexpr int EPT = FftEpt<T>::value;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C2<T> *lds = (C2<T> *)smem_;
+    const int c = blockIdx.y;
+    const int n = CN > 0 ? CN : a.f.n, nl = CN > 0 ? CNL : a.nl, total = nl * n;
+    C2<T> *twl = lds + nl * n;   // W_{2n}^k, k < 2n (compile-time path only)
+    if constexpr (CN > 0) {
+      for (int g = threadIdx.x; g < 2 * CN; g += FFT_NT) twl[g] = a.tw2[g];
     }
-#pragma unroll
-    for (int s = 0; s < BATCH; s++) {
-      const int g = threadIdx.x + (s0 + s) * FFT_NT;
-      if (s0 + s < SL && g < total) {
-        C2<T> w = v[s];
-        if (uniform) {
-          w.x = fac * w.x;
-          w.y = fac * w.y;
-        } else {
+    const int row0 = blockIdx.x * nl;
+    const T *src = a.src[c];
+    const int Nx = a.Nx;
+    // uniform Ms and no geometry: one factor mu0*Ms (copypadmul2.cl:15-19 with vol = 1)
+    const bool uniform = (a.mode != 0) || (a.Ms.arr == nullptr && a.vol == nullptr);
+    const T fac = (a.mode != 0) ? T(1) : T(MU0) * a.Ms.mul;
+    const bool pairs = ((Nx & 1) == 0) || a.mode != 0;   // rows start 8-byte aligned
+    constexpr int SL = CN > 0 ? (CN * CNL + FFT_NT - 1) / FFT_NT : EPT;
+    constexpr int BATCH = SL > EPT / 2 ? EPT / 2 : SL;
+  #pragma unroll
+    for (int s0 = 0; s0 < SL; s0 += BATCH) {
+      C2<T> v[BATCH];
+  #pragma unroll
+      for (int s = 0; s < BATCH; s++) {
+        const int g = threadIdx.x + (s0 + s) * FFT_NT;
+        v[s] = C2<T>{T(0), T(0)};
+        if (s0 + s < SL && g < total) {
           const int l = cdiv<CN>(g, a.f.fn), e = g - l * n;
           const int r = row0 + l;
           if (r < a.rows) {
             const long i0 = (long)r * a.in_pitch + 2 * e;
-            if (2 * e < Nx) w.x = (T(MU0) * a.Ms.at(i0)) * (a.vol ? a.vol[i0] : T(1)) * w.x;
-            if (2 * e + 1 < Nx) w.y = (T(MU0) * a.Ms.at(i0 + 1)) * (a.vol ? a.vol[i0 + 1] : T(1)) * w.y;
+            if (pairs && 2 * e + 1 < (a.mode != 0 ? 2 * n : Nx)) {
+              v[s] = *(const C2<T> *)(src + i0);
+            } else {
+              if (a.mode != 0 || 2 * e < Nx) v[s].x = src[i0];
+              if (a.mode != 0 || 2 * e + 1 < Nx) v[s].y = src[i0 + 1];
+            }
           }
         }
-        lds[g] = w;
+      }
+  #pragma unroll
+      for (int s = 0; s < BATCH; s++) {
+        const int g = threadIdx.x + (s0 + s) * FFT_NT;
+        if (s0 + s < SL && g < total) {
+          C2<T> w = v[s];
+          if (uniform) {
+            w.x = fac * w.x;
+            w.y = fac * w.y;
+          } else {
+            const int l = cdiv<CN>(g, a.f.fn), e = g - l * n;
+            const int r = row0 + l;
+            if (r < a.rows) {
+              const long i0 = (long)r * a.in_pitch + 2 * e;
+              if (2 * e < Nx) w.x = (T(MU0) * a.Ms.at(i0)) * (a.vol ? a.vol[i0] : T(1)) * w.x;
+              if (2 * e + 1 < Nx) w.y = (T(MU0) * a.Ms.at(i0 + 1)) * (a.vol ? a.vol[i0 + 1] : T(1)) * w.y;
+            }
+          }
+          lds[g] = w;
+        }
       }
     }
+    __syncthreads();
+    run_fft<T, CN, CNL, -1, 0, 2>(lds, twl, nl, a.f.fn, a.f);
+    // two-for-one post-processing: X[k] = E[k] + W^k O[k], X[n-k] = conj(E[k] - W^k O[k])
+    const C2<T> *tw2 = CN > 0 ? twl : a.tw2;
+    const int half = n / 2 + 1;
+    for (int g = threadIdx.x; g < nl * half; g += FFT_NT) {
+      const int l = cdiv<(CN > 0 ? CN / 2 + 1 : 0)>(g, a.fhalf), k = g - l * half;
+      const int r = row0 + l;
+      if (r >= a.rows) continue;
+      const C2<T> A = lds[l * n + k];
+      const C2<T> Bc = cconj(lds[l * n + (k == 0 ? 0 : n - k)]);
+      const C2<T> E = C2<T>{T(0.5) * (A.x + Bc.x), T(0.5) * (A.y + Bc.y)};
+      const C2<T> O = C2<T>{T(0.5) * (A.y - Bc.y), T(-0.5) * (A.x - Bc.x)};
+      const C2<T> t = cmul(tw2[k], O);
+      const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+      C2<T> *orow = a.out[c] + ((long)z * a.Py + y) * a.Hp;
+      orow[k] = cadd(E, t);
+      if (n - k != k) orow[n - k] = cconj(csub(E, t));
+    }
   }
-  __syncthreads();
-  run_fft<T, CN, CNL, -1, 0, 2>(lds, twl, nl, a.f.fn, a.f);
-  // two-for-one post-processing: X[k] = E[k] + W^k O[k], X[n-k] = conj(E[k] - W^k O[k])
-  const C2<T> *tw2 = CN > 0 ? twl : a.tw2;
-  const int half = n / 2 + 1;
-  for (int g = threadIdx.x; g < nl * half; g += FFT_NT) {
-    const int l = cdiv<(CN > 0 ? CN / 2 + 1 : 0)>(g, a.fhalf), k = g - l * half;
-    const int r = row0 + l;
-    if (r >= a.rows) continue;
-    const C2<T> A = lds[l * n + k];
-    const C2<T> Bc = cconj(lds[l * n + (k == 0 ? 0 : n - k)]);
-    const C2<T> E = C2<T>{T(0.5) * (A.x + Bc.x), T(0.5) * (A.y + Bc.y)};
-    const C2<T> O = C2<T>{T(0.5) * (A.y - Bc.y), T(-0.5) * (A.x - Bc.x)};
-    const C2<T> t = cmul(tw2[k], O);
-    const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
-    C2<T> *orow = a.out[c] + ((long)z * a.Py + y) * a.Hp;
-    orow[k] = cadd(E, t);
-    if (n - k != k) orow[n - k] = cconj(csub(E, t));
-  }
-}
+};
+template <typename T, int CN, int CNL>
+__global__ __launch_bounds__(FFT_NT) void k_xfwd(XArgs<T> a) { BXfwd<T, CN, CNL>::run(a, hw_bid(), hw_grid()); }
 
 // ---------------------------------------------------------------------------
 // X inverse: half spectrum rows -> real rows (C2R), keeping x < Nx.
@@ -174,82 +181,88 @@ struct XiArgs {
 };
 
 template <typename T, int CN, int CNL>
-__global__ __launch_bounds__(FFT_NT) void k_xinv(XiArgs<T> a) {
-  constexpr int EPT = FftEpt<T>::value;
-  extern __shared__ __align__(16) unsigned char smem_[];
-  C2<T> *lds = (C2<T> *)smem_;
-  const int c = blockIdx.y;
-  const int n = CN > 0 ? CN : a.f.n, nl = CN > 0 ? CNL : a.nl;
-  C2<T> *twl = lds + nl * n;
-  if constexpr (CN > 0) {
-    for (int g = threadIdx.x; g < 2 * CN; g += FFT_NT) twl[g] = a.tw2[g];
-  }
-  const C2<T> *tw2 = a.tw2;   // pre-processing reads the global table (once per element)
-  const int row0 = blockIdx.x * nl;
-  const int half = n / 2 + 1;
-  const int total = nl * half;
-  constexpr int S = CN > 0 ? (CNL * (CN / 2 + 1) + FFT_NT - 1) / FFT_NT : EPT / 2 + 1;
-  for (int g0 = 0; g0 < total; g0 += S * FFT_NT) {
-    C2<T> xk[S], xn[S];
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-      const int g = g0 + threadIdx.x + s * FFT_NT;
-      xk[s] = C2<T>{T(0), T(0)};
-      xn[s] = xk[s];
-      if (g < total) {
-        const int l = cdiv<(CN > 0 ? CN / 2 + 1 : 0)>(g, a.fhalf), k = g - l * half;
+struct BXinv {
+  using Args = XiArgs<T>;
+  static constexpr int NT = FFT_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const XiArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    constexpr int EPT = FftEpt<T>::value;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C2<T> *lds = (C2<T> *)smem_;
+    const int c = blockIdx.y;
+    const int n = CN > 0 ? CN : a.f.n, nl = CN > 0 ? CNL : a.nl;
+    C2<T> *twl = lds + nl * n;
+    if constexpr (CN > 0) {
+      for (int g = threadIdx.x; g < 2 * CN; g += FFT_NT) twl[g] = a.tw2[g];
+    }
+    const C2<T> *tw2 = a.tw2;   // pre-processing reads the global table (once per element)
+    const int row0 = blockIdx.x * nl;
+    const int half = n / 2 + 1;
+    const int total = nl * half;
+    constexpr int S = CN > 0 ? (CNL * (CN / 2 + 1) + FFT_NT - 1) / FFT_NT : EPT / 2 + 1;
+    for (int g0 = 0; g0 < total; g0 += S * FFT_NT) {
+      C2<T> xk[S], xn[S];
+  #pragma unroll
+      for (int s = 0; s < S; s++) {
+        const int g = g0 + threadIdx.x + s * FFT_NT;
+        xk[s] = C2<T>{T(0), T(0)};
+        xn[s] = xk[s];
+        if (g < total) {
+          const int l = cdiv<(CN > 0 ? CN / 2 + 1 : 0)>(g, a.fhalf), k = g - l * half;
+          const int r = row0 + l;
+          if (r < a.rows) {
+            const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+            const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
+            xk[s] = irow[k];
+            xn[s] = irow[n - k];
+          }
+        }
+      }
+  #pragma unroll
+      for (int s = 0; s < S; s++) {
+        const int g = g0 + threadIdx.x + s * FFT_NT;
+        if (g < total) {
+          const int l = cdiv<(CN > 0 ? CN / 2 + 1 : 0)>(g, a.fhalf), k = g - l * half;
+          const C2<T> Xk = xk[s], Xnk = xn[s];
+          // Z'[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k])
+          C2<T> sm = cadd(Xk, cconj(Xnk)), d = csub(Xk, cconj(Xnk));
+          C2<T> t = cmul(cconj(tw2[k]), d);
+          lds[l * n + k] = C2<T>{sm.x - t.y, sm.y + t.x};
+          if (k != 0 && n - k != k) {
+            sm = cadd(Xnk, cconj(Xk));
+            d = csub(Xnk, cconj(Xk));
+            t = cmul(cconj(tw2[n - k]), d);
+            lds[l * n + n - k] = C2<T>{sm.x - t.y, sm.y + t.x};
+          }
+        }
+      }
+    }
+    __syncthreads();
+    run_fft<T, CN, CNL, +1, 0, 2>(lds, twl, nl, a.f.fn, a.f);
+    T *dst = a.dst[c];
+    const int Nx = a.Nx;
+    constexpr int SL = CN > 0 ? (CN * CNL + FFT_NT - 1) / FFT_NT : EPT;
+  #pragma unroll
+    for (int s = 0; s < SL; s++) {
+      const int g = threadIdx.x + s * FFT_NT;
+      if (g < nl * n) {
+        const int l = cdiv<CN>(g, a.f.fn), e = g - l * n;
         const int r = row0 + l;
         if (r < a.rows) {
-          const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
-          const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
-          xk[s] = irow[k];
-          xn[s] = irow[n - k];
-        }
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-      const int g = g0 + threadIdx.x + s * FFT_NT;
-      if (g < total) {
-        const int l = cdiv<(CN > 0 ? CN / 2 + 1 : 0)>(g, a.fhalf), k = g - l * half;
-        const C2<T> Xk = xk[s], Xnk = xn[s];
-        // Z'[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k])
-        C2<T> sm = cadd(Xk, cconj(Xnk)), d = csub(Xk, cconj(Xnk));
-        C2<T> t = cmul(cconj(tw2[k]), d);
-        lds[l * n + k] = C2<T>{sm.x - t.y, sm.y + t.x};
-        if (k != 0 && n - k != k) {
-          sm = cadd(Xnk, cconj(Xk));
-          d = csub(Xnk, cconj(Xk));
-          t = cmul(cconj(tw2[n - k]), d);
-          lds[l * n + n - k] = C2<T>{sm.x - t.y, sm.y + t.x};
+          const C2<T> v = lds[g];
+          const long base = (long)r * Nx;
+          if (2 * e + 1 < Nx && (Nx & 1) == 0) {
+            *(C2<T> *)(dst + base + 2 * e) = v;
+          } else {
+            if (2 * e < Nx) dst[base + 2 * e] = v.x;
+            if (2 * e + 1 < Nx) dst[base + 2 * e + 1] = v.y;
+          }
         }
       }
     }
   }
-  __syncthreads();
-  run_fft<T, CN, CNL, +1, 0, 2>(lds, twl, nl, a.f.fn, a.f);
-  T *dst = a.dst[c];
-  const int Nx = a.Nx;
-  constexpr int SL = CN > 0 ? (CN * CNL + FFT_NT - 1) / FFT_NT : EPT;
-#pragma unroll
-  for (int s = 0; s < SL; s++) {
-    const int g = threadIdx.x + s * FFT_NT;
-    if (g < nl * n) {
-      const int l = cdiv<CN>(g, a.f.fn), e = g - l * n;
-      const int r = row0 + l;
-      if (r < a.rows) {
-        const C2<T> v = lds[g];
-        const long base = (long)r * Nx;
-        if (2 * e + 1 < Nx && (Nx & 1) == 0) {
-          *(C2<T> *)(dst + base + 2 * e) = v;
-        } else {
-          if (2 * e < Nx) dst[base + 2 * e] = v.x;
-          if (2 * e + 1 < Nx) dst[base + 2 * e + 1] = v.y;
-        }
-      }
-    }
-  }
-}
+};
+template <typename T, int CN, int CNL>
+__global__ __launch_bounds__(FFT_NT) void k_xinv(XiArgs<T> a) { BXinv<T, CN, CNL>::run(a, hw_bid(), hw_grid()); }
 
 // ---------------------------------------------------------------------------
 // X axis of odd length (periodic along x with odd Nx, so Px = Nx is odd and
@@ -261,61 +274,73 @@ __global__ __launch_bounds__(FFT_NT) void k_xinv(XiArgs<T> a) {
 // a.f: FftLen of length Px; a.fhalf: H; a.nl lines per work-group.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(FFT_NT) void k_xfwd_odd(XArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem_[];
-  C2<T> *lds = (C2<T> *)smem_;
-  const int c = blockIdx.y;
-  const int n = a.f.n, nl = a.nl, half = n / 2 + 1;
-  const int row0 = blockIdx.x * nl;
-  const T *src = a.src[c];
-  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
-    const int l = (int)a.f.fn.div(g), e = g - l * n;
-    const int r = row0 + l;
-    T v = T(0);
-    if (r < a.rows && e < a.Nx) {
-      const long i = (long)r * a.in_pitch + e;
-      v = src[i];
-      if (a.mode == 0) v = (T(MU0) * a.Ms.at(i)) * (a.vol ? a.vol[i] : T(1)) * v;
+struct BXfwdOdd {
+  using Args = XArgs<T>;
+  static constexpr int NT = FFT_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const XArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C2<T> *lds = (C2<T> *)smem_;
+    const int c = blockIdx.y;
+    const int n = a.f.n, nl = a.nl, half = n / 2 + 1;
+    const int row0 = blockIdx.x * nl;
+    const T *src = a.src[c];
+    for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
+      const int l = (int)a.f.fn.div(g), e = g - l * n;
+      const int r = row0 + l;
+      T v = T(0);
+      if (r < a.rows && e < a.Nx) {
+        const long i = (long)r * a.in_pitch + e;
+        v = src[i];
+        if (a.mode == 0) v = (T(MU0) * a.Ms.at(i)) * (a.vol ? a.vol[i] : T(1)) * v;
+      }
+      lds[g] = C2<T>{v, T(0)};
     }
-    lds[g] = C2<T>{v, T(0)};
+    __syncthreads();
+    lds_fft<T, -1, 0>(lds, nl, a.f.fn, n, 1, a.f);
+    for (int g = threadIdx.x; g < nl * half; g += FFT_NT) {
+      const int l = (int)a.fhalf.div(g), k = g - l * half;
+      const int r = row0 + l;
+      if (r >= a.rows) continue;
+      const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+      a.out[c][((long)z * a.Py + y) * a.Hp + k] = lds[l * n + k];
+    }
   }
-  __syncthreads();
-  lds_fft<T, -1, 0>(lds, nl, a.f.fn, n, 1, a.f);
-  for (int g = threadIdx.x; g < nl * half; g += FFT_NT) {
-    const int l = (int)a.fhalf.div(g), k = g - l * half;
-    const int r = row0 + l;
-    if (r >= a.rows) continue;
-    const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
-    a.out[c][((long)z * a.Py + y) * a.Hp + k] = lds[l * n + k];
-  }
-}
+};
+template <typename T>
+__global__ __launch_bounds__(FFT_NT) void k_xfwd_odd(XArgs<T> a) { BXfwdOdd<T>::run(a, hw_bid(), hw_grid()); }
 
 template <typename T>
-__global__ __launch_bounds__(FFT_NT) void k_xinv_odd(XiArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem_[];
-  C2<T> *lds = (C2<T> *)smem_;
-  const int c = blockIdx.y;
-  const int n = a.f.n, nl = a.nl, half = n / 2 + 1;
-  const int row0 = blockIdx.x * nl;
-  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
-    const int l = (int)a.f.fn.div(g), k = g - l * n;
-    const int r = row0 + l;
-    C2<T> v = C2<T>{T(0), T(0)};
-    if (r < a.rows) {
-      const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
-      const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
-      v = k < half ? irow[k] : cconj(irow[n - k]);
+struct BXinvOdd {
+  using Args = XiArgs<T>;
+  static constexpr int NT = FFT_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const XiArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C2<T> *lds = (C2<T> *)smem_;
+    const int c = blockIdx.y;
+    const int n = a.f.n, nl = a.nl, half = n / 2 + 1;
+    const int row0 = blockIdx.x * nl;
+    for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
+      const int l = (int)a.f.fn.div(g), k = g - l * n;
+      const int r = row0 + l;
+      C2<T> v = C2<T>{T(0), T(0)};
+      if (r < a.rows) {
+        const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+        const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
+        v = k < half ? irow[k] : cconj(irow[n - k]);
+      }
+      lds[g] = v;
     }
-    lds[g] = v;
+    __syncthreads();
+    lds_fft<T, +1, 0>(lds, nl, a.f.fn, n, 1, a.f);
+    for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
+      const int l = (int)a.f.fn.div(g), e = g - l * n;
+      const int r = row0 + l;
+      if (r < a.rows && e < a.Nx) a.dst[c][(long)r * a.Nx + e] = lds[g].x;
+    }
   }
-  __syncthreads();
-  lds_fft<T, +1, 0>(lds, nl, a.f.fn, n, 1, a.f);
-  for (int g = threadIdx.x; g < nl * n; g += FFT_NT) {
-    const int l = (int)a.f.fn.div(g), e = g - l * n;
-    const int r = row0 + l;
-    if (r < a.rows && e < a.Nx) a.dst[c][(long)r * a.Nx + e] = lds[g].x;
-  }
-}
+};
+template <typename T>
+__global__ __launch_bounds__(FFT_NT) void k_xinv_odd(XiArgs<T> a) { BXinvOdd<T>::run(a, hw_bid(), hw_grid()); }
 
 // ---------------------------------------------------------------------------
 // Register four-step X inverse (FP32, zero-padded rows: Px = 2 Nx, n = Px/2
@@ -339,60 +364,66 @@ template <> struct X4Shape<1024> { static constexpr int A = 32, B = 32; };
 // Z'[k] = (X[k] + conj X[n-k]) + i W_2n^-k (X[k] - conj X[n-k]), z = IDFT_n(Z'),
 // x[2j] = Re z[j], x[2j+1] = Im z[j] for j < n/2
 template <typename T, int N>
-__global__ __launch_bounds__(XF4_NT) void k_xi4(XiArgs<T> a) {
-  constexpr int A = X4Shape<N>::A, B = X4Shape<N>::B, G = B / A;
-  constexpr int GROUPS = XF4_NT / B, ROWS = GROUPS * G, TS = B * (A + 1);
-  __shared__ C2<T> lds[ROWS * TS];
-  const int c = blockIdx.y;
-  const int grp = threadIdx.x / B, l = threadIdx.x % B;
-  const int row0 = blockIdx.x * ROWS + grp * G;
-  {
-    const int h = l / A, n1 = l % A, r = row0 + h;
-    C2<T> v[B];
-#pragma unroll
-    for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
-    if (r < a.rows) {
-      const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
-      const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
-      C2<T> xk[B], xn[B];
-#pragma unroll
-      for (int n2 = 0; n2 < B; n2++) {
-        const int k = n1 + A * n2;
-        xk[n2] = irow[k];
-        xn[n2] = irow[N - k];
+struct BXi4 {
+  using Args = XiArgs<T>;
+  static constexpr int NT = XF4_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const XiArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    constexpr int A = X4Shape<N>::A, B = X4Shape<N>::B, G = B / A;
+    constexpr int GROUPS = XF4_NT / B, ROWS = GROUPS * G, TS = B * (A + 1);
+    __shared__ C2<T> lds[ROWS * TS];
+    const int c = blockIdx.y;
+    const int grp = threadIdx.x / B, l = threadIdx.x % B;
+    const int row0 = blockIdx.x * ROWS + grp * G;
+    {
+      const int h = l / A, n1 = l % A, r = row0 + h;
+      C2<T> v[B];
+  #pragma unroll
+      for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
+      if (r < a.rows) {
+        const int z = (int)a.fRy.div(r), y = r - z * (int)a.fRy.d;
+        const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
+        C2<T> xk[B], xn[B];
+  #pragma unroll
+        for (int n2 = 0; n2 < B; n2++) {
+          const int k = n1 + A * n2;
+          xk[n2] = irow[k];
+          xn[n2] = irow[N - k];
+        }
+  #pragma unroll
+        for (int n2 = 0; n2 < B; n2++) {
+          const int k = n1 + A * n2;
+          const C2<T> sm = cadd(xk[n2], cconj(xn[n2])), d = csub(xk[n2], cconj(xn[n2]));
+          const C2<T> t = cmul(cconj(a.tw2[k]), d);
+          v[n2] = C2<T>{sm.x - t.y, sm.y + t.x};
+        }
       }
-#pragma unroll
-      for (int n2 = 0; n2 < B; n2++) {
-        const int k = n1 + A * n2;
-        const C2<T> sm = cadd(xk[n2], cconj(xn[n2])), d = csub(xk[n2], cconj(xn[n2]));
-        const C2<T> t = cmul(cconj(a.tw2[k]), d);
-        v[n2] = C2<T>{sm.x - t.y, sm.y + t.x};
+      dft<B, +1>(v);
+  #pragma unroll
+      for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(a.tw2[2 * n1 * k2]));   // W_n^-(n1 k2)
+      C2<T> *t = lds + (grp * G + h) * TS + n1;
+  #pragma unroll
+      for (int k2 = 0; k2 < B; k2++) t[k2 * (A + 1)] = v[k2];
+    }
+    __syncthreads();
+    T *dst = a.dst[c];
+  #pragma unroll
+    for (int h = 0; h < G; h++) {
+      C2<T> u[A];
+      const C2<T> *t = lds + (grp * G + h) * TS + l * (A + 1);
+  #pragma unroll
+      for (int n1 = 0; n1 < A; n1++) u[n1] = t[n1];
+      dft<A, +1>(u);
+      const int r = row0 + h;
+      if (r < a.rows) {
+        T *orow = dst + (long)r * N;
+  #pragma unroll
+        for (int k1 = 0; k1 < A / 2; k1++) *(float2 *)(orow + 2 * (l + B * k1)) = float2{u[k1].x, u[k1].y};
       }
     }
-    dft<B, +1>(v);
-#pragma unroll
-    for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(a.tw2[2 * n1 * k2]));   // W_n^-(n1 k2)
-    C2<T> *t = lds + (grp * G + h) * TS + n1;
-#pragma unroll
-    for (int k2 = 0; k2 < B; k2++) t[k2 * (A + 1)] = v[k2];
   }
-  __syncthreads();
-  T *dst = a.dst[c];
-#pragma unroll
-  for (int h = 0; h < G; h++) {
-    C2<T> u[A];
-    const C2<T> *t = lds + (grp * G + h) * TS + l * (A + 1);
-#pragma unroll
-    for (int n1 = 0; n1 < A; n1++) u[n1] = t[n1];
-    dft<A, +1>(u);
-    const int r = row0 + h;
-    if (r < a.rows) {
-      T *orow = dst + (long)r * N;
-#pragma unroll
-      for (int k1 = 0; k1 < A / 2; k1++) *(float2 *)(orow + 2 * (l + B * k1)) = float2{u[k1].x, u[k1].y};
-    }
-  }
-}
+};
+template <typename T, int N>
+__global__ __launch_bounds__(XF4_NT) void k_xi4(XiArgs<T> a) { BXi4<T, N>::run(a, hw_bid(), hw_grid()); }
 
 // Register four-step X forward (FP32, Px = 2 Nx, n = Px/2 = Nx = A*B): the
 // two-for-one packed row z[j] = x[2j] + i x[2j+1] is zero for j >= n/2, i.e.
@@ -405,91 +436,97 @@ __global__ __launch_bounds__(XF4_NT) void k_xi4(XiArgs<T> a) {
 // through a cross-lane permute, and each lane stores the X[k] it holds
 // (lane 0 also X[n]).
 template <typename T, int N>
-__global__ __launch_bounds__(XF4_NT) void k_xf4(XArgs<T> a) {
-  constexpr int A = X4Shape<N>::A, B = X4Shape<N>::B, G = B / A;
-  constexpr int GROUPS = XF4_NT / B, ROWS = GROUPS * G, TS = B * (A + 1), HB = B / 2;
-  __shared__ T ldr[ROWS * TS];
-  const int c = blockIdx.y;
-  const int grp = threadIdx.x / B, l = threadIdx.x % B;
-  const int row0 = blockIdx.x * ROWS + grp * G;
-  const bool uniform = a.Ms.arr == nullptr && a.vol == nullptr;
-  const T fac = T(MU0) * a.Ms.mul;
-  const int h1 = l / A, n1 = l % A;
-  {
-    const int r = row0 + h1;
-    C2<T> v[B];
-#pragma unroll
-    for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
-    if (r < a.rows) {
-      const long i0 = (long)r * a.in_pitch;
-      const T *src = a.src[c] + i0;
-#pragma unroll
-      for (int n2 = 0; n2 < HB; n2++) v[n2] = *(const C2<T> *)(src + 2 * (n1 + A * n2));
-      if (uniform) {
-#pragma unroll
-        for (int n2 = 0; n2 < HB; n2++) v[n2] = C2<T>{fac * v[n2].x, fac * v[n2].y};
-      } else {
-#pragma unroll
-        for (int n2 = 0; n2 < HB; n2++) {
-          const long i = i0 + 2 * (n1 + A * n2);
-          v[n2].x = (T(MU0) * a.Ms.at(i)) * (a.vol ? a.vol[i] : T(1)) * v[n2].x;
-          v[n2].y = (T(MU0) * a.Ms.at(i + 1)) * (a.vol ? a.vol[i + 1] : T(1)) * v[n2].y;
+struct BXf4 {
+  using Args = XArgs<T>;
+  static constexpr int NT = XF4_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const XArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    constexpr int A = X4Shape<N>::A, B = X4Shape<N>::B, G = B / A;
+    constexpr int GROUPS = XF4_NT / B, ROWS = GROUPS * G, TS = B * (A + 1), HB = B / 2;
+    __shared__ T ldr[ROWS * TS];
+    const int c = blockIdx.y;
+    const int grp = threadIdx.x / B, l = threadIdx.x % B;
+    const int row0 = blockIdx.x * ROWS + grp * G;
+    const bool uniform = a.Ms.arr == nullptr && a.vol == nullptr;
+    const T fac = T(MU0) * a.Ms.mul;
+    const int h1 = l / A, n1 = l % A;
+    {
+      const int r = row0 + h1;
+      C2<T> v[B];
+  #pragma unroll
+      for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
+      if (r < a.rows) {
+        const long i0 = (long)r * a.in_pitch;
+        const T *src = a.src[c] + i0;
+  #pragma unroll
+        for (int n2 = 0; n2 < HB; n2++) v[n2] = *(const C2<T> *)(src + 2 * (n1 + A * n2));
+        if (uniform) {
+  #pragma unroll
+          for (int n2 = 0; n2 < HB; n2++) v[n2] = C2<T>{fac * v[n2].x, fac * v[n2].y};
+        } else {
+  #pragma unroll
+          for (int n2 = 0; n2 < HB; n2++) {
+            const long i = i0 + 2 * (n1 + A * n2);
+            v[n2].x = (T(MU0) * a.Ms.at(i)) * (a.vol ? a.vol[i] : T(1)) * v[n2].x;
+            v[n2].y = (T(MU0) * a.Ms.at(i + 1)) * (a.vol ? a.vol[i + 1] : T(1)) * v[n2].y;
+          }
         }
       }
-    }
-    dft<B, -1>(v);
-#pragma unroll
-    for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], a.tw2[2 * n1 * k2]);   // W_n^(n1 k2)
-    T *tw = ldr + (grp * G + h1) * TS + n1;
-#pragma unroll
-    for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].x;
-    __syncthreads();
-    C2<T> u[G][A];
-#pragma unroll
-    for (int h = 0; h < G; h++) {
-      const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
-#pragma unroll
-      for (int q = 0; q < A; q++) u[h][q].x = t[q];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].y;
-    __syncthreads();
-#pragma unroll
-    for (int h = 0; h < G; h++) {
-      const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
-#pragma unroll
-      for (int q = 0; q < A; q++) u[h][q].y = t[q];
-    }
-    // unpack and store: lane l = k2 holds Z[k2 + B k1]
-    const int base = threadIdx.x - l;                       // first lane of the group
-    const int src_lane = base + ((B - l) & (B - 1));        // holder of Z[n - k]
-#pragma unroll
-    for (int h = 0; h < G; h++) {
-      dft<A, -1>(u[h]);
-      const int rr = row0 + h;
-      const int z = (int)a.fRy.div(rr < a.rows ? rr : 0), y = (rr < a.rows ? rr : 0) - z * (int)a.fRy.d;
-      C2<T> *orow = a.out[c] + ((long)z * a.Py + y) * a.Hp;
-#pragma unroll
-      for (int k1 = 0; k1 < A; k1++) {
-        // partner element: A-1-k1 of lane B-l, or (A-k1) mod A of lane 0 itself
-        const C2<T> q = u[h][A - 1 - k1];
-        C2<T> Zp{__shfl(q.x, src_lane, 64), __shfl(q.y, src_lane, 64)};
-        if (l == 0) Zp = u[h][(A - k1) & (A - 1)];
-        const C2<T> Zk = u[h][k1];
-        const C2<T> Bc = cconj(Zp);
-        const C2<T> E = C2<T>{T(0.5) * (Zk.x + Bc.x), T(0.5) * (Zk.y + Bc.y)};
-        const C2<T> O = C2<T>{T(0.5) * (Zk.y - Bc.y), T(-0.5) * (Zk.x - Bc.x)};
-        const int k = l + B * k1;
-        const C2<T> t = cmul(a.tw2[k], O);
-        if (rr < a.rows) {
-          orow[k] = cadd(E, t);
-          if (k == 0) orow[N] = cconj(csub(E, t));
+      dft<B, -1>(v);
+  #pragma unroll
+      for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], a.tw2[2 * n1 * k2]);   // W_n^(n1 k2)
+      T *tw = ldr + (grp * G + h1) * TS + n1;
+  #pragma unroll
+      for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].x;
+      __syncthreads();
+      C2<T> u[G][A];
+  #pragma unroll
+      for (int h = 0; h < G; h++) {
+        const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
+  #pragma unroll
+        for (int q = 0; q < A; q++) u[h][q].x = t[q];
+      }
+      __syncthreads();
+  #pragma unroll
+      for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].y;
+      __syncthreads();
+  #pragma unroll
+      for (int h = 0; h < G; h++) {
+        const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
+  #pragma unroll
+        for (int q = 0; q < A; q++) u[h][q].y = t[q];
+      }
+      // unpack and store: lane l = k2 holds Z[k2 + B k1]
+      const int base = threadIdx.x - l;                       // first lane of the group
+      const int src_lane = base + ((B - l) & (B - 1));        // holder of Z[n - k]
+  #pragma unroll
+      for (int h = 0; h < G; h++) {
+        dft<A, -1>(u[h]);
+        const int rr = row0 + h;
+        const int z = (int)a.fRy.div(rr < a.rows ? rr : 0), y = (rr < a.rows ? rr : 0) - z * (int)a.fRy.d;
+        C2<T> *orow = a.out[c] + ((long)z * a.Py + y) * a.Hp;
+  #pragma unroll
+        for (int k1 = 0; k1 < A; k1++) {
+          // partner element: A-1-k1 of lane B-l, or (A-k1) mod A of lane 0 itself
+          const C2<T> q = u[h][A - 1 - k1];
+          C2<T> Zp{__shfl(q.x, src_lane, 64), __shfl(q.y, src_lane, 64)};
+          if (l == 0) Zp = u[h][(A - k1) & (A - 1)];
+          const C2<T> Zk = u[h][k1];
+          const C2<T> Bc = cconj(Zp);
+          const C2<T> E = C2<T>{T(0.5) * (Zk.x + Bc.x), T(0.5) * (Zk.y + Bc.y)};
+          const C2<T> O = C2<T>{T(0.5) * (Zk.y - Bc.y), T(-0.5) * (Zk.x - Bc.x)};
+          const int k = l + B * k1;
+          const C2<T> t = cmul(a.tw2[k], O);
+          if (rr < a.rows) {
+            orow[k] = cadd(E, t);
+            if (k == 0) orow[N] = cconj(csub(E, t));
+          }
         }
       }
     }
   }
-}
+};
+template <typename T, int N>
+__global__ __launch_bounds__(XF4_NT) void k_xf4(XArgs<T> a) { BXf4<T, N>::run(a, hw_bid(), hw_grid()); }
 
 // ---------------------------------------------------------------------------
 // Column pass along y (or z): nl adjacent kx columns per work-group.
@@ -508,47 +545,53 @@ struct ColArgs {
 };
 
 template <typename T, int DIR, int CN, int CNL>
-__global__ __launch_bounds__(FFT_NT) void k_colfft(ColArgs<T> a) {
-  constexpr int EPT = FftEpt<T>::value;
-  extern __shared__ __align__(16) unsigned char smem_[];
-  C2<T> *lds = (C2<T> *)smem_;
-  const int n = CN > 0 ? CN : a.f.n, nl = CN > 0 ? CNL : a.nl, total = nl * n;
-  C2<T> *twl = lds + nl * n;
-  if constexpr (CN > 0) {
-    const C2<T> *tw = (const C2<T> *)a.f.tw;
-    for (int g = threadIdx.x; g < CN; g += FFT_NT) twl[g] = tw[g];
-  }
-  const int kx0 = blockIdx.x * nl;
-  C2<T> *base = a.buf[blockIdx.z] + (long)blockIdx.y * a.pstride;
-  constexpr int SL = CN > 0 ? (CN * CNL + FFT_NT - 1) / FFT_NT : EPT;
-  C2<T> v[SL];
-#pragma unroll
-  for (int s = 0; s < SL; s++) {
-    const int g = threadIdx.x + s * FFT_NT;
-    v[s] = C2<T>{T(0), T(0)};
-    if (g < total) {
-      const int e = cdiv<CNL>(g, a.fnl), l = g - e * nl;
-      const int kx = kx0 + l;
-      if (kx < a.H && e < a.nvalid) v[s] = base[(long)e * a.estride + kx];
+struct BColfft {
+  using Args = ColArgs<T>;
+  static constexpr int NT = FFT_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const ColArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    constexpr int EPT = FftEpt<T>::value;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C2<T> *lds = (C2<T> *)smem_;
+    const int n = CN > 0 ? CN : a.f.n, nl = CN > 0 ? CNL : a.nl, total = nl * n;
+    C2<T> *twl = lds + nl * n;
+    if constexpr (CN > 0) {
+      const C2<T> *tw = (const C2<T> *)a.f.tw;
+      for (int g = threadIdx.x; g < CN; g += FFT_NT) twl[g] = tw[g];
+    }
+    const int kx0 = blockIdx.x * nl;
+    C2<T> *base = a.buf[blockIdx.z] + (long)blockIdx.y * a.pstride;
+    constexpr int SL = CN > 0 ? (CN * CNL + FFT_NT - 1) / FFT_NT : EPT;
+    C2<T> v[SL];
+  #pragma unroll
+    for (int s = 0; s < SL; s++) {
+      const int g = threadIdx.x + s * FFT_NT;
+      v[s] = C2<T>{T(0), T(0)};
+      if (g < total) {
+        const int e = cdiv<CNL>(g, a.fnl), l = g - e * nl;
+        const int kx = kx0 + l;
+        if (kx < a.H && e < a.nvalid) v[s] = base[(long)e * a.estride + kx];
+      }
+    }
+  #pragma unroll
+    for (int s = 0; s < SL; s++) {
+      const int g = threadIdx.x + s * FFT_NT;
+      if (g < total) lds[g] = v[s];   // element-major: g = e*nl + l
+    }
+    __syncthreads();
+    run_fft<T, CN, CNL, DIR, 1, 1>(lds, twl, nl, a.fnl, a.f);
+  #pragma unroll
+    for (int s = 0; s < SL; s++) {
+      const int g = threadIdx.x + s * FFT_NT;
+      if (g < total) {
+        const int e = cdiv<CNL>(g, a.fnl), l = g - e * nl;
+        const int kx = kx0 + l;
+        if (kx < a.H && e < a.nout) base[(long)e * a.estride + kx] = lds[g];
+      }
     }
   }
-#pragma unroll
-  for (int s = 0; s < SL; s++) {
-    const int g = threadIdx.x + s * FFT_NT;
-    if (g < total) lds[g] = v[s];   // element-major: g = e*nl + l
-  }
-  __syncthreads();
-  run_fft<T, CN, CNL, DIR, 1, 1>(lds, twl, nl, a.fnl, a.f);
-#pragma unroll
-  for (int s = 0; s < SL; s++) {
-    const int g = threadIdx.x + s * FFT_NT;
-    if (g < total) {
-      const int e = cdiv<CNL>(g, a.fnl), l = g - e * nl;
-      const int kx = kx0 + l;
-      if (kx < a.H && e < a.nout) base[(long)e * a.estride + kx] = lds[g];
-    }
-  }
-}
+};
+template <typename T, int DIR, int CN, int CNL>
+__global__ __launch_bounds__(FFT_NT) void k_colfft(ColArgs<T> a) { BColfft<T, DIR, CN, CNL>::run(a, hw_bid(), hw_grid()); }
 
 // ---------------------------------------------------------------------------
 // Register four-step column pass for P = A*A (A in 8, 16, 32): C adjacent kx
@@ -580,92 +623,97 @@ __device__ __forceinline__ int xcd_tile(int w, int total) {
 }
 
 template <typename T, int A, int DIR, int C, bool HALF, int ZB>
-__global__ __launch_bounds__(A * C, (HALF && sizeof(C2<T>) * A * (A * C + C) > 65536) ? 4 : 1) void k_coly(
-    ColArgs<T> a) {
-  // transpose row stride A*C + C: a wave's 64 b64 accesses then share each
-  // LDS bank pair exactly twice (the minimum for 8-byte lanes)
-  constexpr int P = A * A, S = A * C + C;
-  // SPLIT: a transpose of A x (A*C) complex values would hold 135 KB of LDS
-  // at A = 32 (one work-group per CU); exchanging the real and then the
-  // imaginary parts halves it, so two work-groups (4 waves per SIMD, hence
-  // the launch bound of the half-column passes; the full-column inverse
-  // would spill under it) overlap their load and arithmetic phases, without
-  // extra registers (each half is read back into the slots just written out)
-  constexpr bool SPLIT = sizeof(C2<T>) * A * S > 65536;
-  extern __shared__ __align__(16) unsigned char smem_[];
-  C2<T> *lds = (C2<T> *)smem_;
-  T *ldr = (T *)smem_;
-  const int ngx = gridDim.x, ngy = gridDim.y;
-  const int w = xcd_tile(blockIdx.x + ngx * (blockIdx.y + ngy * blockIdx.z), ngx * ngy * gridDim.z);
-  const int gx = w % ngx, gyz = w / ngx, gy = gyz % ngy, gz = gyz / ngy;
-  const int col = threadIdx.x % C, i = threadIdx.x / C;
-  const int kx = gx * C + col;
-  const bool valid = kx < a.H;
-  // uniform plane base + one 32-bit lane offset (row i, column kx)
-  const uint32_t loff = (uint32_t)(((long)i * a.estride + min(kx, a.H - 1)) * sizeof(C2<T>));
-  const long rstep = (long)A * a.estride;   // rows n2 -> n2 + 1
-  const C2<T> *tw = (const C2<T> *)a.f.tw;   // W_P^m, m < P
-  constexpr int NIN = (HALF && DIR < 0) ? A / 2 : A;
-  constexpr int NOUT = (HALF && DIR > 0) ? A / 2 : A;
+struct BColy {
+  using Args = ColArgs<T>;
+  static constexpr int NT = A * C, MINB = (HALF && sizeof(C2<T>) * A * (A * C + C) > 65536) ? 4 : 1;
+  static __device__ __forceinline__ void run(const ColArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    // transpose row stride A*C + C: a wave's 64 b64 accesses then share each
+    // LDS bank pair exactly twice (the minimum for 8-byte lanes)
+    constexpr int P = A * A, S = A * C + C;
+    // SPLIT: a transpose of A x (A*C) complex values would hold 135 KB of LDS
+    // at A = 32 (one work-group per CU); exchanging the real and then the
+    // imaginary parts halves it, so two work-groups (4 waves per SIMD, hence
+    // the launch bound of the half-column passes; the full-column inverse
+    // would spill under it) overlap their load and arithmetic phases, without
+    // extra registers (each half is read back into the slots just written out)
+    constexpr bool SPLIT = sizeof(C2<T>) * A * S > 65536;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C2<T> *lds = (C2<T> *)smem_;
+    T *ldr = (T *)smem_;
+    const int ngx = gridDim.x, ngy = gridDim.y;
+    const int w = xcd_tile(blockIdx.x + ngx * (blockIdx.y + ngy * blockIdx.z), ngx * ngy * gridDim.z);
+    const int gx = w % ngx, gyz = w / ngx, gy = gyz % ngy, gz = gyz / ngy;
+    const int col = threadIdx.x % C, i = threadIdx.x / C;
+    const int kx = gx * C + col;
+    const bool valid = kx < a.H;
+    // uniform plane base + one 32-bit lane offset (row i, column kx)
+    const uint32_t loff = (uint32_t)(((long)i * a.estride + min(kx, a.H - 1)) * sizeof(C2<T>));
+    const long rstep = (long)A * a.estride;   // rows n2 -> n2 + 1
+    const C2<T> *tw = (const C2<T> *)a.f.tw;   // W_P^m, m < P
+    constexpr int NIN = (HALF && DIR < 0) ? A / 2 : A;
+    constexpr int NOUT = (HALF && DIR > 0) ? A / 2 : A;
 
-  // ZB planes per work-group; the loads of plane p+1 are issued before plane p
-  // is transformed, so they overlap its arithmetic and LDS exchange
-  C2<T> nxt[NIN];
-  auto load = [&](int plane) {
-    C2<T> *ub = a.buf[gz] + (long)plane * a.pstride;
-#pragma unroll
-    for (int n2 = 0; n2 < NIN; n2++) {
-      const auto *q = at_off<const T>(ub + n2 * rstep, loff);
-      nxt[n2] = C2<T>{q[0], q[1]};
-    }
-  };
-  load(gy * ZB);
-#pragma unroll
-  for (int p = 0; p < ZB; p++) {
-    C2<T> v[A];
-#pragma unroll
-    for (int n2 = 0; n2 < A; n2++) v[n2] = n2 < NIN ? nxt[n2] : C2<T>{T(0), T(0)};
-    if (p + 1 < ZB) load(gy * ZB + p + 1);
-    dft<A, DIR>(v);
-    pin(v);
-#pragma unroll
-    for (int k2 = 1; k2 < A; k2++) {
-      C2<T> t = tw[i * k2];
-      if (DIR > 0) t.y = -t.y;
-      v[k2] = cmul(v[k2], t);
-    }
-    if (p > 0) __syncthreads();   // the previous plane's transpose reads are done
-    if constexpr (SPLIT) {
-#pragma unroll
-      for (int k2 = 0; k2 < A; k2++) ldr[k2 * S + threadIdx.x] = v[k2].x;
-      __syncthreads();
-#pragma unroll
-      for (int n1 = 0; n1 < A; n1++) v[n1].x = ldr[i * S + n1 * C + col];
-      __syncthreads();
-#pragma unroll
-      for (int k2 = 0; k2 < A; k2++) ldr[k2 * S + threadIdx.x] = v[k2].y;
-      __syncthreads();
-#pragma unroll
-      for (int n1 = 0; n1 < A; n1++) v[n1].y = ldr[i * S + n1 * C + col];
-    } else {
-#pragma unroll
-      for (int k2 = 0; k2 < A; k2++) lds[k2 * S + threadIdx.x] = v[k2];
-      __syncthreads();
-#pragma unroll
-      for (int n1 = 0; n1 < A; n1++) v[n1] = lds[i * S + n1 * C + col];
-    }
-    dft<A, DIR>(v);
-    if (valid) {
-      C2<T> *ub = a.buf[gz] + (long)(gy * ZB + p) * a.pstride;
-#pragma unroll
-      for (int k1 = 0; k1 < NOUT; k1++) {
-        auto *q = at_off<T>(ub + k1 * rstep, loff);
-        q[0] = v[k1].x;
-        q[1] = v[k1].y;
+    // ZB planes per work-group; the loads of plane p+1 are issued before plane p
+    // is transformed, so they overlap its arithmetic and LDS exchange
+    C2<T> nxt[NIN];
+    auto load = [&](int plane) {
+      C2<T> *ub = a.buf[gz] + (long)plane * a.pstride;
+  #pragma unroll
+      for (int n2 = 0; n2 < NIN; n2++) {
+        const auto *q = at_off<const T>(ub + n2 * rstep, loff);
+        nxt[n2] = C2<T>{q[0], q[1]};
+      }
+    };
+    load(gy * ZB);
+  #pragma unroll
+    for (int p = 0; p < ZB; p++) {
+      C2<T> v[A];
+  #pragma unroll
+      for (int n2 = 0; n2 < A; n2++) v[n2] = n2 < NIN ? nxt[n2] : C2<T>{T(0), T(0)};
+      if (p + 1 < ZB) load(gy * ZB + p + 1);
+      dft<A, DIR>(v);
+      pin(v);
+  #pragma unroll
+      for (int k2 = 1; k2 < A; k2++) {
+        C2<T> t = tw[i * k2];
+        if (DIR > 0) t.y = -t.y;
+        v[k2] = cmul(v[k2], t);
+      }
+      if (p > 0) __syncthreads();   // the previous plane's transpose reads are done
+      if constexpr (SPLIT) {
+  #pragma unroll
+        for (int k2 = 0; k2 < A; k2++) ldr[k2 * S + threadIdx.x] = v[k2].x;
+        __syncthreads();
+  #pragma unroll
+        for (int n1 = 0; n1 < A; n1++) v[n1].x = ldr[i * S + n1 * C + col];
+        __syncthreads();
+  #pragma unroll
+        for (int k2 = 0; k2 < A; k2++) ldr[k2 * S + threadIdx.x] = v[k2].y;
+        __syncthreads();
+  #pragma unroll
+        for (int n1 = 0; n1 < A; n1++) v[n1].y = ldr[i * S + n1 * C + col];
+      } else {
+  #pragma unroll
+        for (int k2 = 0; k2 < A; k2++) lds[k2 * S + threadIdx.x] = v[k2];
+        __syncthreads();
+  #pragma unroll
+        for (int n1 = 0; n1 < A; n1++) v[n1] = lds[i * S + n1 * C + col];
+      }
+      dft<A, DIR>(v);
+      if (valid) {
+        C2<T> *ub = a.buf[gz] + (long)(gy * ZB + p) * a.pstride;
+  #pragma unroll
+        for (int k1 = 0; k1 < NOUT; k1++) {
+          auto *q = at_off<T>(ub + k1 * rstep, loff);
+          q[0] = v[k1].x;
+          q[1] = v[k1].y;
+        }
       }
     }
   }
-}
+};
+template <typename T, int A, int DIR, int C, bool HALF, int ZB>
+__global__ __launch_bounds__(A * C, (HALF && sizeof(C2<T>) * A * (A * C + C) > 65536) ? 4 : 1) void k_coly(ColArgs<T> a) { BColy<T, A, DIR, C, HALF, ZB>::run(a, hw_bid(), hw_grid()); }
 
 // ---------------------------------------------------------------------------
 // Fused convolution pass along the last axis: forward FFT of the three
@@ -695,162 +743,168 @@ constexpr int CONV_QK = 6;   // tensor prefetch slots per thread: B*(n/2+1) <= F
 constexpr long SMALL_GRID_WG = 768;
 
 template <typename T, int CN, int CB>
-__global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) {
-  constexpr int EPT = FftEpt<T>::value;
-  extern __shared__ __align__(16) unsigned char smem_[];
-  C2<T> *lds = (C2<T> *)smem_;
-  const int n = CN > 0 ? CN : a.f.n, B = CN > 0 ? CB : a.B, nl = 3 * B, total = nl * n;
-  C2<T> *twl = lds + nl * n;
-  if constexpr (CN > 0) {
-    const C2<T> *tw = (const C2<T> *)a.f.tw;
-    for (int g = threadIdx.x; g < CN; g += FFT_NT) twl[g] = tw[g];
-  }
-  const int kx0 = blockIdx.x * B;
-  // 3D: this work-group handles ky and its mirror Py-ky, which use the same
-  // tensor values (kernmulrsymm3d.cl:62-66 flips the yz and xy signs); 2D: one tile
-  const int kya = blockIdx.y;
-  const int kyb = a.Py - kya;
-  const int ntiles = (!a.is2d && kya != 0 && kyb != kya) ? 2 : 1;
-  const int nh = n / 2 + 1;   // unique points along the axis: k and n-k share |K|
-  // tensor values of frequency e, column kx (2D: yz and xz are not used)
-  auto tensor = [&](int e, int kx, T *k) {
-#pragma unroll
-    for (int j = 0; j < 6; j++) k[j] = T(0);
-    if (!a.is2d) {
-      const long I = ((long)e * a.KHy + kya) * a.Hk + kx;
-      k[0] = a.K[0][I]; k[1] = a.K[1][I]; k[2] = a.K[2][I];
-      k[3] = a.K[3][I]; k[4] = a.K[4][I]; k[5] = a.K[5][I];
-    } else {
-      const long I = (long)e * a.Hk + kx;
-      k[0] = a.K[0][I]; k[1] = a.K[1][I]; k[2] = a.K[2][I]; k[5] = a.K[5][I];
-    }
-  };
-  // 1. tensor prefetch (independent of the data, so it overlaps the loads
-  // below); the run-time-length kernel reads it at the multiply instead, to
-  // keep its register count down
-  constexpr int QK = CN > 0 ? (CB * (CN / 2 + 1) + FFT_NT - 1) / FFT_NT : 1;
-  T kv[QK][6];
-  if constexpr (CN > 0) {
-#pragma unroll
-    for (int q = 0; q < QK; q++) {
-      const int g = threadIdx.x + q * FFT_NT;
-#pragma unroll
-      for (int j = 0; j < 6; j++) kv[q][j] = T(0);
-      if (g < B * nh) {
-        const int e = cdiv<CB>(g, a.fB), b = g - e * B;
-        if (kx0 + b < a.H) tensor(e, kx0 + b, kv[q]);
-      }
-    }
-  }
-  // 2. spectrum loads of a tile
-  // (fixed-length kernels: every load of the tile in flight at once; the
-  // run-time-length kernel streams them to keep its register count down)
-  constexpr int SL = CN > 0 ? (3 * CB * CN + FFT_NT - 1) / FFT_NT : 1;
-  (void)EPT;
-  C2<T> v[SL];
-  auto load = [&](int ky) {
-    const long off = (long)ky * a.pstride;
-#pragma unroll
-    for (int s = 0; s < SL; s++) {
-      const int g = threadIdx.x + s * FFT_NT;
-      v[s] = C2<T>{T(0), T(0)};
-      if (g < total) {
-        const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
-        const int c = cdiv<CB>(l, a.fB), b = l - c * B;
-        const int kx = kx0 + b;
-        if (kx < a.H && e < a.nvalid) v[s] = a.buf[c][off + (long)e * a.estride + kx];
-      }
-    }
-  };
-  auto mul = [&](C2<T> *p, T xx, T yy, T zz, T yz, T xz, T xy) {
-    const C2<T> M0 = p[0], M1 = p[B], M2 = p[2 * B];
-    if (!a.is2d) {
-      p[0] = C2<T>{M0.x * xx + M1.x * xy + M2.x * xz, M0.y * xx + M1.y * xy + M2.y * xz};
-      p[B] = C2<T>{M0.x * xy + M1.x * yy + M2.x * yz, M0.y * xy + M1.y * yy + M2.y * yz};
-      p[2 * B] = C2<T>{M0.x * xz + M1.x * yz + M2.x * zz, M0.y * xz + M1.y * yz + M2.y * zz};
-    } else {
-      p[0] = C2<T>{M0.x * xx + M1.x * xy, M0.y * xx + M1.y * xy};
-      p[B] = C2<T>{M0.x * xy + M1.x * yy, M0.y * xy + M1.y * yy};
-      p[2 * B] = C2<T>{M2.x * zz, M2.y * zz};
-    }
-  };
-  for (int t = 0; t < ntiles; t++) {
-    const int ky = (t == 0) ? kya : kyb;
+struct BConv {
+  using Args = ConvArgs<T>;
+  static constexpr int NT = FFT_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const ConvArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    constexpr int EPT = FftEpt<T>::value;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C2<T> *lds = (C2<T> *)smem_;
+    const int n = CN > 0 ? CN : a.f.n, B = CN > 0 ? CB : a.B, nl = 3 * B, total = nl * n;
+    C2<T> *twl = lds + nl * n;
     if constexpr (CN > 0) {
-      load(ky);
-#pragma unroll
-      for (int s = 0; s < SL; s++) {
-        const int g = threadIdx.x + s * FFT_NT;
-        if (g < total) lds[g] = v[s];
-      }
-    } else {
-      const long off = (long)ky * a.pstride;
-#pragma unroll 4
-      for (int g = threadIdx.x; g < total; g += FFT_NT) {
-        const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
-        const int c = cdiv<CB>(l, a.fB), b = l - c * B;
-        const int kx = kx0 + b;
-        C2<T> w{T(0), T(0)};
-        if (kx < a.H && e < a.nvalid) w = a.buf[c][off + (long)e * a.estride + kx];
-        lds[g] = w;
-      }
+      const C2<T> *tw = (const C2<T> *)a.f.tw;
+      for (int g = threadIdx.x; g < CN; g += FFT_NT) twl[g] = tw[g];
     }
-    __syncthreads();
-    run_fft<T, CN, 3 * CB, -1, 1, 1>(lds, twl, nl, a.f3B, a.f);
-    // 3. tensor multiply at k and at its mirror n-k (signs of the odd components flip)
-    const T sy = (t == 0) ? T(1) : T(-1);   // ky mirror: yz and xy change sign
-    auto multiply = [&](int e, int b, const T *k) {
-      const T yz = sy * k[3], xy = sy * k[5];
-      mul(lds + e * nl + b, k[0], k[1], k[2], yz, k[4], xy);
-      const int e2 = n - e;
-      if (e != 0 && e2 != e) {
-        // 3D mirror in z flips yz and xz (kernmulrsymm3d.cl:67-71); 2D mirror in y flips xy
-        if (!a.is2d) mul(lds + e2 * nl + b, k[0], k[1], k[2], -yz, -k[4], xy);
-        else mul(lds + e2 * nl + b, k[0], k[1], k[2], k[3], k[4], -k[5]);
+    const int kx0 = blockIdx.x * B;
+    // 3D: this work-group handles ky and its mirror Py-ky, which use the same
+    // tensor values (kernmulrsymm3d.cl:62-66 flips the yz and xy signs); 2D: one tile
+    const int kya = blockIdx.y;
+    const int kyb = a.Py - kya;
+    const int ntiles = (!a.is2d && kya != 0 && kyb != kya) ? 2 : 1;
+    const int nh = n / 2 + 1;   // unique points along the axis: k and n-k share |K|
+    // tensor values of frequency e, column kx (2D: yz and xz are not used)
+    auto tensor = [&](int e, int kx, T *k) {
+  #pragma unroll
+      for (int j = 0; j < 6; j++) k[j] = T(0);
+      if (!a.is2d) {
+        const long I = ((long)e * a.KHy + kya) * a.Hk + kx;
+        k[0] = a.K[0][I]; k[1] = a.K[1][I]; k[2] = a.K[2][I];
+        k[3] = a.K[3][I]; k[4] = a.K[4][I]; k[5] = a.K[5][I];
+      } else {
+        const long I = (long)e * a.Hk + kx;
+        k[0] = a.K[0][I]; k[1] = a.K[1][I]; k[2] = a.K[2][I]; k[5] = a.K[5][I];
       }
     };
+    // 1. tensor prefetch (independent of the data, so it overlaps the loads
+    // below); the run-time-length kernel reads it at the multiply instead, to
+    // keep its register count down
+    constexpr int QK = CN > 0 ? (CB * (CN / 2 + 1) + FFT_NT - 1) / FFT_NT : 1;
+    T kv[QK][6];
     if constexpr (CN > 0) {
-#pragma unroll
+  #pragma unroll
       for (int q = 0; q < QK; q++) {
         const int g = threadIdx.x + q * FFT_NT;
+  #pragma unroll
+        for (int j = 0; j < 6; j++) kv[q][j] = T(0);
         if (g < B * nh) {
           const int e = cdiv<CB>(g, a.fB), b = g - e * B;
-          if (kx0 + b < a.H) multiply(e, b, kv[q]);
-        }
-      }
-    } else {
-#pragma unroll 1
-      for (int g = threadIdx.x; g < B * nh; g += FFT_NT) {
-        const int e = cdiv<CB>(g, a.fB), b = g - e * B;
-        if (kx0 + b < a.H) {
-          T k[6];
-          tensor(e, kx0 + b, k);
-          multiply(e, b, k);
+          if (kx0 + b < a.H) tensor(e, kx0 + b, kv[q]);
         }
       }
     }
-    __syncthreads();
-    run_fft<T, CN, 3 * CB, +1, 1, 1>(lds, twl, nl, a.f3B, a.f);
-    const long off = (long)ky * a.pstride;
-    auto store = [&](int g) {
-      const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
-      const int c = cdiv<CB>(l, a.fB), b = l - c * B;
-      const int kx = kx0 + b;
-      if (kx < a.H && e < a.nvalid) a.buf[c][off + (long)e * a.estride + kx] = lds[g];
-    };
-    if constexpr (CN > 0) {
-#pragma unroll
+    // 2. spectrum loads of a tile
+    // (fixed-length kernels: every load of the tile in flight at once; the
+    // run-time-length kernel streams them to keep its register count down)
+    constexpr int SL = CN > 0 ? (3 * CB * CN + FFT_NT - 1) / FFT_NT : 1;
+    (void)EPT;
+    C2<T> v[SL];
+    auto load = [&](int ky) {
+      const long off = (long)ky * a.pstride;
+  #pragma unroll
       for (int s = 0; s < SL; s++) {
         const int g = threadIdx.x + s * FFT_NT;
-        if (g < total) store(g);
+        v[s] = C2<T>{T(0), T(0)};
+        if (g < total) {
+          const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
+          const int c = cdiv<CB>(l, a.fB), b = l - c * B;
+          const int kx = kx0 + b;
+          if (kx < a.H && e < a.nvalid) v[s] = a.buf[c][off + (long)e * a.estride + kx];
+        }
       }
-    } else {
-#pragma unroll 4
-      for (int g = threadIdx.x; g < total; g += FFT_NT) store(g);
+    };
+    auto mul = [&](C2<T> *p, T xx, T yy, T zz, T yz, T xz, T xy) {
+      const C2<T> M0 = p[0], M1 = p[B], M2 = p[2 * B];
+      if (!a.is2d) {
+        p[0] = C2<T>{M0.x * xx + M1.x * xy + M2.x * xz, M0.y * xx + M1.y * xy + M2.y * xz};
+        p[B] = C2<T>{M0.x * xy + M1.x * yy + M2.x * yz, M0.y * xy + M1.y * yy + M2.y * yz};
+        p[2 * B] = C2<T>{M0.x * xz + M1.x * yz + M2.x * zz, M0.y * xz + M1.y * yz + M2.y * zz};
+      } else {
+        p[0] = C2<T>{M0.x * xx + M1.x * xy, M0.y * xx + M1.y * xy};
+        p[B] = C2<T>{M0.x * xy + M1.x * yy, M0.y * xy + M1.y * yy};
+        p[2 * B] = C2<T>{M2.x * zz, M2.y * zz};
+      }
+    };
+    for (int t = 0; t < ntiles; t++) {
+      const int ky = (t == 0) ? kya : kyb;
+      if constexpr (CN > 0) {
+        load(ky);
+  #pragma unroll
+        for (int s = 0; s < SL; s++) {
+          const int g = threadIdx.x + s * FFT_NT;
+          if (g < total) lds[g] = v[s];
+        }
+      } else {
+        const long off = (long)ky * a.pstride;
+  #pragma unroll 4
+        for (int g = threadIdx.x; g < total; g += FFT_NT) {
+          const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
+          const int c = cdiv<CB>(l, a.fB), b = l - c * B;
+          const int kx = kx0 + b;
+          C2<T> w{T(0), T(0)};
+          if (kx < a.H && e < a.nvalid) w = a.buf[c][off + (long)e * a.estride + kx];
+          lds[g] = w;
+        }
+      }
+      __syncthreads();
+      run_fft<T, CN, 3 * CB, -1, 1, 1>(lds, twl, nl, a.f3B, a.f);
+      // 3. tensor multiply at k and at its mirror n-k (signs of the odd components flip)
+      const T sy = (t == 0) ? T(1) : T(-1);   // ky mirror: yz and xy change sign
+      auto multiply = [&](int e, int b, const T *k) {
+        const T yz = sy * k[3], xy = sy * k[5];
+        mul(lds + e * nl + b, k[0], k[1], k[2], yz, k[4], xy);
+        const int e2 = n - e;
+        if (e != 0 && e2 != e) {
+          // 3D mirror in z flips yz and xz (kernmulrsymm3d.cl:67-71); 2D mirror in y flips xy
+          if (!a.is2d) mul(lds + e2 * nl + b, k[0], k[1], k[2], -yz, -k[4], xy);
+          else mul(lds + e2 * nl + b, k[0], k[1], k[2], k[3], k[4], -k[5]);
+        }
+      };
+      if constexpr (CN > 0) {
+  #pragma unroll
+        for (int q = 0; q < QK; q++) {
+          const int g = threadIdx.x + q * FFT_NT;
+          if (g < B * nh) {
+            const int e = cdiv<CB>(g, a.fB), b = g - e * B;
+            if (kx0 + b < a.H) multiply(e, b, kv[q]);
+          }
+        }
+      } else {
+  #pragma unroll 1
+        for (int g = threadIdx.x; g < B * nh; g += FFT_NT) {
+          const int e = cdiv<CB>(g, a.fB), b = g - e * B;
+          if (kx0 + b < a.H) {
+            T k[6];
+            tensor(e, kx0 + b, k);
+            multiply(e, b, k);
+          }
+        }
+      }
+      __syncthreads();
+      run_fft<T, CN, 3 * CB, +1, 1, 1>(lds, twl, nl, a.f3B, a.f);
+      const long off = (long)ky * a.pstride;
+      auto store = [&](int g) {
+        const int e = cdiv<3 * CB>(g, a.f3B), l = g - e * nl;
+        const int c = cdiv<CB>(l, a.fB), b = l - c * B;
+        const int kx = kx0 + b;
+        if (kx < a.H && e < a.nvalid) a.buf[c][off + (long)e * a.estride + kx] = lds[g];
+      };
+      if constexpr (CN > 0) {
+  #pragma unroll
+        for (int s = 0; s < SL; s++) {
+          const int g = threadIdx.x + s * FFT_NT;
+          if (g < total) store(g);
+        }
+      } else {
+  #pragma unroll 4
+        for (int g = threadIdx.x; g < total; g += FFT_NT) store(g);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
-}
+};
+template <typename T, int CN, int CB>
+__global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) { BConv<T, CN, CB>::run(a, hw_bid(), hw_grid()); }
 
 // ---------------------------------------------------------------------------
 // Register Z pass (3D with Pz = 2 Nz, Nz in {8, 16, 32}): one lane per
@@ -894,128 +948,134 @@ __device__ __forceinline__ constexpr int zorder(int i) {
 }
 
 template <typename T, int NZ, int FR, int G>
-__global__ __launch_bounds__(CONVZ_NT) void k_convz(ConvZArgs<T> a) {
-  constexpr int PZ = 2 * NZ, KZ = NZ + 1;
-  static_assert(PZ % FR == 0 && FR % G == 0, "rounds must tile the spectrum");
-  __shared__ C2<T> xch[FR * 3 * 64];             // exchange: [freq][component][lane]
-  __shared__ T tens[KZ * 6 * CONVZ_COLS];        // tensor slab: [kz][element][kx]
-  const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int l = threadIdx.x & 63;
-  const int t = l >> 5;
-  const int kx0 = blockIdx.x * CONVZ_COLS;
-  const int kx = kx0 + (l & 31);
-  const int kya = blockIdx.y;
-  const bool mirror = kya != 0 && 2 * kya != a.Py;
-  const bool valid = kx < a.H && (t == 0 || mirror);
-  // every lane loads in-bounds data (clamped column / row); only valid lanes store
-  const int kxc = min(kx, a.H - 1);
-  const int ky = (t && mirror) ? a.Py - kya : kya;
-  // Addresses are a wave-uniform base (plane n, scalar registers) plus one
-  // per-lane 32-bit byte offset shared by every load and store of the lane.
-  C2<T> *plane0 = a.buf[c];
-  const uint32_t loff = (uint32_t)(((long)ky * a.ystride + kxc) * sizeof(C2<T>));
+struct BConvz {
+  using Args = ConvZArgs<T>;
+  static constexpr int NT = CONVZ_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const ConvZArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    constexpr int PZ = 2 * NZ, KZ = NZ + 1;
+    static_assert(PZ % FR == 0 && FR % G == 0, "rounds must tile the spectrum");
+    __shared__ C2<T> xch[FR * 3 * 64];             // exchange: [freq][component][lane]
+    __shared__ T tens[KZ * 6 * CONVZ_COLS];        // tensor slab: [kz][element][kx]
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    const int t = l >> 5;
+    const int kx0 = blockIdx.x * CONVZ_COLS;
+    const int kx = kx0 + (l & 31);
+    const int kya = blockIdx.y;
+    const bool mirror = kya != 0 && 2 * kya != a.Py;
+    const bool valid = kx < a.H && (t == 0 || mirror);
+    // every lane loads in-bounds data (clamped column / row); only valid lanes store
+    const int kxc = min(kx, a.H - 1);
+    const int ky = (t && mirror) ? a.Py - kya : kya;
+    // Addresses are a wave-uniform base (plane n, scalar registers) plus one
+    // per-lane 32-bit byte offset shared by every load and store of the lane.
+    C2<T> *plane0 = a.buf[c];
+    const uint32_t loff = (uint32_t)(((long)ky * a.ystride + kxc) * sizeof(C2<T>));
 
-  // 0. tensor slab of this work-group (32 kx x 6 elements x Nz+1 kz) into LDS
-  // (fixed trip count, unrolled: every load of the slab is in flight at once)
-  {
-    constexpr int TS = KZ * 6 * CONVZ_COLS, TQ = (TS + CONVZ_NT - 1) / CONVZ_NT;
-    const long kzs = (long)a.KHy * a.Hk;
-    const int x = min(kx0 + (int)(threadIdx.x & (CONVZ_COLS - 1)), a.H - 1);
-    T tv[TQ];
-#pragma unroll
-    for (int q = 0; q < TQ; q++) {
-      const int g = threadIdx.x + q * CONVZ_NT;
-      if (TS % CONVZ_NT == 0 || g < TS) {
-        const int ez = g / CONVZ_COLS, e = ez % 6, kz = ez / 6;
-        tv[q] = a.K[e][kz * kzs + (long)kya * a.Hk + x];
+    // 0. tensor slab of this work-group (32 kx x 6 elements x Nz+1 kz) into LDS
+    // (fixed trip count, unrolled: every load of the slab is in flight at once)
+    {
+      constexpr int TS = KZ * 6 * CONVZ_COLS, TQ = (TS + CONVZ_NT - 1) / CONVZ_NT;
+      const long kzs = (long)a.KHy * a.Hk;
+      const int x = min(kx0 + (int)(threadIdx.x & (CONVZ_COLS - 1)), a.H - 1);
+      T tv[TQ];
+  #pragma unroll
+      for (int q = 0; q < TQ; q++) {
+        const int g = threadIdx.x + q * CONVZ_NT;
+        if (TS % CONVZ_NT == 0 || g < TS) {
+          const int ez = g / CONVZ_COLS, e = ez % 6, kz = ez / 6;
+          tv[q] = a.K[e][kz * kzs + (long)kya * a.Hk + x];
+        }
+      }
+  #pragma unroll
+      for (int q = 0; q < TQ; q++) {
+        const int g = threadIdx.x + q * CONVZ_NT;
+        if (TS % CONVZ_NT == 0 || g < TS) tens[g] = tv[q];
       }
     }
-#pragma unroll
-    for (int q = 0; q < TQ; q++) {
-      const int g = threadIdx.x + q * CONVZ_NT;
-      if (TS % CONVZ_NT == 0 || g < TS) tens[g] = tv[q];
-    }
-  }
 
-  // 1. forward: even and odd halves of the spectrum
-  C2<T> E[NZ], O[NZ];
-#pragma unroll
-  for (int n = 0; n < NZ; n++) {
-    const auto *p = at_off<const T>(plane0 + n * a.zstride, loff);
-    E[n] = C2<T>{p[0], p[1]};
-  }
-#pragma unroll
-  for (int n = 0; n < NZ; n++) O[n] = n == 0 ? E[0] : cmul(E[n], cexpi<T>(n, PZ, -1));
-  // one transform at a time: interleaving the two raises the register peak
-  dft<NZ, -1>(E);
-  pin(E);
-  dft<NZ, -1>(O);
-  pin(O);
-
-  // 2. tensor multiply: component c of K . (X0, X1, X2) for each frequency.
-  // Row c of the symmetric tensor: x: (xx, xy, xz), y: (xy, yy, yz), z: (xz, yz, zz).
-  const int e0 = c == 0 ? 0 : c == 1 ? 5 : 4;
-  const int e1 = c == 0 ? 5 : c == 1 ? 1 : 3;
-  const int e2 = c == 0 ? 4 : c == 1 ? 3 : 2;
-  // the ky mirror flips xy and yz; the kz mirror flips xz and yz
-  const T sy = t ? T(-1) : T(1);
-  auto fy = [&](int e) { return (e == 5 || e == 3) ? sy : T(1); };
-  auto fz = [&](int e) { return (e == 4 || e == 3) ? T(-1) : T(1); };
-  const T y0 = fy(e0), y1 = fy(e1), y2 = fy(e2);
-  const T z0 = y0 * fz(e0), z1 = y1 * fz(e1), z2 = y2 * fz(e2);
-  const T *tl = tens + (l & (CONVZ_COLS - 1));
-  C2<T> *xw = xch + c * 64 + l;
-#pragma unroll
-  for (int r = 0; r < PZ / FR; r++) {
-#pragma unroll
-    for (int j = 0; j < FR; j++) {
-      const int k = zorder<NZ>(r * FR + j);
-      xw[j * 192] = (k & 1) ? O[k >> 1] : E[k >> 1];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j0 = 0; j0 < FR; j0 += G) {
-      // groups of G frequencies: bounded look-ahead keeps the live registers low
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int j = j0; j < j0 + G; j++) {
-        const int k = zorder<NZ>(r * FR + j);
-        const int kz = k <= NZ ? k : PZ - k;
-        const C2<T> X0 = xch[(j * 3 + 0) * 64 + l];
-        const C2<T> X1 = xch[(j * 3 + 1) * 64 + l];
-        const C2<T> X2 = xch[(j * 3 + 2) * 64 + l];
-        const T *tk = tl + kz * 6 * CONVZ_COLS;
-        const T k0 = tk[e0 * CONVZ_COLS] * (k <= NZ ? y0 : z0);
-        const T k1 = tk[e1 * CONVZ_COLS] * (k <= NZ ? y1 : z1);
-        const T k2 = tk[e2 * CONVZ_COLS] * (k <= NZ ? y2 : z2);
-        C2<T> Y{X0.x * k0 + X1.x * k1 + X2.x * k2, X0.y * k0 + X1.y * k1 + X2.y * k2};
-        // keep the product here: otherwise it sinks into the guarded store
-        // block below, and all LDS reads of the kernel stay live at once
-        asm volatile("" : "+v"(Y.x), "+v"(Y.y));
-        if (k & 1) O[k >> 1] = Y;
-        else E[k >> 1] = Y;
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    __syncthreads();
-  }
-
-  // 3. inverse and store of the Nz data planes
-  dft<NZ, +1>(E);
-  pin(E);
-  dft<NZ, +1>(O);
-  pin(O);
-  if (valid) {
-#pragma unroll
+    // 1. forward: even and odd halves of the spectrum
+    C2<T> E[NZ], O[NZ];
+  #pragma unroll
     for (int n = 0; n < NZ; n++) {
-      const C2<T> o = n == 0 ? O[0] : cmul(O[n], cexpi<T>(n, PZ, +1));
-      const C2<T> y = cadd(E[n], o);
-      auto *p = at_off<T>(plane0 + n * a.zstride, loff);
-      p[0] = y.x;
-      p[1] = y.y;
+      const auto *p = at_off<const T>(plane0 + n * a.zstride, loff);
+      E[n] = C2<T>{p[0], p[1]};
+    }
+  #pragma unroll
+    for (int n = 0; n < NZ; n++) O[n] = n == 0 ? E[0] : cmul(E[n], cexpi<T>(n, PZ, -1));
+    // one transform at a time: interleaving the two raises the register peak
+    dft<NZ, -1>(E);
+    pin(E);
+    dft<NZ, -1>(O);
+    pin(O);
+
+    // 2. tensor multiply: component c of K . (X0, X1, X2) for each frequency.
+    // Row c of the symmetric tensor: x: (xx, xy, xz), y: (xy, yy, yz), z: (xz, yz, zz).
+    const int e0 = c == 0 ? 0 : c == 1 ? 5 : 4;
+    const int e1 = c == 0 ? 5 : c == 1 ? 1 : 3;
+    const int e2 = c == 0 ? 4 : c == 1 ? 3 : 2;
+    // the ky mirror flips xy and yz; the kz mirror flips xz and yz
+    const T sy = t ? T(-1) : T(1);
+    auto fy = [&](int e) { return (e == 5 || e == 3) ? sy : T(1); };
+    auto fz = [&](int e) { return (e == 4 || e == 3) ? T(-1) : T(1); };
+    const T y0 = fy(e0), y1 = fy(e1), y2 = fy(e2);
+    const T z0 = y0 * fz(e0), z1 = y1 * fz(e1), z2 = y2 * fz(e2);
+    const T *tl = tens + (l & (CONVZ_COLS - 1));
+    C2<T> *xw = xch + c * 64 + l;
+  #pragma unroll
+    for (int r = 0; r < PZ / FR; r++) {
+  #pragma unroll
+      for (int j = 0; j < FR; j++) {
+        const int k = zorder<NZ>(r * FR + j);
+        xw[j * 192] = (k & 1) ? O[k >> 1] : E[k >> 1];
+      }
+      __syncthreads();
+  #pragma unroll
+      for (int j0 = 0; j0 < FR; j0 += G) {
+        // groups of G frequencies: bounded look-ahead keeps the live registers low
+        __builtin_amdgcn_sched_barrier(0);
+  #pragma unroll
+        for (int j = j0; j < j0 + G; j++) {
+          const int k = zorder<NZ>(r * FR + j);
+          const int kz = k <= NZ ? k : PZ - k;
+          const C2<T> X0 = xch[(j * 3 + 0) * 64 + l];
+          const C2<T> X1 = xch[(j * 3 + 1) * 64 + l];
+          const C2<T> X2 = xch[(j * 3 + 2) * 64 + l];
+          const T *tk = tl + kz * 6 * CONVZ_COLS;
+          const T k0 = tk[e0 * CONVZ_COLS] * (k <= NZ ? y0 : z0);
+          const T k1 = tk[e1 * CONVZ_COLS] * (k <= NZ ? y1 : z1);
+          const T k2 = tk[e2 * CONVZ_COLS] * (k <= NZ ? y2 : z2);
+          C2<T> Y{X0.x * k0 + X1.x * k1 + X2.x * k2, X0.y * k0 + X1.y * k1 + X2.y * k2};
+          // keep the product here: otherwise it sinks into the guarded store
+          // block below, and all LDS reads of the kernel stay live at once
+          asm volatile("" : "+v"(Y.x), "+v"(Y.y));
+          if (k & 1) O[k >> 1] = Y;
+          else E[k >> 1] = Y;
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();
+    }
+
+    // 3. inverse and store of the Nz data planes
+    dft<NZ, +1>(E);
+    pin(E);
+    dft<NZ, +1>(O);
+    pin(O);
+    if (valid) {
+  #pragma unroll
+      for (int n = 0; n < NZ; n++) {
+        const C2<T> o = n == 0 ? O[0] : cmul(O[n], cexpi<T>(n, PZ, +1));
+        const C2<T> y = cadd(E[n], o);
+        auto *p = at_off<T>(plane0 + n * a.zstride, loff);
+        p[0] = y.x;
+        p[1] = y.y;
+      }
     }
   }
-}
+};
+template <typename T, int NZ, int FR, int G>
+__global__ __launch_bounds__(CONVZ_NT) void k_convz(ConvZArgs<T> a) { BConvz<T, NZ, FR, G>::run(a, hw_bid(), hw_grid()); }
 
 template <typename T> struct ConvZTable;
 // frequencies per LDS exchange round and per scheduling group (build-time tunables)
@@ -1039,11 +1099,11 @@ static void launch_convz(const ConvZArgs<T> &a, int Nz, hipStream_t s) {
   constexpr int FR = ConvZTable<T>::FR, G = ConvZTable<T>::G;
   const dim3 grid((a.H + CONVZ_COLS - 1) / CONVZ_COLS, a.Py / 2 + 1);
   switch (Nz) {
-    case 8: UM_LAUNCH((k_convz<T, 8, FR, G>), grid, dim3(CONVZ_NT), 0, s, a); break;
-    case 16: UM_LAUNCH((k_convz<T, 16, FR, G>), grid, dim3(CONVZ_NT), 0, s, a); break;
+    case 8: launch_b<BConvz<T, 8, FR, G>>(k_convz<T, 8, FR, G>, grid, dim3(CONVZ_NT), 0, s, a); break;
+    case 16: launch_b<BConvz<T, 16, FR, G>>(k_convz<T, 16, FR, G>, grid, dim3(CONVZ_NT), 0, s, a); break;
     case 32:
       if constexpr (std::is_same<T, float>::value) {
-        UM_LAUNCH((k_convz<T, 32, FR, G>), grid, dim3(CONVZ_NT), 0, s, a);
+        launch_b<BConvz<T, 32, FR, G>>(k_convz<T, 32, FR, G>, grid, dim3(CONVZ_NT), 0, s, a);
         break;
       }
       [[fallthrough]];
@@ -1437,15 +1497,19 @@ template <> struct CtTable<double> {
   static int conv2(int n) { switch (n) { case 256: return 4; case 512: return 2; } return 0; }
 };
 
+template <typename B> struct BTag { using type = B; };
+// a parenthesised type as a macro argument: Unparen<void (X<a, b>)>::type is X<a, b>
+template <typename> struct Unparen;
+template <typename X> struct Unparen<void(X)> { using type = X; };
 template <typename K>
 static void allow_lds(K kern, size_t bytes) {
   if (bytes > 65536) UM_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-#define UM_CT_CASE(KERN, N, NL, LDSB)                                        \
+#define UM_CT_CASE(BODY, KERN, N, NL, LDSB)                                  \
   if (n == N && nl == NL) {                                                  \
     allow_lds(KERN, LDSB);                                                   \
-    UM_LAUNCH(KERN, grid, dim3(FFT_NT), LDSB, s, a);                \
+    launch_b<typename Unparen<void BODY>::type>(KERN, grid, dim3(FFT_NT), LDSB, s, a); \
     UM_HIP(hipGetLastError());                                               \
     return;                                                                  \
   }
@@ -1459,13 +1523,14 @@ template <typename T>
 static bool launch_xi4(const XiArgs<T> &a, int n, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (!x4_ok<T>(n, a.Nx)) return false;
-    auto go = [&](auto kern, int rows_per_wg) {
-      UM_LAUNCH(kern, dim3((a.rows + rows_per_wg - 1) / rows_per_wg, 3), dim3(XF4_NT), 0, s, a);
+    auto go = [&](auto tag, auto kern, int rows_per_wg) {
+      using K = typename decltype(tag)::type;
+      launch_b<K>(kern, dim3((a.rows + rows_per_wg - 1) / rows_per_wg, 3), dim3(XF4_NT), 0, s, a);
       UM_HIP(hipGetLastError());
       return true;
     };
 #define UM_XI4(NN) \
-    if (n == NN) return go(k_xi4<T, NN>, XF4_NT / X4Shape<NN>::B * (X4Shape<NN>::B / X4Shape<NN>::A));
+    if (n == NN) return go(BTag<BXi4<T, NN>>{}, k_xi4<T, NN>, XF4_NT / X4Shape<NN>::B * (X4Shape<NN>::B / X4Shape<NN>::A));
     UM_XI4(64) UM_XI4(128) UM_XI4(256) UM_XI4(512) UM_XI4(1024)
 #undef UM_XI4
   }
@@ -1476,13 +1541,14 @@ template <typename T>
 static bool launch_xf4(const XArgs<T> &a, int n, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (a.mode != 0 || !x4_ok<T>(n, a.Nx) || a.in_pitch != a.Nx) return false;
-    auto go = [&](auto kern, int rows_per_wg) {
-      UM_LAUNCH(kern, dim3((a.rows + rows_per_wg - 1) / rows_per_wg, 3), dim3(XF4_NT), 0, s, a);
+    auto go = [&](auto tag, auto kern, int rows_per_wg) {
+      using K = typename decltype(tag)::type;
+      launch_b<K>(kern, dim3((a.rows + rows_per_wg - 1) / rows_per_wg, 3), dim3(XF4_NT), 0, s, a);
       UM_HIP(hipGetLastError());
       return true;
     };
 #define UM_XF4(NN) \
-    if (n == NN) return go(k_xf4<T, NN>, XF4_NT / X4Shape<NN>::B * (X4Shape<NN>::B / X4Shape<NN>::A));
+    if (n == NN) return go(BTag<BXf4<T, NN>>{}, k_xf4<T, NN>, XF4_NT / X4Shape<NN>::B * (X4Shape<NN>::B / X4Shape<NN>::A));
     UM_XF4(64) UM_XF4(128) UM_XF4(256) UM_XF4(512) UM_XF4(1024)
 #undef UM_XF4
   }
@@ -1493,39 +1559,39 @@ template <typename T>
 static void launch_xfwd(const XArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s) {
   const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + 2 * n);
   if constexpr (std::is_same<T, float>::value) {
-    UM_CT_CASE((k_xfwd<float, 64, 32>), 64, 32, ct)
-    UM_CT_CASE((k_xfwd<float, 128, 16>), 128, 16, ct)
-    UM_CT_CASE((k_xfwd<float, 256, 16>), 256, 16, ct)
-    UM_CT_CASE((k_xfwd<float, 512, 8>), 512, 8, ct)
-    UM_CT_CASE((k_xfwd<float, 1024, 4>), 1024, 4, ct)
+    UM_CT_CASE((BXfwd<float, 64, 32>), (k_xfwd<float, 64, 32>), 64, 32, ct)
+    UM_CT_CASE((BXfwd<float, 128, 16>), (k_xfwd<float, 128, 16>), 128, 16, ct)
+    UM_CT_CASE((BXfwd<float, 256, 16>), (k_xfwd<float, 256, 16>), 256, 16, ct)
+    UM_CT_CASE((BXfwd<float, 512, 8>), (k_xfwd<float, 512, 8>), 512, 8, ct)
+    UM_CT_CASE((BXfwd<float, 1024, 4>), (k_xfwd<float, 1024, 4>), 1024, 4, ct)
     // few rows (small grids): fewer rows per work-group, more work-groups
-    UM_CT_CASE((k_xfwd<float, 64, 8>), 64, 8, ct)
-    UM_CT_CASE((k_xfwd<float, 64, 1>), 64, 1, ct)
-    UM_CT_CASE((k_xfwd<float, 128, 4>), 128, 4, ct)
-    UM_CT_CASE((k_xfwd<float, 128, 1>), 128, 1, ct)
-    UM_CT_CASE((k_xfwd<float, 256, 4>), 256, 4, ct)
-    UM_CT_CASE((k_xfwd<float, 256, 1>), 256, 1, ct)
-    UM_CT_CASE((k_xfwd<float, 512, 2>), 512, 2, ct)
-    UM_CT_CASE((k_xfwd<float, 512, 1>), 512, 1, ct)
-    UM_CT_CASE((k_xfwd<float, 1024, 1>), 1024, 1, ct)
+    UM_CT_CASE((BXfwd<float, 64, 8>), (k_xfwd<float, 64, 8>), 64, 8, ct)
+    UM_CT_CASE((BXfwd<float, 64, 1>), (k_xfwd<float, 64, 1>), 64, 1, ct)
+    UM_CT_CASE((BXfwd<float, 128, 4>), (k_xfwd<float, 128, 4>), 128, 4, ct)
+    UM_CT_CASE((BXfwd<float, 128, 1>), (k_xfwd<float, 128, 1>), 128, 1, ct)
+    UM_CT_CASE((BXfwd<float, 256, 4>), (k_xfwd<float, 256, 4>), 256, 4, ct)
+    UM_CT_CASE((BXfwd<float, 256, 1>), (k_xfwd<float, 256, 1>), 256, 1, ct)
+    UM_CT_CASE((BXfwd<float, 512, 2>), (k_xfwd<float, 512, 2>), 512, 2, ct)
+    UM_CT_CASE((BXfwd<float, 512, 1>), (k_xfwd<float, 512, 1>), 512, 1, ct)
+    UM_CT_CASE((BXfwd<float, 1024, 1>), (k_xfwd<float, 1024, 1>), 1024, 1, ct)
     // small grids of the other X lengths
-    UM_CT_CASE((k_xfwd<float, 32, 32>), 32, 32, ct)
-    UM_CT_CASE((k_xfwd<float, 32, 8>), 32, 8, ct)
-    UM_CT_CASE((k_xfwd<float, 32, 1>), 32, 1, ct)
-    UM_CT_CASE((k_xfwd<float, 48, 32>), 48, 32, ct)
-    UM_CT_CASE((k_xfwd<float, 48, 8>), 48, 8, ct)
-    UM_CT_CASE((k_xfwd<float, 48, 1>), 48, 1, ct)
-    UM_CT_CASE((k_xfwd<float, 96, 16>), 96, 16, ct)
-    UM_CT_CASE((k_xfwd<float, 96, 4>), 96, 4, ct)
-    UM_CT_CASE((k_xfwd<float, 96, 1>), 96, 1, ct)
-    UM_CT_CASE((k_xfwd<float, 192, 16>), 192, 16, ct)
-    UM_CT_CASE((k_xfwd<float, 192, 4>), 192, 4, ct)
-    UM_CT_CASE((k_xfwd<float, 192, 1>), 192, 1, ct)
+    UM_CT_CASE((BXfwd<float, 32, 32>), (k_xfwd<float, 32, 32>), 32, 32, ct)
+    UM_CT_CASE((BXfwd<float, 32, 8>), (k_xfwd<float, 32, 8>), 32, 8, ct)
+    UM_CT_CASE((BXfwd<float, 32, 1>), (k_xfwd<float, 32, 1>), 32, 1, ct)
+    UM_CT_CASE((BXfwd<float, 48, 32>), (k_xfwd<float, 48, 32>), 48, 32, ct)
+    UM_CT_CASE((BXfwd<float, 48, 8>), (k_xfwd<float, 48, 8>), 48, 8, ct)
+    UM_CT_CASE((BXfwd<float, 48, 1>), (k_xfwd<float, 48, 1>), 48, 1, ct)
+    UM_CT_CASE((BXfwd<float, 96, 16>), (k_xfwd<float, 96, 16>), 96, 16, ct)
+    UM_CT_CASE((BXfwd<float, 96, 4>), (k_xfwd<float, 96, 4>), 96, 4, ct)
+    UM_CT_CASE((BXfwd<float, 96, 1>), (k_xfwd<float, 96, 1>), 96, 1, ct)
+    UM_CT_CASE((BXfwd<float, 192, 16>), (k_xfwd<float, 192, 16>), 192, 16, ct)
+    UM_CT_CASE((BXfwd<float, 192, 4>), (k_xfwd<float, 192, 4>), 192, 4, ct)
+    UM_CT_CASE((BXfwd<float, 192, 1>), (k_xfwd<float, 192, 1>), 192, 1, ct)
   } else {
-    UM_CT_CASE((k_xfwd<double, 256, 8>), 256, 8, ct)
-    UM_CT_CASE((k_xfwd<double, 512, 4>), 512, 4, ct)
+    UM_CT_CASE((BXfwd<double, 256, 8>), (k_xfwd<double, 256, 8>), 256, 8, ct)
+    UM_CT_CASE((BXfwd<double, 512, 4>), (k_xfwd<double, 512, 4>), 512, 4, ct)
   }
-  UM_LAUNCH((k_xfwd<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
+  launch_b<BXfwd<T, 0, 0>>(k_xfwd<T, 0, 0>, grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
   UM_HIP(hipGetLastError());
 }
 template <typename T>
@@ -1533,48 +1599,48 @@ static void launch_xinv(const XiArgs<T> &a, int n, int nl, dim3 grid, hipStream_
   if (four_step && launch_xi4<T>(a, n, s)) return;
   const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + 2 * n);
   if constexpr (std::is_same<T, float>::value) {
-    UM_CT_CASE((k_xinv<float, 64, 32>), 64, 32, ct)
-    UM_CT_CASE((k_xinv<float, 128, 16>), 128, 16, ct)
-    UM_CT_CASE((k_xinv<float, 256, 16>), 256, 16, ct)
-    UM_CT_CASE((k_xinv<float, 512, 8>), 512, 8, ct)
-    UM_CT_CASE((k_xinv<float, 1024, 4>), 1024, 4, ct)
-    UM_CT_CASE((k_xinv<float, 32, 32>), 32, 32, ct)
-    UM_CT_CASE((k_xinv<float, 32, 8>), 32, 8, ct)
-    UM_CT_CASE((k_xinv<float, 32, 1>), 32, 1, ct)
-    UM_CT_CASE((k_xinv<float, 48, 32>), 48, 32, ct)
-    UM_CT_CASE((k_xinv<float, 48, 8>), 48, 8, ct)
-    UM_CT_CASE((k_xinv<float, 48, 1>), 48, 1, ct)
-    UM_CT_CASE((k_xinv<float, 96, 16>), 96, 16, ct)
-    UM_CT_CASE((k_xinv<float, 96, 4>), 96, 4, ct)
-    UM_CT_CASE((k_xinv<float, 96, 1>), 96, 1, ct)
-    UM_CT_CASE((k_xinv<float, 192, 16>), 192, 16, ct)
-    UM_CT_CASE((k_xinv<float, 192, 4>), 192, 4, ct)
-    UM_CT_CASE((k_xinv<float, 192, 1>), 192, 1, ct)
+    UM_CT_CASE((BXinv<float, 64, 32>), (k_xinv<float, 64, 32>), 64, 32, ct)
+    UM_CT_CASE((BXinv<float, 128, 16>), (k_xinv<float, 128, 16>), 128, 16, ct)
+    UM_CT_CASE((BXinv<float, 256, 16>), (k_xinv<float, 256, 16>), 256, 16, ct)
+    UM_CT_CASE((BXinv<float, 512, 8>), (k_xinv<float, 512, 8>), 512, 8, ct)
+    UM_CT_CASE((BXinv<float, 1024, 4>), (k_xinv<float, 1024, 4>), 1024, 4, ct)
+    UM_CT_CASE((BXinv<float, 32, 32>), (k_xinv<float, 32, 32>), 32, 32, ct)
+    UM_CT_CASE((BXinv<float, 32, 8>), (k_xinv<float, 32, 8>), 32, 8, ct)
+    UM_CT_CASE((BXinv<float, 32, 1>), (k_xinv<float, 32, 1>), 32, 1, ct)
+    UM_CT_CASE((BXinv<float, 48, 32>), (k_xinv<float, 48, 32>), 48, 32, ct)
+    UM_CT_CASE((BXinv<float, 48, 8>), (k_xinv<float, 48, 8>), 48, 8, ct)
+    UM_CT_CASE((BXinv<float, 48, 1>), (k_xinv<float, 48, 1>), 48, 1, ct)
+    UM_CT_CASE((BXinv<float, 96, 16>), (k_xinv<float, 96, 16>), 96, 16, ct)
+    UM_CT_CASE((BXinv<float, 96, 4>), (k_xinv<float, 96, 4>), 96, 4, ct)
+    UM_CT_CASE((BXinv<float, 96, 1>), (k_xinv<float, 96, 1>), 96, 1, ct)
+    UM_CT_CASE((BXinv<float, 192, 16>), (k_xinv<float, 192, 16>), 192, 16, ct)
+    UM_CT_CASE((BXinv<float, 192, 4>), (k_xinv<float, 192, 4>), 192, 4, ct)
+    UM_CT_CASE((BXinv<float, 192, 1>), (k_xinv<float, 192, 1>), 192, 1, ct)
   } else {
-    UM_CT_CASE((k_xinv<double, 256, 8>), 256, 8, ct)
-    UM_CT_CASE((k_xinv<double, 512, 4>), 512, 4, ct)
+    UM_CT_CASE((BXinv<double, 256, 8>), (k_xinv<double, 256, 8>), 256, 8, ct)
+    UM_CT_CASE((BXinv<double, 512, 4>), (k_xinv<double, 512, 4>), 512, 4, ct)
   }
-  UM_LAUNCH((k_xinv<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
+  launch_b<BXinv<T, 0, 0>>(k_xinv<T, 0, 0>, grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
   UM_HIP(hipGetLastError());
 }
 template <typename T, int DIR>
 static void launch_col(const ColArgs<T> &a, int n, int nl, dim3 grid, hipStream_t s) {
   const size_t ct = sizeof(C2<T>) * ((size_t)nl * n + n);
   if constexpr (std::is_same<T, float>::value) {
-    UM_CT_CASE((k_colfft<float, DIR, 64, 32>), 64, 32, ct)
-    UM_CT_CASE((k_colfft<float, DIR, 128, 16>), 128, 16, ct)
-    UM_CT_CASE((k_colfft<float, DIR, 256, 16>), 256, 16, ct)
-    UM_CT_CASE((k_colfft<float, DIR, 512, 8>), 512, 8, ct)
-    UM_CT_CASE((k_colfft<float, DIR, 1024, 8>), 1024, 8, ct)
-    UM_CT_CASE((k_colfft<float, DIR, 2048, 4>), 2048, 4, ct)
-    UM_CT_CASE((k_colfft<float, DIR, 96, 16>), 96, 16, ct)
-    UM_CT_CASE((k_colfft<float, DIR, 192, 8>), 192, 8, ct)
-    UM_CT_CASE((k_colfft<float, DIR, 384, 8>), 384, 8, ct)
+    UM_CT_CASE((BColfft<float, DIR, 64, 32>), (k_colfft<float, DIR, 64, 32>), 64, 32, ct)
+    UM_CT_CASE((BColfft<float, DIR, 128, 16>), (k_colfft<float, DIR, 128, 16>), 128, 16, ct)
+    UM_CT_CASE((BColfft<float, DIR, 256, 16>), (k_colfft<float, DIR, 256, 16>), 256, 16, ct)
+    UM_CT_CASE((BColfft<float, DIR, 512, 8>), (k_colfft<float, DIR, 512, 8>), 512, 8, ct)
+    UM_CT_CASE((BColfft<float, DIR, 1024, 8>), (k_colfft<float, DIR, 1024, 8>), 1024, 8, ct)
+    UM_CT_CASE((BColfft<float, DIR, 2048, 4>), (k_colfft<float, DIR, 2048, 4>), 2048, 4, ct)
+    UM_CT_CASE((BColfft<float, DIR, 96, 16>), (k_colfft<float, DIR, 96, 16>), 96, 16, ct)
+    UM_CT_CASE((BColfft<float, DIR, 192, 8>), (k_colfft<float, DIR, 192, 8>), 192, 8, ct)
+    UM_CT_CASE((BColfft<float, DIR, 384, 8>), (k_colfft<float, DIR, 384, 8>), 384, 8, ct)
   } else {
-    UM_CT_CASE((k_colfft<double, DIR, 512, 4>), 512, 4, ct)
-    UM_CT_CASE((k_colfft<double, DIR, 1024, 4>), 1024, 4, ct)
+    UM_CT_CASE((BColfft<double, DIR, 512, 4>), (k_colfft<double, DIR, 512, 4>), 512, 4, ct)
+    UM_CT_CASE((BColfft<double, DIR, 1024, 4>), (k_colfft<double, DIR, 1024, 4>), 1024, 4, ct)
   }
-  UM_LAUNCH((k_colfft<T, DIR, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
+  launch_b<BColfft<T, DIR, 0, 0>>(k_colfft<T, DIR, 0, 0>, grid, dim3(FFT_NT), sizeof(C2<T>) * nl * n, s, a);
   UM_HIP(hipGetLastError());
 }
 // Register four-step Y pass when P = A*A (A = 8, 16, 32 in FP32; 8, 16 in FP64)
@@ -1590,21 +1656,22 @@ static bool launch_coly(const ColArgs<T> &a, int P, int nz, hipStream_t s) {
   // planes per work-group (software-pipelined): FP32 forward pass only (the
   // inverse loads twice as many rows and would spill)
   const int zb = (sizeof(T) == 4 && DIR < 0 && nz % UM_COLY_ZB == 0) ? UM_COLY_ZB : 1;
-  auto go = [&](auto kern, int A, int C, int ZB) {
+  auto go = [&](auto tag, auto kern, int A, int C, int ZB) {
+    using K = typename decltype(tag)::type;
     // k_coly's SPLIT: real and imaginary parts exchanged one after the other
     size_t lds = sizeof(C2<T>) * (size_t)A * (A * C + C);
     if (lds > 65536) lds /= 2;
     allow_lds(kern, lds);
     const dim3 grid((a.H + C - 1) / C, nz / ZB, 3);
-    UM_LAUNCH(kern, grid, dim3(A * C), lds, s, a);
+    launch_b<K>(kern, grid, dim3(A * C), lds, s, a);
     UM_HIP(hipGetLastError());
     return true;
   };
 #define UM_COLY(A, C)                                                                                         \
   if (P == A * A) {                                                                                           \
     if constexpr (DIR < 0 && UM_COLY_ZB > 1)                                                                  \
-      if (half && zb == UM_COLY_ZB) return go(k_coly<T, A, DIR, C, true, UM_COLY_ZB>, A, C, UM_COLY_ZB);     \
-    return half ? go(k_coly<T, A, DIR, C, true, 1>, A, C, 1) : go(k_coly<T, A, DIR, C, false, 1>, A, C, 1);  \
+      if (half && zb == UM_COLY_ZB) return go(BTag<BColy<T, A, DIR, C, true, UM_COLY_ZB>>{}, k_coly<T, A, DIR, C, true, UM_COLY_ZB>, A, C, UM_COLY_ZB);     \
+    return half ? go(BTag<BColy<T, A, DIR, C, true, 1>>{}, k_coly<T, A, DIR, C, true, 1>, A, C, 1) : go(BTag<BColy<T, A, DIR, C, false, 1>>{}, k_coly<T, A, DIR, C, false, 1>, A, C, 1);  \
   }
   if constexpr (std::is_same<T, float>::value) {
     UM_COLY(32, UM_COLY_C32)
@@ -1624,44 +1691,44 @@ static void launch_conv(const ConvArgs<T> &a, int n, int B, dim3 grid, hipStream
   const size_t ct = sizeof(C2<T>) * ((size_t)3 * B * n + n);
   if (!a.is2d) {
     if constexpr (std::is_same<T, float>::value) {
-      UM_CT_CASE((k_conv<float, 16, 16>), 16, 16, ct)
-      UM_CT_CASE((k_conv<float, 32, 16>), 32, 16, ct)
-      UM_CT_CASE((k_conv<float, 64, 16>), 64, 16, ct)
-      UM_CT_CASE((k_conv<float, 3, 32>), 3, 32, ct)     // Pz = 2 Nz - 1 for Nz = 2, 3, 4
-      UM_CT_CASE((k_conv<float, 5, 32>), 5, 32, ct)
-      UM_CT_CASE((k_conv<float, 7, 32>), 7, 32, ct)
+      UM_CT_CASE((BConv<float, 16, 16>), (k_conv<float, 16, 16>), 16, 16, ct)
+      UM_CT_CASE((BConv<float, 32, 16>), (k_conv<float, 32, 16>), 32, 16, ct)
+      UM_CT_CASE((BConv<float, 64, 16>), (k_conv<float, 64, 16>), 64, 16, ct)
+      UM_CT_CASE((BConv<float, 3, 32>), (k_conv<float, 3, 32>), 3, 32, ct)     // Pz = 2 Nz - 1 for Nz = 2, 3, 4
+      UM_CT_CASE((BConv<float, 5, 32>), (k_conv<float, 5, 32>), 5, 32, ct)
+      UM_CT_CASE((BConv<float, 7, 32>), (k_conv<float, 7, 32>), 7, 32, ct)
     } else {
-      UM_CT_CASE((k_conv<double, 16, 8>), 16, 8, ct)
-      UM_CT_CASE((k_conv<double, 64, 8>), 64, 8, ct)
+      UM_CT_CASE((BConv<double, 16, 8>), (k_conv<double, 16, 8>), 16, 8, ct)
+      UM_CT_CASE((BConv<double, 64, 8>), (k_conv<double, 64, 8>), 64, 8, ct)
     }
   } else {
     if constexpr (std::is_same<T, float>::value) {
-      UM_CT_CASE((k_conv<float, 64, 16>), 64, 16, ct)
-      UM_CT_CASE((k_conv<float, 128, 16>), 128, 16, ct)
-      UM_CT_CASE((k_conv<float, 256, 8>), 256, 8, ct)
-      UM_CT_CASE((k_conv<float, 512, 4>), 512, 4, ct)
+      UM_CT_CASE((BConv<float, 64, 16>), (k_conv<float, 64, 16>), 64, 16, ct)
+      UM_CT_CASE((BConv<float, 128, 16>), (k_conv<float, 128, 16>), 128, 16, ct)
+      UM_CT_CASE((BConv<float, 256, 8>), (k_conv<float, 256, 8>), 256, 8, ct)
+      UM_CT_CASE((BConv<float, 512, 4>), (k_conv<float, 512, 4>), 512, 4, ct)
       // narrow spectra (small grids): fewer columns per work-group
-      UM_CT_CASE((k_conv<float, 64, 4>), 64, 4, ct)
-      UM_CT_CASE((k_conv<float, 64, 1>), 64, 1, ct)
-      UM_CT_CASE((k_conv<float, 128, 4>), 128, 4, ct)
-      UM_CT_CASE((k_conv<float, 128, 1>), 128, 1, ct)
-      UM_CT_CASE((k_conv<float, 256, 2>), 256, 2, ct)
-      UM_CT_CASE((k_conv<float, 256, 1>), 256, 1, ct)
-      UM_CT_CASE((k_conv<float, 512, 1>), 512, 1, ct)
-      UM_CT_CASE((k_conv<float, 96, 16>), 96, 16, ct)
-      UM_CT_CASE((k_conv<float, 96, 4>), 96, 4, ct)
-      UM_CT_CASE((k_conv<float, 96, 1>), 96, 1, ct)
-      UM_CT_CASE((k_conv<float, 192, 8>), 192, 8, ct)
-      UM_CT_CASE((k_conv<float, 192, 2>), 192, 2, ct)
-      UM_CT_CASE((k_conv<float, 192, 1>), 192, 1, ct)
-      UM_CT_CASE((k_conv<float, 384, 4>), 384, 4, ct)
-      UM_CT_CASE((k_conv<float, 384, 1>), 384, 1, ct)
+      UM_CT_CASE((BConv<float, 64, 4>), (k_conv<float, 64, 4>), 64, 4, ct)
+      UM_CT_CASE((BConv<float, 64, 1>), (k_conv<float, 64, 1>), 64, 1, ct)
+      UM_CT_CASE((BConv<float, 128, 4>), (k_conv<float, 128, 4>), 128, 4, ct)
+      UM_CT_CASE((BConv<float, 128, 1>), (k_conv<float, 128, 1>), 128, 1, ct)
+      UM_CT_CASE((BConv<float, 256, 2>), (k_conv<float, 256, 2>), 256, 2, ct)
+      UM_CT_CASE((BConv<float, 256, 1>), (k_conv<float, 256, 1>), 256, 1, ct)
+      UM_CT_CASE((BConv<float, 512, 1>), (k_conv<float, 512, 1>), 512, 1, ct)
+      UM_CT_CASE((BConv<float, 96, 16>), (k_conv<float, 96, 16>), 96, 16, ct)
+      UM_CT_CASE((BConv<float, 96, 4>), (k_conv<float, 96, 4>), 96, 4, ct)
+      UM_CT_CASE((BConv<float, 96, 1>), (k_conv<float, 96, 1>), 96, 1, ct)
+      UM_CT_CASE((BConv<float, 192, 8>), (k_conv<float, 192, 8>), 192, 8, ct)
+      UM_CT_CASE((BConv<float, 192, 2>), (k_conv<float, 192, 2>), 192, 2, ct)
+      UM_CT_CASE((BConv<float, 192, 1>), (k_conv<float, 192, 1>), 192, 1, ct)
+      UM_CT_CASE((BConv<float, 384, 4>), (k_conv<float, 384, 4>), 384, 4, ct)
+      UM_CT_CASE((BConv<float, 384, 1>), (k_conv<float, 384, 1>), 384, 1, ct)
     } else {
-      UM_CT_CASE((k_conv<double, 256, 4>), 256, 4, ct)
-      UM_CT_CASE((k_conv<double, 512, 2>), 512, 2, ct)
+      UM_CT_CASE((BConv<double, 256, 4>), (k_conv<double, 256, 4>), 256, 4, ct)
+      UM_CT_CASE((BConv<double, 512, 2>), (k_conv<double, 512, 2>), 512, 2, ct)
     }
   }
-  UM_LAUNCH((k_conv<T, 0, 0>), grid, dim3(FFT_NT), sizeof(C2<T>) * 3 * B * n, s, a);
+  launch_b<BConv<T, 0, 0>>(k_conv<T, 0, 0>, grid, dim3(FFT_NT), sizeof(C2<T>) * 3 * B * n, s, a);
   UM_HIP(hipGetLastError());
 }
 #undef UM_CT_CASE
@@ -1685,95 +1752,108 @@ template <int N> struct XFShape {
   static_assert(L / G * B == XF_NT, "the lines of R rows fill the work-group");
 };
 
+template <typename T>
+struct XiFieldArgs {
+  XiArgs<T> x;
+  FieldArgs<T> f;
+};
 template <typename T, int N, bool REG, int EX, bool UNI>
-__global__ __launch_bounds__(XF_NT) void k_xi4_field(XiArgs<T> a, FieldArgs<T> f) {
-  using S = XFShape<N>;
-  constexpr int A = S::A, B = S::B, G = S::G, R = S::R, TS = B * (A + 1), HB = B / 2;
-  static_assert(S::L * N <= S::L * TS, "B_demag lines fit the transpose space");
-  // the transpose moves the real, then the imaginary parts (half the LDS of
-  // complex entries: more work-groups per CU); afterwards line (c, j) holds
-  // B_demag_c of row j, x < N
-  __shared__ __align__(16) T ldr[S::L * TS];
-  const int Ny = (int)a.fRy.d, Nz = a.rows / Ny;
-  const int w = xcd_tile(blockIdx.x, gridDim.x);
-  const int yb = w / Nz, z = w - yb * Nz, y0 = yb * R;
-  const int grp = threadIdx.x / B, l = threadIdx.x % B;
-  C2<T> v[B];
-  const int h1 = l / A, n1 = l % A, line1 = grp * G + h1;
-  {
-    const int c = line1 / R, y = y0 + (line1 - c * R);
-#pragma unroll
-    for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
-    if (y < Ny) {
-      const C2<T> *irow = a.in[c] + ((long)z * a.Py + y) * a.Hp;
-      // two halves of loads: half the staging registers live at a time
-#pragma unroll
-      for (int hb = 0; hb < 2; hb++) {
-        C2<T> xk[HB], xn[HB];
-#pragma unroll
-        for (int q = 0; q < HB; q++) {
-          const int k = n1 + A * (hb * HB + q);
-          xk[q] = irow[k];
-          xn[q] = irow[N - k];
+struct BXi4Field {
+  using Args = XiFieldArgs<T>;
+  static constexpr int NT = XF_NT, MINB = 1;
+  static __device__ __forceinline__ void run(const XiFieldArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    const XiArgs<T> &x = a.x;
+    const FieldArgs<T> &f = a.f;
+    using S = XFShape<N>;
+    constexpr int A = S::A, B = S::B, G = S::G, R = S::R, TS = B * (A + 1), HB = B / 2;
+    static_assert(S::L * N <= S::L * TS, "B_demag lines fit the transpose space");
+    // the transpose moves the real, then the imaginary parts (half the LDS of
+    // complex entries: more work-groups per CU); afterwards line (c, j) holds
+    // B_demag_c of row j, x < N
+    __shared__ __align__(16) T ldr[S::L * TS];
+    const int Ny = (int)x.fRy.d, Nz = x.rows / Ny;
+    const int w = xcd_tile(blockIdx.x, gridDim.x);
+    const int yb = w / Nz, z = w - yb * Nz, y0 = yb * R;
+    const int grp = threadIdx.x / B, l = threadIdx.x % B;
+    C2<T> v[B];
+    const int h1 = l / A, n1 = l % A, line1 = grp * G + h1;
+    {
+      const int c = line1 / R, y = y0 + (line1 - c * R);
+  #pragma unroll
+      for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
+      if (y < Ny) {
+        const C2<T> *irow = x.in[c] + ((long)z * x.Py + y) * x.Hp;
+        // two halves of loads: half the staging registers live at a time
+  #pragma unroll
+        for (int hb = 0; hb < 2; hb++) {
+          C2<T> xk[HB], xn[HB];
+  #pragma unroll
+          for (int q = 0; q < HB; q++) {
+            const int k = n1 + A * (hb * HB + q);
+            xk[q] = irow[k];
+            xn[q] = irow[N - k];
+          }
+  #pragma unroll
+          for (int q = 0; q < HB; q++) {
+            const int n2 = hb * HB + q, k = n1 + A * n2;
+            const C2<T> sm = cadd(xk[q], cconj(xn[q])), d = csub(xk[q], cconj(xn[q]));
+            const C2<T> t = cmul(cconj(x.tw2[k]), d);
+            v[n2] = C2<T>{sm.x - t.y, sm.y + t.x};
+          }
+          __builtin_amdgcn_sched_barrier(0);
         }
-#pragma unroll
-        for (int q = 0; q < HB; q++) {
-          const int n2 = hb * HB + q, k = n1 + A * n2;
-          const C2<T> sm = cadd(xk[q], cconj(xn[q])), d = csub(xk[q], cconj(xn[q]));
-          const C2<T> t = cmul(cconj(a.tw2[k]), d);
-          v[n2] = C2<T>{sm.x - t.y, sm.y + t.x};
-        }
-        __builtin_amdgcn_sched_barrier(0);
       }
+      dft<B, +1>(v);
+  #pragma unroll
+      for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(x.tw2[2 * n1 * k2]));   // W_n^-(n1 k2)
     }
-    dft<B, +1>(v);
-#pragma unroll
-    for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(a.tw2[2 * n1 * k2]));   // W_n^-(n1 k2)
-  }
-  C2<T> u[G][A];
-  T *tw = ldr + line1 * TS + n1;
-#pragma unroll
-  for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].x;
-  __syncthreads();
-#pragma unroll
-  for (int h = 0; h < G; h++) {
-    const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
-#pragma unroll
-    for (int q = 0; q < A; q++) u[h][q].x = t[q];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].y;
-  __syncthreads();
-#pragma unroll
-  for (int h = 0; h < G; h++) {
-    const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
-#pragma unroll
-    for (int q = 0; q < A; q++) u[h][q].y = t[q];
-  }
-  T *bl = ldr;
-  __syncthreads();   // every transpose read is done: the space now takes the real lines
-#pragma unroll
-  for (int h = 0; h < G; h++) {
-    dft<A, +1>(u[h]);
-    T *o = bl + (grp * G + h) * N;
-#pragma unroll
-    for (int k1 = 0; k1 < A / 2; k1++) *(float2 *)(o + 2 * (l + B * k1)) = float2{u[h][k1].x, u[h][k1].y};
-  }
-  __syncthreads();
-  constexpr int NGX = N / 4, NG = R * NGX;
-  for (int g = threadIdx.x; g < NG; g += XF_NT) {
-    const int j = g / NGX, gx = g - j * NGX, y = y0 + j;
-    if (y >= Ny) continue;
-    T bdv[3][4];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const float4 b = *(const float4 *)(bl + (c * R + j) * N + 4 * gx);
-      bdv[c][0] = b.x; bdv[c][1] = b.y; bdv[c][2] = b.z; bdv[c][3] = b.w;
+    C2<T> u[G][A];
+    T *tw = ldr + line1 * TS + n1;
+  #pragma unroll
+    for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].x;
+    __syncthreads();
+  #pragma unroll
+    for (int h = 0; h < G; h++) {
+      const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
+  #pragma unroll
+      for (int q = 0; q < A; q++) u[h][q].x = t[q];
     }
-    field_group4<T, REG, EX, false, UNI, false>(f, 4 * gx, y, z, bdv);   // registers over instructions
+    __syncthreads();
+  #pragma unroll
+    for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].y;
+    __syncthreads();
+  #pragma unroll
+    for (int h = 0; h < G; h++) {
+      const T *t = ldr + (grp * G + h) * TS + l * (A + 1);
+  #pragma unroll
+      for (int q = 0; q < A; q++) u[h][q].y = t[q];
+    }
+    T *bl = ldr;
+    __syncthreads();   // every transpose read is done: the space now takes the real lines
+  #pragma unroll
+    for (int h = 0; h < G; h++) {
+      dft<A, +1>(u[h]);
+      T *o = bl + (grp * G + h) * N;
+  #pragma unroll
+      for (int k1 = 0; k1 < A / 2; k1++) *(float2 *)(o + 2 * (l + B * k1)) = float2{u[h][k1].x, u[h][k1].y};
+    }
+    __syncthreads();
+    constexpr int NGX = N / 4, NG = R * NGX;
+    for (int g = threadIdx.x; g < NG; g += XF_NT) {
+      const int j = g / NGX, gx = g - j * NGX, y = y0 + j;
+      if (y >= Ny) continue;
+      T bdv[3][4];
+  #pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const float4 b = *(const float4 *)(bl + (c * R + j) * N + 4 * gx);
+        bdv[c][0] = b.x; bdv[c][1] = b.y; bdv[c][2] = b.z; bdv[c][3] = b.w;
+      }
+      field_group4<T, REG, EX, false, UNI, false>(f, 4 * gx, y, z, bdv);   // registers over instructions
+    }
   }
-}
+};
+template <typename T, int N, bool REG, int EX, bool UNI>
+__global__ __launch_bounds__(XF_NT) void k_xi4_field(XiFieldArgs<T> a) { BXi4Field<T, N, REG, EX, UNI>::run(a, hw_bid(), hw_grid()); }
 
 // whether the fused kernel covers these field terms and buffers: the
 // four-cell kernel's terms without spin-transfer torques or bulk DMI, every
@@ -1797,13 +1877,15 @@ static void launch_xi4_field(const XiArgs<T> &a, const FieldArgs<T> &f, int n, h
   if constexpr (std::is_same<T, float>::value) {
     const int Ny = a.fRy.d, Nz = a.rows / Ny;
     const bool reg = f.regions != nullptr, uni = field_uniform(f);
-    auto go = [&](auto kern, int R) {
+    auto go = [&](auto tag, auto kern, int R) {
+      using K = typename decltype(tag)::type;
       const dim3 grid((unsigned)(((Ny + R - 1) / R) * Nz));
-      UM_LAUNCH(kern, grid, dim3(XF_NT), 0, s, a, f);
+      launch_b<K>(kern, grid, dim3(XF_NT), 0, s, XiFieldArgs<T>{a, f});
       UM_HIP(hipGetLastError());
     };
 #define UM_XF2(CN, RG, E)                                                                            \
-  { if (uni) go(k_xi4_field<T, CN, RG, E, true>, XFShape<CN>::R); else go(k_xi4_field<T, CN, RG, E, false>, XFShape<CN>::R); }
+  { if (uni) go(BTag<BXi4Field<T, CN, RG, E, true>>{}, k_xi4_field<T, CN, RG, E, true>, XFShape<CN>::R);        \
+    else go(BTag<BXi4Field<T, CN, RG, E, false>>{}, k_xi4_field<T, CN, RG, E, false>, XFShape<CN>::R); }
 #define UM_XF(CN)                                                                                    \
   if (n == CN) {                                                                                     \
     if (f.exch_mode == 0) { if (reg) UM_XF2(CN, true, 0) else UM_XF2(CN, false, 0) }                 \
@@ -2066,7 +2148,7 @@ struct DemagPlan : DemagBase {
     dim3 g((a.rows + nl_x - 1) / nl_x, 3);
     if (xodd) {
       a.f = tx2.len; a.fhalf.init(H);
-      UM_LAUNCH(k_xfwd_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
+      launch_b<BXfwdOdd<T>>(k_xfwd_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_x * P[0], s, a);
       UM_HIP(hipGetLastError());
     } else if (!(xi4 && launch_xf4<T>(a, P[0] / 2, s))) {
       launch_xfwd<T>(a, P[0] / 2, nl_x, g, s);
@@ -2122,7 +2204,7 @@ struct DemagPlan : DemagBase {
     dim3 g((a.rows + nl_xi - 1) / nl_xi, 3);
     if (xodd) {
       a.f = tx2.len; a.fhalf.init(H);
-      UM_LAUNCH(k_xinv_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_xi * P[0], s, a);
+      launch_b<BXinvOdd<T>>(k_xinv_odd<T>, g, dim3(FFT_NT), sizeof(C2<T>) * nl_xi * P[0], s, a);
       UM_HIP(hipGetLastError());
     } else {
       launch_xinv<T>(a, P[0] / 2, nl_xi, g, s, xi4);

@@ -1,6 +1,7 @@
 // Field-term kernels: unfused entry points (opencl/exchange.go, dmi.go,
 // anisotropy.go, region.go, zeromask.go, lltorque.go, temperature.go) and the
 // fused effective-field + torque pass.
+#include "batch.hpp"
 #include "field.hpp"
 
 namespace um {
@@ -169,107 +170,119 @@ __global__ void k_thermal(T *Bx, T *By, T *Bz, uint64_t seed, uint64_t step, uin
 // SetEffectiveField (engine/effectivefield.go:14-24), then tau = LLTorque(m, B)
 // (lltorque2.cl) and FreezeSpins (engine/torque.go:348-352), one pass per cell.
 template <typename T>
-__global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) {
-  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
-  const int iz = blockIdx.z;
-  const Mesh &g = a.g;
-  if (ix >= g.Nx || iy >= g.Ny) return;
-  const long I = g.idx(ix, iy, iz);
-  const V3<T> m0 = load3(a.m, I);
-  V3<T> B = a.bd[0] ? load3(a.bd, I) : v3(T(0), T(0), T(0));
-  if (a.exch_mode != 0 && !is0(m0)) {
-    const V3<T> h =
-        (a.exch_mode == 1)   ? exchange_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.wx, a.wy, a.wz)
-        : (a.exch_mode == 2) ? dmi_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.dLUT, a.cx, a.cy, a.cz, a.openbc)
-                             : dmibulk_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.dLUT, a.cx, a.cy, a.cz,
-                                            a.openbc);
-    const T invMs = inv_msat(a.Ms, I);
-    B.x += h.x * invMs;
-    B.y += h.y * invMs;
-    B.z += h.z * invMs;
-  }
-  if (a.has_uniax) {
-    const V3<T> h = uniaxial_cell(I, m0, a.Ms, a.Ku1, a.Ku2, a.u);
-    B.x += h.x; B.y += h.y; B.z += h.z;
-  }
-  if (a.has_cubic) {
-    const V3<T> h = cubic_cell(I, m0, a.Ms, a.Kc1, a.Kc2, a.Kc3, a.c1, a.c2);
-    B.x += h.x; B.y += h.y; B.z += h.z;
-  }
-  if (a.bextLUT[0]) {
-    const int r = a.regions ? a.regions[I] : 0;
-    B.x += a.bextLUT[0][r];
-    B.y += a.bextLUT[1][r];
-    B.z += a.bextLUT[2][r];
-  }
-  if (a.has_bext) {
-    B.x += a.bext[0].at(I);
-    B.y += a.bext[1].at(I);
-    B.z += a.bext[2].at(I);
-  }
-  if (a.has_therm) {
-    const uint64_t stp = a.step_dev ? (uint64_t)*a.step_dev : a.step;
-    const V3<T> h = thermal_cell(I, a.seed, stp, a.stream, a.kth, a.Ms, a.temp, a.alpha);
-    B.x += h.x; B.y += h.y; B.z += h.z;
-  }
-  if (a.badd[0]) {   // AddCustomField (engine/effectivefield.go:23)
-    const V3<T> h = load3(a.badd, I);
-    B.x += h.x; B.y += h.y; B.z += h.z;
-  }
-  if (a.beff[0]) {
-    a.beff[0][I] = B.x;
-    a.beff[1][I] = B.y;
-    a.beff[2][I] = B.z;
-  }
-  if (a.tau[0]) {
-    V3<T> t;
-    if (a.precess) {
-      t = lltorque_cell(m0, B, a.alpha.at(I));
-    } else {
-      const V3<T> c = vcross(m0, vcross(m0, B));
-      t = v3(-c.x, -c.y, -c.z);
+struct BFieldTorque {
+  using Args = FieldArgs<T>;
+  static constexpr int NT = 256, MINB = 1;
+  static __device__ __forceinline__ void run(const FieldArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    const int iz = blockIdx.z;
+    const Mesh &g = a.g;
+    if (ix >= g.Nx || iy >= g.Ny) return;
+    const long I = g.idx(ix, iy, iz);
+    const V3<T> m0 = load3(a.m, I);
+    V3<T> B = a.bd[0] ? load3(a.bd, I) : v3(T(0), T(0), T(0));
+    if (a.exch_mode != 0 && !is0(m0)) {
+      const V3<T> h =
+          (a.exch_mode == 1)   ? exchange_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.wx, a.wy, a.wz)
+          : (a.exch_mode == 2) ? dmi_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.dLUT, a.cx, a.cy, a.cz, a.openbc)
+                               : dmibulk_cell(g, ix, iy, iz, I, m0, a.m, a.regions, a.aLUT, a.dLUT, a.cx, a.cy, a.cz,
+                                              a.openbc);
+      const T invMs = inv_msat(a.Ms, I);
+      B.x += h.x * invMs;
+      B.y += h.y * invMs;
+      B.z += h.z * invMs;
     }
-    // AddSTTorque (engine/torque.go:147-196), before FreezeSpins
-    if (a.has_zl) {
-      const V3<T> z = zhangli_cell(g, ix, iy, iz, I, m0, a.m, a.Ms, a.alpha, a.stt, a.cx, a.cy, a.cz);
-      t.x += z.x; t.y += z.y; t.z += z.z;
+    if (a.has_uniax) {
+      const V3<T> h = uniaxial_cell(I, m0, a.Ms, a.Ku1, a.Ku2, a.u);
+      B.x += h.x; B.y += h.y; B.z += h.z;
     }
-    if (a.has_sl) {
-      const V3<T> z = slonczewski_cell(I, m0, a.Ms, a.alpha, a.stt);
-      t.x += z.x; t.y += z.y; t.z += z.z;
+    if (a.has_cubic) {
+      const V3<T> h = cubic_cell(I, m0, a.Ms, a.Kc1, a.Kc2, a.Kc3, a.c1, a.c2);
+      B.x += h.x; B.y += h.y; B.z += h.z;
     }
-    if (a.tadd[0]) {   // AddCustomTorques (engine/torque.go:129)
-      const V3<T> z = load3(a.tadd, I);
-      t.x += z.x; t.y += z.y; t.z += z.z;
+    if (a.bextLUT[0]) {
+      const int r = a.regions ? a.regions[I] : 0;
+      B.x += a.bextLUT[0][r];
+      B.y += a.bextLUT[1][r];
+      B.z += a.bextLUT[2][r];
     }
-    if (a.frozenLUT && a.frozenLUT[a.regions ? a.regions[I] : 0] != T(0)) t = v3(T(0), T(0), T(0));
-    a.tau[0][I] = t.x;
-    a.tau[1][I] = t.y;
-    a.tau[2][I] = t.z;
+    if (a.has_bext) {
+      B.x += a.bext[0].at(I);
+      B.y += a.bext[1].at(I);
+      B.z += a.bext[2].at(I);
+    }
+    if (a.has_therm) {
+      const uint64_t stp = a.step_dev ? (uint64_t)*a.step_dev : a.step;
+      const V3<T> h = thermal_cell(I, a.seed, stp, a.stream, a.kth, a.Ms, a.temp, a.alpha);
+      B.x += h.x; B.y += h.y; B.z += h.z;
+    }
+    if (a.badd[0]) {   // AddCustomField (engine/effectivefield.go:23)
+      const V3<T> h = load3(a.badd, I);
+      B.x += h.x; B.y += h.y; B.z += h.z;
+    }
+    if (a.beff[0]) {
+      a.beff[0][I] = B.x;
+      a.beff[1][I] = B.y;
+      a.beff[2][I] = B.z;
+    }
+    if (a.tau[0]) {
+      V3<T> t;
+      if (a.precess) {
+        t = lltorque_cell(m0, B, a.alpha.at(I));
+      } else {
+        const V3<T> c = vcross(m0, vcross(m0, B));
+        t = v3(-c.x, -c.y, -c.z);
+      }
+      // AddSTTorque (engine/torque.go:147-196), before FreezeSpins
+      if (a.has_zl) {
+        const V3<T> z = zhangli_cell(g, ix, iy, iz, I, m0, a.m, a.Ms, a.alpha, a.stt, a.cx, a.cy, a.cz);
+        t.x += z.x; t.y += z.y; t.z += z.z;
+      }
+      if (a.has_sl) {
+        const V3<T> z = slonczewski_cell(I, m0, a.Ms, a.alpha, a.stt);
+        t.x += z.x; t.y += z.y; t.z += z.z;
+      }
+      if (a.tadd[0]) {   // AddCustomTorques (engine/torque.go:129)
+        const V3<T> z = load3(a.tadd, I);
+        t.x += z.x; t.y += z.y; t.z += z.z;
+      }
+      if (a.frozenLUT && a.frozenLUT[a.regions ? a.regions[I] : 0] != T(0)) t = v3(T(0), T(0), T(0));
+      a.tau[0][I] = t.x;
+      a.tau[1][I] = t.y;
+      a.tau[2][I] = t.z;
+    }
   }
-}
+};
+template <typename T>
+__global__ __launch_bounds__(256) void k_field_torque(FieldArgs<T> a) { BFieldTorque<T>::run(a, hw_bid(), hw_grid()); }
 
 // four x cells per thread; consecutive threads take consecutive x groups
 template <typename T, bool REG, int EX, bool STT, bool UNI>
-__global__ __launch_bounds__(256) void k_field_torque4(FieldArgs<T> a) {
-  // 32-bit indices: the launcher takes this kernel only below 2^31 groups
-  const uint32_t ngx = (uint32_t)a.g.Nx >> 2;
-  const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
-  const uint32_t rows = (uint32_t)a.g.Ny * (uint32_t)a.g.Nz;
-  if (gid >= rows * ngx) return;
-  const uint32_t row = gid / ngx;
-  const int gx = (int)(gid - row * ngx);
-  const int iz = (int)(row / (uint32_t)a.g.Ny), iy = (int)(row - (uint32_t)iz * (uint32_t)a.g.Ny);
-  const long I0 = a.g.idx(gx * 4, iy, iz);
-  T bdv[3][4];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    if (a.bd[0]) ld4(a.bd[c] + I0, bdv[c]);
-    else bdv[c][0] = bdv[c][1] = bdv[c][2] = bdv[c][3] = T(0);
+struct BFieldTorque4 {
+  using Args = FieldArgs<T>;
+  static constexpr int NT = 256, MINB = 1;
+  static __device__ __forceinline__ void run(const FieldArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    // 32-bit indices: the launcher takes this kernel only below 2^31 groups
+    const uint32_t ngx = (uint32_t)a.g.Nx >> 2;
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t rows = (uint32_t)a.g.Ny * (uint32_t)a.g.Nz;
+    if (gid >= rows * ngx) return;
+    const uint32_t row = gid / ngx;
+    const int gx = (int)(gid - row * ngx);
+    const int iz = (int)(row / (uint32_t)a.g.Ny), iy = (int)(row - (uint32_t)iz * (uint32_t)a.g.Ny);
+    const long I0 = a.g.idx(gx * 4, iy, iz);
+    T bdv[3][4];
+  #pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if (a.bd[0]) ld4(a.bd[c] + I0, bdv[c]);
+      else bdv[c][0] = bdv[c][1] = bdv[c][2] = bdv[c][3] = T(0);
+    }
+    field_group4<T, REG, EX, STT, UNI>(a, gx * 4, iy, iz, bdv);
   }
-  field_group4<T, REG, EX, STT, UNI>(a, gx * 4, iy, iz, bdv);
-}
+};
+template <typename T, bool REG, int EX, bool STT, bool UNI>
+__global__ __launch_bounds__(256) void k_field_torque4(FieldArgs<T> a) { BFieldTorque4<T, REG, EX, STT, UNI>::run(a, hw_bid(), hw_grid()); }
 
 // whether the four-cell kernel applies: FP32 (in FP64 the four cells need
 // twice the registers and the one-cell kernel keeps a higher occupancy),
@@ -300,9 +313,9 @@ void launch_field_torque(const FieldArgs<T> &a, hipStream_t s) {
     const bool uni = field_uniform(a);
 #define UM_F4(R, E)                                                                          \
   do {                                                                                       \
-    if (stt) UM_LAUNCH((k_field_torque4<T, R, E, true, false>), grid, dim3(256), 0, s, a);  \
-    else if (uni) UM_LAUNCH((k_field_torque4<T, R, E, false, true>), grid, dim3(256), 0, s, a); \
-    else UM_LAUNCH((k_field_torque4<T, R, E, false, false>), grid, dim3(256), 0, s, a);     \
+    if (stt) launch_b<BFieldTorque4<T, R, E, true, false>>(k_field_torque4<T, R, E, true, false>, grid, dim3(256), 0, s, a);  \
+    else if (uni) launch_b<BFieldTorque4<T, R, E, false, true>>(k_field_torque4<T, R, E, false, true>, grid, dim3(256), 0, s, a); \
+    else launch_b<BFieldTorque4<T, R, E, false, false>>(k_field_torque4<T, R, E, false, false>, grid, dim3(256), 0, s, a);     \
   } while (0)
     if constexpr (sizeof(T) == 4) {
       if (a.exch_mode == 0) { if (reg) UM_F4(true, 0); else UM_F4(false, 0); }
@@ -316,7 +329,7 @@ void launch_field_torque(const FieldArgs<T> &a, hipStream_t s) {
   }
   dim3 blk(64, 4, 1);
   dim3 grd((a.g.Nx + 63) / 64, (a.g.Ny + 3) / 4, a.g.Nz);
-  UM_LAUNCH(k_field_torque<T>, grd, blk, 0, s, a);
+  launch_b<BFieldTorque<T>>(k_field_torque<T>, grd, blk, 0, s, a);
   UM_HIP(hipGetLastError());
 }
 template void launch_field_torque<float>(const FieldArgs<float> &, hipStream_t);

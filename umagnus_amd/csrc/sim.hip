@@ -4,11 +4,13 @@
 // step-control norms with the error estimate, so a step reads back one
 // 24-byte result instead of several reductions.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
 #include "adjoint.hpp"
 #include "field.hpp"
+#include "batch.hpp"
 #include "sim.hpp"
 #include "vec.hpp"
 
@@ -27,6 +29,11 @@ __global__ void k_nodemag_vol(T *out, const T *vol, const T *lut, const uint8_t 
     const T v = vol ? vol[i] : T(1);
     out[i] = lut[regions ? regions[i] : 0] != T(0) ? T(0) : v;
   }
+}
+
+static uint64_t next_serial() {
+  static std::atomic<uint64_t> c{1};
+  return c++;
 }
 
 template <typename T>
@@ -104,6 +111,7 @@ struct Sim : SimBase {
   bool capturing = false;
   bool counted = false;     // the captured step's counter increment is queued
   long gen = 0;                  // bumped by every setter: a captured graph is stale
+  const uint64_t serial = next_serial();   // distinguishes handles that reuse an address
   DevBuf gstat;                  // {lastErr, peakErr, lastTorque, step counter}
   bool gstat_live = false;       // gstat holds this session's statistics (uploaded by graph_prepare)
   double *hstat = nullptr;       // pinned mirror
@@ -1161,7 +1169,15 @@ struct Sim : SimBase {
   // stale; returns how many two-step replays fit before `stop`
   // Every Steps/RunWhile starts without the FSAL torque (engine/run.go:207,
   // stepper.Free()), so the ensemble entry point and Steps agree
-  long graph_prepare(long stop) override {
+  long graph_prepare(long stop) override { return prepare_replays(stop, true); }
+  void graph_ensure() override {
+    const void *ptrs[8] = {m, m0, k[0], k[1], k[2], k[3], k[4], k[5]};
+    bool fresh = gexec && g_gen == gen && g_solver == solver && g_dt == fixdt;
+    for (int i = 0; i < 8 && fresh; i++) fresh = (g_ptrs[i] == ptrs[i]);
+    if (!fresh) capture2();
+  }
+  long batch_prepare(long stop) override { return prepare_replays(stop, false); }
+  long prepare_replays(long stop, bool capture) {
     k1_valid = false;
     if (!graph_usable() || nsteps >= stop) return 0;
     step();
@@ -1169,7 +1185,7 @@ struct Sim : SimBase {
     const void *ptrs[8] = {m, m0, k[0], k[1], k[2], k[3], k[4], k[5]};
     bool fresh = gexec && g_gen == gen && g_solver == solver && g_dt == fixdt;
     for (int i = 0; i < 8 && fresh; i++) fresh = (g_ptrs[i] == ptrs[i]);
-    if (!fresh) capture2();
+    if (capture && !fresh) capture2();
     fill_bytes(red.p, red.bytes, st);   // the replays' reductions start from cleared slots
     hstat[0] = lastErr; hstat[1] = peakErr; hstat[2] = lastTorque;
     const unsigned long long ns = (unsigned long long)nsteps;
@@ -1213,6 +1229,43 @@ struct Sim : SimBase {
     for (int i = 0; i < 2; i++) Time = Time + dt;
     nsteps += 2;
     nevals += g_evals;
+  }
+  // the launches of two captured-style steps, recorded (nothing runs)
+  long b_evals = 0;
+  void batch_record(std::vector<BatchEntry> &out) override {
+    UM_REQUIRE(graph_usable(), UM_EUNSUP, "batched steps need graph mode and FixDt");
+    ensure_demag();
+    const double t0 = Time;
+    const long s0 = nsteps, e0 = nevals, u0 = nundone;
+    BatchRec rec;
+    rec.cur = &out;
+    g_rec = &rec;
+    capturing = true;
+    try {
+      step();
+      step();
+    } catch (...) {
+      capturing = false;
+      g_rec = nullptr;
+      Time = t0; nsteps = s0; nevals = e0; nundone = u0;
+      throw;
+    }
+    capturing = false;
+    g_rec = nullptr;
+    b_evals = nevals - e0;
+    Time = t0; nsteps = s0; nevals = e0; nundone = u0;   // nothing ran
+  }
+  std::vector<uint64_t> batch_key() const override {
+    std::vector<uint64_t> k{serial, (uint64_t)gen, (uint64_t)solver, 0, (uint64_t)(uintptr_t)st};
+    std::memcpy(&k[3], &fixdt, 8);
+    const void *ptrs[8] = {m, m0, this->k[0], this->k[1], this->k[2], this->k[3], this->k[4], this->k[5]};
+    for (const void *q : ptrs) k.push_back((uint64_t)(uintptr_t)q);
+    return k;
+  }
+  void batch_advance() override {
+    for (int i = 0; i < 2; i++) Time = Time + dt;
+    nsteps += 2;
+    nevals += b_evals;
   }
   // device statistics back to the host, then the remaining steps eagerly
   void graph_finish(long stop) override {

@@ -90,6 +90,15 @@ struct SimBase {
   virtual void set_profiling(int on) = 0;
   virtual void set_graph(int on) = 0;
   virtual long graph_prepare(long stop) = 0;
+  // batched ensembles: as graph_prepare without capturing a graph; then the
+  // launches of two fixed-dt steps recorded instead of run (batch.hpp),
+  // a key that changes whenever such a recording would, and the host side
+  // of two replayed steps
+  virtual long batch_prepare(long stop) = 0;
+  virtual void graph_ensure() = 0;   // capture the two-step graph if it is stale
+  virtual void batch_record(std::vector<struct BatchEntry> &out) = 0;
+  virtual std::vector<uint64_t> batch_key() const = 0;
+  virtual void batch_advance() = 0;
   virtual void graph_replay() = 0;
   virtual void graph_finish(long stop) = 0;
   virtual long step_count() const = 0;

@@ -1,26 +1,43 @@
 // Vector arithmetic (opencl/madd.go, normalize.go, vecnorm.go), RK stage and
 // error kernels (engine/rk45dp.go, heun.go, rk4.go, stepcontroller.go) and
 // reductions (opencl/reduce.go) accumulated in FP64.
+#include "batch.hpp"
 #include "vec.hpp"
 
 namespace um {
 
 // ---- fills (kernel based, so they may sit inside captured graphs) ----------
-__global__ void k_fill_bytes(uint32_t *dst, size_t nwords) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = 0u;
-}
-__global__ void k_fill_tail(unsigned char *dst, size_t n) {
-  if (threadIdx.x < n) dst[threadIdx.x] = 0;
-}
+struct FillArgs {
+  unsigned char *dst;
+  size_t n;   // words (k_fill_bytes) or bytes (k_fill_tail)
+};
+struct BFillBytes {
+  using Args = FillArgs;
+  static constexpr int NT = 256, MINB = 1;
+  static __device__ __forceinline__ void run(const FillArgs &a, uint3 blockIdx, uint3 gridDim) {
+    uint32_t *dst = (uint32_t *)a.dst;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < a.n; i += (size_t)gridDim.x * blockDim.x)
+      dst[i] = 0u;
+  }
+};
+__global__ __launch_bounds__(256) void k_fill_bytes(FillArgs a) { BFillBytes::run(a, hw_bid(), hw_grid()); }
+struct BFillTail {
+  using Args = FillArgs;
+  static constexpr int NT = 64, MINB = 1;
+  static __device__ __forceinline__ void run(const FillArgs &a, uint3, uint3) {
+    if (threadIdx.x < a.n) a.dst[threadIdx.x] = 0;
+  }
+};
+__global__ __launch_bounds__(64) void k_fill_tail(FillArgs a) { BFillTail::run(a, hw_bid(), hw_grid()); }
 void fill_bytes(void *dst, size_t bytes, hipStream_t s) {
   const size_t nw = bytes / 4;
   if (nw) {
-    UM_LAUNCH(k_fill_bytes, dim3(blocks_for((long)nw, 256, 8192)), dim3(256), 0, s, (uint32_t *)dst, nw);
+    launch_b<BFillBytes>(k_fill_bytes, dim3(blocks_for((long)nw, 256, 8192)), dim3(256), 0, s,
+                (FillArgs{(unsigned char *)dst, nw}));
     UM_HIP(hipGetLastError());
   }
   if (bytes % 4) {
-    UM_LAUNCH(k_fill_tail, dim3(1), dim3(64), 0, s, (unsigned char *)dst + nw * 4, bytes % 4);
+    launch_b<BFillTail>(k_fill_tail, dim3(1), dim3(64), 0, s, (FillArgs{(unsigned char *)dst + nw * 4, bytes % 4}));
     UM_HIP(hipGetLastError());
   }
 }
@@ -41,38 +58,44 @@ template void fill_value<unsigned long long>(unsigned long long *, unsigned long
 
 // ---- stage: madd + normalize ---------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_stage(StageArgs<T> a) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
-    T r[3] = {T(0), T(0), T(0)};
-    const int nc = a.ncomp;
-    for (int c = 0; c < nc; c++) {
-      T acc;
-      if (a.group3 && a.nsrc == 3) {
-        acc = (a.fac[0] * a.src[0][c][i]) + (a.fac[1] * a.src[1][c][i] + a.fac[2] * a.src[2][c][i]);
-      } else {
-        acc = a.fac[0] * a.src[0][c][i];
-        for (int k = 1; k < a.nsrc; k++) acc = acc + a.fac[k] * a.src[k][c][i];
+struct BStage {
+  using Args = StageArgs<T>;
+  static constexpr int NT = 256, MINB = 1;
+  static __device__ __forceinline__ void run(const StageArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
+      T r[3] = {T(0), T(0), T(0)};
+      const int nc = a.ncomp;
+      for (int c = 0; c < nc; c++) {
+        T acc;
+        if (a.group3 && a.nsrc == 3) {
+          acc = (a.fac[0] * a.src[0][c][i]) + (a.fac[1] * a.src[1][c][i] + a.fac[2] * a.src[2][c][i]);
+        } else {
+          acc = a.fac[0] * a.src[0][c][i];
+          for (int k = 1; k < a.nsrc; k++) acc = acc + a.fac[k] * a.src[k][c][i];
+        }
+        r[c] = acc;
       }
-      r[c] = acc;
-    }
-    if (a.normalize && nc == 3) {
-      // normalize2.cl:8-23
-      const T v = a.vol ? a.vol[i] : T(1);
-      T X = v * r[0], Y = v * r[1], Z = v * r[2];
-      const T l = sqrt(X * X + Y * Y + Z * Z);
-      const T inv = (l != T(0)) ? T(1) / l : T(0);
-      if (v == T(0)) {
-        r[0] = r[1] = r[2] = T(0);
-      } else {
-        r[0] = X * inv;
-        r[1] = Y * inv;
-        r[2] = Z * inv;
+      if (a.normalize && nc == 3) {
+        // normalize2.cl:8-23
+        const T v = a.vol ? a.vol[i] : T(1);
+        T X = v * r[0], Y = v * r[1], Z = v * r[2];
+        const T l = sqrt(X * X + Y * Y + Z * Z);
+        const T inv = (l != T(0)) ? T(1) / l : T(0);
+        if (v == T(0)) {
+          r[0] = r[1] = r[2] = T(0);
+        } else {
+          r[0] = X * inv;
+          r[1] = Y * inv;
+          r[2] = Z * inv;
+        }
+        if (a.nrm) a.nrm[i] = l;
       }
-      if (a.nrm) a.nrm[i] = l;
+      for (int c = 0; c < nc; c++) a.dst[c][i] = r[c];
     }
-    for (int c = 0; c < nc; c++) a.dst[c][i] = r[c];
   }
-}
+};
+template <typename T>
+__global__ __launch_bounds__(256) void k_stage(StageArgs<T> a) { BStage<T>::run(a, hw_bid(), hw_grid()); }
 // Same arithmetic, 16 bytes per access: each thread owns W = 16/sizeof(T)
 // consecutive cells of every component, and the source count, component
 // count, madd3 grouping and normalisation are compile-time.
@@ -88,71 +111,77 @@ template <typename V>
 __device__ __forceinline__ void st_nt(void *base, long g, V v) { __builtin_nontemporal_store(v, (V *)base + g); }
 
 template <typename T, int NS, int NC, bool G3, bool NORM>
-__global__ __launch_bounds__(256) void k_stage_v(StageArgs<T> a) {
-  using V = typename Vec16<T>::type;
-  constexpr int W = Vec16<T>::n;
-  const long g = blockIdx.x * 256L + threadIdx.x;
-  if (g * W >= a.N) return;
-  T r[3][W];
-#pragma unroll
-  for (int c = 0; c < NC; c++) {
-    T x[NS][W];
-#pragma unroll
-    for (int k = 0; k < NS; k++) {
-      const V v = ld_nt<V>(a.src[k][c], g);
-      __builtin_memcpy(x[k], &v, sizeof(V));
-    }
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-      T acc;
-      if (G3) {
-        acc = (a.fac[0] * x[0][w]) + (a.fac[1] * x[1][w] + a.fac[2] * x[2][w]);
-      } else {
-        acc = a.fac[0] * x[0][w];
-#pragma unroll
-        for (int k = 1; k < NS; k++) acc = acc + a.fac[k] * x[k][w];
+struct BStageV {
+  using Args = StageArgs<T>;
+  static constexpr int NT = 256, MINB = 1;
+  static __device__ __forceinline__ void run(const StageArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    using V = typename Vec16<T>::type;
+    constexpr int W = Vec16<T>::n;
+    const long g = blockIdx.x * 256L + threadIdx.x;
+    if (g * W >= a.N) return;
+    T r[3][W];
+  #pragma unroll
+    for (int c = 0; c < NC; c++) {
+      T x[NS][W];
+  #pragma unroll
+      for (int k = 0; k < NS; k++) {
+        const V v = ld_nt<V>(a.src[k][c], g);
+        __builtin_memcpy(x[k], &v, sizeof(V));
       }
-      r[c][w] = acc;
-    }
-  }
-  if (NORM) {
-    T vv[W];
-    if (a.vol) {
-      const V v = ld_nt<V>(a.vol, g);
-      __builtin_memcpy(vv, &v, sizeof(V));
-    } else {
-#pragma unroll
-      for (int w = 0; w < W; w++) vv[w] = T(1);
-    }
-    T ln[W];
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-      const T v = vv[w];
-      const T X = v * r[0][w], Y = v * r[1][w], Z = v * r[2][w];
-      const T l = sqrt(X * X + Y * Y + Z * Z);
-      ln[w] = l;
-      const T inv = (l != T(0)) ? T(1) / l : T(0);
-      if (v == T(0)) {
-        r[0][w] = r[1][w] = r[2][w] = T(0);
-      } else {
-        r[0][w] = X * inv;
-        r[1][w] = Y * inv;
-        r[2][w] = Z * inv;
+  #pragma unroll
+      for (int w = 0; w < W; w++) {
+        T acc;
+        if (G3) {
+          acc = (a.fac[0] * x[0][w]) + (a.fac[1] * x[1][w] + a.fac[2] * x[2][w]);
+        } else {
+          acc = a.fac[0] * x[0][w];
+  #pragma unroll
+          for (int k = 1; k < NS; k++) acc = acc + a.fac[k] * x[k][w];
+        }
+        r[c][w] = acc;
       }
     }
-    if (a.nrm) {
+    if (NORM) {
+      T vv[W];
+      if (a.vol) {
+        const V v = ld_nt<V>(a.vol, g);
+        __builtin_memcpy(vv, &v, sizeof(V));
+      } else {
+  #pragma unroll
+        for (int w = 0; w < W; w++) vv[w] = T(1);
+      }
+      T ln[W];
+  #pragma unroll
+      for (int w = 0; w < W; w++) {
+        const T v = vv[w];
+        const T X = v * r[0][w], Y = v * r[1][w], Z = v * r[2][w];
+        const T l = sqrt(X * X + Y * Y + Z * Z);
+        ln[w] = l;
+        const T inv = (l != T(0)) ? T(1) / l : T(0);
+        if (v == T(0)) {
+          r[0][w] = r[1][w] = r[2][w] = T(0);
+        } else {
+          r[0][w] = X * inv;
+          r[1][w] = Y * inv;
+          r[2][w] = Z * inv;
+        }
+      }
+      if (a.nrm) {
+        V v;
+        __builtin_memcpy(&v, ln, sizeof(V));
+        st_nt<V>(a.nrm, g, v);
+      }
+    }
+  #pragma unroll
+    for (int c = 0; c < NC; c++) {
       V v;
-      __builtin_memcpy(&v, ln, sizeof(V));
-      st_nt<V>(a.nrm, g, v);
+      __builtin_memcpy(&v, r[c], sizeof(V));
+      st_nt<V>(a.dst[c], g, v);
     }
   }
-#pragma unroll
-  for (int c = 0; c < NC; c++) {
-    V v;
-    __builtin_memcpy(&v, r[c], sizeof(V));
-    st_nt<V>(a.dst[c], g, v);
-  }
-}
+};
+template <typename T, int NS, int NC, bool G3, bool NORM>
+__global__ __launch_bounds__(256) void k_stage_v(StageArgs<T> a) { BStageV<T, NS, NC, G3, NORM>::run(a, hw_bid(), hw_grid()); }
 
 template <typename T>
 static bool stage_vectorisable(const StageArgs<T> &a) {
@@ -180,14 +209,14 @@ void launch_stage(const StageArgs<T> &a, hipStream_t s) {
     const bool g3 = a.group3 && a.nsrc == 3, nrm = a.normalize && a.ncomp == 3;
 #define UM_STAGE(NS)                                                                          \
   case NS:                                                                                    \
-    if (a.ncomp == 1) UM_LAUNCH((k_stage_v<T, NS, 1, false, false>), grid, dim3(256), 0, s, a); \
-    else if (nrm) UM_LAUNCH((k_stage_v<T, NS, 3, false, true>), grid, dim3(256), 0, s, a);      \
-    else UM_LAUNCH((k_stage_v<T, NS, 3, false, false>), grid, dim3(256), 0, s, a);             \
+    if (a.ncomp == 1) launch_b<BStageV<T, NS, 1, false, false>>(k_stage_v<T, NS, 1, false, false>, grid, dim3(256), 0, s, a); \
+    else if (nrm) launch_b<BStageV<T, NS, 3, false, true>>(k_stage_v<T, NS, 3, false, true>, grid, dim3(256), 0, s, a);      \
+    else launch_b<BStageV<T, NS, 3, false, false>>(k_stage_v<T, NS, 3, false, false>, grid, dim3(256), 0, s, a);             \
     break;
     if (g3) {
-      if (a.ncomp == 1) UM_LAUNCH((k_stage_v<T, 3, 1, true, false>), grid, dim3(256), 0, s, a);
-      else if (nrm) UM_LAUNCH((k_stage_v<T, 3, 3, true, true>), grid, dim3(256), 0, s, a);
-      else UM_LAUNCH((k_stage_v<T, 3, 3, true, false>), grid, dim3(256), 0, s, a);
+      if (a.ncomp == 1) launch_b<BStageV<T, 3, 1, true, false>>(k_stage_v<T, 3, 1, true, false>, grid, dim3(256), 0, s, a);
+      else if (nrm) launch_b<BStageV<T, 3, 3, true, true>>(k_stage_v<T, 3, 3, true, true>, grid, dim3(256), 0, s, a);
+      else launch_b<BStageV<T, 3, 3, true, false>>(k_stage_v<T, 3, 3, true, false>, grid, dim3(256), 0, s, a);
     } else {
       switch (a.nsrc) {
         UM_STAGE(1) UM_STAGE(2) UM_STAGE(3) UM_STAGE(4) UM_STAGE(5) UM_STAGE(6) UM_STAGE(7) UM_STAGE(8)
@@ -197,7 +226,7 @@ void launch_stage(const StageArgs<T> &a, hipStream_t s) {
     UM_HIP(hipGetLastError());
     return;
   }
-  UM_LAUNCH(k_stage<T>, dim3(blocks_for(a.N, 256, 16384)), dim3(256), 0, s, a);
+  launch_b<BStage<T>>(k_stage<T>, dim3(blocks_for(a.N, 256, 16384)), dim3(256), 0, s, a);
   UM_HIP(hipGetLastError());
 }
 template void launch_stage<float>(const StageArgs<float> &, hipStream_t);
@@ -234,109 +263,121 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long *dst) 
 
 // ---- fused error reductions --------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_err(ErrArgs<T> a) {
-  double m0 = 0, m1 = 0, m2 = 0;
-  bool nan = false;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
-    if (a.mode == 0) {
-      T E[3], K7[3];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        // Madd6(Err, k1,k3,k4,k5,k6,k7, e...), engine/rk45dp.go:95
-        T acc = a.e[0] * a.k[0][c][i];
-#pragma unroll
-        for (int j = 1; j < 6; j++) acc = acc + a.e[j] * a.k[j][c][i];
-        E[c] = acc;
-        K7[c] = a.k[5][c][i];
-      }
-      const double e2 = (double)E[0] * E[0] + (double)E[1] * E[1] + (double)E[2] * E[2];
-      const double k2 = (double)K7[0] * K7[0] + (double)K7[1] * K7[1] + (double)K7[2] * K7[2];
-      // VecNorm(errnorm)/VecNorm(ddtnorm), 0 where |k7| = 0 (div.cl:9)
-      const double rel = (k2 == 0.0) ? 0.0 : sqrt(e2) / sqrt(k2);
-      nan |= (e2 != e2);
-      m0 = fmax(m0, e2);
-      m1 = fmax(m1, k2);
-      m2 = fmax(m2, rel);
-    } else if (a.mode == 1) {
-      const double dx = (double)(a.k[0][0][i] - a.k[1][0][i]);
-      const double dy = (double)(a.k[0][1][i] - a.k[1][1][i]);
-      const double dz = (double)(a.k[0][2][i] - a.k[1][2][i]);
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      nan |= (d2 != d2);
-      m0 = fmax(m0, d2);
-    } else {
-      const double x = a.k[0][0][i], y = a.k[0][1][i], z = a.k[0][2][i];
-      const double d2 = x * x + y * y + z * z;
-      nan |= (d2 != d2);
-      m0 = fmax(m0, d2);
-    }
-  }
-  if (nan) m0 = __longlong_as_double(0x7ff8000000000000LL);
-  block_max_to(m0, a.res);
-  if (a.mode == 0) {
-    block_max_to(m1, a.res + 1);
-    block_max_to(m2, a.res + 2);
-  }
-}
-// Same reductions with 16-byte accesses (W cells per access), mode compile-time.
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_err_v(ErrArgs<T> a) {
-  using V = typename Vec16<T>::type;
-  constexpr int W = Vec16<T>::n, NK = MODE == 0 ? 6 : MODE == 1 ? 2 : 1;
-  double m0 = 0, m1 = 0, m2 = 0;
-  bool nan = false;
-  const long groups = a.N / W;
-#pragma unroll 2
-  for (long g = blockIdx.x * 256L + threadIdx.x; g < groups; g += (long)gridDim.x * 256) {
-    T x[NK][3][W];
-#pragma unroll
-    for (int j = 0; j < NK; j++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const V v = ld_nt<V>(a.k[j][c], g);
-        __builtin_memcpy(x[j][c], &v, sizeof(V));
-      }
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-      if (MODE == 0) {
-        T E[3];
-#pragma unroll
+struct BErr {
+  using Args = ErrArgs<T>;
+  static constexpr int NT = 256, MINB = 1;
+  static __device__ __forceinline__ void run(const ErrArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    double m0 = 0, m1 = 0, m2 = 0;
+    bool nan = false;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
+      if (a.mode == 0) {
+        T E[3], K7[3];
+  #pragma unroll
         for (int c = 0; c < 3; c++) {
-          T acc = a.e[0] * x[0][c][w];
-#pragma unroll
-          for (int j = 1; j < 6; j++) acc = acc + a.e[j] * x[j][c][w];
+          // Madd6(Err, k1,k3,k4,k5,k6,k7, e...), engine/rk45dp.go:95
+          T acc = a.e[0] * a.k[0][c][i];
+  #pragma unroll
+          for (int j = 1; j < 6; j++) acc = acc + a.e[j] * a.k[j][c][i];
           E[c] = acc;
+          K7[c] = a.k[5][c][i];
         }
         const double e2 = (double)E[0] * E[0] + (double)E[1] * E[1] + (double)E[2] * E[2];
-        const double k2 = (double)x[5][0][w] * x[5][0][w] + (double)x[5][1][w] * x[5][1][w] +
-                          (double)x[5][2][w] * x[5][2][w];
+        const double k2 = (double)K7[0] * K7[0] + (double)K7[1] * K7[1] + (double)K7[2] * K7[2];
+        // VecNorm(errnorm)/VecNorm(ddtnorm), 0 where |k7| = 0 (div.cl:9)
         const double rel = (k2 == 0.0) ? 0.0 : sqrt(e2) / sqrt(k2);
         nan |= (e2 != e2);
         m0 = fmax(m0, e2);
         m1 = fmax(m1, k2);
         m2 = fmax(m2, rel);
-      } else if (MODE == 1) {
-        const double dx = (double)(x[0][0][w] - x[1][0][w]);
-        const double dy = (double)(x[0][1][w] - x[1][1][w]);
-        const double dz = (double)(x[0][2][w] - x[1][2][w]);
+      } else if (a.mode == 1) {
+        const double dx = (double)(a.k[0][0][i] - a.k[1][0][i]);
+        const double dy = (double)(a.k[0][1][i] - a.k[1][1][i]);
+        const double dz = (double)(a.k[0][2][i] - a.k[1][2][i]);
         const double d2 = dx * dx + dy * dy + dz * dz;
         nan |= (d2 != d2);
         m0 = fmax(m0, d2);
       } else {
-        const double X = x[0][0][w], Y = x[0][1][w], Z = x[0][2][w];
-        const double d2 = X * X + Y * Y + Z * Z;
+        const double x = a.k[0][0][i], y = a.k[0][1][i], z = a.k[0][2][i];
+        const double d2 = x * x + y * y + z * z;
         nan |= (d2 != d2);
         m0 = fmax(m0, d2);
       }
     }
+    if (nan) m0 = __longlong_as_double(0x7ff8000000000000LL);
+    block_max_to(m0, a.res);
+    if (a.mode == 0) {
+      block_max_to(m1, a.res + 1);
+      block_max_to(m2, a.res + 2);
+    }
   }
-  if (nan) m0 = __longlong_as_double(0x7ff8000000000000LL);
-  block_max_to(m0, a.res);
-  if (MODE == 0) {
-    block_max_to(m1, a.res + 1);
-    block_max_to(m2, a.res + 2);
+};
+template <typename T>
+__global__ __launch_bounds__(256) void k_err(ErrArgs<T> a) { BErr<T>::run(a, hw_bid(), hw_grid()); }
+// Same reductions with 16-byte accesses (W cells per access), mode compile-time.
+template <typename T, int MODE>
+struct BErrV {
+  using Args = ErrArgs<T>;
+  static constexpr int NT = 256, MINB = 1;
+  static __device__ __forceinline__ void run(const ErrArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    using V = typename Vec16<T>::type;
+    constexpr int W = Vec16<T>::n, NK = MODE == 0 ? 6 : MODE == 1 ? 2 : 1;
+    double m0 = 0, m1 = 0, m2 = 0;
+    bool nan = false;
+    const long groups = a.N / W;
+  #pragma unroll 2
+    for (long g = blockIdx.x * 256L + threadIdx.x; g < groups; g += (long)gridDim.x * 256) {
+      T x[NK][3][W];
+  #pragma unroll
+      for (int j = 0; j < NK; j++)
+  #pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const V v = ld_nt<V>(a.k[j][c], g);
+          __builtin_memcpy(x[j][c], &v, sizeof(V));
+        }
+  #pragma unroll
+      for (int w = 0; w < W; w++) {
+        if (MODE == 0) {
+          T E[3];
+  #pragma unroll
+          for (int c = 0; c < 3; c++) {
+            T acc = a.e[0] * x[0][c][w];
+  #pragma unroll
+            for (int j = 1; j < 6; j++) acc = acc + a.e[j] * x[j][c][w];
+            E[c] = acc;
+          }
+          const double e2 = (double)E[0] * E[0] + (double)E[1] * E[1] + (double)E[2] * E[2];
+          const double k2 = (double)x[5][0][w] * x[5][0][w] + (double)x[5][1][w] * x[5][1][w] +
+                            (double)x[5][2][w] * x[5][2][w];
+          const double rel = (k2 == 0.0) ? 0.0 : sqrt(e2) / sqrt(k2);
+          nan |= (e2 != e2);
+          m0 = fmax(m0, e2);
+          m1 = fmax(m1, k2);
+          m2 = fmax(m2, rel);
+        } else if (MODE == 1) {
+          const double dx = (double)(x[0][0][w] - x[1][0][w]);
+          const double dy = (double)(x[0][1][w] - x[1][1][w]);
+          const double dz = (double)(x[0][2][w] - x[1][2][w]);
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          nan |= (d2 != d2);
+          m0 = fmax(m0, d2);
+        } else {
+          const double X = x[0][0][w], Y = x[0][1][w], Z = x[0][2][w];
+          const double d2 = X * X + Y * Y + Z * Z;
+          nan |= (d2 != d2);
+          m0 = fmax(m0, d2);
+        }
+      }
+    }
+    if (nan) m0 = __longlong_as_double(0x7ff8000000000000LL);
+    block_max_to(m0, a.res);
+    if (MODE == 0) {
+      block_max_to(m1, a.res + 1);
+      block_max_to(m2, a.res + 2);
+    }
   }
-}
+};
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_err_v(ErrArgs<T> a) { BErrV<T, MODE>::run(a, hw_bid(), hw_grid()); }
 
 template <typename T>
 void launch_err(const ErrArgs<T> &a, hipStream_t s) {
@@ -347,13 +388,13 @@ void launch_err(const ErrArgs<T> &a, hipStream_t s) {
     for (int c = 0; c < 3; c++) vec = vec && ((uintptr_t)a.k[j][c] & 15) == 0;
   if (vec) {
     const dim3 grid(blocks_for(a.N / W, 256, 2048));
-    if (a.mode == 0) UM_LAUNCH((k_err_v<T, 0>), grid, dim3(256), 0, s, a);
-    else if (a.mode == 1) UM_LAUNCH((k_err_v<T, 1>), grid, dim3(256), 0, s, a);
-    else UM_LAUNCH((k_err_v<T, 2>), grid, dim3(256), 0, s, a);
+    if (a.mode == 0) launch_b<BErrV<T, 0>>(k_err_v<T, 0>, grid, dim3(256), 0, s, a);
+    else if (a.mode == 1) launch_b<BErrV<T, 1>>(k_err_v<T, 1>, grid, dim3(256), 0, s, a);
+    else launch_b<BErrV<T, 2>>(k_err_v<T, 2>, grid, dim3(256), 0, s, a);
     UM_HIP(hipGetLastError());
     return;
   }
-  UM_LAUNCH(k_err<T>, dim3(blocks_for(a.N, 256, 2048)), dim3(256), 0, s, a);
+  launch_b<BErr<T>>(k_err<T>, dim3(blocks_for(a.N, 256, 2048)), dim3(256), 0, s, a);
   UM_HIP(hipGetLastError());
 }
 template void launch_err<float>(const ErrArgs<float> &, hipStream_t);
@@ -364,30 +405,49 @@ template void launch_err<double>(const ErrArgs<double> &, hipStream_t);
 // last torque, step counter as u64 bits in gs[3]) and clears the 4 reduction
 // slots for the next reduction, so a captured step needs no fill launches;
 // count: this is the step's last statistics launch.
-__global__ void k_step_stats(unsigned long long *res, double *gs, double h, int mode, int count) {
-  if (threadIdx.x != 0) return;
-  const double r0 = __longlong_as_double((long long)res[0]);
-  const double r1 = __longlong_as_double((long long)res[1]);
-  for (int i = 0; i < 4; i++) res[i] = 0ull;
-  if (count) *(unsigned long long *)(gs + 3) += 1ull;
-  if (mode == 2) {
-    gs[2] = sqrt(r0);
-    return;
+struct StatsArgs {
+  unsigned long long *res;
+  double *gs;
+  double h;
+  int mode, count;
+};
+struct BStepStats {
+  using Args = StatsArgs;
+  static constexpr int NT = 64, MINB = 1;
+  static __device__ __forceinline__ void run(const StatsArgs &a, uint3, uint3) {
+    if (threadIdx.x != 0) return;
+    unsigned long long *res = a.res;
+    double *gs = a.gs;
+    const int mode = a.mode;
+    const double r0 = __longlong_as_double((long long)res[0]);
+    const double r1 = __longlong_as_double((long long)res[1]);
+    for (int i = 0; i < 4; i++) res[i] = 0ull;
+    if (a.count) *(unsigned long long *)(gs + 3) += 1ull;
+    if (mode == 2) {
+      gs[2] = sqrt(r0);
+      return;
+    }
+    const double err = sqrt(mode == 3 ? 3.0 * r0 : r0) * a.h;
+    gs[0] = err;
+    if (err > gs[1]) gs[1] = err;
+    if (mode == 0) gs[2] = sqrt(r1);
   }
-  const double err = sqrt(mode == 3 ? 3.0 * r0 : r0) * h;
-  gs[0] = err;
-  if (err > gs[1]) gs[1] = err;
-  if (mode == 0) gs[2] = sqrt(r1);
-}
-__global__ void k_step_count(unsigned long long *counter) {
-  if (threadIdx.x == 0) *counter += 1ull;
-}
+};
+__global__ __launch_bounds__(64) void k_step_stats(StatsArgs a) { BStepStats::run(a, hw_bid(), hw_grid()); }
+struct BStepCount {
+  using Args = StatsArgs;
+  static constexpr int NT = 64, MINB = 1;
+  static __device__ __forceinline__ void run(const StatsArgs &a, uint3, uint3) {
+    if (threadIdx.x == 0) *a.res += 1ull;
+  }
+};
+__global__ __launch_bounds__(64) void k_step_count(StatsArgs a) { BStepCount::run(a, hw_bid(), hw_grid()); }
 void launch_step_stats(unsigned long long *res, double *gs, double h, int mode, int count, hipStream_t s) {
-  UM_LAUNCH(k_step_stats, dim3(1), dim3(64), 0, s, res, gs, h, mode, count);
+  launch_b<BStepStats>(k_step_stats, dim3(1), dim3(64), 0, s, (StatsArgs{res, gs, h, mode, count}));
   UM_HIP(hipGetLastError());
 }
 void launch_step_count(unsigned long long *counter, hipStream_t s) {
-  UM_LAUNCH(k_step_count, dim3(1), dim3(64), 0, s, counter);
+  launch_b<BStepCount>(k_step_count, dim3(1), dim3(64), 0, s, (StatsArgs{counter, nullptr, 0.0, 0, 0}));
   UM_HIP(hipGetLastError());
 }
 

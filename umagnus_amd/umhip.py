@@ -47,7 +47,8 @@ EXPORTS = [
     "um_sim_get_m", "um_sim_m_device", "um_sim_beff", "um_sim_torque", "um_sim_demag_field",
     "um_sim_set_solver", "um_sim_set_solver_params", "um_sim_get_state", "um_sim_step", "um_sim_steps",
     "um_sim_run", "um_sim_average_m", "um_sim_energy", "um_sim_stream", "um_sim_set_profiling",
-    "um_sim_kernel_times", "um_sim_set_graph", "um_sims_steps", "um_graph_begin", "um_graph_end",
+    "um_sim_kernel_times", "um_sim_set_graph", "um_sims_steps", "um_sims_steps_batched", "um_graph_begin",
+    "um_graph_end",
     "um_graph_launch", "um_graph_destroy", "um_sim_relax", "um_trace_kernels", "um_traced_kernels",
     "um_field_torque", "um_rk45dp_stage", "um_rk45dp_error", "um_rk45dp_error_noalloc",
     "um_sim_add_field_term", "um_sim_clear_field_terms",
@@ -815,6 +816,17 @@ def sims_steps(sims, n):
     their streams (um_sims_steps)."""
     arr = (C.c_void_p * len(sims))(*[s.h for s in sims])
     _call("um_sims_steps", len(sims), arr, int(n))
+
+
+def sims_steps_batched(sims, n):
+    """n steps of every simulation in `sims` in batched launches (one launch per
+    kernel variant and phase for the whole ensemble, um_sims_steps_batched);
+    returns the launches that replay two steps of the ensemble (0: the members
+    were replayed one graph each)."""
+    arr = (C.c_void_p * len(sims))(*[s.h for s in sims])
+    nl = C.c_int(0)
+    _call("um_sims_steps_batched", len(sims), arr, int(n), C.byref(nl))
+    return nl.value
 
 
 class Graph:
