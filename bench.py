@@ -397,7 +397,9 @@ def strip_run():
     """The 128x32x1 strip of 3.90625 x 3.90625 x 3 nm cells: Ms 8e5, Aex 1.3e-11,
     alpha 0.02, m = (1,1,0)/sqrt2 plus a seeded +-0.1 perturbation, B_ext =
     (-24.6, 4.3, 0) mT, adaptive RK45DP (MaxErr 1e-5) for 1 ns, FP32 and FP64:
-    wall time of the run (latency-bound: 4096 cells), steps and <m>(1 ns)."""
+    wall time of the run (latency-bound: 4096 cells), steps and <m>(1 ns),
+    with the step controller on the host (one readback per step) and on the
+    device (graph mode: 8-step graph replays, one readback per replay)."""
     import torch
     from umagnus_amd import umhip
     from umagnus_amd.params import exchange_lut
@@ -409,8 +411,10 @@ def strip_run():
     m /= np.linalg.norm(m, axis=0)
     out = {"workload": "128x32x1 cells of 3.90625x3.90625x3 nm, Ms 8e5, Aex 1.3e-11, alpha 0.02, "
                        "B_ext (-24.6, 4.3, 0) mT, RK45DP MaxErr 1e-5, 1 ns"}
-    for name, dt in (("f32", np.float32), ("f64", np.float64)):
+    for name, dt, graph in (("f32", np.float32, 0), ("f64", np.float64, 0), ("f32_device_control", np.float32, 1),
+                            ("f64_device_control", np.float64, 1)):
         s = umhip.Sim(N, cs, dtype=dt)
+        s.set_graph(graph)
         s.set_scalar(umhip.P_MSAT, 8e5)
         s.set_scalar(umhip.P_ALPHA, 0.02)
         s.set_vector(umhip.P_BEXT, (-24.6e-3, 4.3e-3, 0.0))

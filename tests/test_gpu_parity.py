@@ -835,3 +835,53 @@ def test_dmi_bloch_to_neel_relax(gpu):
     s.relax()
     got = s.average_m()
     assert abs(got[0] - 0.059) < 0.01 and abs(got[1] - 0.035) < 0.01, got
+
+
+# ---- device-controlled adaptive steps -------------------------------------------------------
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("case", ["strip", "film_rejects"])
+def test_device_controlled_run_matches_host(gpu, oracle, prec, case):
+    """Adaptive RK45DP run() with the acceptance, the next dt and the FSAL copy
+    decided on the device (graph mode: 8-step graph replays, one readback per
+    replay) against the same run controlled from the host (one readback per
+    step): same accepted and rejected step counts, same final time, <m> and the
+    per-cell m within rounding (the controller's pow() is the device's, not
+    the host's), and <m> within 1e-4 of the FP64 oracle.  film_rejects starts
+    from a dt far too large, so the first steps are rejected."""
+    dtype = np.float32 if prec == 0 else np.float64
+    if case == "strip":
+        N, cs = (128, 32, 1), (3.90625e-9, 3.90625e-9, 3e-9)
+        p = dict(Ms=8e5, Aex=1.3e-11, alpha=0.02, Bext=(-24.6e-3, 4.3e-3, 0.0))
+        m, dt0, T = strip_m(1), 1e-15, 0.05e-9
+    else:
+        N, cs = (32, 24, 6), (2e-9,) * 3
+        p = dict(Ms=1.1e6, Aex=1.6e-11, Dind=3e-3, Ku1=8e5, alpha=0.1)
+        m, dt0, T = random_unit((6, 24, 32), 31), 2e-12, 5e-12
+    runs = {}
+    for graph in (0, 1):
+        s = gpu.Sim(N, cs, dtype=dtype)
+        setup(s, gpu, dtype=dtype, **p)
+        s.set_m(m)
+        s.set_solver_params(dt=dt0)
+        s.set_graph(graph)
+        s.run(T)
+        runs[graph] = (s.state(), s.get_m(), s.average_m())
+        _say(case, prec, "graph", graph, runs[graph][0])
+    (s0, m0, a0), (s1, m1, a1) = runs[0], runs[1]
+    assert s1["nsteps"] == s0["nsteps"] and s1["nundone"] == s0["nundone"] and s1["nevals"] == s0["nevals"]
+    if case == "film_rejects":
+        assert s0["nundone"] > 0
+    assert abs(s1["t"] - s0["t"]) <= 1e-12 * s0["t"] and abs(s1["t"] - T) <= 1e-9 * T
+    tol = 1e-12 if prec == 1 else 1e-6
+    assert np.abs(a1 - a0).max() <= tol, (a0, a1)
+    assert np.abs(m1.astype(np.float64) - m0).max() <= (1e-10 if prec == 1 else 1e-5)
+    for key in ("lasterr", "peakerr", "lasttorque"):
+        assert abs(s1[key] - s0[key]) <= 1e-6 * max(abs(s0[key]), 1e-300), key
+    so = oracle.Sim(N, cs, dtype=np.float64)
+    setup(so, oracle, dtype=np.float64, **p)
+    so.set_m(m)
+    so.set_solver_params(dt=dt0)
+    so.run(T)
+    d = float(np.abs(a1 - so.average_m()).max())
+    _say("device-controlled vs oracle <m>", d)
+    assert d <= 1e-4

@@ -115,6 +115,15 @@ struct Sim : SimBase {
   DevBuf gstat;                  // {lastErr, peakErr, lastTorque, step counter}
   bool gstat_live = false;       // gstat holds this session's statistics (uploaded by graph_prepare)
   double *hstat = nullptr;       // pinned mirror
+  // device-controlled adaptive RK45DP (graph mode, run()): the step state on
+  // the device, its pinned mirror, and the captured ADAPT_K-step graph
+  static constexpr int ADAPT_K = 8;
+  bool devctl = false;           // steps are being captured in device-controlled form
+  DevBuf astate;
+  AdaptState *hastate = nullptr;
+  hipGraphExec_t agexec = nullptr;
+  long a_gen = -1;
+  const void *a_ptrs[8] = {};
   hipGraphExec_t gexec = nullptr;
   long g_gen = -1;
   int g_solver = -1;
@@ -172,6 +181,8 @@ struct Sim : SimBase {
     if (st) (void)hipStreamSynchronize(st);
     if (demag) demag_destroy(demag);
     if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (agexec) (void)hipGraphExecDestroy(agexec);
+    if (hastate) (void)hipHostFree(hastate);
     if (hstat) (void)hipHostFree(hstat);
     if (hres) (void)hipHostFree(hres);
     if (hsum) (void)hipHostFree(hsum);
@@ -639,6 +650,12 @@ struct Sim : SimBase {
     for (int c = 0; c < 3; c++) a.dst[c] = dst + c * n;
     a.vol = (const T *)vol.p;
     a.N = n;
+    if (devctl) {
+      AdaptState *d = astate.as<AdaptState>();
+      a.dev_dt = &d->dt;
+      a.gam = gammaLL;
+      a.skip = &d->done;
+    }
     launch_stage(a, st);
   }
   // device statistics of a captured step (count: its last statistics launch)
@@ -676,7 +693,8 @@ struct Sim : SimBase {
     if (nz(UM_P_TEMP)) torque_fn(k1);
     const double t0 = Time;
     std::swap(m, m0);   // m0 <- current magnetization; m is overwritten by every stage
-    const T h = (T)(dt * gammaLL);
+    // device-controlled: the stage kernels multiply T(c) by the step's h
+    const T h = devctl ? T(1) : (T)(dt * gammaLL);
     T *k2 = k[1], *k3 = k[2], *k4 = k[3], *k5 = k[4], *k6 = k[5];
     {
       const T *s[] = {m0, k1};
@@ -730,6 +748,12 @@ struct Sim : SimBase {
     const T *src[6] = {k1, k3, k4, k5, k6, k7};
     double r[3];
     err_reduce(0, src, 6, e, r);
+    if (devctl) {   // accept / reject, the next dt and the FSAL copy on the device
+      AdaptPar ap{maxerr, relerr, minslope, headroom, mindt, maxdt, gammaLL, 6, sizeof(T) == 4 ? 1 : 0};
+      launch_adapt<T>((unsigned long long *)red.p, astate.as<AdaptState>(), ap, m, m0, k1, k7, 3 * n, st);
+      counted = true;
+      return;
+    }
     const double err = std::sqrt(r[0]) * (double)h;
     const double maxdm = std::sqrt(r[1]);
     if (err < maxerr || dt <= mindt || fixdt != 0) {
@@ -1283,8 +1307,76 @@ struct Sim : SimBase {
     k1_valid = false;
     const double stop = Time + seconds;
     alarm = stop;
+    if (devctl_usable() && Time < stop) {
+      run_device(stop);
+      return;
+    }
     while (Time < stop) step();
     UM_HIP(hipStreamSynchronize(st));
+  }
+  // adaptive RK45DP in graph mode: accept / reject and the next dt decided on
+  // the device, ADAPT_K steps per graph replay, one readback per replay
+  bool devctl_usable() const {
+    return graph_mode && solver == UM_SOLVER_RK45DP && fixdt == 0 && !profiling && !tape_on && !any_tfn &&
+           !xterm_tfn && !nz(UM_P_TEMP);
+  }
+  void run_device(double stop) {
+    if (!k1_valid) { torque_fn(k[0]); k1_valid = true; }   // RunWhile starts without the FSAL torque
+    if (!hastate) {
+      astate.alloc(sizeof(AdaptState));
+      UM_HIP(hipHostMalloc((void **)&hastate, sizeof(AdaptState), hipHostMallocDefault));
+    }
+    AdaptState h0{};
+    h0.dt = dt; h0.time = Time; h0.last_err = lastErr; h0.peak_err = peakErr; h0.last_torque = lastTorque;
+    h0.stop = stop; h0.nsteps = nsteps; h0.nundone = nundone; h0.nevals = nevals;
+    *hastate = h0;
+    fill_bytes(red.p, red.bytes, st);
+    UM_HIP(hipMemcpyAsync(astate.p, hastate, sizeof(AdaptState), hipMemcpyHostToDevice, st));
+    const void *ptrs[8] = {m, m0, k[0], k[1], k[2], k[3], k[4], k[5]};
+    bool fresh = agexec && a_gen == gen;
+    for (int i = 0; i < 8 && fresh; i++) fresh = (a_ptrs[i] == ptrs[i]);
+    if (!fresh) capture_adaptive();
+    do {
+      UM_HIP(hipGraphLaunch(agexec, st));
+      UM_HIP(hipMemcpyAsync(hastate, astate.p, sizeof(AdaptState), hipMemcpyDeviceToHost, st));
+      UM_HIP(hipStreamSynchronize(st));
+    } while (!hastate->done);
+    dt = hastate->dt; Time = hastate->time; lastErr = hastate->last_err; peakErr = hastate->peak_err;
+    lastTorque = hastate->last_torque; nsteps = (long)hastate->nsteps; nundone = (long)hastate->nundone;
+    nevals = (long)hastate->nevals;
+    UM_REQUIRE(!hastate->error, UM_ESTATE, "Time step too small, check if parameters are sensible");
+  }
+  void capture_adaptive() {
+    if (agexec) { UM_HIP(hipGraphExecDestroy(agexec)); agexec = nullptr; }
+    ensure_demag();
+    UM_HIP(hipStreamSynchronize(st));
+    const double t0 = Time, dt0 = dt;
+    const long s0 = nsteps, e0 = nevals, u0 = nundone;
+    hipGraph_t g = nullptr;
+    UM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    capturing = true;
+    devctl = true;
+    try {
+      for (int i = 0; i < ADAPT_K; i++) step();
+    } catch (...) {
+      capturing = false;
+      devctl = false;
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(st, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      Time = t0; dt = dt0; nsteps = s0; nevals = e0; nundone = u0;
+      throw;
+    }
+    capturing = false;
+    devctl = false;
+    UM_HIP(hipStreamEndCapture(st, &g));
+    hipError_t e = hipGraphInstantiate(&agexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    UM_HIP(e);
+    Time = t0; dt = dt0; nsteps = s0; nevals = e0; nundone = u0;   // capture executed nothing
+    a_gen = gen;
+    const void *ptrs[8] = {m, m0, k[0], k[1], k[2], k[3], k[4], k[5]};
+    for (int i = 0; i < 8; i++) a_ptrs[i] = ptrs[i];
   }
 
   // ---- outputs -----------------------------------------------------------------------

@@ -62,16 +62,19 @@ struct BStage {
   using Args = StageArgs<T>;
   static constexpr int NT = 256, MINB = 1;
   static __device__ __forceinline__ void run(const StageArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
+    if (a.skip && *a.skip) return;
+    const T h = a.dev_dt ? (T)(a.dev_dt[0] * a.gam) : T(1);
+    auto fac = [&](int k) { return (a.dev_dt && k > 0) ? a.fac[k] * h : a.fac[k]; };
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
       T r[3] = {T(0), T(0), T(0)};
       const int nc = a.ncomp;
       for (int c = 0; c < nc; c++) {
         T acc;
         if (a.group3 && a.nsrc == 3) {
-          acc = (a.fac[0] * a.src[0][c][i]) + (a.fac[1] * a.src[1][c][i] + a.fac[2] * a.src[2][c][i]);
+          acc = (fac(0) * a.src[0][c][i]) + (fac(1) * a.src[1][c][i] + fac(2) * a.src[2][c][i]);
         } else {
-          acc = a.fac[0] * a.src[0][c][i];
-          for (int k = 1; k < a.nsrc; k++) acc = acc + a.fac[k] * a.src[k][c][i];
+          acc = fac(0) * a.src[0][c][i];
+          for (int k = 1; k < a.nsrc; k++) acc = acc + fac(k) * a.src[k][c][i];
         }
         r[c] = acc;
       }
@@ -119,6 +122,15 @@ struct BStageV {
     constexpr int W = Vec16<T>::n;
     const long g = blockIdx.x * 256L + threadIdx.x;
     if (g * W >= a.N) return;
+    if (a.skip && *a.skip) return;
+    T fac[8];
+  #pragma unroll
+    for (int k = 0; k < 8; k++) fac[k] = a.fac[k];
+    if (a.dev_dt) {
+      const T h = (T)(a.dev_dt[0] * a.gam);
+  #pragma unroll
+      for (int k = 1; k < 8; k++) fac[k] = a.fac[k] * h;
+    }
     T r[3][W];
   #pragma unroll
     for (int c = 0; c < NC; c++) {
@@ -132,11 +144,11 @@ struct BStageV {
       for (int w = 0; w < W; w++) {
         T acc;
         if (G3) {
-          acc = (a.fac[0] * x[0][w]) + (a.fac[1] * x[1][w] + a.fac[2] * x[2][w]);
+          acc = (fac[0] * x[0][w]) + (fac[1] * x[1][w] + fac[2] * x[2][w]);
         } else {
-          acc = a.fac[0] * x[0][w];
+          acc = fac[0] * x[0][w];
   #pragma unroll
-          for (int k = 1; k < NS; k++) acc = acc + a.fac[k] * x[k][w];
+          for (int k = 1; k < NS; k++) acc = acc + fac[k] * x[k][w];
         }
         r[c][w] = acc;
       }
@@ -450,6 +462,107 @@ void launch_step_count(unsigned long long *counter, hipStream_t s) {
   launch_b<BStepCount>(k_step_count, dim3(1), dim3(64), 0, s, (StatsArgs{counter, nullptr, 0.0, 0, 0}));
   UM_HIP(hipGetLastError());
 }
+
+// ---- device-side control of adaptive RK45DP steps -------------------------------------
+// integralController (engine/stepcontroller.go:35-97) and adaptDt
+// (engine/run.go:150-187) with FixDt = 0 and the alarm at the run's end,
+// restated from Sim::step_rk45dp / Sim::adapt: one thread.
+struct BAdapt {
+  struct Args {
+    unsigned long long *res;
+    AdaptState *st;
+    AdaptPar p;
+  };
+  static constexpr int NT = 64, MINB = 1;
+  static __device__ __forceinline__ void run(const Args &a, uint3, uint3) {
+    if (threadIdx.x != 0) return;
+    AdaptState *st = a.st;
+    const AdaptPar &p = a.p;
+    const double r0 = __longlong_as_double((long long)a.res[0]);
+    const double r1 = __longlong_as_double((long long)a.res[1]);
+    const double r2 = __longlong_as_double((long long)a.res[2]);
+    for (int i = 0; i < 4; i++) a.res[i] = 0ull;
+    st->accepted = 0;
+    if (st->done) return;
+    st->nevals += p.evals_per_step;
+    const double dt = st->dt, t0 = st->time;
+    const double h = p.h_f32 ? (double)(float)(dt * p.gam) : dt * p.gam;
+    const double err = sqrt(r0) * h, maxdm = sqrt(r1);
+    double corr;
+    if (err < p.maxerr || dt <= p.mindt) {
+      int fail = 0;
+      double rlerr = 0;
+      if (!(maxdm < p.minslope)) { rlerr = r2; fail = 1; }
+      if (fail == 0 || p.relerr <= 0.0 || rlerr < p.relerr || dt <= p.mindt) {
+        st->last_err = err;
+        if (err > st->peak_err) st->peak_err = err;
+        st->last_torque = maxdm;
+        st->nsteps++;
+        st->time = t0 + dt;
+        st->accepted = 1;
+        corr = fail == 0 ? pow(p.maxerr / err, 1. / 5) : pow(p.relerr / rlerr, 1. / 5);
+      } else {
+        st->nundone++;
+        corr = pow(p.relerr / rlerr, 1. / 6);
+      }
+    } else {
+      st->nundone++;
+      corr = pow(p.maxerr / err, 1. / 6);
+    }
+    // adaptDt
+    if (isnan(corr)) corr = 1;
+    if (corr == 0) { st->error = 1; st->done = 1; return; }
+    corr *= p.headroom;
+    if (corr > 2) corr = 2;
+    if (corr < 0.5) corr = 0.5;
+    double ndt = dt * corr;
+    if (p.mindt != 0 && ndt < p.mindt) ndt = p.mindt;
+    if (p.maxdt != 0 && ndt > p.maxdt) ndt = p.maxdt;
+    if (ndt == 0) { st->error = 1; st->done = 1; return; }
+    if (st->time < st->stop && st->time + ndt > st->stop) ndt = st->stop - st->time;
+    if (!(ndt > 0)) { st->error = 1; st->done = 1; return; }
+    st->dt = ndt;
+    if (st->accepted && !(st->time < st->stop)) st->done = 1;
+  }
+};
+__global__ __launch_bounds__(64) void k_adapt(BAdapt::Args a) { BAdapt::run(a, hw_bid(), hw_grid()); }
+
+// k1 <- k7 after an accepted step, m <- m0 otherwise (rejected, or past the end)
+template <typename T>
+struct BAdaptCopy {
+  struct Args {
+    const AdaptState *st;
+    T *m;
+    const T *m0;
+    T *k1;
+    const T *k7;
+    long n3;
+  };
+  static constexpr int NT = 256, MINB = 1;
+  static __device__ __forceinline__ void run(const Args &a, uint3 blockIdx, uint3 gridDim) {
+    const bool acc = a.st->accepted != 0;
+    T *dst = acc ? a.k1 : a.m;
+    const T *src = acc ? a.k7 : a.m0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.n3; i += (long)gridDim.x * blockDim.x)
+      dst[i] = src[i];
+  }
+};
+template <typename T>
+__global__ __launch_bounds__(256) void k_adapt_copy(typename BAdaptCopy<T>::Args a) {
+  BAdaptCopy<T>::run(a, hw_bid(), hw_grid());
+}
+
+template <typename T>
+void launch_adapt(unsigned long long *res, AdaptState *st, const AdaptPar &p, T *m, const T *m0, T *k1,
+                  const T *k7, long n3, hipStream_t s) {
+  launch_b<BAdapt>(k_adapt, dim3(1), dim3(64), 0, s, BAdapt::Args{res, st, p});
+  launch_b<BAdaptCopy<T>>(k_adapt_copy<T>, dim3(blocks_for(n3, 256, 8192)), dim3(256), 0, s,
+                          typename BAdaptCopy<T>::Args{st, m, m0, k1, k7, n3});
+}
+template void launch_adapt<float>(unsigned long long *, AdaptState *, const AdaptPar &, float *, const float *,
+                                  float *, const float *, long, hipStream_t);
+template void launch_adapt<double>(unsigned long long *, AdaptState *, const AdaptPar &, double *, const double *,
+                                   double *, const double *, long, hipStream_t);
 
 // ---- deterministic weighted dot sums ------------------------------------------------
 template <typename T>

@@ -24,8 +24,32 @@ struct StageArgs {
   const T *vol;
   T *nrm;                // when normalising: |sum| per cell (before the scaling) or nullptr
   long N;
+  // device-controlled steps: fac[j] (j >= 1) is multiplied by the step's
+  // h = T(dev_dt[0] * gam) formed on the device, and nothing is written
+  // while *skip != 0 (steps after the end of a run)
+  const double *dev_dt;
+  double gam;
+  const int *skip;
 };
 template <typename T> void launch_stage(const StageArgs<T> &a, hipStream_t s);
+
+// Device-side control of adaptive RK45DP steps: the state the host keeps
+// between steps (engine/run.go, stepcontroller.go), on the device.
+struct AdaptState {
+  double dt, time, last_err, peak_err, last_torque, stop;
+  long long nsteps, nundone, nevals;
+  int done, accepted, error, pad;
+};
+struct AdaptPar {
+  double maxerr, relerr, minslope, headroom, mindt, maxdt, gam;
+  int evals_per_step, h_f32;
+};
+// integralController + adaptDt of one RK45DP step from the error kernel's
+// slots (cleared afterwards), then k1 <- k7 (accepted) or m <- m0 (rejected
+// or after the end of the run); the vectors are 3n long
+template <typename T>
+void launch_adapt(unsigned long long *res, AdaptState *st, const AdaptPar &p, T *m, const T *m0, T *k1,
+                  const T *k7, long n3, hipStream_t s);
 
 // Minimizer step (minimize.cl): m0 <- m, m <- ((4 - h^2|k|^2) m + 4 h k) / (4 + h^2|k|^2)
 template <typename T> void launch_min_step(T *m, T *m0, const T *k, T h, long N, hipStream_t s);
