@@ -150,7 +150,9 @@ def batched(steps=60):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         if mode in ("batched", "tuned"):
-            launches = umhip.sims_steps_batched(sims, steps)
+            nl = umhip.sims_steps_batched(sims, steps)
+            if mode == "batched":
+                launches = nl
         elif mode == "interleaved_graphs":
             umhip.sims_steps(sims, steps)
         else:
@@ -319,21 +321,32 @@ def precision_workload(N=(1024, 1024, 8), seed=9, alpha=0.1):
         s.set_solver(mod.SOLVER_RK45DP)
         s.set_solver_params(dt=1e-15, fixdt=0.0, maxerr=1e-5)
         s.set_m(m0)
+        s.set_graph(1)   # adaptive steps controlled on the device
         return s
     return make
 
 
-def precision(times=(1e-12, 1e-11, 5e-11)):
+def precision(times=(1e-12, 1e-11, 5e-11), control=False):
     """FP32 path (FP64 Newell tensor, FP64 error norms) vs the all-FP64 path on
     the precision workload: per-cell m error and relative total-energy error
-    at each of `times` of adaptive RK45DP, the two runs advanced side by side."""
+    at each of `times` of adaptive RK45DP, the two runs advanced side by side.
+    control: a third, all-FP64 run from m0 perturbed by 1e-9 per component
+    (below FP32 resolution), compared with the FP64 run the same way: how far
+    two FP64 trajectories drift apart from a rounding-sized difference."""
     import torch
     from umagnus_amd import umhip
     make = precision_workload()
     s32, s64 = make(umhip, np.float32), make(umhip, np.float64)
-    rows, t_run = [], {32: 0.0, 64: 0.0}
+    sc = None
+    if control:
+        sc = make(umhip, np.float64)
+        mc = sc.get_m() + 1e-9 * np.random.default_rng(77).uniform(-1, 1, (3,) + tuple(sc.get_m().shape[1:]))
+        sc.set_m(mc / np.linalg.norm(mc, axis=0))
+    rows, t_run = [], {32: 0.0, 64: 0.0, "c": 0.0}
     for t in times:
-        for key, s in ((32, s32), (64, s64)):
+        for key, s in ((32, s32), (64, s64), ("c", sc)):
+            if s is None:
+                continue
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             s.run(t - s.state()["t"])
@@ -348,11 +361,43 @@ def precision(times=(1e-12, 1e-11, 5e-11)):
                      "avg_m_f64": [round(float(x), 7) for x in a64],
                      "avg_m_error": float(np.abs(a32 - a64).max()),
                      "rel_energy_error": float(abs(E32 - E64) / abs(E64))})
-    del s32, s64
+        if sc is not None:
+            dc = np.linalg.norm(sc.get_m() - m64, axis=0)
+            rows[-1]["control_f64_perturbed"] = {
+                "max_m_error": float(dc.max()), "rms_m_error": float(np.sqrt((dc ** 2).mean())),
+                "avg_m_error": float(np.abs(sc.average_m() - a64).max()),
+                "rel_energy_error": float(abs(sc.energy(9) - E64) / abs(E64))}
+        print("precision checkpoint", json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    del s32, s64, sc
     torch.cuda.empty_cache()
     return {"workload": "1024x1024x8 cells of 2 nm, 4 Voronoi regions (Ms in [6e5, 1e6], Aex in [1e-11, 2e-11]), "
                         "Newell tensor, alpha 0.1, RK45DP MaxErr 1e-5, seeded random m",
             "checkpoints": rows, "run_s_f32": round(t_run[32], 3), "run_s_f64": round(t_run[64], 3)}
+
+
+def accuracy(sim32, N, cs, m, floor=1e-6):
+    """The timed FP32 path against the all-FP64 GPU path on the state the timed
+    steps reached: per-cell relative B_eff error |dB_i| / max(|B_i|, floor *
+    max|B|) and torque error |dtau_i| / max(|B_i|, floor * max|B|) (the field
+    bounds the torque of a unit m), next to the normwise max|d| / max|ref|."""
+    import torch
+    from umagnus_amd import umhip
+    b32, t32 = sim32.beff().astype(np.float64), sim32.torque().astype(np.float64)
+    s64 = umhip.Sim(N, cs, dtype=np.float64)
+    configure(s64, umhip, np.float64)
+    s64.set_m(m.astype(np.float64))
+    b64, t64 = s64.beff(), s64.torque()
+    del s64
+    torch.cuda.empty_cache()
+    bn = np.linalg.norm(b64, axis=0)
+    den = np.maximum(bn, floor * bn.max())
+    eb = np.linalg.norm(b32 - b64, axis=0) / den
+    et = np.linalg.norm(t32 - t64, axis=0) / den
+    return {"reference": "all-FP64 GPU path, same m", "floor": f"{floor} max|B|",
+            "beff_percell_rel_max": float(eb.max()), "beff_percell_rel_rms": float(np.sqrt((eb ** 2).mean())),
+            "torque_percell_rel_max": float(et.max()),
+            "beff_normwise": float(np.abs(b32 - b64).max() / np.abs(b64).max()),
+            "torque_normwise": float(np.abs(t32 - t64).max() / np.abs(t64).max())}
 
 
 def cpu_model():
@@ -452,6 +497,8 @@ def main():
     ap.add_argument("--no-strip", action="store_true")
     ap.add_argument("--precision-times", type=str, default="1e-12,1e-11,5e-11",
                     help="comma-separated checkpoints (s) of the precision workload")
+    ap.add_argument("--precision-control", action="store_true",
+                    help="also run an all-FP64 control from m0 perturbed by 1e-9")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
@@ -510,8 +557,8 @@ def main():
             "heff_evals_per_s": round(1.0 / eval_s, 1),
         },
         "roofline": {
-            "kernel": "one effective-field + torque evaluation: demag FFT passes (X-fwd R2C, Y-fwd, "
-                      "Z fwd+tensor+inv, Y-inv, X-inv C2R) + four-cell field/torque kernel",
+            "kernel": "one effective-field + torque evaluation: X-fwd R2C (register four-step), Y-fwd, "
+                      "Z fwd+tensor+inv, Y-inv, and the X-inv C2R fused with the four-cell field/torque kernel",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -524,6 +571,7 @@ def main():
             "field_torque_ms": round(t_field / max(n_evals, 1), 4),
         },
     }
+    out["accuracy"] = accuracy(sim, N, cs, m)
     del sim
     torch.cuda.empty_cache()
     if not args.no_batched:
@@ -531,7 +579,8 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint()
     if not args.no_precision:
-        out["precision"] = precision(tuple(float(x) for x in args.precision_times.split(",")))
+        out["precision"] = precision(tuple(float(x) for x in args.precision_times.split(",")),
+                                     args.precision_control)
     if not args.no_autotune:
         out["autotune"] = autotune()
     if not args.no_strip:
