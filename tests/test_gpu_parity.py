@@ -507,7 +507,7 @@ def test_strip_trajectory_vs_oracle(gpu, oracle, prec):
 # ---- the timed kernels at the timed size --------------------------------------------------------
 TIMED_512x512x32 = ["void um::k_xf4<float, 512>(", "void um::k_coly<float, 32, -1, 16, true, 1>(",
                     "void um::k_convz<float, 32, 8, 4>(", "void um::k_coly<float, 32, 1, 16, true, 1>(",
-                    "void um::k_xi4_field<float, 512, false, 2, true>("]
+                    "void um::k_xi4_field<float, 512, false, 2, true, false>("]
 
 
 def test_timed_kernels_512x512x32_vs_oracle(gpu, oracle):
@@ -546,7 +546,7 @@ def test_timed_kernels_512x512x32_vs_oracle(gpu, oracle):
 
 
 @pytest.mark.parametrize("N,expect", [((512, 64, 32), ["void um::k_xf4<float, 512>(",
-                                                       "void um::k_xi4_field<float, 512, false, 2, true>(",
+                                                       "void um::k_xi4_field<float, 512, false, 2, true, false>(",
                                                        "void um::k_convz<float, 32, 8, 4>("]),
                                       ((32, 512, 32), ["void um::k_coly<float, 32, -1, 16, true, 1>(",
                                                        "void um::k_coly<float, 32, 1, 16, true, 1>(",

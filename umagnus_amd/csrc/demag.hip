@@ -1757,10 +1757,14 @@ struct XiFieldArgs {
   XiArgs<T> x;
   FieldArgs<T> f;
 };
-template <typename T, int N, bool REG, int EX, bool UNI>
+// TH = false (no thermal term, checked on the host) at N = 512: at most 128
+// VGPRs without spills (the launch bound's second argument is the minimum
+// number of waves per SIMD with hipcc), so five work-groups share a CU
+// instead of four; the other lengths would spill under that bound
+template <typename T, int N, bool REG, int EX, bool UNI, bool TH>
 struct BXi4Field {
   using Args = XiFieldArgs<T>;
-  static constexpr int NT = XF_NT, MINB = 1;
+  static constexpr int NT = XF_NT, MINB = (!TH && N == 512) ? 4 : 1;
   static __device__ __forceinline__ void run(const XiFieldArgs<T> &a, uint3 blockIdx, uint3 gridDim) {
     const XiArgs<T> &x = a.x;
     const FieldArgs<T> &f = a.f;
@@ -1848,12 +1852,14 @@ struct BXi4Field {
         const float4 b = *(const float4 *)(bl + (c * R + j) * N + 4 * gx);
         bdv[c][0] = b.x; bdv[c][1] = b.y; bdv[c][2] = b.z; bdv[c][3] = b.w;
       }
-      field_group4<T, REG, EX, false, UNI, false>(f, 4 * gx, y, z, bdv);   // registers over instructions
+      field_group4<T, REG, EX, false, UNI, false, TH>(f, 4 * gx, y, z, bdv);   // registers over instructions
     }
   }
 };
-template <typename T, int N, bool REG, int EX, bool UNI>
-__global__ __launch_bounds__(XF_NT) void k_xi4_field(XiFieldArgs<T> a) { BXi4Field<T, N, REG, EX, UNI>::run(a, hw_bid(), hw_grid()); }
+template <typename T, int N, bool REG, int EX, bool UNI, bool TH>
+__global__ __launch_bounds__(XF_NT, (BXi4Field<T, N, REG, EX, UNI, TH>::MINB)) void k_xi4_field(XiFieldArgs<T> a) {
+  BXi4Field<T, N, REG, EX, UNI, TH>::run(a, hw_bid(), hw_grid());
+}
 
 // whether the fused kernel covers these field terms and buffers: the
 // four-cell kernel's terms without spin-transfer torques or bulk DMI, every
@@ -1876,16 +1882,21 @@ template <typename T>
 static void launch_xi4_field(const XiArgs<T> &a, const FieldArgs<T> &f, int n, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     const int Ny = a.fRy.d, Nz = a.rows / Ny;
-    const bool reg = f.regions != nullptr, uni = field_uniform(f);
+    const bool reg = f.regions != nullptr, uni = field_uniform(f), th = f.has_therm != 0;
     auto go = [&](auto tag, auto kern, int R) {
       using K = typename decltype(tag)::type;
       const dim3 grid((unsigned)(((Ny + R - 1) / R) * Nz));
       launch_b<K>(kern, grid, dim3(XF_NT), 0, s, XiFieldArgs<T>{a, f});
       UM_HIP(hipGetLastError());
     };
-#define UM_XF2(CN, RG, E)                                                                            \
-  { if (uni) go(BTag<BXi4Field<T, CN, RG, E, true>>{}, k_xi4_field<T, CN, RG, E, true>, XFShape<CN>::R);        \
-    else go(BTag<BXi4Field<T, CN, RG, E, false>>{}, k_xi4_field<T, CN, RG, E, false>, XFShape<CN>::R); }
+// the thermal-free variants only for uniform parameters (the hot case);
+// per-cell parameters without a thermal term take the TH = true kernel
+#define UM_XF2(CN, RG, E)                                                                                      \
+  { if (uni && !th) go(BTag<BXi4Field<T, CN, RG, E, true, false>>{}, k_xi4_field<T, CN, RG, E, true, false>,    \
+                       XFShape<CN>::R);                                                                          \
+    else if (uni) go(BTag<BXi4Field<T, CN, RG, E, true, true>>{}, k_xi4_field<T, CN, RG, E, true, true>,         \
+                     XFShape<CN>::R);                                                                            \
+    else go(BTag<BXi4Field<T, CN, RG, E, false, true>>{}, k_xi4_field<T, CN, RG, E, false, true>, XFShape<CN>::R); }
 #define UM_XF(CN)                                                                                    \
   if (n == CN) {                                                                                     \
     if (f.exch_mode == 0) { if (reg) UM_XF2(CN, true, 0) else UM_XF2(CN, false, 0) }                 \

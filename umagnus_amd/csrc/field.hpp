@@ -623,7 +623,11 @@ __host__ bool field_uniform(const FieldArgs<T> &a) {
 // HOIST: with UNI, form the stencil weights and the anisotropy / torque
 // constants once per thread instead of once per cell (fewer instructions,
 // more live registers: the choice of the calling kernel).
-template <typename T, bool REG, int EX, bool STT = false, bool UNI = false, bool HOIST = UNI>
+// TH: the thermal term may be present (a.has_therm).  Its FP64 Box-Muller
+// sets the register peak of the whole kernel; a caller that checked
+// has_therm == 0 on the host instantiates TH = false and gets the smaller
+// register budget.
+template <typename T, bool REG, int EX, bool STT = false, bool UNI = false, bool HOIST = UNI, bool TH = true>
 __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int iy, int iz, const G4In<T> &q,
                                            const T (&bdv)[3][4]) {
   const Mesh &g = a.g;
@@ -738,7 +742,7 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
       B.y += bext[1].at(I);
       B.z += bext[2].at(I);
     }
-    if (a.has_therm) {
+    if (TH && a.has_therm) {
       const uint64_t stp = a.step_dev ? (uint64_t)*a.step_dev : a.step;
       const V3<T> h = thermal_cell(I, a.seed, stp, a.stream, a.kth, Ms, temp, alpha);
       B.x += h.x; B.y += h.y; B.z += h.z;
@@ -808,11 +812,11 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
   }
 }
 
-template <typename T, bool REG, int EX, bool STT = false, bool UNI = false, bool HOIST = UNI>
+template <typename T, bool REG, int EX, bool STT = false, bool UNI = false, bool HOIST = UNI, bool TH = true>
 __device__ __forceinline__ void field_group4(const FieldArgs<T> &a, int ix0, int iy, int iz, const T (&bdv)[3][4]) {
   G4In<T> q;
   g4_load<T, REG, (STT && EX == 0) ? 1 : EX>(a, ix0, iy, iz, q);   // STT needs the neighbours
-  g4_compute<T, REG, EX, STT, UNI, HOIST>(a, ix0, iy, iz, q, bdv);
+  g4_compute<T, REG, EX, STT, UNI, HOIST, TH>(a, ix0, iy, iz, q, bdv);
 }
 
 template <typename T> void launch_field_torque(const FieldArgs<T> &a, hipStream_t s);
