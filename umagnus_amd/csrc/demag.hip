@@ -640,6 +640,18 @@ struct BColy {
     extern __shared__ __align__(16) unsigned char smem_[];
     C2<T> *lds = (C2<T> *)smem_;
     T *ldr = (T *)smem_;
+    // twiddles W_P^(i k2) of the step between the transforms, [i][k2] with
+    // rows padded by one entry (the A rows of a wave fall on distinct banks),
+    // after the transpose space: LDS reads instead of global-memory reads in
+    // the middle of the transform
+    C2<T> *twt = (C2<T> *)(smem_ + (SPLIT ? sizeof(T) : sizeof(C2<T>)) * A * S);
+    {
+      const C2<T> *tw = (const C2<T> *)a.f.tw;   // W_P^m, m < P
+      for (int j = threadIdx.x; j < A * A; j += A * C) {
+        const int r = j / A, k2 = j - r * A;
+        twt[r * (A + 1) + k2] = tw[r * k2];
+      }
+    }
     const int ngx = gridDim.x, ngy = gridDim.y;
     const int w = xcd_tile(blockIdx.x + ngx * (blockIdx.y + ngy * blockIdx.z), ngx * ngy * gridDim.z);
     const int gx = w % ngx, gyz = w / ngx, gy = gyz % ngy, gz = gyz / ngy;
@@ -649,7 +661,6 @@ struct BColy {
     // uniform plane base + one 32-bit lane offset (row i, column kx)
     const uint32_t loff = (uint32_t)(((long)i * a.estride + min(kx, a.H - 1)) * sizeof(C2<T>));
     const long rstep = (long)A * a.estride;   // rows n2 -> n2 + 1
-    const C2<T> *tw = (const C2<T> *)a.f.tw;   // W_P^m, m < P
     constexpr int NIN = (HALF && DIR < 0) ? A / 2 : A;
     constexpr int NOUT = (HALF && DIR > 0) ? A / 2 : A;
 
@@ -673,9 +684,10 @@ struct BColy {
       if (p + 1 < ZB) load(gy * ZB + p + 1);
       dft<A, DIR>(v);
       pin(v);
+      if (p == 0) __syncthreads();   // the twiddle table is complete
   #pragma unroll
       for (int k2 = 1; k2 < A; k2++) {
-        C2<T> t = tw[i * k2];
+        C2<T> t = twt[i * (A + 1) + k2];
         if (DIR > 0) t.y = -t.y;
         v[k2] = cmul(v[k2], t);
       }
@@ -918,10 +930,16 @@ __global__ __launch_bounds__(FFT_NT) void k_conv(ConvArgs<T> a) { BConv<T, CN, C
 // and only the Nz planes that hold data are stored.  A work-group is three
 // waves, one per component; lanes 0-31 take ky and lanes 32-63 its mirror
 // Py-ky, which read the same tensor values (kernmulrsymm3d.cl:62-66).
+// When H = 32 q + 1 (H = Px/2 + 1 with Px a multiple of 64), the last kx,
+// the Nyquist column, would take a work-group of its own for every ky with
+// one useful lane in 32; instead the blocks of the last grid column hold the
+// Nyquist column of 32 ky pairs each (lane j: ky pair 32 blockIdx.y + j) and
+// the remaining blocks of that column return at once.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct ConvZArgs {
   int H;           // valid kx columns
+  int nyq;         // 1: the last grid column holds the Nyquist column (H = 32 q + 1)
   int Py, KHy;     // padded y size and Py/2+1
   int Hk;          // tensor row pitch (H rounded up to a 128-byte line)
   long zstride;    // complex elements between z planes of a component
@@ -959,11 +977,15 @@ struct BConvz {
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int l = threadIdx.x & 63;
     const int t = l >> 5;
-    const int kx0 = blockIdx.x * CONVZ_COLS;
-    const int kx = kx0 + (l & 31);
-    const int kya = blockIdx.y;
+    // Nyquist blocks: one kx (H - 1), 32 ky pairs across the lanes
+    const bool nyqb = a.nyq && blockIdx.x == gridDim.x - 1;
+    if (nyqb && (int)blockIdx.y * CONVZ_COLS >= a.KHy) return;   // uniform: the whole work-group
+    const int kx0 = nyqb ? a.H - 1 : blockIdx.x * CONVZ_COLS;
+    const int kx = nyqb ? kx0 : kx0 + (l & 31);
+    const int kyq = nyqb ? (int)blockIdx.y * CONVZ_COLS + (l & 31) : (int)blockIdx.y;
+    const int kya = min(kyq, a.KHy - 1);
     const bool mirror = kya != 0 && 2 * kya != a.Py;
-    const bool valid = kx < a.H && (t == 0 || mirror);
+    const bool valid = kx < a.H && kyq < a.KHy && (t == 0 || mirror);
     // every lane loads in-bounds data (clamped column / row); only valid lanes store
     const int kxc = min(kx, a.H - 1);
     const int ky = (t && mirror) ? a.Py - kya : kya;
@@ -977,14 +999,17 @@ struct BConvz {
     {
       constexpr int TS = KZ * 6 * CONVZ_COLS, TQ = (TS + CONVZ_NT - 1) / CONVZ_NT;
       const long kzs = (long)a.KHy * a.Hk;
-      const int x = min(kx0 + (int)(threadIdx.x & (CONVZ_COLS - 1)), a.H - 1);
+      const int j = threadIdx.x & (CONVZ_COLS - 1);
+      // slab column j: kx0 + j at this ky, or (Nyquist block) kx0 at ky pair 32 blockIdx.y + j
+      const int x = nyqb ? kx0 : min(kx0 + j, a.H - 1);
+      const long yrow = (long)(nyqb ? min((int)blockIdx.y * CONVZ_COLS + j, a.KHy - 1) : kya) * a.Hk;
       T tv[TQ];
   #pragma unroll
       for (int q = 0; q < TQ; q++) {
         const int g = threadIdx.x + q * CONVZ_NT;
         if (TS % CONVZ_NT == 0 || g < TS) {
           const int ez = g / CONVZ_COLS, e = ez % 6, kz = ez / 6;
-          tv[q] = a.K[e][kz * kzs + (long)kya * a.Hk + x];
+          tv[q] = a.K[e][kz * kzs + yrow + x];
         }
       }
   #pragma unroll
@@ -1097,7 +1122,7 @@ template <> struct ConvZTable<double> {
 template <typename T>
 static void launch_convz(const ConvZArgs<T> &a, int Nz, hipStream_t s) {
   constexpr int FR = ConvZTable<T>::FR, G = ConvZTable<T>::G;
-  const dim3 grid((a.H + CONVZ_COLS - 1) / CONVZ_COLS, a.Py / 2 + 1);
+  const dim3 grid(a.nyq ? (a.H - 1) / CONVZ_COLS + 1 : (a.H + CONVZ_COLS - 1) / CONVZ_COLS, a.Py / 2 + 1);
   switch (Nz) {
     case 8: launch_b<BConvz<T, 8, FR, G>>(k_convz<T, 8, FR, G>, grid, dim3(CONVZ_NT), 0, s, a); break;
     case 16: launch_b<BConvz<T, 16, FR, G>>(k_convz<T, 16, FR, G>, grid, dim3(CONVZ_NT), 0, s, a); break;
@@ -1661,6 +1686,7 @@ static bool launch_coly(const ColArgs<T> &a, int P, int nz, hipStream_t s) {
     // k_coly's SPLIT: real and imaginary parts exchanged one after the other
     size_t lds = sizeof(C2<T>) * (size_t)A * (A * C + C);
     if (lds > 65536) lds /= 2;
+    lds += sizeof(C2<T>) * (size_t)A * (A + 1);   // the twiddle table
     allow_lds(kern, lds);
     const dim3 grid((a.H + C - 1) / C, nz / ZB, 3);
     launch_b<K>(kern, grid, dim3(A * C), lds, s, a);
@@ -1775,6 +1801,15 @@ struct BXi4Field {
     // complex entries: more work-groups per CU); afterwards line (c, j) holds
     // B_demag_c of row j, x < N
     __shared__ __align__(16) T ldr[S::L * TS];
+    // twiddles of the step between the two transforms, W_N^(n1 k2) at
+    // [n1][k2] (rows padded by one entry: the A rows fall on distinct banks);
+    // read from LDS instead of global memory, their latency stays out of the
+    // transform
+    __shared__ C2<T> twb[A * (B + 1)];
+    for (int i = threadIdx.x; i < A * B; i += XF_NT) {
+      const int r = i / B, k2 = i - r * B;
+      twb[r * (B + 1) + k2] = x.tw2[2 * r * k2];
+    }
     const int Ny = (int)x.fRy.d, Nz = x.rows / Ny;
     const int w = xcd_tile(blockIdx.x, gridDim.x);
     const int yb = w / Nz, z = w - yb * Nz, y0 = yb * R;
@@ -1808,8 +1843,9 @@ struct BXi4Field {
         }
       }
       dft<B, +1>(v);
+      __syncthreads();   // the twiddle table is complete
   #pragma unroll
-      for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(x.tw2[2 * n1 * k2]));   // W_n^-(n1 k2)
+      for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(twb[n1 * (B + 1) + k2]));   // W_n^-(n1 k2)
     }
     C2<T> u[G][A];
     T *tw = ldr + line1 * TS + n1;
@@ -2186,6 +2222,7 @@ struct DemagPlan : DemagBase {
     if (!is2d && zreg) {
       ConvZArgs<T> za{};
       za.H = H; za.Py = P[1]; za.KHy = KHy; za.Hk = Hk; za.zstride = (long)P[1] * Hp; za.ystride = Hp;
+      za.nyq = H > 1 && (H - 1) % CONVZ_COLS == 0;
       for (int c = 0; c < 6; c++) za.K[c] = K[c];
       for (int c = 0; c < 3; c++) za.buf[c] = spec[c].as<C2<T>>();
       launch_convz<T>(za, N[2], s);

@@ -597,6 +597,65 @@ __device__ __forceinline__ void g4_load(const FieldArgs<T> &a, int ix0, int iy, 
   }
 }
 
+// Interior fast path of the exchange (+ interfacial DMI) stencil for a
+// region-free group of uniform A and D.  It applies when every m the group
+// reads is clearly non-zero (|x|+|y|+|z| >= 1e-15, so no dot product m.m can
+// underflow: is0() is false for all of them) and, with DMI, every in-plane
+// neighbour exists (interior or periodic); then no neighbour is vacuum and no
+// boundary extrapolation happens, and the per-neighbour sums of exchange_nb /
+// dmi_nb reduce to
+//   h = s_x (x- + x+ - 2 m0) + s_y (y- + y+ - 2 m0) + s_z (z- + z+ - 2 m0)
+//   DMI: h_x += D/c_x (x+_z - x-_z), h_z -= D/c_x (x+_x - x-_x) + D/c_y (y+_y - y-_y),
+//        h_y += D/c_y (y+_z - y-_z)
+// (the same terms, summed in another order).  A z neighbour outside a
+// non-periodic mesh is loaded clamped (it is m0 itself) and adds zero, as in
+// exchange_nb.
+template <typename T>
+__device__ __forceinline__ T g4_nzmin(T m, T x, T y, T z) {
+  return fmin(m, fabs(x) + fabs(y) + fabs(z));
+}
+template <typename T>
+__device__ __forceinline__ bool g4_fast_ok(const G4In<T> &q, bool z6) {
+  T mn = T(1);
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    mn = g4_nzmin(mn, q.mc[0][w], q.mc[1][w], q.mc[2][w]);
+    mn = g4_nzmin(mn, q.myl[0][w], q.myl[1][w], q.myl[2][w]);
+    mn = g4_nzmin(mn, q.myh[0][w], q.myh[1][w], q.myh[2][w]);
+    if (z6) {
+      mn = g4_nzmin(mn, q.mzl[0][w], q.mzl[1][w], q.mzl[2][w]);
+      mn = g4_nzmin(mn, q.mzh[0][w], q.mzh[1][w], q.mzh[2][w]);
+    }
+  }
+  mn = g4_nzmin(mn, q.mxl[0], q.mxl[1], q.mxl[2]);
+  mn = g4_nzmin(mn, q.mxr[0], q.mxr[1], q.mxr[2]);
+  return mn >= T(1e-15);
+}
+// exchange (+ DMI) field of cell w of a fast group, before the 1/Ms factor
+template <typename T, int EX>
+__device__ __forceinline__ V3<T> g4_exch_fast(const G4In<T> &q, int w, bool z6, const NbPre<T> &p) {
+  T h[3];
+  const T sx = EX == 1 ? p.s[0] : p.e[0], sy = EX == 1 ? p.s[1] : p.e[1], sz = EX == 1 ? p.s[2] : p.e[2];
+  T xl[3], xr[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    xl[c] = w > 0 ? q.mc[c][w - 1] : q.mxl[c];
+    xr[c] = w < 3 ? q.mc[c][w + 1] : q.mxr[c];
+    const T m0 = q.mc[c][w];
+    T v = sx * ((xl[c] + xr[c]) - T(2) * m0);
+    v += sy * ((q.myl[c][w] + q.myh[c][w]) - T(2) * m0);
+    if (z6) v += sz * ((q.mzl[c][w] + q.mzh[c][w]) - T(2) * m0);
+    h[c] = v;
+  }
+  if constexpr (EX == 2) {
+    h[0] += p.dc[0] * (xr[2] - xl[2]);
+    h[2] -= p.dc[0] * (xr[0] - xl[0]);
+    h[1] += p.dc[1] * (q.myh[2][w] - q.myl[2][w]);
+    h[2] -= p.dc[1] * (q.myh[1][w] - q.myl[1][w]);
+  }
+  return v3(h[0], h[1], h[2]);
+}
+
 // Effective field and torque of the group from its loaded inputs; bdv: demag
 // field of the four cells (zero when absent).  Same term order as k_field_torque.
 // EX: 0 none, 1 exchange, 2 + interfacial DMI, 3 + bulk DMI; STT: Zhang-Li and
@@ -671,13 +730,27 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
   const bool inxl = (ix0 - 1 >= 0) || (g.pbc & 1), inxr = (ix0 + 4 < g.Nx) || (g.pbc & 1);
   const bool inyl = (iy - 1 >= 0) || (g.pbc & 2), inyh = (iy + 1 < g.Ny) || (g.pbc & 2);
   const bool inzl = (iz - 1 >= 0) || (g.pbc & 4), inzh = (iz + 1 < g.Nz) || (g.pbc & 4);
+  // the interior fast path (region-free, uniform parameters, no STT)
+  constexpr bool FASTOK = UNI && !REG && !STT && (EX == 1 || EX == 2);
+  bool fast = false;
+  NbPre<T> fpre{};
+  if constexpr (FASTOK) {
+    fast = (EX == 1 || (inxl && inxr && inyl && inyh)) && g4_fast_ok(q, nnb == 6);
+    fpre = PRE ? nbp : nb_pre(q.A0, q.D0, a.wx, a.wy, a.wz, a.cx, a.cy, a.cz);
+  }
   T ob[3][4], ot[3][4];
 #pragma unroll
   for (int w = 0; w < 4; w++) {
     const long I = I0 + w;
     const V3<T> m0 = v3(q.mc[0][w], q.mc[1][w], q.mc[2][w]);
     V3<T> B = v3(bdv[0][w], bdv[1][w], bdv[2][w]);
-    if (EX != 0 && !is0(m0)) {
+    if (FASTOK && fast) {
+      const V3<T> h = g4_exch_fast<T, EX>(q, w, nnb == 6, fpre);
+      const T invMs = H ? invMs_u : inv_msat(Ms, I);
+      B.x += h.x * invMs;
+      B.y += h.y * invMs;
+      B.z += h.z * invMs;
+    } else if (EX != 0 && !is0(m0)) {
       V3<T> mn[6];
       int rn[6];
       mn[0] = w > 0 ? v3(q.mc[0][w - 1], q.mc[1][w - 1], q.mc[2][w - 1]) : v3(q.mxl[0], q.mxl[1], q.mxl[2]);
