@@ -353,6 +353,13 @@ __global__ __launch_bounds__(FFT_NT) void k_xinv_odd(XiArgs<T> a) { BXinvOdd<T>:
 // ---------------------------------------------------------------------------
 constexpr int XF4_NT = 128;
 
+// exp(dir * 2 pi i j / n); constant once the caller's loops are unrolled
+template <typename T>
+__device__ __forceinline__ C2<T> cexpi(int j, int n, int dir) {
+  const double ang = 2.0 * M_PI * (double)j / (double)n;
+  return C2<T>{T(__builtin_cos(ang)), T(dir * __builtin_sin(ang))};
+}
+
 template <int N> struct X4Shape;
 template <> struct X4Shape<64> { static constexpr int A = 8, B = 8; };
 template <> struct X4Shape<128> { static constexpr int A = 8, B = 16; };
@@ -371,11 +378,18 @@ struct BXi4 {
     constexpr int A = X4Shape<N>::A, B = X4Shape<N>::B, G = B / A;
     constexpr int GROUPS = XF4_NT / B, ROWS = GROUPS * G, TS = B * (A + 1);
     __shared__ C2<T> lds[ROWS * TS];
+    // twiddles W_n^(n1 k2) between the transforms, [n1][k2] (rows padded by one)
+    __shared__ C2<T> twb[A * (B + 1)];
+    for (int i = threadIdx.x; i < A * B; i += XF4_NT) {
+      const int q = i / B, k2 = i - q * B;
+      twb[q * (B + 1) + k2] = a.tw2[2 * q * k2];
+    }
     const int c = blockIdx.y;
     const int grp = threadIdx.x / B, l = threadIdx.x % B;
     const int row0 = blockIdx.x * ROWS + grp * G;
     {
       const int h = l / A, n1 = l % A, r = row0 + h;
+      const C2<T> w1 = a.tw2[n1];   // W_2n^n1: W_2n^k = W_2n^n1 W_2n^(A n2) with the second factor constant
       C2<T> v[B];
   #pragma unroll
       for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
@@ -391,15 +405,16 @@ struct BXi4 {
         }
   #pragma unroll
         for (int n2 = 0; n2 < B; n2++) {
-          const int k = n1 + A * n2;
           const C2<T> sm = cadd(xk[n2], cconj(xn[n2])), d = csub(xk[n2], cconj(xn[n2]));
-          const C2<T> t = cmul(cconj(a.tw2[k]), d);
+          const C2<T> wk = n2 == 0 ? w1 : cmul(w1, cexpi<T>(A * n2, 2 * N, -1));
+          const C2<T> t = cmul(cconj(wk), d);
           v[n2] = C2<T>{sm.x - t.y, sm.y + t.x};
         }
       }
       dft<B, +1>(v);
+      __syncthreads();   // the twiddle table is complete
   #pragma unroll
-      for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(a.tw2[2 * n1 * k2]));   // W_n^-(n1 k2)
+      for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], cconj(twb[n1 * (B + 1) + k2]));   // W_n^-(n1 k2)
       C2<T> *t = lds + (grp * G + h) * TS + n1;
   #pragma unroll
       for (int k2 = 0; k2 < B; k2++) t[k2 * (A + 1)] = v[k2];
@@ -443,6 +458,12 @@ struct BXf4 {
     constexpr int A = X4Shape<N>::A, B = X4Shape<N>::B, G = B / A;
     constexpr int GROUPS = XF4_NT / B, ROWS = GROUPS * G, TS = B * (A + 1), HB = B / 2;
     __shared__ T ldr[ROWS * TS];
+    // twiddles W_n^(n1 k2) between the transforms, [n1][k2] (rows padded by one)
+    __shared__ C2<T> twb[A * (B + 1)];
+    for (int i = threadIdx.x; i < A * B; i += XF4_NT) {
+      const int q = i / B, k2 = i - q * B;
+      twb[q * (B + 1) + k2] = a.tw2[2 * q * k2];
+    }
     const int c = blockIdx.y;
     const int grp = threadIdx.x / B, l = threadIdx.x % B;
     const int row0 = blockIdx.x * ROWS + grp * G;
@@ -472,8 +493,9 @@ struct BXf4 {
         }
       }
       dft<B, -1>(v);
+      __syncthreads();   // the twiddle table is complete
   #pragma unroll
-      for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], a.tw2[2 * n1 * k2]);   // W_n^(n1 k2)
+      for (int k2 = 1; k2 < B; k2++) v[k2] = cmul(v[k2], twb[n1 * (B + 1) + k2]);   // W_n^(n1 k2)
       T *tw = ldr + (grp * G + h1) * TS + n1;
   #pragma unroll
       for (int k2 = 0; k2 < B; k2++) tw[k2 * (A + 1)] = v[k2].x;
@@ -495,7 +517,8 @@ struct BXf4 {
   #pragma unroll
         for (int q = 0; q < A; q++) u[h][q].y = t[q];
       }
-      // unpack and store: lane l = k2 holds Z[k2 + B k1]
+      // unpack and store: lane l = k2 holds Z[k2 + B k1]; W_2n^k = W_2n^l W_2n^(B k1)
+      const C2<T> wl = a.tw2[l];
       const int base = threadIdx.x - l;                       // first lane of the group
       const int src_lane = base + ((B - l) & (B - 1));        // holder of Z[n - k]
   #pragma unroll
@@ -515,7 +538,7 @@ struct BXf4 {
           const C2<T> E = C2<T>{T(0.5) * (Zk.x + Bc.x), T(0.5) * (Zk.y + Bc.y)};
           const C2<T> O = C2<T>{T(0.5) * (Zk.y - Bc.y), T(-0.5) * (Zk.x - Bc.x)};
           const int k = l + B * k1;
-          const C2<T> t = cmul(a.tw2[k], O);
+          const C2<T> t = cmul(k1 == 0 ? wl : cmul(wl, cexpi<T>(B * k1, 2 * N, -1)), O);
           if (rr < a.rows) {
             orow[k] = cadd(E, t);
             if (k == 0) orow[N] = cconj(csub(E, t));
@@ -951,12 +974,6 @@ struct ConvZArgs {
 constexpr int CONVZ_NT = 192;
 constexpr int CONVZ_COLS = 32;
 
-// exp(dir * 2 pi i j / n); constant once the caller's loops are unrolled
-template <typename T>
-__device__ __forceinline__ C2<T> cexpi(int j, int n, int dir) {
-  const double ang = 2.0 * M_PI * (double)j / (double)n;
-  return C2<T>{T(__builtin_cos(ang)), T(dir * __builtin_sin(ang))};
-}
 
 // Visiting order of the Pz frequencies: 0, Nz, then the mirror pairs (p, Pz-p)
 // next to each other so a pair shares its tensor loads.
