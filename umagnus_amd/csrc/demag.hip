@@ -1011,29 +1011,25 @@ struct BConvz {
     C2<T> *plane0 = a.buf[c];
     const uint32_t loff = (uint32_t)(((long)ky * a.ystride + kxc) * sizeof(C2<T>));
 
-    // 0. tensor slab of this work-group (32 kx x 6 elements x Nz+1 kz) into LDS
-    // (fixed trip count, unrolled: every load of the slab is in flight at once)
+    // 0. tensor slab of this work-group (32 kx x 6 elements x Nz+1 kz) and the
+    // lane's column: every global load is issued before the first use of any
+    // (fixed trip counts, unrolled, then a scheduling barrier).  Thread t
+    // loads element e = t / 32 of slab column t % 32 for every kz; the
+    // element's base pointer is selected from the uniform arguments instead of
+    // being read per lane.
+    static_assert(CONVZ_NT == 6 * CONVZ_COLS, "one slab element per 32 threads");
+    const int j = threadIdx.x & (CONVZ_COLS - 1), eld = threadIdx.x / CONVZ_COLS;
+    T tv[KZ];
     {
-      constexpr int TS = KZ * 6 * CONVZ_COLS, TQ = (TS + CONVZ_NT - 1) / CONVZ_NT;
       const long kzs = (long)a.KHy * a.Hk;
-      const int j = threadIdx.x & (CONVZ_COLS - 1);
       // slab column j: kx0 + j at this ky, or (Nyquist block) kx0 at ky pair 32 blockIdx.y + j
       const int x = nyqb ? kx0 : min(kx0 + j, a.H - 1);
       const long yrow = (long)(nyqb ? min((int)blockIdx.y * CONVZ_COLS + j, a.KHy - 1) : kya) * a.Hk;
-      T tv[TQ];
+      const T *Ke = eld == 0 ? a.K[0] : eld == 1 ? a.K[1] : eld == 2 ? a.K[2] : eld == 3 ? a.K[3]
+                  : eld == 4 ? a.K[4] : a.K[5];
+      Ke += yrow + x;
   #pragma unroll
-      for (int q = 0; q < TQ; q++) {
-        const int g = threadIdx.x + q * CONVZ_NT;
-        if (TS % CONVZ_NT == 0 || g < TS) {
-          const int ez = g / CONVZ_COLS, e = ez % 6, kz = ez / 6;
-          tv[q] = a.K[e][kz * kzs + yrow + x];
-        }
-      }
-  #pragma unroll
-      for (int q = 0; q < TQ; q++) {
-        const int g = threadIdx.x + q * CONVZ_NT;
-        if (TS % CONVZ_NT == 0 || g < TS) tens[g] = tv[q];
-      }
+      for (int kz = 0; kz < KZ; kz++) tv[kz] = Ke[kz * kzs];
     }
 
     // 1. forward: even and odd halves of the spectrum
@@ -1043,6 +1039,9 @@ struct BConvz {
       const auto *p = at_off<const T>(plane0 + n * a.zstride, loff);
       E[n] = C2<T>{p[0], p[1]};
     }
+    __builtin_amdgcn_sched_barrier(0);
+  #pragma unroll
+    for (int kz = 0; kz < KZ; kz++) tens[(kz * 6 + eld) * CONVZ_COLS + j] = tv[kz];
   #pragma unroll
     for (int n = 0; n < NZ; n++) O[n] = n == 0 ? E[0] : cmul(E[n], cexpi<T>(n, PZ, -1));
     // one transform at a time: interleaving the two raises the register peak
@@ -1838,7 +1837,10 @@ struct BXi4Field {
   #pragma unroll
       for (int n2 = 0; n2 < B; n2++) v[n2] = C2<T>{T(0), T(0)};
       if (y < Ny) {
-        const C2<T> *irow = x.in[c] + ((long)z * x.Py + y) * x.Hp;
+        // component base selected from the uniform arguments (an indexed read
+        // of x.in would be a per-lane load ahead of every spectrum load)
+        const C2<T> *inc = c == 0 ? x.in[0] : c == 1 ? x.in[1] : x.in[2];
+        const C2<T> *irow = inc + ((long)z * x.Py + y) * x.Hp;
         // two halves of loads: half the staging registers live at a time
   #pragma unroll
         for (int hb = 0; hb < 2; hb++) {
