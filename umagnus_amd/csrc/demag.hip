@@ -353,6 +353,37 @@ __global__ __launch_bounds__(FFT_NT) void k_xinv_odd(XiArgs<T> a) { BXinvOdd<T>:
 // ---------------------------------------------------------------------------
 constexpr int XF4_NT = 128;
 
+// Twiddle table of the step between the two register transforms of a
+// four-step pass: W_n^(r k2) (r < R, k2 < K) at [r][k2], rows padded by one
+// entry.  Loading and storing are split so that a kernel issues the table's
+// global loads after its data loads and writes the table once both arrived:
+// no memory round trip of its own ahead of the data.
+template <typename T, int R, int K, int NT>
+struct TwStage {
+  static constexpr int Q = (R * K + NT - 1) / NT;
+  C2<T> v[Q];
+  __device__ __forceinline__ void load(const C2<T> *tw, int step) {   // tw[step r k2] = W_n^(r k2)
+  #pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int i = (int)threadIdx.x + q * NT;
+      if ((R * K) % NT == 0 || i < R * K) {
+        const int r = i / K, k2 = i - r * K;
+        v[q] = tw[step * r * k2];
+      }
+    }
+  }
+  __device__ __forceinline__ void store(C2<T> *lds) const {
+  #pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int i = (int)threadIdx.x + q * NT;
+      if ((R * K) % NT == 0 || i < R * K) {
+        const int r = i / K, k2 = i - r * K;
+        lds[r * (K + 1) + k2] = v[q];
+      }
+    }
+  }
+};
+
 // exp(dir * 2 pi i j / n); constant once the caller's loops are unrolled
 template <typename T>
 __device__ __forceinline__ C2<T> cexpi(int j, int n, int dir) {
@@ -380,10 +411,7 @@ struct BXi4 {
     __shared__ C2<T> lds[ROWS * TS];
     // twiddles W_n^(n1 k2) between the transforms, [n1][k2] (rows padded by one)
     __shared__ C2<T> twb[A * (B + 1)];
-    for (int i = threadIdx.x; i < A * B; i += XF4_NT) {
-      const int q = i / B, k2 = i - q * B;
-      twb[q * (B + 1) + k2] = a.tw2[2 * q * k2];
-    }
+    TwStage<T, A, B, XF4_NT> twt;
     const int c = blockIdx.y;
     const int grp = threadIdx.x / B, l = threadIdx.x % B;
     const int row0 = blockIdx.x * ROWS + grp * G;
@@ -403,6 +431,8 @@ struct BXi4 {
           xk[n2] = irow[k];
           xn[n2] = irow[N - k];
         }
+        twt.load(a.tw2, 2);   // after the data loads
+        __builtin_amdgcn_sched_barrier(0);
   #pragma unroll
         for (int n2 = 0; n2 < B; n2++) {
           const C2<T> sm = cadd(xk[n2], cconj(xn[n2])), d = csub(xk[n2], cconj(xn[n2]));
@@ -410,7 +440,10 @@ struct BXi4 {
           const C2<T> t = cmul(cconj(wk), d);
           v[n2] = C2<T>{sm.x - t.y, sm.y + t.x};
         }
+      } else {
+        twt.load(a.tw2, 2);
       }
+      twt.store(twb);
       dft<B, +1>(v);
       __syncthreads();   // the twiddle table is complete
   #pragma unroll
@@ -460,10 +493,7 @@ struct BXf4 {
     __shared__ T ldr[ROWS * TS];
     // twiddles W_n^(n1 k2) between the transforms, [n1][k2] (rows padded by one)
     __shared__ C2<T> twb[A * (B + 1)];
-    for (int i = threadIdx.x; i < A * B; i += XF4_NT) {
-      const int q = i / B, k2 = i - q * B;
-      twb[q * (B + 1) + k2] = a.tw2[2 * q * k2];
-    }
+    TwStage<T, A, B, XF4_NT> twt;
     const int c = blockIdx.y;
     const int grp = threadIdx.x / B, l = threadIdx.x % B;
     const int row0 = blockIdx.x * ROWS + grp * G;
@@ -480,6 +510,8 @@ struct BXf4 {
         const T *src = a.src[c] + i0;
   #pragma unroll
         for (int n2 = 0; n2 < HB; n2++) v[n2] = *(const C2<T> *)(src + 2 * (n1 + A * n2));
+        twt.load(a.tw2, 2);   // after the data loads
+        __builtin_amdgcn_sched_barrier(0);
         if (uniform) {
   #pragma unroll
           for (int n2 = 0; n2 < HB; n2++) v[n2] = C2<T>{fac * v[n2].x, fac * v[n2].y};
@@ -491,7 +523,10 @@ struct BXf4 {
             v[n2].y = (T(MU0) * a.Ms.at(i + 1)) * (a.vol ? a.vol[i + 1] : T(1)) * v[n2].y;
           }
         }
+      } else {
+        twt.load(a.tw2, 2);
       }
+      twt.store(twb);
       dft<B, -1>(v);
       __syncthreads();   // the twiddle table is complete
   #pragma unroll
@@ -668,13 +703,7 @@ struct BColy {
     // after the transpose space: LDS reads instead of global-memory reads in
     // the middle of the transform
     C2<T> *twt = (C2<T> *)(smem_ + (SPLIT ? sizeof(T) : sizeof(C2<T>)) * A * S);
-    {
-      const C2<T> *tw = (const C2<T> *)a.f.tw;   // W_P^m, m < P
-      for (int j = threadIdx.x; j < A * A; j += A * C) {
-        const int r = j / A, k2 = j - r * A;
-        twt[r * (A + 1) + k2] = tw[r * k2];
-      }
-    }
+    TwStage<T, A, A, A * C> twv;   // loaded after the first plane's data
     const int ngx = gridDim.x, ngy = gridDim.y;
     const int w = xcd_tile(blockIdx.x + ngx * (blockIdx.y + ngy * blockIdx.z), ngx * ngy * gridDim.z);
     const int gx = w % ngx, gyz = w / ngx, gy = gyz % ngy, gz = gyz / ngy;
@@ -699,6 +728,9 @@ struct BColy {
       }
     };
     load(gy * ZB);
+    twv.load((const C2<T> *)a.f.tw, 1);   // W_P^m, m < P
+    __builtin_amdgcn_sched_barrier(0);
+    twv.store(twt);
   #pragma unroll
     for (int p = 0; p < ZB; p++) {
       C2<T> v[A];
@@ -967,7 +999,8 @@ struct ConvZArgs {
   int Hk;          // tensor row pitch (H rounded up to a 128-byte line)
   long zstride;    // complex elements between z planes of a component
   long ystride;    // complex elements between ky rows
-  const T *K[6];   // xx yy zz yz xz xy, each Hk x KHy x (Nz+1)
+  const T *K;      // xx yy zz yz xz xy, each Hk x KHy x (Nz+1), component e at K + e kstride
+  long kstride;
   C2<T> *buf[3];
 };
 
@@ -1015,8 +1048,8 @@ struct BConvz {
     // lane's column: every global load is issued before the first use of any
     // (fixed trip counts, unrolled, then a scheduling barrier).  Thread t
     // loads element e = t / 32 of slab column t % 32 for every kz; the
-    // element's base pointer is selected from the uniform arguments instead of
-    // being read per lane.
+    // element's base is the tensor base plus e component strides (no
+    // per-lane read of a pointer argument).
     static_assert(CONVZ_NT == 6 * CONVZ_COLS, "one slab element per 32 threads");
     const int j = threadIdx.x & (CONVZ_COLS - 1), eld = threadIdx.x / CONVZ_COLS;
     T tv[KZ];
@@ -1025,9 +1058,7 @@ struct BConvz {
       // slab column j: kx0 + j at this ky, or (Nyquist block) kx0 at ky pair 32 blockIdx.y + j
       const int x = nyqb ? kx0 : min(kx0 + j, a.H - 1);
       const long yrow = (long)(nyqb ? min((int)blockIdx.y * CONVZ_COLS + j, a.KHy - 1) : kya) * a.Hk;
-      const T *Ke = eld == 0 ? a.K[0] : eld == 1 ? a.K[1] : eld == 2 ? a.K[2] : eld == 3 ? a.K[3]
-                  : eld == 4 ? a.K[4] : a.K[5];
-      Ke += yrow + x;
+      const T *Ke = a.K + eld * a.kstride + yrow + x;
   #pragma unroll
       for (int kz = 0; kz < KZ; kz++) tv[kz] = Ke[kz * kzs];
     }
@@ -2242,7 +2273,7 @@ struct DemagPlan : DemagBase {
       ConvZArgs<T> za{};
       za.H = H; za.Py = P[1]; za.KHy = KHy; za.Hk = Hk; za.zstride = (long)P[1] * Hp; za.ystride = Hp;
       za.nyq = H > 1 && (H - 1) % CONVZ_COLS == 0;
-      for (int c = 0; c < 6; c++) za.K[c] = K[c];
+      za.K = K[0]; za.kstride = kcount;
       for (int c = 0; c < 3; c++) za.buf[c] = spec[c].as<C2<T>>();
       launch_convz<T>(za, N[2], s);
       return;
