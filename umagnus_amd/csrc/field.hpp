@@ -602,14 +602,11 @@ __device__ __forceinline__ void g4_load(const FieldArgs<T> &a, int ix0, int iy, 
 // reads is clearly non-zero (|x|+|y|+|z| >= 1e-15, so no dot product m.m can
 // underflow: is0() is false for all of them) and, with DMI, every in-plane
 // neighbour exists (interior or periodic); then no neighbour is vacuum and no
-// boundary extrapolation happens, and the per-neighbour sums of exchange_nb /
-// dmi_nb reduce to
-//   h = s_x (x- + x+ - 2 m0) + s_y (y- + y+ - 2 m0) + s_z (z- + z+ - 2 m0)
-//   DMI: h_x += D/c_x (x+_z - x-_z), h_z -= D/c_x (x+_x - x-_x) + D/c_y (y+_y - y-_y),
-//        h_y += D/c_y (y+_z - y-_z)
-// (the same terms, summed in another order).  A z neighbour outside a
-// non-periodic mesh is loaded clamped (it is m0 itself) and adds zero, as in
-// exchange_nb.
+// boundary extrapolation happens, and exchange_nb / dmi_nb reduce to their
+// per-neighbour sums without the vacuum and boundary selects.  The terms are
+// added in the same order with the same weights, so the result is the same
+// to the bit.  A z neighbour outside a non-periodic mesh is loaded clamped
+// (it is m0 itself) and adds zero, as in exchange_nb.
 template <typename T>
 __device__ __forceinline__ T g4_nzmin(T m, T x, T y, T z) {
   return fmin(m, fabs(x) + fabs(y) + fabs(z));
@@ -631,29 +628,56 @@ __device__ __forceinline__ bool g4_fast_ok(const G4In<T> &q, bool z6) {
   mn = g4_nzmin(mn, q.mxr[0], q.mxr[1], q.mxr[2]);
   return mn >= T(1e-15);
 }
-// exchange (+ DMI) field of cell w of a fast group, before the 1/Ms factor
+// exchange (+ DMI) field of cell w of a fast group, before the 1/Ms factor:
+// exchange_nb's (EX 1) or dmi_nb's (EX 2) sums term by term
 template <typename T, int EX>
 __device__ __forceinline__ V3<T> g4_exch_fast(const G4In<T> &q, int w, bool z6, const NbPre<T> &p) {
-  T h[3];
-  const T sx = EX == 1 ? p.s[0] : p.e[0], sy = EX == 1 ? p.s[1] : p.e[1], sz = EX == 1 ? p.s[2] : p.e[2];
-  T xl[3], xr[3];
+  const V3<T> m0 = v3(q.mc[0][w], q.mc[1][w], q.mc[2][w]);
+  V3<T> nb[6];
+  nb[0] = w > 0 ? v3(q.mc[0][w - 1], q.mc[1][w - 1], q.mc[2][w - 1]) : v3(q.mxl[0], q.mxl[1], q.mxl[2]);
+  nb[1] = w < 3 ? v3(q.mc[0][w + 1], q.mc[1][w + 1], q.mc[2][w + 1]) : v3(q.mxr[0], q.mxr[1], q.mxr[2]);
+  nb[2] = v3(q.myl[0][w], q.myl[1][w], q.myl[2][w]);
+  nb[3] = v3(q.myh[0][w], q.myh[1][w], q.myh[2][w]);
+  nb[4] = z6 ? v3(q.mzl[0][w], q.mzl[1][w], q.mzl[2][w]) : m0;
+  nb[5] = z6 ? v3(q.mzh[0][w], q.mzh[1][w], q.mzh[2][w]) : m0;
+  V3<T> h = v3(T(0), T(0), T(0));
+  if constexpr (EX == 1) {
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    xl[c] = w > 0 ? q.mc[c][w - 1] : q.mxl[c];
-    xr[c] = w < 3 ? q.mc[c][w + 1] : q.mxr[c];
-    const T m0 = q.mc[c][w];
-    T v = sx * ((xl[c] + xr[c]) - T(2) * m0);
-    v += sy * ((q.myl[c][w] + q.myh[c][w]) - T(2) * m0);
-    if (z6) v += sz * ((q.mzl[c][w] + q.mzh[c][w]) - T(2) * m0);
-    h[c] = v;
+    for (int j = 0; j < 6; j++) {
+      if (j >= 4 && !z6) break;
+      const T s = p.s[j >> 1];
+      h.x += s * (nb[j].x - m0.x);
+      h.y += s * (nb[j].y - m0.y);
+      h.z += s * (nb[j].z - m0.z);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int axis = j >> 1;
+      const T sgn = (j & 1) ? T(1) : T(-1);
+      const T e = p.e[axis], dc = p.dc[axis];
+      h.x += e * (nb[j].x - m0.x);
+      h.y += e * (nb[j].y - m0.y);
+      h.z += e * (nb[j].z - m0.z);
+      if (axis == 0) {
+        h.x += dc * (sgn * nb[j].z);
+        h.z -= dc * (sgn * nb[j].x);
+      } else {
+        h.y += dc * (sgn * nb[j].z);
+        h.z -= dc * (sgn * nb[j].y);
+      }
+    }
+    if (z6) {
+#pragma unroll
+      for (int j = 4; j < 6; j++) {
+        const T e = p.e[2];
+        h.x += e * (nb[j].x - m0.x);
+        h.y += e * (nb[j].y - m0.y);
+        h.z += e * (nb[j].z - m0.z);
+      }
+    }
   }
-  if constexpr (EX == 2) {
-    h[0] += p.dc[0] * (xr[2] - xl[2]);
-    h[2] -= p.dc[0] * (xr[0] - xl[0]);
-    h[1] += p.dc[1] * (q.myh[2][w] - q.myl[2][w]);
-    h[2] -= p.dc[1] * (q.myh[1][w] - q.myl[1][w]);
-  }
-  return v3(h[0], h[1], h[2]);
+  return h;
 }
 
 // Effective field and torque of the group from its loaded inputs; bdv: demag
