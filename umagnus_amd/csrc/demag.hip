@@ -999,8 +999,7 @@ struct ConvZArgs {
   int Hk;          // tensor row pitch (H rounded up to a 128-byte line)
   long zstride;    // complex elements between z planes of a component
   long ystride;    // complex elements between ky rows
-  const T *K;      // xx yy zz yz xz xy, each Hk x KHy x (Nz+1), component e at K + e kstride
-  long kstride;
+  const T *K;      // xx yy zz yz xz xy as [ky][kz][element][kx] (row pitch Hk): one ky's slab contiguous
   C2<T> *buf[3];
 };
 
@@ -1047,18 +1046,18 @@ struct BConvz {
     // 0. tensor slab of this work-group (32 kx x 6 elements x Nz+1 kz) and the
     // lane's column: every global load is issued before the first use of any
     // (fixed trip counts, unrolled, then a scheduling barrier).  Thread t
-    // loads element e = t / 32 of slab column t % 32 for every kz; the
-    // element's base is the tensor base plus e component strides (no
-    // per-lane read of a pointer argument).
+    // loads element e = t / 32 of slab column t % 32 for every kz.  The
+    // tensor is stored [ky][kz][element][kx], so the slab of one ky pair is
+    // one contiguous range (no per-lane read of a pointer argument).
     static_assert(CONVZ_NT == 6 * CONVZ_COLS, "one slab element per 32 threads");
     const int j = threadIdx.x & (CONVZ_COLS - 1), eld = threadIdx.x / CONVZ_COLS;
     T tv[KZ];
     {
-      const long kzs = (long)a.KHy * a.Hk;
+      const long kzs = 6L * a.Hk;
       // slab column j: kx0 + j at this ky, or (Nyquist block) kx0 at ky pair 32 blockIdx.y + j
       const int x = nyqb ? kx0 : min(kx0 + j, a.H - 1);
-      const long yrow = (long)(nyqb ? min((int)blockIdx.y * CONVZ_COLS + j, a.KHy - 1) : kya) * a.Hk;
-      const T *Ke = a.K + eld * a.kstride + yrow + x;
+      const long ky_ = nyqb ? min((int)blockIdx.y * CONVZ_COLS + j, a.KHy - 1) : kya;
+      const T *Ke = a.K + ((ky_ * KZ) * 6 + eld) * a.Hk + x;
   #pragma unroll
       for (int kz = 0; kz < KZ; kz++) tv[kz] = Ke[kz * kzs];
     }
@@ -1450,19 +1449,22 @@ __global__ void k_demag_mirror(BfArgs a) {
 template <typename T>
 __global__ void k_extract_kspec(const double2 *__restrict__ spec, T *__restrict__ out, int H, int Hk, int Hp,
                                 int Py, int KHy, int KHz, double scale,
-                                unsigned long long *__restrict__ maximg_bits) {
+                                unsigned long long *__restrict__ maximg_bits, int zlay, int comp) {
   const long total = (long)Hk * KHy * KHz;
   double mi = 0;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
     const int kx = (int)(t % Hk);
     const int ky = (int)((t / Hk) % KHy);
     const int kz = (int)(t / ((long)Hk * KHy));
+    // zlay: [ky][kz][element][kx], the Z pass's slab of one ky contiguous;
+    // otherwise [element][kz][ky][kx] (out already offset by the element)
+    const long o = zlay ? (((long)ky * KHz + kz) * 6 + comp) * Hk + kx : t;
     if (kx >= H) {   // row padding
-      out[t] = T(0);
+      out[o] = T(0);
       continue;
     }
     const double2 v = spec[((long)kz * Py + ky) * Hp + kx];
-    out[t] = (T)(v.x * scale);
+    out[o] = (T)(v.x * scale);
     mi = fmax(mi, fabs(v.y));
   }
   atomicMax(maximg_bits, (unsigned long long)__double_as_longlong(mi));
@@ -2219,8 +2221,8 @@ struct DemagPlan : DemagBase {
         UM_HIP(hipGetLastError());
       }
       UM_LAUNCH(k_extract_kspec<T>, dim3(blocks_for(kcount, 256, 4096)), dim3(256), 0, s,
-                         (const double2 *)sp.p, out, H, Hk, Hpd, P[1], KHy, KHz, scale,
-                         (unsigned long long *)mx.p);
+                         (const double2 *)sp.p, zreg ? kern.as<T>() : out, H, Hk, Hpd, P[1], KHy, KHz, scale,
+                         (unsigned long long *)mx.p, zreg ? 1 : 0, c);
       UM_HIP(hipGetLastError());
     }
     unsigned long long bits = 0;
@@ -2273,7 +2275,7 @@ struct DemagPlan : DemagBase {
       ConvZArgs<T> za{};
       za.H = H; za.Py = P[1]; za.KHy = KHy; za.Hk = Hk; za.zstride = (long)P[1] * Hp; za.ystride = Hp;
       za.nyq = H > 1 && (H - 1) % CONVZ_COLS == 0;
-      za.K = K[0]; za.kstride = kcount;
+      za.K = kern.as<T>();
       for (int c = 0; c < 3; c++) za.buf[c] = spec[c].as<C2<T>>();
       launch_convz<T>(za, N[2], s);
       return;
@@ -2521,9 +2523,16 @@ struct DemagPlan : DemagBase {
   void kspec(int c, void *host) override {
     UM_REQUIRE(c >= 0 && c < 6, UM_EINVAL, "component out of range");
     UM_HIP(hipDeviceSynchronize());
-    // rows of H values out of the padded pitch Hk
-    UM_HIP(hipMemcpy2D(host, sizeof(T) * H, kern.as<T>() + c * kcount, sizeof(T) * Hk, sizeof(T) * H,
-                       (size_t)KHy * KHz, hipMemcpyDeviceToHost));
+    // rows of H values out of the padded pitch Hk, returned as [kz][ky][kx]
+    if (zreg) {   // stored [ky][kz][element][kx]
+      for (int ky = 0; ky < KHy; ky++)
+        UM_HIP(hipMemcpy2D((T *)host + (size_t)ky * H, sizeof(T) * H * KHy,
+                           kern.as<T>() + ((size_t)ky * KHz * 6 + c) * Hk, sizeof(T) * Hk * 6, sizeof(T) * H,
+                           (size_t)KHz, hipMemcpyDeviceToHost));
+    } else {
+      UM_HIP(hipMemcpy2D(host, sizeof(T) * H, kern.as<T>() + c * kcount, sizeof(T) * Hk, sizeof(T) * H,
+                         (size_t)KHy * KHz, hipMemcpyDeviceToHost));
+    }
   }
 };
 
