@@ -597,16 +597,17 @@ __device__ __forceinline__ void g4_load(const FieldArgs<T> &a, int ix0, int iy, 
   }
 }
 
-// Interior fast path of the exchange (+ interfacial DMI) stencil for a
-// region-free group of uniform A and D.  It applies when every m the group
-// reads is clearly non-zero (|x|+|y|+|z| >= 1e-15, so no dot product m.m can
-// underflow: is0() is false for all of them) and, with DMI, every in-plane
-// neighbour exists (interior or periodic); then no neighbour is vacuum and no
-// boundary extrapolation happens, and exchange_nb / dmi_nb reduce to their
-// per-neighbour sums without the vacuum and boundary selects.  The terms are
-// added in the same order with the same weights, so the result is the same
-// to the bit.  A z neighbour outside a non-periodic mesh is loaded clamped
-// (it is m0 itself) and adds zero, as in exchange_nb.
+// Fast path of the exchange stencil for a region-free group of uniform A.  It
+// applies when every m the group reads is clearly non-zero (|x|+|y|+|z| >=
+// 1e-15, so no dot product m.m can underflow: is0() is false for all of
+// them); then no neighbour is vacuum and exchange_nb reduces to its
+// per-neighbour sums without the vacuum selects.  The terms are added in the
+// same order with the same weights, so the result is the same to the bit.  A
+// neighbour outside a non-periodic mesh is loaded clamped (it is m0 itself)
+// and adds zero, as in exchange_nb.  (With interfacial DMI the x edges need
+// the boundary extrapolation, every wave of a row holds an edge group, and a
+// fast path there ran next to the general one: 145 us against 141 without it
+// in the fused X inverse kernel at 512x512x32, so it is exchange-only.)
 template <typename T>
 __device__ __forceinline__ T g4_nzmin(T m, T x, T y, T z) {
   return fmin(m, fabs(x) + fabs(y) + fabs(z));
@@ -628,9 +629,9 @@ __device__ __forceinline__ bool g4_fast_ok(const G4In<T> &q, bool z6) {
   mn = g4_nzmin(mn, q.mxr[0], q.mxr[1], q.mxr[2]);
   return mn >= T(1e-15);
 }
-// exchange (+ DMI) field of cell w of a fast group, before the 1/Ms factor:
-// exchange_nb's (EX 1) or dmi_nb's (EX 2) sums term by term
-template <typename T, int EX>
+// exchange field of cell w of a fast group, before the 1/Ms factor:
+// exchange_nb's sums term by term
+template <typename T>
 __device__ __forceinline__ V3<T> g4_exch_fast(const G4In<T> &q, int w, bool z6, const NbPre<T> &p) {
   const V3<T> m0 = v3(q.mc[0][w], q.mc[1][w], q.mc[2][w]);
   V3<T> nb[6];
@@ -641,41 +642,13 @@ __device__ __forceinline__ V3<T> g4_exch_fast(const G4In<T> &q, int w, bool z6, 
   nb[4] = z6 ? v3(q.mzl[0][w], q.mzl[1][w], q.mzl[2][w]) : m0;
   nb[5] = z6 ? v3(q.mzh[0][w], q.mzh[1][w], q.mzh[2][w]) : m0;
   V3<T> h = v3(T(0), T(0), T(0));
-  if constexpr (EX == 1) {
 #pragma unroll
-    for (int j = 0; j < 6; j++) {
-      if (j >= 4 && !z6) break;
-      const T s = p.s[j >> 1];
-      h.x += s * (nb[j].x - m0.x);
-      h.y += s * (nb[j].y - m0.y);
-      h.z += s * (nb[j].z - m0.z);
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int axis = j >> 1;
-      const T sgn = (j & 1) ? T(1) : T(-1);
-      const T e = p.e[axis], dc = p.dc[axis];
-      h.x += e * (nb[j].x - m0.x);
-      h.y += e * (nb[j].y - m0.y);
-      h.z += e * (nb[j].z - m0.z);
-      if (axis == 0) {
-        h.x += dc * (sgn * nb[j].z);
-        h.z -= dc * (sgn * nb[j].x);
-      } else {
-        h.y += dc * (sgn * nb[j].z);
-        h.z -= dc * (sgn * nb[j].y);
-      }
-    }
-    if (z6) {
-#pragma unroll
-      for (int j = 4; j < 6; j++) {
-        const T e = p.e[2];
-        h.x += e * (nb[j].x - m0.x);
-        h.y += e * (nb[j].y - m0.y);
-        h.z += e * (nb[j].z - m0.z);
-      }
-    }
+  for (int j = 0; j < 6; j++) {
+    if (j >= 4 && !z6) break;
+    const T s = p.s[j >> 1];
+    h.x += s * (nb[j].x - m0.x);
+    h.y += s * (nb[j].y - m0.y);
+    h.z += s * (nb[j].z - m0.z);
   }
   return h;
 }
@@ -754,12 +727,12 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
   const bool inxl = (ix0 - 1 >= 0) || (g.pbc & 1), inxr = (ix0 + 4 < g.Nx) || (g.pbc & 1);
   const bool inyl = (iy - 1 >= 0) || (g.pbc & 2), inyh = (iy + 1 < g.Ny) || (g.pbc & 2);
   const bool inzl = (iz - 1 >= 0) || (g.pbc & 4), inzh = (iz + 1 < g.Nz) || (g.pbc & 4);
-  // the interior fast path (region-free, uniform parameters, no STT)
-  constexpr bool FASTOK = UNI && !REG && !STT && (EX == 1 || EX == 2);
+  // the fast path (exchange only, region-free, uniform parameters, no STT)
+  constexpr bool FASTOK = UNI && !REG && !STT && EX == 1;
   bool fast = false;
   NbPre<T> fpre{};
   if constexpr (FASTOK) {
-    fast = (EX == 1 || (inxl && inxr && inyl && inyh)) && g4_fast_ok(q, nnb == 6);
+    fast = g4_fast_ok(q, nnb == 6);
     fpre = PRE ? nbp : nb_pre(q.A0, q.D0, a.wx, a.wy, a.wz, a.cx, a.cy, a.cz);
   }
   T ob[3][4], ot[3][4];
@@ -769,7 +742,7 @@ __device__ __forceinline__ void g4_compute(const FieldArgs<T> &a, int ix0, int i
     const V3<T> m0 = v3(q.mc[0][w], q.mc[1][w], q.mc[2][w]);
     V3<T> B = v3(bdv[0][w], bdv[1][w], bdv[2][w]);
     if (FASTOK && fast) {
-      const V3<T> h = g4_exch_fast<T, EX>(q, w, nnb == 6, fpre);
+      const V3<T> h = g4_exch_fast<T>(q, w, nnb == 6, fpre);
       const T invMs = H ? invMs_u : inv_msat(Ms, I);
       B.x += h.x * invMs;
       B.y += h.y * invMs;
